@@ -1,0 +1,644 @@
+"""ctypes binding of libaxs.so (include/axs.h) plus the benchmark cases.
+
+The solver runs only through the HIP library: importing this module fails
+loudly when libaxs.so has not been built, and there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "lib" / "libaxs.so"
+
+# ---- enums (include/axs.h) ----
+BC_DIRICHLET, BC_NEUMANN, BC_FULLY_DEVELOPED, BC_MICHAELIS_MENTEN, BC_HILL, BC_NONE, BC_PULSATILE = range(7)
+BOUNDARY_WALL, BOUNDARY_VELOCITY_INLET, BOUNDARY_PRESSURE_OUTLET, BOUNDARY_SYMMETRY, BOUNDARY_FAR_FIELD = range(5)
+CONV_UPWIND, CONV_CENTRAL, CONV_SOU, CONV_QUICK = range(4)
+GRAD_GREEN_GAUSS, GRAD_LSQ = range(2)
+LINEAR_JACOBI, LINEAR_GS_RB = range(2)
+TIME_BDF1, TIME_BDF2 = range(2)
+NORM_L1, NORM_L2, NORM_LINF = range(3)
+SCALE_NONE, SCALE_N, SCALE_SQRT_N, SCALE_DIAGONAL, SCALE_CONTINUITY = range(5)
+PREC_FP64, PREC_MIXED = range(2)
+VAR_U, VAR_V, VAR_P = range(3)
+RES_U, RES_V, RES_CONTINUITY = range(3)
+KERNEL_JACOBI_UV, KERNEL_MOMENTUM, KERNEL_MG_SMOOTH, KERNEL_FACE_FLUX, KERNEL_PP_COEFF = range(5)
+
+FIELDS = {
+    "u": 0, "v": 1, "p": 2, "pp": 3, "grad_uz": 4, "grad_ur": 5, "grad_vz": 6, "grad_vr": 7,
+    "grad_pz": 8, "grad_pr": 9, "continuity": 10, "vmag": 11, "cell_re": 12, "mdot": 13,
+    "du": 14, "dv": 15,
+    "u_ac": 16, "u_b": 17, "u_invac": 18, "u_af": 19,
+    "v_ac": 20, "v_b": 21, "v_invac": 22, "v_af": 23,
+    "pp_ac": 24, "pp_b": 25, "pp_invac": 26, "pp_af": 27,
+    "inv_dpn": 40, "wp": 41, "dpb": 42, "inv_a2d": 43, "lsq_wz": 44, "lsq_wr": 45,
+    "lsq_inv_zz": 46, "lsq_inv_zr": 47, "lsq_inv_rr": 48, "length2d": 49,
+}
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_u8p = C.POINTER(C.c_uint8)
+
+
+class MeshView(C.Structure):
+    _fields_ = [
+        ("n_cells", C.c_int), ("n_faces", C.c_int),
+        ("cell_center_z", _dp), ("cell_center_r", _dp), ("cell_volume", _dp), ("cell_area2d", _dp),
+        ("cell_face_start", _ip), ("cell_face_ids", _ip),
+        ("face_owner", _ip), ("face_neighbor", _ip),
+        ("face_normal_z", _dp), ("face_normal_r", _dp), ("face_center_z", _dp), ("face_center_r", _dp),
+        ("face_area", _dp), ("face_group", _ip),
+    ]
+
+
+class BCView(C.Structure):
+    _fields_ = [
+        ("n_groups", C.c_int), ("type", _u8p), ("boundary_type", _u8p), ("value", _dp),
+        ("length", _dp), ("amplitude", _dp), ("frequency", _dp),
+    ]
+
+
+class ResidualConfig(C.Structure):
+    _fields_ = [("enabled", C.c_int), ("norm", C.c_int), ("scaling", C.c_int), ("tol", C.c_double)]
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("convection_scheme", C.c_int), ("gradient_scheme", C.c_int), ("linear_solver", C.c_int),
+        ("time_scheme", C.c_int), ("linear_max_iter", C.c_int), ("use_multigrid", C.c_int),
+        ("add_convection", C.c_int), ("transient", C.c_int), ("use_non_orth", C.c_int),
+        ("dt", C.c_double), ("t_end", C.c_double),
+        ("max_iter", C.c_int), ("check_conv", C.c_int),
+        ("mg_max_iter", C.c_int), ("mg_weight", C.c_double), ("mg_coarse_sweeps", C.c_int),
+        ("mg_pre_sweeps", C.c_int), ("mg_post_sweeps", C.c_int),
+        ("alpha_u", C.c_double), ("alpha_p", C.c_double), ("rho", C.c_double), ("mu", C.c_double),
+        ("residual", ResidualConfig * 3),
+        ("structured_mesh", C.c_int),
+        ("mg_max_levels", C.c_int), ("mg_smoother", C.c_int), ("precision", C.c_int),
+        ("coarse_direct", C.c_int),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int), ("converged", C.c_int), ("stopped", C.c_int),
+        ("n_cells", C.c_int), ("n_faces", C.c_int), ("n_face_refs", C.c_int),
+        ("n_colors", C.c_int), ("mg_levels", C.c_int), ("mg_level_cells", C.c_int * 32),
+        ("setup_ms", C.c_double), ("solve_ms", C.c_double), ("last_residual", C.c_double * 3),
+        ("bytes_per_iteration", C.c_double), ("vcycle_bytes", C.c_double),
+    ]
+
+
+class BlockSpec(C.Structure):
+    _fields_ = [
+        ("z0", C.c_double), ("r0", C.c_double),
+        ("axial_length", C.c_double), ("axial_cells", C.c_int), ("axial_grading", C.c_int),
+        ("axial_coef", C.c_double),
+        ("radial_length", C.c_double), ("radial_cells", C.c_int), ("radial_grading", C.c_int),
+        ("radial_coef", C.c_double),
+        ("edge_group", C.c_int * 4),
+    ]
+
+
+class NozzleSpec(C.Structure):
+    _fields_ = [
+        ("r_in", C.c_double), ("l_in", C.c_double), ("cone_deg", C.c_double), ("r_throat", C.c_double),
+        ("l_throat", C.c_double), ("diff_deg", C.c_double), ("r_out", C.c_double), ("l_out", C.c_double),
+        ("spacing", C.c_double), ("jitter", C.c_double),
+        ("seed_points", C.c_uint64), ("seed_jitter", C.c_uint64),
+    ]
+
+
+PROGRESS_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_char_p), _dp)
+
+
+def load_library(path: os.PathLike | None = None) -> C.CDLL:
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(f"libaxs.so not built ({p}); run __graft_entry__.build() first")
+    lib = C.CDLL(str(p))
+    S = C.c_void_p
+    sig = {
+        "axs_config_defaults": (None, [C.POINTER(Config)]),
+        "axs_create": (C.c_int, [C.POINTER(S), C.c_int]),
+        "axs_destroy": (None, [S]),
+        "axs_last_error": (C.c_char_p, [S]),
+        "axs_set_mesh": (C.c_int, [S, C.POINTER(MeshView)]),
+        "axs_set_bcs": (C.c_int, [S, C.c_int, C.POINTER(BCView)]),
+        "axs_configure": (C.c_int, [S, C.POINTER(Config)]),
+        "axs_run": (C.c_int, [S, PROGRESS_CB, C.c_void_p, C.c_int]),
+        "axs_run_timed": (C.c_int, [S, C.c_int, _dp]),
+        "axs_bench_kernel": (C.c_int, [S, C.c_int, C.c_int, _dp, _dp]),
+        "axs_request_stop": (None, [S]),
+        "axs_can_continue": (C.c_int, [S, C.c_char_p, C.c_size_t]),
+        "axs_get_field": (C.c_int, [S, C.c_int, _dp, C.c_size_t]),
+        "axs_field_length": (C.c_size_t, [S, C.c_int]),
+        "axs_get_residual_history": (C.c_int, [S, C.c_int, _dp, C.c_int, _ip]),
+        "axs_get_stats": (C.c_int, [S, C.POINTER(Stats)]),
+        "axs_mg_create": (C.c_int, [C.POINTER(S), C.c_int, _ip, _ip, C.c_int]),
+        "axs_mg_destroy": (None, [S]),
+        "axs_mg_levels": (C.c_int, [S]),
+        "axs_mg_level_info": (C.c_int, [S, C.c_int, _ip, _ip]),
+        "axs_mg_get_maps": (C.c_int, [S, C.c_int, _ip, _ip, _ip, _ip]),
+        "axs_mg_solve": (C.c_int, [S, _dp, _dp, _dp, _dp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
+                                   C.c_int, C.c_int]),
+        "axs_mg_bench": (C.c_int, [S, _dp, _dp, _dp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int,
+                                   C.c_int, C.c_int, _dp, _dp, _dp]),
+        "axs_build_coloring": (C.c_int, [C.c_int, _ip, _ip, _ip, _ip, _ip]),
+        "axs_mesh_multiblock": (C.c_int, [C.POINTER(S), C.c_int, C.POINTER(BlockSpec)]),
+        "axs_mesh_triangles": (C.c_int, [C.POINTER(S), C.c_int, _dp, _dp, C.c_int, _ip, C.c_int, _dp, _dp, _dp,
+                                         _dp, _ip, C.c_double]),
+        "axs_mesh_nozzle": (C.c_int, [C.POINTER(S), C.POINTER(NozzleSpec)]),
+        "axs_mesh_get_view": (C.c_int, [S, C.POINTER(MeshView)]),
+        "axs_mesh_get_triangles": (C.c_int, [S, _ip, _dp, _dp, _ip, _ip]),
+        "axs_mesh_destroy": (None, [S]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+_LIB: C.CDLL | None = None
+
+
+def lib() -> C.CDLL:
+    global _LIB
+    if _LIB is None:
+        _LIB = load_library()
+    return _LIB
+
+
+class AxsError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"axs status {status}: {msg}")
+        self.status = status
+
+
+def _check(status: int, handle=None) -> None:
+    if status != 0:
+        msg = lib().axs_last_error(handle)
+        raise AxsError(status, msg.decode() if msg else "")
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+def _iptr(a: np.ndarray):
+    return a.ctypes.data_as(_ip)
+
+
+# ---------------------------------------------------------------------------
+# Mesh: plain numpy arrays (FVMeshHostPacked layout)
+# ---------------------------------------------------------------------------
+MESH_KEYS_D = ["cell_center_z", "cell_center_r", "cell_volume", "cell_area2d", "face_normal_z", "face_normal_r",
+               "face_center_z", "face_center_r", "face_area"]
+MESH_KEYS_I = ["cell_face_start", "cell_face_ids", "face_owner", "face_neighbor", "face_group"]
+
+
+@dataclass
+class Mesh:
+    arrays: dict
+    points: np.ndarray | None = None  # (n, 2) for triangle meshes
+    tris: np.ndarray | None = None
+
+    @property
+    def n_cells(self) -> int:
+        return int(self.arrays["cell_center_z"].size)
+
+    @property
+    def n_faces(self) -> int:
+        return int(self.arrays["face_owner"].size)
+
+    def view(self) -> MeshView:
+        a = self.arrays
+        v = MeshView()
+        v.n_cells = self.n_cells
+        v.n_faces = self.n_faces
+        for k in MESH_KEYS_D:
+            setattr(v, k, _dptr(a[k]))
+        for k in MESH_KEYS_I:
+            setattr(v, k, _iptr(a[k]))
+        return v
+
+    def save(self, path: os.PathLike) -> None:
+        extra = {}
+        if self.points is not None:
+            extra = {"points": self.points, "tris": self.tris}
+        np.savez_compressed(path, **self.arrays, **extra)
+
+    @staticmethod
+    def load(path: os.PathLike) -> "Mesh":
+        with np.load(path, allow_pickle=False) as z:
+            arrays = {k: np.ascontiguousarray(z[k]) for k in MESH_KEYS_D + MESH_KEYS_I}
+            pts = z["points"] if "points" in z.files else None
+            tris = z["tris"] if "tris" in z.files else None
+        return Mesh(arrays, pts, tris)
+
+
+def _mesh_from_handle(h) -> Mesh:
+    v = MeshView()
+    _check(lib().axs_mesh_get_view(h, C.byref(v)))
+    n, f = v.n_cells, v.n_faces
+    s = v.cell_face_start[n]
+    sizes = {"cell_center_z": n, "cell_center_r": n, "cell_volume": n, "cell_area2d": n, "face_normal_z": f,
+             "face_normal_r": f, "face_center_z": f, "face_center_r": f, "face_area": f,
+             "cell_face_start": n + 1, "cell_face_ids": s, "face_owner": f, "face_neighbor": f, "face_group": f}
+    arrays = {}
+    for k, cnt in sizes.items():
+        ptr = getattr(v, k)
+        dt = np.float64 if k in MESH_KEYS_D else np.int32
+        arrays[k] = np.ctypeslib.as_array(ptr, shape=(cnt,)).astype(dt, copy=True)
+    npts, ntri = C.c_int(), C.c_int()
+    _check(lib().axs_mesh_get_triangles(h, C.byref(npts), None, None, C.byref(ntri), None))
+    pts = tris = None
+    if ntri.value > 0:
+        pz = np.zeros(npts.value)
+        pr = np.zeros(npts.value)
+        tr = np.zeros(3 * ntri.value, dtype=np.int32)
+        _check(lib().axs_mesh_get_triangles(h, None, _dptr(pz), _dptr(pr), None, _iptr(tr)))
+        pts = np.stack([pz, pr], axis=1)
+        tris = tr.reshape(-1, 3)
+    lib().axs_mesh_destroy(h)
+    return Mesh(arrays, pts, tris)
+
+
+def multiblock_mesh(blocks: list[dict]) -> Mesh:
+    """blocks: dicts with z0, r0, axial=(length, cells[, coef]), radial=(length, cells[, coef]),
+    groups=(west, east, south, north)."""
+    specs = (BlockSpec * len(blocks))()
+    for i, b in enumerate(blocks):
+        s = specs[i]
+        s.z0, s.r0 = b["z0"], b["r0"]
+        al, ac = b["axial"][0], b["axial"][1]
+        acoef = b["axial"][2] if len(b["axial"]) > 2 else 1.0
+        rl, rc = b["radial"][0], b["radial"][1]
+        rcoef = b["radial"][2] if len(b["radial"]) > 2 else 1.0
+        s.axial_length, s.axial_cells, s.axial_grading, s.axial_coef = al, ac, int(acoef != 1.0), acoef
+        s.radial_length, s.radial_cells, s.radial_grading, s.radial_coef = rl, rc, int(rcoef != 1.0), rcoef
+        for e in range(4):
+            s.edge_group[e] = b["groups"][e]
+    h = C.c_void_p()
+    _check(lib().axs_mesh_multiblock(C.byref(h), len(blocks), specs))
+    return _mesh_from_handle(h)
+
+
+def triangle_mesh(points: np.ndarray, tris: np.ndarray, segments: list[tuple], tol: float) -> Mesh:
+    """segments: (z0, r0, z1, r1, group) boundary segments for face classification."""
+    pz = np.ascontiguousarray(points[:, 0], dtype=np.float64)
+    pr = np.ascontiguousarray(points[:, 1], dtype=np.float64)
+    t = np.ascontiguousarray(tris, dtype=np.int32).ravel()
+    seg = np.array([s[:4] for s in segments], dtype=np.float64).reshape(-1, 4)
+    sg = np.array([s[4] for s in segments], dtype=np.int32)
+    cols = [np.ascontiguousarray(seg[:, i]) for i in range(4)]
+    h = C.c_void_p()
+    _check(lib().axs_mesh_triangles(C.byref(h), pz.size, _dptr(pz), _dptr(pr), t.size // 3, _iptr(t), len(segments),
+                                    _dptr(cols[0]), _dptr(cols[1]), _dptr(cols[2]), _dptr(cols[3]), _iptr(sg), tol))
+    return _mesh_from_handle(h)
+
+
+def nozzle_mesh(spacing: float, jitter: float = 0.05, seed_points: int = 3001, seed_jitter: int = 3002,
+                **geometry) -> Mesh:
+    g = dict(NOZZLE_GEOMETRY)
+    g.update(geometry)
+    s = NozzleSpec()
+    for k, v in g.items():
+        setattr(s, k, v)
+    s.spacing, s.jitter, s.seed_points, s.seed_jitter = spacing, jitter, seed_points, seed_jitter
+    h = C.c_void_p()
+    _check(lib().axs_mesh_nozzle(C.byref(h), C.byref(s)))
+    return _mesh_from_handle(h)
+
+
+# ---------------------------------------------------------------------------
+# Boundary conditions
+# ---------------------------------------------------------------------------
+@dataclass
+class BC:
+    """Per-group boundary condition of one variable."""
+    type: list
+    value: list
+    length: list | None = None
+    boundary_type: list | None = None
+    amplitude: list | None = None
+    frequency: list | None = None
+
+    def view(self):
+        n = len(self.type)
+        keep = {
+            "type": np.asarray(self.type, dtype=np.uint8),
+            "value": np.asarray(self.value, dtype=np.float64),
+            "length": np.asarray(self.length if self.length is not None else [0.0] * n, dtype=np.float64),
+            "bt": np.asarray(self.boundary_type if self.boundary_type is not None else [0] * n, dtype=np.uint8),
+            "amp": np.asarray(self.amplitude if self.amplitude is not None else [0.0] * n, dtype=np.float64),
+            "freq": np.asarray(self.frequency if self.frequency is not None else [0.0] * n, dtype=np.float64),
+        }
+        v = BCView()
+        v.n_groups = n
+        v.type = keep["type"].ctypes.data_as(_u8p)
+        v.boundary_type = keep["bt"].ctypes.data_as(_u8p)
+        v.value = _dptr(keep["value"])
+        v.length = _dptr(keep["length"])
+        v.amplitude = _dptr(keep["amp"])
+        v.frequency = _dptr(keep["freq"])
+        return v, keep
+
+
+# Default BC type of each (boundary type, variable) (getDefaultBCType,
+# manager/source/boundary_func.cpp:117-197)
+_DEFAULT_BC = {
+    BOUNDARY_VELOCITY_INLET: (BC_DIRICHLET, BC_DIRICHLET, BC_NEUMANN),
+    BOUNDARY_PRESSURE_OUTLET: (BC_NEUMANN, BC_NEUMANN, BC_DIRICHLET),
+    BOUNDARY_WALL: (BC_DIRICHLET, BC_DIRICHLET, BC_NEUMANN),
+    BOUNDARY_SYMMETRY: (BC_NEUMANN, BC_DIRICHLET, BC_NEUMANN),
+    BOUNDARY_FAR_FIELD: (BC_DIRICHLET, BC_DIRICHLET, BC_NEUMANN),
+}
+
+
+def default_bcs(group_types: list[int], overrides: dict | None = None) -> dict:
+    """BCs of u, v, p for groups of the given boundary types, default values 0.
+    overrides: {(group, var): (type, value)}."""
+    out = {}
+    for var in (VAR_U, VAR_V, VAR_P):
+        types = [_DEFAULT_BC[t][var] for t in group_types]
+        values = [0.0] * len(group_types)
+        out[var] = BC(types, values, boundary_type=list(group_types))
+    for (g, var), spec in (overrides or {}).items():
+        bc = out[var]
+        bc.type[g] = spec[0]
+        bc.value[g] = spec[1]
+        if len(spec) > 2:
+            bc.amplitude = bc.amplitude or [0.0] * len(group_types)
+            bc.frequency = bc.frequency or [0.0] * len(group_types)
+            bc.amplitude[g], bc.frequency[g] = spec[2], spec[3]
+    return out
+
+
+def default_config(**kw) -> Config:
+    c = Config()
+    lib().axs_config_defaults(C.byref(c))
+    for k, v in kw.items():
+        if k == "residual_tol":
+            for i in range(3):
+                c.residual[i].tol = v[i]
+        else:
+            setattr(c, k, v)
+    return c
+
+
+# ---------------------------------------------------------------------------
+# Solver
+# ---------------------------------------------------------------------------
+class Solver:
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        _check(lib().axs_create(C.byref(self._h), device))
+        self._keep = []
+        self.mesh: Mesh | None = None
+
+    def close(self) -> None:
+        if self._h:
+            lib().axs_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_mesh(self, mesh: Mesh) -> None:
+        self.mesh = mesh
+        v = mesh.view()
+        _check(lib().axs_set_mesh(self._h, C.byref(v)), self._h)
+
+    def set_bcs(self, bcs: dict) -> None:
+        for var, bc in bcs.items():
+            v, keep = bc.view()
+            _check(lib().axs_set_bcs(self._h, var, C.byref(v)), self._h)
+
+    def configure(self, cfg: Config) -> None:
+        self.cfg = cfg
+        _check(lib().axs_configure(self._h, C.byref(cfg)), self._h)
+
+    def run(self, progress=None, continue_run: bool = False) -> None:
+        if progress is None:
+            cb = PROGRESS_CB()
+        else:
+            def _cb(user, it, n, names, values):
+                progress(it, [values[i] for i in range(n)])
+            cb = PROGRESS_CB(_cb)
+        self._cb = cb
+        _check(lib().axs_run(self._h, cb, None, int(continue_run)), self._h)
+
+    def run_timed(self, n: int) -> float:
+        ms = C.c_double()
+        _check(lib().axs_run_timed(self._h, n, C.byref(ms)), self._h)
+        return ms.value
+
+    def bench_kernel(self, kernel: int, repeats: int = 50) -> tuple[float, float]:
+        """(average launch ms, algorithmic bytes per launch) of one kernel."""
+        ms, by = C.c_double(), C.c_double()
+        _check(lib().axs_bench_kernel(self._h, kernel, repeats, C.byref(ms), C.byref(by)), self._h)
+        return ms.value, by.value
+
+    def request_stop(self) -> None:
+        lib().axs_request_stop(self._h)
+
+    def can_continue(self) -> tuple[bool, str]:
+        buf = C.create_string_buffer(256)
+        ok = lib().axs_can_continue(self._h, buf, 256)
+        return bool(ok), buf.value.decode()
+
+    def field(self, name: str) -> np.ndarray:
+        fid = FIELDS[name]
+        n = lib().axs_field_length(self._h, fid)
+        out = np.zeros(n)
+        _check(lib().axs_get_field(self._h, fid, _dptr(out), n), self._h)
+        return out
+
+    def history(self, res: int) -> np.ndarray:
+        cnt = C.c_int()
+        _check(lib().axs_get_residual_history(self._h, res, None, 0, C.byref(cnt)), self._h)
+        out = np.zeros(cnt.value)
+        if cnt.value:
+            _check(lib().axs_get_residual_history(self._h, res, _dptr(out), cnt.value, C.byref(cnt)), self._h)
+        return out
+
+    def stats(self) -> Stats:
+        s = Stats()
+        _check(lib().axs_get_stats(self._h, C.byref(s)), self._h)
+        return s
+
+
+class Multigrid:
+    """Stand-alone agglomeration multigrid on the device."""
+
+    def __init__(self, face_start: np.ndarray, face_neighbor: np.ndarray, max_levels: int = 0):
+        self.fs = np.ascontiguousarray(face_start, dtype=np.int32)
+        self.fn = np.ascontiguousarray(face_neighbor, dtype=np.int32)
+        self.n = self.fs.size - 1
+        self._h = C.c_void_p()
+        _check(lib().axs_mg_create(C.byref(self._h), self.n, _iptr(self.fs), _iptr(self.fn), max_levels))
+
+    def close(self):
+        if self._h:
+            lib().axs_mg_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def levels(self) -> int:
+        return lib().axs_mg_levels(self._h)
+
+    def level_info(self, level: int) -> tuple[int, int]:
+        n, r = C.c_int(), C.c_int()
+        _check(lib().axs_mg_level_info(self._h, level, C.byref(n), C.byref(r)))
+        return n.value, r.value
+
+    def maps(self, level: int) -> dict:
+        n, r = self.level_info(level)
+        fs = np.zeros(n + 1, np.int32)
+        fn = np.zeros(r, np.int32)
+        coarsest = level == self.levels - 1
+        c2c = np.zeros(0 if coarsest else n, np.int32)
+        s2s = np.zeros(0 if coarsest else r, np.int32)
+        _check(lib().axs_mg_get_maps(self._h, level, _iptr(fs), _iptr(fn),
+                                     _iptr(c2c) if c2c.size else None, _iptr(s2s) if s2s.size else None))
+        return {"face_start": fs, "face_neighbor": fn, "cell_to_coarse": c2c, "slot_to_coarse_slot": s2s}
+
+    def solve(self, AC, AF, b, x, cycles=3, weight=0.6, pre=1, post=1, coarse=30, precision=PREC_FP64,
+              coarse_direct=0) -> np.ndarray:
+        AC, AF, b = (np.ascontiguousarray(a, dtype=np.float64) for a in (AC, AF, b))
+        x = np.array(x, dtype=np.float64, copy=True)
+        _check(lib().axs_mg_solve(self._h, _dptr(AC), _dptr(AF), _dptr(b), _dptr(x), cycles, weight, pre, post,
+                                  coarse, precision, coarse_direct))
+        return x
+
+    def bench(self, AC, AF, b, cycles=3, weight=0.6, pre=1, post=1, coarse=30, precision=PREC_FP64,
+              coarse_direct=0, repeats=20):
+        AC, AF, b = (np.ascontiguousarray(a, dtype=np.float64) for a in (AC, AF, b))
+        ms, by = C.c_double(), C.c_double()
+        x = np.zeros(self.n)
+        _check(lib().axs_mg_bench(self._h, _dptr(AC), _dptr(AF), _dptr(b), cycles, weight, pre, post, coarse,
+                                  precision, coarse_direct, repeats, C.byref(ms), C.byref(by), _dptr(x)))
+        return ms.value, by.value, x
+
+
+def coloring(face_start: np.ndarray, face_neighbor: np.ndarray):
+    fs = np.ascontiguousarray(face_start, dtype=np.int32)
+    fn = np.ascontiguousarray(face_neighbor, dtype=np.int32)
+    n = fs.size - 1
+    cs = np.zeros(n + 1, np.int32)
+    order = np.zeros(n, np.int32)
+    nc = C.c_int()
+    _check(lib().axs_build_coloring(n, _iptr(fs), _iptr(fn), _iptr(cs), _iptr(order), C.byref(nc)))
+    return cs[: nc.value + 1].copy(), order
+
+
+# ---------------------------------------------------------------------------
+# Benchmark / parity cases (the configurations of the benchmark suite)
+# ---------------------------------------------------------------------------
+# group ids used by every case
+G_INLET, G_OUTLET, G_WALL, G_AXIS = 0, 1, 2, 3
+GROUP_TYPES = [BOUNDARY_VELOCITY_INLET, BOUNDARY_PRESSURE_OUTLET, BOUNDARY_WALL, BOUNDARY_SYMMETRY]
+
+NOZZLE_GEOMETRY = dict(r_in=0.006, l_in=0.03, cone_deg=20.0, r_throat=0.002, l_throat=0.05, diff_deg=10.0,
+                       r_out=0.006, l_out=0.05)
+
+
+@dataclass
+class Case:
+    name: str
+    mesh: Mesh
+    bcs: dict
+    config: Config
+    info: dict = field(default_factory=dict)
+
+
+def pipe_case(nz: int = 128, nr: int = 32, L: float = 0.2, R: float = 0.01, u_in: float = 0.005,
+              rho: float = 1000.0, mu: float = 1e-3, tol: float = 1e-6, max_iter: int = 200000, **cfg) -> Case:
+    """Steady laminar pipe flow on a structured r-z grid (Re = rho*u_in*2R/mu = 100
+    for the defaults): first-order upwind, Green-Gauss, FP64, weighted-Jacobi MG."""
+    mesh = multiblock_mesh([dict(z0=0.0, r0=0.0, axial=(L, nz), radial=(R, nr),
+                                 groups=(G_INLET, G_OUTLET, G_AXIS, G_WALL))])
+    bcs = default_bcs(GROUP_TYPES, {(G_INLET, VAR_U): (BC_DIRICHLET, u_in)})
+    c = default_config(rho=rho, mu=mu, gradient_scheme=GRAD_GREEN_GAUSS, convection_scheme=CONV_UPWIND,
+                       max_iter=max_iter, structured_mesh=1, residual_tol=(tol, tol, tol))
+    for k, v in cfg.items():
+        setattr(c, k, v)
+    return Case("pipe", mesh, bcs, c, dict(L=L, R=R, u_in=u_in, rho=rho, mu=mu))
+
+
+def expansion_case(nz=(137, 887), nr=(85, 171), bias: float = 1.05, re_throat: float = 500.0, rho=1000.0,
+                   mu=1e-3, tol=1e-6, max_iter=200000, **cfg) -> Case:
+    """Structured three-block sudden expansion: r 0.002 -> 0.006 at z = 0.04,
+    length 0.3; geometric radial grading toward each wall; SOU + LSQ."""
+    r1, r2, z1, L = 0.002, 0.006, 0.04, 0.3
+    u_in = re_throat * mu / (rho * 2 * r1)
+    # geometric progression ratio `bias` per cell toward the wall of each band:
+    # band 1 (0..r1) grows toward r1, band 2 (r1..r2) grows toward r2
+    inv = 1.0 / bias
+    blocks = [
+        dict(z0=0.0, r0=0.0, axial=(z1, nz[0]), radial=(r1, nr[0], inv), groups=(G_INLET, -1, G_AXIS, G_WALL)),
+        dict(z0=z1, r0=0.0, axial=(L - z1, nz[1]), radial=(r1, nr[0], inv), groups=(-1, G_OUTLET, G_AXIS, -1)),
+        dict(z0=z1, r0=r1, axial=(L - z1, nz[1]), radial=(r2 - r1, nr[1], inv), groups=(G_WALL, G_OUTLET, -1, G_WALL)),
+    ]
+    mesh = multiblock_mesh(blocks)
+    bcs = default_bcs(GROUP_TYPES, {(G_INLET, VAR_U): (BC_DIRICHLET, u_in)})
+    c = default_config(rho=rho, mu=mu, gradient_scheme=GRAD_LSQ, convection_scheme=CONV_SOU, max_iter=max_iter,
+                       structured_mesh=1, residual_tol=(tol, tol, tol))
+    for k, v in cfg.items():
+        setattr(c, k, v)
+    return Case("expansion", mesh, bcs, c, dict(u_in=u_in, r1=r1, r2=r2))
+
+
+def nozzle_spacing_for(cells: float) -> float:
+    """Poisson-disk radius giving about `cells` triangles on the nozzle (the
+    sampler places ~0.72 points per r^2 of area, two triangles per point)."""
+    g = NOZZLE_GEOMETRY
+    lc = (g["r_in"] - g["r_throat"]) / math.tan(math.radians(g["cone_deg"]))
+    ld = (g["r_out"] - g["r_throat"]) / math.tan(math.radians(g["diff_deg"]))
+    area = (g["r_in"] * g["l_in"] + 0.5 * (g["r_in"] + g["r_throat"]) * lc + g["r_throat"] * g["l_throat"]
+            + 0.5 * (g["r_out"] + g["r_throat"]) * ld + g["r_out"] * g["l_out"])
+    return math.sqrt(area * 2 * NOZZLE_DENSITY / cells)
+
+
+NOZZLE_DENSITY = 0.6273
+
+
+def nozzle_case(cells: float = 1.0e6, spacing: float | None = None, re_throat=500.0, rho=1000.0, mu=1e-3, tol=1e-6,
+                max_iter=200000, mesh: Mesh | None = None, **cfg) -> Case:
+    """Unstructured triangular nozzle (Poisson-disk + 5 % jitter + Delaunay),
+    non-orthogonal corrector on, agglomeration MG."""
+    if mesh is None:
+        mesh = nozzle_mesh(spacing if spacing is not None else nozzle_spacing_for(cells))
+    g = NOZZLE_GEOMETRY
+    u_throat = re_throat * mu / (rho * 2 * g["r_throat"])
+    u_in = u_throat * (g["r_throat"] / g["r_in"]) ** 2
+    bcs = default_bcs(GROUP_TYPES, {(G_INLET, VAR_U): (BC_DIRICHLET, u_in)})
+    c = default_config(rho=rho, mu=mu, use_non_orth=1, max_iter=max_iter, residual_tol=(tol, tol, tol))
+    for k, v in cfg.items():
+        setattr(c, k, v)
+    return Case("nozzle", mesh, bcs, c, dict(u_in=u_in, u_throat=u_throat))
+
+
+def make_solver(case: Case, device: int = 0) -> Solver:
+    s = Solver(device)
+    s.set_mesh(case.mesh)
+    s.set_bcs(case.bcs)
+    s.configure(case.config)
+    return s

@@ -1,0 +1,112 @@
+// device.hpp -- device data views and the kernel launch interface.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace axs {
+
+constexpr int kBlock = 256;   // threads per workgroup (4 wave64)
+
+// Mesh on the device: SoA cells + faces (FVMeshDevice, structs/solver_struct.h:333-397).
+struct DMesh {
+    int N = 0, F = 0, S = 0;
+    const double *cZ, *cR, *vol, *invA2D, *lsqInvZZ, *lsqInvZR, *lsqInvRR;
+    const int *faceStart, *faceIDs;
+    const int *slotNb;      // neighbour cell of each slot, -1 on a boundary face
+    const int *owner, *neighbor, *group;
+    const double *nZ, *nR, *fZ, *fR, *area, *invDPN, *wP, *dPB, *lsqWZ, *lsqWR, *length2D;
+};
+
+// Boundary conditions of one variable (BoundaryFieldDevice, solver_struct.h:431-454).
+struct DBC {
+    int nGroups = 0;
+    const uint8_t* type = nullptr;
+    const double* value = nullptr;
+    const double* length = nullptr;
+};
+
+// Run control shared by the host and the per-iteration graph.
+struct Ctl {
+    int done;          // set once the outer loop must end: every kernel returns at once
+    int iter;          // currentIteration
+    int k;             // iteration index within the current time step
+    int maxIter;
+    int checkConv;
+    int transient;
+    int bdf2Step;      // BDF2 term active in this time step
+    int nHist;         // residual checks recorded
+    int histCap;       // capacity of DState::hist in checks
+    int converged;
+    int N;
+    int enabled[3], norm[3], scaling[3];
+    double tol[3];
+    double resVal[3], scaleVal[3];
+};
+
+// Everything the SIMPLE kernels touch (VariablesSimple + Coefficients).
+struct DState {
+    double *u, *v, *p, *pp, *uTmp, *vTmp, *ppTmp;
+    double *DU, *DV, *gPZ, *gPR, *gUZ, *gUR, *gVZ, *gVR, *mDot;
+    double *uOld, *vOld, *uOld2, *vOld2;
+    double *uAC, *ub, *uInv, *vAC, *vb, *vInv, *AFuv;
+    double *ppAC, *ppb, *ppInv, *ppAF;
+    double *partial;   // per-block residual partial sums
+    double *hist;      // residual history: U, V, Continuity, iteration per check
+    Ctl* ctl;
+};
+
+struct Physics {
+    double rho, mu, alphaU, alphaP, dt;
+    int conv, scheme, gradScheme, nonOrtho;
+    int transient;   // add the unsteady term (BDF1, or BDF2 when ctl->bdf2Step)
+};
+
+// One multigrid level on the device (MultigridLevel, multigrid.cuh:60-106)
+struct DLevel {
+    int n = 0, refs = 0, nCoarse = 0;
+    int *start = nullptr, *nb = nullptr;
+    double *AC = nullptr, *b = nullptr, *AF = nullptr;
+    double *x = nullptr, *xNew = nullptr;
+    int *toCoarse = nullptr, *slotToCoarseSlot = nullptr;
+    int *memberStart = nullptr, *members = nullptr;
+    int *cslotStart = nullptr, *cslotFine = nullptr;
+    // FP32 copies for the mixed-precision V-cycle
+    float *AC32 = nullptr, *b32 = nullptr, *AF32 = nullptr, *x32 = nullptr, *xNew32 = nullptr;
+};
+
+inline int blocksFor(long n, int t = kBlock) { return (int)((n + t - 1) / t); }
+
+// ---- launchers (kernels.hip) ----
+void launchGradUV(hipStream_t s, const DMesh& m, DBC bu, DBC bv, DState st, int scheme);
+void launchGradField(hipStream_t s, const DMesh& m, DBC bc, const double* phi, double* gz, double* gr,
+                     int scheme, const Ctl* ctl);
+void launchMomentum(hipStream_t s, const DMesh& m, DBC bu, DBC bv, DBC bp, DState st, Physics ph);
+void launchJacobiUV(hipStream_t s, const DMesh& m, DState st, const double* uo, const double* vo, double* un,
+                    double* vn);
+void launchGSColorUV(hipStream_t s, const DMesh& m, DState st, const int* order, int begin, int count);
+void launchJacobiSys(hipStream_t s, const DMesh& m, const double* b, const double* inv, const double* AF,
+                     const double* xo, double* xn, const Ctl* ctl);
+void launchGSColorSys(hipStream_t s, const DMesh& m, const double* b, const double* inv, const double* AF, double* x,
+                      const int* order, int begin, int count, const Ctl* ctl);
+void launchPPCoeff(hipStream_t s, const DMesh& m, DBC bp, DState st, Physics ph);
+void launchFaceFlux(hipStream_t s, const DMesh& m, DBC bu, DBC bv, DBC bp, DState st, Physics ph);
+void launchPPRhs(hipStream_t s, const DMesh& m, DState st, Physics ph, int cross);
+void launchGradPPUpdate(hipStream_t s, const DMesh& m, DBC ppbc, DState st, Physics ph);
+void launchMassFluxUpdate(hipStream_t s, const DMesh& m, DBC bp, DState st, Physics ph);
+void launchResidual(hipStream_t s, const DMesh& m, DState st, int blocks);
+void launchFinish(hipStream_t s, DState st, int blocks);
+void launchStepBegin(hipStream_t s, const DMesh& m, DState st, int bdf2);
+void launchCopy(hipStream_t s, double* dst, const double* src, int n, const Ctl* ctl);
+void launchFill(hipStream_t s, double* dst, double value, int n, const Ctl* ctl);
+
+// multigrid (FP64)
+void launchMGCoarseOp(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl);
+void launchMGJacobi(hipStream_t s, const DLevel& L, double w, const Ctl* ctl);
+void launchMGSmoothBlock(hipStream_t s, const DLevel& L, double w, int sweeps, const Ctl* ctl);
+void launchMGRestrict(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl);
+void launchMGProlongSmooth(hipStream_t s, const DLevel& f, const DLevel& c, double w, const Ctl* ctl);
+void launchMGProlong(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl);
+
+}  // namespace axs

@@ -1,0 +1,408 @@
+// capi.cpp -- extern "C" entry points of libaxs.so (declared in include/axs.h).
+//
+// Every entry point catches C++ exceptions and maps them to axs_status codes;
+// the message is kept on the handle (or thread-locally for handle-less calls)
+// for axs_last_error.  The reference aborts on a CUDA error (CUDA_CHECK,
+// manager/header/gpu_utils.h:5-11) and prints validation failures to its
+// console (Solver::runCheck, solver.cpp:530-661); here both come back as codes.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "axs.h"
+#include "mesh_types.hpp"
+#include "solver.hpp"
+#include "topology.hpp"
+
+using namespace axs;
+
+struct axs_solver {
+    Solver* s = nullptr;
+};
+
+struct axs_mg {
+    MGDevice mg;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Arena arena;
+    double *AC = nullptr, *b = nullptr, *AF = nullptr, *x = nullptr, *xNew = nullptr;
+    int n = 0, refs = 0;
+};
+
+struct axs_mesh {
+    HostMesh m;
+};
+
+namespace {
+
+thread_local std::string tlsError;
+
+template <class F>
+axs_status guarded(std::string* err, F&& f) {
+    try {
+        f();
+        if (err) err->clear();
+        return AXS_OK;
+    } catch (const HipError& e) {
+        (err ? *err : tlsError) = e.what();
+        return AXS_ERR_DEVICE;
+    } catch (const std::invalid_argument& e) {
+        (err ? *err : tlsError) = e.what();
+        return AXS_ERR_INVALID;
+    } catch (const std::out_of_range& e) {
+        (err ? *err : tlsError) = e.what();
+        return AXS_ERR_RANGE;
+    } catch (const std::logic_error& e) {
+        // domain_error (unsupported option) derives from logic_error
+        (err ? *err : tlsError) = e.what();
+        return dynamic_cast<const std::domain_error*>(&e) ? AXS_ERR_UNSUPPORTED : AXS_ERR_STATE;
+    } catch (const std::bad_alloc&) {
+        (err ? *err : tlsError) = "out of host memory";
+        return AXS_ERR_DEVICE;
+    } catch (const std::exception& e) {
+        (err ? *err : tlsError) = e.what();
+        return AXS_ERR_DEVICE;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+void axs_config_defaults(axs_config* c) {
+    if (!c) return;
+    std::memset(c, 0, sizeof(*c));
+    c->convection_scheme = AXS_CONV_UPWIND;   // solver_struct.h:206
+    c->gradient_scheme = AXS_GRAD_LSQ;        // :207
+    c->linear_solver = AXS_LINEAR_JACOBI;     // :208
+    c->time_scheme = AXS_TIME_BDF1;           // :209
+    c->linear_max_iter = 20;                  // :212
+    c->use_multigrid = 1;                     // :213
+    c->add_convection = 1;                    // :215
+    c->transient = 0;                         // :216
+    c->use_non_orth = 0;                      // :226
+    c->dt = 0.1;                              // :228
+    c->t_end = 2.0;                           // :229
+    c->max_iter = 50;                         // :159
+    c->check_conv = 1;                        // :160
+    c->mg_max_iter = 3;                       // :173
+    c->mg_weight = 0.6;                       // :178
+    c->mg_coarse_sweeps = 30;                 // :183
+    c->mg_pre_sweeps = 1;                     // :187
+    c->mg_post_sweeps = 1;                    // :188
+    c->alpha_u = 0.7;                         // :312
+    c->alpha_p = 0.3;                         // :313
+    c->rho = 998.0;                           // src/header/setting.cuh:63
+    c->mu = 0.0010518;                        // src/header/setting.cuh:64
+    for (int i = 0; i < AXS_RES_COUNT; i++) { // initConfigResiduals, solver.cpp:87-101
+        c->residual[i].enabled = 1;
+        c->residual[i].norm = AXS_NORM_L1;
+        c->residual[i].scaling = AXS_SCALE_DIAGONAL;
+        c->residual[i].tol = 0.001;
+    }
+    c->structured_mesh = 0;
+    c->mg_max_levels = 0;
+    c->mg_smoother = AXS_MG_SMOOTH_JACOBI;
+    c->precision = AXS_PREC_FP64;
+    c->coarse_direct = 0;
+}
+
+axs_status axs_create(axs_solver** out, int device) {
+    if (!out) return AXS_ERR_INVALID;
+    *out = nullptr;
+    return guarded(nullptr, [&] {
+        int count = 0;
+        AXS_HIP(hipGetDeviceCount(&count));
+        if (device < 0 || device >= count) throw std::invalid_argument("no such device");
+        axs_solver* h = new axs_solver();
+        try {
+            h->s = new Solver(device);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+void axs_destroy(axs_solver* h) {
+    if (!h) return;
+    delete h->s;
+    delete h;
+}
+
+const char* axs_last_error(const axs_solver* h) {
+    if (!h || !h->s) return tlsError.c_str();
+    return h->s->lastError.c_str();
+}
+
+axs_status axs_set_mesh(axs_solver* h, const axs_mesh_view* v) {
+    if (!h || !v) return AXS_ERR_INVALID;
+    return guarded(&h->s->lastError, [&] { h->s->setMesh(*v); });
+}
+
+axs_status axs_set_bcs(axs_solver* h, int var, const axs_bc_view* v) {
+    if (!h || !v) return AXS_ERR_INVALID;
+    return guarded(&h->s->lastError, [&] { h->s->setBC(var, *v); });
+}
+
+axs_status axs_configure(axs_solver* h, const axs_config* c) {
+    if (!h || !c) return AXS_ERR_INVALID;
+    return guarded(&h->s->lastError, [&] { h->s->configure(*c); });
+}
+
+axs_status axs_run(axs_solver* h, axs_progress_cb cb, void* user, int continue_run) {
+    if (!h) return AXS_ERR_INVALID;
+    return guarded(&h->s->lastError, [&] { h->s->run(cb, user, continue_run != 0); });
+}
+
+axs_status axs_run_timed(axs_solver* h, int n, double* ms) {
+    if (!h || !ms) return AXS_ERR_INVALID;
+    return guarded(&h->s->lastError, [&] { h->s->runTimed(n, ms); });
+}
+
+axs_status axs_bench_kernel(axs_solver* h, int kernel, int repeats, double* ms, double* bytes) {
+    if (!h || !ms || !bytes) return AXS_ERR_INVALID;
+    return guarded(&h->s->lastError, [&] { h->s->benchKernel(kernel, repeats, ms, bytes); });
+}
+
+void axs_request_stop(axs_solver* h) {
+    if (h && h->s) h->s->requestStop();
+}
+
+int axs_can_continue(const axs_solver* h, char* reason, size_t len) {
+    if (!h) return 0;
+    std::string why;
+    const bool ok = h->s->canContinue(why);
+    if (reason && len > 0) {
+        std::strncpy(reason, why.c_str(), len - 1);
+        reason[len - 1] = 0;
+    }
+    return ok ? 1 : 0;
+}
+
+axs_status axs_get_field(axs_solver* h, int field, double* out, size_t n) {
+    if (!h || !out) return AXS_ERR_INVALID;
+    return guarded(&h->s->lastError, [&] { h->s->getField(field, out, n); });
+}
+
+size_t axs_field_length(const axs_solver* h, int field) { return h ? h->s->fieldLength(field) : 0; }
+
+axs_status axs_get_residual_history(axs_solver* h, int res, double* out, int max_n, int* count) {
+    if (!h) return AXS_ERR_INVALID;
+    return guarded(&h->s->lastError, [&] {
+        const int c = h->s->residualHistory(res, out, max_n);
+        if (count) *count = c;
+    });
+}
+
+axs_status axs_get_stats(axs_solver* h, axs_stats* out) {
+    if (!h || !out) return AXS_ERR_INVALID;
+    return guarded(&h->s->lastError, [&] { h->s->stats(*out); });
+}
+
+// ---------------- multigrid, stand-alone ----------------
+axs_status axs_mg_create(axs_mg** out, int n_cells, const int* face_start, const int* face_neighbor, int max_levels) {
+    if (!out || n_cells <= 0 || !face_start || !face_neighbor) return AXS_ERR_INVALID;
+    *out = nullptr;
+    return guarded(nullptr, [&] {
+        if (face_start[0] != 0) throw std::invalid_argument("face_start[0] must be 0");
+        const int S = face_start[n_cells];
+        for (int k = 0; k < S; k++)
+            if (face_neighbor[k] < -1 || face_neighbor[k] >= n_cells) throw std::invalid_argument("neighbour out of range");
+        std::vector<int> st(face_start, face_start + n_cells + 1), nb(face_neighbor, face_neighbor + S);
+        axs_mg* h = new axs_mg();
+        try {
+            AXS_HIP(hipGetDevice(&h->device));
+            AXS_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+            h->mg.build(n_cells, st, nb, max_levels);
+            h->n = n_cells;
+            h->refs = S;
+            h->AC = h->arena.zeros<double>(n_cells);
+            h->b = h->arena.zeros<double>(n_cells);
+            h->AF = h->arena.zeros<double>(S);
+            h->x = h->arena.zeros<double>(n_cells);
+            h->xNew = h->arena.zeros<double>(n_cells);
+            h->mg.bindLevel0(h->AC, h->b, h->AF, h->x, h->xNew);
+        } catch (...) {
+            if (h->stream) (void)hipStreamDestroy(h->stream);
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+void axs_mg_destroy(axs_mg* h) {
+    if (!h) return;
+    (void)hipStreamSynchronize(h->stream);
+    h->arena.release();
+    h->mg.arena.release();
+    (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+int axs_mg_levels(const axs_mg* h) { return h ? (int)h->mg.host.size() : 0; }
+
+axs_status axs_mg_level_info(const axs_mg* h, int level, int* n_cells, int* n_refs) {
+    if (!h) return AXS_ERR_INVALID;
+    if (level < 0 || level >= (int)h->mg.host.size()) return AXS_ERR_RANGE;
+    if (n_cells) *n_cells = h->mg.host[level].n;
+    if (n_refs) *n_refs = h->mg.host[level].refs();
+    return AXS_OK;
+}
+
+axs_status axs_mg_get_maps(const axs_mg* h, int level, int* fs, int* fn, int* c2c, int* s2s) {
+    if (!h) return AXS_ERR_INVALID;
+    if (level < 0 || level >= (int)h->mg.host.size()) return AXS_ERR_RANGE;
+    const Level& L = h->mg.host[level];
+    if (fs) std::copy(L.start.begin(), L.start.end(), fs);
+    if (fn) std::copy(L.nb.begin(), L.nb.end(), fn);
+    if (c2c) std::copy(L.toCoarse.begin(), L.toCoarse.end(), c2c);
+    if (s2s) std::copy(L.slotToCoarseSlot.begin(), L.slotToCoarseSlot.end(), s2s);
+    return AXS_OK;
+}
+
+static void mgUpload(axs_mg* h, const double* AC, const double* AF, const double* b, const double* x) {
+    AXS_HIP(hipMemcpy(h->AC, AC, h->n * sizeof(double), hipMemcpyHostToDevice));
+    AXS_HIP(hipMemcpy(h->AF, AF, std::max(h->refs, 0) * sizeof(double), hipMemcpyHostToDevice));
+    AXS_HIP(hipMemcpy(h->b, b, h->n * sizeof(double), hipMemcpyHostToDevice));
+    if (x) AXS_HIP(hipMemcpy(h->x, x, h->n * sizeof(double), hipMemcpyHostToDevice));
+    else AXS_HIP(hipMemset(h->x, 0, h->n * sizeof(double)));
+}
+
+static MGParams mgParams(int cycles, double weight, int pre, int post, int coarse, int precision, int direct) {
+    if (precision != AXS_PREC_FP64) throw std::domain_error("mixed precision is not built in this library");
+    if (direct) throw std::domain_error("coarse direct solve is not built in this library");
+    if (pre < 0 || post < 0 || coarse < 0) throw std::invalid_argument("negative sweep count");
+    MGParams p;
+    p.cycles = cycles;
+    p.weight = weight;
+    p.pre = pre;
+    p.post = post;
+    p.coarseSweeps = coarse;
+    p.precision = precision;
+    p.coarseDirect = direct;
+    return p;
+}
+
+axs_status axs_mg_solve(axs_mg* h, const double* AC, const double* AF, const double* b, double* x, int cycles,
+                        double weight, int pre, int post, int coarse, int precision, int direct) {
+    if (!h || !AC || !AF || !b || !x) return AXS_ERR_INVALID;
+    return guarded(nullptr, [&] {
+        const MGParams p = mgParams(cycles, weight, pre, post, coarse, precision, direct);
+        AXS_HIP(hipSetDevice(h->device));
+        mgUpload(h, AC, AF, b, x);
+        h->mg.enqueueSolve(h->stream, p, nullptr, true);
+        AXS_HIP(hipStreamSynchronize(h->stream));
+        AXS_HIP(hipGetLastError());
+        AXS_HIP(hipMemcpy(x, h->x, h->n * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+axs_status axs_mg_bench(axs_mg* h, const double* AC, const double* AF, const double* b, int cycles, double weight,
+                        int pre, int post, int coarse, int precision, int direct, int repeats, double* ms,
+                        double* bytes, double* x_out) {
+    if (!h || !AC || !AF || !b || !ms || !bytes || repeats <= 0) return AXS_ERR_INVALID;
+    return guarded(nullptr, [&] {
+        const MGParams p = mgParams(cycles, weight, pre, post, coarse, precision, direct);
+        AXS_HIP(hipSetDevice(h->device));
+        mgUpload(h, AC, AF, b, nullptr);
+        h->mg.enqueueSolve(h->stream, p, nullptr, true);  // warm-up
+        hipEvent_t e0, e1;
+        AXS_HIP(hipEventCreate(&e0));
+        AXS_HIP(hipEventCreate(&e1));
+        AXS_HIP(hipEventRecord(e0, h->stream));
+        for (int r = 0; r < repeats; r++) h->mg.enqueueSolve(h->stream, p, nullptr, true);
+        AXS_HIP(hipEventRecord(e1, h->stream));
+        AXS_HIP(hipEventSynchronize(e1));
+        float f = 0.0f;
+        AXS_HIP(hipEventElapsedTime(&f, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        AXS_HIP(hipGetLastError());
+        *ms = f / repeats;
+        *bytes = h->mg.bytesPerSolve(p, true);
+        if (x_out) AXS_HIP(hipMemcpy(x_out, h->x, h->n * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+axs_status axs_build_coloring(int n, const int* fs, const int* fn, int* color_start, int* cell_order, int* n_colors) {
+    if (n < 0 || !fs || !fn || !color_start || !cell_order || !n_colors) return AXS_ERR_INVALID;
+    return guarded(nullptr, [&] {
+        std::vector<int> cs, order;
+        *n_colors = greedyColoring(n, fs, fn, cs, order);
+        std::copy(cs.begin(), cs.end(), color_start);
+        std::copy(order.begin(), order.end(), cell_order);
+    });
+}
+
+// ---------------- case generator ----------------
+axs_status axs_mesh_multiblock(axs_mesh** out, int n_blocks, const axs_block_spec* blocks) {
+    if (!out) return AXS_ERR_INVALID;
+    *out = nullptr;
+    return guarded(nullptr, [&] {
+        axs_mesh* h = new axs_mesh();
+        const std::string err = buildMultiblock(h->m, n_blocks, blocks);
+        if (!err.empty()) {
+            delete h;
+            throw std::invalid_argument(err);
+        }
+        *out = h;
+    });
+}
+
+axs_status axs_mesh_triangles(axs_mesh** out, int n_points, const double* pz, const double* pr, int n_tris,
+                              const int* tris, int n_seg, const double* sz0, const double* sr0, const double* sz1,
+                              const double* sr1, const int* sg, double tol) {
+    if (!out || n_points <= 0 || n_tris <= 0 || !pz || !pr || !tris) return AXS_ERR_INVALID;
+    if (n_seg > 0 && (!sz0 || !sr0 || !sz1 || !sr1 || !sg)) return AXS_ERR_INVALID;
+    *out = nullptr;
+    return guarded(nullptr, [&] {
+        axs_mesh* h = new axs_mesh();
+        std::vector<double> z(pz, pz + n_points), r(pr, pr + n_points);
+        std::vector<int> t(tris, tris + 3 * (size_t)n_tris);
+        const std::string err = buildTriangles(h->m, z, r, t, n_seg, sz0, sr0, sz1, sr1, sg, tol);
+        if (!err.empty()) {
+            delete h;
+            throw std::invalid_argument(err);
+        }
+        *out = h;
+    });
+}
+
+axs_status axs_mesh_nozzle(axs_mesh** out, const axs_nozzle_spec* spec) {
+    if (!out || !spec) return AXS_ERR_INVALID;
+    *out = nullptr;
+    return guarded(nullptr, [&] {
+        axs_mesh* h = new axs_mesh();
+        const std::string err = buildNozzle(h->m, *spec);
+        if (!err.empty()) {
+            delete h;
+            throw std::invalid_argument(err);
+        }
+        *out = h;
+    });
+}
+
+axs_status axs_mesh_get_view(const axs_mesh* h, axs_mesh_view* v) {
+    if (!h || !v) return AXS_ERR_INVALID;
+    *v = h->m.view();
+    return AXS_OK;
+}
+
+axs_status axs_mesh_get_triangles(const axs_mesh* h, int* n_points, double* pz, double* pr, int* n_tris, int* tris) {
+    if (!h) return AXS_ERR_INVALID;
+    if (n_points) *n_points = (int)h->m.pointZ.size();
+    if (n_tris) *n_tris = (int)h->m.tris.size() / 3;
+    if (pz) std::copy(h->m.pointZ.begin(), h->m.pointZ.end(), pz);
+    if (pr) std::copy(h->m.pointR.begin(), h->m.pointR.end(), pr);
+    if (tris) std::copy(h->m.tris.begin(), h->m.tris.end(), tris);
+    return AXS_OK;
+}
+
+void axs_mesh_destroy(axs_mesh* h) { delete h; }
+
+}  // extern "C"

@@ -1,0 +1,909 @@
+// solver.cpp -- host driver of the MI355X SIMPLE solver.
+//
+// Solver::runSimple (solver/source/solver.cpp:753-1396) restated for one GPU:
+// the whole outer iteration -- assembly, 2 x linearMaxIter momentum sweeps,
+// Rhie-Chow, the pressure-correction multigrid solve(s), the corrections and
+// the residual norms -- is captured once as a hipGraph and replayed per
+// iteration.  The convergence test runs on the device (kFinish), which raises
+// ctl->done; the host launches graphs in growing batches and polls ctl between
+// batches, so there is no per-iteration host synchronisation and no iteration
+// runs past the one the reference would stop at.
+#include "solver.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <functional>
+
+namespace axs {
+
+namespace {
+constexpr int kSmoothBlockMax = 2048;  // cells a single workgroup smooths in LDS
+}
+
+void Arena::release() {
+    for (void* p : ptrs_) (void)hipFree(p);
+    ptrs_.clear();
+}
+
+// ---------------------------------------------------------------------------
+// MGDevice
+// ---------------------------------------------------------------------------
+void MGDevice::build(int n, const std::vector<int>& start, const std::vector<int>& nb, int maxLevels) {
+    host = buildHierarchy(n, start, nb, maxLevels);
+    arena.release();
+    dev.assign(host.size(), DLevel{});
+    for (size_t l = 0; l < host.size(); l++) {
+        const Level& h = host[l];
+        DLevel& d = dev[l];
+        d.n = h.n;
+        d.refs = h.refs();
+        d.start = arena.upload(h.start);
+        d.nb = arena.upload(h.nb);
+        if (l > 0) {
+            d.AC = arena.zeros<double>(h.n);
+            d.b = arena.zeros<double>(h.n);
+            d.AF = arena.zeros<double>(h.refs());
+            d.x = arena.zeros<double>(h.n);
+            d.xNew = arena.zeros<double>(h.n);
+        }
+        if (!h.coarsest()) {
+            d.nCoarse = host[l + 1].n;
+            d.toCoarse = arena.upload(h.toCoarse);
+            d.slotToCoarseSlot = arena.upload(h.slotToCoarseSlot);
+            d.memberStart = arena.upload(h.memberStart);
+            d.members = arena.upload(h.members);
+            d.cslotStart = arena.upload(h.cslotStart);
+            d.cslotFine = arena.upload(h.cslotFine);
+        }
+    }
+}
+
+void MGDevice::bindLevel0(double* AC, double* b, double* AF, double* x, double* xNew) {
+    dev[0].AC = AC;
+    dev[0].b = b;
+    dev[0].AF = AF;
+    dev[0].x = x;
+    dev[0].xNew = xNew;
+    x0Home_ = x;
+    x0Other_ = xNew;
+}
+
+void MGDevice::swapX(int l) { std::swap(dev[l].x, dev[l].xNew); }
+
+// smoothen / smoothenRegular / smoothenSingleBlock, multigrid.cu:664-725
+void MGDevice::smooth(hipStream_t s, int l, int sweeps, const MGParams& p, const Ctl* ctl) {
+    DLevel& L = dev[l];
+    if (L.n <= 0 || sweeps <= 0) return;
+    if (sweeps > 1 && L.n <= kSmoothBlockMax) {
+        launchMGSmoothBlock(s, L, p.weight, sweeps, ctl);
+        return;
+    }
+    for (int i = 0; i < sweeps; i++) {
+        launchMGJacobi(s, L, p.weight, ctl);
+        swapX(l);
+    }
+}
+
+// vCycle, multigrid.cu:607-662
+void MGDevice::vcycle(hipStream_t s, int l, const MGParams& p, const Ctl* ctl) {
+    if (host[l].coarsest()) {
+        smooth(s, l, p.coarseSweeps, p, ctl);
+        return;
+    }
+    smooth(s, l, p.pre, p, ctl);
+    launchMGRestrict(s, dev[l], dev[l + 1], ctl);
+    vcycle(s, l + 1, p, ctl);
+    if (p.post >= 1) {
+        launchMGProlongSmooth(s, dev[l], dev[l + 1], p.weight, ctl);
+        swapX(l);
+        smooth(s, l, p.post - 1, p, ctl);
+    } else {
+        launchMGProlong(s, dev[l], dev[l + 1], ctl);
+    }
+}
+
+void MGDevice::enqueueSolve(hipStream_t s, const MGParams& p, const Ctl* ctl, bool buildOperators) {
+    if (buildOperators)
+        for (size_t l = 1; l < dev.size(); l++) launchMGCoarseOp(s, dev[l - 1], dev[l], ctl);
+    const int cycles = p.cycles > 1 ? p.cycles : 1;  // cycleCount, multigrid.cuh:226
+    for (int c = 0; c < cycles; c++) vcycle(s, 0, p, ctl);
+    if (dev[0].x != x0Home_) {
+        launchCopy(s, x0Home_, dev[0].x, dev[0].n, ctl);
+        dev[0].x = x0Home_;
+        dev[0].xNew = x0Other_;
+    }
+}
+
+// Algorithmic bytes of one solve (distinct elements read + written, FP64 / int32).
+double MGDevice::bytesPerSolve(const MGParams& p, bool buildOperators) const {
+    double bytes = 0.0;
+    const int L = (int)host.size();
+    auto jac = [&](int l) { return 36.0 * host[l].n + 12.0 * host[l].refs(); };
+    auto smoothBytes = [&](int l, int sweeps) {
+        if (sweeps <= 0) return 0.0;
+        if (sweeps > 1 && host[l].n <= kSmoothBlockMax) return 32.0 * host[l].n + 12.0 * host[l].refs();
+        return sweeps * jac(l);
+    };
+    if (buildOperators)
+        for (int l = 1; l < L; l++) {
+            const Level& f = host[l - 1];
+            const Level& c = host[l];
+            bytes += 16.0 * f.n + 16.0 * f.refs() + 4.0 * f.cslotFine.size() + 16.0 * c.n + 12.0 * c.refs();
+        }
+    const int cycles = p.cycles > 1 ? p.cycles : 1;
+    double cyc = 0.0;
+    for (int l = 0; l < L; l++) {
+        if (host[l].coarsest()) {
+            cyc += smoothBytes(l, p.coarseSweeps);
+            break;
+        }
+        const double nc = host[l + 1].n;
+        cyc += smoothBytes(l, p.pre);
+        cyc += 32.0 * host[l].n + 12.0 * host[l].refs() + 20.0 * nc;  // residual + restriction
+        if (p.post >= 1) cyc += 40.0 * host[l].n + 12.0 * host[l].refs() + 8.0 * nc + smoothBytes(l, p.post - 1);
+        else cyc += 20.0 * host[l].n + 8.0 * nc;
+    }
+    return bytes + cycles * cyc;
+}
+
+// ---------------------------------------------------------------------------
+// Solver
+// ---------------------------------------------------------------------------
+Solver::Solver(int device) : device_(device) {
+    AXS_HIP(hipSetDevice(device_));
+    AXS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    AXS_HIP(hipHostMalloc((void**)&hctl_, sizeof(Ctl), hipHostMallocDefault));
+    std::memset(hctl_, 0, sizeof(Ctl));
+    axs_config_defaults(&cfg_);
+}
+
+Solver::~Solver() {
+    (void)hipSetDevice(device_);
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (exec_) (void)hipGraphExecDestroy(exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    if (hctl_) (void)hipHostFree(hctl_);
+    if (hhist_) (void)hipHostFree(hhist_);
+    mg_.arena.release();
+    stateArena_.release();
+    meshArena_.release();
+    bcArena_.release();
+    colorArena_.release();
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Solver::setMesh(const axs_mesh_view& v) {
+    if (running_) throw std::logic_error("solver is running");
+    if (v.n_cells <= 0 || v.n_faces <= 0) throw std::invalid_argument("mesh has no cells or faces");
+    const int N = v.n_cells, F = v.n_faces;
+    if (!v.cell_face_start || v.cell_face_start[0] != 0) throw std::invalid_argument("cell_face_start[0] must be 0");
+    for (int c = 0; c < N; c++)
+        if (v.cell_face_start[c + 1] < v.cell_face_start[c]) throw std::invalid_argument("cell_face_start not monotone");
+    const int S = v.cell_face_start[N];
+    for (int k = 0; k < S; k++)
+        if (v.cell_face_ids[k] < 0 || v.cell_face_ids[k] >= F) throw std::invalid_argument("cell_face_ids out of range");
+    for (int f = 0; f < F; f++) {
+        if (v.face_owner[f] < 0 || v.face_owner[f] >= N) throw std::invalid_argument("face_owner out of range");
+        if (v.face_neighbor[f] < -1 || v.face_neighbor[f] >= N) throw std::invalid_argument("face_neighbor out of range");
+    }
+    HostMesh m;
+    m.nCells = N;
+    m.nFaces = F;
+    m.cellCenterZ.assign(v.cell_center_z, v.cell_center_z + N);
+    m.cellCenterR.assign(v.cell_center_r, v.cell_center_r + N);
+    m.cellVolume.assign(v.cell_volume, v.cell_volume + N);
+    m.cellArea2D.assign(v.cell_area2d, v.cell_area2d + N);
+    m.cellFaceStart.assign(v.cell_face_start, v.cell_face_start + N + 1);
+    m.cellFaceIDs.assign(v.cell_face_ids, v.cell_face_ids + S);
+    m.faceOwner.assign(v.face_owner, v.face_owner + F);
+    m.faceNeighbor.assign(v.face_neighbor, v.face_neighbor + F);
+    m.faceGroup.assign(v.face_group, v.face_group + F);
+    m.faceNormalZ.assign(v.face_normal_z, v.face_normal_z + F);
+    m.faceNormalR.assign(v.face_normal_r, v.face_normal_r + F);
+    m.faceCenterZ.assign(v.face_center_z, v.face_center_z + F);
+    m.faceCenterR.assign(v.face_center_r, v.face_center_r + F);
+    m.faceArea.assign(v.face_area, v.face_area + F);
+    mesh_ = std::move(m);
+    computeGeometry(mesh_, geom_);
+    cellNeighbourCSR(mesh_, csrStart_, csrNb_);
+    haveMesh_ = true;
+    meshUploaded_ = false;
+    mgBuilt_ = false;
+    nColors_ = 0;
+    colorStart_.clear();
+}
+
+void Solver::setBC(int var, const axs_bc_view& v) {
+    if (running_) throw std::logic_error("solver is running");
+    if (var < 0 || var > 2) throw std::invalid_argument("bad variable");
+    if (v.n_groups < 0 || (v.n_groups > 0 && (!v.type || !v.value))) throw std::invalid_argument("bad bc view");
+    HostBC h;
+    h.nGroups = v.n_groups;
+    for (int g = 0; g < v.n_groups; g++) {
+        const uint8_t t = v.type[g];
+        if (t > AXS_BC_PULSATILE) throw std::invalid_argument("bad bc type");
+        if (t == AXS_BC_MICHAELIS_MENTEN || t == AXS_BC_HILL)
+            throw std::domain_error("reactive-wall kinetics are not part of the momentum / pressure path");
+        h.pulsatile.push_back(t == AXS_BC_PULSATILE);
+        h.type.push_back(t == AXS_BC_PULSATILE ? (uint8_t)AXS_BC_DIRICHLET : t);  // memory_manger.cpp:96-100
+        h.value.push_back(v.value[g]);
+        h.length.push_back(v.length ? v.length[g] : 0.0);
+        h.amplitude.push_back(v.amplitude ? v.amplitude[g] : 0.0);
+        h.frequency.push_back(v.frequency ? v.frequency[g] : 0.0);
+    }
+    bc_[var] = std::move(h);
+}
+
+void Solver::configure(const axs_config& c) {
+    if (running_) throw std::logic_error("solver is running");
+    if (c.precision != AXS_PREC_FP64 && c.precision != AXS_PREC_MIXED) throw std::invalid_argument("bad precision");
+    cfg_ = c;
+    haveCfg_ = true;
+}
+
+void Solver::uploadMesh() {
+    meshArena_.release();
+    const HostMesh& m = mesh_;
+    DMesh d{};
+    d.N = m.nCells;
+    d.F = m.nFaces;
+    d.S = (int)m.cellFaceIDs.size();
+    d.cZ = meshArena_.upload(m.cellCenterZ);
+    d.cR = meshArena_.upload(m.cellCenterR);
+    d.vol = meshArena_.upload(m.cellVolume);
+    d.invA2D = meshArena_.upload(geom_.invA2D);
+    d.lsqInvZZ = meshArena_.upload(geom_.lsqInvZZ);
+    d.lsqInvZR = meshArena_.upload(geom_.lsqInvZR);
+    d.lsqInvRR = meshArena_.upload(geom_.lsqInvRR);
+    d.faceStart = meshArena_.upload(m.cellFaceStart);
+    d.faceIDs = meshArena_.upload(m.cellFaceIDs);
+    d.slotNb = meshArena_.upload(csrNb_);
+    d.owner = meshArena_.upload(m.faceOwner);
+    d.neighbor = meshArena_.upload(m.faceNeighbor);
+    d.group = meshArena_.upload(m.faceGroup);
+    d.nZ = meshArena_.upload(m.faceNormalZ);
+    d.nR = meshArena_.upload(m.faceNormalR);
+    d.fZ = meshArena_.upload(m.faceCenterZ);
+    d.fR = meshArena_.upload(m.faceCenterR);
+    d.area = meshArena_.upload(m.faceArea);
+    d.invDPN = meshArena_.upload(geom_.invDPN);
+    d.wP = meshArena_.upload(geom_.wP);
+    d.dPB = meshArena_.upload(geom_.dPB);
+    d.lsqWZ = meshArena_.upload(geom_.lsqWZ);
+    d.lsqWR = meshArena_.upload(geom_.lsqWR);
+    d.length2D = meshArena_.upload(geom_.length2D);
+    dm_ = d;
+    meshUploaded_ = true;
+}
+
+// allocateSimple / allocateCoefficients / allocateResiduals: zero initial state
+// (memory_manger.cpp:270-285, 691-773, 802-832)
+void Solver::allocateState() {
+    stateArena_.release();
+    const int N = dm_.N, F = dm_.F, S = dm_.S;
+    DState s{};
+    for (double** p : {&s.u, &s.v, &s.p, &s.pp, &s.uTmp, &s.vTmp, &s.ppTmp, &s.DU, &s.DV, &s.gPZ, &s.gPR, &s.gUZ,
+                       &s.gUR, &s.gVZ, &s.gVR, &s.uOld, &s.vOld, &s.uOld2, &s.vOld2, &s.uAC, &s.ub, &s.uInv, &s.vAC,
+                       &s.vb, &s.vInv, &s.ppAC, &s.ppb, &s.ppInv})
+        *p = stateArena_.zeros<double>(N);
+    s.mDot = stateArena_.zeros<double>(F);
+    s.AFuv = stateArena_.zeros<double>(S);
+    s.ppAF = stateArena_.zeros<double>(S);
+    partialBlocks_ = blocksFor(N);
+    s.partial = stateArena_.zeros<double>((size_t)partialBlocks_ * 9);
+    s.ctl = stateArena_.zeros<Ctl>(1);
+    s.hist = nullptr;
+    histCap_ = 0;
+    ds_ = s;
+    std::memset(hctl_, 0, sizeof(Ctl));
+    stateAllocated_ = true;
+}
+
+// createBoundaryFieldHost/Device (memory_manger.cpp:51-189) with the value of
+// each pulsatile group at `time` (structs/boundary_struct.h:184-192)
+void Solver::uploadBCs(double time) {
+    const double twoPi = 2.0 * 3.14159265358979323846;
+    auto values = [&](const HostBC& h) {
+        std::vector<double> v(h.value);
+        for (int g = 0; g < h.nGroups; g++)
+            if (h.pulsatile[g]) v[g] = h.value[g] * (1.0 + h.amplitude[g] * std::sin(twoPi * h.frequency[g] * time));
+        return v;
+    };
+    if (bcValueDev_[0] == nullptr) {
+        bcArena_.release();
+        for (int var = 0; var < 3; var++) {
+            const HostBC& h = bc_[var];
+            DBC d{};
+            d.nGroups = h.nGroups;
+            d.type = bcArena_.upload(h.type);
+            bcValueDev_[var] = bcArena_.upload(values(h));
+            d.value = bcValueDev_[var];
+            d.length = bcArena_.upload(h.length);
+            dbc_[var] = d;
+        }
+        // ppBC: the p types with every value zeroed (solver.cpp:788-795)
+        ppbc_ = dbc_[AXS_VAR_P];
+        ppbc_.value = bcArena_.zeros<double>(std::max(1, bc_[AXS_VAR_P].nGroups));
+        return;
+    }
+    for (int var = 0; var < 2; var++) {
+        const std::vector<double> v = values(bc_[var]);
+        if (!v.empty())
+            AXS_HIP(hipMemcpy(bcValueDev_[var], v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+}
+
+Physics Solver::physics() const {
+    const axs_config& c = cfg_;
+    Physics ph{};
+    ph.rho = c.rho;
+    ph.mu = c.mu;
+    ph.alphaU = c.alpha_u;
+    ph.alphaP = c.alpha_p;
+    ph.dt = c.transient ? c.dt : 0.0;
+    ph.transient = c.transient ? 1 : 0;
+    ph.conv = c.add_convection ? 1 : 0;
+    // QUICK is clamped to upwind on load (file_manager.cpp:273-275) and falls
+    // to upwind inside the kernel (solver_util.cuh:765-766)
+    ph.scheme = (c.convection_scheme == AXS_CONV_QUICK) ? AXS_CONV_UPWIND : c.convection_scheme;
+    ph.gradScheme = c.gradient_scheme;
+    ph.nonOrtho = c.use_non_orth ? 1 : 0;
+    return ph;
+}
+
+// One SIMPLE outer iteration, solver.cpp:1116-1273
+void Solver::enqueueIteration(hipStream_t s) {
+    const axs_config& c = cfg_;
+    const Physics ph = physics();
+    const bool needGrad = c.add_convection && (c.convection_scheme == AXS_CONV_SOU || c.convection_scheme == AXS_CONV_QUICK);
+    const DBC &bu = dbc_[AXS_VAR_U], &bv = dbc_[AXS_VAR_V], &bp = dbc_[AXS_VAR_P];
+    DState st = ds_;
+
+    if (ph.nonOrtho || needGrad) launchGradUV(s, dm_, bu, bv, st, ph.gradScheme);
+    launchMomentum(s, dm_, bu, bv, bp, st, ph);
+
+    // momentum solves (LinearSolver::run, linear_solver.cu:91-148, 209-271)
+    if (c.linear_solver == AXS_LINEAR_GS_RB && nColors_ > 0) {
+        for (int it = 0; it < c.linear_max_iter; it++)
+            for (int col = 0; col < nColors_; col++) {
+                const int begin = colorStart_[col], count = colorStart_[col + 1] - begin;
+                if (count > 0) launchGSColorUV(s, dm_, st, dOrder_, begin, count);
+            }
+    } else {
+        double *ux = st.u, *ut = st.uTmp, *vx = st.v, *vt = st.vTmp;
+        for (int it = 0; it < c.linear_max_iter; it++) {
+            launchJacobiUV(s, dm_, st, ux, vx, ut, vt);
+            std::swap(ux, ut);
+            std::swap(vx, vt);
+        }
+        if (ux != st.u) {
+            launchCopy(s, st.u, ux, dm_.N, st.ctl);
+            launchCopy(s, st.v, vx, dm_.N, st.ctl);
+        }
+    }
+
+    launchPPCoeff(s, dm_, bp, st, ph);
+    launchFaceFlux(s, dm_, bu, bv, bp, st, ph);
+    launchFill(s, st.pp, 0.0, dm_.N, st.ctl);
+
+    MGParams mp;
+    mp.cycles = c.mg_max_iter;
+    mp.weight = c.mg_weight;
+    mp.pre = c.mg_pre_sweeps;
+    mp.post = c.mg_post_sweeps;
+    mp.coarseSweeps = c.mg_coarse_sweeps;
+    mp.precision = c.precision;
+    mp.coarseDirect = c.coarse_direct;
+    for (int corr = 0; corr <= ph.nonOrtho; corr++) {
+        const int cross = corr > 0 ? 1 : 0;
+        if (cross) launchGradField(s, dm_, ppbc_, st.pp, st.gPZ, st.gPR, ph.gradScheme, st.ctl);
+        launchPPRhs(s, dm_, st, ph, cross);
+        if (c.use_multigrid) {
+            // the operator is the same for every corrector pass: build it once
+            mg_.enqueueSolve(s, mp, st.ctl, corr == 0);
+        } else {
+            // ppSolver: the configured linear solver on p' (solver.cpp:921-924, 1203-1208)
+            if (c.linear_solver == AXS_LINEAR_GS_RB && nColors_ > 0) {
+                for (int it = 0; it < c.linear_max_iter; it++)
+                    for (int col = 0; col < nColors_; col++) {
+                        const int begin = colorStart_[col], count = colorStart_[col + 1] - begin;
+                        if (count > 0)
+                            launchGSColorSys(s, dm_, st.ppb, st.ppInv, st.ppAF, st.pp, dOrder_, begin, count, st.ctl);
+                    }
+            } else {
+                double *x = st.pp, *t = st.ppTmp;
+                for (int it = 0; it < c.linear_max_iter; it++) {
+                    launchJacobiSys(s, dm_, st.ppb, st.ppInv, st.ppAF, x, t, st.ctl);
+                    std::swap(x, t);
+                }
+                if (x != st.pp) launchCopy(s, st.pp, x, dm_.N, st.ctl);
+            }
+        }
+    }
+    launchGradPPUpdate(s, dm_, ppbc_, st, ph);
+    launchMassFluxUpdate(s, dm_, bp, st, ph);
+    launchResidual(s, dm_, st, partialBlocks_);
+    launchFinish(s, st, partialBlocks_);
+}
+
+void Solver::buildGraph() {
+    if (exec_) {
+        AXS_HIP(hipGraphExecDestroy(exec_));
+        exec_ = nullptr;
+    }
+    if (graph_) {
+        AXS_HIP(hipGraphDestroy(graph_));
+        graph_ = nullptr;
+    }
+    AXS_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    try {
+        enqueueIteration(stream_);
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(stream_, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    AXS_HIP(hipStreamEndCapture(stream_, &graph_));
+    AXS_HIP(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+    AXS_HIP(hipGraphUpload(exec_, stream_));
+}
+
+void Solver::launchBatch(int n) {
+    for (int i = 0; i < n; i++) AXS_HIP(hipGraphLaunch(exec_, stream_));
+}
+
+// Read ctl and the new history entries, report them.
+void Solver::pullControl(axs_progress_cb cb, void* user) {
+    const int before = hctl_->nHist;
+    AXS_HIP(hipMemcpyAsync(hctl_, ds_.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream_));
+    AXS_HIP(hipStreamSynchronize(stream_));
+    const int after = hctl_->nHist;
+    if (after > before) {
+        const int cnt = after - before;
+        AXS_HIP(hipMemcpy(hhist_, ds_.hist + (size_t)before * 4, (size_t)cnt * 4 * sizeof(double),
+                          hipMemcpyDeviceToHost));
+        static const char* names[3] = {"U", "V", "Continuity"};
+        for (int i = 0; i < cnt; i++) {
+            const double* r = hhist_ + (size_t)i * 4;
+            for (int j = 0; j < 3; j++) history_[j].push_back(r[j]);
+            if (cb) cb(user, (int)r[3], 3, names, r);
+        }
+    }
+}
+
+void Solver::run(axs_progress_cb cb, void* user, bool continueRun) {
+    if (!haveMesh_) throw std::logic_error("no mesh");
+    bool expected = false;
+    if (!running_.compare_exchange_strong(expected, true)) throw std::logic_error("solver is already running");
+    struct Guard {
+        std::atomic<bool>& r;
+        ~Guard() { r = false; }
+    } guard{running_};
+    AXS_HIP(hipSetDevice(device_));
+    const auto t0 = std::chrono::steady_clock::now();
+    axs_config& c = cfg_;
+    // runCheck, solver.cpp:530-661
+    if (c.max_iter < 1) c.max_iter = 50;
+    if (c.check_conv < 1) c.check_conv = 1;
+    if (c.linear_max_iter < 1) c.linear_max_iter = 20;
+    if (c.structured_mesh) c.use_non_orth = 0;
+    if (c.transient && (!std::isfinite(c.dt) || c.dt <= 0.0 || !std::isfinite(c.t_end) || c.t_end <= 0.0))
+        throw std::invalid_argument("Transient run needs a positive time step and end time.");
+    if (!std::isfinite(c.rho) || !std::isfinite(c.mu) || c.rho <= 0.0 || c.mu <= 0.0)
+        throw std::invalid_argument("Solver needs positive density and viscosity.");
+    for (int v = 0; v < 2; v++)
+        for (int g = 0; g < bc_[v].nGroups; g++)
+            if (bc_[v].pulsatile[g]) {
+                if (!c.transient) throw std::invalid_argument("Pulsatile inlet requires Transient Solver to be enabled.");
+                if (!std::isfinite(bc_[v].value[g]) || !(bc_[v].amplitude[g] >= 0.0) || !(bc_[v].frequency[g] > 0.0))
+                    throw std::invalid_argument("Pulsatile inlet needs a finite mean velocity, A >= 0, and f > 0 Hz.");
+            }
+    if (c.precision != AXS_PREC_FP64) throw std::domain_error("mixed precision is not built in this library");
+
+    if (!meshUploaded_) uploadMesh();
+    std::string why;
+    const bool cont = continueRun && canContinue(why);
+    if (!cont) {
+        allocateState();
+        currentIteration_ = 0;
+        for (auto& h : history_) h.clear();
+        contCells_ = mesh_.nCells;
+        contFaces_ = mesh_.nFaces;
+        contRefs_ = (int)mesh_.cellFaceIDs.size();
+    }
+    bcValueDev_[0] = nullptr;
+    uploadBCs(0.0);
+    if (!mgBuilt_ || mgBuiltLevels_ != c.mg_max_levels) {
+        mg_.build(mesh_.nCells, csrStart_, csrNb_, c.mg_max_levels);
+        mgBuilt_ = true;
+        mgBuiltLevels_ = c.mg_max_levels;
+    }
+    mg_.bindLevel0(ds_.ppAC, ds_.ppb, ds_.ppAF, ds_.pp, ds_.ppTmp);
+    if (c.linear_solver == AXS_LINEAR_GS_RB && nColors_ == 0) {
+        std::vector<int> order;
+        nColors_ = greedyColoring(mesh_.nCells, csrStart_.data(), csrNb_.data(), colorStart_, order);
+        colorArena_.release();
+        dOrder_ = colorArena_.upload(order);
+    }
+    // residual history buffer for this run
+    const int cap = c.max_iter + 1;
+    if (cap > histCap_) {
+        ds_.hist = stateArena_.zeros<double>((size_t)cap * 4);
+        histCap_ = cap;
+    }
+    if (cap > hhistCap_) {
+        if (hhist_) AXS_HIP(hipHostFree(hhist_));
+        AXS_HIP(hipHostMalloc((void**)&hhist_, (size_t)cap * 4 * sizeof(double), hipHostMallocDefault));
+        hhistCap_ = cap;
+    }
+    buildGraph();
+
+    // control block (scaleVal persists across continued runs like the pinned
+    // ConfigResidual::scaleVal does)
+    AXS_HIP(hipMemcpy(hctl_, ds_.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    Ctl& k = *hctl_;
+    k.done = 0;
+    k.iter = currentIteration_;
+    k.k = 0;
+    k.maxIter = c.max_iter;
+    k.checkConv = c.check_conv;
+    k.transient = c.transient ? 1 : 0;
+    k.bdf2Step = 0;
+    k.nHist = 0;
+    k.histCap = histCap_;
+    k.converged = 0;
+    k.N = mesh_.nCells;
+    for (int i = 0; i < 3; i++) {
+        k.enabled[i] = c.residual[i].enabled;
+        k.norm[i] = c.residual[i].norm;
+        k.scaling[i] = c.residual[i].scaling;
+        k.tol[i] = c.residual[i].tol;
+    }
+    AXS_HIP(hipMemcpy(ds_.ctl, hctl_, sizeof(Ctl), hipMemcpyHostToDevice));
+    stop_ = false;
+    stopped_ = false;
+    converged_ = false;
+    const auto t1 = std::chrono::steady_clock::now();
+    setupMs_ = std::chrono::duration<double, std::milli>(t1 - t0).count();
+
+    auto iterateUntilDone = [&]() {
+        int batch = 1;
+        while (true) {
+            if (stop_) {
+                stopped_ = true;
+                break;
+            }
+            const int remaining = hctl_->maxIter - hctl_->k;
+            launchBatch(std::max(1, std::min(batch, remaining)));
+            pullControl(cb, user);
+            if (hctl_->done) break;
+            batch = std::min(batch * 2, 64);
+        }
+    };
+    if (!c.transient) {
+        iterateUntilDone();
+    } else {
+        const int steps = (int)std::ceil(c.t_end / c.dt);
+        const bool bdf2 = c.time_scheme == AXS_TIME_BDF2;
+        for (int t = 0; t < steps && !stopped_; t++) {
+            AXS_HIP(hipStreamSynchronize(stream_));
+            uploadBCs((double)(t + 1) * c.dt);
+            launchStepBegin(stream_, dm_, ds_, bdf2 ? 1 : 0);
+            AXS_HIP(hipMemcpyAsync(hctl_, ds_.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream_));
+            AXS_HIP(hipStreamSynchronize(stream_));
+            hctl_->done = 0;
+            hctl_->k = 0;
+            hctl_->converged = 0;
+            hctl_->nHist = 0;
+            hctl_->bdf2Step = (bdf2 && t > 0) ? 1 : 0;
+            AXS_HIP(hipMemcpy(ds_.ctl, hctl_, sizeof(Ctl), hipMemcpyHostToDevice));
+            iterateUntilDone();
+        }
+    }
+    currentIteration_ = hctl_->iter;
+    converged_ = hctl_->converged != 0;
+    // output gradients with the final fields (solver.cpp:1330-1336)
+    launchGradField(stream_, dm_, dbc_[AXS_VAR_U], ds_.u, ds_.gUZ, ds_.gUR, c.gradient_scheme, nullptr);
+    launchGradField(stream_, dm_, dbc_[AXS_VAR_V], ds_.v, ds_.gVZ, ds_.gVR, c.gradient_scheme, nullptr);
+    launchGradField(stream_, dm_, dbc_[AXS_VAR_P], ds_.p, ds_.gPZ, ds_.gPR, c.gradient_scheme, nullptr);
+    AXS_HIP(hipStreamSynchronize(stream_));
+    AXS_HIP(hipGetLastError());
+    solveMs_ = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
+    ready_ = true;
+}
+
+// Benchmark: exactly n graph launches (outer iterations) from the current
+// state with the convergence test disabled, timed with HIP events.
+void Solver::runTimed(int n, double* ms) {
+    if (!ready_ || !exec_) throw std::logic_error("run the solver once before timing it");
+    if (n <= 0) throw std::invalid_argument("n must be positive");
+    AXS_HIP(hipSetDevice(device_));
+    AXS_HIP(hipMemcpy(hctl_, ds_.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    Ctl saved = *hctl_;
+    hctl_->done = 0;
+    hctl_->k = 0;
+    hctl_->maxIter = n;
+    hctl_->converged = 0;
+    hctl_->nHist = 0;
+    for (int i = 0; i < 3; i++) hctl_->enabled[i] = 0;
+    AXS_HIP(hipMemcpy(ds_.ctl, hctl_, sizeof(Ctl), hipMemcpyHostToDevice));
+    hipEvent_t e0, e1;
+    AXS_HIP(hipEventCreate(&e0));
+    AXS_HIP(hipEventCreate(&e1));
+    AXS_HIP(hipEventRecord(e0, stream_));
+    launchBatch(n);
+    AXS_HIP(hipEventRecord(e1, stream_));
+    AXS_HIP(hipEventSynchronize(e1));
+    float f = 0.0f;
+    AXS_HIP(hipEventElapsedTime(&f, e0, e1));
+    *ms = f;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    AXS_HIP(hipMemcpy(hctl_, ds_.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    const int iter = hctl_->iter;
+    *hctl_ = saved;
+    hctl_->iter = iter;
+    AXS_HIP(hipMemcpy(ds_.ctl, hctl_, sizeof(Ctl), hipMemcpyHostToDevice));
+    currentIteration_ = iter;
+}
+
+// Time `repeats` eager launches of one kernel from the current state.  The
+// launches run with no control block (they ignore ctl->done) and only
+// rewrite buffers the next outer iteration overwrites anyway.
+void Solver::benchKernel(int kernel, int repeats, double* ms, double* bytes) {
+    if (!ready_) throw std::logic_error("run the solver once before timing a kernel");
+    if (repeats <= 0) throw std::invalid_argument("repeats must be positive");
+    AXS_HIP(hipSetDevice(device_));
+    AXS_HIP(hipStreamSynchronize(stream_));
+    DState st = ds_;
+    st.ctl = nullptr;
+    const Physics ph = physics();
+    const double N = dm_.N, F = dm_.F, S = dm_.S;
+    const bool lsq = cfg_.gradient_scheme == AXS_GRAD_LSQ;
+    std::function<void()> launch;
+    switch (kernel) {
+    case AXS_KERNEL_JACOBI_UV:
+        // reads b, invAC, x of u and v + CSR + shared AF; writes x of u and v
+        *bytes = 68.0 * N + 12.0 * S;
+        launch = [&] { launchJacobiUV(stream_, dm_, st, st.u, st.v, st.uTmp, st.vTmp); };
+        break;
+    case AXS_KERNEL_MOMENTUM: {
+        const bool needGrad = ph.nonOrtho || (cfg_.add_convection && ph.scheme == AXS_CONV_SOU);
+        *bytes = 4.0 * N + 24.0 * N + (lsq ? 24.0 * N : 8.0 * N) + 24.0 * N + 80.0 * N + 12.0 * F + 88.0 * F +
+                 12.0 * S + (needGrad ? 32.0 * N : 0.0);
+        launch = [&] { launchMomentum(stream_, dm_, dbc_[0], dbc_[1], dbc_[2], st, ph); };
+        break;
+    }
+    case AXS_KERNEL_MG_SMOOTH: {
+        if (mg_.dev.empty()) throw std::logic_error("no multigrid hierarchy");
+        *bytes = 36.0 * N + 12.0 * S;
+        DLevel L = mg_.dev[0];
+        launch = [&, L] { launchMGJacobi(stream_, L, cfg_.mg_weight, nullptr); };
+        break;
+    }
+    case AXS_KERNEL_FACE_FLUX:
+        *bytes = 68.0 * F + 56.0 * N;
+        launch = [&] { launchFaceFlux(stream_, dm_, dbc_[0], dbc_[1], dbc_[2], st, ph); };
+        break;
+    case AXS_KERNEL_PP_COEFF:
+        *bytes = 36.0 * N + 12.0 * S + 60.0 * F;
+        launch = [&] { launchPPCoeff(stream_, dm_, dbc_[2], st, ph); };
+        break;
+    default: throw std::invalid_argument("unknown kernel id");
+    }
+    launch();  // warm-up
+    hipEvent_t e0, e1;
+    AXS_HIP(hipEventCreate(&e0));
+    AXS_HIP(hipEventCreate(&e1));
+    AXS_HIP(hipEventRecord(e0, stream_));
+    for (int i = 0; i < repeats; i++) launch();
+    AXS_HIP(hipEventRecord(e1, stream_));
+    AXS_HIP(hipEventSynchronize(e1));
+    float f = 0.0f;
+    AXS_HIP(hipEventElapsedTime(&f, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    AXS_HIP(hipGetLastError());
+    *ms = (double)f / repeats;
+}
+
+bool Solver::canContinue(std::string& reason) const {  // solver.cpp:469-528
+    if (running_ && !stateAllocated_) {
+        reason = "Solver is already running.";
+        return false;
+    }
+    if (!ready_ || !stateAllocated_) {
+        reason = "Run the solver once before continuing.";
+        return false;
+    }
+    if (mesh_.nCells != contCells_) {
+        reason = "Mesh cell count changed.";
+        return false;
+    }
+    if (mesh_.nFaces != contFaces_) {
+        reason = "Mesh face count changed.";
+        return false;
+    }
+    if ((int)mesh_.cellFaceIDs.size() != contRefs_) {
+        reason = "Mesh connectivity changed.";
+        return false;
+    }
+    return true;
+}
+
+size_t Solver::fieldLength(int field) const {
+    if (!haveMesh_) return 0;
+    const size_t N = mesh_.nCells, F = mesh_.nFaces, S = mesh_.cellFaceIDs.size();
+    if (field >= AXS_FIELD_U && field <= AXS_FIELD_CELL_RE) return N;
+    if (field == AXS_FIELD_MDOT) return F;
+    if (field == AXS_FIELD_DU || field == AXS_FIELD_DV) return N;
+    if (field >= AXS_FIELD_U_AC && field <= AXS_FIELD_PP_AF)
+        return ((field - AXS_FIELD_U_AC) % 4 == 3) ? S : N;
+    switch (field) {
+    case AXS_GEOM_INV_A2D:
+    case AXS_GEOM_LSQ_INV_ZZ:
+    case AXS_GEOM_LSQ_INV_ZR:
+    case AXS_GEOM_LSQ_INV_RR: return N;
+    case AXS_GEOM_INV_DPN:
+    case AXS_GEOM_WP:
+    case AXS_GEOM_DPB:
+    case AXS_GEOM_LSQ_WZ:
+    case AXS_GEOM_LSQ_WR:
+    case AXS_GEOM_LENGTH2D: return F;
+    default: return 0;
+    }
+}
+
+void Solver::getField(int field, double* out, size_t n) {
+    const size_t len = fieldLength(field);
+    if (len == 0) throw std::invalid_argument("unknown field");
+    if (n != len) throw std::out_of_range("output length does not match the field");
+    AXS_HIP(hipSetDevice(device_));
+    auto down = [&](const double* src) {
+        AXS_HIP(hipMemcpy(out, src, n * sizeof(double), hipMemcpyDeviceToHost));
+    };
+    auto host = [&](const std::vector<double>& v) { std::copy(v.begin(), v.end(), out); };
+    const bool fieldNeedsState = field < AXS_GEOM_INV_DPN;
+    if (fieldNeedsState && !stateAllocated_) throw std::logic_error("no solution yet");
+    const int N = mesh_.nCells;
+    switch (field) {
+    case AXS_FIELD_U: down(ds_.u); break;
+    case AXS_FIELD_V: down(ds_.v); break;
+    case AXS_FIELD_P: down(ds_.p); break;
+    case AXS_FIELD_PP: down(ds_.pp); break;
+    case AXS_FIELD_GRAD_UZ: down(ds_.gUZ); break;
+    case AXS_FIELD_GRAD_UR: down(ds_.gUR); break;
+    case AXS_FIELD_GRAD_VZ: down(ds_.gVZ); break;
+    case AXS_FIELD_GRAD_VR: down(ds_.gVR); break;
+    case AXS_FIELD_GRAD_PZ: down(ds_.gPZ); break;
+    case AXS_FIELD_GRAD_PR: down(ds_.gPR); break;
+    case AXS_FIELD_MDOT: down(ds_.mDot); break;
+    case AXS_FIELD_DU: down(ds_.DU); break;
+    case AXS_FIELD_DV: down(ds_.DV); break;
+    case AXS_FIELD_U_AC: down(ds_.uAC); break;
+    case AXS_FIELD_U_B: down(ds_.ub); break;
+    case AXS_FIELD_U_INVAC: down(ds_.uInv); break;
+    case AXS_FIELD_U_AF: down(ds_.AFuv); break;
+    case AXS_FIELD_V_AC: down(ds_.vAC); break;
+    case AXS_FIELD_V_B: down(ds_.vb); break;
+    case AXS_FIELD_V_INVAC: down(ds_.vInv); break;
+    case AXS_FIELD_V_AF: down(ds_.AFuv); break;
+    case AXS_FIELD_PP_AC: down(ds_.ppAC); break;
+    case AXS_FIELD_PP_B: down(ds_.ppb); break;
+    case AXS_FIELD_PP_INVAC: down(ds_.ppInv); break;
+    case AXS_FIELD_PP_AF: down(ds_.ppAF); break;
+    case AXS_FIELD_CONTINUITY: {  // getMassImbalance, solver.cpp:298-331
+        std::vector<double> md(mesh_.nFaces);
+        AXS_HIP(hipMemcpy(md.data(), ds_.mDot, md.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int c = 0; c < N; c++) {
+            double imb = 0.0;
+            for (int k = mesh_.cellFaceStart[c]; k < mesh_.cellFaceStart[c + 1]; k++) {
+                const int f = mesh_.cellFaceIDs[k];
+                if (mesh_.faceOwner[f] == c) imb += md[f];
+                else if (mesh_.faceNeighbor[f] == c) imb -= md[f];
+            }
+            out[c] = imb;
+        }
+        break;
+    }
+    case AXS_FIELD_VMAG:
+    case AXS_FIELD_CELL_RE: {  // createDerivedVelocitySolutions, solver.cpp:221-263
+        std::vector<double> u(N), v(N);
+        AXS_HIP(hipMemcpy(u.data(), ds_.u, N * sizeof(double), hipMemcpyDeviceToHost));
+        AXS_HIP(hipMemcpy(v.data(), ds_.v, N * sizeof(double), hipMemcpyDeviceToHost));
+        for (int i = 0; i < N; i++) {
+            const double sp = std::sqrt(u[i] * u[i] + v[i] * v[i]);
+            if (field == AXS_FIELD_VMAG) {
+                out[i] = sp;
+            } else {
+                const double h = std::sqrt(std::max(mesh_.cellArea2D[i], 0.0));
+                out[i] = (cfg_.mu > 0.0) ? cfg_.rho * sp * h / cfg_.mu : 0.0;
+            }
+        }
+        break;
+    }
+    case AXS_GEOM_INV_DPN: host(geom_.invDPN); break;
+    case AXS_GEOM_WP: host(geom_.wP); break;
+    case AXS_GEOM_DPB: host(geom_.dPB); break;
+    case AXS_GEOM_INV_A2D: host(geom_.invA2D); break;
+    case AXS_GEOM_LSQ_WZ: host(geom_.lsqWZ); break;
+    case AXS_GEOM_LSQ_WR: host(geom_.lsqWR); break;
+    case AXS_GEOM_LSQ_INV_ZZ: host(geom_.lsqInvZZ); break;
+    case AXS_GEOM_LSQ_INV_ZR: host(geom_.lsqInvZR); break;
+    case AXS_GEOM_LSQ_INV_RR: host(geom_.lsqInvRR); break;
+    case AXS_GEOM_LENGTH2D: host(geom_.length2D); break;
+    default: throw std::invalid_argument("unknown field");
+    }
+}
+
+int Solver::residualHistory(int res, double* out, int maxN) const {
+    if (res < 0 || res > 2) throw std::invalid_argument("bad residual index");
+    const auto& h = history_[res];
+    const int n = std::min<int>(maxN, (int)h.size());
+    if (out) std::copy(h.begin(), h.begin() + n, out);
+    return (int)h.size();
+}
+
+// Algorithmic bytes of one outer iteration of this schedule (distinct
+// elements read + written per launch; FP64 8 B, int32 4 B).
+double Solver::bytesPerIteration() const {
+    const double N = mesh_.nCells, F = mesh_.nFaces, S = (double)mesh_.cellFaceIDs.size();
+    const axs_config& c = cfg_;
+    const bool lsq = c.gradient_scheme == AXS_GRAD_LSQ;
+    const bool needGrad = c.add_convection && (c.convection_scheme == AXS_CONV_SOU || c.convection_scheme == AXS_CONV_QUICK);
+    const double gradGeom = lsq ? (28.0 * N + 4.0 * S + 24.0 * F) : (12.0 * N + 4.0 * S + 48.0 * F);
+    double b = 0.0;
+    if (c.use_non_orth || needGrad) b += gradGeom + 4 * 8.0 * N + 2 * 8.0 * N;              // grad u, v
+    b += gradGeom + 8.0 * N + 16.0 * N + 8.0 * F                                            // grad p, mDot
+         + 8.0 * 16 * F + 8.0 * 7 * N + 8.0 * S + 4.0 * S + 12.0 * F                          // faces, cells, AF
+         + 8.0 * 8 * N;                                                                     // 8 outputs
+    b += c.linear_max_iter * (68.0 * N + 12.0 * S);                                         // u, v sweeps
+    b += 16.0 * N + 4.0 * S + 8.0 * 6 * F + 8.0 * S + 16.0 * N;                             // pp coeff
+    b += 8.0 * 6 * N + 8.0 * 9 * F + 8.0 * F;                                               // face flux
+    b += 8.0 * N;                                                                           // pp = 0
+    const int passes = c.use_non_orth ? 2 : 1;
+    for (int p = 0; p < passes; p++) {
+        b += 8.0 * F + 4.0 * S + 4.0 * N + 8.0 * N;                                          // pp rhs
+        if (p > 0) b += gradGeom + 8.0 * N + 16.0 * N + 16.0 * N + 16.0 * F;                 // grad p', cross
+        MGParams mp;
+        mp.cycles = c.mg_max_iter;
+        mp.weight = c.mg_weight;
+        mp.pre = c.mg_pre_sweeps;
+        mp.post = c.mg_post_sweeps;
+        mp.coarseSweeps = c.mg_coarse_sweeps;
+        b += mg_.host.empty() ? 0.0 : mg_.bytesPerSolve(mp, p == 0);
+    }
+    b += gradGeom + 8.0 * N + 16.0 * N + 8.0 * 6 * N + 8.0 * 3 * N;                         // grad p' + update
+    b += 8.0 * 10 * F + 8.0 * 4 * N + 8.0 * F;                                              // mass flux
+    b += 8.0 * 6 * N + 12.0 * S + 8.0 * F + 4.0 * S;                                        // residuals
+    return b;
+}
+
+void Solver::stats(axs_stats& s) const {
+    std::memset(&s, 0, sizeof(s));
+    s.iterations = currentIteration_;
+    s.converged = converged_ ? 1 : 0;
+    s.stopped = stopped_ ? 1 : 0;
+    s.n_cells = mesh_.nCells;
+    s.n_faces = mesh_.nFaces;
+    s.n_face_refs = (int)mesh_.cellFaceIDs.size();
+    s.n_colors = nColors_;
+    s.mg_levels = (int)mg_.host.size();
+    for (int l = 0; l < (int)mg_.host.size() && l < 32; l++) s.mg_level_cells[l] = mg_.host[l].n;
+    s.setup_ms = setupMs_;
+    s.solve_ms = solveMs_;
+    for (int i = 0; i < 3; i++) s.last_residual[i] = history_[i].empty() ? 0.0 : history_[i].back();
+    s.bytes_per_iteration = haveMesh_ ? bytesPerIteration() : 0.0;
+    MGParams mp;
+    mp.cycles = cfg_.mg_max_iter;
+    mp.weight = cfg_.mg_weight;
+    mp.pre = cfg_.mg_pre_sweeps;
+    mp.post = cfg_.mg_post_sweeps;
+    mp.coarseSweeps = cfg_.mg_coarse_sweeps;
+    s.vcycle_bytes = mg_.host.empty() ? 0.0 : mg_.bytesPerSolve(mp, true);
+}
+
+}  // namespace axs

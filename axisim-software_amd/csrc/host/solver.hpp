@@ -1,0 +1,177 @@
+// solver.hpp -- the MI355X SIMPLE solver behind the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../hip/device.hpp"
+#include "axs.h"
+#include "mesh_types.hpp"
+#include "topology.hpp"
+
+namespace axs {
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define AXS_HIP(call)                                                                                   \
+    do {                                                                                                \
+        hipError_t e_ = (call);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
+            throw ::axs::HipError(std::string(#call) + ": " + hipGetErrorString(e_));                   \
+    } while (0)
+
+// Owns device allocations; frees them on destruction.
+class Arena {
+public:
+    Arena() = default;
+    Arena(const Arena&) = delete;
+    Arena& operator=(const Arena&) = delete;
+    ~Arena() { release(); }
+    template <class T>
+    T* alloc(size_t n) {
+        void* p = nullptr;
+        AXS_HIP(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+        ptrs_.push_back(p);
+        return (T*)p;
+    }
+    template <class T>
+    T* upload(const std::vector<T>& v) {
+        T* p = alloc<T>(v.size());
+        if (!v.empty()) AXS_HIP(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+        return p;
+    }
+    template <class T>
+    T* zeros(size_t n) {
+        T* p = alloc<T>(n);
+        AXS_HIP(hipMemset(p, 0, std::max<size_t>(n, 1) * sizeof(T)));
+        return p;
+    }
+    void release();
+
+private:
+    std::vector<void*> ptrs_;
+};
+
+// Multigrid hierarchy resident on the device.
+struct MGParams {
+    int cycles = 3;
+    double weight = 0.6;
+    int pre = 1, post = 1, coarseSweeps = 30;
+    int precision = AXS_PREC_FP64;
+    int coarseDirect = 0;
+};
+
+class MGDevice {
+public:
+    std::vector<Level> host;
+    std::vector<DLevel> dev;
+    Arena arena;
+    // Coarsest-level dense LU (FP64), for coarse_direct.
+    double* lu = nullptr;
+    int* piv = nullptr;
+
+    void build(int n, const std::vector<int>& start, const std::vector<int>& nb, int maxLevels);
+    // Level 0 operator and x buffers are owned by the caller.
+    void bindLevel0(double* AC, double* b, double* AF, double* x, double* xNew);
+    // Enqueue one solve (enqueueRun, multigrid.cu:766-811).  Leaves the answer
+    // in the level-0 buffer that was bound as x.
+    void enqueueSolve(hipStream_t s, const MGParams& p, const Ctl* ctl, bool buildOperators);
+    double bytesPerSolve(const MGParams& p, bool buildOperators) const;
+
+private:
+    void vcycle(hipStream_t s, int l, const MGParams& p, const Ctl* ctl);
+    void smooth(hipStream_t s, int l, int sweeps, const MGParams& p, const Ctl* ctl);
+    void swapX(int l);
+    double* x0Home_ = nullptr;
+    double* x0Other_ = nullptr;
+};
+
+class Solver {
+public:
+    explicit Solver(int device);
+    ~Solver();
+
+    void setMesh(const axs_mesh_view& v);
+    void setBC(int var, const axs_bc_view& v);
+    void configure(const axs_config& c);
+    void run(axs_progress_cb cb, void* user, bool continueRun);
+    void runTimed(int n, double* ms);
+    void benchKernel(int kernel, int repeats, double* ms, double* bytes);
+    void requestStop() { stop_ = true; }
+    bool canContinue(std::string& reason) const;
+    void getField(int field, double* out, size_t n);
+    size_t fieldLength(int field) const;
+    int residualHistory(int res, double* out, int maxN) const;
+    void stats(axs_stats& s) const;
+
+    std::string lastError;
+
+private:
+    struct HostBC {
+        int nGroups = 0;
+        std::vector<uint8_t> type;        // device type (PULSATILE -> DIRICHLET)
+        std::vector<uint8_t> pulsatile;
+        std::vector<double> value, length, amplitude, frequency;
+    };
+
+    void uploadMesh();
+    void allocateState();
+    void uploadBCs(double time);
+    void buildGraph();
+    void enqueueIteration(hipStream_t s);
+    Physics physics() const;
+    void launchBatch(int n);
+    void pullControl(axs_progress_cb cb, void* user);
+    double bytesPerIteration() const;
+
+    int device_ = 0;
+    hipStream_t stream_ = nullptr;
+    HostMesh mesh_;
+    bool haveMesh_ = false;
+    Geometry geom_;
+    std::vector<int> csrStart_, csrNb_;
+    axs_config cfg_{};
+    bool haveCfg_ = false;
+    HostBC bc_[3];
+
+    Arena meshArena_, stateArena_;
+    DMesh dm_{};
+    DState ds_{};
+    DBC dbc_[3]{}, ppbc_{};
+    double* bcValueDev_[3] = {nullptr, nullptr, nullptr};
+    Arena bcArena_;
+    bool meshUploaded_ = false, stateAllocated_ = false;
+    int partialBlocks_ = 0;
+    int histCap_ = 0;
+
+    MGDevice mg_;
+    bool mgBuilt_ = false;
+    int mgBuiltLevels_ = -1;
+    std::vector<int> colorStart_;
+    int* dOrder_ = nullptr;
+    Arena colorArena_;
+    int nColors_ = 0;
+
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t exec_ = nullptr;
+
+    Ctl* hctl_ = nullptr;   // pinned mirror
+    double* hhist_ = nullptr;
+    int hhistCap_ = 0;
+
+    std::atomic<bool> stop_{false};
+    std::atomic<bool> running_{false};
+    int currentIteration_ = 0;
+    bool converged_ = false, stopped_ = false, ready_ = false;
+    std::vector<double> history_[3];
+    double setupMs_ = 0.0, solveMs_ = 0.0;
+    int contCells_ = -1, contFaces_ = -1, contRefs_ = -1;
+};
+
+}  // namespace axs

@@ -1,0 +1,253 @@
+// topology.cpp -- set-up builders: geometry, neighbour CSR, colouring and the
+// agglomeration hierarchy with its GPU gather tables.
+#include "topology.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace axs {
+
+namespace {
+constexpr double kPi = 3.14159265358979323846;
+}
+
+// preComputeVariables, memory_manger.cpp:512-617.  Same expressions in the same
+// order, so the device reads bit-identical geometry.
+void computeGeometry(const HostMesh& m, Geometry& g) {
+    const int N = m.nCells, F = m.nFaces;
+    g.invDPN.assign(F, 0.0);
+    g.wP.assign(F, 0.0);
+    g.dPB.assign(F, 0.0);
+    g.length2D.assign(F, 0.0);
+    g.invA2D.resize(N);
+    for (int c = 0; c < N; c++) g.invA2D[c] = 2.0 * kPi * m.cellCenterR[c] / m.cellVolume[c];
+    std::vector<double> offZ(F), offR(F);
+    g.lsqWZ.resize(F);
+    g.lsqWR.resize(F);
+    for (int f = 0; f < F; f++) {
+        g.length2D[f] = m.faceArea[f] / (2 * kPi * m.faceCenterR[f]);
+        const int P = m.faceOwner[f], Nb = m.faceNeighbor[f];
+        const double pfz = m.faceCenterZ[f] - m.cellCenterZ[P];
+        const double pfr = m.faceCenterR[f] - m.cellCenterR[P];
+        const double nz = m.faceNormalZ[f], nr = m.faceNormalR[f];
+        if (Nb < 0) {
+            g.dPB[f] = std::abs(pfz * nz + pfr * nr);
+        } else {
+            const double nfz = m.faceCenterZ[f] - m.cellCenterZ[Nb];
+            const double nfr = m.faceCenterR[f] - m.cellCenterR[Nb];
+            const double pnz = m.cellCenterZ[Nb] - m.cellCenterZ[P];
+            const double pnr = m.cellCenterR[Nb] - m.cellCenterR[P];
+            const double dP = std::abs(pfz * nz + pfr * nr);
+            const double dN = std::abs(nfz * nz + nfr * nr);
+            const double dPN = std::sqrt(pnz * pnz + pnr * pnr);
+            g.invDPN[f] = 1.0 / dPN;
+            g.wP[f] = dN / (dP + dN);
+        }
+        // least-squares offsets: neighbour centre, or face centre on a boundary
+        const double dz = (Nb >= 0 ? m.cellCenterZ[Nb] : m.faceCenterZ[f]) - m.cellCenterZ[P];
+        const double dr = (Nb >= 0 ? m.cellCenterR[Nb] : m.faceCenterR[f]) - m.cellCenterR[P];
+        const double w = 1.0 / (dz * dz + dr * dr);
+        offZ[f] = dz;
+        offR[f] = dr;
+        g.lsqWZ[f] = w * dz;
+        g.lsqWR[f] = w * dr;
+    }
+    g.lsqInvZZ.assign(N, 0.0);
+    g.lsqInvZR.assign(N, 0.0);
+    g.lsqInvRR.assign(N, 0.0);
+    for (int c = 0; c < N; c++) {
+        double zz = 0.0, zr = 0.0, rr = 0.0;
+        for (int k = m.cellFaceStart[c]; k < m.cellFaceStart[c + 1]; k++) {
+            const int f = m.cellFaceIDs[k];
+            zz += g.lsqWZ[f] * offZ[f];
+            zr += g.lsqWZ[f] * offR[f];
+            rr += g.lsqWR[f] * offR[f];
+        }
+        const double det = zz * rr - zr * zr;
+        if (std::abs(det) <= 1.0e-30) continue;
+        g.lsqInvZZ[c] = rr / det;
+        g.lsqInvZR[c] = -zr / det;
+        g.lsqInvRR[c] = zz / det;
+    }
+}
+
+void cellNeighbourCSR(const HostMesh& m, std::vector<int>& start, std::vector<int>& nb) {
+    start.assign(m.cellFaceStart.begin(), m.cellFaceStart.end());
+    start[0] = 0;
+    nb.resize(m.cellFaceIDs.size());
+    for (int c = 0; c < m.nCells; c++)
+        for (int k = m.cellFaceStart[c]; k < m.cellFaceStart[c + 1]; k++) {
+            const int f = m.cellFaceIDs[k];
+            int other = -1;
+            if (f >= 0 && f < m.nFaces) {
+                if (m.faceOwner[f] == c) other = m.faceNeighbor[f];
+                else if (m.faceNeighbor[f] == c) other = m.faceOwner[f];
+            }
+            nb[k] = other;
+        }
+}
+
+int greedyColoring(int n, const int* start, const int* nb, std::vector<int>& colorStart,
+                   std::vector<int>& cellOrder) {
+    colorStart.clear();
+    cellOrder.clear();
+    if (n <= 0) return 0;
+    std::vector<int> color(n, -1);
+    std::vector<int> usedBy;  // usedBy[c] == cell  <=>  colour c is taken by a neighbour of cell
+    int nColors = 0;
+    for (int c = 0; c < n; c++) {
+        for (int k = start[c]; k < start[c + 1]; k++) {
+            const int o = nb[k];
+            if (o < 0 || color[o] < 0) continue;
+            if (color[o] >= (int)usedBy.size()) usedBy.resize(color[o] + 1, -1);
+            usedBy[color[o]] = c;
+        }
+        int pick = 0;
+        while (pick < (int)usedBy.size() && usedBy[pick] == c) pick++;
+        color[c] = pick;
+        nColors = std::max(nColors, pick + 1);
+    }
+    colorStart.assign(nColors + 1, 0);
+    for (int c = 0; c < n; c++) colorStart[color[c] + 1]++;
+    for (int k = 0; k < nColors; k++) colorStart[k + 1] += colorStart[k];
+    cellOrder.resize(n);
+    std::vector<int> cursor(colorStart.begin(), colorStart.end() - 1);
+    for (int c = 0; c < n; c++) cellOrder[cursor[color[c]]++] = c;
+    return nColors;
+}
+
+namespace {
+
+// buildAgglomerationMap, multigrid.cu:79-194: breadth-first groups of up to 4
+// seeded in cell order, stranded singletons merged into their smallest
+// neighbouring group, then renumbered by first appearance.
+int aggregate(const Level& f, std::vector<int>& owner) {
+    const int kTarget = 4;
+    owner.assign(f.n, -1);
+    int groups = 0;
+    int grown[kTarget];
+    for (int seed = 0; seed < f.n; seed++) {
+        if (owner[seed] >= 0) continue;
+        const int g = groups++;
+        owner[seed] = g;
+        int count = 1;
+        grown[0] = seed;
+        for (int q = 0; q < count && count < kTarget; q++) {
+            const int cell = grown[q];
+            for (int k = f.start[cell]; k < f.start[cell + 1] && count < kTarget; k++) {
+                const int o = f.nb[k];
+                if (o < 0 || owner[o] >= 0) continue;
+                owner[o] = g;
+                grown[count++] = o;
+            }
+        }
+    }
+    std::vector<int> size(groups, 0);
+    for (int c = 0; c < f.n; c++) size[owner[c]]++;
+    for (int c = 0; c < f.n; c++) {
+        const int g = owner[c];
+        if (size[g] != 1) continue;
+        int best = -1;
+        for (int k = f.start[c]; k < f.start[c + 1]; k++) {
+            const int o = f.nb[k];
+            if (o < 0) continue;
+            const int go = owner[o];
+            if (go == g) continue;
+            if (best < 0 || size[go] < size[best]) best = go;
+        }
+        if (best < 0) continue;
+        owner[c] = best;
+        size[best]++;
+        size[g] = 0;
+    }
+    std::vector<int> renum(groups, -1);
+    int next = 0;
+    for (int c = 0; c < f.n; c++) {
+        int& r = renum[owner[c]];
+        if (r < 0) r = next++;
+        owner[c] = r;
+    }
+    return next;
+}
+
+// coarsenGrid, multigrid.cu:196-287: coarse CSR sorted by (coarse cell, coarse
+// neighbour) and the fine-slot -> coarse-slot map; plus the gather tables.
+Level coarsen(Level& f) {
+    Level c;
+    c.n = aggregate(f, f.toCoarse);
+    struct Ref { int row, col, slot; };
+    std::vector<Ref> refs;
+    refs.reserve(f.nb.size());
+    for (int i = 0; i < f.n; i++) {
+        const int ci = f.toCoarse[i];
+        for (int k = f.start[i]; k < f.start[i + 1]; k++) {
+            const int o = f.nb[k];
+            if (o < 0) continue;
+            const int co = f.toCoarse[o];
+            if (co != ci) refs.push_back(Ref{ci, co, k});
+        }
+    }
+    // slots are visited in ascending order, so a stable sort on (row, col)
+    // keeps each coarse entry's fine slots ascending
+    std::stable_sort(refs.begin(), refs.end(),
+                     [](const Ref& a, const Ref& b) { return a.row != b.row ? a.row < b.row : a.col < b.col; });
+    f.slotToCoarseSlot.assign(f.nb.size(), -1);
+    c.start.assign(c.n + 1, 0);
+    f.cslotStart.clear();
+    f.cslotFine.clear();
+    for (size_t i = 0; i < refs.size(); i++) {
+        const bool fresh = i == 0 || refs[i].row != refs[i - 1].row || refs[i].col != refs[i - 1].col;
+        if (fresh) {
+            c.nb.push_back(refs[i].col);
+            c.start[refs[i].row + 1]++;
+            f.cslotStart.push_back((int)f.cslotFine.size());
+        }
+        f.slotToCoarseSlot[refs[i].slot] = (int)c.nb.size() - 1;
+        f.cslotFine.push_back(refs[i].slot);
+    }
+    f.cslotStart.push_back((int)f.cslotFine.size());
+    for (int i = 0; i < c.n; i++) c.start[i + 1] += c.start[i];
+    // members of each coarse cell, ascending fine index
+    f.memberStart.assign(c.n + 1, 0);
+    for (int i = 0; i < f.n; i++) f.memberStart[f.toCoarse[i] + 1]++;
+    for (int i = 0; i < c.n; i++) f.memberStart[i + 1] += f.memberStart[i];
+    f.members.resize(f.n);
+    std::vector<int> cur(f.memberStart.begin(), f.memberStart.end() - 1);
+    for (int i = 0; i < f.n; i++) f.members[cur[f.toCoarse[i]]++] = i;
+    return c;
+}
+
+void clearCoarseMaps(Level& f) {
+    f.toCoarse.clear();
+    f.slotToCoarseSlot.clear();
+    f.memberStart.clear();
+    f.members.clear();
+    f.cslotStart.clear();
+    f.cslotFine.clear();
+}
+
+}  // namespace
+
+std::vector<Level> buildHierarchy(int n, const std::vector<int>& start, const std::vector<int>& nb,
+                                  int maxLevels) {
+    const int kMinCells = 64;          // multigrid.cu:55
+    const double kMaxShrink = 0.8;     // multigrid.cu:60
+    std::vector<Level> levels;
+    Level fine;
+    fine.n = n;
+    fine.start = start;
+    fine.nb = nb;
+    while (fine.n > kMinCells && !(maxLevels > 0 && (int)levels.size() + 1 >= maxLevels)) {
+        Level coarse = coarsen(fine);
+        if (coarse.n >= (int)(fine.n * kMaxShrink)) {
+            clearCoarseMaps(fine);
+            break;
+        }
+        levels.push_back(std::move(fine));
+        fine = std::move(coarse);
+    }
+    levels.push_back(std::move(fine));
+    return levels;
+}
+
+}  // namespace axs

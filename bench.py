@@ -1,0 +1,145 @@
+"""Benchmark: SIMPLE outer iterations/s on the ~1M-cell unstructured nozzle.
+
+One "step" is one SIMPLE outer iteration (assembly, 2 x 20 momentum Jacobi
+sweeps, Rhie-Chow, two non-orthogonal pressure-correction passes with 3
+agglomeration-multigrid V-cycles each, corrections, residual norms), replayed
+as one hipGraph.  Inputs are resident in HBM when the timed region starts.
+
+Prints one JSON line (see the repository's benchmark contract):
+  value       outer iterations/s on one MI355X (FP64 reference arithmetic)
+  roofline    the dominant kernel (u+v momentum Jacobi sweep) measured with HIP
+              events on the solver's stream, against the 8 TB/s HBM peak
+  mg_vcycle   the p' multigrid solve (3 V-cycles) against the same peak
+  cpu_baseline the FP64 oracle on this host, one thread, a bounded sample of the
+              same workload
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "axisim-software_amd"))
+sys.path.insert(0, str(ROOT / "tests"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cells", type=float, default=1.0e6)
+    ap.add_argument("--cpu-iters", type=int, default=10, help="oracle sample size (outer iterations)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the solver does not shard "
+              f"across GPUs)", file=sys.stderr)
+        return 2
+
+    import numpy as np
+
+    import axs
+
+    t0 = time.time()
+    case = axs.nozzle_case(cells=args.cells, max_iter=args.warmup)
+    m = case.mesh
+    log(f"mesh: {m.n_cells} cells, {m.n_faces} faces ({time.time() - t0:.1f} s)")
+
+    s = axs.make_solver(case)
+    t1 = time.time()
+    s.run()  # cold start + warm-up iterations (also builds hierarchy and graph)
+    st = s.stats()
+    log(f"warm-up: {st.iterations} iterations, setup {st.setup_ms:.0f} ms, mg levels {st.mg_levels} "
+        f"({time.time() - t1:.1f} s)")
+
+    ms = s.run_timed(args.steps)
+    it_s = args.steps / (ms / 1000.0)
+    log(f"timed: {args.steps} iterations in {ms:.1f} ms -> {it_s:.1f} it/s")
+
+    # dominant kernel: u+v Jacobi sweep (2 x 20 per iteration... one launch = one sweep of both)
+    k_ms, k_bytes = s.bench_kernel(axs.KERNEL_JACOBI_UV, 200)
+    achieved = k_bytes / (k_ms * 1e-3) / 1e9
+    log(f"jacobi_uv: {k_ms * 1000:.1f} us/launch, {achieved:.0f} GB/s")
+
+    # p' system of the last iteration -> stand-alone multigrid solve timing
+    fs = m.arrays["cell_face_start"]
+    mg = axs.Multigrid(fs, _slot_neighbours(m))
+    AC, AF, b = s.field("pp_ac"), s.field("pp_af"), s.field("pp_b")
+    mg_ms, mg_bytes, _ = mg.bench(AC, AF, b, cycles=case.config.mg_max_iter, repeats=20)
+    mg_gbs = mg_bytes / (mg_ms * 1e-3) / 1e9
+    log(f"mg solve: {mg_ms:.3f} ms, {mg_gbs:.0f} GB/s")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle_binding import Oracle
+
+        cfg = case.config
+        cfg.max_iter = args.cpu_iters
+        o = Oracle(m, case.bcs, cfg)
+        tc = time.time()
+        o.run()
+        dt = time.time() - tc
+        cpu = {"value": round(o.iterations / dt, 4), "unit": "outer_iter/s", "cores": 1, "kind": "port",
+               "sample": f"{o.iterations} outer iterations of the same {m.n_cells}-cell nozzle case from a cold "
+                         f"start, FP64 oracle (oracle/oracle.cpp), one thread, {dt:.1f} s"}
+        log(f"cpu baseline: {cpu['value']} it/s")
+
+    out = {
+        "metric": "SIMPLE outer iter/s at 1M cells (unstructured nozzle, non-orthogonal correction, agglomeration MG)",
+        "value": round(it_s, 3),
+        "unit": "outer_iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded Poisson-disk + 5% jitter + Delaunay nozzle mesh (seeds 3001/3002), cold start",
+        "config": {
+            "workload": "nozzle_1M_triangles_simple_fp64",
+            "cells": m.n_cells, "faces": m.n_faces, "face_refs": int(fs[-1]),
+            "mg_levels": st.mg_levels, "convection": "upwind", "gradient": "lsq", "non_orth": True,
+            "momentum_sweeps": case.config.linear_max_iter, "mg_cycles": case.config.mg_max_iter,
+            "cell_updates_per_s": round(it_s * m.n_cells, 1),
+            "bytes_per_iteration": st.bytes_per_iteration,
+            "iteration_hbm_frac": round(st.bytes_per_iteration * it_s / 1e9 / HBM_PEAK_GBS, 4),
+        },
+        "roofline": {"bound": "hbm", "kernel": "kJacobiUV", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": k_bytes, "us_per_launch": round(k_ms * 1000, 3)},
+        "mg_vcycle": {"bound": "hbm", "achieved": round(mg_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": round(mg_gbs / HBM_PEAK_GBS, 4), "ms_per_solve": round(mg_ms, 4),
+                      "bytes_per_solve": mg_bytes, "cycles": case.config.mg_max_iter},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def _slot_neighbours(m) -> "np.ndarray":
+    import numpy as np
+
+    a = m.arrays
+    ids = a["cell_face_ids"]
+    cells = np.repeat(np.arange(m.n_cells, dtype=np.int32), np.diff(a["cell_face_start"]))
+    own = a["face_owner"][ids]
+    nb = a["face_neighbor"][ids]
+    return np.where(own == cells, nb, own).astype(np.int32)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

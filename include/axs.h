@@ -1,0 +1,311 @@
+/*
+ * axs.h -- C ABI of the MI355X SIMPLE / multigrid solver (libaxs.so).
+ *
+ * Plain C, POD views, one opaque handle per solver.  Every pointer handed in
+ * is host memory and is copied before the call returns unless the entry point
+ * says "device".  All entry points return an axs_status; axs_last_error()
+ * gives the message of the last failure on that handle (thread-local text for
+ * entry points without a handle, read with axs_last_error(NULL)).
+ *
+ * What each entry point replaces in the reference (Luippe/AxiSim-Software):
+ *   axs_create / axs_destroy      Solver::Solver(Config&), ~Solver           solver/header/solver.h:21-24, solver/source/solver.cpp:104-114
+ *   axs_set_mesh                  Mesh::getFVMesh() copy + createFVMeshDevice solver/source/solver.cpp:774-785, manager/source/memory_manger.cpp:621-681
+ *   axs_set_bcs                   createBoundarySolverDevice(boundaryGroups)  manager/source/memory_manger.cpp:51-237
+ *   axs_configure                 Solver::configSolver/configSimple/configMultigrid/cfg/simple.*Relaxation/f
+ *                                 (written directly by the GUI)               solver/header/solver.h:92-139, structs/solver_struct.h:158-263
+ *   axs_run                       Solver::run(Mesh&) -> runSimple (blocking form; the C++ shim owns the thread)
+ *                                                                             solver/source/solver.cpp:146-173, 753-1396
+ *   axs_request_stop              Solver::requestStop()                       solver/source/solver.cpp:175-184
+ *   axs_can_continue              Solver::canContinue(mesh, &reason)          solver/source/solver.cpp:469-528
+ *   axs_get_field                 Solver::solutions[name].field               solver/source/solver.cpp:186-263
+ *   axs_get_residual_history      ResidualPlot::add(iteration, cfg) feed      graphics/gui/source/residual_plot.cpp:113-130
+ *   axs_get_stats                 console lines: hierarchy, colours, timing   solver/source/multigrid.cu:727-764, solver/source/solver.cpp:836-841, 1318-1323
+ *   axs_mg_*                      MultigridSolver (stand-alone p' solve)      solver/source/multigrid.cu:23-31, 298-325, 766-811
+ *   axs_build_coloring            buildMeshColoring                           manager/source/memory_manger.cpp:334-409
+ *   axs_mesh_*                    FVMesh producers: toPackedMesh / createUnstructuredMesh
+ *                                                                             structs/multiblock.h:59-444, solver/source/mesh.cpp:1835-2002
+ */
+#ifndef AXS_H
+#define AXS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int axs_status;
+enum {
+    AXS_OK = 0,
+    AXS_ERR_INVALID = 1,      /* bad argument or inconsistent input            */
+    AXS_ERR_DEVICE = 2,       /* HIP runtime error (the reference aborts here) */
+    AXS_ERR_STATE = 3,        /* call out of order (no mesh, still running ...)*/
+    AXS_ERR_UNSUPPORTED = 4,  /* option the GPU path does not implement        */
+    AXS_ERR_RANGE = 5         /* size or index out of range                    */
+};
+
+/* Enum values equal the reference's enum order (structs/boundary_struct.h:88-97,
+ * 406-412; structs/solver_struct.h:19-63, 193-196). */
+enum { AXS_BC_DIRICHLET = 0, AXS_BC_NEUMANN = 1, AXS_BC_FULLY_DEVELOPED = 2,
+       AXS_BC_MICHAELIS_MENTEN = 3, AXS_BC_HILL = 4, AXS_BC_NONE = 5, AXS_BC_PULSATILE = 6 };
+enum { AXS_BOUNDARY_WALL = 0, AXS_BOUNDARY_VELOCITY_INLET = 1, AXS_BOUNDARY_PRESSURE_OUTLET = 2,
+       AXS_BOUNDARY_SYMMETRY = 3, AXS_BOUNDARY_FAR_FIELD = 4 };
+enum { AXS_CONV_UPWIND = 0, AXS_CONV_CENTRAL = 1, AXS_CONV_SOU = 2, AXS_CONV_QUICK = 3 };
+enum { AXS_GRAD_GREEN_GAUSS = 0, AXS_GRAD_LSQ = 1 };
+enum { AXS_LINEAR_JACOBI = 0, AXS_LINEAR_GS_RB = 1 };
+enum { AXS_TIME_BDF1 = 0, AXS_TIME_BDF2 = 1 };
+enum { AXS_NORM_L1 = 0, AXS_NORM_L2 = 1, AXS_NORM_LINF = 2 };
+enum { AXS_SCALE_NONE = 0, AXS_SCALE_N = 1, AXS_SCALE_SQRT_N = 2, AXS_SCALE_DIAGONAL = 3,
+       AXS_SCALE_CONTINUITY = 4 };
+/* Build extensions with no reference counterpart. */
+enum { AXS_PREC_FP64 = 0,      /* reference arithmetic and schedule                  */
+       AXS_PREC_MIXED = 1 };   /* FP32 smoothers + V-cycles, FP64 residual/flux/coarse */
+enum { AXS_MG_SMOOTH_JACOBI = 0, AXS_MG_SMOOTH_RBGS = 1 };
+enum { AXS_VAR_U = 0, AXS_VAR_V = 1, AXS_VAR_P = 2 };
+enum { AXS_RES_U = 0, AXS_RES_V = 1, AXS_RES_CONTINUITY = 2, AXS_RES_COUNT = 3 };
+
+/* Face-based mesh, structure of arrays (FVMeshHostPacked, structs/solver_struct.h:400-429).
+ * Slot k of cell c is cell_face_ids[cell_face_start[c] + k]. */
+typedef struct axs_mesh_view {
+    int n_cells;
+    int n_faces;
+    const double* cell_center_z;
+    const double* cell_center_r;
+    const double* cell_volume;     /* revolved volume 2*pi*r_c*A2D           */
+    const double* cell_area2d;     /* planar cross-section                   */
+    const int* cell_face_start;    /* n_cells + 1                            */
+    const int* cell_face_ids;      /* cell_face_start[n_cells]               */
+    const int* face_owner;
+    const int* face_neighbor;      /* -1 on a boundary face                  */
+    const double* face_normal_z;   /* unit, owner -> neighbour / outward     */
+    const double* face_normal_r;
+    const double* face_center_z;
+    const double* face_center_r;
+    const double* face_area;       /* revolved area 2*pi*r_f*L               */
+    const int* face_group;         /* boundary group id, -1 if none          */
+} axs_mesh_view;
+
+/* Boundary conditions of one variable, indexed by boundary group id
+ * (BoundaryFieldHost, structs/solver_struct.h:464-476).  PULSATILE groups
+ * are Dirichlet on the device with value(t) = value*(1 + amplitude*sin(2 pi f t))
+ * (structs/boundary_struct.h:184-192). */
+typedef struct axs_bc_view {
+    int n_groups;
+    const uint8_t* type;           /* AXS_BC_*                                     */
+    const uint8_t* boundary_type;  /* AXS_BOUNDARY_*, may be NULL (WALL)          */
+    const double* value;           /* mean / prescribed value                     */
+    const double* length;          /* group length, fully-developed profile       */
+    const double* amplitude;       /* pulsatile only, may be NULL                 */
+    const double* frequency;       /* pulsatile only (Hz), may be NULL            */
+} axs_bc_view;
+
+typedef struct axs_residual_config {
+    int enabled;
+    int norm;       /* AXS_NORM_*    */
+    int scaling;    /* AXS_SCALE_*   */
+    double tol;
+} axs_residual_config;
+
+typedef struct axs_config {
+    /* ConfigSolver (structs/solver_struct.h:203-231) */
+    int convection_scheme;
+    int gradient_scheme;
+    int linear_solver;
+    int time_scheme;
+    int linear_max_iter;
+    int use_multigrid;
+    int add_convection;
+    int transient;
+    int use_non_orth;
+    double dt;
+    double t_end;
+    /* ConfigSimple (solver_struct.h:158-163) */
+    int max_iter;
+    int check_conv;
+    /* ConfigMultigrid (solver_struct.h:167-190) */
+    int mg_max_iter;
+    double mg_weight;
+    int mg_coarse_sweeps;
+    int mg_pre_sweeps;
+    int mg_post_sweeps;
+    /* VariablesSimple relaxation (solver_struct.h:312-313) and FluidPropertyConfig */
+    double alpha_u;
+    double alpha_p;
+    double rho;
+    double mu;
+    /* ConfigResidual for U, V, Continuity (solver_struct.h:240-263) */
+    axs_residual_config residual[AXS_RES_COUNT];
+    /* mesh type: a structured mesh forces the non-orthogonal corrector off
+     * (Solver::runCheck, solver/source/solver.cpp:553-555) */
+    int structured_mesh;
+    /* ---- build extensions (no reference counterpart) ---- */
+    int mg_max_levels;    /* 0 = reference rule (coarsen to <= 64 cells)        */
+    int mg_smoother;      /* AXS_MG_SMOOTH_*                                    */
+    int precision;        /* AXS_PREC_*                                         */
+    int coarse_direct;    /* 1 = FP64 LU solve on the coarsest level            */
+} axs_config;
+
+/* Reference defaults (structs/solver_struct.h, src/header/setting.cuh:61-71,
+ * solver/source/solver.cpp:87-101). */
+void axs_config_defaults(axs_config* cfg);
+
+typedef struct axs_solver axs_solver;
+
+/* Progress callback, called after every residual check on the solving thread
+ * (the feed of ResidualPlot::add and the console line, solver.cpp:1245-1266).
+ * values[i] is the normalised residual of names[i]. */
+typedef void (*axs_progress_cb)(void* user, int iteration, int n, const char* const* names,
+                                const double* values);
+
+axs_status axs_create(axs_solver** out, int device);
+void axs_destroy(axs_solver* s);
+const char* axs_last_error(const axs_solver* s);
+
+axs_status axs_set_mesh(axs_solver* s, const axs_mesh_view* mesh);
+axs_status axs_set_bcs(axs_solver* s, int var, const axs_bc_view* bc);
+axs_status axs_configure(axs_solver* s, const axs_config* cfg);
+
+/* Blocking solve.  continue_run != 0 keeps fields and the iteration counter
+ * when the mesh counts match (Continue Solver). */
+axs_status axs_run(axs_solver* s, axs_progress_cb cb, void* user, int continue_run);
+void axs_request_stop(axs_solver* s);
+/* 1 if a run may continue on the current mesh, else 0 with a reason. */
+int axs_can_continue(const axs_solver* s, char* reason, size_t reason_len);
+
+/* Output fields (host copy).  n must equal the field's length. */
+enum {
+    AXS_FIELD_U = 0, AXS_FIELD_V = 1, AXS_FIELD_P = 2, AXS_FIELD_PP = 3,
+    AXS_FIELD_GRAD_UZ = 4, AXS_FIELD_GRAD_UR = 5, AXS_FIELD_GRAD_VZ = 6, AXS_FIELD_GRAD_VR = 7,
+    AXS_FIELD_GRAD_PZ = 8, AXS_FIELD_GRAD_PR = 9,
+    AXS_FIELD_CONTINUITY = 10, AXS_FIELD_VMAG = 11, AXS_FIELD_CELL_RE = 12,
+    AXS_FIELD_MDOT = 13,            /* length n_faces */
+    AXS_FIELD_DU = 14, AXS_FIELD_DV = 15,
+    /* internal state after the last iteration, for parity checks */
+    AXS_FIELD_U_AC = 16, AXS_FIELD_U_B = 17, AXS_FIELD_U_INVAC = 18, AXS_FIELD_U_AF = 19,
+    AXS_FIELD_V_AC = 20, AXS_FIELD_V_B = 21, AXS_FIELD_V_INVAC = 22, AXS_FIELD_V_AF = 23,
+    AXS_FIELD_PP_AC = 24, AXS_FIELD_PP_B = 25, AXS_FIELD_PP_INVAC = 26, AXS_FIELD_PP_AF = 27,
+    /* precomputed geometry (preComputeVariables, memory_manger.cpp:512-617) */
+    AXS_GEOM_INV_DPN = 40, AXS_GEOM_WP = 41, AXS_GEOM_DPB = 42, AXS_GEOM_INV_A2D = 43,
+    AXS_GEOM_LSQ_WZ = 44, AXS_GEOM_LSQ_WR = 45, AXS_GEOM_LSQ_INV_ZZ = 46,
+    AXS_GEOM_LSQ_INV_ZR = 47, AXS_GEOM_LSQ_INV_RR = 48, AXS_GEOM_LENGTH2D = 49
+};
+axs_status axs_get_field(axs_solver* s, int field, double* out, size_t n);
+/* Number of elements field `field` holds (0 if unknown). */
+size_t axs_field_length(const axs_solver* s, int field);
+
+/* Normalised residual history of one residual (AXS_RES_*): one entry per
+ * residual check.  Returns the number of entries written through *count. */
+axs_status axs_get_residual_history(axs_solver* s, int res, double* out, int max_n, int* count);
+
+typedef struct axs_stats {
+    int iterations;        /* outer iterations done (currentIteration)           */
+    int converged;
+    int stopped;
+    int n_cells, n_faces, n_face_refs;
+    int n_colors;          /* greedy colours, 0 if not built                     */
+    int mg_levels;
+    int mg_level_cells[32];
+    double setup_ms;       /* upload + hierarchy + graph capture                 */
+    double solve_ms;       /* outer loop wall time                               */
+    double last_residual[AXS_RES_COUNT];
+    double bytes_per_iteration;  /* algorithmic bytes of one outer iteration       */
+    double vcycle_bytes;         /* algorithmic bytes of one p' multigrid solve    */
+} axs_stats;
+axs_status axs_get_stats(axs_solver* s, axs_stats* out);
+
+/* Run exactly `n` outer iterations from the current state without touching the
+ * convergence bookkeeping flags, timing them on the GPU with HIP events (one
+ * graph launch per iteration).  For benchmarks; *ms receives the elapsed time. */
+axs_status axs_run_timed(axs_solver* s, int n, double* ms);
+
+/* Time one kernel of the outer iteration: `repeats` eager launches on the
+ * solver's stream between two HIP events, from the current state (run the
+ * solver first).  *ms = average launch duration, *bytes = algorithmic bytes of
+ * one launch.  No reference counterpart: a benchmark hook. */
+enum {
+    AXS_KERNEL_JACOBI_UV = 0,   /* one u+v momentum Jacobi sweep             */
+    AXS_KERNEL_MOMENTUM = 1,    /* fused momentum assembly                   */
+    AXS_KERNEL_MG_SMOOTH = 2,   /* one weighted-Jacobi sweep on the p' level 0 */
+    AXS_KERNEL_FACE_FLUX = 3,   /* Rhie-Chow face mass flux                  */
+    AXS_KERNEL_PP_COEFF = 4     /* pressure-correction matrix                */
+};
+axs_status axs_bench_kernel(axs_solver* s, int kernel, int repeats, double* ms, double* bytes);
+
+/* ---------------- Agglomeration multigrid, stand-alone ---------------- */
+typedef struct axs_mg axs_mg;
+/* Build the hierarchy of the cell graph (buildLevels, multigrid.cu:298-325).
+ * max_levels 0 = reference rule. */
+axs_status axs_mg_create(axs_mg** out, int n_cells, const int* face_start,
+                         const int* face_neighbor, int max_levels);
+void axs_mg_destroy(axs_mg* mg);
+int axs_mg_levels(const axs_mg* mg);
+axs_status axs_mg_level_info(const axs_mg* mg, int level, int* n_cells, int* n_refs);
+/* Host copies of one level's maps; any pointer may be NULL.  cell_to_coarse and
+ * fine_slot_to_coarse_slot are empty (untouched) on the coarsest level. */
+axs_status axs_mg_get_maps(const axs_mg* mg, int level, int* face_start, int* face_neighbor,
+                           int* cell_to_coarse, int* fine_slot_to_coarse_slot);
+/* Solve A x = b with `cycles` V-cycles from the given x (host arrays, level-0
+ * CSR order).  AC, AF, b, x as in Coefficients (solver_struct.h:103-120). */
+axs_status axs_mg_solve(axs_mg* mg, const double* AC, const double* AF, const double* b,
+                        double* x, int cycles, double weight, int pre_sweeps, int post_sweeps,
+                        int coarse_sweeps, int precision, int coarse_direct);
+/* Benchmark form: the system is uploaded once, then `repeats` solves of
+ * `cycles` V-cycles are timed with HIP events (x restarts from zero each solve).
+ * *ms = average time of one solve; *bytes = algorithmic bytes of one solve. */
+axs_status axs_mg_bench(axs_mg* mg, const double* AC, const double* AF, const double* b,
+                        int cycles, double weight, int pre_sweeps, int post_sweeps,
+                        int coarse_sweeps, int precision, int coarse_direct, int repeats,
+                        double* ms, double* bytes, double* x_out);
+
+/* Greedy colouring in cell order + colour-major cell order
+ * (buildMeshColoring, memory_manger.cpp:334-409).  color_start needs
+ * n_cells + 1 entries; *n_colors receives the colour count. */
+axs_status axs_build_coloring(int n_cells, const int* face_start, const int* face_neighbor,
+                              int* color_start, int* cell_order, int* n_colors);
+
+/* ---------------- Case generator (input producer) ---------------- */
+typedef struct axs_mesh axs_mesh;
+
+/* One rectangular block of a conformal multiblock grid (makeRectBlock,
+ * structs/multiblock.h:118-136).  grading 0 = uniform, 1 = geometric
+ * progression with ratio coef.  edge_group: West, East, South, North. */
+typedef struct axs_block_spec {
+    double z0, r0;
+    double axial_length;  int axial_cells;  int axial_grading;  double axial_coef;
+    double radial_length; int radial_cells; int radial_grading; double radial_coef;
+    int edge_group[4];
+} axs_block_spec;
+axs_status axs_mesh_multiblock(axs_mesh** out, int n_blocks, const axs_block_spec* blocks);
+
+/* Triangles -> FVMesh (createUnstructuredMesh, mesh.cpp:1848-2002); boundary
+ * faces take the group of the nearest tagged boundary segment within tol
+ * (groupAtPoint, manager/source/math_func.cpp:175-195). */
+axs_status axs_mesh_triangles(axs_mesh** out, int n_points, const double* point_z,
+                              const double* point_r, int n_tris, const int* tris,
+                              int n_segments, const double* seg_z0, const double* seg_r0,
+                              const double* seg_z1, const double* seg_r1, const int* seg_group,
+                              double tol);
+
+/* Seeded Poisson-disk + jitter + Delaunay mesh of an axisymmetric nozzle:
+ * inlet pipe (radius r_in, length l_in), cone of half-angle cone_deg down to
+ * r_throat, throat of length l_throat, diffuser of half-angle diff_deg back to
+ * r_out, outlet pipe of length l_out.  spacing = Poisson-disk radius,
+ * jitter = fraction of spacing.  Groups: 0 inlet, 1 outlet, 2 wall, 3 axis. */
+typedef struct axs_nozzle_spec {
+    double r_in, l_in, cone_deg, r_throat, l_throat, diff_deg, r_out, l_out;
+    double spacing, jitter;
+    uint64_t seed_points, seed_jitter;
+} axs_nozzle_spec;
+axs_status axs_mesh_nozzle(axs_mesh** out, const axs_nozzle_spec* spec);
+
+axs_status axs_mesh_get_view(const axs_mesh* m, axs_mesh_view* view);
+/* Triangle mesh only: point and triangle arrays (NULL to query counts). */
+axs_status axs_mesh_get_triangles(const axs_mesh* m, int* n_points, double* point_z,
+                                  double* point_r, int* n_tris, int* tris);
+void axs_mesh_destroy(axs_mesh* m);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AXS_H */
