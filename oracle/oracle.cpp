@@ -1259,6 +1259,13 @@ int orc_set_bcs(orc_solver* h, int var, const axs_bc_view* v) {
 }
 
 int orc_run(orc_solver* h, int continue_run) { return h->o.run(continue_run); }
+
+// Replace the configuration between runs (the GUI writes Solver's config
+// members directly between runs; a continued run picks them up).
+void orc_configure(orc_solver* h, const axs_config* cfg) {
+    h->o.cfg = *cfg;
+    for (int i = 0; i < AXS_RES_COUNT; i++) h->o.rc[i].cfg = cfg->residual[i];
+}
 void orc_set_stop_at(orc_solver* h, int iteration) {
     h->o.stopAtIteration = iteration;
     h->o.stopRequested = false;

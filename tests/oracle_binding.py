@@ -29,6 +29,7 @@ def olib() -> C.CDLL:
             "orc_destroy": (None, [V]),
             "orc_set_bcs": (C.c_int, [V, C.c_int, C.POINTER(axs.BCView)]),
             "orc_run": (C.c_int, [V, C.c_int]),
+            "orc_configure": (None, [V, C.POINTER(axs.Config)]),
             "orc_set_stop_at": (None, [V, C.c_int]),
             "orc_iterations": (C.c_int, [V]),
             "orc_converged": (C.c_int, [V]),
@@ -83,6 +84,9 @@ class Oracle:
 
     def run(self, continue_run: bool = False) -> int:
         return olib().orc_run(self._h, int(continue_run))
+
+    def configure(self, cfg: axs.Config) -> None:
+        olib().orc_configure(self._h, C.byref(cfg))
 
     def stop_at(self, iteration: int) -> None:
         olib().orc_set_stop_at(self._h, iteration)
