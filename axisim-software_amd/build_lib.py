@@ -76,6 +76,24 @@ def build_product(jobs: int = 8) -> Path:
     return LIB
 
 
+def build_compat() -> Path:
+    """The C++ Solver drop-in (compat/) as a static archive plus the demo that
+    drives it like the GUI does; plain g++ against include/axs.h + libaxs.so."""
+    obj_dir = PKG / "_build"
+    obj_dir.mkdir(exist_ok=True)
+    inc = [f"-I{ROOT / 'include'}", f"-I{PKG / 'compat'}"]
+    obj = obj_dir / "compat_solver.o"
+    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-Wall", *inc, "-c", str(PKG / "compat" / "solver.cpp"), "-o", str(obj)])
+    archive = LIB_DIR / "libaxs_compat.a"
+    if archive.exists():
+        archive.unlink()
+    _run(["ar", "rcs", str(archive), str(obj)])
+    demo = LIB_DIR / "compat_demo"
+    _run(["g++", "-std=c++17", "-O2", "-Wall", *inc, str(PKG / "compat" / "compat_demo.cpp"), str(archive),
+          f"-L{LIB_DIR}", "-laxs", f"-Wl,-rpath,$ORIGIN", "-pthread", "-o", str(demo)])
+    return demo
+
+
 def build_oracle() -> Path:
     """g++ build of the CPU oracle (FP64, no contraction)."""
     tmp = ORACLE_LIB.with_suffix(".so.tmp")
@@ -100,6 +118,7 @@ def build_reference_checkers() -> bool:
 
 def build_all(jobs: int = 8) -> None:
     build_product(jobs)
+    build_compat()
     build_oracle()
     build_reference_checkers()
 

@@ -1,0 +1,222 @@
+// compat/solver.hpp -- C++ drop-in for AxiSim's `class Solver`
+// (solver/header/solver.h:18-185) on top of the libaxs.so C ABI.
+//
+// The GUI, mesher and Results talk to the reference solver through this class:
+// they write the config members directly, call run(mesh) / requestStop() /
+// canContinue(), and read `solutions` once `solverRunning` drops.  This shim
+// keeps those names, defaults and threading (one worker std::thread per run,
+// atomics for the running / stop flags), and forwards the solve to axs_run().
+//
+// The types mirror the reference's (structs/solver_struct.h:9-263,
+// structs/boundary_struct.h:88-193, src/header/setting.cuh:61-71) field by
+// field so existing call sites compile unchanged; only what the SIMPLE path
+// reads is carried.
+#pragma once
+
+#include <atomic>
+#include <cstdint>
+#include <functional>
+#include <map>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "axs.h"
+
+namespace axs::compat {
+
+enum class ResidualType : uint8_t { RESIDUAL_SCALED = 0, RESIDUAL_RAW = 1, RESIDUAL_RMS = 2 };
+enum class ResidualNormType : uint8_t { RESIDUAL_L1 = 0, RESIDUAL_L2 = 1, RESIDUAL_LINF = 2 };
+enum class ResidualScalingType : uint8_t {
+    RESIDUAL_SCALING_NONE = 0, RESIDUAL_SCALING_N = 1, RESIDUAL_SCALING_SQRT_N = 2,
+    RESIDUAL_SCALING_DIAGONAL = 3, RESIDUAL_SCALING_CONTINUITY = 4
+};
+enum class LinearSolverType : uint8_t { LINEAR_JACOBI = 0, LINEAR_GS_RB = 1 };
+enum class VelocitySolverType : uint8_t { SOLVER_SIMPLE = 0 };
+enum class ConvectionScheme : uint8_t { CONV_UPWIND, CONV_CENTRAL, CONV_SECOND_ORDER_UPWIND, CONV_QUICK };
+enum class GradientScheme : uint8_t { GRAD_GREEN_GAUSS = 0, GRAD_LSQ = 1 };
+enum class TimeScheme : uint8_t { TIME_FIRST_ORDER = 0, TIME_SECOND_ORDER = 1 };
+enum class BCType : uint8_t { DIRICHLET, NEUMANN, FULLY_DEVELOPED, MICHAELIS_MENTEN, HILL, NONE, PULSATILE };
+enum class BoundaryType { WALL, VELOCITY_INLET, PRESSURE_OUTLET, SYMMETRY, FAR_FIELD };
+enum class BoundaryVariable : uint8_t { UVelocity, VVelocity, Pressure, StaticTemperature, Concentration, None };
+enum class MeshType { Structured, Unstructured };
+
+struct ConfigSimple {
+    int maxIter = 50;
+    int checkConv = 1;
+    double momTol = 1e-8;
+    double ppTol = 1e-5;
+};
+
+struct ConfigMultigrid {
+    int maxIter = 3;
+    double weight = 0.6;
+    int linearSweep = 30;
+    int linearPreSweep = 1;
+    int linearPostSweep = 1;
+    // build extensions (no reference counterpart)
+    int maxLevels = 0;
+    int precision = AXS_PREC_FP64;
+    int coarseDirect = 0;
+};
+
+struct ConfigSolver {
+    VelocitySolverType velocitySolver = VelocitySolverType::SOLVER_SIMPLE;
+    ConvectionScheme convectionScheme = ConvectionScheme::CONV_UPWIND;
+    GradientScheme gradientScheme = GradientScheme::GRAD_LSQ;
+    LinearSolverType type = LinearSolverType::LINEAR_JACOBI;
+    TimeScheme timeScheme = TimeScheme::TIME_FIRST_ORDER;
+    int linearMaxIter = 20;
+    bool useMultigrid = true;
+    bool addConvectionTerm = true;
+    bool transient = false;
+    bool useNonOrthCorrector = false;
+    double dt = 0.1;
+    double tEnd = 2.0;
+    int saveKeyFrameIter = 2;
+};
+
+struct ConfigResidual {
+    ResidualType type = ResidualType::RESIDUAL_SCALED;
+    ResidualNormType normType = ResidualNormType::RESIDUAL_L1;
+    ResidualScalingType scaleType = ResidualScalingType::RESIDUAL_SCALING_DIAGONAL;
+    bool enabled = false;
+    double resValue = 0.0;  // last normalised residual (the reference's *resVal)
+    double tol = 0.001;
+};
+
+struct FluidPropertyConfig {
+    double rho = 998.0;
+    double mu = 0.0010518;
+    double cp = 4180.0;
+    double k = 0.6;
+    double D = 3.0277e-9;
+};
+
+struct VariablesSimple {  // only the relaxation factors are configuration
+    double momentumRelaxation = 0.7;
+    double pressureRelaxation = 0.3;
+};
+
+struct SolverFieldOption {
+    bool solveEnergy = false;
+    bool solveConcentration = false;
+};
+
+struct Vec2 {
+    double z = 0.0;
+    double r = 0.0;
+};
+
+struct FVCell {
+    Vec2 center;
+    double area2D = 0.0;
+    double volume = 0.0;
+    std::vector<int> faceIDs;
+};
+
+struct FVFace {
+    int owner = -1;
+    int neighbor = -1;
+    Vec2 normal;
+    Vec2 center;
+    double length2D = 0.0;
+    double area = 0.0;
+    int boundaryGroupID = -1;
+};
+
+struct FVMesh {
+    std::vector<FVCell> cells;
+    std::vector<FVFace> faces;
+    int numCells() const { return (int)cells.size(); }
+    int numFaces() const { return (int)faces.size(); }
+};
+
+// One boundary condition (BoundaryCondition, boundary_struct.h:134-193):
+// the type plus the scalars the SIMPLE path reads.
+struct BoundaryCondition {
+    BCType type = BCType::NONE;
+    double value = 0.0;
+    double amplitude = 0.1;  // pulsatile only
+    double frequency = 1.0;  // pulsatile only (Hz)
+};
+
+struct BoundarySegmentGroup {
+    int id = -1;
+    std::string name;
+    BoundaryType type = BoundaryType::WALL;
+    float totalLength = 0.0f;
+    std::unordered_map<int, BoundaryCondition> bcs;  // keyed by (int)BoundaryVariable
+};
+
+// What Solver::run reads from Mesh (mesh.h:226-245).
+struct Mesh {
+    bool isReady = false;
+    MeshType currentMeshType = MeshType::Structured;
+    FVMesh fvMesh;
+    std::vector<BoundarySegmentGroup> boundaryGroups;
+    const FVMesh& getFVMesh() const { return fvMesh; }
+};
+
+struct SolutionField {
+    std::vector<double> field;
+};
+
+// getDefaultBCType + getEffectiveBC (manager/source/boundary_func.cpp:104-197)
+BCType defaultBCType(BoundaryType t, BoundaryVariable v);
+BoundaryCondition effectiveBC(const BoundarySegmentGroup& g, BoundaryVariable v);
+
+class Solver {
+public:
+    Solver();
+    ~Solver();
+    Solver(const Solver&) = delete;
+    Solver& operator=(const Solver&) = delete;
+
+    bool continueSolver = false;
+    bool isReady = false;
+    std::atomic<bool> solverRunning{false};
+    std::atomic<bool> stopRequested{false};
+
+    // Starts the solve on a worker thread and returns (solver.cpp:146-173).
+    void run(Mesh& mesh);
+    void requestStop();
+    bool canContinue(const Mesh& mesh, std::string* reason = nullptr) const;
+    void shutdown();  // join the worker
+    void reset();
+
+    // configuration, written directly by the settings panels
+    FluidPropertyConfig f;
+    SolverFieldOption fieldOption;
+    ConfigSimple configSimple;
+    ConfigSolver configSolver;
+    ConfigMultigrid configMultigrid;
+    VariablesSimple simple;
+    std::unordered_map<std::string, ConfigResidual> cfg;
+
+    // results, read once solverRunning drops
+    std::unordered_map<std::string, SolutionField> solutions;
+    FVMesh fvMesh;
+    int currentIteration() const { return currentIteration_; }
+    std::string lastMessage() const;
+
+    // ResidualPlot::add / console feed: called on the worker thread after
+    // every residual check with (iteration, {"U", "V", "Continuity"} values).
+    std::function<void(int, const std::map<std::string, double>&)> onResidual;
+
+private:
+    void runSimple(Mesh mesh);
+    bool runCheck(const Mesh& mesh);
+    static void progressThunk(void* user, int iteration, int n, const char* const* names, const double* values);
+
+    axs_solver* handle_ = nullptr;
+    std::thread worker_;
+    mutable std::mutex msgMutex_;
+    std::string message_;
+    int currentIteration_ = 0;
+    bool structured_ = true;
+};
+
+}  // namespace axs::compat

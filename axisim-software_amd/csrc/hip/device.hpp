@@ -52,6 +52,8 @@ struct DState {
     double *uOld, *vOld, *uOld2, *vOld2;
     double *uAC, *ub, *uInv, *vAC, *vb, *vInv, *AFuv;
     double *ppAC, *ppb, *ppInv, *ppAF;
+    // FP32 copies written by kMomentum in mixed precision (null otherwise)
+    float *AF32, *uInv32, *vInv32;
     double *partial;   // per-block residual partial sums
     double *hist;      // residual history: U, V, Continuity, iteration per check
     Ctl* ctl;
@@ -69,11 +71,13 @@ struct DLevel {
     int *start = nullptr, *nb = nullptr;
     double *AC = nullptr, *b = nullptr, *AF = nullptr;
     double *x = nullptr, *xNew = nullptr;
+    double* res = nullptr;  // per-cell residual / diagonal scratch (levels with a coarser one)
     int *toCoarse = nullptr, *slotToCoarseSlot = nullptr;
     int *memberStart = nullptr, *members = nullptr;
     int *cslotStart = nullptr, *cslotFine = nullptr;
     // FP32 copies for the mixed-precision V-cycle
     float *AC32 = nullptr, *b32 = nullptr, *AF32 = nullptr, *x32 = nullptr, *xNew32 = nullptr;
+    float* res32 = nullptr;
 };
 
 inline int blocksFor(long n, int t = kBlock) { return (int)((n + t - 1) / t); }
@@ -108,5 +112,15 @@ void launchMGSmoothBlock(hipStream_t s, const DLevel& L, double w, int sweeps, c
 void launchMGRestrict(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl);
 void launchMGProlongSmooth(hipStream_t s, const DLevel& f, const DLevel& c, double w, const Ctl* ctl);
 void launchMGProlong(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl);
+// V-cycle of levels lt..L-1 in one workgroup (levels: device copy of the DLevel
+// array); stageBytes > 0 stages the coarsest level in that much LDS
+void launchMGTail(hipStream_t s, const DLevel* levels, int lt, int L, double w, int pre, int post, int coarseSweeps,
+                  bool fp32, const double* inv, size_t stageBytes, const Ctl* ctl);
+// Galerkin product in two coalesced passes (fine diagonals, then gathers)
+void launchMGCoarseDiag(hipStream_t s, const DLevel& f, const Ctl* ctl);
+void launchMGCoarseGather(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl);
+// residual per fine cell + ordered gather per coarse cell
+void launchMGResidual(hipStream_t s, const DLevel& f, const Ctl* ctl);
+void launchMGRestrictSum(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl);
 
 }  // namespace axs

@@ -334,10 +334,12 @@ __global__ void __launch_bounds__(kBlock) kMomentum(DMesh m, DBC bu, DBC bv, DBC
                 }
             }
             s.AFuv[k] = af;
+            if (s.AF32) s.AF32[k] = (float)af;
         } else {
             boundaryTransport(m, bu, n, f, area, mu, F, acU, bU);
             boundaryTransport(m, bv, n, f, area, mu, F, acV, bV);
             s.AFuv[k] = 0.0;
+            if (s.AF32) s.AF32[k] = 0.0f;
         }
     }
     if (ph.conv) {
@@ -390,6 +392,10 @@ __global__ void __launch_bounds__(kBlock) kMomentum(DMesh m, DBC bu, DBC bv, DBC
     s.vInv[n] = iV;
     s.DU[n] = V * iU;
     s.DV[n] = V * iV;
+    if (s.uInv32) {
+        s.uInv32[n] = (float)iU;
+        s.vInv32[n] = (float)iV;
+    }
 }
 
 // ===========================================================================
@@ -753,6 +759,66 @@ __global__ void __launch_bounds__(kBlock) kMGCoarseOp(DLevel f, DLevel c, const 
     }
 }
 
+// The same Galerkin product in two coalesced passes: (1) one thread per fine
+// cell forms diag_n = AC[n] + its internal off-diagonals (slot order) into
+// f.res; (2) one thread per coarse cell sums diag over its members (ascending)
+// and one thread per coarse slot sums its fine slots (ascending).  Same sums in
+// the same order as kMGCoarseOp.
+__global__ void __launch_bounds__(kBlock) kMGCoarseDiag(DLevel f, const Ctl* ctl) {
+    if (finished(ctl)) return;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= f.n) return;
+    double diag = f.AC[n];
+    for (int k = f.start[n]; k < f.start[n + 1]; k++) {
+        if (f.nb[k] < 0) continue;
+        if (f.slotToCoarseSlot[k] < 0) diag += f.AF[k];
+    }
+    f.res[n] = diag;
+}
+
+__global__ void __launch_bounds__(kBlock) kMGCoarseGather(DLevel f, DLevel c, const Ctl* ctl) {
+    if (finished(ctl)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < c.refs) {
+        double a = 0.0;
+        for (int j = f.cslotStart[i]; j < f.cslotStart[i + 1]; j++) a += f.AF[f.cslotFine[j]];
+        c.AF[i] = a;
+        if (c.AF32) c.AF32[i] = (float)a;
+    } else if (i < c.refs + c.n) {
+        const int r = i - c.refs;
+        double acc = 0.0;
+        for (int j = f.memberStart[r]; j < f.memberStart[r + 1]; j++) acc += f.res[f.members[j]];
+        c.AC[r] = acc;
+        if (c.AC32) c.AC32[r] = (float)acc;
+    }
+}
+
+// computeResidual (residualAll sign = true, residuals.cuh:17-60) into f.res,
+// one thread per fine cell; kMGRestrictSum then gathers it per coarse cell.
+__global__ void __launch_bounds__(kBlock) kMGResidual(DLevel f, const Ctl* ctl) {
+    if (finished(ctl)) return;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= f.n) return;
+    double Ax = f.AC[n] * f.x[n];
+    for (int k = f.start[n]; k < f.start[n + 1]; k++) {
+        const int nb = f.nb[k];
+        if (nb >= 0) Ax += f.AF[k] * f.x[nb];
+    }
+    f.res[n] = f.b[n] - Ax;
+}
+
+// buildRestrictionKernel (multigrid.cu:398-410) as a gather in ascending fine
+// order + the coarse x memset (multigrid.cu:650-655)
+__global__ void __launch_bounds__(kBlock) kMGRestrictSum(DLevel f, DLevel c, const Ctl* ctl) {
+    if (finished(ctl)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n) return;
+    double acc = 0.0;
+    for (int j = f.memberStart[i]; j < f.memberStart[i + 1]; j++) acc += f.res[f.members[j]];
+    c.b[i] = acc;
+    c.x[i] = 0.0;
+}
+
 // jacobiRow, multigrid.cu:447-484
 __device__ __forceinline__ double jacobiRow(const DLevel& L, const double* x, int n, double w) {
     const double AC = L.AC[n];
@@ -856,8 +922,225 @@ __global__ void __launch_bounds__(kBlock) kMGProlong(DLevel f, DLevel c, const C
 }
 
 // ===========================================================================
+// Tail of the V-cycle: every level from `lt` down, in ONE workgroup.
+// Coarse levels are tiny and launch-latency bound; here each sweep is a
+// workgroup-wide loop + barrier instead of a launch.  Same arithmetic, in the
+// same order, as kMGJacobi / kMGRestrict / kMGProlongSmooth / the coarsest
+// solve; the x ping-pong is tracked per level in registers, and the answer of
+// level lt is left in that level's own x buffer for the prolongation above.
+// ===========================================================================
+namespace {
+template <class T>
+struct LV {
+    const T *AC, *AF;
+    T *b, *x, *xNew;
+};
+template <class T>
+__device__ __forceinline__ LV<T> view(const DLevel& L);
+template <>
+__device__ __forceinline__ LV<double> view<double>(const DLevel& L) {
+    return LV<double>{L.AC, L.AF, L.b, L.x, L.xNew};
+}
+template <>
+__device__ __forceinline__ LV<float> view<float>(const DLevel& L) {
+    return LV<float>{L.AC32, L.AF32, L.b32, L.x32, L.xNew32};
+}
+template <class T>
+__device__ __forceinline__ T tabs(T v) {
+    return v < T(0) ? -v : v;
+}
+}  // namespace
+
+template <class T>
+__global__ void __launch_bounds__(1024) kMGTail(const DLevel* __restrict__ Ls, int lt, int L, double wd, int pre,
+                                                int post, int coarseSweeps, const double* inv, int stage,
+                                                const Ctl* ctl) {
+    if (finished(ctl)) return;
+    constexpr int kMax = 32;  // levels lt..L-1, L - lt <= kMax (checked by the host)
+    const T w = (T)wd;
+    const int t = threadIdx.x, T_ = blockDim.x;
+    T* xa[kMax];
+    T* xb[kMax];
+    for (int l = lt; l < L; l++) {
+        const LV<T> v = view<T>(Ls[l]);
+        xa[l - lt] = v.x;
+        xb[l - lt] = v.xNew;
+    }
+    auto sweeps = [&](int l, int count) {
+        const DLevel& D = Ls[l];
+        const LV<T> v = view<T>(D);
+        for (int it = 0; it < count; it++) {
+            const T* x = xa[l - lt];
+            T* y = xb[l - lt];
+            for (int n = t; n < D.n; n += T_) {
+                const T AC = v.AC[n];
+                T r;
+                if (tabs(AC) < T(1.0e-30)) {
+                    r = x[n];
+                } else {
+                    T Ax = AC * x[n];
+                    for (int k = D.start[n]; k < D.start[n + 1]; k++) {
+                        const int nb = D.nb[k];
+                        if (nb != -1) Ax += v.AF[k] * x[nb];
+                    }
+                    r = x[n] + w * (v.b[n] - Ax) / AC;
+                }
+                y[n] = r;
+            }
+            __syncthreads();
+            T* tmp = xa[l - lt];
+            xa[l - lt] = xb[l - lt];
+            xb[l - lt] = tmp;
+        }
+    };
+    // descent
+    for (int l = lt; l < L - 1; l++) {
+        sweeps(l, pre);
+        const DLevel& F = Ls[l];
+        const DLevel& C = Ls[l + 1];
+        const LV<T> f = view<T>(F);
+        const LV<T> c = view<T>(C);
+        const T* x = xa[l - lt];
+        for (int i = t; i < C.n; i += T_) {
+            T acc = T(0);
+            for (int j = F.memberStart[i]; j < F.memberStart[i + 1]; j++) {
+                const int n = F.members[j];
+                T Ax = f.AC[n] * x[n];
+                for (int k = F.start[n]; k < F.start[n + 1]; k++) {
+                    const int nb = F.nb[k];
+                    if (nb >= 0) Ax += f.AF[k] * x[nb];
+                }
+                acc += f.b[n] - Ax;
+            }
+            c.b[i] = acc;
+            xa[l + 1 - lt][i] = T(0);
+        }
+        __syncthreads();
+    }
+    // coarsest
+    {
+        const DLevel& D = Ls[L - 1];
+        if (inv != nullptr) {
+            const LV<T> v = view<T>(D);
+            for (int r = t; r < D.n; r += T_) {
+                double acc = 0.0;
+                for (int c = 0; c < D.n; c++) acc += inv[r * D.n + c] * (double)v.b[c];
+                xa[L - 1 - lt][r] = (T)acc;
+            }
+            __syncthreads();
+        } else if (stage) {
+            // coarsest level staged in LDS for its sweeps: operator, CSR and the
+            // x ping-pong never leave the CU
+            extern __shared__ unsigned char smem[];
+            const LV<T> v = view<T>(D);
+            const int nC = D.n, rC = D.start[nC];
+            T* sx = (T*)smem;
+            T* sy = sx + nC;
+            T* sAC = sy + nC;
+            T* sb = sAC + nC;
+            T* sAF = sb + nC;
+            int* sst = (int*)(sAF + rC);
+            int* snb = sst + nC + 1;
+            const T* x0 = xa[L - 1 - lt];
+            for (int n = t; n < nC; n += T_) {
+                sx[n] = x0[n];
+                sAC[n] = v.AC[n];
+                sb[n] = v.b[n];
+            }
+            for (int n = t; n <= nC; n += T_) sst[n] = D.start[n];
+            for (int k = t; k < rC; k += T_) {
+                sAF[k] = v.AF[k];
+                snb[k] = D.nb[k];
+            }
+            __syncthreads();
+            for (int it = 0; it < coarseSweeps; it++) {
+                for (int n = t; n < nC; n += T_) {
+                    const T AC = sAC[n];
+                    T r;
+                    if (tabs(AC) < T(1.0e-30)) {
+                        r = sx[n];
+                    } else {
+                        T Ax = AC * sx[n];
+                        for (int k = sst[n]; k < sst[n + 1]; k++) {
+                            const int nb = snb[k];
+                            if (nb != -1) Ax += sAF[k] * sx[nb];
+                        }
+                        r = sx[n] + w * (sb[n] - Ax) / AC;
+                    }
+                    sy[n] = r;
+                }
+                __syncthreads();
+                T* tmp = sx;
+                sx = sy;
+                sy = tmp;
+            }
+            T* xo = xa[L - 1 - lt];
+            for (int n = t; n < nC; n += T_) xo[n] = sx[n];
+            __syncthreads();
+        } else {
+            sweeps(L - 1, coarseSweeps);
+        }
+    }
+    // ascent
+    for (int l = L - 2; l >= lt; l--) {
+        const DLevel& F = Ls[l];
+        const LV<T> f = view<T>(F);
+        const T* xc = xa[l + 1 - lt];
+        if (post >= 1) {
+            const T* x = xa[l - lt];
+            T* y = xb[l - lt];
+            for (int n = t; n < F.n; n += T_) {
+                const T xn = x[n] + xc[F.toCoarse[n]];
+                const T AC = f.AC[n];
+                if (tabs(AC) < T(1.0e-30)) {
+                    y[n] = xn;
+                    continue;
+                }
+                T Ax = AC * xn;
+                for (int k = F.start[n]; k < F.start[n + 1]; k++) {
+                    const int nb = F.nb[k];
+                    if (nb != -1) Ax += f.AF[k] * (x[nb] + xc[F.toCoarse[nb]]);
+                }
+                y[n] = xn + w * (f.b[n] - Ax) / AC;
+            }
+            __syncthreads();
+            T* tmp = xa[l - lt];
+            xa[l - lt] = xb[l - lt];
+            xb[l - lt] = tmp;
+            sweeps(l, post - 1);
+        } else {
+            T* x = xa[l - lt];
+            for (int n = t; n < F.n; n += T_) x[n] += xc[F.toCoarse[n]];
+            __syncthreads();
+        }
+    }
+    // leave level lt's answer in its own x buffer
+    const LV<T> top = view<T>(Ls[lt]);
+    if (xa[0] != top.x)
+        for (int n = t; n < Ls[lt].n; n += T_) top.x[n] = xa[0][n];
+}
+
+// ===========================================================================
 // launchers
 // ===========================================================================
+void launchMGTail(hipStream_t s, const DLevel* levels, int lt, int L, double w, int pre, int post, int coarseSweeps,
+                  bool fp32, const double* inv, size_t stageBytes, const Ctl* ctl) {
+    const int stage = stageBytes > 0 ? 1 : 0;
+    if (fp32) kMGTail<float><<<1, 1024, stageBytes, s>>>(levels, lt, L, w, pre, post, coarseSweeps, inv, stage, ctl);
+    else kMGTail<double><<<1, 1024, stageBytes, s>>>(levels, lt, L, w, pre, post, coarseSweeps, inv, stage, ctl);
+}
+void launchMGCoarseDiag(hipStream_t s, const DLevel& f, const Ctl* ctl) {
+    kMGCoarseDiag<<<blocksFor(f.n), kBlock, 0, s>>>(f, ctl);
+}
+void launchMGCoarseGather(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl) {
+    kMGCoarseGather<<<blocksFor((long)c.refs + c.n), kBlock, 0, s>>>(f, c, ctl);
+}
+void launchMGResidual(hipStream_t s, const DLevel& f, const Ctl* ctl) {
+    kMGResidual<<<blocksFor(f.n), kBlock, 0, s>>>(f, ctl);
+}
+void launchMGRestrictSum(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl) {
+    kMGRestrictSum<<<blocksFor(c.n), kBlock, 0, s>>>(f, c, ctl);
+}
 void launchGradUV(hipStream_t s, const DMesh& m, DBC bu, DBC bv, DState st, int scheme) {
     kGradUV<<<blocksFor(m.N), kBlock, 0, s>>>(m, bu, bv, st, scheme);
 }

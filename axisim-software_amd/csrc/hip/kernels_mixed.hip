@@ -1,0 +1,424 @@
+// kernels_mixed.hip -- FP32 inner solves with FP64 defect correction.
+//
+// Build extension with no reference counterpart.  The momentum and pressure
+// correction systems are still assembled in FP64 (kMomentum, kPPCoeff); only
+// the inner iterations run in FP32, on the defect of the FP64 system:
+//
+//   momentum:  r0 = b - A u0 (FP64), then the reference's Jacobi sweeps on
+//              delta (delta_0 = 0, delta_{k+1} = D^-1 (r0 - sum AF delta_k)) in
+//              FP32, and u = u0 + delta in FP64.  In exact arithmetic this is the
+//              same iterate as sweeping u itself.
+//   p':        every V-cycle is x += B (b - A x) with the FP64 residual and an
+//              FP32 V-cycle B started from zero; a linear multigrid cycle is
+//              exactly that affine map, so again only the rounding differs.
+//              The coarsest level is solved directly in FP64 (Gauss-Jordan
+//              inverse, refreshed once per p' solve).
+// Rounding errors of the FP32 parts scale with the defect, which goes to zero
+// as SIMPLE converges, so the converged fields keep FP64 accuracy.
+#include <cmath>
+
+#include "device.hpp"
+#include "mixed.hpp"
+
+namespace axs {
+
+namespace {
+__device__ __forceinline__ bool finished(const Ctl* ctl) { return ctl != nullptr && ctl->done != 0; }
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// momentum
+// ---------------------------------------------------------------------------
+
+// r0 = b - A u0 for u and v (FP64), stored FP32, and the first sweep from zero:
+// delta_1 = invAC * r0.
+__global__ void __launch_bounds__(kBlock) kMomDefect(DMesh m, DState s, MixedState x) {
+    if (finished(s.ctl)) return;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= m.N) return;
+    double au = s.uAC[n] * s.u[n], av = s.vAC[n] * s.v[n];
+    for (int k = m.faceStart[n]; k < m.faceStart[n + 1]; k++) {
+        const int nb = m.slotNb[k];
+        if (nb >= 0) {
+            const double a = s.AFuv[k];
+            au += a * s.u[nb];
+            av += a * s.v[nb];
+        }
+    }
+    const float ru = (float)(s.ub[n] - au), rv = (float)(s.vb[n] - av);
+    x.ru[n] = ru;
+    x.rv[n] = rv;
+    x.du[n] = x.invU[n] * ru;
+    x.dv[n] = x.invV[n] * rv;
+}
+
+// one FP32 Jacobi sweep of the defect equations for u and v
+__global__ void __launch_bounds__(kBlock) kMomSweep32(DMesh m, const Ctl* ctl, MixedState x, const float* __restrict__ uo,
+                                                     const float* __restrict__ vo, float* __restrict__ un,
+                                                     float* __restrict__ vn) {
+    if (finished(ctl)) return;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= m.N) return;
+    float au = x.ru[n], av = x.rv[n];
+    for (int k = m.faceStart[n]; k < m.faceStart[n + 1]; k++) {
+        const int nb = m.slotNb[k];
+        if (nb >= 0) {
+            const float a = x.AF32[k];
+            au -= a * uo[nb];
+            av -= a * vo[nb];
+        }
+    }
+    const float iu = x.invU[n], iv = x.invV[n];
+    un[n] = (iu != 0.0f) ? au * iu : uo[n];
+    vn[n] = (iv != 0.0f) ? av * iv : vo[n];
+}
+
+// last FP32 sweep fused with u += delta, v += delta (FP64)
+__global__ void __launch_bounds__(kBlock) kMomSweepApply(DMesh m, DState s, MixedState x, const float* __restrict__ uo,
+                                                        const float* __restrict__ vo, int sweep) {
+    if (finished(s.ctl)) return;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= m.N) return;
+    float du = uo[n], dv = vo[n];
+    if (sweep) {
+        float au = x.ru[n], av = x.rv[n];
+        for (int k = m.faceStart[n]; k < m.faceStart[n + 1]; k++) {
+            const int nb = m.slotNb[k];
+            if (nb >= 0) {
+                const float a = x.AF32[k];
+                au -= a * uo[nb];
+                av -= a * vo[nb];
+            }
+        }
+        const float iu = x.invU[n], iv = x.invV[n];
+        du = (iu != 0.0f) ? au * iu : uo[n];
+        dv = (iv != 0.0f) ? av * iv : vo[n];
+    }
+    s.u[n] += (double)du;
+    s.v[n] += (double)dv;
+}
+
+// ---------------------------------------------------------------------------
+// multigrid, FP32 levels
+// ---------------------------------------------------------------------------
+
+// Level-0 defect r = b - A x (FP64) -> FP32 right-hand side of the correction
+// cycle; with pre >= 1 also the first weighted-Jacobi sweep from e = 0.
+__global__ void __launch_bounds__(kBlock) kMGDefect(DLevel L, double w, int firstSweep, const Ctl* ctl) {
+    if (finished(ctl)) return;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= L.n) return;
+    double Ax = L.AC[n] * L.x[n];
+    for (int k = L.start[n]; k < L.start[n + 1]; k++) {
+        const int nb = L.nb[k];
+        if (nb >= 0) Ax += L.AF[k] * L.x[nb];
+    }
+    const float r = (float)(L.b[n] - Ax);
+    L.b32[n] = r;
+    if (firstSweep) {
+        const float ac = L.AC32[n];
+        L.xNew32[n] = (fabsf(ac) < 1.0e-30f) ? 0.0f : (float)w * r / ac;
+    } else {
+        L.x32[n] = 0.0f;
+    }
+}
+
+__device__ __forceinline__ float jacobiRow32(const DLevel& L, const float* x, int n, float w) {
+    const float AC = L.AC32[n];
+    if (fabsf(AC) < 1.0e-30f) return x[n];
+    float Ax = AC * x[n];
+    for (int k = L.start[n]; k < L.start[n + 1]; k++) {
+        const int nb = L.nb[k];
+        if (nb != -1) Ax += L.AF32[k] * x[nb];
+    }
+    return x[n] + w * (L.b32[n] - Ax) / AC;
+}
+
+__global__ void __launch_bounds__(kBlock) kMGJacobi32(DLevel L, float w, const Ctl* ctl) {
+    if (finished(ctl)) return;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= L.n) return;
+    L.xNew32[n] = jacobiRow32(L, L.x32, n, w);
+}
+
+__global__ void __launch_bounds__(1024) kMGSmoothBlock32(DLevel L, float w, int sweeps, const Ctl* ctl) {
+    if (finished(ctl)) return;
+    extern __shared__ float sx32[];
+    float* xa = sx32;
+    float* xb = sx32 + L.n;
+    for (int n = threadIdx.x; n < L.n; n += blockDim.x) xa[n] = L.x32[n];
+    __syncthreads();
+    for (int it = 0; it < sweeps; it++) {
+        for (int n = threadIdx.x; n < L.n; n += blockDim.x) {
+            const float AC = L.AC32[n];
+            float r;
+            if (fabsf(AC) < 1.0e-30f) {
+                r = xa[n];
+            } else {
+                float Ax = AC * xa[n];
+                for (int k = L.start[n]; k < L.start[n + 1]; k++) {
+                    const int nb = L.nb[k];
+                    if (nb != -1) Ax += L.AF32[k] * xa[nb];
+                }
+                r = xa[n] + w * (L.b32[n] - Ax) / AC;
+            }
+            xb[n] = r;
+        }
+        __syncthreads();
+        float* t = xa;
+        xa = xb;
+        xb = t;
+    }
+    for (int n = threadIdx.x; n < L.n; n += blockDim.x) L.x32[n] = xa[n];
+}
+
+__global__ void __launch_bounds__(kBlock) kMGRestrict32(DLevel f, DLevel c, const Ctl* ctl) {
+    if (finished(ctl)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n) return;
+    float acc = 0.0f;
+    for (int j = f.memberStart[i]; j < f.memberStart[i + 1]; j++) {
+        const int n = f.members[j];
+        float Ax = f.AC32[n] * f.x32[n];
+        for (int k = f.start[n]; k < f.start[n + 1]; k++) {
+            const int nb = f.nb[k];
+            if (nb >= 0) Ax += f.AF32[k] * f.x32[nb];
+        }
+        acc += f.b32[n] - Ax;
+    }
+    c.b32[i] = acc;
+    c.x32[i] = 0.0f;
+}
+
+__global__ void __launch_bounds__(kBlock) kMGResidual32(DLevel f, const Ctl* ctl) {
+    if (finished(ctl)) return;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= f.n) return;
+    float Ax = f.AC32[n] * f.x32[n];
+    for (int k = f.start[n]; k < f.start[n + 1]; k++) {
+        const int nb = f.nb[k];
+        if (nb >= 0) Ax += f.AF32[k] * f.x32[nb];
+    }
+    f.res32[n] = f.b32[n] - Ax;
+}
+
+__global__ void __launch_bounds__(kBlock) kMGRestrictSum32(DLevel f, DLevel c, const Ctl* ctl) {
+    if (finished(ctl)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n) return;
+    float acc = 0.0f;
+    for (int j = f.memberStart[i]; j < f.memberStart[i + 1]; j++) acc += f.res32[f.members[j]];
+    c.b32[i] = acc;
+    c.x32[i] = 0.0f;
+}
+
+__global__ void __launch_bounds__(kBlock) kMGProlongSmooth32(DLevel f, DLevel c, float w, const Ctl* ctl) {
+    if (finished(ctl)) return;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= f.n) return;
+    const float xn = f.x32[n] + c.x32[f.toCoarse[n]];
+    const float AC = f.AC32[n];
+    if (fabsf(AC) < 1.0e-30f) {
+        f.xNew32[n] = xn;
+        return;
+    }
+    float Ax = AC * xn;
+    for (int k = f.start[n]; k < f.start[n + 1]; k++) {
+        const int nb = f.nb[k];
+        if (nb != -1) Ax += f.AF32[k] * (f.x32[nb] + c.x32[f.toCoarse[nb]]);
+    }
+    f.xNew32[n] = xn + w * (f.b32[n] - Ax) / AC;
+}
+
+__global__ void __launch_bounds__(kBlock) kMGProlong32(DLevel f, DLevel c, const Ctl* ctl) {
+    if (finished(ctl)) return;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= f.n) return;
+    f.x32[n] += c.x32[f.toCoarse[n]];
+}
+
+// x (FP64) += e (FP32): the refinement step closing each correction cycle
+__global__ void __launch_bounds__(kBlock) kMGApply(DLevel L, const Ctl* ctl) {
+    if (finished(ctl)) return;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= L.n) return;
+    L.x[n] += (double)L.x32[n];
+}
+
+// FP32 copies of a level operator (level 0 each p' solve; coarse levels are
+// cast inside kMGCoarseOp64)
+__global__ void __launch_bounds__(kBlock) kMGCast(DLevel L, const Ctl* ctl) {
+    if (finished(ctl)) return;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n < L.n) L.AC32[n] = (float)L.AC[n];
+    for (int k = n; k < L.refs; k += gridDim.x * blockDim.x) L.AF32[k] = (float)L.AF[k];
+}
+
+// FP64 Galerkin product (same gather as kMGCoarseOp) + FP32 copy of the result
+__global__ void __launch_bounds__(kBlock) kMGCoarseOp64(DLevel f, DLevel c, const Ctl* ctl) {
+    if (finished(ctl)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n) return;
+    double acc = 0.0;
+    for (int j = f.memberStart[i]; j < f.memberStart[i + 1]; j++) {
+        const int n = f.members[j];
+        double diag = f.AC[n];
+        for (int k = f.start[n]; k < f.start[n + 1]; k++) {
+            if (f.nb[k] < 0) continue;
+            if (f.slotToCoarseSlot[k] < 0) diag += f.AF[k];
+        }
+        acc += diag;
+    }
+    c.AC[i] = acc;
+    c.AC32[i] = (float)acc;
+    for (int q = c.start[i]; q < c.start[i + 1]; q++) {
+        double a = 0.0;
+        for (int j = f.cslotStart[q]; j < f.cslotStart[q + 1]; j++) a += f.AF[f.cslotFine[j]];
+        c.AF[q] = a;
+        c.AF32[q] = (float)a;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// coarsest level: FP64 dense inverse (Gauss-Jordan, partial pivoting), one
+// workgroup; n <= kDirectMax.  inv is n x n row-major scratch, aug 2 n x n.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024) kCoarseInvert(DLevel L, double* aug, double* inv, int* status,
+                                                     const Ctl* ctl) {
+    if (finished(ctl)) return;
+    const int n = L.n, W = 2 * n, t = threadIdx.x, T = blockDim.x;
+    __shared__ int pivRow;
+    __shared__ double pivVal;
+    __shared__ double redV[1024];
+    __shared__ int redI[1024];
+    for (int e = t; e < n * W; e += T) {
+        const int r = e / W, col = e % W;
+        aug[e] = (col == n + r) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    for (int r = t; r < n; r += T) {
+        aug[r * W + r] = L.AC[r];
+        for (int k = L.start[r]; k < L.start[r + 1]; k++)
+            if (L.nb[k] >= 0) aug[r * W + L.nb[k]] += L.AF[k];
+    }
+    __syncthreads();
+    int singular = 0;
+    for (int p = 0; p < n; p++) {
+        double best = -1.0;
+        int bi = p;
+        for (int r = p + t; r < n; r += T) {
+            const double v = fabs(aug[r * W + p]);
+            if (v > best) {
+                best = v;
+                bi = r;
+            }
+        }
+        redV[t] = best;
+        redI[t] = bi;
+        __syncthreads();
+        for (int st = T / 2; st > 0; st >>= 1) {
+            if (t < st && (redV[t + st] > redV[t] || (redV[t + st] == redV[t] && redI[t + st] < redI[t]))) {
+                redV[t] = redV[t + st];
+                redI[t] = redI[t + st];
+            }
+            __syncthreads();
+        }
+        if (t == 0) {
+            pivRow = redI[0];
+            pivVal = aug[redI[0] * W + p];
+        }
+        __syncthreads();
+        if (pivVal == 0.0) {
+            singular = 1;
+            break;
+        }
+        if (pivRow != p)
+            for (int col = t; col < W; col += T) {
+                const double a = aug[p * W + col];
+                aug[p * W + col] = aug[pivRow * W + col];
+                aug[pivRow * W + col] = a;
+            }
+        __syncthreads();
+        const double inv0 = 1.0 / pivVal;
+        for (int col = t; col < W; col += T) aug[p * W + col] *= inv0;
+        __syncthreads();
+        for (int e = t; e < n * W; e += T) {
+            const int r = e / W, col = e % W;
+            if (r == p || col == p) continue;
+            aug[e] -= aug[r * W + p] * aug[p * W + col];
+        }
+        __syncthreads();
+        for (int r = t; r < n; r += T)
+            if (r != p) aug[r * W + p] = 0.0;
+        __syncthreads();
+    }
+    for (int e = t; e < n * n; e += T) inv[e] = aug[(e / n) * W + n + (e % n)];
+    if (t == 0) *status = singular;
+}
+
+// x_c = A_c^-1 b_c (FP64 product, FP32 in / out)
+__global__ void __launch_bounds__(kBlock) kCoarseApply(DLevel L, const double* inv, const Ctl* ctl) {
+    if (finished(ctl)) return;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= L.n) return;
+    double acc = 0.0;
+    for (int c = 0; c < L.n; c++) acc += inv[r * L.n + c] * (double)L.b32[c];
+    L.x32[r] = (float)acc;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+void launchMomDefect(hipStream_t s, const DMesh& m, DState st, MixedState x) {
+    kMomDefect<<<blocksFor(m.N), kBlock, 0, s>>>(m, st, x);
+}
+void launchMomSweep32(hipStream_t s, const DMesh& m, const Ctl* ctl, MixedState x, const float* uo, const float* vo,
+                      float* un, float* vn) {
+    kMomSweep32<<<blocksFor(m.N), kBlock, 0, s>>>(m, ctl, x, uo, vo, un, vn);
+}
+void launchMomSweepApply(hipStream_t s, const DMesh& m, DState st, MixedState x, const float* uo, const float* vo,
+                         int sweep) {
+    kMomSweepApply<<<blocksFor(m.N), kBlock, 0, s>>>(m, st, x, uo, vo, sweep);
+}
+void launchMGDefect(hipStream_t s, const DLevel& L, double w, int firstSweep, const Ctl* ctl) {
+    kMGDefect<<<blocksFor(L.n), kBlock, 0, s>>>(L, w, firstSweep, ctl);
+}
+void launchMGJacobi32(hipStream_t s, const DLevel& L, float w, const Ctl* ctl) {
+    kMGJacobi32<<<blocksFor(L.n), kBlock, 0, s>>>(L, w, ctl);
+}
+void launchMGSmoothBlock32(hipStream_t s, const DLevel& L, float w, int sweeps, const Ctl* ctl) {
+    const int threads = L.n >= 1024 ? 1024 : ((L.n + 63) / 64) * 64;
+    kMGSmoothBlock32<<<1, threads, 2 * (size_t)L.n * sizeof(float), s>>>(L, w, sweeps, ctl);
+}
+void launchMGRestrict32(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl) {
+    kMGRestrict32<<<blocksFor(c.n), kBlock, 0, s>>>(f, c, ctl);
+}
+void launchMGResidual32(hipStream_t s, const DLevel& f, const Ctl* ctl) {
+    kMGResidual32<<<blocksFor(f.n), kBlock, 0, s>>>(f, ctl);
+}
+void launchMGRestrictSum32(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl) {
+    kMGRestrictSum32<<<blocksFor(c.n), kBlock, 0, s>>>(f, c, ctl);
+}
+void launchMGProlongSmooth32(hipStream_t s, const DLevel& f, const DLevel& c, float w, const Ctl* ctl) {
+    kMGProlongSmooth32<<<blocksFor(f.n), kBlock, 0, s>>>(f, c, w, ctl);
+}
+void launchMGProlong32(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl) {
+    kMGProlong32<<<blocksFor(f.n), kBlock, 0, s>>>(f, c, ctl);
+}
+void launchMGApply(hipStream_t s, const DLevel& L, const Ctl* ctl) {
+    kMGApply<<<blocksFor(L.n), kBlock, 0, s>>>(L, ctl);
+}
+void launchMGCast(hipStream_t s, const DLevel& L, const Ctl* ctl) {
+    kMGCast<<<blocksFor(L.n), kBlock, 0, s>>>(L, ctl);
+}
+void launchMGCoarseOp64(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl) {
+    kMGCoarseOp64<<<blocksFor(c.n), kBlock, 0, s>>>(f, c, ctl);
+}
+void launchCoarseInvert(hipStream_t s, const DLevel& L, double* aug, double* inv, int* status, const Ctl* ctl) {
+    kCoarseInvert<<<1, 1024, 0, s>>>(L, aug, inv, status, ctl);
+}
+void launchCoarseApply(hipStream_t s, const DLevel& L, const double* inv, const Ctl* ctl) {
+    kCoarseApply<<<blocksFor(L.n), kBlock, 0, s>>>(L, inv, ctl);
+}
+
+}  // namespace axs

@@ -274,8 +274,9 @@ static void mgUpload(axs_mg* h, const double* AC, const double* AF, const double
 }
 
 static MGParams mgParams(int cycles, double weight, int pre, int post, int coarse, int precision, int direct) {
-    if (precision != AXS_PREC_FP64) throw std::domain_error("mixed precision is not built in this library");
-    if (direct) throw std::domain_error("coarse direct solve is not built in this library");
+    if (precision != AXS_PREC_FP64 && precision != AXS_PREC_MIXED) throw std::invalid_argument("bad precision");
+    if (direct && precision != AXS_PREC_MIXED)
+        throw std::domain_error("the FP64 coarsest direct solve is built for mixed precision");
     if (pre < 0 || post < 0 || coarse < 0) throw std::invalid_argument("negative sweep count");
     MGParams p;
     p.cycles = cycles;

@@ -20,6 +20,7 @@ namespace axs {
 
 namespace {
 constexpr int kSmoothBlockMax = 2048;  // cells a single workgroup smooths in LDS
+constexpr int kTailMax = 1024;         // levels at most this small run in the tail kernel
 }
 
 void Arena::release() {
@@ -33,6 +34,12 @@ void Arena::release() {
 void MGDevice::build(int n, const std::vector<int>& start, const std::vector<int>& nb, int maxLevels) {
     host = buildHierarchy(n, start, nb, maxLevels);
     arena.release();
+    arena32.release();
+    mixedReady = false;
+    aug = inv = nullptr;
+    invStatus = nullptr;
+    dLevels_ = nullptr;
+    tail_ = -1;
     dev.assign(host.size(), DLevel{});
     for (size_t l = 0; l < host.size(); l++) {
         const Level& h = host[l];
@@ -56,8 +63,17 @@ void MGDevice::build(int n, const std::vector<int>& start, const std::vector<int
             d.members = arena.upload(h.members);
             d.cslotStart = arena.upload(h.cslotStart);
             d.cslotFine = arena.upload(h.cslotFine);
+            d.res = arena.zeros<double>(h.n);
         }
     }
+}
+
+// LDS needed to stage the coarsest level in the tail kernel (0 = do not stage)
+size_t MGDevice::stageBytes(bool fp32) const {
+    const Level& c = host.back();
+    const size_t v = fp32 ? sizeof(float) : sizeof(double);
+    const size_t bytes = (4 * (size_t)c.n + c.refs()) * v + ((size_t)c.n + 1 + c.refs()) * sizeof(int);
+    return bytes <= 64 * 1024 ? bytes : 0;
 }
 
 void MGDevice::bindLevel0(double* AC, double* b, double* AF, double* x, double* xNew) {
@@ -68,6 +84,19 @@ void MGDevice::bindLevel0(double* AC, double* b, double* AF, double* x, double* 
     dev[0].xNew = xNew;
     x0Home_ = x;
     x0Other_ = xNew;
+    syncLevels();
+}
+
+void MGDevice::syncLevels() {
+    tail_ = -1;
+    for (int l = 0; l < (int)host.size(); l++)
+        if (host[l].n <= kTailMax) {
+            tail_ = l;
+            break;
+        }
+    if (tail_ >= 0 && (int)host.size() - tail_ > 32) tail_ = -1;  // the kernel tracks at most 32 levels
+    if (!dLevels_) dLevels_ = arena.alloc<DLevel>(host.size());
+    AXS_HIP(hipMemcpy(dLevels_, dev.data(), dev.size() * sizeof(DLevel), hipMemcpyHostToDevice));
 }
 
 void MGDevice::swapX(int l) { std::swap(dev[l].x, dev[l].xNew); }
@@ -88,12 +117,18 @@ void MGDevice::smooth(hipStream_t s, int l, int sweeps, const MGParams& p, const
 
 // vCycle, multigrid.cu:607-662
 void MGDevice::vcycle(hipStream_t s, int l, const MGParams& p, const Ctl* ctl) {
+    if (l == tail_) {
+        launchMGTail(s, dLevels_, l, (int)host.size(), p.weight, p.pre, p.post, p.coarseSweeps, false, nullptr,
+                     stageBytes(false), ctl);
+        return;
+    }
     if (host[l].coarsest()) {
         smooth(s, l, p.coarseSweeps, p, ctl);
         return;
     }
     smooth(s, l, p.pre, p, ctl);
-    launchMGRestrict(s, dev[l], dev[l + 1], ctl);
+    launchMGResidual(s, dev[l], ctl);
+    launchMGRestrictSum(s, dev[l], dev[l + 1], ctl);
     vcycle(s, l + 1, p, ctl);
     if (p.post >= 1) {
         launchMGProlongSmooth(s, dev[l], dev[l + 1], p.weight, ctl);
@@ -104,9 +139,106 @@ void MGDevice::vcycle(hipStream_t s, int l, const MGParams& p, const Ctl* ctl) {
     }
 }
 
+void MGDevice::swapX32(int l) { std::swap(dev[l].x32, dev[l].xNew32); }
+
+void MGDevice::ensureMixed() {
+    if (mixedReady) return;
+    arena32.release();
+    for (size_t l = 0; l < host.size(); l++) {
+        DLevel& d = dev[l];
+        d.AC32 = arena32.zeros<float>(d.n);
+        d.AF32 = arena32.zeros<float>(d.refs);
+        d.b32 = arena32.zeros<float>(d.n);
+        d.x32 = arena32.zeros<float>(d.n);
+        d.xNew32 = arena32.zeros<float>(d.n);
+        if (!host[l].coarsest()) d.res32 = arena32.zeros<float>(d.n);
+    }
+    const int nc = coarsestCells();
+    if (nc <= kDirectMax) {
+        aug = arena32.zeros<double>((size_t)2 * nc * nc);
+        inv = arena32.zeros<double>((size_t)nc * nc);
+        invStatus = arena32.zeros<int>(1);
+    }
+    mixedReady = true;
+    syncLevels();
+}
+
+void MGDevice::smooth32(hipStream_t s, int l, int sweeps, const MGParams& p, const Ctl* ctl) {
+    DLevel& L = dev[l];
+    if (L.n <= 0 || sweeps <= 0) return;
+    if (sweeps > 1 && L.n <= kSmoothBlockMax) {
+        launchMGSmoothBlock32(s, L, (float)p.weight, sweeps, ctl);
+        return;
+    }
+    for (int i = 0; i < sweeps; i++) {
+        launchMGJacobi32(s, L, (float)p.weight, ctl);
+        swapX32(l);
+    }
+}
+
+void MGDevice::vcycle32(hipStream_t s, int l, bool skipPre, const MGParams& p, const Ctl* ctl) {
+    if (l == tail_ && l > 0) {
+        launchMGTail(s, dLevels_, l, (int)host.size(), p.weight, p.pre, p.post, p.coarseSweeps, true,
+                     p.coarseDirect ? inv : nullptr, p.coarseDirect ? 0 : stageBytes(true), ctl);
+        return;
+    }
+    if (host[l].coarsest()) {
+        if (p.coarseDirect) launchCoarseApply(s, dev[l], inv, ctl);
+        else smooth32(s, l, skipPre ? p.coarseSweeps - 1 : p.coarseSweeps, p, ctl);
+        return;
+    }
+    if (!skipPre) smooth32(s, l, p.pre, p, ctl);
+    else smooth32(s, l, p.pre - 1, p, ctl);
+    launchMGResidual32(s, dev[l], ctl);
+    launchMGRestrictSum32(s, dev[l], dev[l + 1], ctl);
+    vcycle32(s, l + 1, false, p, ctl);
+    if (p.post >= 1) {
+        launchMGProlongSmooth32(s, dev[l], dev[l + 1], (float)p.weight, ctl);
+        swapX32(l);
+        smooth32(s, l, p.post - 1, p, ctl);
+    } else {
+        launchMGProlong32(s, dev[l], dev[l + 1], ctl);
+    }
+}
+
+// Mixed precision: every cycle is x += B (b - A x), the FP64 defect feeding an
+// FP32 V-cycle started from zero (the same affine map as the FP64 cycle).
+void MGDevice::enqueueSolveMixed(hipStream_t s, const MGParams& p, const Ctl* ctl, bool buildOperators) {
+    ensureMixed();
+    const int L = (int)host.size();
+    if (p.coarseDirect && coarsestCells() > kDirectMax)
+        throw std::domain_error("coarsest level too large for the dense FP64 solve");
+    if (buildOperators) {
+        launchMGCast(s, dev[0], ctl);
+        for (int l = 1; l < L; l++) {
+            launchMGCoarseDiag(s, dev[l - 1], ctl);
+            launchMGCoarseGather(s, dev[l - 1], dev[l], ctl);  // also writes the FP32 copies
+        }
+        if (p.coarseDirect) launchCoarseInvert(s, dev[L - 1], aug, inv, invStatus, ctl);
+    }
+    const int cycles = p.cycles > 1 ? p.cycles : 1;
+    // the first sweep from e = 0 is fused into the defect kernel
+    const bool single = L == 1;
+    const bool first = single ? (!p.coarseDirect && p.coarseSweeps >= 1) : (p.pre >= 1);
+    for (int c = 0; c < cycles; c++) {
+        launchMGDefect(s, dev[0], p.weight, first ? 1 : 0, ctl);
+        if (first) swapX32(0);
+        vcycle32(s, 0, first, p, ctl);
+        launchMGApply(s, dev[0], ctl);
+    }
+}
+
 void MGDevice::enqueueSolve(hipStream_t s, const MGParams& p, const Ctl* ctl, bool buildOperators) {
+    if (p.precision == AXS_PREC_MIXED) {
+        enqueueSolveMixed(s, p, ctl, buildOperators);
+        return;
+    }
+    if (p.coarseDirect) throw std::domain_error("the FP64 coarsest direct solve is built for mixed precision only");
     if (buildOperators)
-        for (size_t l = 1; l < dev.size(); l++) launchMGCoarseOp(s, dev[l - 1], dev[l], ctl);
+        for (size_t l = 1; l < dev.size(); l++) {
+            launchMGCoarseDiag(s, dev[l - 1], ctl);
+            launchMGCoarseGather(s, dev[l - 1], dev[l], ctl);
+        }
     const int cycles = p.cycles > 1 ? p.cycles : 1;  // cycleCount, multigrid.cuh:226
     for (int c = 0; c < cycles; c++) vcycle(s, 0, p, ctl);
     if (dev[0].x != x0Home_) {
@@ -118,6 +250,38 @@ void MGDevice::enqueueSolve(hipStream_t s, const MGParams& p, const Ctl* ctl, bo
 
 // Algorithmic bytes of one solve (distinct elements read + written, FP64 / int32).
 double MGDevice::bytesPerSolve(const MGParams& p, bool buildOperators) const {
+    if (p.precision == AXS_PREC_MIXED) {
+        const int L = (int)host.size();
+        double bytes = 0.0;
+        if (buildOperators) {
+            bytes += 12.0 * host[0].n + 12.0 * host[0].refs();  // cast level 0
+            for (int l = 1; l < L; l++) {
+                const Level& f = host[l - 1];
+                const Level& c = host[l];
+                bytes += 16.0 * f.n + 16.0 * f.refs() + 4.0 * f.cslotFine.size() + 20.0 * c.n + 16.0 * c.refs();
+            }
+        }
+        auto jac = [&](int l) { return 20.0 * host[l].n + 8.0 * host[l].refs(); };
+        auto sm = [&](int l, int sweeps) {
+            if (sweeps <= 0) return 0.0;
+            if (sweeps > 1 && host[l].n <= kSmoothBlockMax) return 16.0 * host[l].n + 8.0 * host[l].refs();
+            return sweeps * jac(l);
+        };
+        double cyc = 32.0 * host[0].n + 12.0 * host[0].refs() + 8.0 * host[0].n;  // FP64 defect (+ first sweep)
+        for (int l = 0; l < L; l++) {
+            const int pre = (l == 0) ? p.pre - 1 : p.pre;
+            if (host[l].coarsest()) {
+                cyc += p.coarseDirect ? 8.0 * host[l].n * host[l].n : sm(l, p.coarseSweeps);
+                break;
+            }
+            const double nc = host[l + 1].n;
+            cyc += sm(l, pre) + 16.0 * host[l].n + 8.0 * host[l].refs() + 12.0 * nc;
+            cyc += (p.post >= 1) ? 20.0 * host[l].n + 8.0 * host[l].refs() + 4.0 * nc + sm(l, p.post - 1)
+                                 : 12.0 * host[l].n + 4.0 * nc;
+        }
+        cyc += 20.0 * host[0].n;  // x += e
+        return bytes + (p.cycles > 1 ? p.cycles : 1) * cyc;
+    }
     double bytes = 0.0;
     const int L = (int)host.size();
     auto jac = [&](int l) { return 36.0 * host[l].n + 12.0 * host[l].refs(); };
@@ -130,7 +294,8 @@ double MGDevice::bytesPerSolve(const MGParams& p, bool buildOperators) const {
         for (int l = 1; l < L; l++) {
             const Level& f = host[l - 1];
             const Level& c = host[l];
-            bytes += 16.0 * f.n + 16.0 * f.refs() + 4.0 * f.cslotFine.size() + 16.0 * c.n + 12.0 * c.refs();
+            // fine diagonals (20N + 16S) + ordered gathers (12N + 12X + 12S_c + 12N_c)
+            bytes += 32.0 * f.n + 16.0 * f.refs() + 12.0 * f.cslotFine.size() + 12.0 * c.n + 12.0 * c.refs();
         }
     const int cycles = p.cycles > 1 ? p.cycles : 1;
     double cyc = 0.0;
@@ -141,7 +306,7 @@ double MGDevice::bytesPerSolve(const MGParams& p, bool buildOperators) const {
         }
         const double nc = host[l + 1].n;
         cyc += smoothBytes(l, p.pre);
-        cyc += 32.0 * host[l].n + 12.0 * host[l].refs() + 20.0 * nc;  // residual + restriction
+        cyc += 48.0 * host[l].n + 12.0 * host[l].refs() + 20.0 * nc;  // residual pass + ordered gather
         if (p.post >= 1) cyc += 40.0 * host[l].n + 12.0 * host[l].refs() + 8.0 * nc + smoothBytes(l, p.post - 1);
         else cyc += 20.0 * host[l].n + 8.0 * nc;
     }
@@ -211,6 +376,7 @@ void Solver::setMesh(const axs_mesh_view& v) {
     haveMesh_ = true;
     meshUploaded_ = false;
     mgBuilt_ = false;
+    mixedAllocated_ = false;
     nColors_ = 0;
     colorStart_.clear();
 }
@@ -301,6 +467,22 @@ void Solver::allocateState() {
     stateAllocated_ = true;
 }
 
+void Solver::allocateMixed() {
+    if (!mixedAllocated_) {
+        mixedArena_.release();
+        const int N = dm_.N, S = dm_.S;
+        MixedState x{};
+        x.AF32 = mixedArena_.zeros<float>(S);
+        for (float** p : {&x.invU, &x.invV, &x.ru, &x.rv, &x.du, &x.dv, &x.du2, &x.dv2})
+            *p = mixedArena_.zeros<float>(N);
+        mx_ = x;
+        mixedAllocated_ = true;
+    }
+    ds_.AF32 = mx_.AF32;
+    ds_.uInv32 = mx_.invU;
+    ds_.vInv32 = mx_.invV;
+}
+
 // createBoundaryFieldHost/Device (memory_manger.cpp:51-189) with the value of
 // each pulsatile group at `time` (structs/boundary_struct.h:184-192)
 void Solver::uploadBCs(double time) {
@@ -365,7 +547,19 @@ void Solver::enqueueIteration(hipStream_t s) {
     launchMomentum(s, dm_, bu, bv, bp, st, ph);
 
     // momentum solves (LinearSolver::run, linear_solver.cu:91-148, 209-271)
-    if (c.linear_solver == AXS_LINEAR_GS_RB && nColors_ > 0) {
+    if (c.precision == AXS_PREC_MIXED) {
+        // FP32 Jacobi on the FP64 defect: sweep 1 inside kMomDefect, sweeps
+        // 2..L-1 in FP32, sweep L fused with u += delta
+        const int L = c.linear_max_iter;
+        launchMomDefect(s, dm_, st, mx_);
+        float *du = mx_.du, *dv = mx_.dv, *du2 = mx_.du2, *dv2 = mx_.dv2;
+        for (int it = 2; it < L; it++) {
+            launchMomSweep32(s, dm_, st.ctl, mx_, du, dv, du2, dv2);
+            std::swap(du, du2);
+            std::swap(dv, dv2);
+        }
+        launchMomSweepApply(s, dm_, st, mx_, du, dv, L >= 2 ? 1 : 0);
+    } else if (c.linear_solver == AXS_LINEAR_GS_RB && nColors_ > 0) {
         for (int it = 0; it < c.linear_max_iter; it++)
             for (int col = 0; col < nColors_; col++) {
                 const int begin = colorStart_[col], count = colorStart_[col + 1] - begin;
@@ -501,7 +695,11 @@ void Solver::run(axs_progress_cb cb, void* user, bool continueRun) {
                 if (!std::isfinite(bc_[v].value[g]) || !(bc_[v].amplitude[g] >= 0.0) || !(bc_[v].frequency[g] > 0.0))
                     throw std::invalid_argument("Pulsatile inlet needs a finite mean velocity, A >= 0, and f > 0 Hz.");
             }
-    if (c.precision != AXS_PREC_FP64) throw std::domain_error("mixed precision is not built in this library");
+    const bool mixed = c.precision == AXS_PREC_MIXED;
+    if (mixed && c.linear_solver != AXS_LINEAR_JACOBI)
+        throw std::domain_error("mixed precision momentum sweeps are Jacobi only");
+    if (mixed && !c.use_multigrid) throw std::domain_error("mixed precision needs the multigrid pressure solve");
+    if (c.coarse_direct && !mixed) throw std::domain_error("the FP64 coarsest direct solve is built for mixed precision");
 
     if (!meshUploaded_) uploadMesh();
     std::string why;
@@ -522,6 +720,16 @@ void Solver::run(axs_progress_cb cb, void* user, bool continueRun) {
         mgBuiltLevels_ = c.mg_max_levels;
     }
     mg_.bindLevel0(ds_.ppAC, ds_.ppb, ds_.ppAF, ds_.pp, ds_.ppTmp);
+    if (mixed) {
+        allocateMixed();
+        mg_.ensureMixed();
+        if (c.coarse_direct && mg_.coarsestCells() > kDirectMax)
+            throw std::domain_error("coarsest level too large for the dense FP64 solve");
+    } else {
+        ds_.AF32 = nullptr;
+        ds_.uInv32 = nullptr;
+        ds_.vInv32 = nullptr;
+    }
     if (c.linear_solver == AXS_LINEAR_GS_RB && nColors_ == 0) {
         std::vector<int> order;
         nColors_ = greedyColoring(mesh_.nCells, csrStart_.data(), csrNb_.data(), colorStart_, order);

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../hip/device.hpp"
+#include "../hip/mixed.hpp"
 #include "axs.h"
 #include "mesh_types.hpp"
 #include "topology.hpp"
@@ -72,24 +73,38 @@ public:
     std::vector<Level> host;
     std::vector<DLevel> dev;
     Arena arena;
-    // Coarsest-level dense LU (FP64), for coarse_direct.
-    double* lu = nullptr;
-    int* piv = nullptr;
+    Arena arena32;  // FP32 copies and the coarsest inverse (mixed precision)
+    // Coarsest-level FP64 inverse (coarse_direct): Gauss-Jordan work + result.
+    double* aug = nullptr;
+    double* inv = nullptr;
+    int* invStatus = nullptr;
+    bool mixedReady = false;
 
     void build(int n, const std::vector<int>& start, const std::vector<int>& nb, int maxLevels);
     // Level 0 operator and x buffers are owned by the caller.
     void bindLevel0(double* AC, double* b, double* AF, double* x, double* xNew);
+    // Allocate the FP32 buffers of every level (idempotent).
+    void ensureMixed();
     // Enqueue one solve (enqueueRun, multigrid.cu:766-811).  Leaves the answer
     // in the level-0 buffer that was bound as x.
     void enqueueSolve(hipStream_t s, const MGParams& p, const Ctl* ctl, bool buildOperators);
     double bytesPerSolve(const MGParams& p, bool buildOperators) const;
+    int coarsestCells() const { return host.empty() ? 0 : host.back().n; }
+    size_t stageBytes(bool fp32) const;
 
 private:
     void vcycle(hipStream_t s, int l, const MGParams& p, const Ctl* ctl);
     void smooth(hipStream_t s, int l, int sweeps, const MGParams& p, const Ctl* ctl);
+    void enqueueSolveMixed(hipStream_t s, const MGParams& p, const Ctl* ctl, bool buildOperators);
+    void vcycle32(hipStream_t s, int l, bool skipPre, const MGParams& p, const Ctl* ctl);
+    void smooth32(hipStream_t s, int l, int sweeps, const MGParams& p, const Ctl* ctl);
     void swapX(int l);
+    void swapX32(int l);
+    void syncLevels();  // upload the DLevel array the tail kernel reads
     double* x0Home_ = nullptr;
     double* x0Other_ = nullptr;
+    DLevel* dLevels_ = nullptr;
+    int tail_ = -1;  // first level run by the single-workgroup tail kernel
 };
 
 class Solver {
@@ -149,6 +164,11 @@ private:
     bool meshUploaded_ = false, stateAllocated_ = false;
     int partialBlocks_ = 0;
     int histCap_ = 0;
+
+    MixedState mx_{};
+    Arena mixedArena_;
+    bool mixedAllocated_ = false;
+    void allocateMixed();
 
     MGDevice mg_;
     bool mgBuilt_ = false;
