@@ -145,6 +145,7 @@ def load_library(path: os.PathLike | None = None) -> C.CDLL:
         "axs_mg_destroy": (None, [S]),
         "axs_mg_levels": (C.c_int, [S]),
         "axs_mg_level_info": (C.c_int, [S, C.c_int, _ip, _ip]),
+        "axs_mg_tail_info": (C.c_int, [S, C.c_int, _ip, _ip]),
         "axs_mg_get_maps": (C.c_int, [S, C.c_int, _ip, _ip, _ip, _ip]),
         "axs_mg_solve": (C.c_int, [S, _dp, _dp, _dp, _dp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
                                    C.c_int, C.c_int]),
@@ -508,6 +509,12 @@ class Multigrid:
         n, r = C.c_int(), C.c_int()
         _check(lib().axs_mg_level_info(self._h, level, C.byref(n), C.byref(r)))
         return n.value, r.value
+
+    def tail_info(self, precision: int = PREC_FP64) -> tuple[int, int]:
+        """(first level of the single-workgroup LDS tail or -1, its LDS bytes)"""
+        lv, by = C.c_int(), C.c_int()
+        _check(lib().axs_mg_tail_info(self._h, precision, C.byref(lv), C.byref(by)))
+        return lv.value, by.value
 
     def maps(self, level: int) -> dict:
         n, r = self.level_info(level)

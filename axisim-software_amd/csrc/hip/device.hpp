@@ -52,8 +52,10 @@ struct DState {
     double *uOld, *vOld, *uOld2, *vOld2;
     double *uAC, *ub, *uInv, *vAC, *vb, *vInv, *AFuv;
     double *ppAC, *ppb, *ppInv, *ppAF;
+    double2 *uvA, *uvB;  // interleaved (u, v) iterates of the momentum Jacobi sweeps
     // FP32 copies written by kMomentum in mixed precision (null otherwise)
-    float *AF32, *uInv32, *vInv32;
+    float* AF32;
+    float2* uvInv32;  // (1/AC_u, 1/AC_v) interleaved
     double *partial;   // per-block residual partial sums
     double *hist;      // residual history: U, V, Continuity, iteration per check
     Ctl* ctl;
@@ -80,6 +82,31 @@ struct DLevel {
     float* res32 = nullptr;
 };
 
+// Where one level of the V-cycle tail lives in the tail kernel's LDS image:
+// offsets in elements of the value type (AC, b, x0, x1, AF) and of int
+// (start, nb, toC, mS, mem).  Built by the host, read by the kernel.
+struct TailLevel {
+    int n, refs, nc;
+    int AC, b, x0, x1, AF;
+    int start, nb, toC, mS, mem;
+};
+// One staged array: `words` 32-bit words from src to word `dst` of the LDS
+// image; `first` = words of the copies before it.
+struct TailCopy {
+    const void* src;
+    int dst, first, words;
+};
+constexpr int kTailLevels = 8;         // levels the tail kernel may hold
+constexpr int kTailCopies = 72;        // staged arrays it may copy (3 + 8 per level)
+constexpr int kTailStaticLds = 2048;   // LDS the tail kernel declares statically
+// Everything a tail launch needs (device copies of the plan and copy list).
+struct TailLaunch {
+    const TailLevel* plan = nullptr;
+    const TailCopy* copies = nullptr;
+    int ncopy = 0, totalWords = 0, lt = -1, nl = 0, intBase = 0, threads = 0;
+    size_t ldsBytes = 0;
+};
+
 inline int blocksFor(long n, int t = kBlock) { return (int)((n + t - 1) / t); }
 
 // ---- launchers (kernels.hip) ----
@@ -89,6 +116,10 @@ void launchGradField(hipStream_t s, const DMesh& m, DBC bc, const double* phi, d
 void launchMomentum(hipStream_t s, const DMesh& m, DBC bu, DBC bv, DBC bp, DState st, Physics ph);
 void launchJacobiUV(hipStream_t s, const DMesh& m, DState st, const double* uo, const double* vo, double* un,
                     double* vn);
+// the same sweep with either side interleaved as (u, v) pairs: a non-null uvo
+// (uvn) replaces uo/vo (un/vn)
+void launchJacobiUVPacked(hipStream_t s, const DMesh& m, DState st, const double* uo, const double* vo,
+                          const double2* uvo, double* un, double* vn, double2* uvn);
 void launchGSColorUV(hipStream_t s, const DMesh& m, DState st, const int* order, int begin, int count);
 void launchJacobiSys(hipStream_t s, const DMesh& m, const double* b, const double* inv, const double* AF,
                      const double* xo, double* xn, const Ctl* ctl);
@@ -106,21 +137,24 @@ void launchCopy(hipStream_t s, double* dst, const double* src, int n, const Ctl*
 void launchFill(hipStream_t s, double* dst, double value, int n, const Ctl* ctl);
 
 // multigrid (FP64)
-void launchMGCoarseOp(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl);
 void launchMGJacobi(hipStream_t s, const DLevel& L, double w, const Ctl* ctl);
 void launchMGSmoothBlock(hipStream_t s, const DLevel& L, double w, int sweeps, const Ctl* ctl);
-void launchMGRestrict(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl);
 void launchMGProlongSmooth(hipStream_t s, const DLevel& f, const DLevel& c, double w, const Ctl* ctl);
 void launchMGProlong(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl);
-// V-cycle of levels lt..L-1 in one workgroup (levels: device copy of the DLevel
-// array); stageBytes > 0 stages the coarsest level in that much LDS
-void launchMGTail(hipStream_t s, const DLevel* levels, int lt, int L, double w, int pre, int post, int coarseSweeps,
-                  bool fp32, const double* inv, size_t stageBytes, const Ctl* ctl);
+// V-cycle of levels lt..lt+nl-1 in one workgroup, all of them staged in LDS
+// (levels: device copy of the DLevel array)
+void launchMGTail(hipStream_t s, const DLevel* levels, const TailLaunch& t, double w, int preDone, int pre, int post,
+                  int coarseSweeps, bool fp32, const double* inv, const Ctl* ctl);
+// LDS one workgroup may use on the current device (opting the tail kernel in
+// beyond 64 KiB where the device allows it)
+int tailLdsLimit();
 // Galerkin product in two coalesced passes (fine diagonals, then gathers)
 void launchMGCoarseDiag(hipStream_t s, const DLevel& f, const Ctl* ctl);
 void launchMGCoarseGather(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl);
-// residual per fine cell + ordered gather per coarse cell
+// residual per fine cell + ordered gather per coarse cell (sweep1: also the
+// coarse level's first smoothing sweep from zero); fp32: the FP32 copies
 void launchMGResidual(hipStream_t s, const DLevel& f, const Ctl* ctl);
-void launchMGRestrictSum(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl);
+void launchMGRestrictSum(hipStream_t s, const DLevel& f, const DLevel& c, double w, int sweep1, bool fp32,
+                         const Ctl* ctl);
 
 }  // namespace axs

@@ -392,19 +392,35 @@ __global__ void __launch_bounds__(kBlock) kMomentum(DMesh m, DBC bu, DBC bv, DBC
     s.vInv[n] = iV;
     s.DU[n] = V * iU;
     s.DV[n] = V * iV;
-    if (s.uInv32) {
-        s.uInv32[n] = (float)iU;
-        s.vInv32[n] = (float)iV;
-    }
+    if (s.uvInv32) s.uvInv32[n] = make_float2((float)iU, (float)iV);
 }
 
 // ===========================================================================
 // Momentum smoothers: one Jacobi sweep of u and v together (jacobi,
 // linear_solver.cu:9-33); the two systems share CSR and off-diagonals.
 // ===========================================================================
-__global__ void __launch_bounds__(kBlock) kJacobiUV(DMesh m, DState s, const double* __restrict__ uo,
-                                                    const double* __restrict__ vo, double* __restrict__ un,
-                                                    double* __restrict__ vn) {
+// The u and v iterates either as two arrays or interleaved as (u, v) pairs.
+// The sweeps in between the first and the last keep them interleaved, so a
+// neighbour's u and v arrive in one 16-byte load from one cache line instead
+// of two loads from two lines (the gathers are what the sweep over-fetches).
+struct UVIn {
+    const double *u, *v;
+    const double2* uv;
+    __device__ __forceinline__ double2 at(int i) const { return uv ? uv[i] : make_double2(u[i], v[i]); }
+};
+struct UVOut {
+    double *u, *v;
+    double2* uv;
+    __device__ __forceinline__ void put(int i, double a, double b) const {
+        if (uv) uv[i] = make_double2(a, b);
+        else {
+            u[i] = a;
+            v[i] = b;
+        }
+    }
+};
+
+__global__ void __launch_bounds__(kBlock) kJacobiUV(DMesh m, DState s, UVIn xo, UVOut xn) {
     if (finished(s.ctl)) return;
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= m.N) return;
@@ -413,13 +429,14 @@ __global__ void __launch_bounds__(kBlock) kJacobiUV(DMesh m, DState s, const dou
         const int nb = m.slotNb[k];
         if (nb >= 0) {
             const double a = s.AFuv[k];
-            au -= a * uo[nb];
-            av -= a * vo[nb];
+            const double2 x = xo.at(nb);
+            au -= a * x.x;
+            av -= a * x.y;
         }
     }
     const double iu = s.uInv[n], iv = s.vInv[n];
-    un[n] = (iu != 0.0) ? au * iu : uo[n];
-    vn[n] = (iv != 0.0) ? av * iv : vo[n];
+    const double2 x = xo.at(n);
+    xn.put(n, (iu != 0.0) ? au * iu : x.x, (iv != 0.0) ? av * iv : x.y);
 }
 
 // gaussSeidelColorSweep, linear_solver.cu:42-72, u and v together
@@ -733,37 +750,14 @@ __global__ void __launch_bounds__(kBlock) kFill(double* dst, double value, int n
 // Agglomeration multigrid, FP64 (multigrid.cu:350-541)
 // ===========================================================================
 
-// Galerkin coarse operator by gather: one thread per coarse cell sums its
-// members' diagonal + internal faces (ascending fine index) and its coarse
-// row's off-diagonals from their fine slots (ascending fine slot).  Same
-// additions as the reference's per-fine-row scatter, in a fixed order.
-__global__ void __launch_bounds__(kBlock) kMGCoarseOp(DLevel f, DLevel c, const Ctl* ctl) {
-    if (finished(ctl)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= c.n) return;
-    double acc = 0.0;
-    for (int j = f.memberStart[i]; j < f.memberStart[i + 1]; j++) {
-        const int n = f.members[j];
-        double diag = f.AC[n];
-        for (int k = f.start[n]; k < f.start[n + 1]; k++) {
-            if (f.nb[k] < 0) continue;
-            if (f.slotToCoarseSlot[k] < 0) diag += f.AF[k];
-        }
-        acc += diag;
-    }
-    c.AC[i] = acc;
-    for (int q = c.start[i]; q < c.start[i + 1]; q++) {
-        double a = 0.0;
-        for (int j = f.cslotStart[q]; j < f.cslotStart[q + 1]; j++) a += f.AF[f.cslotFine[j]];
-        c.AF[q] = a;
-    }
-}
-
-// The same Galerkin product in two coalesced passes: (1) one thread per fine
-// cell forms diag_n = AC[n] + its internal off-diagonals (slot order) into
-// f.res; (2) one thread per coarse cell sums diag over its members (ascending)
-// and one thread per coarse slot sums its fine slots (ascending).  Same sums in
-// the same order as kMGCoarseOp.
+// Galerkin coarse operator (buildCoarseOperator, multigrid.cu:350-389) by
+// gather, in two coalesced passes: (1) one thread per fine cell forms
+// diag_n = AC[n] + its internal off-diagonals (slot order) into f.res; (2) one
+// thread per coarse cell sums diag over its members (ascending fine index) and
+// one thread per coarse slot sums its fine slots (ascending fine slot).  The
+// same additions as the reference's per-fine-row scatter, in a fixed order.
+// (One thread per coarse cell doing both passes is a chain of dependent loads
+// per member and runs several times slower even on small levels.)
 __global__ void __launch_bounds__(kBlock) kMGCoarseDiag(DLevel f, const Ctl* ctl) {
     if (finished(ctl)) return;
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -805,18 +799,6 @@ __global__ void __launch_bounds__(kBlock) kMGResidual(DLevel f, const Ctl* ctl) 
         if (nb >= 0) Ax += f.AF[k] * f.x[nb];
     }
     f.res[n] = f.b[n] - Ax;
-}
-
-// buildRestrictionKernel (multigrid.cu:398-410) as a gather in ascending fine
-// order + the coarse x memset (multigrid.cu:650-655)
-__global__ void __launch_bounds__(kBlock) kMGRestrictSum(DLevel f, DLevel c, const Ctl* ctl) {
-    if (finished(ctl)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= c.n) return;
-    double acc = 0.0;
-    for (int j = f.memberStart[i]; j < f.memberStart[i + 1]; j++) acc += f.res[f.members[j]];
-    c.b[i] = acc;
-    c.x[i] = 0.0;
 }
 
 // jacobiRow, multigrid.cu:447-484
@@ -872,27 +854,6 @@ __global__ void __launch_bounds__(1024) kMGSmoothBlock(DLevel L, double w, int s
     for (int n = threadIdx.x; n < L.n; n += blockDim.x) L.x[n] = xa[n];
 }
 
-// computeResidual (residualAll sign=true) + buildRestrictionKernel
-// (multigrid.cu:398-410) + the coarse x memset (multigrid.cu:650-655): per
-// coarse cell, the residuals of its members summed in ascending fine order.
-__global__ void __launch_bounds__(kBlock) kMGRestrict(DLevel f, DLevel c, const Ctl* ctl) {
-    if (finished(ctl)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= c.n) return;
-    double acc = 0.0;
-    for (int j = f.memberStart[i]; j < f.memberStart[i + 1]; j++) {
-        const int n = f.members[j];
-        double Ax = f.AC[n] * f.x[n];
-        for (int k = f.start[n]; k < f.start[n + 1]; k++) {
-            const int nb = f.nb[k];
-            if (nb >= 0) Ax += f.AF[k] * f.x[nb];
-        }
-        acc += f.b[n] - Ax;
-    }
-    c.b[i] = acc;
-    c.x[i] = 0.0;
-}
-
 // buildProlongationKernel (multigrid.cu:415-427) fused into the first
 // post-smoothing sweep: x + P x_c is formed on the fly for the cell and its
 // neighbours (the same sums the prolongation would store).
@@ -922,212 +883,333 @@ __global__ void __launch_bounds__(kBlock) kMGProlong(DLevel f, DLevel c, const C
 }
 
 // ===========================================================================
-// Tail of the V-cycle: every level from `lt` down, in ONE workgroup.
-// Coarse levels are tiny and launch-latency bound; here each sweep is a
-// workgroup-wide loop + barrier instead of a launch.  Same arithmetic, in the
-// same order, as kMGJacobi / kMGRestrict / kMGProlongSmooth / the coarsest
-// solve; the x ping-pong is tracked per level in registers, and the answer of
-// level lt is left in that level's own x buffer for the prolongation above.
+// Restriction, FP64 and FP32 (the mixed-precision cycle) from one template.
 // ===========================================================================
 namespace {
 template <class T>
 struct LV {
     const T *AC, *AF;
-    T *b, *x, *xNew;
+    T *b, *x, *xNew, *res;
 };
 template <class T>
 __device__ __forceinline__ LV<T> view(const DLevel& L);
 template <>
 __device__ __forceinline__ LV<double> view<double>(const DLevel& L) {
-    return LV<double>{L.AC, L.AF, L.b, L.x, L.xNew};
+    return LV<double>{L.AC, L.AF, L.b, L.x, L.xNew, L.res};
 }
 template <>
 __device__ __forceinline__ LV<float> view<float>(const DLevel& L) {
-    return LV<float>{L.AC32, L.AF32, L.b32, L.x32, L.xNew32};
+    return LV<float>{L.AC32, L.AF32, L.b32, L.x32, L.xNew32, L.res32};
 }
 template <class T>
 __device__ __forceinline__ T tabs(T v) {
     return v < T(0) ? -v : v;
 }
+// jacobiRow (multigrid.cu:447-484) applied to x = 0: every product with x is a
+// (signed) zero, b - Ax == b, and 0 + w*b/AC is the reference's value bit for
+// bit (a zero result comes out +0 either way).
+template <class T>
+__device__ __forceinline__ T sweepFromZero(T b, T AC, T w) {
+    if (tabs(AC) < T(1.0e-30)) return T(0);
+    return T(0) + w * b / AC;
+}
+}  // namespace
+
+// buildRestrictionKernel (multigrid.cu:398-410) as a gather in ascending fine
+// order of the residuals kMGResidual left in f.res, then the coarse x memset
+// (multigrid.cu:650-655) -- or, with sweep1, the coarse level's first
+// pre-smoothing sweep, which starts from that zero.
+template <class T>
+__global__ void __launch_bounds__(kBlock) kMGRestrictSum(DLevel f, DLevel c, double w, int sweep1, const Ctl* ctl) {
+    if (finished(ctl)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n) return;
+    const LV<T> fv = view<T>(f);
+    const LV<T> cv = view<T>(c);
+    T acc = T(0);
+    for (int j = f.memberStart[i]; j < f.memberStart[i + 1]; j++) acc += fv.res[f.members[j]];
+    cv.b[i] = acc;
+    cv.x[i] = sweep1 ? sweepFromZero(acc, cv.AC[i], (T)w) : T(0);
+}
+
+// ===========================================================================
+// Tail of the V-cycle: the coarsest levels, from `lt` down, in ONE workgroup
+// with every one of them staged in LDS (operator, CSR, transfer maps, b and
+// the x ping-pong).  Coarse levels are tiny and launch-latency bound; here a
+// sweep is an LDS pass + barrier instead of a kernel launch.  Thread t owns
+// cell t of every level (the host launches at least as many threads as the
+// largest tail level has cells) and keeps that cell's row in registers, so a
+// sweep waits for one round of independent LDS reads instead of a chain per
+// slot.  Same arithmetic, in the same order, as kMGJacobi / kMGResidual +
+// kMGRestrictSum / kMGProlongSmooth / the coarsest solve.  Level lt's x and b
+// come in from its global buffers (x after `preDone` sweeps) and its answer
+// goes back to its x buffer.
+// ===========================================================================
+namespace {
+constexpr int kRowK = 12;     // slots a row keeps in registers (longer rows loop over LDS)
+constexpr int kMemberK = 8;   // aggregate members gathered without a loop
+constexpr int kCopyR = 24;    // staged words each thread holds per round
+constexpr int kMaxCopies = kTailCopies;
+
+template <class T>
+struct TRow {
+    T ac, b;
+    int k0, deg;
+    int nb[kRowK];
+    T af[kRowK];
+};
+
+template <class T>
+__device__ __forceinline__ void loadRow(TRow<T>& r, const T* AC, const T* b, const T* AF, const int* st,
+                                        const int* nbs, int n) {
+    r.ac = AC[n];
+    r.b = b[n];
+    r.k0 = st[n];
+    r.deg = st[n + 1] - r.k0;
+#pragma unroll
+    for (int k = 0; k < kRowK; k++) {
+        const bool in = k < r.deg;
+        r.nb[k] = in ? nbs[r.k0 + k] : -1;
+        r.af[k] = in ? AF[r.k0 + k] : T(0);
+    }
+}
+
+// Ax of the row: AC*xn + AF*x(nb) over the slots with a neighbour, in slot
+// order.  xat(j) gives the (possibly prolonged) x of cell j.
+template <class T, class XAt>
+__device__ __forceinline__ T rowAx(const TRow<T>& r, T xn, XAt xat, const T* AF, const int* nbs) {
+    T Ax = r.ac * xn;
+    if (r.deg <= kRowK) {
+        T xv[kRowK];
+#pragma unroll
+        for (int k = 0; k < kRowK; k++) xv[k] = r.nb[k] != -1 ? xat(r.nb[k]) : T(0);
+#pragma unroll
+        for (int k = 0; k < kRowK; k++)
+            if (r.nb[k] != -1) Ax += r.af[k] * xv[k];
+    } else {
+        for (int k = r.k0; k < r.k0 + r.deg; k++) {
+            const int nb = nbs[k];
+            if (nb != -1) Ax += AF[k] * xat(nb);
+        }
+    }
+    return Ax;
+}
 }  // namespace
 
 template <class T>
-__global__ void __launch_bounds__(1024) kMGTail(const DLevel* __restrict__ Ls, int lt, int L, double wd, int pre,
-                                                int post, int coarseSweeps, const double* inv, int stage,
-                                                const Ctl* ctl) {
+__global__ void __launch_bounds__(1024) kMGTail(const DLevel* __restrict__ Ls, const TailLevel* __restrict__ tl,
+                                                const TailCopy* __restrict__ copies, int ncopy, int totalWords,
+                                                int lt, int nl, int intBase, double wd, int preDone, int pre,
+                                                int post, int coarseSweeps, const double* inv, const Ctl* ctl) {
     if (finished(ctl)) return;
-    constexpr int kMax = 32;  // levels lt..L-1, L - lt <= kMax (checked by the host)
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ const unsigned* cpSrc[kMaxCopies];
+    __shared__ int cpDst[kMaxCopies], cpFirst[kMaxCopies + 1];
+    T* sT = (T*)smem;
+    int* sI = (int*)(smem + intBase);
+    unsigned* sW = (unsigned*)smem;
     const T w = (T)wd;
-    const int t = threadIdx.x, T_ = blockDim.x;
-    T* xa[kMax];
-    T* xb[kMax];
-    for (int l = lt; l < L; l++) {
-        const LV<T> v = view<T>(Ls[l]);
-        xa[l - lt] = v.x;
-        xb[l - lt] = v.xNew;
+    const int t = threadIdx.x, NT = blockDim.x;
+
+    // ---- stage: every word of the image loaded before any is stored, so the
+    // global reads of all arrays of all levels are in flight together
+    for (int i = t; i < ncopy; i += NT) {
+        cpSrc[i] = (const unsigned*)copies[i].src;
+        cpDst[i] = copies[i].dst;
+        cpFirst[i] = copies[i].first;
     }
-    auto sweeps = [&](int l, int count) {
-        const DLevel& D = Ls[l];
-        const LV<T> v = view<T>(D);
+    if (t == 0) cpFirst[ncopy] = totalWords;
+    __syncthreads();
+    for (int base = 0; base < totalWords; base += kCopyR * NT) {
+        unsigned v[kCopyR];
+        int seg = 0;
+#pragma unroll
+        for (int r = 0; r < kCopyR; r++) {
+            const int wd_ = base + r * NT + t;
+            if (wd_ < totalWords) {
+                while (cpFirst[seg + 1] <= wd_) seg++;
+                v[r] = cpSrc[seg][wd_ - cpFirst[seg]];
+            }
+        }
+        seg = 0;
+#pragma unroll
+        for (int r = 0; r < kCopyR; r++) {
+            const int wd_ = base + r * NT + t;
+            if (wd_ < totalWords) {
+                while (cpFirst[seg + 1] <= wd_) seg++;
+                sW[cpDst[seg] + (wd_ - cpFirst[seg])] = v[r];
+            }
+        }
+    }
+    __syncthreads();
+
+    unsigned flip = 0;  // bit q set: level q's current x is its x1 array
+    auto X = [&](const TailLevel& S, int q) { return sT + (((flip >> q) & 1u) ? S.x1 : S.x0); };
+    auto Y = [&](const TailLevel& S, int q) { return sT + (((flip >> q) & 1u) ? S.x0 : S.x1); };
+    // jacobiRow sweeps of level q, the row held in registers across them
+    auto sweeps = [&](int q, int count) {
+        if (count <= 0) return;
+        const TailLevel S = tl[q];
+        const T* AF = sT + S.AF;
+        const int* nbs = sI + S.nb;
+        const bool own = t < S.n;
+        TRow<T> r;
+        if (own) loadRow(r, sT + S.AC, sT + S.b, AF, sI + S.start, nbs, t);
         for (int it = 0; it < count; it++) {
-            const T* x = xa[l - lt];
-            T* y = xb[l - lt];
-            for (int n = t; n < D.n; n += T_) {
-                const T AC = v.AC[n];
-                T r;
-                if (tabs(AC) < T(1.0e-30)) {
-                    r = x[n];
+            const T* x = X(S, q);
+            T* y = Y(S, q);
+            if (own) {
+                const T xn = x[t];
+                T res;
+                if (tabs(r.ac) < T(1.0e-30)) {
+                    res = xn;
                 } else {
-                    T Ax = AC * x[n];
-                    for (int k = D.start[n]; k < D.start[n + 1]; k++) {
-                        const int nb = D.nb[k];
-                        if (nb != -1) Ax += v.AF[k] * x[nb];
-                    }
-                    r = x[n] + w * (v.b[n] - Ax) / AC;
+                    const T Ax = rowAx(r, xn, [&](int j) { return x[j]; }, AF, nbs);
+                    res = xn + w * (r.b - Ax) / r.ac;
                 }
-                y[n] = r;
+                y[t] = res;
             }
             __syncthreads();
-            T* tmp = xa[l - lt];
-            xa[l - lt] = xb[l - lt];
-            xb[l - lt] = tmp;
+            flip ^= 1u << q;
         }
     };
-    // descent
-    for (int l = lt; l < L - 1; l++) {
-        sweeps(l, pre);
-        const DLevel& F = Ls[l];
-        const DLevel& C = Ls[l + 1];
-        const LV<T> f = view<T>(F);
-        const LV<T> c = view<T>(C);
-        const T* x = xa[l - lt];
-        for (int i = t; i < C.n; i += T_) {
-            T acc = T(0);
-            for (int j = F.memberStart[i]; j < F.memberStart[i + 1]; j++) {
-                const int n = F.members[j];
-                T Ax = f.AC[n] * x[n];
-                for (int k = F.start[n]; k < F.start[n + 1]; k++) {
-                    const int nb = F.nb[k];
-                    if (nb >= 0) Ax += f.AF[k] * x[nb];
-                }
-                acc += f.b[n] - Ax;
+
+    // ---- descent
+    int done = preDone;
+    for (int q = 0; q < nl - 1; q++) {
+        sweeps(q, pre - done);
+        const TailLevel F = tl[q];
+        const TailLevel C = tl[q + 1];
+        const bool coarsest = q + 1 == nl - 1;
+        const int s1 = coarsest ? (inv == nullptr && coarseSweeps >= 1) : (pre >= 1);
+        // residual of each fine cell into the free x buffer of level q ...
+        {
+            const T* x = X(F, q);
+            T* res = Y(F, q);
+            const T* AF = sT + F.AF;
+            const int* nbs = sI + F.nb;
+            if (t < F.n) {
+                TRow<T> r;
+                loadRow(r, sT + F.AC, sT + F.b, AF, sI + F.start, nbs, t);
+                const T Ax = rowAx(r, x[t], [&](int j) { return x[j]; }, AF, nbs);
+                res[t] = r.b - Ax;
             }
-            c.b[i] = acc;
-            xa[l + 1 - lt][i] = T(0);
         }
         __syncthreads();
+        // ... then gathered per coarse cell in ascending fine order
+        {
+            const T* res = Y(F, q);
+            const int* mS = sI + F.mS;
+            const int* mem = sI + F.mem;
+            T* xc = X(C, q + 1);
+            if (t < C.n) {
+                const int j0 = mS[t], cnt = mS[t + 1] - j0;
+                T acc = T(0);
+                if (cnt <= kMemberK) {
+                    T rv[kMemberK];
+#pragma unroll
+                    for (int j = 0; j < kMemberK; j++) rv[j] = j < cnt ? res[mem[j0 + j]] : T(0);
+#pragma unroll
+                    for (int j = 0; j < kMemberK; j++)
+                        if (j < cnt) acc += rv[j];
+                } else {
+                    for (int j = j0; j < j0 + cnt; j++) acc += res[mem[j]];
+                }
+                sT[C.b + t] = acc;
+                xc[t] = s1 ? sweepFromZero(acc, sT[C.AC + t], w) : T(0);
+            }
+        }
+        __syncthreads();
+        done = s1;
     }
-    // coarsest
+
+    // ---- coarsest
     {
-        const DLevel& D = Ls[L - 1];
+        const int q = nl - 1;
+        const TailLevel S = tl[q];
         if (inv != nullptr) {
-            const LV<T> v = view<T>(D);
-            for (int r = t; r < D.n; r += T_) {
+            const T* b = sT + S.b;
+            T* x = X(S, q);
+            if (t < S.n) {
                 double acc = 0.0;
-                for (int c = 0; c < D.n; c++) acc += inv[r * D.n + c] * (double)v.b[c];
-                xa[L - 1 - lt][r] = (T)acc;
+                for (int c = 0; c < S.n; c++) acc += inv[t * S.n + c] * (double)b[c];
+                x[t] = (T)acc;
             }
-            __syncthreads();
-        } else if (stage) {
-            // coarsest level staged in LDS for its sweeps: operator, CSR and the
-            // x ping-pong never leave the CU
-            extern __shared__ unsigned char smem[];
-            const LV<T> v = view<T>(D);
-            const int nC = D.n, rC = D.start[nC];
-            T* sx = (T*)smem;
-            T* sy = sx + nC;
-            T* sAC = sy + nC;
-            T* sb = sAC + nC;
-            T* sAF = sb + nC;
-            int* sst = (int*)(sAF + rC);
-            int* snb = sst + nC + 1;
-            const T* x0 = xa[L - 1 - lt];
-            for (int n = t; n < nC; n += T_) {
-                sx[n] = x0[n];
-                sAC[n] = v.AC[n];
-                sb[n] = v.b[n];
-            }
-            for (int n = t; n <= nC; n += T_) sst[n] = D.start[n];
-            for (int k = t; k < rC; k += T_) {
-                sAF[k] = v.AF[k];
-                snb[k] = D.nb[k];
-            }
-            __syncthreads();
-            for (int it = 0; it < coarseSweeps; it++) {
-                for (int n = t; n < nC; n += T_) {
-                    const T AC = sAC[n];
-                    T r;
-                    if (tabs(AC) < T(1.0e-30)) {
-                        r = sx[n];
-                    } else {
-                        T Ax = AC * sx[n];
-                        for (int k = sst[n]; k < sst[n + 1]; k++) {
-                            const int nb = snb[k];
-                            if (nb != -1) Ax += sAF[k] * sx[nb];
-                        }
-                        r = sx[n] + w * (sb[n] - Ax) / AC;
-                    }
-                    sy[n] = r;
-                }
-                __syncthreads();
-                T* tmp = sx;
-                sx = sy;
-                sy = tmp;
-            }
-            T* xo = xa[L - 1 - lt];
-            for (int n = t; n < nC; n += T_) xo[n] = sx[n];
             __syncthreads();
         } else {
-            sweeps(L - 1, coarseSweeps);
+            sweeps(q, coarseSweeps - done);
         }
     }
-    // ascent
-    for (int l = L - 2; l >= lt; l--) {
-        const DLevel& F = Ls[l];
-        const LV<T> f = view<T>(F);
-        const T* xc = xa[l + 1 - lt];
+
+    // ---- ascent: buildProlongationKernel fused into the first post sweep
+    for (int q = nl - 2; q >= 0; q--) {
+        const TailLevel F = tl[q];
+        const TailLevel C = tl[q + 1];
+        const T* xc = X(C, q + 1);
+        const int* toC = sI + F.toC;
         if (post >= 1) {
-            const T* x = xa[l - lt];
-            T* y = xb[l - lt];
-            for (int n = t; n < F.n; n += T_) {
-                const T xn = x[n] + xc[F.toCoarse[n]];
-                const T AC = f.AC[n];
-                if (tabs(AC) < T(1.0e-30)) {
-                    y[n] = xn;
-                    continue;
+            const T* x = X(F, q);
+            T* y = Y(F, q);
+            const T* AF = sT + F.AF;
+            const int* nbs = sI + F.nb;
+            if (t < F.n) {
+                TRow<T> r;
+                loadRow(r, sT + F.AC, sT + F.b, AF, sI + F.start, nbs, t);
+                const T xn = x[t] + xc[toC[t]];
+                if (tabs(r.ac) < T(1.0e-30)) {
+                    y[t] = xn;
+                } else {
+                    const T Ax = rowAx(r, xn, [&](int j) { return x[j] + xc[toC[j]]; }, AF, nbs);
+                    y[t] = xn + w * (r.b - Ax) / r.ac;
                 }
-                T Ax = AC * xn;
-                for (int k = F.start[n]; k < F.start[n + 1]; k++) {
-                    const int nb = F.nb[k];
-                    if (nb != -1) Ax += f.AF[k] * (x[nb] + xc[F.toCoarse[nb]]);
-                }
-                y[n] = xn + w * (f.b[n] - Ax) / AC;
             }
             __syncthreads();
-            T* tmp = xa[l - lt];
-            xa[l - lt] = xb[l - lt];
-            xb[l - lt] = tmp;
-            sweeps(l, post - 1);
+            flip ^= 1u << q;
+            sweeps(q, post - 1);
         } else {
-            T* x = xa[l - lt];
-            for (int n = t; n < F.n; n += T_) x[n] += xc[F.toCoarse[n]];
+            T* x = X(F, q);
+            if (t < F.n) x[t] += xc[toC[t]];
             __syncthreads();
         }
     }
-    // leave level lt's answer in its own x buffer
-    const LV<T> top = view<T>(Ls[lt]);
-    if (xa[0] != top.x)
-        for (int n = t; n < Ls[lt].n; n += T_) top.x[n] = xa[0][n];
+
+    // ---- level lt's answer back to its own x buffer
+    const TailLevel top = tl[0];
+    const T* x = X(top, 0);
+    T* out = view<T>(Ls[lt]).x;
+    if (t < top.n) out[t] = x[t];
 }
 
 // ===========================================================================
 // launchers
 // ===========================================================================
-void launchMGTail(hipStream_t s, const DLevel* levels, int lt, int L, double w, int pre, int post, int coarseSweeps,
-                  bool fp32, const double* inv, size_t stageBytes, const Ctl* ctl) {
-    const int stage = stageBytes > 0 ? 1 : 0;
-    if (fp32) kMGTail<float><<<1, 1024, stageBytes, s>>>(levels, lt, L, w, pre, post, coarseSweeps, inv, stage, ctl);
-    else kMGTail<double><<<1, 1024, stageBytes, s>>>(levels, lt, L, w, pre, post, coarseSweeps, inv, stage, ctl);
+int tailLdsLimit() {
+    int dev = 0, limit = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 64 * 1024;
+    if (hipDeviceGetAttribute(&limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || limit <= 0)
+        return 64 * 1024;
+    if (limit > 64 * 1024) {
+        // dynamic LDS beyond 64 KiB has to be opted into per kernel; the
+        // kernel's static LDS comes out of the same budget
+        const int dyn = limit - kTailStaticLds;
+        if (hipFuncSetAttribute((const void*)kMGTail<double>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn) !=
+                hipSuccess ||
+            hipFuncSetAttribute((const void*)kMGTail<float>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn) !=
+                hipSuccess) {
+            (void)hipGetLastError();
+            return 64 * 1024;
+        }
+    }
+    return limit;
+}
+void launchMGTail(hipStream_t s, const DLevel* levels, const TailLaunch& t, double w, int preDone, int pre, int post,
+                  int coarseSweeps, bool fp32, const double* inv, const Ctl* ctl) {
+    if (fp32)
+        kMGTail<float><<<1, t.threads, t.ldsBytes, s>>>(levels, t.plan, t.copies, t.ncopy, t.totalWords, t.lt, t.nl,
+                                                        t.intBase, w, preDone, pre, post, coarseSweeps, inv, ctl);
+    else
+        kMGTail<double><<<1, t.threads, t.ldsBytes, s>>>(levels, t.plan, t.copies, t.ncopy, t.totalWords, t.lt, t.nl,
+                                                         t.intBase, w, preDone, pre, post, coarseSweeps, inv, ctl);
 }
 void launchMGCoarseDiag(hipStream_t s, const DLevel& f, const Ctl* ctl) {
     kMGCoarseDiag<<<blocksFor(f.n), kBlock, 0, s>>>(f, ctl);
@@ -1138,8 +1220,10 @@ void launchMGCoarseGather(hipStream_t s, const DLevel& f, const DLevel& c, const
 void launchMGResidual(hipStream_t s, const DLevel& f, const Ctl* ctl) {
     kMGResidual<<<blocksFor(f.n), kBlock, 0, s>>>(f, ctl);
 }
-void launchMGRestrictSum(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl) {
-    kMGRestrictSum<<<blocksFor(c.n), kBlock, 0, s>>>(f, c, ctl);
+void launchMGRestrictSum(hipStream_t s, const DLevel& f, const DLevel& c, double w, int sweep1, bool fp32,
+                         const Ctl* ctl) {
+    if (fp32) kMGRestrictSum<float><<<blocksFor(c.n), kBlock, 0, s>>>(f, c, w, sweep1, ctl);
+    else kMGRestrictSum<double><<<blocksFor(c.n), kBlock, 0, s>>>(f, c, w, sweep1, ctl);
 }
 void launchGradUV(hipStream_t s, const DMesh& m, DBC bu, DBC bv, DState st, int scheme) {
     kGradUV<<<blocksFor(m.N), kBlock, 0, s>>>(m, bu, bv, st, scheme);
@@ -1153,7 +1237,11 @@ void launchMomentum(hipStream_t s, const DMesh& m, DBC bu, DBC bv, DBC bp, DStat
 }
 void launchJacobiUV(hipStream_t s, const DMesh& m, DState st, const double* uo, const double* vo, double* un,
                     double* vn) {
-    kJacobiUV<<<blocksFor(m.N), kBlock, 0, s>>>(m, st, uo, vo, un, vn);
+    kJacobiUV<<<blocksFor(m.N), kBlock, 0, s>>>(m, st, UVIn{uo, vo, nullptr}, UVOut{un, vn, nullptr});
+}
+void launchJacobiUVPacked(hipStream_t s, const DMesh& m, DState st, const double* uo, const double* vo,
+                          const double2* uvo, double* un, double* vn, double2* uvn) {
+    kJacobiUV<<<blocksFor(m.N), kBlock, 0, s>>>(m, st, UVIn{uo, vo, uvo}, UVOut{un, vn, uvn});
 }
 void launchGSColorUV(hipStream_t s, const DMesh& m, DState st, const int* order, int begin, int count) {
     kGSColorUV<<<blocksFor(count), kBlock, 0, s>>>(m, st, order, begin, count);
@@ -1194,18 +1282,12 @@ void launchCopy(hipStream_t s, double* dst, const double* src, int n, const Ctl*
 void launchFill(hipStream_t s, double* dst, double value, int n, const Ctl* ctl) {
     kFill<<<blocksFor(n), kBlock, 0, s>>>(dst, value, n, ctl);
 }
-void launchMGCoarseOp(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl) {
-    kMGCoarseOp<<<blocksFor(c.n), kBlock, 0, s>>>(f, c, ctl);
-}
 void launchMGJacobi(hipStream_t s, const DLevel& L, double w, const Ctl* ctl) {
     kMGJacobi<<<blocksFor(L.n), kBlock, 0, s>>>(L, w, ctl);
 }
 void launchMGSmoothBlock(hipStream_t s, const DLevel& L, double w, int sweeps, const Ctl* ctl) {
     const int threads = L.n >= 1024 ? 1024 : ((L.n + 63) / 64) * 64;
     kMGSmoothBlock<<<1, threads, 2 * (size_t)L.n * sizeof(double), s>>>(L, w, sweeps, ctl);
-}
-void launchMGRestrict(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl) {
-    kMGRestrict<<<blocksFor(c.n), kBlock, 0, s>>>(f, c, ctl);
 }
 void launchMGProlongSmooth(hipStream_t s, const DLevel& f, const DLevel& c, double w, const Ctl* ctl) {
     kMGProlongSmooth<<<blocksFor(f.n), kBlock, 0, s>>>(f, c, w, ctl);

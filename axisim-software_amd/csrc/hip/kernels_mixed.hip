@@ -46,56 +46,47 @@ __global__ void __launch_bounds__(kBlock) kMomDefect(DMesh m, DState s, MixedSta
         }
     }
     const float ru = (float)(s.ub[n] - au), rv = (float)(s.vb[n] - av);
-    x.ru[n] = ru;
-    x.rv[n] = rv;
-    x.du[n] = x.invU[n] * ru;
-    x.dv[n] = x.invV[n] * rv;
+    const float2 iv = x.inv[n];
+    x.r[n] = make_float2(ru, rv);
+    x.d[n] = make_float2(iv.x * ru, iv.y * rv);
 }
 
-// one FP32 Jacobi sweep of the defect equations for u and v
-__global__ void __launch_bounds__(kBlock) kMomSweep32(DMesh m, const Ctl* ctl, MixedState x, const float* __restrict__ uo,
-                                                     const float* __restrict__ vo, float* __restrict__ un,
-                                                     float* __restrict__ vn) {
-    if (finished(ctl)) return;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= m.N) return;
-    float au = x.ru[n], av = x.rv[n];
+// one FP32 Jacobi sweep of the (u, v) defect equations
+__device__ __forceinline__ float2 momSweep32(const DMesh& m, const MixedState& x, const float2* __restrict__ dO,
+                                             int n) {
+    const float2 r = x.r[n];
+    float au = r.x, av = r.y;
     for (int k = m.faceStart[n]; k < m.faceStart[n + 1]; k++) {
         const int nb = m.slotNb[k];
         if (nb >= 0) {
             const float a = x.AF32[k];
-            au -= a * uo[nb];
-            av -= a * vo[nb];
+            const float2 d = dO[nb];
+            au -= a * d.x;
+            av -= a * d.y;
         }
     }
-    const float iu = x.invU[n], iv = x.invV[n];
-    un[n] = (iu != 0.0f) ? au * iu : uo[n];
-    vn[n] = (iv != 0.0f) ? av * iv : vo[n];
+    const float2 iv = x.inv[n];
+    const float2 d = dO[n];
+    return make_float2((iv.x != 0.0f) ? au * iv.x : d.x, (iv.y != 0.0f) ? av * iv.y : d.y);
+}
+
+__global__ void __launch_bounds__(kBlock) kMomSweep32(DMesh m, const Ctl* ctl, MixedState x,
+                                                     const float2* __restrict__ dO, float2* __restrict__ dN) {
+    if (finished(ctl)) return;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= m.N) return;
+    dN[n] = momSweep32(m, x, dO, n);
 }
 
 // last FP32 sweep fused with u += delta, v += delta (FP64)
-__global__ void __launch_bounds__(kBlock) kMomSweepApply(DMesh m, DState s, MixedState x, const float* __restrict__ uo,
-                                                        const float* __restrict__ vo, int sweep) {
+__global__ void __launch_bounds__(kBlock) kMomSweepApply(DMesh m, DState s, MixedState x,
+                                                        const float2* __restrict__ dO, int sweep) {
     if (finished(s.ctl)) return;
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= m.N) return;
-    float du = uo[n], dv = vo[n];
-    if (sweep) {
-        float au = x.ru[n], av = x.rv[n];
-        for (int k = m.faceStart[n]; k < m.faceStart[n + 1]; k++) {
-            const int nb = m.slotNb[k];
-            if (nb >= 0) {
-                const float a = x.AF32[k];
-                au -= a * uo[nb];
-                av -= a * vo[nb];
-            }
-        }
-        const float iu = x.invU[n], iv = x.invV[n];
-        du = (iu != 0.0f) ? au * iu : uo[n];
-        dv = (iv != 0.0f) ? av * iv : vo[n];
-    }
-    s.u[n] += (double)du;
-    s.v[n] += (double)dv;
+    const float2 d = sweep ? momSweep32(m, x, dO, n) : dO[n];
+    s.u[n] += (double)d.x;
+    s.v[n] += (double)d.y;
 }
 
 // ---------------------------------------------------------------------------
@@ -117,7 +108,7 @@ __global__ void __launch_bounds__(kBlock) kMGDefect(DLevel L, double w, int firs
     L.b32[n] = r;
     if (firstSweep) {
         const float ac = L.AC32[n];
-        L.xNew32[n] = (fabsf(ac) < 1.0e-30f) ? 0.0f : (float)w * r / ac;
+        L.x32[n] = (fabsf(ac) < 1.0e-30f) ? 0.0f : (float)w * r / ac;
     } else {
         L.x32[n] = 0.0f;
     }
@@ -172,24 +163,6 @@ __global__ void __launch_bounds__(1024) kMGSmoothBlock32(DLevel L, float w, int 
     for (int n = threadIdx.x; n < L.n; n += blockDim.x) L.x32[n] = xa[n];
 }
 
-__global__ void __launch_bounds__(kBlock) kMGRestrict32(DLevel f, DLevel c, const Ctl* ctl) {
-    if (finished(ctl)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= c.n) return;
-    float acc = 0.0f;
-    for (int j = f.memberStart[i]; j < f.memberStart[i + 1]; j++) {
-        const int n = f.members[j];
-        float Ax = f.AC32[n] * f.x32[n];
-        for (int k = f.start[n]; k < f.start[n + 1]; k++) {
-            const int nb = f.nb[k];
-            if (nb >= 0) Ax += f.AF32[k] * f.x32[nb];
-        }
-        acc += f.b32[n] - Ax;
-    }
-    c.b32[i] = acc;
-    c.x32[i] = 0.0f;
-}
-
 __global__ void __launch_bounds__(kBlock) kMGResidual32(DLevel f, const Ctl* ctl) {
     if (finished(ctl)) return;
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -200,16 +173,6 @@ __global__ void __launch_bounds__(kBlock) kMGResidual32(DLevel f, const Ctl* ctl
         if (nb >= 0) Ax += f.AF32[k] * f.x32[nb];
     }
     f.res32[n] = f.b32[n] - Ax;
-}
-
-__global__ void __launch_bounds__(kBlock) kMGRestrictSum32(DLevel f, DLevel c, const Ctl* ctl) {
-    if (finished(ctl)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= c.n) return;
-    float acc = 0.0f;
-    for (int j = f.memberStart[i]; j < f.memberStart[i + 1]; j++) acc += f.res32[f.members[j]];
-    c.b32[i] = acc;
-    c.x32[i] = 0.0f;
 }
 
 __global__ void __launch_bounds__(kBlock) kMGProlongSmooth32(DLevel f, DLevel c, float w, const Ctl* ctl) {
@@ -252,31 +215,6 @@ __global__ void __launch_bounds__(kBlock) kMGCast(DLevel L, const Ctl* ctl) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n < L.n) L.AC32[n] = (float)L.AC[n];
     for (int k = n; k < L.refs; k += gridDim.x * blockDim.x) L.AF32[k] = (float)L.AF[k];
-}
-
-// FP64 Galerkin product (same gather as kMGCoarseOp) + FP32 copy of the result
-__global__ void __launch_bounds__(kBlock) kMGCoarseOp64(DLevel f, DLevel c, const Ctl* ctl) {
-    if (finished(ctl)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= c.n) return;
-    double acc = 0.0;
-    for (int j = f.memberStart[i]; j < f.memberStart[i + 1]; j++) {
-        const int n = f.members[j];
-        double diag = f.AC[n];
-        for (int k = f.start[n]; k < f.start[n + 1]; k++) {
-            if (f.nb[k] < 0) continue;
-            if (f.slotToCoarseSlot[k] < 0) diag += f.AF[k];
-        }
-        acc += diag;
-    }
-    c.AC[i] = acc;
-    c.AC32[i] = (float)acc;
-    for (int q = c.start[i]; q < c.start[i + 1]; q++) {
-        double a = 0.0;
-        for (int j = f.cslotStart[q]; j < f.cslotStart[q + 1]; j++) a += f.AF[f.cslotFine[j]];
-        c.AF[q] = a;
-        c.AF32[q] = (float)a;
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -372,13 +310,11 @@ __global__ void __launch_bounds__(kBlock) kCoarseApply(DLevel L, const double* i
 void launchMomDefect(hipStream_t s, const DMesh& m, DState st, MixedState x) {
     kMomDefect<<<blocksFor(m.N), kBlock, 0, s>>>(m, st, x);
 }
-void launchMomSweep32(hipStream_t s, const DMesh& m, const Ctl* ctl, MixedState x, const float* uo, const float* vo,
-                      float* un, float* vn) {
-    kMomSweep32<<<blocksFor(m.N), kBlock, 0, s>>>(m, ctl, x, uo, vo, un, vn);
+void launchMomSweep32(hipStream_t s, const DMesh& m, const Ctl* ctl, MixedState x, const float2* dO, float2* dN) {
+    kMomSweep32<<<blocksFor(m.N), kBlock, 0, s>>>(m, ctl, x, dO, dN);
 }
-void launchMomSweepApply(hipStream_t s, const DMesh& m, DState st, MixedState x, const float* uo, const float* vo,
-                         int sweep) {
-    kMomSweepApply<<<blocksFor(m.N), kBlock, 0, s>>>(m, st, x, uo, vo, sweep);
+void launchMomSweepApply(hipStream_t s, const DMesh& m, DState st, MixedState x, const float2* dO, int sweep) {
+    kMomSweepApply<<<blocksFor(m.N), kBlock, 0, s>>>(m, st, x, dO, sweep);
 }
 void launchMGDefect(hipStream_t s, const DLevel& L, double w, int firstSweep, const Ctl* ctl) {
     kMGDefect<<<blocksFor(L.n), kBlock, 0, s>>>(L, w, firstSweep, ctl);
@@ -390,14 +326,8 @@ void launchMGSmoothBlock32(hipStream_t s, const DLevel& L, float w, int sweeps, 
     const int threads = L.n >= 1024 ? 1024 : ((L.n + 63) / 64) * 64;
     kMGSmoothBlock32<<<1, threads, 2 * (size_t)L.n * sizeof(float), s>>>(L, w, sweeps, ctl);
 }
-void launchMGRestrict32(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl) {
-    kMGRestrict32<<<blocksFor(c.n), kBlock, 0, s>>>(f, c, ctl);
-}
 void launchMGResidual32(hipStream_t s, const DLevel& f, const Ctl* ctl) {
     kMGResidual32<<<blocksFor(f.n), kBlock, 0, s>>>(f, ctl);
-}
-void launchMGRestrictSum32(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl) {
-    kMGRestrictSum32<<<blocksFor(c.n), kBlock, 0, s>>>(f, c, ctl);
 }
 void launchMGProlongSmooth32(hipStream_t s, const DLevel& f, const DLevel& c, float w, const Ctl* ctl) {
     kMGProlongSmooth32<<<blocksFor(f.n), kBlock, 0, s>>>(f, c, w, ctl);
@@ -410,9 +340,6 @@ void launchMGApply(hipStream_t s, const DLevel& L, const Ctl* ctl) {
 }
 void launchMGCast(hipStream_t s, const DLevel& L, const Ctl* ctl) {
     kMGCast<<<blocksFor(L.n), kBlock, 0, s>>>(L, ctl);
-}
-void launchMGCoarseOp64(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl) {
-    kMGCoarseOp64<<<blocksFor(c.n), kBlock, 0, s>>>(f, c, ctl);
 }
 void launchCoarseInvert(hipStream_t s, const DLevel& L, double* aug, double* inv, int* status, const Ctl* ctl) {
     kCoarseInvert<<<1, 1024, 0, s>>>(L, aug, inv, status, ctl);

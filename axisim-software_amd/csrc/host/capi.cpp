@@ -254,6 +254,15 @@ axs_status axs_mg_level_info(const axs_mg* h, int level, int* n_cells, int* n_re
     return AXS_OK;
 }
 
+axs_status axs_mg_tail_info(const axs_mg* h, int precision, int* first_level, int* lds_bytes) {
+    if (!h) return AXS_ERR_INVALID;
+    if (precision != AXS_PREC_FP64 && precision != AXS_PREC_MIXED) return AXS_ERR_INVALID;
+    const bool fp32 = precision == AXS_PREC_MIXED;
+    if (first_level) *first_level = h->mg.tailLevel(fp32);
+    if (lds_bytes) *lds_bytes = (int)h->mg.tailBytes(fp32);
+    return AXS_OK;
+}
+
 axs_status axs_mg_get_maps(const axs_mg* h, int level, int* fs, int* fn, int* c2c, int* s2s) {
     if (!h) return AXS_ERR_INVALID;
     if (level < 0 || level >= (int)h->mg.host.size()) return AXS_ERR_RANGE;

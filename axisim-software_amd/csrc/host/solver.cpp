@@ -19,8 +19,8 @@
 namespace axs {
 
 namespace {
-constexpr int kSmoothBlockMax = 2048;  // cells a single workgroup smooths in LDS
-constexpr int kTailMax = 1024;         // levels at most this small run in the tail kernel
+constexpr int kSmoothBlockMax = 2048;     // cells a single workgroup smooths in LDS
+constexpr int kTailMax = 1024;            // levels at most this small may run in the tail kernel
 }
 
 void Arena::release() {
@@ -39,7 +39,8 @@ void MGDevice::build(int n, const std::vector<int>& start, const std::vector<int
     aug = inv = nullptr;
     invStatus = nullptr;
     dLevels_ = nullptr;
-    tail_ = -1;
+    tail64_ = TailPlan{};
+    tail32_ = TailPlan{};
     dev.assign(host.size(), DLevel{});
     for (size_t l = 0; l < host.size(); l++) {
         const Level& h = host[l];
@@ -68,12 +69,127 @@ void MGDevice::build(int n, const std::vector<int>& start, const std::vector<int
     }
 }
 
-// LDS needed to stage the coarsest level in the tail kernel (0 = do not stage)
-size_t MGDevice::stageBytes(bool fp32) const {
-    const Level& c = host.back();
-    const size_t v = fp32 ? sizeof(float) : sizeof(double);
-    const size_t bytes = (4 * (size_t)c.n + c.refs()) * v + ((size_t)c.n + 1 + c.refs()) * sizeof(int);
-    return bytes <= 64 * 1024 ? bytes : 0;
+// LDS image of levels lt..L-1 for the tail kernel: values (AC, b, x ping-pong,
+// AF) first, then ints (CSR, and the transfer maps of every level but the last)
+MGDevice::TailPlan MGDevice::planTail(int lt, size_t vsize) const {
+    TailPlan t;
+    const int L = (int)host.size();
+    t.lt = lt;
+    t.nl = L - lt;
+    long tv = 0, ti = 0;
+    int maxN = 0;
+    for (int l = lt; l < L; l++) {
+        const Level& h = host[l];
+        const bool last = l == L - 1;
+        TailLevel e{};
+        e.n = h.n;
+        e.refs = h.refs();
+        e.nc = last ? 0 : host[l + 1].n;
+        e.AC = (int)tv; tv += h.n;
+        e.b = (int)tv; tv += h.n;
+        e.x0 = (int)tv; tv += h.n;
+        e.x1 = (int)tv; tv += h.n;
+        e.AF = (int)tv; tv += e.refs;
+        e.start = (int)ti; ti += h.n + 1;
+        e.nb = (int)ti; ti += e.refs;
+        if (!last) {
+            e.toC = (int)ti; ti += h.n;
+            e.mS = (int)ti; ti += e.nc + 1;
+            e.mem = (int)ti; ti += h.n;
+        }
+        t.lv.push_back(e);
+        maxN = std::max(maxN, h.n);
+    }
+    t.intBase = (int)(((size_t)tv * vsize + 15) / 16 * 16);
+    t.bytes = (size_t)t.intBase + (size_t)ti * sizeof(int);
+    t.threads = std::min(1024, std::max(64, (maxN + 63) / 64 * 64));
+    return t;
+}
+
+// The tail starts at the first level that is small enough, whose LDS image of
+// it and every coarser level fits one workgroup.  Builds the list of arrays the
+// kernel stages (device pointers of the levels: x and b of the first level, the
+// operator, CSR and transfer maps of every level).
+void MGDevice::chooseTail(TailPlan& t, size_t vsize, bool fp32) {
+    TailLevel* dev0 = t.dev;
+    TailCopy* cp0 = t.devCopies;
+    t = TailPlan{};
+    t.dev = dev0;
+    t.devCopies = cp0;
+    const int L = (int)host.size();
+    const size_t budget = ldsLimit_ > kTailStaticLds ? (size_t)(ldsLimit_ - kTailStaticLds) : 0;
+    for (int l = 0; l < L; l++) {
+        if (host[l].n > kTailMax || L - l > kTailLevels) continue;
+        TailPlan c = planTail(l, vsize);
+        if (c.bytes > budget) continue;
+        c.dev = dev0;
+        c.devCopies = cp0;
+        t = c;
+        break;
+    }
+    if (t.lt < 0) return;
+    std::vector<TailCopy> cps;
+    int words = 0;
+    const int tw = (int)(vsize / 4), base = t.intBase / 4;
+    auto add = [&](const void* src, int dstWord, long count) {
+        if (count <= 0) return;
+        if (!src) throw std::logic_error("multigrid tail: level buffer not allocated");
+        cps.push_back(TailCopy{src, dstWord, words, (int)count});
+        words += (int)count;
+    };
+    for (int q = 0; q < t.nl; q++) {
+        const DLevel& d = dev[t.lt + q];
+        const TailLevel& e = t.lv[q];
+        const bool last = q + 1 == t.nl;
+        add(fp32 ? (const void*)d.AC32 : (const void*)d.AC, e.AC * tw, (long)e.n * tw);
+        if (q == 0) {
+            add(fp32 ? (const void*)d.b32 : (const void*)d.b, e.b * tw, (long)e.n * tw);
+            add(fp32 ? (const void*)d.x32 : (const void*)d.x, e.x0 * tw, (long)e.n * tw);
+        }
+        add(fp32 ? (const void*)d.AF32 : (const void*)d.AF, e.AF * tw, (long)e.refs * tw);
+        add(d.start, base + e.start, e.n + 1);
+        add(d.nb, base + e.nb, e.refs);
+        if (!last) {
+            add(d.toCoarse, base + e.toC, e.n);
+            add(d.memberStart, base + e.mS, e.nc + 1);
+            add(d.members, base + e.mem, e.n);
+        }
+    }
+    if ((int)cps.size() > kTailCopies) throw std::logic_error("multigrid tail: too many staged arrays");
+    t.ncopy = (int)cps.size();
+    t.totalWords = words;
+    if (!t.dev) t.dev = arena.alloc<TailLevel>(kTailLevels);
+    if (!t.devCopies) t.devCopies = arena.alloc<TailCopy>(kTailCopies);
+    AXS_HIP(hipMemcpy(t.dev, t.lv.data(), t.lv.size() * sizeof(TailLevel), hipMemcpyHostToDevice));
+    AXS_HIP(hipMemcpy(t.devCopies, cps.data(), cps.size() * sizeof(TailCopy), hipMemcpyHostToDevice));
+}
+
+TailLaunch MGDevice::TailPlan::launch() const {
+    TailLaunch l;
+    l.plan = dev;
+    l.copies = devCopies;
+    l.ncopy = ncopy;
+    l.totalWords = totalWords;
+    l.lt = lt;
+    l.nl = nl;
+    l.intBase = intBase;
+    l.threads = threads;
+    l.ldsBytes = bytes;
+    return l;
+}
+
+// bytes the tail kernel moves between HBM and LDS: its staging reads and the
+// answer written back
+double MGDevice::tailTrafficBytes(const TailPlan& t, size_t vsize) const {
+    if (t.lt < 0) return 0.0;
+    double b = 0.0;
+    for (int q = 0; q < t.nl; q++) {
+        const TailLevel& e = t.lv[q];
+        b += (double)vsize * e.n + 4.0 * (e.n + 1) + (vsize + 4.0) * e.refs;  // AC, start, AF + nb
+        if (q + 1 < t.nl) b += 8.0 * e.n + 4.0 * (e.nc + 1);                   // toCoarse, members, memberStart
+        if (q == 0) b += 3.0 * vsize * e.n;                                      // b, x in; x out
+    }
+    return b;
 }
 
 void MGDevice::bindLevel0(double* AC, double* b, double* AF, double* x, double* xNew) {
@@ -87,16 +203,15 @@ void MGDevice::bindLevel0(double* AC, double* b, double* AF, double* x, double* 
     syncLevels();
 }
 
+// The tail kernel reads the level pointers from the device copy made here; the
+// host never swaps the x buffers of a tail level (only the tail smooths them),
+// so that copy stays valid for them.
 void MGDevice::syncLevels() {
-    tail_ = -1;
-    for (int l = 0; l < (int)host.size(); l++)
-        if (host[l].n <= kTailMax) {
-            tail_ = l;
-            break;
-        }
-    if (tail_ >= 0 && (int)host.size() - tail_ > 32) tail_ = -1;  // the kernel tracks at most 32 levels
     if (!dLevels_) dLevels_ = arena.alloc<DLevel>(host.size());
     AXS_HIP(hipMemcpy(dLevels_, dev.data(), dev.size() * sizeof(DLevel), hipMemcpyHostToDevice));
+    if (ldsLimit_ == 0) ldsLimit_ = tailLdsLimit();
+    chooseTail(tail64_, sizeof(double), false);
+    if (mixedReady) chooseTail(tail32_, sizeof(float), true);
 }
 
 void MGDevice::swapX(int l) { std::swap(dev[l].x, dev[l].xNew); }
@@ -115,21 +230,31 @@ void MGDevice::smooth(hipStream_t s, int l, int sweeps, const MGParams& p, const
     }
 }
 
-// vCycle, multigrid.cu:607-662
-void MGDevice::vcycle(hipStream_t s, int l, const MGParams& p, const Ctl* ctl) {
-    if (l == tail_) {
-        launchMGTail(s, dLevels_, l, (int)host.size(), p.weight, p.pre, p.post, p.coarseSweeps, false, nullptr,
-                     stageBytes(false), ctl);
+// computeResidual + buildRestrictionKernel + the coarse x memset
+// (multigrid.cu:641-655), with the coarse level's first sweep from that zero
+// folded in when sweep1 is set.
+void MGDevice::restrictTo(hipStream_t s, int l, int sweep1, bool fp32, const MGParams& p, const Ctl* ctl) {
+    if (fp32) launchMGResidual32(s, dev[l], ctl);
+    else launchMGResidual(s, dev[l], ctl);
+    launchMGRestrictSum(s, dev[l], dev[l + 1], p.weight, sweep1, fp32, ctl);
+}
+
+// vCycle, multigrid.cu:607-662.  preDone: pre-smoothing sweeps of level l
+// already applied (by the restriction above it).
+void MGDevice::vcycle(hipStream_t s, int l, int preDone, const MGParams& p, const Ctl* ctl) {
+    if (l == tail64_.lt) {
+        launchMGTail(s, dLevels_, tail64_.launch(), p.weight, preDone, p.pre, p.post, p.coarseSweeps, false, nullptr,
+                     ctl);
         return;
     }
     if (host[l].coarsest()) {
-        smooth(s, l, p.coarseSweeps, p, ctl);
+        smooth(s, l, p.coarseSweeps - preDone, p, ctl);
         return;
     }
-    smooth(s, l, p.pre, p, ctl);
-    launchMGResidual(s, dev[l], ctl);
-    launchMGRestrictSum(s, dev[l], dev[l + 1], ctl);
-    vcycle(s, l + 1, p, ctl);
+    smooth(s, l, p.pre - preDone, p, ctl);
+    const int s1 = host[l + 1].coarsest() ? (p.coarseSweeps >= 1) : (p.pre >= 1);
+    restrictTo(s, l, s1, false, p, ctl);
+    vcycle(s, l + 1, s1, p, ctl);
     if (p.post >= 1) {
         launchMGProlongSmooth(s, dev[l], dev[l + 1], p.weight, ctl);
         swapX(l);
@@ -176,22 +301,21 @@ void MGDevice::smooth32(hipStream_t s, int l, int sweeps, const MGParams& p, con
     }
 }
 
-void MGDevice::vcycle32(hipStream_t s, int l, bool skipPre, const MGParams& p, const Ctl* ctl) {
-    if (l == tail_ && l > 0) {
-        launchMGTail(s, dLevels_, l, (int)host.size(), p.weight, p.pre, p.post, p.coarseSweeps, true,
-                     p.coarseDirect ? inv : nullptr, p.coarseDirect ? 0 : stageBytes(true), ctl);
+void MGDevice::vcycle32(hipStream_t s, int l, int preDone, const MGParams& p, const Ctl* ctl) {
+    if (l == tail32_.lt) {
+        launchMGTail(s, dLevels_, tail32_.launch(), p.weight, preDone, p.pre, p.post, p.coarseSweeps, true,
+                     p.coarseDirect ? inv : nullptr, ctl);
         return;
     }
     if (host[l].coarsest()) {
         if (p.coarseDirect) launchCoarseApply(s, dev[l], inv, ctl);
-        else smooth32(s, l, skipPre ? p.coarseSweeps - 1 : p.coarseSweeps, p, ctl);
+        else smooth32(s, l, p.coarseSweeps - preDone, p, ctl);
         return;
     }
-    if (!skipPre) smooth32(s, l, p.pre, p, ctl);
-    else smooth32(s, l, p.pre - 1, p, ctl);
-    launchMGResidual32(s, dev[l], ctl);
-    launchMGRestrictSum32(s, dev[l], dev[l + 1], ctl);
-    vcycle32(s, l + 1, false, p, ctl);
+    smooth32(s, l, p.pre - preDone, p, ctl);
+    const int s1 = host[l + 1].coarsest() ? (!p.coarseDirect && p.coarseSweeps >= 1) : (p.pre >= 1);
+    restrictTo(s, l, s1, true, p, ctl);
+    vcycle32(s, l + 1, s1, p, ctl);
     if (p.post >= 1) {
         launchMGProlongSmooth32(s, dev[l], dev[l + 1], (float)p.weight, ctl);
         swapX32(l);
@@ -210,21 +334,26 @@ void MGDevice::enqueueSolveMixed(hipStream_t s, const MGParams& p, const Ctl* ct
         throw std::domain_error("coarsest level too large for the dense FP64 solve");
     if (buildOperators) {
         launchMGCast(s, dev[0], ctl);
-        for (int l = 1; l < L; l++) {
-            launchMGCoarseDiag(s, dev[l - 1], ctl);
-            launchMGCoarseGather(s, dev[l - 1], dev[l], ctl);  // also writes the FP32 copies
-        }
+        this->buildOperators(s, ctl);  // also writes the FP32 copies
         if (p.coarseDirect) launchCoarseInvert(s, dev[L - 1], aug, inv, invStatus, ctl);
     }
     const int cycles = p.cycles > 1 ? p.cycles : 1;
     // the first sweep from e = 0 is fused into the defect kernel
     const bool single = L == 1;
-    const bool first = single ? (!p.coarseDirect && p.coarseSweeps >= 1) : (p.pre >= 1);
+    const int first = (single ? (!p.coarseDirect && p.coarseSweeps >= 1) : (p.pre >= 1)) ? 1 : 0;
     for (int c = 0; c < cycles; c++) {
-        launchMGDefect(s, dev[0], p.weight, first ? 1 : 0, ctl);
-        if (first) swapX32(0);
+        launchMGDefect(s, dev[0], p.weight, first, ctl);
         vcycle32(s, 0, first, p, ctl);
         launchMGApply(s, dev[0], ctl);
+    }
+}
+
+// Galerkin coarse operators of every level (buildCoarseOperator,
+// multigrid.cu:350-389), each in two coalesced passes.
+void MGDevice::buildOperators(hipStream_t s, const Ctl* ctl) {
+    for (size_t l = 1; l < dev.size(); l++) {
+        launchMGCoarseDiag(s, dev[l - 1], ctl);
+        launchMGCoarseGather(s, dev[l - 1], dev[l], ctl);
     }
 }
 
@@ -234,13 +363,9 @@ void MGDevice::enqueueSolve(hipStream_t s, const MGParams& p, const Ctl* ctl, bo
         return;
     }
     if (p.coarseDirect) throw std::domain_error("the FP64 coarsest direct solve is built for mixed precision only");
-    if (buildOperators)
-        for (size_t l = 1; l < dev.size(); l++) {
-            launchMGCoarseDiag(s, dev[l - 1], ctl);
-            launchMGCoarseGather(s, dev[l - 1], dev[l], ctl);
-        }
+    if (buildOperators) this->buildOperators(s, ctl);
     const int cycles = p.cycles > 1 ? p.cycles : 1;  // cycleCount, multigrid.cuh:226
-    for (int c = 0; c < cycles; c++) vcycle(s, 0, p, ctl);
+    for (int c = 0; c < cycles; c++) vcycle(s, 0, 0, p, ctl);
     if (dev[0].x != x0Home_) {
         launchCopy(s, x0Home_, dev[0].x, dev[0].n, ctl);
         dev[0].x = x0Home_;
@@ -249,68 +374,62 @@ void MGDevice::enqueueSolve(hipStream_t s, const MGParams& p, const Ctl* ctl, bo
 }
 
 // Algorithmic bytes of one solve (distinct elements read + written, FP64 / int32).
+// Per launch: distinct elements read + written, values of size v (8 or 4),
+// int32 indices 4 B.  Mirrors the launches enqueueSolve makes.
 double MGDevice::bytesPerSolve(const MGParams& p, bool buildOperators) const {
-    if (p.precision == AXS_PREC_MIXED) {
-        const int L = (int)host.size();
-        double bytes = 0.0;
-        if (buildOperators) {
-            bytes += 12.0 * host[0].n + 12.0 * host[0].refs();  // cast level 0
-            for (int l = 1; l < L; l++) {
-                const Level& f = host[l - 1];
-                const Level& c = host[l];
-                bytes += 16.0 * f.n + 16.0 * f.refs() + 4.0 * f.cslotFine.size() + 20.0 * c.n + 16.0 * c.refs();
-            }
-        }
-        auto jac = [&](int l) { return 20.0 * host[l].n + 8.0 * host[l].refs(); };
-        auto sm = [&](int l, int sweeps) {
-            if (sweeps <= 0) return 0.0;
-            if (sweeps > 1 && host[l].n <= kSmoothBlockMax) return 16.0 * host[l].n + 8.0 * host[l].refs();
-            return sweeps * jac(l);
-        };
-        double cyc = 32.0 * host[0].n + 12.0 * host[0].refs() + 8.0 * host[0].n;  // FP64 defect (+ first sweep)
-        for (int l = 0; l < L; l++) {
-            const int pre = (l == 0) ? p.pre - 1 : p.pre;
-            if (host[l].coarsest()) {
-                cyc += p.coarseDirect ? 8.0 * host[l].n * host[l].n : sm(l, p.coarseSweeps);
-                break;
-            }
-            const double nc = host[l + 1].n;
-            cyc += sm(l, pre) + 16.0 * host[l].n + 8.0 * host[l].refs() + 12.0 * nc;
-            cyc += (p.post >= 1) ? 20.0 * host[l].n + 8.0 * host[l].refs() + 4.0 * nc + sm(l, p.post - 1)
-                                 : 12.0 * host[l].n + 4.0 * nc;
-        }
-        cyc += 20.0 * host[0].n;  // x += e
-        return bytes + (p.cycles > 1 ? p.cycles : 1) * cyc;
-    }
-    double bytes = 0.0;
+    const bool mixed = p.precision == AXS_PREC_MIXED;
+    const double v = mixed ? 4.0 : 8.0;
     const int L = (int)host.size();
-    auto jac = [&](int l) { return 36.0 * host[l].n + 12.0 * host[l].refs(); };
-    auto smoothBytes = [&](int l, int sweeps) {
-        if (sweeps <= 0) return 0.0;
-        if (sweeps > 1 && host[l].n <= kSmoothBlockMax) return 32.0 * host[l].n + 12.0 * host[l].refs();
-        return sweeps * jac(l);
-    };
-    if (buildOperators)
+    const TailPlan& tail = mixed ? tail32_ : tail64_;
+    double bytes = 0.0;
+    if (buildOperators) {
+        if (mixed) bytes += 12.0 * host[0].n + 12.0 * host[0].refs();  // FP32 cast of level 0
+        const double w32 = mixed ? 4.0 : 0.0;                         // FP32 copies written
         for (int l = 1; l < L; l++) {
             const Level& f = host[l - 1];
             const Level& c = host[l];
-            // fine diagonals (20N + 16S) + ordered gathers (12N + 12X + 12S_c + 12N_c)
-            bytes += 32.0 * f.n + 16.0 * f.refs() + 12.0 * f.cslotFine.size() + 12.0 * c.n + 12.0 * c.refs();
+            const double X = (double)f.cslotFine.size();
+            // diagonals (AC, start, nb/AF/slot map; one diagonal per fine cell written
+            // and re-read) + ordered gathers + coarse writes
+            bytes += 32.0 * f.n + 16.0 * f.refs() + 12.0 * X + (12.0 + w32) * c.n + (12.0 + w32) * c.refs();
         }
-    const int cycles = p.cycles > 1 ? p.cycles : 1;
+        if (mixed && p.coarseDirect) bytes += 8.0 * 3.0 * host.back().n * host.back().n;
+    }
+    auto jac = [&](int l) { return (4.0 * v + 4.0) * host[l].n + (v + 4.0) * host[l].refs(); };
+    auto sm = [&](int l, int sweeps) {
+        if (sweeps <= 0) return 0.0;
+        if (sweeps > 1 && host[l].n <= kSmoothBlockMax) return (4.0 * v + 4.0) * host[l].n + (v + 4.0) * host[l].refs();
+        return sweeps * jac(l);
+    };
     double cyc = 0.0;
+    int preDone = 0;
+    if (mixed) {
+        // FP64 defect (AC, b, x, start + AF/nb; writes b32, x32 (+ reads AC32))
+        cyc += 32.0 * host[0].n + 12.0 * host[0].refs() + 8.0 * host[0].n + (p.pre >= 1 ? 4.0 * host[0].n : 0.0);
+        preDone = (L == 1 ? (!p.coarseDirect && p.coarseSweeps >= 1) : (p.pre >= 1)) ? 1 : 0;
+    }
     for (int l = 0; l < L; l++) {
-        if (host[l].coarsest()) {
-            cyc += smoothBytes(l, p.coarseSweeps);
+        if (l == tail.lt) {
+            cyc += tailTrafficBytes(tail, (size_t)v);
             break;
         }
-        const double nc = host[l + 1].n;
-        cyc += smoothBytes(l, p.pre);
-        cyc += 48.0 * host[l].n + 12.0 * host[l].refs() + 20.0 * nc;  // residual pass + ordered gather
-        if (p.post >= 1) cyc += 40.0 * host[l].n + 12.0 * host[l].refs() + 8.0 * nc + smoothBytes(l, p.post - 1);
-        else cyc += 20.0 * host[l].n + 8.0 * nc;
+        if (host[l].coarsest()) {
+            cyc += (mixed && p.coarseDirect) ? 8.0 * host[l].n * host[l].n + 2.0 * v * host[l].n
+                                             : sm(l, p.coarseSweeps - preDone);
+            break;
+        }
+        const double n = host[l].n, S = host[l].refs(), nc = host[l + 1].n;
+        cyc += sm(l, p.pre - preDone);
+        const int s1 = host[l + 1].coarsest() ? (!(mixed && p.coarseDirect) && p.coarseSweeps >= 1) : (p.pre >= 1);
+        // restriction: fine rows (AC, b, x, start, members) + the residual written and
+        // re-read + memberStart, coarse b and x (+ coarse AC for the folded sweep)
+        cyc += (5.0 * v + 8.0) * n + (v + 4.0) * S + (2.0 * v + 4.0) * nc + (s1 ? v * nc : 0.0);
+        if (p.post >= 1) cyc += (4.0 * v + 8.0) * n + (v + 4.0) * S + v * nc + sm(l, p.post - 1);
+        else cyc += (2.0 * v + 4.0) * n + v * nc;
+        preDone = s1;
     }
-    return bytes + cycles * cyc;
+    if (mixed) cyc += 20.0 * host[0].n;  // x += e (FP64 x read + write, FP32 e)
+    return bytes + (p.cycles > 1 ? p.cycles : 1) * cyc;
 }
 
 // ---------------------------------------------------------------------------
@@ -457,6 +576,8 @@ void Solver::allocateState() {
     s.mDot = stateArena_.zeros<double>(F);
     s.AFuv = stateArena_.zeros<double>(S);
     s.ppAF = stateArena_.zeros<double>(S);
+    s.uvA = stateArena_.zeros<double2>(N);
+    s.uvB = stateArena_.zeros<double2>(N);
     partialBlocks_ = blocksFor(N);
     s.partial = stateArena_.zeros<double>((size_t)partialBlocks_ * 9);
     s.ctl = stateArena_.zeros<Ctl>(1);
@@ -473,14 +594,12 @@ void Solver::allocateMixed() {
         const int N = dm_.N, S = dm_.S;
         MixedState x{};
         x.AF32 = mixedArena_.zeros<float>(S);
-        for (float** p : {&x.invU, &x.invV, &x.ru, &x.rv, &x.du, &x.dv, &x.du2, &x.dv2})
-            *p = mixedArena_.zeros<float>(N);
+        for (float2** p : {&x.inv, &x.r, &x.d, &x.d2}) *p = mixedArena_.zeros<float2>(N);
         mx_ = x;
         mixedAllocated_ = true;
     }
     ds_.AF32 = mx_.AF32;
-    ds_.uInv32 = mx_.invU;
-    ds_.vInv32 = mx_.invV;
+    ds_.uvInv32 = mx_.inv;
 }
 
 // createBoundaryFieldHost/Device (memory_manger.cpp:51-189) with the value of
@@ -552,30 +671,33 @@ void Solver::enqueueIteration(hipStream_t s) {
         // 2..L-1 in FP32, sweep L fused with u += delta
         const int L = c.linear_max_iter;
         launchMomDefect(s, dm_, st, mx_);
-        float *du = mx_.du, *dv = mx_.dv, *du2 = mx_.du2, *dv2 = mx_.dv2;
+        float2 *d = mx_.d, *d2 = mx_.d2;
         for (int it = 2; it < L; it++) {
-            launchMomSweep32(s, dm_, st.ctl, mx_, du, dv, du2, dv2);
-            std::swap(du, du2);
-            std::swap(dv, dv2);
+            launchMomSweep32(s, dm_, st.ctl, mx_, d, d2);
+            std::swap(d, d2);
         }
-        launchMomSweepApply(s, dm_, st, mx_, du, dv, L >= 2 ? 1 : 0);
+        launchMomSweepApply(s, dm_, st, mx_, d, L >= 2 ? 1 : 0);
     } else if (c.linear_solver == AXS_LINEAR_GS_RB && nColors_ > 0) {
         for (int it = 0; it < c.linear_max_iter; it++)
             for (int col = 0; col < nColors_; col++) {
                 const int begin = colorStart_[col], count = colorStart_[col + 1] - begin;
                 if (count > 0) launchGSColorUV(s, dm_, st, dOrder_, begin, count);
             }
-    } else {
-        double *ux = st.u, *ut = st.uTmp, *vx = st.v, *vt = st.vTmp;
-        for (int it = 0; it < c.linear_max_iter; it++) {
-            launchJacobiUV(s, dm_, st, ux, vx, ut, vt);
-            std::swap(ux, ut);
-            std::swap(vx, vt);
+    } else if (c.linear_max_iter >= 2) {
+        // u, v -> (u,v) pairs -> ... -> (u,v) pairs -> u, v: the same sweeps, the
+        // iterates in between interleaved; the last one lands in u, v directly
+        const int L = c.linear_max_iter;
+        double2 *a = st.uvA, *b = st.uvB;
+        launchJacobiUVPacked(s, dm_, st, st.u, st.v, nullptr, nullptr, nullptr, a);
+        for (int it = 1; it < L - 1; it++) {
+            launchJacobiUVPacked(s, dm_, st, nullptr, nullptr, a, nullptr, nullptr, b);
+            std::swap(a, b);
         }
-        if (ux != st.u) {
-            launchCopy(s, st.u, ux, dm_.N, st.ctl);
-            launchCopy(s, st.v, vx, dm_.N, st.ctl);
-        }
+        launchJacobiUVPacked(s, dm_, st, nullptr, nullptr, a, st.u, st.v, nullptr);
+    } else if (c.linear_max_iter == 1) {
+        launchJacobiUV(s, dm_, st, st.u, st.v, st.uTmp, st.vTmp);
+        launchCopy(s, st.u, st.uTmp, dm_.N, st.ctl);
+        launchCopy(s, st.v, st.vTmp, dm_.N, st.ctl);
     }
 
     launchPPCoeff(s, dm_, bp, st, ph);
@@ -727,8 +849,7 @@ void Solver::run(axs_progress_cb cb, void* user, bool continueRun) {
             throw std::domain_error("coarsest level too large for the dense FP64 solve");
     } else {
         ds_.AF32 = nullptr;
-        ds_.uInv32 = nullptr;
-        ds_.vInv32 = nullptr;
+        ds_.uvInv32 = nullptr;
     }
     if (c.linear_solver == AXS_LINEAR_GS_RB && nColors_ == 0) {
         std::vector<int> order;
@@ -874,9 +995,10 @@ void Solver::benchKernel(int kernel, int repeats, double* ms, double* bytes) {
     std::function<void()> launch;
     switch (kernel) {
     case AXS_KERNEL_JACOBI_UV:
-        // reads b, invAC, x of u and v + CSR + shared AF; writes x of u and v
+        // the interleaved sweep (all but the first and last of each solve): reads
+        // b, invAC, x of u and v + CSR + shared AF; writes x of u and v
         *bytes = 68.0 * N + 12.0 * S;
-        launch = [&] { launchJacobiUV(stream_, dm_, st, st.u, st.v, st.uTmp, st.vTmp); };
+        launch = [&] { launchJacobiUVPacked(stream_, dm_, st, nullptr, nullptr, st.uvA, nullptr, nullptr, st.uvB); };
         break;
     case AXS_KERNEL_MOMENTUM: {
         const bool needGrad = ph.nonOrtho || (cfg_.add_convection && ph.scheme == AXS_CONV_SOU);

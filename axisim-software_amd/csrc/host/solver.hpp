@@ -90,21 +90,39 @@ public:
     void enqueueSolve(hipStream_t s, const MGParams& p, const Ctl* ctl, bool buildOperators);
     double bytesPerSolve(const MGParams& p, bool buildOperators) const;
     int coarsestCells() const { return host.empty() ? 0 : host.back().n; }
-    size_t stageBytes(bool fp32) const;
+    // first level of the LDS-resident tail (-1: no tail) and its LDS bytes
+    int tailLevel(bool fp32) const { return (fp32 ? tail32_ : tail64_).lt; }
+    size_t tailBytes(bool fp32) const { return (fp32 ? tail32_ : tail64_).bytes; }
 
 private:
-    void vcycle(hipStream_t s, int l, const MGParams& p, const Ctl* ctl);
+    // The tail kernel's LDS image of levels lt..lt+nl-1 (value type of size vsize)
+    struct TailPlan {
+        int lt = -1, nl = 0, intBase = 0, threads = 0;
+        size_t bytes = 0;
+        std::vector<TailLevel> lv;
+        TailLevel* dev = nullptr;      // device copy of lv
+        TailCopy* devCopies = nullptr;  // device copy of the staging list
+        int ncopy = 0, totalWords = 0;
+        TailLaunch launch() const;
+    };
+    TailPlan planTail(int lt, size_t vsize) const;
+    void chooseTail(TailPlan& t, size_t vsize, bool fp32);
+    double tailTrafficBytes(const TailPlan& t, size_t vsize) const;
+    void vcycle(hipStream_t s, int l, int preDone, const MGParams& p, const Ctl* ctl);
     void smooth(hipStream_t s, int l, int sweeps, const MGParams& p, const Ctl* ctl);
+    void restrictTo(hipStream_t s, int l, int sweep1, bool fp32, const MGParams& p, const Ctl* ctl);
     void enqueueSolveMixed(hipStream_t s, const MGParams& p, const Ctl* ctl, bool buildOperators);
-    void vcycle32(hipStream_t s, int l, bool skipPre, const MGParams& p, const Ctl* ctl);
+    void vcycle32(hipStream_t s, int l, int preDone, const MGParams& p, const Ctl* ctl);
     void smooth32(hipStream_t s, int l, int sweeps, const MGParams& p, const Ctl* ctl);
+    void buildOperators(hipStream_t s, const Ctl* ctl);
     void swapX(int l);
     void swapX32(int l);
-    void syncLevels();  // upload the DLevel array the tail kernel reads
+    void syncLevels();  // upload the DLevel array and the tail plans the tail kernel reads
     double* x0Home_ = nullptr;
     double* x0Other_ = nullptr;
     DLevel* dLevels_ = nullptr;
-    int tail_ = -1;  // first level run by the single-workgroup tail kernel
+    TailPlan tail64_, tail32_;
+    int ldsLimit_ = 0;
 };
 
 class Solver {

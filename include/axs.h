@@ -241,6 +241,11 @@ axs_status axs_mg_create(axs_mg** out, int n_cells, const int* face_start,
 void axs_mg_destroy(axs_mg* mg);
 int axs_mg_levels(const axs_mg* mg);
 axs_status axs_mg_level_info(const axs_mg* mg, int level, int* n_cells, int* n_refs);
+/* Where the V-cycle switches to its single-workgroup, LDS-resident tail for the
+ * given precision: *first_level = -1 when no tail fits; *lds_bytes = its LDS
+ * image.  Build extension: the reference launches every level separately
+ * (multigrid.cu:607-662) and smooths only the coarsest in one block (501-541). */
+axs_status axs_mg_tail_info(const axs_mg* mg, int precision, int* first_level, int* lds_bytes);
 /* Host copies of one level's maps; any pointer may be NULL.  cell_to_coarse and
  * fine_slot_to_coarse_slot are empty (untouched) on the coarsest level. */
 axs_status axs_mg_get_maps(const axs_mg* mg, int level, int* face_start, int* face_neighbor,

@@ -44,15 +44,40 @@ def test_vcycle_matches_oracle(require_gpu, cycles):
     np.testing.assert_allclose(xg, xr, rtol=0, atol=1e-13 * np.abs(xr).max())
 
 
-def test_vcycles_match_textbook_on_nozzle(require_gpu):
-    """Deep hierarchy (tail kernel, split restriction, staged coarsest level)
-    against the textbook FP64 cycle."""
-    from mg_textbook import TextbookMG, spmat
+# (pre, post, coarse): the first sweep from zero folded into the restriction
+# (pre >= 1), none to fold (pre 0, coarse 0), several left after it (pre 2)
+SWEEPS = [(1, 1, 30), (0, 1, 30), (2, 2, 7), (1, 0, 0), (3, 1, 1)]
 
-    case = axs.nozzle_case(mesh=axs.nozzle_mesh(spacing=1.2e-4))
+
+@pytest.mark.parametrize("sweeps", SWEEPS, ids=lambda s: "pre%d_post%d_coarse%d" % s)
+@pytest.mark.parametrize("size", ["tail_at_0", "tail_below"])
+def test_vcycle_sweep_counts_bit_exact(require_gpu, sweeps, size):
+    pre, post, coarse = sweeps
+    case = axs.pipe_case(nz=32, nr=8) if size == "tail_at_0" else axs.pipe_case()
     fs, fn, AC, AF, b = pp_system(case)
     g = axs.Multigrid(fs, fn)
+    tail, lds = g.tail_info()
+    print("levels", [g.level_info(l)[0] for l in range(g.levels)], "tail", tail, "lds", lds)
+    assert (tail == 0) if size == "tail_at_0" else (tail > 0)
+    x0 = np.linspace(-1.0, 1.0, b.size) * 1e-3  # a non-zero start for level 0
+    xg = g.solve(AC, AF, b, x0, cycles=2, pre=pre, post=post, coarse=coarse)
+    xr = OracleMG(fs, fn).solve(AC, AF, b, x0, cycles=2, pre=pre, post=post, coarse=coarse)
+    np.testing.assert_array_equal(xg, xr)
+
+
+def test_vcycles_match_textbook_on_nozzle(require_gpu):
+    """Deep hierarchy (two-pass restriction on level 0, one-pass below it, the
+    LDS-resident tail) against the textbook FP64 cycle and the oracle."""
+    from mg_textbook import TextbookMG, spmat
+
+    case = axs.nozzle_case(mesh=axs.nozzle_mesh(spacing=1.1e-4))
+    fs, fn, AC, AF, b = pp_system(case)
+    g = axs.Multigrid(fs, fn)
+    sizes = [g.level_info(l)[0] for l in range(g.levels)]
+    print("levels", sizes, "tail", g.tail_info(), "fp32 tail", g.tail_info(axs.PREC_MIXED))
     assert g.levels >= 5
+    assert sizes[0] > 65536 >= sizes[1]  # both restriction forms run
+    assert 0 < g.tail_info()[0] < g.levels - 1  # the tail holds more than the coarsest level
     x = g.solve(AC, AF, b, np.zeros_like(b), cycles=3)
     tb = TextbookMG(spmat(fs, fn, AC, AF), [g.maps(l)["cell_to_coarse"] for l in range(g.levels - 1)])
     ref = tb.solve(b, np.zeros_like(b), 3)
