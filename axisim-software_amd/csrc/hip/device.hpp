@@ -90,20 +90,14 @@ struct TailLevel {
     int AC, b, x0, x1, AF;
     int start, nb, toC, mS, mem;
 };
-// One staged array: `words` 32-bit words from src to word `dst` of the LDS
-// image; `first` = words of the copies before it.
-struct TailCopy {
-    const void* src;
-    int dst, first, words;
-};
 constexpr int kTailLevels = 8;         // levels the tail kernel may hold
-constexpr int kTailCopies = 72;        // staged arrays it may copy (3 + 8 per level)
-constexpr int kTailStaticLds = 2048;   // LDS the tail kernel declares statically
-// Everything a tail launch needs (device copies of the plan and copy list).
+constexpr int kTailStaticLds = 2048;   // LDS reserved next to the tail's dynamic image
+// Everything a tail launch needs: the plan (device copy) and the global image
+// of the tail levels' arrays (16-byte vectors), staged into LDS as it is.
 struct TailLaunch {
     const TailLevel* plan = nullptr;
-    const TailCopy* copies = nullptr;
-    int ncopy = 0, totalWords = 0, lt = -1, nl = 0, intBase = 0, threads = 0;
+    unsigned char* image = nullptr;
+    int vecs = 0, lt = -1, nl = 0, intBase = 0, threads = 0;
     size_t ldsBytes = 0;
 };
 
@@ -143,8 +137,8 @@ void launchMGProlongSmooth(hipStream_t s, const DLevel& f, const DLevel& c, doub
 void launchMGProlong(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl);
 // V-cycle of levels lt..lt+nl-1 in one workgroup, all of them staged in LDS
 // (levels: device copy of the DLevel array)
-void launchMGTail(hipStream_t s, const DLevel* levels, const TailLaunch& t, double w, int preDone, int pre, int post,
-                  int coarseSweeps, bool fp32, const double* inv, const Ctl* ctl);
+void launchMGTail(hipStream_t s, const TailLaunch& t, double w, int preDone, int pre, int post, int coarseSweeps,
+                  bool fp32, const double* inv, const Ctl* ctl);
 // LDS one workgroup may use on the current device (opting the tail kernel in
 // beyond 64 KiB where the device allows it)
 int tailLdsLimit();

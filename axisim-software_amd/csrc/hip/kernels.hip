@@ -948,8 +948,7 @@ __global__ void __launch_bounds__(kBlock) kMGRestrictSum(DLevel f, DLevel c, dou
 namespace {
 constexpr int kRowK = 12;     // slots a row keeps in registers (longer rows loop over LDS)
 constexpr int kMemberK = 8;   // aggregate members gathered without a loop
-constexpr int kCopyR = 24;    // staged words each thread holds per round
-constexpr int kMaxCopies = kTailCopies;
+constexpr int kCopyR = 8;     // staged 16-byte vectors each thread holds per round
 
 template <class T>
 struct TRow {
@@ -997,48 +996,30 @@ __device__ __forceinline__ T rowAx(const TRow<T>& r, T xn, XAt xat, const T* AF,
 }  // namespace
 
 template <class T>
-__global__ void __launch_bounds__(1024) kMGTail(const DLevel* __restrict__ Ls, const TailLevel* __restrict__ tl,
-                                                const TailCopy* __restrict__ copies, int ncopy, int totalWords,
-                                                int lt, int nl, int intBase, double wd, int preDone, int pre,
+__global__ void __launch_bounds__(1024) kMGTail(const TailLevel* __restrict__ tl, uint4* __restrict__ image,
+                                                int vecs, int nl, int intBase, double wd, int preDone, int pre,
                                                 int post, int coarseSweeps, const double* inv, const Ctl* ctl) {
     if (finished(ctl)) return;
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ const unsigned* cpSrc[kMaxCopies];
-    __shared__ int cpDst[kMaxCopies], cpFirst[kMaxCopies + 1];
     T* sT = (T*)smem;
     int* sI = (int*)(smem + intBase);
-    unsigned* sW = (unsigned*)smem;
+    uint4* sV = (uint4*)smem;
     const T w = (T)wd;
     const int t = threadIdx.x, NT = blockDim.x;
 
-    // ---- stage: every word of the image loaded before any is stored, so the
-    // global reads of all arrays of all levels are in flight together
-    for (int i = t; i < ncopy; i += NT) {
-        cpSrc[i] = (const unsigned*)copies[i].src;
-        cpDst[i] = copies[i].dst;
-        cpFirst[i] = copies[i].first;
-    }
-    if (t == 0) cpFirst[ncopy] = totalWords;
-    __syncthreads();
-    for (int base = 0; base < totalWords; base += kCopyR * NT) {
-        unsigned v[kCopyR];
-        int seg = 0;
+    // ---- stage: the image is the LDS layout itself; each round loads all its
+    // vectors before storing any, so they are in flight together
+    for (int base = 0; base < vecs; base += kCopyR * NT) {
+        uint4 v[kCopyR];
 #pragma unroll
         for (int r = 0; r < kCopyR; r++) {
-            const int wd_ = base + r * NT + t;
-            if (wd_ < totalWords) {
-                while (cpFirst[seg + 1] <= wd_) seg++;
-                v[r] = cpSrc[seg][wd_ - cpFirst[seg]];
-            }
+            const int i = base + r * NT + t;
+            if (i < vecs) v[r] = image[i];
         }
-        seg = 0;
 #pragma unroll
         for (int r = 0; r < kCopyR; r++) {
-            const int wd_ = base + r * NT + t;
-            if (wd_ < totalWords) {
-                while (cpFirst[seg + 1] <= wd_) seg++;
-                sW[cpDst[seg] + (wd_ - cpFirst[seg])] = v[r];
-            }
+            const int i = base + r * NT + t;
+            if (i < vecs) sV[i] = v[r];
         }
     }
     __syncthreads();
@@ -1173,10 +1154,10 @@ __global__ void __launch_bounds__(1024) kMGTail(const DLevel* __restrict__ Ls, c
         }
     }
 
-    // ---- level lt's answer back to its own x buffer
+    // ---- level lt's answer back to its x buffer (x0 of the global image)
     const TailLevel top = tl[0];
     const T* x = X(top, 0);
-    T* out = view<T>(Ls[lt]).x;
+    T* out = (T*)image + top.x0;
     if (t < top.n) out[t] = x[t];
 }
 
@@ -1202,14 +1183,14 @@ int tailLdsLimit() {
     }
     return limit;
 }
-void launchMGTail(hipStream_t s, const DLevel* levels, const TailLaunch& t, double w, int preDone, int pre, int post,
-                  int coarseSweeps, bool fp32, const double* inv, const Ctl* ctl) {
+void launchMGTail(hipStream_t s, const TailLaunch& t, double w, int preDone, int pre, int post, int coarseSweeps,
+                  bool fp32, const double* inv, const Ctl* ctl) {
     if (fp32)
-        kMGTail<float><<<1, t.threads, t.ldsBytes, s>>>(levels, t.plan, t.copies, t.ncopy, t.totalWords, t.lt, t.nl,
-                                                        t.intBase, w, preDone, pre, post, coarseSweeps, inv, ctl);
+        kMGTail<float><<<1, t.threads, t.ldsBytes, s>>>(t.plan, (uint4*)t.image, t.vecs, t.nl, t.intBase, w, preDone,
+                                                        pre, post, coarseSweeps, inv, ctl);
     else
-        kMGTail<double><<<1, t.threads, t.ldsBytes, s>>>(levels, t.plan, t.copies, t.ncopy, t.totalWords, t.lt, t.nl,
-                                                         t.intBase, w, preDone, pre, post, coarseSweeps, inv, ctl);
+        kMGTail<double><<<1, t.threads, t.ldsBytes, s>>>(t.plan, (uint4*)t.image, t.vecs, t.nl, t.intBase, w, preDone,
+                                                         pre, post, coarseSweeps, inv, ctl);
 }
 void launchMGCoarseDiag(hipStream_t s, const DLevel& f, const Ctl* ctl) {
     kMGCoarseDiag<<<blocksFor(f.n), kBlock, 0, s>>>(f, ctl);

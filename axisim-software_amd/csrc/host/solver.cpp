@@ -38,39 +38,81 @@ void MGDevice::build(int n, const std::vector<int>& start, const std::vector<int
     mixedReady = false;
     aug = inv = nullptr;
     invStatus = nullptr;
-    dLevels_ = nullptr;
-    tail64_ = TailPlan{};
+    if (ldsLimit_ == 0) ldsLimit_ = tailLdsLimit();
+    tail64_ = selectTail(sizeof(double));
     tail32_ = TailPlan{};
+    if (tail64_.lt >= 0) tail64_.image = arena.zeros<unsigned char>(tail64_.bytes);
     dev.assign(host.size(), DLevel{});
     for (size_t l = 0; l < host.size(); l++) {
         const Level& h = host[l];
         DLevel& d = dev[l];
         d.n = h.n;
         d.refs = h.refs();
-        d.start = arena.upload(h.start);
-        d.nb = arena.upload(h.nb);
-        if (l > 0) {
-            d.AC = arena.zeros<double>(h.n);
-            d.b = arena.zeros<double>(h.n);
-            d.AF = arena.zeros<double>(h.refs());
-            d.x = arena.zeros<double>(h.n);
-            d.xNew = arena.zeros<double>(h.n);
+        if (tail64_.holds((int)l)) {
+            // operator, iterates, CSR and maps inside the tail's image
+            const TailLevel& e = tail64_.lv[l - tail64_.lt];
+            double* v = (double*)tail64_.image;
+            d.AC = v + e.AC;
+            d.b = v + e.b;
+            d.x = v + e.x0;
+            d.xNew = v + e.x1;
+            d.AF = v + e.AF;
+            placeInts(tail64_, (int)l, d);
+        } else {
+            d.start = arena.upload(h.start);
+            d.nb = arena.upload(h.nb);
+            if (!h.coarsest()) {
+                d.toCoarse = arena.upload(h.toCoarse);
+                d.memberStart = arena.upload(h.memberStart);
+                d.members = arena.upload(h.members);
+            }
+            if (l > 0) {
+                d.AC = arena.zeros<double>(h.n);
+                d.b = arena.zeros<double>(h.n);
+                d.AF = arena.zeros<double>(h.refs());
+                d.x = arena.zeros<double>(h.n);
+                d.xNew = arena.zeros<double>(h.n);
+            }
         }
         if (!h.coarsest()) {
             d.nCoarse = host[l + 1].n;
-            d.toCoarse = arena.upload(h.toCoarse);
             d.slotToCoarseSlot = arena.upload(h.slotToCoarseSlot);
-            d.memberStart = arena.upload(h.memberStart);
-            d.members = arena.upload(h.members);
             d.cslotStart = arena.upload(h.cslotStart);
             d.cslotFine = arena.upload(h.cslotFine);
             d.res = arena.zeros<double>(h.n);
         }
     }
+    if (tail64_.lt >= 0) {
+        tail64_.dev = arena.alloc<TailLevel>(tail64_.lv.size());
+        AXS_HIP(hipMemcpy(tail64_.dev, tail64_.lv.data(), tail64_.lv.size() * sizeof(TailLevel),
+                          hipMemcpyHostToDevice));
+    }
+}
+
+// Upload level l's CSR and transfer maps into the int part of the tail image
+// and point the level at them.
+void MGDevice::placeInts(const TailPlan& t, int l, DLevel& d) {
+    const Level& h = host[l];
+    const TailLevel& e = t.lv[l - t.lt];
+    int* ip = (int*)(t.image + t.intBase);
+    auto put = [&](int off, const std::vector<int>& src) {
+        if (!src.empty())
+            AXS_HIP(hipMemcpy(ip + off, src.data(), src.size() * sizeof(int), hipMemcpyHostToDevice));
+        return ip + off;
+    };
+    d.start = put(e.start, h.start);
+    d.nb = put(e.nb, h.nb);
+    if (!h.coarsest()) {
+        d.toCoarse = put(e.toC, h.toCoarse);
+        d.memberStart = put(e.mS, h.memberStart);
+        d.members = put(e.mem, h.members);
+    }
 }
 
 // LDS image of levels lt..L-1 for the tail kernel: values (AC, b, x ping-pong,
-// AF) first, then ints (CSR, and the transfer maps of every level but the last)
+// AF) first, then ints (CSR, and the transfer maps of every level but the last).
+// The tail levels' device arrays live in a global copy of exactly this image,
+// so the kernel stages it with one contiguous copy.
 MGDevice::TailPlan MGDevice::planTail(int lt, size_t vsize) const {
     TailPlan t;
     const int L = (int)host.size();
@@ -101,75 +143,30 @@ MGDevice::TailPlan MGDevice::planTail(int lt, size_t vsize) const {
         maxN = std::max(maxN, h.n);
     }
     t.intBase = (int)(((size_t)tv * vsize + 15) / 16 * 16);
-    t.bytes = (size_t)t.intBase + (size_t)ti * sizeof(int);
+    t.bytes = ((size_t)t.intBase + (size_t)ti * sizeof(int) + 15) / 16 * 16;
     t.threads = std::min(1024, std::max(64, (maxN + 63) / 64 * 64));
     return t;
 }
 
-// The tail starts at the first level that is small enough, whose LDS image of
-// it and every coarser level fits one workgroup.  Builds the list of arrays the
-// kernel stages (device pointers of the levels: x and b of the first level, the
-// operator, CSR and transfer maps of every level).
-void MGDevice::chooseTail(TailPlan& t, size_t vsize, bool fp32) {
-    TailLevel* dev0 = t.dev;
-    TailCopy* cp0 = t.devCopies;
-    t = TailPlan{};
-    t.dev = dev0;
-    t.devCopies = cp0;
+// The tail starts at the first level below level 0 (whose arrays belong to the
+// caller) that is small enough and whose LDS image, with every coarser level,
+// fits one workgroup.
+MGDevice::TailPlan MGDevice::selectTail(size_t vsize) const {
     const int L = (int)host.size();
     const size_t budget = ldsLimit_ > kTailStaticLds ? (size_t)(ldsLimit_ - kTailStaticLds) : 0;
-    for (int l = 0; l < L; l++) {
+    for (int l = 1; l < L; l++) {
         if (host[l].n > kTailMax || L - l > kTailLevels) continue;
         TailPlan c = planTail(l, vsize);
-        if (c.bytes > budget) continue;
-        c.dev = dev0;
-        c.devCopies = cp0;
-        t = c;
-        break;
+        if (c.bytes <= budget) return c;
     }
-    if (t.lt < 0) return;
-    std::vector<TailCopy> cps;
-    int words = 0;
-    const int tw = (int)(vsize / 4), base = t.intBase / 4;
-    auto add = [&](const void* src, int dstWord, long count) {
-        if (count <= 0) return;
-        if (!src) throw std::logic_error("multigrid tail: level buffer not allocated");
-        cps.push_back(TailCopy{src, dstWord, words, (int)count});
-        words += (int)count;
-    };
-    for (int q = 0; q < t.nl; q++) {
-        const DLevel& d = dev[t.lt + q];
-        const TailLevel& e = t.lv[q];
-        const bool last = q + 1 == t.nl;
-        add(fp32 ? (const void*)d.AC32 : (const void*)d.AC, e.AC * tw, (long)e.n * tw);
-        if (q == 0) {
-            add(fp32 ? (const void*)d.b32 : (const void*)d.b, e.b * tw, (long)e.n * tw);
-            add(fp32 ? (const void*)d.x32 : (const void*)d.x, e.x0 * tw, (long)e.n * tw);
-        }
-        add(fp32 ? (const void*)d.AF32 : (const void*)d.AF, e.AF * tw, (long)e.refs * tw);
-        add(d.start, base + e.start, e.n + 1);
-        add(d.nb, base + e.nb, e.refs);
-        if (!last) {
-            add(d.toCoarse, base + e.toC, e.n);
-            add(d.memberStart, base + e.mS, e.nc + 1);
-            add(d.members, base + e.mem, e.n);
-        }
-    }
-    if ((int)cps.size() > kTailCopies) throw std::logic_error("multigrid tail: too many staged arrays");
-    t.ncopy = (int)cps.size();
-    t.totalWords = words;
-    if (!t.dev) t.dev = arena.alloc<TailLevel>(kTailLevels);
-    if (!t.devCopies) t.devCopies = arena.alloc<TailCopy>(kTailCopies);
-    AXS_HIP(hipMemcpy(t.dev, t.lv.data(), t.lv.size() * sizeof(TailLevel), hipMemcpyHostToDevice));
-    AXS_HIP(hipMemcpy(t.devCopies, cps.data(), cps.size() * sizeof(TailCopy), hipMemcpyHostToDevice));
+    return TailPlan{};
 }
 
 TailLaunch MGDevice::TailPlan::launch() const {
     TailLaunch l;
     l.plan = dev;
-    l.copies = devCopies;
-    l.ncopy = ncopy;
-    l.totalWords = totalWords;
+    l.image = image;
+    l.vecs = (int)(bytes / 16);
     l.lt = lt;
     l.nl = nl;
     l.intBase = intBase;
@@ -178,18 +175,10 @@ TailLaunch MGDevice::TailPlan::launch() const {
     return l;
 }
 
-// bytes the tail kernel moves between HBM and LDS: its staging reads and the
-// answer written back
+// bytes the tail kernel moves between HBM and LDS: the image in, level lt's x out
 double MGDevice::tailTrafficBytes(const TailPlan& t, size_t vsize) const {
     if (t.lt < 0) return 0.0;
-    double b = 0.0;
-    for (int q = 0; q < t.nl; q++) {
-        const TailLevel& e = t.lv[q];
-        b += (double)vsize * e.n + 4.0 * (e.n + 1) + (vsize + 4.0) * e.refs;  // AC, start, AF + nb
-        if (q + 1 < t.nl) b += 8.0 * e.n + 4.0 * (e.nc + 1);                   // toCoarse, members, memberStart
-        if (q == 0) b += 3.0 * vsize * e.n;                                      // b, x in; x out
-    }
-    return b;
+    return (double)t.bytes + (double)vsize * t.lv[0].n;
 }
 
 void MGDevice::bindLevel0(double* AC, double* b, double* AF, double* x, double* xNew) {
@@ -200,18 +189,6 @@ void MGDevice::bindLevel0(double* AC, double* b, double* AF, double* x, double* 
     dev[0].xNew = xNew;
     x0Home_ = x;
     x0Other_ = xNew;
-    syncLevels();
-}
-
-// The tail kernel reads the level pointers from the device copy made here; the
-// host never swaps the x buffers of a tail level (only the tail smooths them),
-// so that copy stays valid for them.
-void MGDevice::syncLevels() {
-    if (!dLevels_) dLevels_ = arena.alloc<DLevel>(host.size());
-    AXS_HIP(hipMemcpy(dLevels_, dev.data(), dev.size() * sizeof(DLevel), hipMemcpyHostToDevice));
-    if (ldsLimit_ == 0) ldsLimit_ = tailLdsLimit();
-    chooseTail(tail64_, sizeof(double), false);
-    if (mixedReady) chooseTail(tail32_, sizeof(float), true);
 }
 
 void MGDevice::swapX(int l) { std::swap(dev[l].x, dev[l].xNew); }
@@ -243,8 +220,7 @@ void MGDevice::restrictTo(hipStream_t s, int l, int sweep1, bool fp32, const MGP
 // already applied (by the restriction above it).
 void MGDevice::vcycle(hipStream_t s, int l, int preDone, const MGParams& p, const Ctl* ctl) {
     if (l == tail64_.lt) {
-        launchMGTail(s, dLevels_, tail64_.launch(), p.weight, preDone, p.pre, p.post, p.coarseSweeps, false, nullptr,
-                     ctl);
+        launchMGTail(s, tail64_.launch(), p.weight, preDone, p.pre, p.post, p.coarseSweeps, false, nullptr, ctl);
         return;
     }
     if (host[l].coarsest()) {
@@ -269,14 +245,35 @@ void MGDevice::swapX32(int l) { std::swap(dev[l].x32, dev[l].xNew32); }
 void MGDevice::ensureMixed() {
     if (mixedReady) return;
     arena32.release();
+    tail32_ = selectTail(sizeof(float));
+    if (tail32_.lt >= 0) tail32_.image = arena32.zeros<unsigned char>(tail32_.bytes);
     for (size_t l = 0; l < host.size(); l++) {
         DLevel& d = dev[l];
-        d.AC32 = arena32.zeros<float>(d.n);
-        d.AF32 = arena32.zeros<float>(d.refs);
-        d.b32 = arena32.zeros<float>(d.n);
-        d.x32 = arena32.zeros<float>(d.n);
-        d.xNew32 = arena32.zeros<float>(d.n);
+        if (tail32_.holds((int)l)) {
+            // FP32 copies inside the FP32 tail image, with its own copy of the
+            // CSR and maps (the FP64 arrays keep theirs)
+            const TailLevel& e = tail32_.lv[l - tail32_.lt];
+            float* v = (float*)tail32_.image;
+            d.AC32 = v + e.AC;
+            d.b32 = v + e.b;
+            d.x32 = v + e.x0;
+            d.xNew32 = v + e.x1;
+            d.AF32 = v + e.AF;
+            DLevel scratch = d;
+            placeInts(tail32_, (int)l, scratch);
+        } else {
+            d.AC32 = arena32.zeros<float>(d.n);
+            d.AF32 = arena32.zeros<float>(d.refs);
+            d.b32 = arena32.zeros<float>(d.n);
+            d.x32 = arena32.zeros<float>(d.n);
+            d.xNew32 = arena32.zeros<float>(d.n);
+        }
         if (!host[l].coarsest()) d.res32 = arena32.zeros<float>(d.n);
+    }
+    if (tail32_.lt >= 0) {
+        tail32_.dev = arena32.alloc<TailLevel>(tail32_.lv.size());
+        AXS_HIP(hipMemcpy(tail32_.dev, tail32_.lv.data(), tail32_.lv.size() * sizeof(TailLevel),
+                          hipMemcpyHostToDevice));
     }
     const int nc = coarsestCells();
     if (nc <= kDirectMax) {
@@ -285,7 +282,6 @@ void MGDevice::ensureMixed() {
         invStatus = arena32.zeros<int>(1);
     }
     mixedReady = true;
-    syncLevels();
 }
 
 void MGDevice::smooth32(hipStream_t s, int l, int sweeps, const MGParams& p, const Ctl* ctl) {
@@ -303,7 +299,7 @@ void MGDevice::smooth32(hipStream_t s, int l, int sweeps, const MGParams& p, con
 
 void MGDevice::vcycle32(hipStream_t s, int l, int preDone, const MGParams& p, const Ctl* ctl) {
     if (l == tail32_.lt) {
-        launchMGTail(s, dLevels_, tail32_.launch(), p.weight, preDone, p.pre, p.post, p.coarseSweeps, true,
+        launchMGTail(s, tail32_.launch(), p.weight, preDone, p.pre, p.post, p.coarseSweeps, true,
                      p.coarseDirect ? inv : nullptr, ctl);
         return;
     }

@@ -100,13 +100,14 @@ private:
         int lt = -1, nl = 0, intBase = 0, threads = 0;
         size_t bytes = 0;
         std::vector<TailLevel> lv;
-        TailLevel* dev = nullptr;      // device copy of lv
-        TailCopy* devCopies = nullptr;  // device copy of the staging list
-        int ncopy = 0, totalWords = 0;
+        TailLevel* dev = nullptr;        // device copy of lv
+        unsigned char* image = nullptr;  // the tail levels' arrays, laid out as in LDS
+        bool holds(int l) const { return lt >= 1 && l >= lt; }
         TailLaunch launch() const;
     };
     TailPlan planTail(int lt, size_t vsize) const;
-    void chooseTail(TailPlan& t, size_t vsize, bool fp32);
+    TailPlan selectTail(size_t vsize) const;
+    void placeInts(const TailPlan& t, int l, DLevel& d);
     double tailTrafficBytes(const TailPlan& t, size_t vsize) const;
     void vcycle(hipStream_t s, int l, int preDone, const MGParams& p, const Ctl* ctl);
     void smooth(hipStream_t s, int l, int sweeps, const MGParams& p, const Ctl* ctl);
@@ -117,10 +118,8 @@ private:
     void buildOperators(hipStream_t s, const Ctl* ctl);
     void swapX(int l);
     void swapX32(int l);
-    void syncLevels();  // upload the DLevel array and the tail plans the tail kernel reads
     double* x0Home_ = nullptr;
     double* x0Other_ = nullptr;
-    DLevel* dLevels_ = nullptr;
     TailPlan tail64_, tail32_;
     int ldsLimit_ = 0;
 };
