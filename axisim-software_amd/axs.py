@@ -26,6 +26,7 @@ TIME_BDF1, TIME_BDF2 = range(2)
 NORM_L1, NORM_L2, NORM_LINF = range(3)
 SCALE_NONE, SCALE_N, SCALE_SQRT_N, SCALE_DIAGONAL, SCALE_CONTINUITY = range(5)
 PREC_FP64, PREC_MIXED = range(2)
+MG_SMOOTH_JACOBI, MG_SMOOTH_RBGS = range(2)
 VAR_U, VAR_V, VAR_P = range(3)
 RES_U, RES_V, RES_CONTINUITY = range(3)
 KERNEL_JACOBI_UV, KERNEL_MOMENTUM, KERNEL_MG_SMOOTH, KERNEL_FACE_FLUX, KERNEL_PP_COEFF = range(5)
@@ -148,9 +149,9 @@ def load_library(path: os.PathLike | None = None) -> C.CDLL:
         "axs_mg_tail_info": (C.c_int, [S, C.c_int, _ip, _ip]),
         "axs_mg_get_maps": (C.c_int, [S, C.c_int, _ip, _ip, _ip, _ip]),
         "axs_mg_solve": (C.c_int, [S, _dp, _dp, _dp, _dp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
-                                   C.c_int, C.c_int]),
+                                   C.c_int, C.c_int, C.c_int]),
         "axs_mg_bench": (C.c_int, [S, _dp, _dp, _dp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int,
-                                   C.c_int, C.c_int, _dp, _dp, _dp]),
+                                   C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]),
         "axs_build_coloring": (C.c_int, [C.c_int, _ip, _ip, _ip, _ip, _ip]),
         "axs_mesh_multiblock": (C.c_int, [C.POINTER(S), C.c_int, C.POINTER(BlockSpec)]),
         "axs_mesh_triangles": (C.c_int, [C.POINTER(S), C.c_int, _dp, _dp, C.c_int, _ip, C.c_int, _dp, _dp, _dp,
@@ -528,20 +529,21 @@ class Multigrid:
         return {"face_start": fs, "face_neighbor": fn, "cell_to_coarse": c2c, "slot_to_coarse_slot": s2s}
 
     def solve(self, AC, AF, b, x, cycles=3, weight=0.6, pre=1, post=1, coarse=30, precision=PREC_FP64,
-              coarse_direct=0) -> np.ndarray:
+              coarse_direct=0, smoother=MG_SMOOTH_JACOBI) -> np.ndarray:
         AC, AF, b = (np.ascontiguousarray(a, dtype=np.float64) for a in (AC, AF, b))
         x = np.array(x, dtype=np.float64, copy=True)
         _check(lib().axs_mg_solve(self._h, _dptr(AC), _dptr(AF), _dptr(b), _dptr(x), cycles, weight, pre, post,
-                                  coarse, precision, coarse_direct))
+                                  coarse, precision, coarse_direct, smoother))
         return x
 
     def bench(self, AC, AF, b, cycles=3, weight=0.6, pre=1, post=1, coarse=30, precision=PREC_FP64,
-              coarse_direct=0, repeats=20):
+              coarse_direct=0, smoother=MG_SMOOTH_JACOBI, repeats=20):
         AC, AF, b = (np.ascontiguousarray(a, dtype=np.float64) for a in (AC, AF, b))
         ms, by = C.c_double(), C.c_double()
         x = np.zeros(self.n)
         _check(lib().axs_mg_bench(self._h, _dptr(AC), _dptr(AF), _dptr(b), cycles, weight, pre, post, coarse,
-                                  precision, coarse_direct, repeats, C.byref(ms), C.byref(by), _dptr(x)))
+                                  precision, coarse_direct, smoother, repeats, C.byref(ms), C.byref(by),
+                                  _dptr(x)))
         return ms.value, by.value, x
 
 

@@ -135,6 +135,14 @@ void launchMGJacobi(hipStream_t s, const DLevel& L, double w, const Ctl* ctl);
 void launchMGSmoothBlock(hipStream_t s, const DLevel& L, double w, int sweeps, const Ctl* ctl);
 void launchMGProlongSmooth(hipStream_t s, const DLevel& f, const DLevel& c, double w, const Ctl* ctl);
 void launchMGProlong(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl);
+// multi-colour Gauss-Seidel smoothing: one colour (order[begin .. begin+count)),
+// or every sweep and colour of a level of at most kSmoothBlockGSMax cells in one
+// workgroup (colorStart: device copy, nColors + 1 entries)
+constexpr int kSmoothBlockGSMax = 4096;
+void launchMGGSColor(hipStream_t s, const DLevel& L, const int* order, int begin, int count, bool fp32,
+                     const Ctl* ctl);
+void launchMGSmoothBlockGS(hipStream_t s, const DLevel& L, const int* order, const int* colorStart, int nColors,
+                           int sweeps, bool fp32, const Ctl* ctl);
 // V-cycle of levels lt..lt+nl-1 in one workgroup, all of them staged in LDS
 // (levels: device copy of the DLevel array)
 void launchMGTail(hipStream_t s, const TailLaunch& t, double w, int preDone, int pre, int post, int coarseSweeps,

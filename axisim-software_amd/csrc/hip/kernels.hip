@@ -933,6 +933,60 @@ __global__ void __launch_bounds__(kBlock) kMGRestrictSum(DLevel f, DLevel c, dou
 }
 
 // ===========================================================================
+// Multi-colour Gauss-Seidel smoothing of a level (build extension; the
+// reference smooths with weighted Jacobi only).  Cells of one colour share no
+// face, so each colour is a parallel in-place update:
+//   x_n = (b_n - sum AF x_nb) / AC_n   (rows with |AC| < 1e-30 keep x_n)
+// and the result does not depend on the order within a colour.
+// ===========================================================================
+namespace {
+template <class T>
+__device__ __forceinline__ T gsRow(const DLevel& L, const LV<T>& v, const T* x, int n) {
+    T val = v.b[n];
+    for (int k = L.start[n]; k < L.start[n + 1]; k++) {
+        const int nb = L.nb[k];
+        if (nb >= 0) val -= v.AF[k] * x[nb];
+    }
+    return val / v.AC[n];
+}
+}  // namespace
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) kMGGSColor(DLevel L, const int* __restrict__ order, int begin, int count,
+                                                     const Ctl* ctl) {
+    if (finished(ctl)) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const LV<T> v = view<T>(L);
+    const int n = order[begin + t];
+    if (tabs(v.AC[n]) < T(1.0e-30)) return;
+    v.x[n] = gsRow(L, v, v.x, n);
+}
+
+// every sweep and colour of a small level in one workgroup, x held in LDS
+template <class T>
+__global__ void __launch_bounds__(1024) kMGSmoothBlockGS(DLevel L, const int* __restrict__ order,
+                                                         const int* __restrict__ colorStart, int nColors,
+                                                         int sweeps, const Ctl* ctl) {
+    if (finished(ctl)) return;
+    extern __shared__ __align__(16) unsigned char smemGS[];
+    T* x = (T*)smemGS;
+    const LV<T> v = view<T>(L);
+    for (int n = threadIdx.x; n < L.n; n += blockDim.x) x[n] = v.x[n];
+    __syncthreads();
+    for (int s = 0; s < sweeps; s++)
+        for (int c = 0; c < nColors; c++) {
+            const int c0 = colorStart[c], c1 = colorStart[c + 1];
+            for (int t = c0 + (int)threadIdx.x; t < c1; t += blockDim.x) {
+                const int n = order[t];
+                if (!(tabs(v.AC[n]) < T(1.0e-30))) x[n] = gsRow(L, v, x, n);
+            }
+            __syncthreads();
+        }
+    for (int n = threadIdx.x; n < L.n; n += blockDim.x) v.x[n] = x[n];
+}
+
+// ===========================================================================
 // Tail of the V-cycle: the coarsest levels, from `lt` down, in ONE workgroup
 // with every one of them staged in LDS (operator, CSR, transfer maps, b and
 // the x ping-pong).  Coarse levels are tiny and launch-latency bound; here a
@@ -1275,6 +1329,22 @@ void launchMGProlongSmooth(hipStream_t s, const DLevel& f, const DLevel& c, doub
 }
 void launchMGProlong(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl) {
     kMGProlong<<<blocksFor(f.n), kBlock, 0, s>>>(f, c, ctl);
+}
+void launchMGGSColor(hipStream_t s, const DLevel& L, const int* order, int begin, int count, bool fp32,
+                     const Ctl* ctl) {
+    if (count <= 0) return;
+    if (fp32) kMGGSColor<float><<<blocksFor(count), kBlock, 0, s>>>(L, order, begin, count, ctl);
+    else kMGGSColor<double><<<blocksFor(count), kBlock, 0, s>>>(L, order, begin, count, ctl);
+}
+void launchMGSmoothBlockGS(hipStream_t s, const DLevel& L, const int* order, const int* colorStart, int nColors,
+                           int sweeps, bool fp32, const Ctl* ctl) {
+    const int threads = L.n >= 1024 ? 1024 : ((L.n + 63) / 64) * 64;
+    if (fp32)
+        kMGSmoothBlockGS<float><<<1, threads, (size_t)L.n * sizeof(float), s>>>(L, order, colorStart, nColors,
+                                                                               sweeps, ctl);
+    else
+        kMGSmoothBlockGS<double><<<1, threads, (size_t)L.n * sizeof(double), s>>>(L, order, colorStart, nColors,
+                                                                                 sweeps, ctl);
 }
 
 }  // namespace axs

@@ -282,8 +282,11 @@ static void mgUpload(axs_mg* h, const double* AC, const double* AF, const double
     else AXS_HIP(hipMemset(h->x, 0, h->n * sizeof(double)));
 }
 
-static MGParams mgParams(int cycles, double weight, int pre, int post, int coarse, int precision, int direct) {
+static MGParams mgParams(int cycles, double weight, int pre, int post, int coarse, int precision, int direct,
+                         int smoother) {
     if (precision != AXS_PREC_FP64 && precision != AXS_PREC_MIXED) throw std::invalid_argument("bad precision");
+    if (smoother != AXS_MG_SMOOTH_JACOBI && smoother != AXS_MG_SMOOTH_RBGS)
+        throw std::invalid_argument("bad multigrid smoother");
     if (direct && precision != AXS_PREC_MIXED)
         throw std::domain_error("the FP64 coarsest direct solve is built for mixed precision");
     if (pre < 0 || post < 0 || coarse < 0) throw std::invalid_argument("negative sweep count");
@@ -295,14 +298,15 @@ static MGParams mgParams(int cycles, double weight, int pre, int post, int coars
     p.coarseSweeps = coarse;
     p.precision = precision;
     p.coarseDirect = direct;
+    p.smoother = smoother;
     return p;
 }
 
 axs_status axs_mg_solve(axs_mg* h, const double* AC, const double* AF, const double* b, double* x, int cycles,
-                        double weight, int pre, int post, int coarse, int precision, int direct) {
+                        double weight, int pre, int post, int coarse, int precision, int direct, int smoother) {
     if (!h || !AC || !AF || !b || !x) return AXS_ERR_INVALID;
     return guarded(nullptr, [&] {
-        const MGParams p = mgParams(cycles, weight, pre, post, coarse, precision, direct);
+        const MGParams p = mgParams(cycles, weight, pre, post, coarse, precision, direct, smoother);
         AXS_HIP(hipSetDevice(h->device));
         mgUpload(h, AC, AF, b, x);
         h->mg.enqueueSolve(h->stream, p, nullptr, true);
@@ -313,11 +317,11 @@ axs_status axs_mg_solve(axs_mg* h, const double* AC, const double* AF, const dou
 }
 
 axs_status axs_mg_bench(axs_mg* h, const double* AC, const double* AF, const double* b, int cycles, double weight,
-                        int pre, int post, int coarse, int precision, int direct, int repeats, double* ms,
-                        double* bytes, double* x_out) {
+                        int pre, int post, int coarse, int precision, int direct, int smoother, int repeats,
+                        double* ms, double* bytes, double* x_out) {
     if (!h || !AC || !AF || !b || !ms || !bytes || repeats <= 0) return AXS_ERR_INVALID;
     return guarded(nullptr, [&] {
-        const MGParams p = mgParams(cycles, weight, pre, post, coarse, precision, direct);
+        const MGParams p = mgParams(cycles, weight, pre, post, coarse, precision, direct, smoother);
         AXS_HIP(hipSetDevice(h->device));
         mgUpload(h, AC, AF, b, nullptr);
         h->mg.enqueueSolve(h->stream, p, nullptr, true);  // warm-up

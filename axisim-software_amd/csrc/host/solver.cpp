@@ -35,6 +35,9 @@ void MGDevice::build(int n, const std::vector<int>& start, const std::vector<int
     host = buildHierarchy(n, start, nb, maxLevels);
     arena.release();
     arena32.release();
+    colorStart_.clear();
+    dOrder_.clear();
+    dColorStart_.clear();
     mixedReady = false;
     aug = inv = nullptr;
     invStatus = nullptr;
@@ -242,6 +245,52 @@ void MGDevice::vcycle(hipStream_t s, int l, int preDone, const MGParams& p, cons
 
 void MGDevice::swapX32(int l) { std::swap(dev[l].x32, dev[l].xNew32); }
 
+// Greedy colouring of every level's graph (buildMeshColoring order), built on
+// first use of the Gauss-Seidel smoother.
+void MGDevice::ensureColoring() {
+    if (colorStart_.size() == host.size()) return;
+    colorStart_.assign(host.size(), {});
+    dOrder_.assign(host.size(), nullptr);
+    dColorStart_.assign(host.size(), nullptr);
+    for (size_t l = 0; l < host.size(); l++) {
+        std::vector<int> order;
+        greedyColoring(host[l].n, host[l].start.data(), host[l].nb.data(), colorStart_[l], order);
+        dOrder_[l] = arena.upload(order);
+        dColorStart_[l] = arena.upload(colorStart_[l]);
+    }
+}
+
+void MGDevice::smoothGS(hipStream_t s, int l, int sweeps, bool fp32, const Ctl* ctl) {
+    const DLevel& L = dev[l];
+    if (L.n <= 0 || sweeps <= 0) return;
+    const int nColors = (int)colorStart_[l].size() - 1;
+    if (L.n <= kSmoothBlockGSMax) {
+        launchMGSmoothBlockGS(s, L, dOrder_[l], dColorStart_[l], nColors, sweeps, fp32, ctl);
+        return;
+    }
+    for (int it = 0; it < sweeps; it++)
+        for (int c = 0; c < nColors; c++) {
+            const int begin = colorStart_[l][c];
+            launchMGGSColor(s, L, dOrder_[l], begin, colorStart_[l][c + 1] - begin, fp32, ctl);
+        }
+}
+
+// vCycle (multigrid.cu:607-662) with Gauss-Seidel smoothing: x stays in place
+// on every level, the prolongation is a plain x += P x_c.
+void MGDevice::vcycleGS(hipStream_t s, int l, bool fp32, const MGParams& p, const Ctl* ctl) {
+    if (host[l].coarsest()) {
+        if (fp32 && p.coarseDirect) launchCoarseApply(s, dev[l], inv, ctl);
+        else smoothGS(s, l, p.coarseSweeps, fp32, ctl);
+        return;
+    }
+    smoothGS(s, l, p.pre, fp32, ctl);
+    restrictTo(s, l, 0, fp32, p, ctl);
+    vcycleGS(s, l + 1, fp32, p, ctl);
+    if (fp32) launchMGProlong32(s, dev[l], dev[l + 1], ctl);
+    else launchMGProlong(s, dev[l], dev[l + 1], ctl);
+    smoothGS(s, l, p.post, fp32, ctl);
+}
+
 void MGDevice::ensureMixed() {
     if (mixedReady) return;
     arena32.release();
@@ -336,6 +385,15 @@ void MGDevice::enqueueSolveMixed(hipStream_t s, const MGParams& p, const Ctl* ct
     const int cycles = p.cycles > 1 ? p.cycles : 1;
     // the first sweep from e = 0 is fused into the defect kernel
     const bool single = L == 1;
+    if (p.smoother == AXS_MG_SMOOTH_RBGS) {
+        ensureColoring();
+        for (int c = 0; c < cycles; c++) {
+            launchMGDefect(s, dev[0], p.weight, 0, ctl);
+            vcycleGS(s, 0, true, p, ctl);
+            launchMGApply(s, dev[0], ctl);
+        }
+        return;
+    }
     const int first = (single ? (!p.coarseDirect && p.coarseSweeps >= 1) : (p.pre >= 1)) ? 1 : 0;
     for (int c = 0; c < cycles; c++) {
         launchMGDefect(s, dev[0], p.weight, first, ctl);
@@ -361,6 +419,11 @@ void MGDevice::enqueueSolve(hipStream_t s, const MGParams& p, const Ctl* ctl, bo
     if (p.coarseDirect) throw std::domain_error("the FP64 coarsest direct solve is built for mixed precision only");
     if (buildOperators) this->buildOperators(s, ctl);
     const int cycles = p.cycles > 1 ? p.cycles : 1;  // cycleCount, multigrid.cuh:226
+    if (p.smoother == AXS_MG_SMOOTH_RBGS) {
+        ensureColoring();
+        for (int c = 0; c < cycles; c++) vcycleGS(s, 0, false, p, ctl);
+        return;  // x never leaves the level-0 buffer bound as x
+    }
     for (int c = 0; c < cycles; c++) vcycle(s, 0, 0, p, ctl);
     if (dev[0].x != x0Home_) {
         launchCopy(s, x0Home_, dev[0].x, dev[0].n, ctl);
@@ -390,6 +453,28 @@ double MGDevice::bytesPerSolve(const MGParams& p, bool buildOperators) const {
             bytes += 32.0 * f.n + 16.0 * f.refs() + 12.0 * X + (12.0 + w32) * c.n + (12.0 + w32) * c.refs();
         }
         if (mixed && p.coarseDirect) bytes += 8.0 * 3.0 * host.back().n * host.back().n;
+    }
+    if (p.smoother == AXS_MG_SMOOTH_RBGS) {
+        // a colour sweep pass reads AC, b, start, the colour order and the row (AF, nb)
+        // and updates x in place; a level held in one workgroup is read once
+        auto gs = [&](int l, int sweeps) {
+            if (sweeps <= 0) return 0.0;
+            const double pass = (4.0 * v + 8.0) * host[l].n + (v + 4.0) * host[l].refs();
+            return host[l].n <= kSmoothBlockGSMax ? pass : sweeps * pass;
+        };
+        double cyc = mixed ? 32.0 * host[0].n + 12.0 * host[0].refs() + 8.0 * host[0].n + 20.0 * host[0].n : 0.0;
+        for (int l = 0; l < L; l++) {
+            const double n = host[l].n, S = host[l].refs();
+            if (host[l].coarsest()) {
+                cyc += (mixed && p.coarseDirect) ? 8.0 * n * n + 2.0 * v * n : gs(l, p.coarseSweeps);
+                break;
+            }
+            const double nc = host[l + 1].n;
+            cyc += gs(l, p.pre) + gs(l, p.post);
+            cyc += (5.0 * v + 8.0) * n + (v + 4.0) * S + (2.0 * v + 4.0) * nc;  // residual + restriction
+            cyc += (2.0 * v + 4.0) * n + v * nc;                                  // prolongation
+        }
+        return bytes + (p.cycles > 1 ? p.cycles : 1) * cyc;
     }
     auto jac = [&](int l) { return (4.0 * v + 4.0) * host[l].n + (v + 4.0) * host[l].refs(); };
     auto sm = [&](int l, int sweeps) {
@@ -517,9 +602,25 @@ void Solver::setBC(int var, const axs_bc_view& v) {
     bc_[var] = std::move(h);
 }
 
+// The p' multigrid settings of a configuration (ConfigMultigrid, solver_struct.h:167-190)
+static MGParams mgParamsOf(const axs_config& c) {
+    MGParams mp;
+    mp.cycles = c.mg_max_iter;
+    mp.weight = c.mg_weight;
+    mp.pre = c.mg_pre_sweeps;
+    mp.post = c.mg_post_sweeps;
+    mp.coarseSweeps = c.mg_coarse_sweeps;
+    mp.precision = c.precision;
+    mp.coarseDirect = c.coarse_direct;
+    mp.smoother = c.mg_smoother;
+    return mp;
+}
+
 void Solver::configure(const axs_config& c) {
     if (running_) throw std::logic_error("solver is running");
     if (c.precision != AXS_PREC_FP64 && c.precision != AXS_PREC_MIXED) throw std::invalid_argument("bad precision");
+    if (c.mg_smoother != AXS_MG_SMOOTH_JACOBI && c.mg_smoother != AXS_MG_SMOOTH_RBGS)
+        throw std::invalid_argument("bad multigrid smoother");
     cfg_ = c;
     haveCfg_ = true;
 }
@@ -700,14 +801,7 @@ void Solver::enqueueIteration(hipStream_t s) {
     launchFaceFlux(s, dm_, bu, bv, bp, st, ph);
     launchFill(s, st.pp, 0.0, dm_.N, st.ctl);
 
-    MGParams mp;
-    mp.cycles = c.mg_max_iter;
-    mp.weight = c.mg_weight;
-    mp.pre = c.mg_pre_sweeps;
-    mp.post = c.mg_post_sweeps;
-    mp.coarseSweeps = c.mg_coarse_sweeps;
-    mp.precision = c.precision;
-    mp.coarseDirect = c.coarse_direct;
+    const MGParams mp = mgParamsOf(c);
     for (int corr = 0; corr <= ph.nonOrtho; corr++) {
         const int cross = corr > 0 ? 1 : 0;
         if (cross) launchGradField(s, dm_, ppbc_, st.pp, st.gPZ, st.gPR, ph.gradScheme, st.ctl);
@@ -756,6 +850,7 @@ void Solver::buildGraph() {
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(stream_, &g);
         if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();  // leave no capture error behind for the next call
         throw;
     }
     AXS_HIP(hipStreamEndCapture(stream_, &graph_));
@@ -838,6 +933,7 @@ void Solver::run(axs_progress_cb cb, void* user, bool continueRun) {
         mgBuiltLevels_ = c.mg_max_levels;
     }
     mg_.bindLevel0(ds_.ppAC, ds_.ppb, ds_.ppAF, ds_.pp, ds_.ppTmp);
+    if (c.use_multigrid && c.mg_smoother == AXS_MG_SMOOTH_RBGS) mg_.ensureColoring();  // no allocation in capture
     if (mixed) {
         allocateMixed();
         mg_.ensureMixed();
@@ -1194,13 +1290,7 @@ double Solver::bytesPerIteration() const {
     for (int p = 0; p < passes; p++) {
         b += 8.0 * F + 4.0 * S + 4.0 * N + 8.0 * N;                                          // pp rhs
         if (p > 0) b += gradGeom + 8.0 * N + 16.0 * N + 16.0 * N + 16.0 * F;                 // grad p', cross
-        MGParams mp;
-        mp.cycles = c.mg_max_iter;
-        mp.weight = c.mg_weight;
-        mp.pre = c.mg_pre_sweeps;
-        mp.post = c.mg_post_sweeps;
-        mp.coarseSweeps = c.mg_coarse_sweeps;
-        b += mg_.host.empty() ? 0.0 : mg_.bytesPerSolve(mp, p == 0);
+        b += mg_.host.empty() ? 0.0 : mg_.bytesPerSolve(mgParamsOf(c), p == 0);
     }
     b += gradGeom + 8.0 * N + 16.0 * N + 8.0 * 6 * N + 8.0 * 3 * N;                         // grad p' + update
     b += 8.0 * 10 * F + 8.0 * 4 * N + 8.0 * F;                                              // mass flux
@@ -1223,13 +1313,7 @@ void Solver::stats(axs_stats& s) const {
     s.solve_ms = solveMs_;
     for (int i = 0; i < 3; i++) s.last_residual[i] = history_[i].empty() ? 0.0 : history_[i].back();
     s.bytes_per_iteration = haveMesh_ ? bytesPerIteration() : 0.0;
-    MGParams mp;
-    mp.cycles = cfg_.mg_max_iter;
-    mp.weight = cfg_.mg_weight;
-    mp.pre = cfg_.mg_pre_sweeps;
-    mp.post = cfg_.mg_post_sweeps;
-    mp.coarseSweeps = cfg_.mg_coarse_sweeps;
-    s.vcycle_bytes = mg_.host.empty() ? 0.0 : mg_.bytesPerSolve(mp, true);
+    s.vcycle_bytes = mg_.host.empty() ? 0.0 : mg_.bytesPerSolve(mgParamsOf(cfg_), true);
 }
 
 }  // namespace axs

@@ -66,6 +66,7 @@ struct MGParams {
     int pre = 1, post = 1, coarseSweeps = 30;
     int precision = AXS_PREC_FP64;
     int coarseDirect = 0;
+    int smoother = AXS_MG_SMOOTH_JACOBI;
 };
 
 class MGDevice {
@@ -85,6 +86,9 @@ public:
     void bindLevel0(double* AC, double* b, double* AF, double* x, double* xNew);
     // Allocate the FP32 buffers of every level (idempotent).
     void ensureMixed();
+    // Colour every level for the Gauss-Seidel smoother (idempotent; allocates,
+    // so it runs before any graph capture).
+    void ensureColoring();
     // Enqueue one solve (enqueueRun, multigrid.cu:766-811).  Leaves the answer
     // in the level-0 buffer that was bound as x.
     void enqueueSolve(hipStream_t s, const MGParams& p, const Ctl* ctl, bool buildOperators);
@@ -116,6 +120,11 @@ private:
     void vcycle32(hipStream_t s, int l, int preDone, const MGParams& p, const Ctl* ctl);
     void smooth32(hipStream_t s, int l, int sweeps, const MGParams& p, const Ctl* ctl);
     void buildOperators(hipStream_t s, const Ctl* ctl);
+    // multi-colour Gauss-Seidel smoother (AXS_MG_SMOOTH_RBGS)
+    void smoothGS(hipStream_t s, int l, int sweeps, bool fp32, const Ctl* ctl);
+    void vcycleGS(hipStream_t s, int l, bool fp32, const MGParams& p, const Ctl* ctl);
+    std::vector<std::vector<int>> colorStart_;
+    std::vector<int*> dOrder_, dColorStart_;
     void swapX(int l);
     void swapX32(int l);
     double* x0Home_ = nullptr;

@@ -251,17 +251,18 @@ axs_status axs_mg_tail_info(const axs_mg* mg, int precision, int* first_level, i
 axs_status axs_mg_get_maps(const axs_mg* mg, int level, int* face_start, int* face_neighbor,
                            int* cell_to_coarse, int* fine_slot_to_coarse_slot);
 /* Solve A x = b with `cycles` V-cycles from the given x (host arrays, level-0
- * CSR order).  AC, AF, b, x as in Coefficients (solver_struct.h:103-120). */
+ * CSR order).  AC, AF, b, x as in Coefficients (solver_struct.h:103-120).
+ * smoother: AXS_MG_SMOOTH_* (the reference smooths with weighted Jacobi). */
 axs_status axs_mg_solve(axs_mg* mg, const double* AC, const double* AF, const double* b,
                         double* x, int cycles, double weight, int pre_sweeps, int post_sweeps,
-                        int coarse_sweeps, int precision, int coarse_direct);
+                        int coarse_sweeps, int precision, int coarse_direct, int smoother);
 /* Benchmark form: the system is uploaded once, then `repeats` solves of
  * `cycles` V-cycles are timed with HIP events (x restarts from zero each solve).
  * *ms = average time of one solve; *bytes = algorithmic bytes of one solve. */
 axs_status axs_mg_bench(axs_mg* mg, const double* AC, const double* AF, const double* b,
                         int cycles, double weight, int pre_sweeps, int post_sweeps,
-                        int coarse_sweeps, int precision, int coarse_direct, int repeats,
-                        double* ms, double* bytes, double* x_out);
+                        int coarse_sweeps, int precision, int coarse_direct, int smoother,
+                        int repeats, double* ms, double* bytes, double* x_out);
 
 /* Greedy colouring in cell order + colour-major cell order
  * (buildMeshColoring, memory_manger.cpp:334-409).  color_start needs

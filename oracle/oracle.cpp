@@ -294,6 +294,7 @@ struct MGLevel {
     Grid grid;
     Coeff coeff;
     std::vector<double> x, res;
+    std::vector<int> colorStart, cellOrder;  // greedy colouring (Gauss-Seidel smoother)
 };
 
 struct MGConfig {
@@ -301,6 +302,7 @@ struct MGConfig {
     double weight = 0.6;
     int linearSweep = 30;
     int pre = 1, post = 1;
+    int smoother = AXS_MG_SMOOTH_JACOBI;
 };
 
 // jacobiRow, multigrid.cu:447-484
@@ -329,6 +331,33 @@ void smoothen(MGLevel& L, double weight, int sweeps) {
         for (int n = 0; n < N; n++) jacobiRow(L.coeff, L.x, xNew, n, weight);
         L.x.swap(xNew);
     }
+}
+
+// Build extension (no reference counterpart): multi-colour Gauss-Seidel smoothing
+// of a level, colours from buildMeshColoring (memory_manger.cpp:334-409) of the
+// level's graph, each colour updated in place in colour-major cell order:
+// x_n = (b_n - sum AF x_nb) / AC_n, rows with |AC| < 1e-30 left unchanged (as
+// jacobiRow leaves them).
+void smoothenGS(MGLevel& L, int sweeps) {
+    const int N = L.grid.nCells;
+    if (N <= 0 || sweeps <= 0) return;
+    if (L.colorStart.empty())
+        buildColoring(N, L.grid.faceStart.data(), L.grid.faceNeighbor.data(), L.colorStart, L.cellOrder);
+    const Coeff& c = L.coeff;
+    const int nColors = (int)L.colorStart.size() - 1;
+    for (int s = 0; s < sweeps; s++)
+        for (int col = 0; col < nColors; col++)
+            for (int t = L.colorStart[col]; t < L.colorStart[col + 1]; t++) {
+                const int n = L.cellOrder[t];
+                const double AC = c.AC[n];
+                if (std::fabs(AC) < 1.0e-30) continue;
+                double val = c.b[n];
+                for (int k = c.faceStart[n]; k < c.faceStart[n + 1]; k++) {
+                    const int nb = c.faceNeighbor[k];
+                    if (nb >= 0) val -= c.AF[k] * L.x[nb];
+                }
+                L.x[n] = val / AC;
+            }
 }
 
 // residualRaw(sign = true, no scaling), residuals.cuh:17-48
@@ -397,11 +426,15 @@ struct Multigrid {
         const int nL = (int)levels.size();
         if (l < 0 || l >= nL) return;
         MGLevel& L = levels[l];
+        auto smooth = [&](int sweeps) {
+            if (cfg.smoother == AXS_MG_SMOOTH_RBGS) smoothenGS(L, sweeps);
+            else smoothen(L, cfg.weight, sweeps);
+        };
         if (L.grid.isCoarsest()) {
-            smoothen(L, cfg.weight, cfg.linearSweep);
+            smooth(cfg.linearSweep);
             return;
         }
-        smoothen(L, cfg.weight, cfg.pre);
+        smooth(cfg.pre);
         residualSigned(L.coeff, L.x, L.res);
         MGLevel& C = levels[l + 1];
         // buildRestriction, multigrid.cu:398-410, 568-582
@@ -411,7 +444,7 @@ struct Multigrid {
         vCycle(l + 1);
         // buildProlongationKernel, multigrid.cu:415-427
         for (int n = 0; n < L.grid.nCells; n++) L.x[n] += C.x[L.grid.cellToCoarse[n]];
-        smoothen(L, cfg.weight, cfg.post);
+        smooth(cfg.post);
     }
 
     // enqueueRun, multigrid.cu:766-811
@@ -1146,6 +1179,7 @@ struct Oracle {
             mg.cfg.linearSweep = cfg.mg_coarse_sweeps;
             mg.cfg.pre = cfg.mg_pre_sweeps;
             mg.cfg.post = cfg.mg_post_sweeps;
+            mg.cfg.smoother = cfg.mg_smoother;
         }
         const bool transient = cfg.transient != 0;
         const bool bdf2 = transient && cfg.time_scheme == AXS_TIME_BDF2;
@@ -1406,7 +1440,7 @@ int orc_mg_get_maps(const orc_mg* h, int l, int* fs, int* fn, int* c2c, int* s2s
     return AXS_OK;
 }
 int orc_mg_solve(orc_mg* h, const double* AC, const double* AF, const double* b, double* x, int cycles,
-                 double weight, int pre, int post, int coarse_sweeps) {
+                 double weight, int pre, int post, int coarse_sweeps, int smoother) {
     Multigrid& mg = h->mg;
     const int N = mg.levels[0].grid.nCells, S = mg.levels[0].grid.nRefs();
     Coeff A;
@@ -1422,6 +1456,7 @@ int orc_mg_solve(orc_mg* h, const double* AC, const double* AF, const double* b,
     mg.cfg.pre = pre;
     mg.cfg.post = post;
     mg.cfg.linearSweep = coarse_sweeps;
+    mg.cfg.smoother = smoother;
     mg.run(A, xv);
     std::memcpy(x, xv.data(), N * sizeof(double));
     return AXS_OK;

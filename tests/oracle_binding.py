@@ -43,7 +43,8 @@ def olib() -> C.CDLL:
             "orc_mg_levels": (C.c_int, [V]),
             "orc_mg_level_info": (C.c_int, [V, C.c_int, _ip, _ip]),
             "orc_mg_get_maps": (C.c_int, [V, C.c_int, _ip, _ip, _ip, _ip]),
-            "orc_mg_solve": (C.c_int, [V, _dp, _dp, _dp, _dp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int]),
+            "orc_mg_solve": (C.c_int, [V, _dp, _dp, _dp, _dp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
+                                       C.c_int]),
             "orc_mg_get_operator": (C.c_int, [V, C.c_int, _dp, _dp]),
             "orc_residual_value": (None, [C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_int, _dp, _dp]),
         }
@@ -177,10 +178,10 @@ class OracleMG:
                                _i(s2s) if s2s.size else None)
         return {"face_start": fs, "face_neighbor": fn, "cell_to_coarse": c2c, "slot_to_coarse_slot": s2s}
 
-    def solve(self, AC, AF, b, x, cycles=3, weight=0.6, pre=1, post=1, coarse=30) -> np.ndarray:
+    def solve(self, AC, AF, b, x, cycles=3, weight=0.6, pre=1, post=1, coarse=30, smoother=0) -> np.ndarray:
         AC, AF, b = (np.ascontiguousarray(a, dtype=np.float64) for a in (AC, AF, b))
         x = np.array(x, dtype=np.float64, copy=True)
-        olib().orc_mg_solve(self._h, _d(AC), _d(AF), _d(b), _d(x), cycles, weight, pre, post, coarse)
+        olib().orc_mg_solve(self._h, _d(AC), _d(AF), _d(b), _d(x), cycles, weight, pre, post, coarse, smoother)
         return x
 
     def operator(self, l: int):

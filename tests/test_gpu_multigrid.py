@@ -85,3 +85,22 @@ def test_vcycles_match_textbook_on_nozzle(require_gpu):
     # and against the oracle bit for bit
     xr = OracleMG(fs, fn).solve(AC, AF, b, np.zeros_like(b), cycles=3)
     np.testing.assert_array_equal(x, xr)
+
+
+@pytest.mark.parametrize("which", ["pipe", "nozzle"])
+@pytest.mark.parametrize("sweeps", [(1, 1, 30), (2, 0, 3)], ids=lambda s: "pre%d_post%d_coarse%d" % s)
+def test_gauss_seidel_vcycle_bit_exact(require_gpu, which, sweeps):
+    """Multi-colour Gauss-Seidel smoothing (per-colour launches on large levels,
+    one workgroup on small ones) against the oracle, bit for bit."""
+    pre, post, coarse = sweeps
+    case = axs.pipe_case() if which == "pipe" else axs.nozzle_case(mesh=axs.nozzle_mesh(spacing=1.1e-4))
+    fs, fn, AC, AF, b = pp_system(case)
+    g = axs.Multigrid(fs, fn)
+    x0 = np.linspace(-1.0, 1.0, b.size) * 1e-3
+    xg = g.solve(AC, AF, b, x0, cycles=2, pre=pre, post=post, coarse=coarse, smoother=axs.MG_SMOOTH_RBGS)
+    xr = OracleMG(fs, fn).solve(AC, AF, b, x0, cycles=2, pre=pre, post=post, coarse=coarse, smoother=1)
+    np.testing.assert_array_equal(xg, xr)
+    # the FP32 cycles in FP64 refinement stay on the FP64 iterate
+    x32 = g.solve(AC, AF, b, x0, cycles=2, pre=pre, post=post, coarse=coarse, smoother=axs.MG_SMOOTH_RBGS,
+                  precision=axs.PREC_MIXED)
+    assert np.linalg.norm(x32 - xr) / np.linalg.norm(xr) < 1e-5

@@ -83,6 +83,21 @@ def test_pipe_c1_converged(require_gpu):
     assert np.sqrt(np.mean((u[sel] - ua) ** 2)) / 0.01 < 0.005
 
 
+def test_pipe_c1_gauss_seidel_multigrid(require_gpu):
+    """C1 with the multi-colour Gauss-Seidel multigrid smoother (build
+    extension): per-iteration state against the oracle's restatement of the same
+    smoother, then convergence within one iteration of it."""
+    s, o = run_both(axs.pipe_case(mg_smoother=axs.MG_SMOOTH_RBGS), iters=20)
+    compare(s, o, STATE_FIELDS, 1e-12)
+    case = axs.pipe_case(mg_smoother=axs.MG_SMOOTH_RBGS)
+    s, o = run_both(case)
+    st = s.stats()
+    print("iterations GS-MG", st.iterations, "oracle", o.iterations)
+    assert o.converged and st.converged
+    assert abs(st.iterations - o.iterations) <= 1, (st.iterations, o.iterations)
+    compare(s, o, ["u", "v", "p", "mdot"], 1e-8)
+
+
 @pytest.mark.parametrize("scheme", [axs.CONV_CENTRAL, axs.CONV_SOU])
 @pytest.mark.parametrize("grad", [axs.GRAD_GREEN_GAUSS, axs.GRAD_LSQ])
 def test_schemes_small_pipe(require_gpu, scheme, grad):

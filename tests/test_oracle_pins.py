@@ -437,6 +437,28 @@ def test_vcycles_converge_to_direct_solve():
     np.testing.assert_allclose(x, ref, rtol=1e-6, atol=1e-9 * np.abs(ref).max())
 
 
+def test_gauss_seidel_multigrid_is_textbook_and_converges():
+    """The multi-colour Gauss-Seidel smoother (build extension) against a
+    textbook V-cycle that visits each colour's cells in the oracle's colour
+    order with dense row updates, and its fixed point against the direct solve."""
+    from mg_textbook import TextbookMG, spmat
+
+    o = _run(axs.pipe_case(nz=32, nr=8), 2)
+    fs, fn = o.csr()
+    AC, AF, b = o.field("pp_ac") * 1.02, o.field("pp_af"), o.field("pp_b")
+    mg = OracleMG(fs, fn)
+    maps = [mg.maps(l) for l in range(mg.levels)]
+    colorings = [oracle_coloring(m["face_start"], m["face_neighbor"]) for m in maps]
+    tb = TextbookMG(spmat(fs, fn, AC, AF), [m["cell_to_coarse"] for m in maps[:-1]], colorings)
+    for pre, post, coarse in [(1, 1, 30), (2, 0, 3)]:
+        got = mg.solve(AC, AF, b, np.zeros_like(b), cycles=2, pre=pre, post=post, coarse=coarse, smoother=1)
+        ref = tb.solve(b, np.zeros_like(b), 2, gs=True, pre=pre, post=post, coarse=coarse)
+        np.testing.assert_allclose(got, ref, rtol=1e-11, atol=1e-13 * np.abs(ref).max())
+    x = mg.solve(AC, AF, b, np.zeros_like(b), cycles=150, smoother=1)
+    direct = np.linalg.solve(dense(fs, fn, AC, AF), b)
+    np.testing.assert_allclose(x, direct, rtol=1e-6, atol=1e-9 * np.abs(direct).max())
+
+
 # ---------------------------------------------------------------- residual norms
 def test_residual_norm_closed_forms():
     res = np.array([1.0, -2.0, 3.0, -4.0])
