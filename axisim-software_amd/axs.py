@@ -45,6 +45,7 @@ FIELDS = {
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 _u8p = C.POINTER(C.c_uint8)
+_fp = C.POINTER(C.c_float)
 
 
 class MeshView(C.Structure):
@@ -153,6 +154,17 @@ def load_library(path: os.PathLike | None = None) -> C.CDLL:
         "axs_mg_bench": (C.c_int, [S, _dp, _dp, _dp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int,
                                    C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]),
         "axs_build_coloring": (C.c_int, [C.c_int, _ip, _ip, _ip, _ip, _ip]),
+        "axs_net_create": (C.c_int, [C.POINTER(S), C.c_int, C.c_uint64, C.c_int]),
+        "axs_net_destroy": (None, [S]),
+        "axs_net_num_params": (C.c_int, [S]),
+        "axs_net_get_params": (C.c_int, [S, _fp]),
+        "axs_net_set_params": (C.c_int, [S, _fp]),
+        "axs_net_add_sample": (C.c_int, [S, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp, _ip]),
+        "axs_net_clear_samples": (C.c_int, [S]),
+        "axs_net_loss_grad": (C.c_int, [S, C.c_int, _dp, _fp]),
+        "axs_net_train": (C.c_int, [S, C.c_int, C.c_int, C.c_double, C.c_uint64, _dp]),
+        "axs_net_predict": (C.c_int, [S, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp]),
+        "axs_set_learned_guess": (C.c_int, [S, S, C.c_int]),
         "axs_mesh_multiblock": (C.c_int, [C.POINTER(S), C.c_int, C.POINTER(BlockSpec)]),
         "axs_mesh_triangles": (C.c_int, [C.POINTER(S), C.c_int, _dp, _dp, C.c_int, _ip, C.c_int, _dp, _dp, _dp,
                                          _dp, _ip, C.c_double]),
@@ -196,6 +208,10 @@ def _dptr(a: np.ndarray):
 
 def _iptr(a: np.ndarray):
     return a.ctypes.data_as(_ip)
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(_fp)
 
 
 # ---------------------------------------------------------------------------
@@ -446,6 +462,12 @@ class Solver:
         _check(lib().axs_run_timed(self._h, n, C.byref(ms)), self._h)
         return ms.value
 
+    def set_learned_guess(self, net: "Net | None", level: int = -1) -> None:
+        """Learned initial guess for the first p' solve of each outer iteration
+        (None: off).  level: multigrid level the net runs on (-1: automatic)."""
+        self._net = net
+        _check(lib().axs_set_learned_guess(self._h, net._h if net is not None else None, level), self._h)
+
     def bench_kernel(self, kernel: int, repeats: int = 50) -> tuple[float, float]:
         """(average launch ms, algorithmic bytes per launch) of one kernel."""
         ms, by = C.c_double(), C.c_double()
@@ -545,6 +567,70 @@ class Multigrid:
                                   precision, coarse_direct, smoother, repeats, C.byref(ms), C.byref(by),
                                   _dptr(x)))
         return ms.value, by.value, x
+
+
+class Net:
+    """Message-passing network for the learned initial guess, trained on the GPU."""
+
+    def __init__(self, layers: int = 4, seed: int = 0, device: int = 0):
+        self._h = C.c_void_p()
+        _check(lib().axs_net_create(C.byref(self._h), layers, seed, device))
+        self.layers = layers
+
+    def close(self):
+        if self._h:
+            lib().axs_net_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def num_params(self) -> int:
+        return lib().axs_net_num_params(self._h)
+
+    def get_params(self) -> np.ndarray:
+        p = np.zeros(self.num_params, np.float32)
+        _check(lib().axs_net_get_params(self._h, _fptr(p)))
+        return p
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.num_params
+        _check(lib().axs_net_set_params(self._h, _fptr(p)))
+
+    def add_sample(self, fs, fn, AC, AF, b, x) -> int:
+        fs, fn = (np.ascontiguousarray(a, dtype=np.int32) for a in (fs, fn))
+        AC, AF, b, x = (np.ascontiguousarray(a, dtype=np.float64) for a in (AC, AF, b, x))
+        idx = C.c_int()
+        _check(lib().axs_net_add_sample(self._h, fs.size - 1, _iptr(fs), _iptr(fn), _dptr(AC), _dptr(AF),
+                                        _dptr(b), _dptr(x), C.byref(idx)))
+        return idx.value
+
+    def clear_samples(self) -> None:
+        _check(lib().axs_net_clear_samples(self._h))
+
+    def loss_grad(self, sample: int):
+        loss = C.c_double()
+        g = np.zeros(self.num_params, np.float32)
+        _check(lib().axs_net_loss_grad(self._h, sample, C.byref(loss), _fptr(g)))
+        return loss.value, g
+
+    def train(self, epochs: int, batch: int = 8, lr: float = 1e-3, seed: int = 0) -> np.ndarray:
+        losses = np.zeros(epochs)
+        _check(lib().axs_net_train(self._h, epochs, batch, lr, seed, _dptr(losses)))
+        return losses
+
+    def predict(self, fs, fn, AC, AF, b) -> np.ndarray:
+        fs, fn = (np.ascontiguousarray(a, dtype=np.int32) for a in (fs, fn))
+        AC, AF, b = (np.ascontiguousarray(a, dtype=np.float64) for a in (AC, AF, b))
+        x = np.zeros(fs.size - 1)
+        _check(lib().axs_net_predict(self._h, fs.size - 1, _iptr(fs), _iptr(fn), _dptr(AC), _dptr(AF), _dptr(b),
+                                     _dptr(x)))
+        return x
 
 
 def coloring(face_start: np.ndarray, face_neighbor: np.ndarray):

@@ -26,7 +26,9 @@ ORACLE_LIB = ORACLE_DIR / "liboracle.so"
 SOURCES = [
     "csrc/hip/kernels.hip",
     "csrc/hip/kernels_mixed.hip",
+    "csrc/hip/mpnn.hip",
     "csrc/host/solver.cpp",
+    "csrc/host/net.cpp",
     "csrc/host/capi.cpp",
     "csrc/host/topology.cpp",
     "csrc/host/casegen.cpp",

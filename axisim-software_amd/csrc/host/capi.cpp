@@ -13,6 +13,7 @@
 
 #include "axs.h"
 #include "mesh_types.hpp"
+#include "net.hpp"
 #include "solver.hpp"
 #include "topology.hpp"
 
@@ -33,6 +34,10 @@ struct axs_mg {
 
 struct axs_mesh {
     HostMesh m;
+};
+
+struct axs_net {
+    Mpnn* net = nullptr;
 };
 
 namespace {
@@ -418,5 +423,77 @@ axs_status axs_mesh_get_triangles(const axs_mesh* h, int* n_points, double* pz, 
 }
 
 void axs_mesh_destroy(axs_mesh* h) { delete h; }
+
+// ---------------- learned initial guess ----------------
+axs_status axs_net_create(axs_net** out, int layers, uint64_t seed, int device) {
+    if (!out) return AXS_ERR_INVALID;
+    *out = nullptr;
+    return guarded(nullptr, [&] {
+        int count = 0;
+        AXS_HIP(hipGetDeviceCount(&count));
+        if (device < 0 || device >= count) throw std::invalid_argument("no such device");
+        auto* h = new axs_net();
+        try {
+            h->net = new Mpnn(layers, seed, device);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+void axs_net_destroy(axs_net* h) {
+    if (!h) return;
+    delete h->net;
+    delete h;
+}
+
+int axs_net_num_params(const axs_net* h) { return h ? h->net->numParams() : 0; }
+
+axs_status axs_net_get_params(const axs_net* h, float* out) {
+    if (!h || !out) return AXS_ERR_INVALID;
+    return guarded(nullptr, [&] { h->net->getParams(out); });
+}
+
+axs_status axs_net_set_params(axs_net* h, const float* in) {
+    if (!h || !in) return AXS_ERR_INVALID;
+    return guarded(nullptr, [&] { h->net->setParams(in); });
+}
+
+axs_status axs_net_add_sample(axs_net* h, int n, const int* fs, const int* fn, const double* AC, const double* AF,
+                              const double* b, const double* x, int* index) {
+    if (!h) return AXS_ERR_INVALID;
+    return guarded(nullptr, [&] {
+        const int i = h->net->addSample(n, fs, fn, AC, AF, b, x);
+        if (index) *index = i;
+    });
+}
+
+axs_status axs_net_clear_samples(axs_net* h) {
+    if (!h) return AXS_ERR_INVALID;
+    return guarded(nullptr, [&] { h->net->clearSamples(); });
+}
+
+axs_status axs_net_loss_grad(axs_net* h, int sample, double* loss, float* grad) {
+    if (!h || !loss) return AXS_ERR_INVALID;
+    return guarded(nullptr, [&] { *loss = h->net->lossGrad(sample, grad); });
+}
+
+axs_status axs_net_train(axs_net* h, int epochs, int batch, double lr, uint64_t seed, double* loss_per_epoch) {
+    if (!h) return AXS_ERR_INVALID;
+    return guarded(nullptr, [&] { h->net->train(epochs, batch, lr, seed, loss_per_epoch); });
+}
+
+axs_status axs_net_predict(axs_net* h, int n, const int* fs, const int* fn, const double* AC, const double* AF,
+                           const double* b, double* x) {
+    if (!h) return AXS_ERR_INVALID;
+    return guarded(nullptr, [&] { h->net->predict(n, fs, fn, AC, AF, b, x); });
+}
+
+axs_status axs_set_learned_guess(axs_solver* h, axs_net* net, int level) {
+    if (!h) return AXS_ERR_INVALID;
+    return guarded(&h->s->lastError, [&] { h->s->setLearnedGuess(net ? net->net : nullptr, level); });
+}
 
 }  // extern "C"

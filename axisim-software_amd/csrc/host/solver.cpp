@@ -10,6 +10,8 @@
 // runs past the one the reference would stop at.
 #include "solver.hpp"
 
+#include "net.hpp"
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -377,11 +379,7 @@ void MGDevice::enqueueSolveMixed(hipStream_t s, const MGParams& p, const Ctl* ct
     const int L = (int)host.size();
     if (p.coarseDirect && coarsestCells() > kDirectMax)
         throw std::domain_error("coarsest level too large for the dense FP64 solve");
-    if (buildOperators) {
-        launchMGCast(s, dev[0], ctl);
-        this->buildOperators(s, ctl);  // also writes the FP32 copies
-        if (p.coarseDirect) launchCoarseInvert(s, dev[L - 1], aug, inv, invStatus, ctl);
-    }
+    if (buildOperators) enqueueOperators(s, p, ctl);
     const int cycles = p.cycles > 1 ? p.cycles : 1;
     // the first sweep from e = 0 is fused into the defect kernel
     const bool single = L == 1;
@@ -402,6 +400,17 @@ void MGDevice::enqueueSolveMixed(hipStream_t s, const MGParams& p, const Ctl* ct
     }
 }
 
+void MGDevice::enqueueOperators(hipStream_t s, const MGParams& p, const Ctl* ctl) {
+    if (p.precision == AXS_PREC_MIXED) {
+        ensureMixed();
+        launchMGCast(s, dev[0], ctl);
+        buildOperators(s, ctl);  // also writes the FP32 copies
+        if (p.coarseDirect) launchCoarseInvert(s, dev[host.size() - 1], aug, inv, invStatus, ctl);
+    } else {
+        buildOperators(s, ctl);
+    }
+}
+
 // Galerkin coarse operators of every level (buildCoarseOperator,
 // multigrid.cu:350-389), each in two coalesced passes.
 void MGDevice::buildOperators(hipStream_t s, const Ctl* ctl) {
@@ -417,7 +426,7 @@ void MGDevice::enqueueSolve(hipStream_t s, const MGParams& p, const Ctl* ctl, bo
         return;
     }
     if (p.coarseDirect) throw std::domain_error("the FP64 coarsest direct solve is built for mixed precision only");
-    if (buildOperators) this->buildOperators(s, ctl);
+    if (buildOperators) enqueueOperators(s, p, ctl);
     const int cycles = p.cycles > 1 ? p.cycles : 1;  // cycleCount, multigrid.cuh:226
     if (p.smoother == AXS_MG_SMOOTH_RBGS) {
         ensureColoring();
@@ -531,6 +540,8 @@ Solver::~Solver() {
     if (graph_) (void)hipGraphDestroy(graph_);
     if (hctl_) (void)hipHostFree(hctl_);
     if (hhist_) (void)hipHostFree(hhist_);
+    delete netSetup_;
+    netSetup_ = nullptr;
     mg_.arena.release();
     stateArena_.release();
     meshArena_.release();
@@ -799,14 +810,19 @@ void Solver::enqueueIteration(hipStream_t s) {
 
     launchPPCoeff(s, dm_, bp, st, ph);
     launchFaceFlux(s, dm_, bu, bv, bp, st, ph);
-    launchFill(s, st.pp, 0.0, dm_.N, st.ctl);
+    const bool guess = c.use_multigrid && net_ != nullptr && netLevel_ > 0;
+    if (!guess) launchFill(s, st.pp, 0.0, dm_.N, st.ctl);
 
     const MGParams mp = mgParamsOf(c);
     for (int corr = 0; corr <= ph.nonOrtho; corr++) {
         const int cross = corr > 0 ? 1 : 0;
         if (cross) launchGradField(s, dm_, ppbc_, st.pp, st.gPZ, st.gPR, ph.gradScheme, st.ctl);
         launchPPRhs(s, dm_, st, ph, cross);
-        if (c.use_multigrid) {
+        if (c.use_multigrid && corr == 0 && guess) {
+            // operators, then the learned guess in place of p' = 0, then the cycles
+            enqueueGuess(s, mp);
+            mg_.enqueueSolve(s, mp, st.ctl, false);
+        } else if (c.use_multigrid) {
             // the operator is the same for every corrector pass: build it once
             mg_.enqueueSolve(s, mp, st.ctl, corr == 0);
         } else {
@@ -934,6 +950,7 @@ void Solver::run(axs_progress_cb cb, void* user, bool continueRun) {
     }
     mg_.bindLevel0(ds_.ppAC, ds_.ppb, ds_.ppAF, ds_.pp, ds_.ppTmp);
     if (c.use_multigrid && c.mg_smoother == AXS_MG_SMOOTH_RBGS) mg_.ensureColoring();  // no allocation in capture
+    setupGuess();
     if (mixed) {
         allocateMixed();
         mg_.ensureMixed();
@@ -1296,6 +1313,75 @@ double Solver::bytesPerIteration() const {
     b += 8.0 * 10 * F + 8.0 * 4 * N + 8.0 * F;                                              // mass flux
     b += 8.0 * 6 * N + 12.0 * S + 8.0 * F + 4.0 * S;                                        // residuals
     return b;
+}
+
+// ---------------------------------------------------------------------------
+// learned initial guess (build extension)
+// ---------------------------------------------------------------------------
+struct Solver::NetSetup {
+    NetGraphDev graph;
+    NetWork work;
+    Arena arena;
+    std::vector<double*> vec;  // per level 1..netLevel_: restricted b, later the prolonged guess
+    double* x = nullptr;       // the guess on netLevel_
+};
+
+void Solver::setLearnedGuess(Mpnn* net, int level) {
+    if (running_) throw std::logic_error("solver is running");
+    if (level < -1 || level == 0) throw std::invalid_argument("guess level must be -1 or a coarse level >= 1");
+    net_ = net;
+    netLevelReq_ = level;
+    netLevel_ = -1;
+}
+
+// Allocates everything the guess needs before the iteration graph is captured.
+void Solver::setupGuess() {
+    netLevel_ = -1;
+    delete netSetup_;
+    netSetup_ = nullptr;
+    if (!net_ || !cfg_.use_multigrid) return;
+    const int L = (int)mg_.host.size();
+    int k = -1;
+    if (netLevelReq_ >= 1) {
+        if (netLevelReq_ >= L) throw std::out_of_range("guess level beyond the multigrid hierarchy");
+        k = netLevelReq_;
+    } else {
+        for (int l = 1; l < L; l++)
+            if (mg_.host[l].n <= kGuessCells) {
+                k = l;
+                break;
+            }
+    }
+    if (k < 1) return;  // a single level: nothing coarser to run on
+    netSetup_ = new NetSetup();
+    netSetup_->graph.build(mg_.host[k].n, mg_.host[k].start, mg_.host[k].nb);
+    netSetup_->work.ensure(netSetup_->graph.g.nPad, net_->layers());
+    netSetup_->vec.assign(k + 1, nullptr);
+    for (int l = 1; l <= k; l++) netSetup_->vec[l] = netSetup_->arena.zeros<double>(mg_.host[l].n);
+    netSetup_->x = netSetup_->arena.zeros<double>(mg_.host[k].n);
+    netLevel_ = k;
+}
+
+// b restricted to level k (sums over aggregates), the net's guess there, and
+// that guess prolonged (piecewise constant) to p' on level 0.
+void Solver::enqueueGuess(hipStream_t s, const MGParams& mp) {
+    const int k = netLevel_;
+    const Ctl* ctl = ds_.ctl;
+    NetSetup& ns = *netSetup_;
+    mg_.enqueueOperators(s, mp, ctl);
+    const double* in = ds_.ppb;
+    for (int l = 0; l < k; l++) {
+        const DLevel& d = mg_.dev[l];
+        netLaunchRestrict(s, mg_.host[l + 1].n, d.memberStart, d.members, in, ns.vec[l + 1], ctl);
+        in = ns.vec[l + 1];
+    }
+    net_->enqueueForward(s, ns.graph.g, mg_.dev[k].AC, mg_.dev[k].AF, ns.vec[k], ns.x, ns.work);
+    const double* src = ns.x;
+    for (int l = k - 1; l >= 0; l--) {
+        double* out = l == 0 ? ds_.pp : ns.vec[l];
+        netLaunchProlong(s, mg_.host[l].n, mg_.dev[l].toCoarse, src, out, ctl);
+        src = out;
+    }
 }
 
 void Solver::stats(axs_stats& s) const {

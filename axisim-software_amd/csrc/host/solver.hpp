@@ -10,6 +10,7 @@
 
 #include "../hip/device.hpp"
 #include "../hip/mixed.hpp"
+#include "../hip/mpnn.hpp"
 #include "axs.h"
 #include "mesh_types.hpp"
 #include "topology.hpp"
@@ -92,6 +93,9 @@ public:
     // Enqueue one solve (enqueueRun, multigrid.cu:766-811).  Leaves the answer
     // in the level-0 buffer that was bound as x.
     void enqueueSolve(hipStream_t s, const MGParams& p, const Ctl* ctl, bool buildOperators);
+    // Only the operator part of a solve: Galerkin operators of every level (and
+    // their FP32 copies and the coarsest inverse in mixed precision).
+    void enqueueOperators(hipStream_t s, const MGParams& p, const Ctl* ctl);
     double bytesPerSolve(const MGParams& p, bool buildOperators) const;
     int coarsestCells() const { return host.empty() ? 0 : host.back().n; }
     // first level of the LDS-resident tail (-1: no tail) and its LDS bytes
@@ -150,6 +154,11 @@ public:
     size_t fieldLength(int field) const;
     int residualHistory(int res, double* out, int maxN) const;
     void stats(axs_stats& s) const;
+    // Learned initial guess of the first p' solve of every outer iteration
+    // (net null: off).  level: multigrid level the net runs on, -1 = the first
+    // one with at most kGuessCells cells.
+    void setLearnedGuess(class Mpnn* net, int level);
+    static constexpr int kGuessCells = 16384;
 
     std::string lastError;
 
@@ -197,6 +206,14 @@ private:
     void allocateMixed();
 
     MGDevice mg_;
+    // learned guess: net, its level, the level's graph, work buffers and the
+    // restricted right-hand sides / prolonged guesses of levels 1..level
+    class Mpnn* net_ = nullptr;
+    int netLevelReq_ = -1, netLevel_ = -1;
+    struct NetSetup;
+    NetSetup* netSetup_ = nullptr;
+    void setupGuess();
+    void enqueueGuess(hipStream_t s, const MGParams& mp);
     bool mgBuilt_ = false;
     int mgBuiltLevels_ = -1;
     std::vector<int> colorStart_;

@@ -264,6 +264,47 @@ axs_status axs_mg_bench(axs_mg* mg, const double* AC, const double* AF, const do
                         int coarse_sweeps, int precision, int coarse_direct, int smoother,
                         int repeats, double* ms, double* bytes, double* x_out);
 
+/* ---------------- Learned initial guess (build extension) ----------------
+ * No counterpart in the reference.  A message-passing network on a cell graph
+ * (Jacobi-scaled operator weights, hidden width 64, BF16 weights, FP32
+ * accumulate on the matrix cores) maps the right-hand side b of A x = b to a
+ * guess of x.  It is trained on the GPU by its own forward and backward kernels
+ * on (system, solution) pairs, and in the solver it replaces p' = 0 before the
+ * first pressure-correction solve of every outer iteration, run on one coarse
+ * multigrid level and prolonged to the mesh.  The guess is alpha x with
+ * alpha = x.b / x.Ax, the multiple of the network's direction x closest to the
+ * solution in the energy norm, so it never starts the solve further from it
+ * than zero does; the multigrid cycles then run as configured, and the answer
+ * is still the multigrid's.  Parameters are FP32 in
+ * the order W_1 (32 x 64), W_2 .. W_L (128 x 64, row-major, input-major), w_out (64). */
+typedef struct axs_net axs_net;
+axs_status axs_net_create(axs_net** out, int layers, uint64_t seed, int device);
+void axs_net_destroy(axs_net* net);
+int axs_net_num_params(const axs_net* net);
+axs_status axs_net_get_params(const axs_net* net, float* out);
+axs_status axs_net_set_params(axs_net* net, const float* in);
+/* A training pair, stored on the device: CSR graph (face_neighbor -1 on boundary
+ * slots), operator AC/AF, right-hand side b and the solution x of A x = b.
+ * *index (may be NULL) receives its sample number. */
+axs_status axs_net_add_sample(axs_net* net, int n_cells, const int* face_start, const int* face_neighbor,
+                              const double* AC, const double* AF, const double* b, const double* x,
+                              int* index);
+axs_status axs_net_clear_samples(axs_net* net);
+/* Loss of one sample, 1 - (x.b)^2 / (x.Ax * b.x*): the share of the solution's
+ * energy |x*|_A^2 = b.x* the guess alpha x leaves uncaptured; and its gradient
+ * (num_params FP32, may be NULL). */
+axs_status axs_net_loss_grad(axs_net* net, int sample, double* loss, float* grad);
+/* Adam over all samples, `batch` per step; loss_per_epoch (may be NULL) gets
+ * the mean sample loss of each epoch. */
+axs_status axs_net_train(axs_net* net, int epochs, int batch, double lr, uint64_t seed, double* loss_per_epoch);
+/* The guess x for one system (host arrays). */
+axs_status axs_net_predict(axs_net* net, int n_cells, const int* face_start, const int* face_neighbor,
+                           const double* AC, const double* AF, const double* b, double* x);
+/* Use `net` for the first p' solve of every outer iteration (NULL: off, the
+ * default).  level: multigrid level it runs on, -1 = the first with at most
+ * 16384 cells.  The net must outlive its use by the solver. */
+axs_status axs_set_learned_guess(axs_solver* s, axs_net* net, int level);
+
 /* Greedy colouring in cell order + colour-major cell order
  * (buildMeshColoring, memory_manger.cpp:334-409).  color_start needs
  * n_cells + 1 entries; *n_colors receives the colour count. */
