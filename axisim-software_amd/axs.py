@@ -170,6 +170,7 @@ def load_library(path: os.PathLike | None = None) -> C.CDLL:
                                          _dp, _ip, C.c_double]),
         "axs_mesh_nozzle": (C.c_int, [C.POINTER(S), C.POINTER(NozzleSpec)]),
         "axs_mesh_get_view": (C.c_int, [S, C.POINTER(MeshView)]),
+        "axs_mesh_concat": (C.c_int, [C.POINTER(S), C.c_int, C.POINTER(MeshView), C.c_int]),
         "axs_mesh_get_triangles": (C.c_int, [S, _ip, _dp, _dp, _ip, _ip]),
         "axs_mesh_destroy": (None, [S]),
     }
@@ -729,6 +730,49 @@ def nozzle_case(cells: float = 1.0e6, spacing: float | None = None, re_throat=50
     for k, v in cfg.items():
         setattr(c, k, v)
     return Case("nozzle", mesh, bcs, c, dict(u_in=u_in, u_throat=u_throat))
+
+
+def concat_meshes(meshes: list[Mesh], groups_per_mesh: int = 4) -> Mesh:
+    """Disjoint union of meshes for a batched run (mesh c's boundary group g
+    becomes g + c * groups_per_mesh)."""
+    views = (MeshView * len(meshes))()
+    for i, m in enumerate(meshes):
+        views[i] = m.view()
+    h = C.c_void_p()
+    _check(lib().axs_mesh_concat(C.byref(h), len(meshes), views, groups_per_mesh))
+    return _mesh_from_handle(h)  # copies the arrays and releases the handle
+
+
+def batch_case(cases: list[Case], groups_per_mesh: int = 4, **cfg) -> Case:
+    """Many cases as one: the union mesh, every case's BCs on its own groups, the
+    configuration of the first case (overridden by cfg).  info["ranges"] holds
+    each case's (first cell, cell count) in the union."""
+    mesh = concat_meshes([c.mesh for c in cases], groups_per_mesh)
+    bcs = {}
+    for var in (VAR_U, VAR_V, VAR_P):
+        cols = {k: [] for k in ("type", "value", "length", "boundary_type", "amplitude", "frequency")}
+        for c in cases:
+            bc = c.bcs[var]
+            n = len(bc.type)
+            if n > groups_per_mesh:
+                raise ValueError("a case has more boundary groups than groups_per_mesh")
+            pad = groups_per_mesh - n
+            for k, default in (("type", BC_NONE), ("value", 0.0), ("length", 0.0), ("boundary_type", 0),
+                               ("amplitude", 0.0), ("frequency", 0.0)):
+                vals = getattr(bc, k)
+                cols[k] += (list(vals) if vals is not None else [default] * n) + [default] * pad
+        bcs[var] = BC(cols["type"], cols["value"], length=cols["length"], boundary_type=cols["boundary_type"],
+                      amplitude=cols["amplitude"], frequency=cols["frequency"])
+    c0 = cases[0].config
+    config = Config()
+    C.pointer(config)[0] = c0
+    for k, v in cfg.items():
+        setattr(config, k, v)
+    ranges, off = [], 0
+    for c in cases:
+        ranges.append((off, c.mesh.n_cells))
+        off += c.mesh.n_cells
+    return Case("batch", mesh, bcs, config, dict(ranges=ranges, names=[c.name for c in cases]))
 
 
 def make_solver(case: Case, device: int = 0) -> Solver:

@@ -406,6 +406,20 @@ axs_status axs_mesh_nozzle(axs_mesh** out, const axs_nozzle_spec* spec) {
     });
 }
 
+axs_status axs_mesh_concat(axs_mesh** out, int count, const axs_mesh_view* meshes, int groups_per_mesh) {
+    if (!out || !meshes || count <= 0) return AXS_ERR_INVALID;
+    *out = nullptr;
+    return guarded(nullptr, [&] {
+        axs_mesh* h = new axs_mesh();
+        const std::string err = concatMeshes(h->m, count, meshes, groups_per_mesh);
+        if (!err.empty()) {
+            delete h;
+            throw std::invalid_argument(err);
+        }
+        *out = h;
+    });
+}
+
 axs_status axs_mesh_get_view(const axs_mesh* h, axs_mesh_view* v) {
     if (!h || !v) return AXS_ERR_INVALID;
     *v = h->m.view();

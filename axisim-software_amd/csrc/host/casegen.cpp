@@ -738,3 +738,53 @@ std::string buildNozzle(HostMesh& out, const axs_nozzle_spec& sp) {
 }
 
 }  // namespace axs
+
+namespace axs {
+
+std::string concatMeshes(HostMesh& out, int count, const axs_mesh_view* meshes, int groupsPerMesh) {
+    if (count <= 0 || !meshes) return "no meshes to concatenate";
+    if (groupsPerMesh <= 0) return "groups per mesh must be positive";
+    out = HostMesh{};
+    long cells = 0, faces = 0, slots = 0;
+    for (int c = 0; c < count; c++) {
+        const axs_mesh_view& v = meshes[c];
+        if (v.n_cells <= 0 || v.n_faces <= 0 || !v.cell_face_start) return "empty mesh in the batch";
+        cells += v.n_cells;
+        faces += v.n_faces;
+        slots += v.cell_face_start[v.n_cells];
+    }
+    if (cells > 0x7fffffffL || faces > 0x7fffffffL || slots > 0x7fffffffL) return "batch too large for 32-bit indices";
+    out.nCells = (int)cells;
+    out.nFaces = (int)faces;
+    out.cellFaceStart.reserve(cells + 1);
+    out.cellFaceStart.push_back(0);
+    int cellOff = 0, faceOff = 0;
+    for (int c = 0; c < count; c++) {
+        const axs_mesh_view& v = meshes[c];
+        auto app = [](std::vector<double>& dst, const double* src, int n) { dst.insert(dst.end(), src, src + n); };
+        app(out.cellCenterZ, v.cell_center_z, v.n_cells);
+        app(out.cellCenterR, v.cell_center_r, v.n_cells);
+        app(out.cellVolume, v.cell_volume, v.n_cells);
+        app(out.cellArea2D, v.cell_area2d, v.n_cells);
+        const int slot0 = out.cellFaceStart.back();
+        for (int i = 0; i < v.n_cells; i++) out.cellFaceStart.push_back(slot0 + v.cell_face_start[i + 1]);
+        for (int k = 0; k < v.cell_face_start[v.n_cells]; k++) out.cellFaceIDs.push_back(v.cell_face_ids[k] + faceOff);
+        for (int f = 0; f < v.n_faces; f++) {
+            out.faceOwner.push_back(v.face_owner[f] + cellOff);
+            out.faceNeighbor.push_back(v.face_neighbor[f] >= 0 ? v.face_neighbor[f] + cellOff : -1);
+            const int g = v.face_group[f];
+            if (g >= groupsPerMesh) return "boundary group beyond groups per mesh";
+            out.faceGroup.push_back(g >= 0 ? g + c * groupsPerMesh : g);
+        }
+        app(out.faceNormalZ, v.face_normal_z, v.n_faces);
+        app(out.faceNormalR, v.face_normal_r, v.n_faces);
+        app(out.faceCenterZ, v.face_center_z, v.n_faces);
+        app(out.faceCenterR, v.face_center_r, v.n_faces);
+        app(out.faceArea, v.face_area, v.n_faces);
+        cellOff += v.n_cells;
+        faceOff += v.n_faces;
+    }
+    return {};
+}
+
+}  // namespace axs

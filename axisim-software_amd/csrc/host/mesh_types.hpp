@@ -51,5 +51,10 @@ std::string buildTriangles(HostMesh& out, const std::vector<double>& pz, const s
                            const std::vector<int>& tris, int nSeg, const double* sz0, const double* sr0,
                            const double* sz1, const double* sr1, const int* sgroup, double tol);
 std::string buildNozzle(HostMesh& out, const axs_nozzle_spec& spec);
+// Disjoint union of meshes for a batched run: cells, faces and slots of mesh c
+// follow those of meshes 0..c-1 in order, and its boundary group g becomes
+// g + c * groupsPerMesh.  Every per-cell and per-face step of the solver then
+// treats each case exactly as it would alone.
+std::string concatMeshes(HostMesh& out, int count, const axs_mesh_view* meshes, int groupsPerMesh);
 
 }  // namespace axs

@@ -346,6 +346,13 @@ typedef struct axs_nozzle_spec {
 } axs_nozzle_spec;
 axs_status axs_mesh_nozzle(axs_mesh** out, const axs_nozzle_spec* spec);
 
+/* Batched runs (build extension; the reference solves one case at a time):
+ * the disjoint union of `count` meshes, each case's cells, faces and slots in
+ * order after the previous ones and its boundary group g renumbered
+ * g + c * groups_per_mesh, so one solver (one launch per kernel) advances every
+ * case, each exactly as it would alone apart from the shared multigrid
+ * hierarchy's coarsest levels and the pooled residual norms. */
+axs_status axs_mesh_concat(axs_mesh** out, int count, const axs_mesh_view* meshes, int groups_per_mesh);
 axs_status axs_mesh_get_view(const axs_mesh* m, axs_mesh_view* view);
 /* Triangle mesh only: point and triangle arrays (NULL to query counts). */
 axs_status axs_mesh_get_triangles(const axs_mesh* m, int* n_points, double* point_z,
