@@ -137,6 +137,7 @@ def load_library(path: os.PathLike | None = None) -> C.CDLL:
         "axs_run": (C.c_int, [S, PROGRESS_CB, C.c_void_p, C.c_int]),
         "axs_run_timed": (C.c_int, [S, C.c_int, _dp]),
         "axs_bench_kernel": (C.c_int, [S, C.c_int, C.c_int, _dp, _dp]),
+        "axs_autotune": (C.c_int, [S, C.c_int, _dp, _dp]),
         "axs_request_stop": (None, [S]),
         "axs_can_continue": (C.c_int, [S, C.c_char_p, C.c_size_t]),
         "axs_get_field": (C.c_int, [S, C.c_int, _dp, C.c_size_t]),
@@ -463,6 +464,13 @@ class Solver:
         _check(lib().axs_run_timed(self._h, n, C.byref(ms)), self._h)
         return ms.value
 
+    def autotune(self, repeats: int = 20) -> tuple[float, float]:
+        """Choose workgroup sizes by timing (after a run); returns the summed
+        kernel ms at the default and at the chosen sizes."""
+        a, b = C.c_double(), C.c_double()
+        _check(lib().axs_autotune(self._h, repeats, C.byref(a), C.byref(b)), self._h)
+        return a.value, b.value
+
     def set_learned_guess(self, net: "Net | None", level: int = -1) -> None:
         """Learned initial guess for the first p' solve of each outer iteration
         (None: off).  level: multigrid level the net runs on (-1: automatic)."""
@@ -730,6 +738,46 @@ def nozzle_case(cells: float = 1.0e6, spacing: float | None = None, re_throat=50
     for k, v in cfg.items():
         setattr(c, k, v)
     return Case("nozzle", mesh, bcs, c, dict(u_in=u_in, u_throat=u_throat))
+
+
+def sweep_cases(count: int = 64, n_min: float = 2.0e4, n_max: float = 5.0e5, amplitude: float = 0.3,
+                frequency: float = 1.0, seed_template: int = 4001, seed_size: int = 4002, seed_params: int = 4003,
+                rho: float = 1000.0, mu: float = 1e-3) -> list[Case]:
+    """A parameter sweep of axisymmetric cases for a batched run: template
+    pipe / expansion / nozzle uniformly (seed_template), cell count
+    log-uniform in [n_min, n_max] (seed_size), radius uniform in
+    [0.002, 0.01] m and Reynolds number uniform in [50, 1000] (seed_params), and
+    a pulsatile inlet u = mean (1 + amplitude sin(2 pi frequency t)).  Pipe and
+    nozzle geometries scale with the radius; the expansion keeps its own."""
+    rt, rn, rp = (np.random.default_rng(s) for s in (seed_template, seed_size, seed_params))
+    out = []
+    for i in range(count):
+        t = int(rt.integers(3))
+        N = float(np.exp(rn.uniform(np.log(n_min), np.log(n_max))))
+        R = float(rp.uniform(0.002, 0.01))
+        Re = float(rp.uniform(50.0, 1000.0))
+        if t == 0:
+            nr = max(4, int(round(math.sqrt(N / 8.0))))
+            nz = max(8, int(round(N / nr)))
+            case = pipe_case(nz=nz, nr=nr, L=20.0 * R, R=R, u_in=Re * mu / (rho * 2.0 * R), rho=rho, mu=mu)
+        elif t == 1:
+            f = math.sqrt(N / (137 * 85 + 887 * 256))
+            case = expansion_case(nz=(max(4, round(137 * f)), max(8, round(887 * f))),
+                                  nr=(max(4, round(85 * f)), max(4, round(171 * f))), re_throat=Re, rho=rho, mu=mu)
+        else:
+            scale = R / NOZZLE_GEOMETRY["r_in"]
+            geo = {k: (v * scale if k not in ("cone_deg", "diff_deg") else v) for k, v in NOZZLE_GEOMETRY.items()}
+            mesh = nozzle_mesh(nozzle_spacing_for(N) * scale, seed_points=5000 + i, seed_jitter=6000 + i, **geo)
+            # nozzle_case sets the throat velocity from the unscaled throat radius
+            case = nozzle_case(mesh=mesh, re_throat=Re / scale, rho=rho, mu=mu)
+        u = case.bcs[VAR_U]
+        u.type[G_INLET] = BC_PULSATILE
+        u.amplitude = [0.0] * len(u.type)
+        u.frequency = [0.0] * len(u.type)
+        u.amplitude[G_INLET], u.frequency[G_INLET] = amplitude, frequency
+        case.info.update(template=("pipe", "expansion", "nozzle")[t], radius=R, re=Re)
+        out.append(case)
+    return out
 
 
 def concat_meshes(meshes: list[Mesh], groups_per_mesh: int = 4) -> Mesh:

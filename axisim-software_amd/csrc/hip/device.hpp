@@ -12,6 +12,7 @@ constexpr int kBlock = 256;   // threads per workgroup (4 wave64)
 // Mesh on the device: SoA cells + faces (FVMeshDevice, structs/solver_struct.h:333-397).
 struct DMesh {
     int N = 0, F = 0, S = 0;
+    int sweepBlock = kBlock;  // workgroup size of the momentum sweep (autotuned)
     const double *cZ, *cR, *vol, *invA2D, *lsqInvZZ, *lsqInvZR, *lsqInvRR;
     const int *faceStart, *faceIDs;
     const int *slotNb;      // neighbour cell of each slot, -1 on a boundary face
@@ -70,6 +71,7 @@ struct Physics {
 // One multigrid level on the device (MultigridLevel, multigrid.cuh:60-106)
 struct DLevel {
     int n = 0, refs = 0, nCoarse = 0;
+    int block = kBlock;  // workgroup size of this level's per-cell kernels (autotuned)
     int *start = nullptr, *nb = nullptr;
     double *AC = nullptr, *b = nullptr, *AF = nullptr;
     double *x = nullptr, *xNew = nullptr;
@@ -144,7 +146,6 @@ void launchMGGSColor(hipStream_t s, const DLevel& L, const int* order, int begin
 void launchMGSmoothBlockGS(hipStream_t s, const DLevel& L, const int* order, const int* colorStart, int nColors,
                            int sweeps, bool fp32, const Ctl* ctl);
 // V-cycle of levels lt..lt+nl-1 in one workgroup, all of them staged in LDS
-// (levels: device copy of the DLevel array)
 void launchMGTail(hipStream_t s, const TailLaunch& t, double w, int preDone, int pre, int post, int coarseSweeps,
                   bool fp32, const double* inv, const Ctl* ctl);
 // LDS one workgroup may use on the current device (opting the tail kernel in

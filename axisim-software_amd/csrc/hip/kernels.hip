@@ -1253,12 +1253,12 @@ void launchMGCoarseGather(hipStream_t s, const DLevel& f, const DLevel& c, const
     kMGCoarseGather<<<blocksFor((long)c.refs + c.n), kBlock, 0, s>>>(f, c, ctl);
 }
 void launchMGResidual(hipStream_t s, const DLevel& f, const Ctl* ctl) {
-    kMGResidual<<<blocksFor(f.n), kBlock, 0, s>>>(f, ctl);
+    kMGResidual<<<blocksFor(f.n, f.block), f.block, 0, s>>>(f, ctl);
 }
 void launchMGRestrictSum(hipStream_t s, const DLevel& f, const DLevel& c, double w, int sweep1, bool fp32,
                          const Ctl* ctl) {
-    if (fp32) kMGRestrictSum<float><<<blocksFor(c.n), kBlock, 0, s>>>(f, c, w, sweep1, ctl);
-    else kMGRestrictSum<double><<<blocksFor(c.n), kBlock, 0, s>>>(f, c, w, sweep1, ctl);
+    if (fp32) kMGRestrictSum<float><<<blocksFor(c.n, c.block), c.block, 0, s>>>(f, c, w, sweep1, ctl);
+    else kMGRestrictSum<double><<<blocksFor(c.n, c.block), c.block, 0, s>>>(f, c, w, sweep1, ctl);
 }
 void launchGradUV(hipStream_t s, const DMesh& m, DBC bu, DBC bv, DState st, int scheme) {
     kGradUV<<<blocksFor(m.N), kBlock, 0, s>>>(m, bu, bv, st, scheme);
@@ -1272,11 +1272,11 @@ void launchMomentum(hipStream_t s, const DMesh& m, DBC bu, DBC bv, DBC bp, DStat
 }
 void launchJacobiUV(hipStream_t s, const DMesh& m, DState st, const double* uo, const double* vo, double* un,
                     double* vn) {
-    kJacobiUV<<<blocksFor(m.N), kBlock, 0, s>>>(m, st, UVIn{uo, vo, nullptr}, UVOut{un, vn, nullptr});
+    kJacobiUV<<<blocksFor(m.N, m.sweepBlock), m.sweepBlock, 0, s>>>(m, st, UVIn{uo, vo, nullptr}, UVOut{un, vn, nullptr});
 }
 void launchJacobiUVPacked(hipStream_t s, const DMesh& m, DState st, const double* uo, const double* vo,
                           const double2* uvo, double* un, double* vn, double2* uvn) {
-    kJacobiUV<<<blocksFor(m.N), kBlock, 0, s>>>(m, st, UVIn{uo, vo, uvo}, UVOut{un, vn, uvn});
+    kJacobiUV<<<blocksFor(m.N, m.sweepBlock), m.sweepBlock, 0, s>>>(m, st, UVIn{uo, vo, uvo}, UVOut{un, vn, uvn});
 }
 void launchGSColorUV(hipStream_t s, const DMesh& m, DState st, const int* order, int begin, int count) {
     kGSColorUV<<<blocksFor(count), kBlock, 0, s>>>(m, st, order, begin, count);
@@ -1318,23 +1318,23 @@ void launchFill(hipStream_t s, double* dst, double value, int n, const Ctl* ctl)
     kFill<<<blocksFor(n), kBlock, 0, s>>>(dst, value, n, ctl);
 }
 void launchMGJacobi(hipStream_t s, const DLevel& L, double w, const Ctl* ctl) {
-    kMGJacobi<<<blocksFor(L.n), kBlock, 0, s>>>(L, w, ctl);
+    kMGJacobi<<<blocksFor(L.n, L.block), L.block, 0, s>>>(L, w, ctl);
 }
 void launchMGSmoothBlock(hipStream_t s, const DLevel& L, double w, int sweeps, const Ctl* ctl) {
     const int threads = L.n >= 1024 ? 1024 : ((L.n + 63) / 64) * 64;
     kMGSmoothBlock<<<1, threads, 2 * (size_t)L.n * sizeof(double), s>>>(L, w, sweeps, ctl);
 }
 void launchMGProlongSmooth(hipStream_t s, const DLevel& f, const DLevel& c, double w, const Ctl* ctl) {
-    kMGProlongSmooth<<<blocksFor(f.n), kBlock, 0, s>>>(f, c, w, ctl);
+    kMGProlongSmooth<<<blocksFor(f.n, f.block), f.block, 0, s>>>(f, c, w, ctl);
 }
 void launchMGProlong(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl) {
-    kMGProlong<<<blocksFor(f.n), kBlock, 0, s>>>(f, c, ctl);
+    kMGProlong<<<blocksFor(f.n, f.block), f.block, 0, s>>>(f, c, ctl);
 }
 void launchMGGSColor(hipStream_t s, const DLevel& L, const int* order, int begin, int count, bool fp32,
                      const Ctl* ctl) {
     if (count <= 0) return;
-    if (fp32) kMGGSColor<float><<<blocksFor(count), kBlock, 0, s>>>(L, order, begin, count, ctl);
-    else kMGGSColor<double><<<blocksFor(count), kBlock, 0, s>>>(L, order, begin, count, ctl);
+    if (fp32) kMGGSColor<float><<<blocksFor(count, L.block), L.block, 0, s>>>(L, order, begin, count, ctl);
+    else kMGGSColor<double><<<blocksFor(count, L.block), L.block, 0, s>>>(L, order, begin, count, ctl);
 }
 void launchMGSmoothBlockGS(hipStream_t s, const DLevel& L, const int* order, const int* colorStart, int nColors,
                            int sweeps, bool fp32, const Ctl* ctl) {

@@ -317,29 +317,29 @@ void launchMomSweepApply(hipStream_t s, const DMesh& m, DState st, MixedState x,
     kMomSweepApply<<<blocksFor(m.N), kBlock, 0, s>>>(m, st, x, dO, sweep);
 }
 void launchMGDefect(hipStream_t s, const DLevel& L, double w, int firstSweep, const Ctl* ctl) {
-    kMGDefect<<<blocksFor(L.n), kBlock, 0, s>>>(L, w, firstSweep, ctl);
+    kMGDefect<<<blocksFor(L.n, L.block), L.block, 0, s>>>(L, w, firstSweep, ctl);
 }
 void launchMGJacobi32(hipStream_t s, const DLevel& L, float w, const Ctl* ctl) {
-    kMGJacobi32<<<blocksFor(L.n), kBlock, 0, s>>>(L, w, ctl);
+    kMGJacobi32<<<blocksFor(L.n, L.block), L.block, 0, s>>>(L, w, ctl);
 }
 void launchMGSmoothBlock32(hipStream_t s, const DLevel& L, float w, int sweeps, const Ctl* ctl) {
     const int threads = L.n >= 1024 ? 1024 : ((L.n + 63) / 64) * 64;
     kMGSmoothBlock32<<<1, threads, 2 * (size_t)L.n * sizeof(float), s>>>(L, w, sweeps, ctl);
 }
 void launchMGResidual32(hipStream_t s, const DLevel& f, const Ctl* ctl) {
-    kMGResidual32<<<blocksFor(f.n), kBlock, 0, s>>>(f, ctl);
+    kMGResidual32<<<blocksFor(f.n, f.block), f.block, 0, s>>>(f, ctl);
 }
 void launchMGProlongSmooth32(hipStream_t s, const DLevel& f, const DLevel& c, float w, const Ctl* ctl) {
-    kMGProlongSmooth32<<<blocksFor(f.n), kBlock, 0, s>>>(f, c, w, ctl);
+    kMGProlongSmooth32<<<blocksFor(f.n, f.block), f.block, 0, s>>>(f, c, w, ctl);
 }
 void launchMGProlong32(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl) {
-    kMGProlong32<<<blocksFor(f.n), kBlock, 0, s>>>(f, c, ctl);
+    kMGProlong32<<<blocksFor(f.n, f.block), f.block, 0, s>>>(f, c, ctl);
 }
 void launchMGApply(hipStream_t s, const DLevel& L, const Ctl* ctl) {
-    kMGApply<<<blocksFor(L.n), kBlock, 0, s>>>(L, ctl);
+    kMGApply<<<blocksFor(L.n, L.block), L.block, 0, s>>>(L, ctl);
 }
 void launchMGCast(hipStream_t s, const DLevel& L, const Ctl* ctl) {
-    kMGCast<<<blocksFor(L.n), kBlock, 0, s>>>(L, ctl);
+    kMGCast<<<blocksFor(L.n, L.block), L.block, 0, s>>>(L, ctl);
 }
 void launchCoarseInvert(hipStream_t s, const DLevel& L, double* aug, double* inv, int* status, const Ctl* ctl) {
     kCoarseInvert<<<1, 1024, 0, s>>>(L, aug, inv, status, ctl);

@@ -173,6 +173,11 @@ axs_status axs_bench_kernel(axs_solver* h, int kernel, int repeats, double* ms, 
     return guarded(&h->s->lastError, [&] { h->s->benchKernel(kernel, repeats, ms, bytes); });
 }
 
+axs_status axs_autotune(axs_solver* h, int repeats, double* ms_default, double* ms_tuned) {
+    if (!h) return AXS_ERR_INVALID;
+    return guarded(&h->s->lastError, [&] { h->s->autotune(repeats, ms_default, ms_tuned); });
+}
+
 void axs_request_stop(axs_solver* h) {
     if (h && h->s) h->s->requestStop();
 }

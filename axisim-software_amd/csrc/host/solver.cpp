@@ -1316,6 +1316,87 @@ double Solver::bytesPerIteration() const {
 }
 
 // ---------------------------------------------------------------------------
+// autotuner (build extension): workgroup size per kernel family and level
+// ---------------------------------------------------------------------------
+void Solver::autotune(int repeats, double* msDefault, double* msTuned) {
+    if (running_) throw std::logic_error("solver is running");
+    if (!stateAllocated_ || !meshUploaded_) throw std::logic_error("run the solver before autotuning");
+    if (repeats <= 0) throw std::invalid_argument("repeats must be positive");
+    AXS_HIP(hipSetDevice(device_));
+    AXS_HIP(hipStreamSynchronize(stream_));
+    hipEvent_t e0, e1;
+    AXS_HIP(hipEventCreate(&e0));
+    AXS_HIP(hipEventCreate(&e1));
+    auto timeIt = [&](const std::function<void()>& launch) {
+        launch();
+        AXS_HIP(hipEventRecord(e0, stream_));
+        for (int r = 0; r < repeats; r++) launch();
+        AXS_HIP(hipEventRecord(e1, stream_));
+        AXS_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        AXS_HIP(hipEventElapsedTime(&ms, e0, e1));
+        return (double)ms / repeats;
+    };
+    const int cands[] = {64, 128, 256};
+    double def = 0.0, best = 0.0;
+    // pick the fastest candidate of one family; set(c) applies a size
+    auto tune = [&](const std::function<void(int)>& set, const std::function<void()>& launch) {
+        double bt = 1e300;
+        int bc = kBlock;
+        for (int c : cands) {
+            set(c);
+            const double t = timeIt(launch);
+            if (c == kBlock) def += t;
+            if (t < bt) {
+                bt = t;
+                bc = c;
+            }
+        }
+        set(bc);
+        best += bt;
+    };
+    DState st = ds_;
+    st.ctl = nullptr;
+    // the interleaved momentum sweep, writing the sweep scratch buffers
+    tune([&](int c) { dm_.sweepBlock = c; },
+         [&] { launchJacobiUVPacked(stream_, dm_, st, nullptr, nullptr, st.uvA, nullptr, nullptr, st.uvB); });
+    // each multigrid level outside the single-workgroup tail: smoothing,
+    // residual and prolongation kernels (they write only the level's scratch)
+    if (cfg_.use_multigrid && mgBuilt_) {
+        const bool fp32 = cfg_.precision == AXS_PREC_MIXED;
+        const int L = (int)mg_.host.size();
+        const int tail = mg_.tailLevel(fp32);
+        const double w = cfg_.mg_weight;
+        for (int l = 0; l < L; l++) {
+            if (tail >= 0 && l >= tail) break;
+            DLevel& d = mg_.dev[l];
+            const bool coarsest = mg_.host[l].coarsest();
+            tune([&](int c) { d.block = c; },
+                 [&] {
+                     if (fp32) {
+                         launchMGJacobi32(stream_, d, (float)w, nullptr);
+                         if (!coarsest) {
+                             launchMGResidual32(stream_, d, nullptr);
+                             launchMGProlongSmooth32(stream_, d, mg_.dev[l + 1], (float)w, nullptr);
+                         }
+                     } else {
+                         launchMGJacobi(stream_, d, w, nullptr);
+                         if (!coarsest) {
+                             launchMGResidual(stream_, d, nullptr);
+                             launchMGProlongSmooth(stream_, d, mg_.dev[l + 1], w, nullptr);
+                         }
+                     }
+                 });
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    AXS_HIP(hipGetLastError());
+    if (msDefault) *msDefault = def;
+    if (msTuned) *msTuned = best;
+}
+
+// ---------------------------------------------------------------------------
 // learned initial guess (build extension)
 // ---------------------------------------------------------------------------
 struct Solver::NetSetup {

@@ -148,6 +148,12 @@ public:
     void run(axs_progress_cb cb, void* user, bool continueRun);
     void runTimed(int n, double* ms);
     void benchKernel(int kernel, int repeats, double* ms, double* bytes);
+    // Workgroup sizes of the momentum sweep and of every multigrid level's
+    // per-cell kernels, each chosen from 64 / 128 / 256 threads by timing them on
+    // the current state (after a run).  *msDefault / *msTuned: summed kernel
+    // times at 256 threads and at the chosen sizes.
+    void autotune(int repeats, double* msDefault, double* msTuned);
+    int sweepBlock() const { return dm_.sweepBlock; }
     void requestStop() { stop_ = true; }
     bool canContinue(std::string& reason) const;
     void getField(int field, double* out, size_t n);

@@ -42,6 +42,11 @@ def main() -> int:
     ap.add_argument("--cpu-iters", type=int, default=10, help="oracle sample size (outer iterations)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default workgroup sizes")
+    ap.add_argument("--no-c4", action="store_true", help="skip the batched-sweep leg")
+    ap.add_argument("--c4-cases", type=int, default=64)
+    ap.add_argument("--c4-steps", type=int, default=200)
+    ap.add_argument("--c4-iters", type=int, default=5, help="outer iterations per time step at most")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -64,7 +69,16 @@ def main() -> int:
     st = s.stats()
     log(f"warm-up: {st.iterations} iterations, setup {st.setup_ms:.0f} ms, mg levels {st.mg_levels} "
         f"({time.time() - t1:.1f} s)")
+    tune = None
+    if not args.no_autotune:
+        d_ms, t_ms = s.autotune(20)
+        tune = {"default_ms": round(d_ms, 4), "tuned_ms": round(t_ms, 4)}
+        case.config.max_iter = 2  # a continued run adds max_iter iterations
+        s.configure(case.config)
+        s.run(continue_run=True)  # rebuilds the iteration graph with the chosen sizes
+        log(f"autotune: tuned kernels {d_ms:.3f} -> {t_ms:.3f} ms per pass")
 
+    pre_iters = s.stats().iterations
     ms = s.run_timed(args.steps)
     it_s = args.steps / (ms / 1000.0)
     log(f"timed: {args.steps} iterations in {ms:.1f} ms -> {it_s:.1f} it/s")
@@ -91,7 +105,7 @@ def main() -> int:
     # same number of iterations from the same cold start
     mixed = None
     if not args.no_mixed:
-        mcase = axs.nozzle_case(mesh=m, max_iter=args.warmup, precision=axs.PREC_MIXED)
+        mcase = axs.nozzle_case(mesh=m, max_iter=pre_iters, precision=axs.PREC_MIXED)
         sm = axs.make_solver(mcase)
         sm.run()
         mms = sm.run_timed(args.steps)
@@ -101,12 +115,43 @@ def main() -> int:
             a, r = sm.field(name), s.field(name)
             err[name] = float(np.linalg.norm(a - r) / max(np.linalg.norm(r), 1e-300))
         log(f"mixed: {mit_s:.1f} it/s ({mit_s / it_s:.2f}x), rel L2 vs FP64 after "
-            f"{args.warmup + args.steps} iterations {err}")
+            f"{pre_iters + args.steps} iterations {err}")
         mixed = {"value": round(mit_s, 3), "unit": "outer_iter/s", "ms_per_step": round(mms / args.steps, 5),
                  "speedup_vs_fp64": round(mit_s / it_s, 3), "dtype": "f32 sweeps/V-cycles, f64 defects",
-                 "rel_l2_vs_fp64": err, "iterations_compared": args.warmup + args.steps,
+                 "rel_l2_vs_fp64": err, "iterations_compared": pre_iters + args.steps,
                  "mg_solve_ms": round(mg32_ms, 4), "mg_solve_gbs": round(mg32_gbs, 1)}
         del sm
+
+    # batched sweep: many cases of uneven size in one solver, every launch covers all
+    c4 = None
+    if not args.no_c4:
+        tc = time.time()
+        cases = axs.sweep_cases(args.c4_cases)
+        cells = sum(c.mesh.n_cells for c in cases)
+        batch = axs.batch_case(cases, transient=1, time_scheme=axs.TIME_BDF2, dt=1e-3, t_end=2e-3,
+                               max_iter=args.c4_iters, use_non_orth=1, structured_mesh=0,
+                               gradient_scheme=axs.GRAD_LSQ, convection_scheme=axs.CONV_UPWIND)
+        log(f"c4: {len(cases)} cases, {cells} cells ({time.time() - tc:.1f} s to generate)")
+        sb = axs.make_solver(batch)
+        sb.run()  # two time steps: hierarchy, graph, warm caches
+        c4tune = None if args.no_autotune else sb.autotune(10)
+        batch.config.t_end = args.c4_steps * 1e-3
+        sb.configure(batch.config)
+        t2 = time.time()
+        sb.run()
+        wall = time.time() - t2
+        sst = sb.stats()
+        cups = cells * sst.iterations / (sst.solve_ms / 1e3)
+        log(f"c4: {args.c4_steps} BDF2 steps, {sst.iterations} outer iterations in {wall:.2f} s -> "
+            f"{cups:.3e} cell-updates/s")
+        templates = [c.info["template"] for c in cases]
+        c4 = {"cases": len(cases), "cells": cells, "templates": {t: templates.count(t) for t in set(templates)},
+              "time_steps": args.c4_steps, "dt": 1e-3, "max_iter_per_step": args.c4_iters,
+              "outer_iterations": sst.iterations, "solve_s": round(sst.solve_ms / 1e3, 3),
+              "cell_updates_per_s": round(cups, 1), "per_case_latency_s": round(wall, 3),
+              "mg_levels": sst.mg_levels,
+              "autotune_ms": None if c4tune is None else [round(c4tune[0], 4), round(c4tune[1], 4)]}
+        del sb
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -151,7 +196,9 @@ def main() -> int:
         "mg_vcycle": {"bound": "hbm", "achieved": round(mg_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                       "frac": round(mg_gbs / HBM_PEAK_GBS, 4), "ms_per_solve": round(mg_ms, 4),
                       "bytes_per_solve": mg_bytes, "cycles": case.config.mg_max_iter},
+        "autotune": tune,
         "mixed_precision": mixed,
+        "batched_sweep": c4,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

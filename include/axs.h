@@ -231,6 +231,12 @@ enum {
     AXS_KERNEL_PP_COEFF = 4     /* pressure-correction matrix                */
 };
 axs_status axs_bench_kernel(axs_solver* s, int kernel, int repeats, double* ms, double* bytes);
+/* Autotuner (build extension): choose the workgroup size (64, 128 or 256
+ * threads) of the momentum sweep and of each multigrid level's per-cell kernels
+ * by timing them on the solver's current state; call after a run, the next run
+ * uses the choice.  *ms_default / *ms_tuned (may be NULL): summed kernel times
+ * of one pass at 256 threads and at the chosen sizes. */
+axs_status axs_autotune(axs_solver* s, int repeats, double* ms_default, double* ms_tuned);
 
 /* ---------------- Agglomeration multigrid, stand-alone ---------------- */
 typedef struct axs_mg axs_mg;
