@@ -47,12 +47,16 @@ def main() -> int:
     ap.add_argument("--c4-cases", type=int, default=64)
     ap.add_argument("--c4-steps", type=int, default=200)
     ap.add_argument("--c4-iters", type=int, default=5, help="outer iterations per time step at most")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="only launch the measured kernels and a 512 MiB copy, for rocprofv3 --pmc passes")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the solver does not shard "
               f"across GPUs)", file=sys.stderr)
         return 2
+    if args.traffic_probe:
+        return traffic_probe()
 
     import numpy as np
 
@@ -191,7 +195,7 @@ def main() -> int:
             "iteration_hbm_frac": round(st.bytes_per_iteration * it_s / 1e9 / HBM_PEAK_GBS, 4),
         },
         "roofline": {"bound": "hbm", "kernel": "kJacobiUV", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic("kJacobiUV"),
                      "bytes_per_launch": k_bytes, "us_per_launch": round(k_ms * 1000, 3)},
         "mg_vcycle": {"bound": "hbm", "achieved": round(mg_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                       "frac": round(mg_gbs / HBM_PEAK_GBS, 4), "ms_per_solve": round(mg_ms, 4),
@@ -202,6 +206,45 @@ def main() -> int:
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    return 0
+
+
+TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_nozzle1M_pmc.txt"
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE, scaled by a copy of known size), or None."""
+    try:
+        for line in TRAFFIC_FILE.read_text().splitlines():
+            parts = line.split()
+            if len(parts) >= 2 and parts[0] == f"traffic_bytes_per_launch[{kernel}]":
+                return float(parts[1])
+    except OSError:
+        pass
+    return None
+
+
+def traffic_probe() -> int:
+    """The kernels the benchmark reports, launched on the 1M-cell nozzle from a
+    few iterations in, then a plain copy of 512 MiB that calibrates the counters."""
+    import torch
+
+    import axs
+
+    case = axs.nozzle_case(max_iter=5)
+    s = axs.make_solver(case)
+    s.run()
+    for k in (axs.KERNEL_JACOBI_UV, axs.KERNEL_MOMENTUM, axs.KERNEL_FACE_FLUX, axs.KERNEL_PP_COEFF):
+        ms, by = s.bench_kernel(k, 10)
+        log(f"kernel {k}: {ms * 1000:.1f} us, {by:.0f} algorithmic bytes")
+    x = torch.empty(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
+    x.fill_(1.0)
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    log(f"copy: {x.numel() * 8} bytes read and written per launch")
     return 0
 
 
