@@ -29,7 +29,7 @@ PREC_FP64, PREC_MIXED = range(2)
 MG_SMOOTH_JACOBI, MG_SMOOTH_RBGS = range(2)
 VAR_U, VAR_V, VAR_P = range(3)
 RES_U, RES_V, RES_CONTINUITY = range(3)
-KERNEL_JACOBI_UV, KERNEL_MOMENTUM, KERNEL_MG_SMOOTH, KERNEL_FACE_FLUX, KERNEL_PP_COEFF = range(5)
+KERNEL_JACOBI_UV, KERNEL_MOMENTUM, KERNEL_MG_SMOOTH, KERNEL_FACE_FLUX, KERNEL_PP_COEFF, KERNEL_JACOBI_UV_FUSED = range(6)
 
 FIELDS = {
     "u": 0, "v": 1, "p": 2, "pp": 3, "grad_uz": 4, "grad_ur": 5, "grad_vz": 6, "grad_vr": 7,
@@ -94,6 +94,7 @@ class Stats(C.Structure):
         ("n_colors", C.c_int), ("mg_levels", C.c_int), ("mg_level_cells", C.c_int * 32),
         ("setup_ms", C.c_double), ("solve_ms", C.c_double), ("last_residual", C.c_double * 3),
         ("bytes_per_iteration", C.c_double), ("vcycle_bytes", C.c_double),
+        ("sweeps_per_launch", C.c_int), ("sweep_block_cells", C.c_int),
     ]
 
 
@@ -138,6 +139,7 @@ def load_library(path: os.PathLike | None = None) -> C.CDLL:
         "axs_run_timed": (C.c_int, [S, C.c_int, _dp]),
         "axs_bench_kernel": (C.c_int, [S, C.c_int, C.c_int, _dp, _dp]),
         "axs_autotune": (C.c_int, [S, C.c_int, _dp, _dp]),
+        "axs_set_sweep_fusion": (C.c_int, [S, C.c_int, C.c_int]),
         "axs_request_stop": (None, [S]),
         "axs_can_continue": (C.c_int, [S, C.c_char_p, C.c_size_t]),
         "axs_get_field": (C.c_int, [S, C.c_int, _dp, C.c_size_t]),
@@ -470,6 +472,11 @@ class Solver:
         a, b = C.c_double(), C.c_double()
         _check(lib().axs_autotune(self._h, repeats, C.byref(a), C.byref(b)), self._h)
         return a.value, b.value
+
+    def set_sweep_fusion(self, sweeps: int, block_cells: int = 1024) -> None:
+        """Momentum Jacobi sweeps per launch (1, 2 or 3) and cells per fused
+        block (512 or 1024) for the next run; bit-identical results."""
+        _check(lib().axs_set_sweep_fusion(self._h, sweeps, block_cells), self._h)
 
     def set_learned_guess(self, net: "Net | None", level: int = -1) -> None:
         """Learned initial guess for the first p' solve of each outer iteration

@@ -18,7 +18,22 @@ struct DMesh {
     const int *slotNb;      // neighbour cell of each slot, -1 on a boundary face
     const int *owner, *neighbor, *group;
     const double *nZ, *nR, *fZ, *fR, *area, *invDPN, *wP, *dPB, *lsqWZ, *lsqWR, *length2D;
+    // face data in slot order, oriented for the slot's cell (SlotGeometry)
+    const double *sArea, *sInvDPN, *sNZ, *sNR, *sW, *sTZ, *sTR, *sLsqWZ, *sLsqWR, *sLen;
+    const uint8_t* sFlags;  // bit 0: the cell owns the face; bit 1: face on the axis
 };
+
+// Blocks of consecutive cells with their halo rings (SweepBlocks, device copy):
+// the fused momentum sweeps run `sweeps` Jacobi sweeps per launch, one
+// workgroup of `cells` threads per block, the intermediate iterates in LDS.
+struct DSweepBlocks {
+    int cells = 0, rings = 0, nBlocks = 0;
+    const int* haloStart = nullptr;
+    const int* ringEnd = nullptr;
+    const int* halo = nullptr;
+    const int2* sorted = nullptr;
+};
+constexpr int kSweepRegSlots = 4;   // slots of a block cell kept in registers across the sweeps
 
 // Boundary conditions of one variable (BoundaryFieldDevice, solver_struct.h:431-454).
 struct DBC {
@@ -116,6 +131,13 @@ void launchJacobiUV(hipStream_t s, const DMesh& m, DState st, const double* uo, 
 // (uvn) replaces uo/vo (un/vn)
 void launchJacobiUVPacked(hipStream_t s, const DMesh& m, DState st, const double* uo, const double* vo,
                           const double2* uvo, double* un, double* vn, double2* uvn);
+// `sweeps` (2 or 3, at most sb.rings + 1) Jacobi sweeps in one launch; the
+// mesh's cells have at most kSweepRegSlots slots and every block's halo fits
+// sweepHaloMax(sb.cells)
+int sweepHaloMax(int cells);
+void launchJacobiUVBlock(hipStream_t s, const DMesh& m, DState st, const DSweepBlocks& sb, int sweeps,
+                         const double* uo, const double* vo, const double2* uvo, double* un, double* vn,
+                         double2* uvn);
 void launchGSColorUV(hipStream_t s, const DMesh& m, DState st, const int* order, int begin, int count);
 void launchJacobiSys(hipStream_t s, const DMesh& m, const double* b, const double* inv, const double* AF,
                      const double* xo, double* xn, const Ctl* ctl);

@@ -14,6 +14,7 @@
 // returns when the outer loop has already ended, so surplus graph launches
 // queued ahead of the host's convergence poll do nothing.
 #include <cmath>
+#include <stdexcept>
 
 #include "device.hpp"
 
@@ -55,6 +56,17 @@ __device__ __forceinline__ double prescribed(const DMesh& m, int f, uint8_t t, d
     return v * (1.0 - x * x);
 }
 
+// boundary branch of interpolateFieldToFace, solver_util.cuh:263-297
+__device__ __forceinline__ double boundaryFaceValue(const DMesh& m, const DBC& bc, double phiP, int f) {
+    const int g = m.group[f];
+    if (g < 0 || g >= bc.nGroups) return phiP;
+    const uint8_t t = bc.type[g];
+    const double v = bc.value[g];
+    if (t == 0 || t == 2) return prescribed(m, f, t, v, bc.length[g]);
+    if (t == 1) return phiP + v * m.dPB[f];
+    return phiP;
+}
+
 // interpolateFieldToFace, solver_util.cuh:240-297
 __device__ __forceinline__ double interpToFace(const DMesh& m, const DBC& bc, const double* phi, int c,
                                                int f) {
@@ -65,13 +77,7 @@ __device__ __forceinline__ double interpToFace(const DMesh& m, const DBC& bc, co
         const double w = faceWeight(m, c, f);
         return w * phiP + (1.0 - w) * phi[o];
     }
-    const int g = m.group[f];
-    if (g < 0 || g >= bc.nGroups) return phiP;
-    const uint8_t t = bc.type[g];
-    const double v = bc.value[g];
-    if (t == 0 || t == 2) return prescribed(m, f, t, v, bc.length[g]);
-    if (t == 1) return phiP + v * m.dPB[f];
-    return phiP;
+    return boundaryFaceValue(m, bc, phiP, f);
 }
 
 // nonOrthoScalarDiffusionFlux, solver_util.cuh:383-430
@@ -97,68 +103,152 @@ __device__ __forceinline__ double nonOrthoFlux(const DMesh& m, int c, int f, con
     return gamma * (Tz * gfz + Tr * gfr);
 }
 
-// phiGradientGreenGauss / phiGradientLeastSquare, solver_util.cuh:458-588
-__device__ void cellGradient(const DMesh& m, const DBC& bc, const double* phi, int c, int scheme, double& gZ,
-                             double& gR) {
+// ---- the same helpers over the slots of cell c (k: slot, o = slotNb[k]) ----
+// The cell loops read the face data in slot order, oriented for c (sNZ/sNR =
+// the normal pointing out of c, sW = c's interpolation weight, sTZ/sTR = the
+// non-orthogonal vector A n - (A/|d|) d towards o, sLsqW* signed so that
+// dphi = phi_o - phi_c), which the host derives from the face arrays with the
+// reference's own expressions.  Each result is bit-identical to the face form;
+// only boundary slots still read face-indexed data.
+__device__ __forceinline__ bool slotOwns(const DMesh& m, int k) { return (m.sFlags[k] & 1) != 0; }
+
+// normalCorrToFace; the squares of the normal do not see its orientation
+__device__ __forceinline__ double normalCorrSlot(const DMesh& m, const DState& s, int c, int k, int o) {
+    const double nz = m.sNZ[k], nr = m.sNR[k];
+    const double DP = normalCorr(s, c, nz, nr);
+    if (o < 0) return DP;
+    const double DN = normalCorr(s, o, nz, nr);
+    const double w = m.sW[k];
+    return w * DP + (1.0 - w) * DN;
+}
+
+__device__ __forceinline__ double interpSlot(const DMesh& m, const DBC& bc, const double* phi, int c, int k,
+                                             int o) {
+    const double phiP = phi[c];
+    if (o >= 0) {
+        const double w = m.sW[k];
+        return w * phiP + (1.0 - w) * phi[o];
+    }
+    return boundaryFaceValue(m, bc, phiP, m.faceIDs[k]);
+}
+
+// nonOrthoFlux: zero on a boundary slot
+__device__ __forceinline__ double nonOrthoSlot(const DMesh& m, int c, int k, int o, const double* gz,
+                                               const double* gr, double gamma) {
+    if (o < 0) return 0.0;
+    const double Tz = m.sTZ[k], Tr = m.sTR[k];
+    const double w = m.sW[k];
+    const double gfz = w * gz[c] + (1.0 - w) * gz[o];
+    const double gfr = w * gr[c] + (1.0 - w) * gr[o];
+    return gamma * (Tz * gfz + Tr * gfr);
+}
+
+// phiGradientGreenGauss / phiGradientLeastSquare, solver_util.cuh:458-588, of
+// NF fields of cell c in one walk over its slots (the geometry is read once
+// for all of them; each field's sums run in the reference's order).  In the
+// least-squares form dphi = phi_o - phi_c with the weight negated where c is
+// the face's neighbour: (-w)(a - b) == w(b - a) exactly.
+template <int NF>
+__device__ void cellGradients(const DMesh& m, const DBC* bc, const double* const* phi, int c, int scheme,
+                              double* gZ, double* gR) {
     const int k0 = m.faceStart[c], k1 = m.faceStart[c + 1];
+    double phiP[NF];
+#pragma unroll
+    for (int i = 0; i < NF; i++) phiP[i] = phi[i][c];
     if (scheme == 0) {
-        double gz = 0.0, gr = 0.0, clz = 0.0, clr = 0.0, axisPhi = 0.0;
+        double gz[NF], gr[NF], axisPhi[NF];
+        double clz = 0.0, clr = 0.0;
+#pragma unroll
+        for (int i = 0; i < NF; i++) gz[i] = gr[i] = axisPhi[i] = 0.0;
         bool axis = false;
         for (int k = k0; k < k1; k++) {
-            const int f = m.faceIDs[k];
-            double nz = m.nZ[f], nr = m.nR[f];
-            if (m.neighbor[f] == c) {
-                nz = -nz;
-                nr = -nr;
+            const int o = m.slotNb[k];
+            const double nz = m.sNZ[k], nr = m.sNR[k];
+            double phiF[NF];
+            if (o >= 0) {
+                const double w = m.sW[k];
+#pragma unroll
+                for (int i = 0; i < NF; i++) phiF[i] = w * phiP[i] + (1.0 - w) * phi[i][o];
+            } else {
+                const int f = m.faceIDs[k];
+#pragma unroll
+                for (int i = 0; i < NF; i++) phiF[i] = boundaryFaceValue(m, bc[i], phiP[i], f);
             }
-            const double phiF = interpToFace(m, bc, phi, c, f);
-            if (m.fR[f] > 1.0e-30) {
-                const double L = m.length2D[f];
-                gz += phiF * nz * L;
-                gr += phiF * nr * L;
+            if (!(m.sFlags[k] & 2)) {
+                const double L = m.sLen[k];
+#pragma unroll
+                for (int i = 0; i < NF; i++) {
+                    gz[i] += phiF[i] * nz * L;
+                    gr[i] += phiF[i] * nr * L;
+                }
                 clz += nz * L;
                 clr += nr * L;
             } else {
                 axis = true;
-                axisPhi = phiF;
+#pragma unroll
+                for (int i = 0; i < NF; i++) axisPhi[i] = phiF[i];
             }
-        }
-        if (axis) {
-            gz += axisPhi * (-clz);
-            gr += axisPhi * (-clr);
         }
         const double inv = m.invA2D[c];
-        gZ = gz * inv;
-        gR = gr * inv;
-    } else {
-        const double phiP = phi[c];
-        double bz = 0.0, br = 0.0;
-        for (int k = k0; k < k1; k++) {
-            const int f = m.faceIDs[k];
-            const int ow = m.owner[f], nb = m.neighbor[f];
-            double dphi;
-            if (nb >= 0) {
-                const bool own = (ow == c);
-                const double phiNb = phi[own ? nb : ow];
-                dphi = own ? (phiNb - phiP) : (phiP - phiNb);
-            } else {
-                dphi = interpToFace(m, bc, phi, c, f) - phiP;
+#pragma unroll
+        for (int i = 0; i < NF; i++) {
+            if (axis) {
+                gz[i] += axisPhi[i] * (-clz);
+                gr[i] += axisPhi[i] * (-clr);
             }
-            bz += m.lsqWZ[f] * dphi;
-            br += m.lsqWR[f] * dphi;
+            gZ[i] = gz[i] * inv;
+            gR[i] = gr[i] * inv;
         }
-        const double izr = m.lsqInvZR[c];
-        gZ = m.lsqInvZZ[c] * bz + izr * br;
-        gR = izr * bz + m.lsqInvRR[c] * br;
+    } else {
+        double bz[NF], br[NF];
+#pragma unroll
+        for (int i = 0; i < NF; i++) bz[i] = br[i] = 0.0;
+        for (int k = k0; k < k1; k++) {
+            const int o = m.slotNb[k];
+            const double wz = m.sLsqWZ[k], wr = m.sLsqWR[k];
+            double dphi[NF];
+            if (o >= 0) {
+#pragma unroll
+                for (int i = 0; i < NF; i++) dphi[i] = phi[i][o] - phiP[i];
+            } else {
+                const int f = m.faceIDs[k];
+#pragma unroll
+                for (int i = 0; i < NF; i++) dphi[i] = boundaryFaceValue(m, bc[i], phiP[i], f) - phiP[i];
+            }
+#pragma unroll
+            for (int i = 0; i < NF; i++) {
+                bz[i] += wz * dphi[i];
+                br[i] += wr * dphi[i];
+            }
+        }
+        const double izz = m.lsqInvZZ[c], izr = m.lsqInvZR[c], irr = m.lsqInvRR[c];
+#pragma unroll
+        for (int i = 0; i < NF; i++) {
+            gZ[i] = izz * bz[i] + izr * br[i];
+            gR[i] = izr * bz[i] + irr * br[i];
+        }
     }
 }
 
-// higherOrderFaceValue, solver_util.cuh:724-768
-__device__ __forceinline__ double higherOrder(const DMesh& m, const DBC& bc, const double* phi, const double* gz,
-                                              const double* gr, int scheme, int n, int nb, int f, double F,
-                                              double phiUD) {
-    if (scheme == 1) return interpToFace(m, bc, phi, n, f);
+__device__ __forceinline__ void cellGradient(const DMesh& m, const DBC& bc, const double* phi, int c, int scheme,
+                                             double& gZ, double& gR) {
+    double z[1], r[1];
+    cellGradients<1>(m, &bc, &phi, c, scheme, z, r);
+    gZ = z[0];
+    gR = r[0];
+}
+
+// higherOrderFaceValue (solver_util.cuh:724-768) on the interior slot k of cell n
+// (neighbour nb)
+__device__ __forceinline__ double higherOrderSlot(const DMesh& m, const double* phi, const double* gz,
+                                                  const double* gr, int scheme, int n, int nb, int k, double F,
+                                                  double phiUD) {
+    if (scheme == 1) {
+        const double w = m.sW[k];
+        return w * phi[n] + (1.0 - w) * phi[nb];
+    }
     if (scheme == 2) {
+        const int f = m.faceIDs[k];
         const int up = (F > 0.0) ? n : nb;
         const double dz = m.fZ[f] - m.cZ[up];
         const double dr = m.fR[f] - m.cR[up];
@@ -255,13 +345,14 @@ __global__ void __launch_bounds__(kBlock) kGradUV(DMesh m, DBC bu, DBC bv, DStat
     if (finished(s.ctl)) return;
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= m.N) return;
-    double gz, gr;
-    cellGradient(m, bu, s.u, n, scheme, gz, gr);
-    s.gUZ[n] = gz;
-    s.gUR[n] = gr;
-    cellGradient(m, bv, s.v, n, scheme, gz, gr);
-    s.gVZ[n] = gz;
-    s.gVR[n] = gr;
+    const DBC bc[2] = {bu, bv};
+    const double* phi[2] = {s.u, s.v};
+    double gz[2], gr[2];
+    cellGradients<2>(m, bc, phi, n, scheme, gz, gr);
+    s.gUZ[n] = gz[0];
+    s.gUR[n] = gr[0];
+    s.gVZ[n] = gz[1];
+    s.gVR[n] = gr[1];
 }
 
 __global__ void __launch_bounds__(kBlock) kGradField(DMesh m, DBC bc, const double* phi, double* gzo,
@@ -296,46 +387,45 @@ __global__ void __launch_bounds__(kBlock) kMomentum(DMesh m, DBC bu, DBC bv, DBC
     double acU = 0.0, bU = 0.0, acV = 0.0, bV = 0.0, netF = 0.0;
     const int k0 = m.faceStart[n], k1 = m.faceStart[n + 1];
     for (int k = k0; k < k1; k++) {
-        const int f = m.faceIDs[k];
-        const int ow = m.owner[f], nb = m.neighbor[f];
-        const double area = m.area[f];
+        const int o = m.slotNb[k];
+        const double area = m.sArea[k];
         double F = 0.0;
         if (ph.conv) {
-            const double Fo = s.mDot[f] * 1.0;
-            if (ow == n) F = Fo;
-            else if (nb == n) F = -Fo;
+            // a boundary face is always owned by its one cell
+            const double Fo = s.mDot[m.faceIDs[k]] * 1.0;
+            F = slotOwns(m, k) ? Fo : -Fo;
             if (fabs(F) <= 1.0e-30) F = 0.0;
             netF += F;
         }
-        if (nb >= 0) {
-            const int o = (ow == n) ? nb : ow;
-            const double K = mu * area * m.invDPN[f];
+        if (o >= 0) {
+            const double K = mu * area * m.sInvDPN[k];
             double af = 0.0;
             acU += K;
             af -= K;
-            if (ph.nonOrtho) bU += nonOrthoFlux(m, n, f, s.gUZ, s.gUR, mu);
+            if (ph.nonOrtho) bU += nonOrthoSlot(m, n, k, o, s.gUZ, s.gUR, mu);
             if (F != 0.0) {
                 acU += fmax(F, 0.0);
                 af += fmin(F, 0.0);
                 if (ph.scheme != 0) {
                     const double ud = (F > 0.0) ? s.u[n] : s.u[o];
-                    const double ho = higherOrder(m, bu, s.u, s.gUZ, s.gUR, ph.scheme, n, o, f, F, ud);
+                    const double ho = higherOrderSlot(m, s.u, s.gUZ, s.gUR, ph.scheme, n, o, k, F, ud);
                     bU -= F * (ho - ud);
                 }
             }
             acV += K;
-            if (ph.nonOrtho) bV += nonOrthoFlux(m, n, f, s.gVZ, s.gVR, mu);
+            if (ph.nonOrtho) bV += nonOrthoSlot(m, n, k, o, s.gVZ, s.gVR, mu);
             if (F != 0.0) {
                 acV += fmax(F, 0.0);
                 if (ph.scheme != 0) {
                     const double ud = (F > 0.0) ? s.v[n] : s.v[o];
-                    const double ho = higherOrder(m, bv, s.v, s.gVZ, s.gVR, ph.scheme, n, o, f, F, ud);
+                    const double ho = higherOrderSlot(m, s.v, s.gVZ, s.gVR, ph.scheme, n, o, k, F, ud);
                     bV -= F * (ho - ud);
                 }
             }
             s.AFuv[k] = af;
             if (s.AF32) s.AF32[k] = (float)af;
         } else {
+            const int f = m.faceIDs[k];
             boundaryTransport(m, bu, n, f, area, mu, F, acU, bU);
             boundaryTransport(m, bv, n, f, area, mu, F, acV, bV);
             s.AFuv[k] = 0.0;
@@ -418,6 +508,7 @@ struct UVOut {
             v[i] = b;
         }
     }
+    __device__ __forceinline__ void put(int i, double2 x) const { put(i, x.x, x.y); }
 };
 
 __global__ void __launch_bounds__(kBlock) kJacobiUV(DMesh m, DState s, UVIn xo, UVOut xn) {
@@ -437,6 +528,113 @@ __global__ void __launch_bounds__(kBlock) kJacobiUV(DMesh m, DState s, UVIn xo, 
     const double iu = s.uInv[n], iv = s.vInv[n];
     const double2 x = xo.at(n);
     xn.put(n, (iu != 0.0) ? au * iu : x.x, (iv != 0.0) ? av * iv : x.y);
+}
+
+// S Jacobi sweeps of u and v in one launch (temporal blocking).  Workgroup b
+// owns the C cells [bC, bC + C); sweep j updates them and the halo rings
+// 1 .. S - j, the iterates of sweeps 1 .. S - 1 live in LDS, and only sweep S
+// writes global memory.  Thread t holds two rows in registers for the whole
+// launch: block cell bC + t and halo entry t (a halo has at most C / 2 cells),
+// each as b, 1/AC and up to kSweepRegSlots off-diagonals with their
+// neighbours, all loaded up front, so the CSR and coefficients are read once
+// per S sweeps and no sweep waits on a chain of global loads.  A neighbour's
+// LDS slot is found once (block cells directly, halo cells by binary search in
+// the block's sorted halo).  Every row is the same expression as in kJacobiUV,
+// so the result is bit-identical to S launches of it.
+struct SweepRow {
+    double af[kSweepRegSlots];
+    int nb[kSweepRegSlots];   // neighbour cell, -1: none (boundary or past the row's slots)
+    int loc[kSweepRegSlots];  // the neighbour's LDS slot
+    double bu, bv, iu, iv;
+};
+
+__device__ __forceinline__ void loadSweepRow(SweepRow& r, const DMesh& m, const DState& s, int c) {
+    const int k0 = m.faceStart[c], ns = m.faceStart[c + 1] - k0;
+#pragma unroll
+    for (int q = 0; q < kSweepRegSlots; q++) {
+        r.nb[q] = (q < ns) ? m.slotNb[k0 + q] : -1;
+        r.af[q] = (q < ns) ? s.AFuv[k0 + q] : 0.0;
+    }
+    r.bu = s.ub[c];
+    r.bv = s.vb[c];
+    r.iu = s.uInv[c];
+    r.iv = s.vInv[c];
+}
+
+// jacobi's row (kJacobiUV) from registers; xat(q) = the neighbour of slot q
+template <typename XAt>
+__device__ __forceinline__ double2 sweepRow(const SweepRow& r, XAt xat, double2 self) {
+    double au = r.bu, av = r.bv;
+#pragma unroll
+    for (int q = 0; q < kSweepRegSlots; q++) {
+        if (r.nb[q] >= 0) {
+            const double2 x = xat(q);
+            au -= r.af[q] * x.x;
+            av -= r.af[q] * x.y;
+        }
+    }
+    return make_double2((r.iu != 0.0) ? au * r.iu : self.x, (r.iv != 0.0) ? av * r.iv : self.y);
+}
+
+template <int C>
+__device__ __forceinline__ int sweepLocal(int c, int a, int nOwn, const int2* keys, int nh) {
+    if (c >= a && c < a + nOwn) return c - a;
+    int lo = 0, hi = nh;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (keys[mid].x < c) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo < nh) ? keys[lo].y : 0;  // always found for the cells a sweep reads
+}
+
+template <int C>
+constexpr int sweepHalo() { return C / 2; }
+
+template <int C, int S>
+__global__ void __launch_bounds__(C) kJacobiUVBlock(DMesh m, DState s, DSweepBlocks sb, UVIn xo, UVOut xn) {
+    constexpr int H = sweepHalo<C>();
+    constexpr int kMask = S > 2 ? 1 : 0;  // sweep j < S writes LDS buffer (j - 1) & kMask
+    __shared__ double2 buf[kMask + 1][C + H];
+    __shared__ int2 keys[H];
+    if (finished(s.ctl)) return;
+    const int t = threadIdx.x, blk = blockIdx.x;
+    const int a = blk * C, nOwn = min(C, m.N - a);
+    const int h0 = sb.haloStart[blk], nh = sb.haloStart[blk + 1] - h0;
+    // halo entries of rings 1 .. S - 1: the ones sweep 1 updates
+    const int nHalo1 = (S > 1) ? sb.ringEnd[blk * sb.rings + S - 2] : 0;
+    const bool mine = t < nOwn, haloRow = t < nHalo1;
+    const int n = a + t;
+    const int hc = haloRow ? sb.halo[h0 + t] : 0;
+    SweepRow own, hal;
+    if (mine) loadSweepRow(own, m, s, n);
+    if (haloRow) loadSweepRow(hal, m, s, hc);
+    for (int i = t; i < nh; i += C) keys[i] = sb.sorted[h0 + i];
+    // sweep 1: global iterate in, LDS out
+    double2* out = buf[0];
+    if (mine) out[t] = sweepRow(own, [&](int q) { return xo.at(own.nb[q]); }, xo.at(n));
+    if (haloRow) out[C + t] = sweepRow(hal, [&](int q) { return xo.at(hal.nb[q]); }, xo.at(hc));
+    __syncthreads();
+    // halo entries updated again by sweeps 2 .. S - 1 (rings 1 .. S - 2)
+    const bool haloLds = (S > 2) && t < sb.ringEnd[blk * sb.rings + (S > 2 ? S - 3 : 0)];
+#pragma unroll
+    for (int q = 0; q < kSweepRegSlots; q++) {
+        if (mine) own.loc[q] = (own.nb[q] >= 0) ? sweepLocal<C>(own.nb[q], a, nOwn, keys, nh) : 0;
+        if (haloLds) hal.loc[q] = (hal.nb[q] >= 0) ? sweepLocal<C>(hal.nb[q], a, nOwn, keys, nh) : 0;
+    }
+#pragma unroll
+    for (int j = 2; j <= S; j++) {
+        const double2* in = buf[(j - 2) & kMask];
+        if (j == S) {
+            if (mine) xn.put(n, sweepRow(own, [&](int q) { return in[own.loc[q]]; }, in[t]));
+        } else {
+            double2* o = buf[(j - 1) & kMask];
+            const int nHaloJ = sb.ringEnd[blk * sb.rings + (S - j) - 1];
+            if (mine) o[t] = sweepRow(own, [&](int q) { return in[own.loc[q]]; }, in[t]);
+            if (t < nHaloJ) o[C + t] = sweepRow(hal, [&](int q) { return in[hal.loc[q]]; }, in[C + t]);
+            __syncthreads();
+        }
+    }
 }
 
 // gaussSeidelColorSweep, linear_solver.cu:42-72, u and v together
@@ -501,16 +699,16 @@ __global__ void __launch_bounds__(kBlock) kPPCoeff(DMesh m, DBC bp, DState s, Ph
     if (n >= m.N) return;
     double ac = 0.0;
     for (int k = m.faceStart[n]; k < m.faceStart[n + 1]; k++) {
-        const int f = m.faceIDs[k];
-        const int nb = m.neighbor[f];
-        const double area = m.area[f];
-        const double Df = normalCorrToFace(m, s, n, f);
+        const int o = m.slotNb[k];
+        const double area = m.sArea[k];
+        const double Df = normalCorrSlot(m, s, n, k, o);
         double af = 0.0;
-        if (nb >= 0) {
-            const double K = ph.rho * area * Df * m.invDPN[f];
+        if (o >= 0) {
+            const double K = ph.rho * area * Df * m.sInvDPN[k];
             ac += K;
             af -= K;
         } else {
+            const int f = m.faceIDs[k];
             const int g = m.group[f];
             if (g >= 0 && g < bp.nGroups && bp.type[g] == 0) {
                 const double K = ph.rho * area * Df / m.dPB[f];
@@ -544,13 +742,13 @@ __global__ void __launch_bounds__(kBlock) kPPRhs(DMesh m, DState s, Physics ph, 
     if (n >= m.N) return;
     double imb = 0.0, cs = 0.0;
     for (int k = m.faceStart[n]; k < m.faceStart[n + 1]; k++) {
-        const int f = m.faceIDs[k];
-        const double md = s.mDot[f];
-        if (m.owner[f] == n) imb += md;
-        else if (m.neighbor[f] == n) imb -= md;
+        const double md = s.mDot[m.faceIDs[k]];
+        if (slotOwns(m, k)) imb += md;
+        else imb -= md;
         if (cross) {
-            const double Df = normalCorrToFace(m, s, n, f);
-            cs += nonOrthoFlux(m, n, f, s.gPZ, s.gPR, ph.rho * Df);
+            const int o = m.slotNb[k];
+            const double Df = normalCorrSlot(m, s, n, k, o);
+            cs += nonOrthoSlot(m, n, k, o, s.gPZ, s.gPR, ph.rho * Df);
         }
     }
     s.ppb[n] = -imb + cs;
@@ -621,9 +819,9 @@ __global__ void __launch_bounds__(kBlock) kResidual(DMesh m, DState s) {
                 axu += a * s.u[nb];
                 axv += a * s.v[nb];
             }
-            const int f = m.faceIDs[k];
-            if (m.owner[f] == n) imb += s.mDot[f];
-            else if (m.neighbor[f] == n) imb -= s.mDot[f];
+            const double md = s.mDot[m.faceIDs[k]];
+            if (slotOwns(m, k)) imb += md;
+            else imb -= md;
         }
         const double ru = fabs(s.ub[n] - axu), rv = fabs(s.vb[n] - axv);
         v[0] = ru;
@@ -1277,6 +1475,27 @@ void launchJacobiUV(hipStream_t s, const DMesh& m, DState st, const double* uo, 
 void launchJacobiUVPacked(hipStream_t s, const DMesh& m, DState st, const double* uo, const double* vo,
                           const double2* uvo, double* un, double* vn, double2* uvn) {
     kJacobiUV<<<blocksFor(m.N, m.sweepBlock), m.sweepBlock, 0, s>>>(m, st, UVIn{uo, vo, uvo}, UVOut{un, vn, uvn});
+}
+int sweepHaloMax(int cells) {
+    if (cells == 512) return sweepHalo<512>();
+    if (cells == 1024) return sweepHalo<1024>();
+    return 0;
+}
+void launchJacobiUVBlock(hipStream_t s, const DMesh& m, DState st, const DSweepBlocks& sb, int sweeps,
+                         const double* uo, const double* vo, const double2* uvo, double* un, double* vn,
+                         double2* uvn) {
+    if (sweeps < 2 || sweeps > 3 || sweeps > sb.rings + 1) throw std::invalid_argument("bad fused sweep count");
+    const UVIn in{uo, vo, uvo};
+    const UVOut out{un, vn, uvn};
+    if (sb.cells == 512) {
+        if (sweeps == 2) kJacobiUVBlock<512, 2><<<sb.nBlocks, 512, 0, s>>>(m, st, sb, in, out);
+        else kJacobiUVBlock<512, 3><<<sb.nBlocks, 512, 0, s>>>(m, st, sb, in, out);
+    } else if (sb.cells == 1024) {
+        if (sweeps == 2) kJacobiUVBlock<1024, 2><<<sb.nBlocks, 1024, 0, s>>>(m, st, sb, in, out);
+        else kJacobiUVBlock<1024, 3><<<sb.nBlocks, 1024, 0, s>>>(m, st, sb, in, out);
+    } else {
+        throw std::invalid_argument("fused sweeps take blocks of 512 or 1024 cells");
+    }
 }
 void launchGSColorUV(hipStream_t s, const DMesh& m, DState st, const int* order, int begin, int count) {
     kGSColorUV<<<blocksFor(count), kBlock, 0, s>>>(m, st, order, begin, count);

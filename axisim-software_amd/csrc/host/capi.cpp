@@ -178,6 +178,11 @@ axs_status axs_autotune(axs_solver* h, int repeats, double* ms_default, double* 
     return guarded(&h->s->lastError, [&] { h->s->autotune(repeats, ms_default, ms_tuned); });
 }
 
+axs_status axs_set_sweep_fusion(axs_solver* h, int sweeps, int block_cells) {
+    if (!h) return AXS_ERR_INVALID;
+    return guarded(&h->s->lastError, [&] { h->s->setSweepFusion(sweeps, block_cells); });
+}
+
 void axs_request_stop(axs_solver* h) {
     if (h && h->s) h->s->requestStop();
 }

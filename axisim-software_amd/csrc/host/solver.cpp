@@ -590,6 +590,7 @@ void Solver::setMesh(const axs_mesh_view& v) {
     mixedAllocated_ = false;
     nColors_ = 0;
     colorStart_.clear();
+    sweepBlocks_ = SweepBlocks{};
 }
 
 void Solver::setBC(int var, const axs_bc_view& v) {
@@ -667,6 +668,21 @@ void Solver::uploadMesh() {
     d.lsqWZ = meshArena_.upload(geom_.lsqWZ);
     d.lsqWR = meshArena_.upload(geom_.lsqWR);
     d.length2D = meshArena_.upload(geom_.length2D);
+    {
+        SlotGeometry sg;
+        computeSlotGeometry(m, geom_, sg);
+        d.sArea = meshArena_.upload(sg.area);
+        d.sInvDPN = meshArena_.upload(sg.invDPN);
+        d.sNZ = meshArena_.upload(sg.nz);
+        d.sNR = meshArena_.upload(sg.nr);
+        d.sW = meshArena_.upload(sg.w);
+        d.sTZ = meshArena_.upload(sg.tz);
+        d.sTR = meshArena_.upload(sg.tr);
+        d.sLsqWZ = meshArena_.upload(sg.lsqWZ);
+        d.sLsqWR = meshArena_.upload(sg.lsqWR);
+        d.sLen = meshArena_.upload(sg.len2D);
+        d.sFlags = meshArena_.upload(sg.flags);
+    }
     dm_ = d;
     meshUploaded_ = true;
 }
@@ -794,14 +810,21 @@ void Solver::enqueueIteration(hipStream_t s) {
     } else if (c.linear_max_iter >= 2) {
         // u, v -> (u,v) pairs -> ... -> (u,v) pairs -> u, v: the same sweeps, the
         // iterates in between interleaved; the last one lands in u, v directly
+        // Fused: chunks of up to fuse_ sweeps per launch, the same sequence of
+        // iterates; no launch reads and writes the same array.
         const int L = c.linear_max_iter;
         double2 *a = st.uvA, *b = st.uvB;
-        launchJacobiUVPacked(s, dm_, st, st.u, st.v, nullptr, nullptr, nullptr, a);
-        for (int it = 1; it < L - 1; it++) {
-            launchJacobiUVPacked(s, dm_, st, nullptr, nullptr, a, nullptr, nullptr, b);
-            std::swap(a, b);
+        const std::vector<int> chunks = sweepChunks(L);
+        for (size_t i = 0; i < chunks.size(); i++) {
+            const bool first = i == 0, last = i + 1 == chunks.size();
+            const double *uo = first ? st.u : nullptr, *vo = first ? st.v : nullptr;
+            const double2* uvo = first ? nullptr : a;
+            double *un = last ? st.u : nullptr, *vn = last ? st.v : nullptr;
+            double2* uvn = last ? nullptr : (first ? a : b);
+            if (chunks[i] == 1) launchJacobiUVPacked(s, dm_, st, uo, vo, uvo, un, vn, uvn);
+            else launchJacobiUVBlock(s, dm_, st, dsb_, chunks[i], uo, vo, uvo, un, vn, uvn);
+            if (!first) std::swap(a, b);
         }
-        launchJacobiUVPacked(s, dm_, st, nullptr, nullptr, a, st.u, st.v, nullptr);
     } else if (c.linear_max_iter == 1) {
         launchJacobiUV(s, dm_, st, st.u, st.v, st.uTmp, st.vTmp);
         launchCopy(s, st.u, st.uTmp, dm_.N, st.ctl);
@@ -850,7 +873,7 @@ void Solver::enqueueIteration(hipStream_t s) {
     launchFinish(s, st, partialBlocks_);
 }
 
-void Solver::buildGraph() {
+void Solver::dropGraph() {
     if (exec_) {
         AXS_HIP(hipGraphExecDestroy(exec_));
         exec_ = nullptr;
@@ -859,6 +882,10 @@ void Solver::buildGraph() {
         AXS_HIP(hipGraphDestroy(graph_));
         graph_ = nullptr;
     }
+}
+
+void Solver::buildGraph() {
+    dropGraph();
     AXS_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     try {
         enqueueIteration(stream_);
@@ -960,6 +987,7 @@ void Solver::run(axs_progress_cb cb, void* user, bool continueRun) {
         ds_.AF32 = nullptr;
         ds_.uvInv32 = nullptr;
     }
+    prepareSweepBlocks();
     if (c.linear_solver == AXS_LINEAR_GS_RB && nColors_ == 0) {
         std::vector<int> order;
         nColors_ = greedyColoring(mesh_.nCells, csrStart_.data(), csrNb_.data(), colorStart_, order);
@@ -1109,10 +1137,23 @@ void Solver::benchKernel(int kernel, int repeats, double* ms, double* bytes) {
         *bytes = 68.0 * N + 12.0 * S;
         launch = [&] { launchJacobiUVPacked(stream_, dm_, st, nullptr, nullptr, st.uvA, nullptr, nullptr, st.uvB); };
         break;
+    case AXS_KERNEL_JACOBI_UV_FUSED:
+        // fuse_ sweeps in one launch: the same distinct data as one sweep (the
+        // halo rows and the block rows read again are not counted)
+        if (fuse_ < 2) throw std::logic_error("fused momentum sweeps are not in use");
+        *bytes = 68.0 * N + 12.0 * S;
+        launch = [&] {
+            launchJacobiUVBlock(stream_, dm_, st, dsb_, fuse_, nullptr, nullptr, st.uvA, nullptr, nullptr, st.uvB);
+        };
+        break;
     case AXS_KERNEL_MOMENTUM: {
         const bool needGrad = ph.nonOrtho || (cfg_.add_convection && ph.scheme == AXS_CONV_SOU);
-        *bytes = 4.0 * N + 24.0 * N + (lsq ? 24.0 * N : 8.0 * N) + 24.0 * N + 80.0 * N + 12.0 * F + 88.0 * F +
-                 12.0 * S + (needGrad ? 32.0 * N : 0.0);
+        // cells: faceStart, vol, cR, u, v, p, the gradient inverse, 10 outputs (+
+        // the u, v gradients); slots: slotNb, faceIDs, flags, area, 1/|d|, the
+        // non-orthogonal vector, weight, mDot (gather), AF out (+ LSQ weights, or
+        // the oriented normal and length)
+        *bytes = 4.0 * N + 40.0 * N + (lsq ? 24.0 * N : 8.0 * N) + 80.0 * N + (needGrad ? 32.0 * N : 0.0) +
+                 65.0 * S + (lsq ? 16.0 * S : 25.0 * S);
         launch = [&] { launchMomentum(stream_, dm_, dbc_[0], dbc_[1], dbc_[2], st, ph); };
         break;
     }
@@ -1128,7 +1169,9 @@ void Solver::benchKernel(int kernel, int repeats, double* ms, double* bytes) {
         launch = [&] { launchFaceFlux(stream_, dm_, dbc_[0], dbc_[1], dbc_[2], st, ph); };
         break;
     case AXS_KERNEL_PP_COEFF:
-        *bytes = 36.0 * N + 12.0 * S + 60.0 * F;
+        // cells: faceStart, DU, DV, AC and 1/AC out; slots: slotNb, area, 1/|d|,
+        // the oriented normal, weight, AF out (boundary slots' face data aside)
+        *bytes = 36.0 * N + 52.0 * S;
         launch = [&] { launchPPCoeff(stream_, dm_, dbc_[2], st, ph); };
         break;
     default: throw std::invalid_argument("unknown kernel id");
@@ -1288,30 +1331,89 @@ int Solver::residualHistory(int res, double* out, int maxN) const {
 
 // Algorithmic bytes of one outer iteration of this schedule (distinct
 // elements read + written per launch; FP64 8 B, int32 4 B).
+void Solver::setSweepFusion(int sweeps, int blockCells) {
+    if (running_) throw std::logic_error("solver is running");
+    if (sweeps < 1 || sweeps > 3) throw std::invalid_argument("sweeps per launch must be 1, 2 or 3");
+    if (blockCells != 512 && blockCells != 1024) throw std::invalid_argument("fused blocks hold 512 or 1024 cells");
+    fuseReq_ = sweeps;
+    fuseCellsReq_ = blockCells;
+}
+
+// Build (or reuse) the blocks and halo rings for the largest sweep count
+// <= fuseReq_ whose halos fit the kernel's LDS and whose cells' rows fit its
+// registers; fuse_ = 1 when none does.
+void Solver::prepareSweepBlocks() {
+    fuse_ = 1;
+    const int hmax = sweepHaloMax(fuseCellsReq_);
+    for (int s = fuseReq_; s >= 2; s--) {
+        if (sweepBlocks_.cells != fuseCellsReq_ || sweepBlocks_.rings != s - 1) {
+            SweepBlocks sb;
+            buildSweepBlocks(mesh_.nCells, csrStart_, csrNb_, fuseCellsReq_, s - 1, sb);
+            if (sb.maxSlots > kSweepRegSlots || sb.maxHalo > hmax) continue;
+            sweepBlocks_ = std::move(sb);
+            dropGraph();  // it holds the old blocks' addresses
+            AXS_HIP(hipStreamSynchronize(stream_));
+            sweepArena_.release();
+            dsb_.cells = sweepBlocks_.cells;
+            dsb_.rings = sweepBlocks_.rings;
+            dsb_.nBlocks = sweepBlocks_.nBlocks;
+            dsb_.haloStart = sweepArena_.upload(sweepBlocks_.haloStart);
+            dsb_.ringEnd = sweepArena_.upload(sweepBlocks_.ringEnd);
+            dsb_.halo = sweepArena_.upload(sweepBlocks_.halo.empty() ? std::vector<int>{0} : sweepBlocks_.halo);
+            dsb_.sorted = reinterpret_cast<const int2*>(
+                sweepArena_.upload(sweepBlocks_.sorted.empty() ? std::vector<int>{0, 0} : sweepBlocks_.sorted));
+        }
+        fuse_ = s;
+        return;
+    }
+}
+
+// Sweeps per launch for a momentum solve of `sweeps` Jacobi sweeps: chunks of
+// fuse_, none of 1 when avoidable, and at least two launches (the first reads
+// u, v and the last writes them).
+std::vector<int> Solver::sweepChunks(int sweeps) const {
+    std::vector<int> out;
+    if (fuse_ >= 2) {
+        for (int left = sweeps; left > 0;) {
+            int n = std::min(fuse_, left);
+            if (left - n == 1 && n > 2) n--;
+            out.push_back(n);
+            left -= n;
+        }
+    }
+    if (out.size() < 2) out.assign(sweeps, 1);
+    return out;
+}
+
 double Solver::bytesPerIteration() const {
     const double N = mesh_.nCells, F = mesh_.nFaces, S = (double)mesh_.cellFaceIDs.size();
     const axs_config& c = cfg_;
     const bool lsq = c.gradient_scheme == AXS_GRAD_LSQ;
     const bool needGrad = c.add_convection && (c.convection_scheme == AXS_CONV_SOU || c.convection_scheme == AXS_CONV_QUICK);
-    const double gradGeom = lsq ? (28.0 * N + 4.0 * S + 24.0 * F) : (12.0 * N + 4.0 * S + 48.0 * F);
+    // Distinct bytes each kernel reads and writes with this layout: per-cell
+    // arrays, the slot-ordered face data of the cell loops (per slot: slotNb 4,
+    // the oriented geometry it uses, faceIDs 4 + flags 1 where it reads mDot),
+    // face arrays in the per-face kernels.
+    const double gradGeom = lsq ? (28.0 * N + 20.0 * S) : (12.0 * N + 37.0 * S);
     double b = 0.0;
-    if (c.use_non_orth || needGrad) b += gradGeom + 4 * 8.0 * N + 2 * 8.0 * N;              // grad u, v
-    b += gradGeom + 8.0 * N + 16.0 * N + 8.0 * F                                            // grad p, mDot
-         + 8.0 * 16 * F + 8.0 * 7 * N + 8.0 * S + 4.0 * S + 12.0 * F                          // faces, cells, AF
-         + 8.0 * 8 * N;                                                                     // 8 outputs
-    b += c.linear_max_iter * (68.0 * N + 12.0 * S);                                         // u, v sweeps
-    b += 16.0 * N + 4.0 * S + 8.0 * 6 * F + 8.0 * S + 16.0 * N;                             // pp coeff
+    if (c.use_non_orth || needGrad) b += gradGeom + 2 * 8.0 * N + 4 * 8.0 * N;              // grad u, v (one pass)
+    b += gradGeom + 8.0 * N                                                                 // grad p
+         + 65.0 * S + 8.0 * 4 * N + (c.use_non_orth || needGrad ? 32.0 * N : 0.0)            // slots, cells
+         + 8.0 * 10 * N;                                                                    // 10 outputs
+    const bool fused = c.linear_solver == AXS_LINEAR_JACOBI && c.precision != AXS_PREC_MIXED;
+    b += (fused ? sweepChunks(c.linear_max_iter).size() : c.linear_max_iter) * (68.0 * N + 12.0 * S);  // u, v sweeps
+    b += 16.0 * N + 44.0 * S + 8.0 * S + 16.0 * N;                                          // pp coeff
     b += 8.0 * 6 * N + 8.0 * 9 * F + 8.0 * F;                                               // face flux
     b += 8.0 * N;                                                                           // pp = 0
     const int passes = c.use_non_orth ? 2 : 1;
     for (int p = 0; p < passes; p++) {
-        b += 8.0 * F + 4.0 * S + 4.0 * N + 8.0 * N;                                          // pp rhs
-        if (p > 0) b += gradGeom + 8.0 * N + 16.0 * N + 16.0 * N + 16.0 * F;                 // grad p', cross
+        b += 13.0 * S + 4.0 * N + 8.0 * N;                                                   // pp rhs
+        if (p > 0) b += gradGeom + 8.0 * N + 16.0 * N + 44.0 * S + 32.0 * N;                 // grad p', cross
         b += mg_.host.empty() ? 0.0 : mg_.bytesPerSolve(mgParamsOf(c), p == 0);
     }
     b += gradGeom + 8.0 * N + 16.0 * N + 8.0 * 6 * N + 8.0 * 3 * N;                         // grad p' + update
     b += 8.0 * 10 * F + 8.0 * 4 * N + 8.0 * F;                                              // mass flux
-    b += 8.0 * 6 * N + 12.0 * S + 8.0 * F + 4.0 * S;                                        // residuals
+    b += 8.0 * 6 * N + 12.0 * S + 13.0 * S;                                                 // residuals
     return b;
 }
 
@@ -1360,6 +1462,43 @@ void Solver::autotune(int repeats, double* msDefault, double* msTuned) {
     // the interleaved momentum sweep, writing the sweep scratch buffers
     tune([&](int c) { dm_.sweepBlock = c; },
          [&] { launchJacobiUVPacked(stream_, dm_, st, nullptr, nullptr, st.uvA, nullptr, nullptr, st.uvB); });
+    // sweeps fused per launch and the fused block size, timed over a whole FP64
+    // momentum solve (scratch in, scratch out); the default is 3 sweeps of
+    // 512-cell blocks
+    if (cfg_.linear_solver == AXS_LINEAR_JACOBI && cfg_.precision != AXS_PREC_MIXED && cfg_.linear_max_iter >= 2) {
+        const int L = cfg_.linear_max_iter;
+        auto solveTime = [&] {
+            const std::vector<int> ch = sweepChunks(L);
+            return timeIt([&] {
+                for (int n : ch) {
+                    if (n == 1) launchJacobiUVPacked(stream_, dm_, st, nullptr, nullptr, st.uvA, nullptr, nullptr, st.uvB);
+                    else launchJacobiUVBlock(stream_, dm_, st, dsb_, n, nullptr, nullptr, st.uvA, nullptr, nullptr, st.uvB);
+                }
+            });
+        };
+        // the default request first (whatever count its halos allow), then
+        // every count and block size that fits
+        const int options[][2] = {{3, 512}, {1, 512}, {2, 512}, {2, 1024}, {3, 1024}};
+        double bt = 1e300;
+        int bs = 1, bcells = 512;
+        for (size_t i = 0; i < sizeof(options) / sizeof(options[0]); i++) {
+            fuseReq_ = options[i][0];
+            fuseCellsReq_ = options[i][1];
+            prepareSweepBlocks();
+            if (i > 0 && fuse_ != options[i][0]) continue;  // halo or rows do not fit
+            const double t = solveTime();
+            if (i == 0) def += t;
+            if (t < bt) {
+                bt = t;
+                bs = fuse_;
+                bcells = options[i][1];
+            }
+        }
+        fuseReq_ = bs;
+        fuseCellsReq_ = bcells;
+        prepareSweepBlocks();
+        best += bt;
+    }
     // each multigrid level outside the single-workgroup tail: smoothing,
     // residual and prolongation kernels (they write only the level's scratch)
     if (cfg_.use_multigrid && mgBuilt_) {
@@ -1481,6 +1620,8 @@ void Solver::stats(axs_stats& s) const {
     for (int i = 0; i < 3; i++) s.last_residual[i] = history_[i].empty() ? 0.0 : history_[i].back();
     s.bytes_per_iteration = haveMesh_ ? bytesPerIteration() : 0.0;
     s.vcycle_bytes = mg_.host.empty() ? 0.0 : mg_.bytesPerSolve(mgParamsOf(cfg_), true);
+    s.sweeps_per_launch = fuse_;
+    s.sweep_block_cells = fuse_ >= 2 ? sweepBlocks_.cells : 0;
 }
 
 }  // namespace axs

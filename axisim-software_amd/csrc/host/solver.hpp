@@ -165,6 +165,11 @@ public:
     // one with at most kGuessCells cells.
     void setLearnedGuess(class Mpnn* net, int level);
     static constexpr int kGuessCells = 16384;
+    // Momentum Jacobi sweeps fused per launch (1 = one sweep per launch, 2 or
+    // 3) and the cells per fused block (512 or 1024).  The next run uses the
+    // largest count <= sweeps whose halos fit; results are unchanged.
+    void setSweepFusion(int sweeps, int blockCells);
+    int sweepsPerLaunch() const { return fuse_; }
 
     std::string lastError;
 
@@ -222,6 +227,16 @@ private:
     void enqueueGuess(hipStream_t s, const MGParams& mp);
     bool mgBuilt_ = false;
     int mgBuiltLevels_ = -1;
+    // fused momentum sweeps: requested and in-use sweeps per launch, the blocks
+    // and halo rings (host and device copies, built for fuseCellsReq_ and
+    // fuse_ - 1 rings)
+    int fuseReq_ = 3, fuseCellsReq_ = 512, fuse_ = 1;
+    SweepBlocks sweepBlocks_;
+    DSweepBlocks dsb_{};
+    Arena sweepArena_;
+    void prepareSweepBlocks();
+    void dropGraph();
+    std::vector<int> sweepChunks(int sweeps) const;
     std::vector<int> colorStart_;
     int* dOrder_ = nullptr;
     Arena colorArena_;

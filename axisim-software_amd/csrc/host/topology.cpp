@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <stdexcept>
 
 namespace axs {
 
@@ -71,6 +72,43 @@ void computeGeometry(const HostMesh& m, Geometry& g) {
     }
 }
 
+void computeSlotGeometry(const HostMesh& m, const Geometry& g, SlotGeometry& s) {
+    const size_t S = m.cellFaceIDs.size();
+    for (auto* v : {&s.area, &s.invDPN, &s.nz, &s.nr, &s.w, &s.tz, &s.tr, &s.lsqWZ, &s.lsqWR, &s.len2D})
+        v->assign(S, 0.0);
+    s.flags.assign(S, 0);
+    for (int c = 0; c < m.nCells; c++)
+        for (int k = m.cellFaceStart[c]; k < m.cellFaceStart[c + 1]; k++) {
+            const int f = m.cellFaceIDs[k];
+            if (f < 0 || f >= m.nFaces) continue;
+            const int ow = m.faceOwner[f], nb = m.faceNeighbor[f];
+            const bool own = ow == c;
+            // orientation as in nonOrthoScalarDiffusionFlux / phiGradientGreenGauss:
+            // the stored normal is flipped where c is the face's neighbour
+            const bool flip = nb == c;
+            const double nz = flip ? -m.faceNormalZ[f] : m.faceNormalZ[f];
+            const double nr = flip ? -m.faceNormalR[f] : m.faceNormalR[f];
+            s.area[k] = m.faceArea[f];
+            s.invDPN[k] = g.invDPN[f];
+            s.nz[k] = nz;
+            s.nr[k] = nr;
+            s.w[k] = own ? g.wP[f] : 1.0 - g.wP[f];
+            s.lsqWZ[k] = own ? g.lsqWZ[f] : -g.lsqWZ[f];
+            s.lsqWR[k] = own ? g.lsqWR[f] : -g.lsqWR[f];
+            s.len2D[k] = g.length2D[f];
+            s.flags[k] = (uint8_t)((own ? SlotGeometry::kOwner : 0) | (m.faceCenterR[f] > 1.0e-30 ? 0 : SlotGeometry::kAxis));
+            if (nb >= 0) {
+                const int o = own ? nb : ow;
+                const double area = m.faceArea[f];
+                const double dz = m.cellCenterZ[o] - m.cellCenterZ[c];
+                const double dr = m.cellCenterR[o] - m.cellCenterR[c];
+                const double aOverNd = area * g.invDPN[f];
+                s.tz[k] = area * nz - aOverNd * dz;
+                s.tr[k] = area * nr - aOverNd * dr;
+            }
+        }
+}
+
 void cellNeighbourCSR(const HostMesh& m, std::vector<int>& start, std::vector<int>& nb) {
     start.assign(m.cellFaceStart.begin(), m.cellFaceStart.end());
     start[0] = 0;
@@ -85,6 +123,53 @@ void cellNeighbourCSR(const HostMesh& m, std::vector<int>& start, std::vector<in
             }
             nb[k] = other;
         }
+}
+
+void buildSweepBlocks(int nCells, const std::vector<int>& start, const std::vector<int>& nb, int cells, int rings,
+                      SweepBlocks& out) {
+    if (cells <= 0 || rings < 1) throw std::invalid_argument("sweep blocks need cells > 0 and rings >= 1");
+    out = SweepBlocks{};
+    out.cells = cells;
+    out.rings = rings;
+    out.nBlocks = (nCells + cells - 1) / cells;
+    out.haloStart.assign(1, 0);
+    out.ringEnd.reserve((size_t)out.nBlocks * rings);
+    for (int c = 0; c < nCells; c++) out.maxSlots = std::max(out.maxSlots, start[c + 1] - start[c]);
+    std::vector<int> stamp(nCells, -1), frontier, next, ring;
+    for (int b = 0; b < out.nBlocks; b++) {
+        const int a = b * cells, e = std::min(nCells, a + cells);
+        const size_t h0 = out.halo.size();
+        frontier.clear();
+        for (int c = a; c < e; c++) {
+            stamp[c] = b;
+            frontier.push_back(c);
+        }
+        for (int r = 1; r <= rings; r++) {
+            ring.clear();
+            for (int c : frontier)
+                for (int k = start[c]; k < start[c + 1]; k++) {
+                    const int o = nb[k];
+                    if (o >= 0 && stamp[o] != b) {
+                        stamp[o] = b;
+                        ring.push_back(o);
+                    }
+                }
+            std::sort(ring.begin(), ring.end());
+            out.halo.insert(out.halo.end(), ring.begin(), ring.end());
+            out.ringEnd.push_back((int)(out.halo.size() - h0));
+            frontier.swap(ring);
+        }
+        const int nh = (int)(out.halo.size() - h0);
+        out.maxHalo = std::max(out.maxHalo, nh);
+        next.resize(nh);
+        for (int i = 0; i < nh; i++) next[i] = i;
+        std::sort(next.begin(), next.end(), [&](int x, int y) { return out.halo[h0 + x] < out.halo[h0 + y]; });
+        for (int i : next) {
+            out.sorted.push_back(out.halo[h0 + i]);
+            out.sorted.push_back(cells + i);
+        }
+        out.haloStart.push_back((int)out.halo.size());
+    }
 }
 
 int greedyColoring(int n, const int* start, const int* nb, std::vector<int>& colorStart,

@@ -1,6 +1,7 @@
 // topology.hpp -- host-side set-up builders of the solver (run once per mesh).
 #pragma once
 
+#include <cstdint>
 #include <vector>
 
 #include "mesh_types.hpp"
@@ -13,8 +14,44 @@ struct Geometry {
 };
 void computeGeometry(const HostMesh& m, Geometry& g);
 
+// The face data the cell loops read, copied into slot order (slot k of cell c
+// = face cellFaceIDs[k]) and oriented for c.  Every entry is the face value,
+// its negation, 1 - w, or the reference's own expression for the non-orthogonal
+// vector evaluated once here, so the kernels reading it compute bit-identical
+// results while streaming slots instead of gathering faces from both sides.
+struct SlotGeometry {
+    enum : uint8_t { kOwner = 1, kAxis = 2 };
+    std::vector<double> area, invDPN;   // face area, 1/|d_PN|
+    std::vector<double> nz, nr;         // normal, negated where c is the neighbour
+    std::vector<double> w;              // interpolation weight of c: wP, or 1 - wP for the neighbour
+    std::vector<double> tz, tr;         // A n - (A / |d_PN|) d_cN (interior faces)
+    std::vector<double> lsqWZ, lsqWR;   // LSQ weights, negated where c is the neighbour
+    std::vector<double> len2D;          // planar face length
+    std::vector<uint8_t> flags;         // kOwner: c owns the face; kAxis: face centre on the axis
+};
+void computeSlotGeometry(const HostMesh& m, const Geometry& g, SlotGeometry& s);
+
 // Cell -> neighbour CSR in cell.faceIDs order (buildCellFaceCSR, memory_manger.cpp:287-323).
 void cellNeighbourCSR(const HostMesh& m, std::vector<int>& start, std::vector<int>& nb);
+
+// Blocks of `cells` consecutive cells and their halo rings, for running several
+// Jacobi sweeps in one workgroup: ring r of a block holds the cells at graph
+// distance r from it.  Sweep j of s fused sweeps updates the block and rings
+// 1 .. s - j, so s sweeps need rings 1 .. s - 1.  Per block, the halo lists
+// ring 1's cells, then ring 2's, ..., each ring ascending; `sorted` lists the
+// same cells ascending as (cell, local index) pairs, local index = cells + the
+// cell's position in the halo.
+struct SweepBlocks {
+    int cells = 0, rings = 0, nBlocks = 0;
+    int maxHalo = 0;   // largest halo of any block
+    int maxSlots = 0;  // most slots of any cell
+    std::vector<int> haloStart;  // nBlocks + 1 offsets into halo / sorted
+    std::vector<int> ringEnd;    // [block * rings + r - 1]: halo entries of rings 1..r
+    std::vector<int> halo;
+    std::vector<int> sorted;     // interleaved (cell, local index)
+};
+void buildSweepBlocks(int nCells, const std::vector<int>& start, const std::vector<int>& nb, int cells, int rings,
+                      SweepBlocks& out);
 
 // Greedy smallest-free colouring in cell order, colour-major order
 // (buildMeshColoring, memory_manger.cpp:334-409).  Returns the colour count.

@@ -211,6 +211,8 @@ typedef struct axs_stats {
     double last_residual[AXS_RES_COUNT];
     double bytes_per_iteration;  /* algorithmic bytes of one outer iteration       */
     double vcycle_bytes;         /* algorithmic bytes of one p' multigrid solve    */
+    int sweeps_per_launch;       /* momentum Jacobi sweeps fused per launch        */
+    int sweep_block_cells;       /* cells per fused block (0: not fused)          */
 } axs_stats;
 axs_status axs_get_stats(axs_solver* s, axs_stats* out);
 
@@ -228,7 +230,8 @@ enum {
     AXS_KERNEL_MOMENTUM = 1,    /* fused momentum assembly                   */
     AXS_KERNEL_MG_SMOOTH = 2,   /* one weighted-Jacobi sweep on the p' level 0 */
     AXS_KERNEL_FACE_FLUX = 3,   /* Rhie-Chow face mass flux                  */
-    AXS_KERNEL_PP_COEFF = 4     /* pressure-correction matrix                */
+    AXS_KERNEL_PP_COEFF = 4,    /* pressure-correction matrix                */
+    AXS_KERNEL_JACOBI_UV_FUSED = 5  /* the fused launch of several u+v sweeps  */
 };
 axs_status axs_bench_kernel(axs_solver* s, int kernel, int repeats, double* ms, double* bytes);
 /* Autotuner (build extension): choose the workgroup size (64, 128 or 256
@@ -237,6 +240,13 @@ axs_status axs_bench_kernel(axs_solver* s, int kernel, int repeats, double* ms, 
  * uses the choice.  *ms_default / *ms_tuned (may be NULL): summed kernel times
  * of one pass at 256 threads and at the chosen sizes. */
 axs_status axs_autotune(axs_solver* s, int repeats, double* ms_default, double* ms_tuned);
+/* Momentum Jacobi sweeps per launch (build extension): `sweeps` (1, 2 or 3)
+ * sweeps of u and v run in one launch, one workgroup per block of
+ * `block_cells` (512 or 1024) consecutive cells with the intermediate iterates
+ * in LDS; the results are bit-identical to one sweep per launch.  The next run
+ * uses the largest count <= sweeps that the mesh's halos allow (axs_stats::
+ * sweeps_per_launch).  Default 3 sweeps, 512 cells; axs_autotune also picks it. */
+axs_status axs_set_sweep_fusion(axs_solver* s, int sweeps, int block_cells);
 
 /* ---------------- Agglomeration multigrid, stand-alone ---------------- */
 typedef struct axs_mg axs_mg;
