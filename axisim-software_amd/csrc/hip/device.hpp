@@ -87,6 +87,8 @@ struct Physics {
 struct DLevel {
     int n = 0, refs = 0, nCoarse = 0;
     int block = kBlock;  // workgroup size of this level's per-cell kernels (autotuned)
+    int maxRow = 0;      // most off-diagonal slots of a row
+    int maxMembers = 0;  // most fine cells of one coarse cell (levels with a coarser one)
     int *start = nullptr, *nb = nullptr;
     double *AC = nullptr, *b = nullptr, *AF = nullptr;
     double *x = nullptr, *xNew = nullptr;

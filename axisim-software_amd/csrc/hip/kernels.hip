@@ -107,40 +107,34 @@ __device__ __forceinline__ double nonOrthoFlux(const DMesh& m, int c, int f, con
 // The cell loops read the face data in slot order, oriented for c (sNZ/sNR =
 // the normal pointing out of c, sW = c's interpolation weight, sTZ/sTR = the
 // non-orthogonal vector A n - (A/|d|) d towards o, sLsqW* signed so that
-// dphi = phi_o - phi_c), which the host derives from the face arrays with the
-// reference's own expressions.  Each result is bit-identical to the face form;
-// only boundary slots still read face-indexed data.
-__device__ __forceinline__ bool slotOwns(const DMesh& m, int k) { return (m.sFlags[k] & 1) != 0; }
+// dphi = phi_o - phi_c, sFlags bit 0 = c owns the face), which the host derives
+// from the face arrays with the reference's own expressions.  Each result is
+// bit-identical to the face form (the squares of the normal in
+// normalCorrToFace do not see its orientation); only boundary slots still read
+// face-indexed data.
+// Cells with at most kSlotBatch slots (triangles, quads) walk their slots in
+// three dependent rounds of loads instead of one round per slot: every slot's
+// data (indices clamped into the cell's range, so every load is in bounds and
+// can issue at once), then every neighbour gather, then the arithmetic slot by
+// slot in the reference's order.  Larger cells take the plain loop.
+constexpr int kSlotBatch = 4;
+__device__ __forceinline__ int slotAt(int k0, int ns, int q) { return k0 + (q < ns ? q : ns - 1); }
 
-// normalCorrToFace; the squares of the normal do not see its orientation
-__device__ __forceinline__ double normalCorrSlot(const DMesh& m, const DState& s, int c, int k, int o) {
-    const double nz = m.sNZ[k], nr = m.sNR[k];
-    const double DP = normalCorr(s, c, nz, nr);
-    if (o < 0) return DP;
-    const double DN = normalCorr(s, o, nz, nr);
-    const double w = m.sW[k];
-    return w * DP + (1.0 - w) * DN;
-}
-
-__device__ __forceinline__ double interpSlot(const DMesh& m, const DBC& bc, const double* phi, int c, int k,
-                                             int o) {
-    const double phiP = phi[c];
-    if (o >= 0) {
-        const double w = m.sW[k];
-        return w * phiP + (1.0 - w) * phi[o];
-    }
-    return boundaryFaceValue(m, bc, phiP, m.faceIDs[k]);
-}
-
-// nonOrthoFlux: zero on a boundary slot
-__device__ __forceinline__ double nonOrthoSlot(const DMesh& m, int c, int k, int o, const double* gz,
-                                               const double* gr, double gamma) {
-    if (o < 0) return 0.0;
-    const double Tz = m.sTZ[k], Tr = m.sTR[k];
-    const double w = m.sW[k];
-    const double gfz = w * gz[c] + (1.0 - w) * gz[o];
-    const double gfr = w * gr[c] + (1.0 - w) * gr[o];
+// nonOrthoFlux from loaded values (T, w, the gradients at c and o)
+__device__ __forceinline__ double nonOrthoVal(double Tz, double Tr, double w, double gzc, double gzo, double grc,
+                                              double gro, double gamma) {
+    const double gfz = w * gzc + (1.0 - w) * gzo;
+    const double gfr = w * grc + (1.0 - w) * gro;
     return gamma * (Tz * gfz + Tr * gfr);
+}
+
+// normalCorrToFace from loaded values (o < 0: boundary slot)
+__device__ __forceinline__ double normalCorrVal(double DUc, double DVc, double DUo, double DVo, double nz, double nr,
+                                                double w, int o) {
+    const double DP = DUc * nz * nz + DVc * nr * nr;
+    if (o < 0) return DP;
+    const double DN = DUo * nz * nz + DVo * nr * nr;
+    return w * DP + (1.0 - w) * DN;
 }
 
 // phiGradientGreenGauss / phiGradientLeastSquare, solver_util.cuh:458-588, of
@@ -151,7 +145,8 @@ __device__ __forceinline__ double nonOrthoSlot(const DMesh& m, int c, int k, int
 template <int NF>
 __device__ void cellGradients(const DMesh& m, const DBC* bc, const double* const* phi, int c, int scheme,
                               double* gZ, double* gR) {
-    const int k0 = m.faceStart[c], k1 = m.faceStart[c + 1];
+    const int k0 = m.faceStart[c], ns = m.faceStart[c + 1] - k0;
+    const bool batch = ns > 0 && ns <= kSlotBatch;
     double phiP[NF];
 #pragma unroll
     for (int i = 0; i < NF; i++) phiP[i] = phi[i][c];
@@ -161,21 +156,17 @@ __device__ void cellGradients(const DMesh& m, const DBC* bc, const double* const
 #pragma unroll
         for (int i = 0; i < NF; i++) gz[i] = gr[i] = axisPhi[i] = 0.0;
         bool axis = false;
-        for (int k = k0; k < k1; k++) {
-            const int o = m.slotNb[k];
-            const double nz = m.sNZ[k], nr = m.sNR[k];
+        auto step = [&](int k, int o, double nz, double nr, double w, uint8_t fl, double L, const double* phiO) {
             double phiF[NF];
             if (o >= 0) {
-                const double w = m.sW[k];
 #pragma unroll
-                for (int i = 0; i < NF; i++) phiF[i] = w * phiP[i] + (1.0 - w) * phi[i][o];
+                for (int i = 0; i < NF; i++) phiF[i] = w * phiP[i] + (1.0 - w) * phiO[i];
             } else {
                 const int f = m.faceIDs[k];
 #pragma unroll
                 for (int i = 0; i < NF; i++) phiF[i] = boundaryFaceValue(m, bc[i], phiP[i], f);
             }
-            if (!(m.sFlags[k] & 2)) {
-                const double L = m.sLen[k];
+            if (!(fl & 2)) {
 #pragma unroll
                 for (int i = 0; i < NF; i++) {
                     gz[i] += phiF[i] * nz * L;
@@ -187,6 +178,39 @@ __device__ void cellGradients(const DMesh& m, const DBC* bc, const double* const
                 axis = true;
 #pragma unroll
                 for (int i = 0; i < NF; i++) axisPhi[i] = phiF[i];
+            }
+        };
+        if (batch) {
+            int o[kSlotBatch];
+            uint8_t fl[kSlotBatch];
+            double nz[kSlotBatch], nr[kSlotBatch], w[kSlotBatch], L[kSlotBatch], po[kSlotBatch][NF];
+#pragma unroll
+            for (int q = 0; q < kSlotBatch; q++) {
+                const int k = slotAt(k0, ns, q);
+                o[q] = m.slotNb[k];
+                nz[q] = m.sNZ[k];
+                nr[q] = m.sNR[k];
+                w[q] = m.sW[k];
+                fl[q] = m.sFlags[k];
+                L[q] = m.sLen[k];
+            }
+#pragma unroll
+            for (int q = 0; q < kSlotBatch; q++) {
+                const int oc = o[q] >= 0 ? o[q] : c;
+#pragma unroll
+                for (int i = 0; i < NF; i++) po[q][i] = phi[i][oc];
+            }
+#pragma unroll
+            for (int q = 0; q < kSlotBatch; q++)
+                if (q < ns) step(k0 + q, o[q], nz[q], nr[q], w[q], fl[q], L[q], po[q]);
+        } else {
+            for (int k = k0; k < k0 + ns; k++) {
+                const int o = m.slotNb[k];
+                const int oc = o >= 0 ? o : c;
+                double po[NF];
+#pragma unroll
+                for (int i = 0; i < NF; i++) po[i] = phi[i][oc];
+                step(k, o, m.sNZ[k], m.sNR[k], m.sW[k], m.sFlags[k], m.sLen[k], po);
             }
         }
         const double inv = m.invA2D[c];
@@ -203,13 +227,11 @@ __device__ void cellGradients(const DMesh& m, const DBC* bc, const double* const
         double bz[NF], br[NF];
 #pragma unroll
         for (int i = 0; i < NF; i++) bz[i] = br[i] = 0.0;
-        for (int k = k0; k < k1; k++) {
-            const int o = m.slotNb[k];
-            const double wz = m.sLsqWZ[k], wr = m.sLsqWR[k];
+        auto step = [&](int k, int o, double wz, double wr, const double* phiO) {
             double dphi[NF];
             if (o >= 0) {
 #pragma unroll
-                for (int i = 0; i < NF; i++) dphi[i] = phi[i][o] - phiP[i];
+                for (int i = 0; i < NF; i++) dphi[i] = phiO[i] - phiP[i];
             } else {
                 const int f = m.faceIDs[k];
 #pragma unroll
@@ -219,6 +241,35 @@ __device__ void cellGradients(const DMesh& m, const DBC* bc, const double* const
             for (int i = 0; i < NF; i++) {
                 bz[i] += wz * dphi[i];
                 br[i] += wr * dphi[i];
+            }
+        };
+        if (batch) {
+            int o[kSlotBatch];
+            double wz[kSlotBatch], wr[kSlotBatch], po[kSlotBatch][NF];
+#pragma unroll
+            for (int q = 0; q < kSlotBatch; q++) {
+                const int k = slotAt(k0, ns, q);
+                o[q] = m.slotNb[k];
+                wz[q] = m.sLsqWZ[k];
+                wr[q] = m.sLsqWR[k];
+            }
+#pragma unroll
+            for (int q = 0; q < kSlotBatch; q++) {
+                const int oc = o[q] >= 0 ? o[q] : c;
+#pragma unroll
+                for (int i = 0; i < NF; i++) po[q][i] = phi[i][oc];
+            }
+#pragma unroll
+            for (int q = 0; q < kSlotBatch; q++)
+                if (q < ns) step(k0 + q, o[q], wz[q], wr[q], po[q]);
+        } else {
+            for (int k = k0; k < k0 + ns; k++) {
+                const int o = m.slotNb[k];
+                const int oc = o >= 0 ? o : c;
+                double po[NF];
+#pragma unroll
+                for (int i = 0; i < NF; i++) po[i] = phi[i][oc];
+                step(k, o, m.sLsqWZ[k], m.sLsqWR[k], po);
             }
         }
         const double izz = m.lsqInvZZ[c], izr = m.lsqInvZR[c], irr = m.lsqInvRR[c];
@@ -239,22 +290,19 @@ __device__ __forceinline__ void cellGradient(const DMesh& m, const DBC& bc, cons
 }
 
 // higherOrderFaceValue (solver_util.cuh:724-768) on the interior slot k of cell n
-// (neighbour nb)
-__device__ __forceinline__ double higherOrderSlot(const DMesh& m, const double* phi, const double* gz,
-                                                  const double* gr, int scheme, int n, int nb, int k, double F,
-                                                  double phiUD) {
-    if (scheme == 1) {
-        const double w = m.sW[k];
-        return w * phi[n] + (1.0 - w) * phi[nb];
-    }
+// (neighbour nb, c's weight w, phi at n and nb loaded)
+__device__ __forceinline__ double higherOrderVal(const DMesh& m, const double* phi, const double* gz,
+                                                 const double* gr, int scheme, int n, int nb, int k, double F,
+                                                 double phiUD, double w, double phiN, double phiO) {
+    if (scheme == 1) return w * phiN + (1.0 - w) * phiO;
     if (scheme == 2) {
         const int f = m.faceIDs[k];
         const int up = (F > 0.0) ? n : nb;
         const double dz = m.fZ[f] - m.cZ[up];
         const double dr = m.fR[f] - m.cR[up];
         const double ho = phi[up] + gz[up] * dz + gr[up] * dr;
-        const double lo = fmin(phi[n], phi[nb]);
-        const double hi = fmax(phi[n], phi[nb]);
+        const double lo = fmin(phiN, phiO);
+        const double hi = fmax(phiN, phiO);
         return fmin(fmax(ho, lo), hi);
     }
     return phiUD;
@@ -385,51 +433,106 @@ __global__ void __launch_bounds__(kBlock) kMomentum(DMesh m, DBC bu, DBC bv, DBC
     if (n >= m.N) return;
     const double mu = ph.mu;
     double acU = 0.0, bU = 0.0, acV = 0.0, bV = 0.0, netF = 0.0;
-    const int k0 = m.faceStart[n], k1 = m.faceStart[n + 1];
-    for (int k = k0; k < k1; k++) {
-        const int o = m.slotNb[k];
-        const double area = m.sArea[k];
+    const int k0 = m.faceStart[n], ns = m.faceStart[n + 1] - k0;
+    const bool needW = ph.nonOrtho || ph.scheme == 1;
+    const bool needUV = ph.scheme != 0;
+    const double uN = s.u[n], vN = s.v[n];
+    const double guzN = ph.nonOrtho ? s.gUZ[n] : 0.0, gurN = ph.nonOrtho ? s.gUR[n] : 0.0;
+    const double gvzN = ph.nonOrtho ? s.gVZ[n] : 0.0, gvrN = ph.nonOrtho ? s.gVR[n] : 0.0;
+    // one slot (the values at the neighbour o already loaded); returns AF
+    auto step = [&](int k, int o, uint8_t fl, double area, double invDPN, double md, double tz, double tr,
+                    double w, double guzO, double gurO, double gvzO, double gvrO, double uO, double vO) -> double {
         double F = 0.0;
         if (ph.conv) {
             // a boundary face is always owned by its one cell
-            const double Fo = s.mDot[m.faceIDs[k]] * 1.0;
-            F = slotOwns(m, k) ? Fo : -Fo;
+            const double Fo = md * 1.0;
+            F = (fl & 1) ? Fo : -Fo;
             if (fabs(F) <= 1.0e-30) F = 0.0;
             netF += F;
         }
         if (o >= 0) {
-            const double K = mu * area * m.sInvDPN[k];
+            const double K = mu * area * invDPN;
             double af = 0.0;
             acU += K;
             af -= K;
-            if (ph.nonOrtho) bU += nonOrthoSlot(m, n, k, o, s.gUZ, s.gUR, mu);
+            if (ph.nonOrtho) bU += nonOrthoVal(tz, tr, w, guzN, guzO, gurN, gurO, mu);
             if (F != 0.0) {
                 acU += fmax(F, 0.0);
                 af += fmin(F, 0.0);
                 if (ph.scheme != 0) {
-                    const double ud = (F > 0.0) ? s.u[n] : s.u[o];
-                    const double ho = higherOrderSlot(m, s.u, s.gUZ, s.gUR, ph.scheme, n, o, k, F, ud);
+                    const double ud = (F > 0.0) ? uN : uO;
+                    const double ho = higherOrderVal(m, s.u, s.gUZ, s.gUR, ph.scheme, n, o, k, F, ud, w, uN, uO);
                     bU -= F * (ho - ud);
                 }
             }
             acV += K;
-            if (ph.nonOrtho) bV += nonOrthoSlot(m, n, k, o, s.gVZ, s.gVR, mu);
+            if (ph.nonOrtho) bV += nonOrthoVal(tz, tr, w, gvzN, gvzO, gvrN, gvrO, mu);
             if (F != 0.0) {
                 acV += fmax(F, 0.0);
                 if (ph.scheme != 0) {
-                    const double ud = (F > 0.0) ? s.v[n] : s.v[o];
-                    const double ho = higherOrderSlot(m, s.v, s.gVZ, s.gVR, ph.scheme, n, o, k, F, ud);
+                    const double ud = (F > 0.0) ? vN : vO;
+                    const double ho = higherOrderVal(m, s.v, s.gVZ, s.gVR, ph.scheme, n, o, k, F, ud, w, vN, vO);
                     bV -= F * (ho - ud);
                 }
             }
-            s.AFuv[k] = af;
-            if (s.AF32) s.AF32[k] = (float)af;
-        } else {
-            const int f = m.faceIDs[k];
-            boundaryTransport(m, bu, n, f, area, mu, F, acU, bU);
-            boundaryTransport(m, bv, n, f, area, mu, F, acV, bV);
-            s.AFuv[k] = 0.0;
-            if (s.AF32) s.AF32[k] = 0.0f;
+            return af;
+        }
+        const int f = m.faceIDs[k];
+        boundaryTransport(m, bu, n, f, area, mu, F, acU, bU);
+        boundaryTransport(m, bv, n, f, area, mu, F, acV, bV);
+        return 0.0;
+    };
+    if (ns > 0 && ns <= kSlotBatch) {
+        int o[kSlotBatch], fid[kSlotBatch];
+        uint8_t fl[kSlotBatch];
+        double area[kSlotBatch], inv[kSlotBatch], tz[kSlotBatch], tr[kSlotBatch], w[kSlotBatch];
+        double md[kSlotBatch], guz[kSlotBatch], gur[kSlotBatch], gvz[kSlotBatch], gvr[kSlotBatch];
+        double uo[kSlotBatch], vo[kSlotBatch], af[kSlotBatch];
+#pragma unroll
+        for (int q = 0; q < kSlotBatch; q++) {
+            const int k = slotAt(k0, ns, q);
+            o[q] = m.slotNb[k];
+            fid[q] = m.faceIDs[k];
+            fl[q] = m.sFlags[k];
+            area[q] = m.sArea[k];
+            inv[q] = m.sInvDPN[k];
+            tz[q] = ph.nonOrtho ? m.sTZ[k] : 0.0;
+            tr[q] = ph.nonOrtho ? m.sTR[k] : 0.0;
+            w[q] = needW ? m.sW[k] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < kSlotBatch; q++) {
+            const int oc = o[q] >= 0 ? o[q] : n;
+            md[q] = ph.conv ? s.mDot[fid[q]] : 0.0;
+            guz[q] = ph.nonOrtho ? s.gUZ[oc] : 0.0;
+            gur[q] = ph.nonOrtho ? s.gUR[oc] : 0.0;
+            gvz[q] = ph.nonOrtho ? s.gVZ[oc] : 0.0;
+            gvr[q] = ph.nonOrtho ? s.gVR[oc] : 0.0;
+            uo[q] = needUV ? s.u[oc] : 0.0;
+            vo[q] = needUV ? s.v[oc] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < kSlotBatch; q++)
+            if (q < ns)
+                af[q] = step(k0 + q, o[q], fl[q], area[q], inv[q], md[q], tz[q], tr[q], w[q], guz[q], gur[q], gvz[q],
+                             gvr[q], uo[q], vo[q]);
+#pragma unroll
+        for (int q = 0; q < kSlotBatch; q++)
+            if (q < ns) {
+                s.AFuv[k0 + q] = af[q];
+                if (s.AF32) s.AF32[k0 + q] = (float)af[q];
+            }
+    } else {
+        for (int k = k0; k < k0 + ns; k++) {
+            const int o = m.slotNb[k];
+            const int oc = o >= 0 ? o : n;
+            const double a = step(k, o, m.sFlags[k], m.sArea[k], m.sInvDPN[k], ph.conv ? s.mDot[m.faceIDs[k]] : 0.0,
+                                  ph.nonOrtho ? m.sTZ[k] : 0.0, ph.nonOrtho ? m.sTR[k] : 0.0, needW ? m.sW[k] : 0.0,
+                                  ph.nonOrtho ? s.gUZ[oc] : 0.0, ph.nonOrtho ? s.gUR[oc] : 0.0,
+                                  ph.nonOrtho ? s.gVZ[oc] : 0.0, ph.nonOrtho ? s.gVR[oc] : 0.0,
+                                  needUV ? s.u[oc] : 0.0, needUV ? s.v[oc] : 0.0);
+            s.AFuv[k] = a;
+            if (s.AF32) s.AF32[k] = (float)a;
         }
     }
     if (ph.conv) {
@@ -698,13 +801,14 @@ __global__ void __launch_bounds__(kBlock) kPPCoeff(DMesh m, DBC bp, DState s, Ph
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= m.N) return;
     double ac = 0.0;
-    for (int k = m.faceStart[n]; k < m.faceStart[n + 1]; k++) {
-        const int o = m.slotNb[k];
-        const double area = m.sArea[k];
-        const double Df = normalCorrSlot(m, s, n, k, o);
+    const int k0 = m.faceStart[n], ns = m.faceStart[n + 1] - k0;
+    const double DUn = s.DU[n], DVn = s.DV[n];
+    auto step = [&](int k, int o, double area, double nz, double nr, double w, double invDPN, double DUo,
+                    double DVo) -> double {
+        const double Df = normalCorrVal(DUn, DVn, DUo, DVo, nz, nr, w, o);
         double af = 0.0;
         if (o >= 0) {
-            const double K = ph.rho * area * Df * m.sInvDPN[k];
+            const double K = ph.rho * area * Df * invDPN;
             ac += K;
             af -= K;
         } else {
@@ -715,7 +819,40 @@ __global__ void __launch_bounds__(kBlock) kPPCoeff(DMesh m, DBC bp, DState s, Ph
                 ac += K;
             }
         }
-        s.ppAF[k] = af;
+        return af;
+    };
+    if (ns > 0 && ns <= kSlotBatch) {
+        int o[kSlotBatch];
+        double area[kSlotBatch], nz[kSlotBatch], nr[kSlotBatch], w[kSlotBatch], inv[kSlotBatch];
+        double du[kSlotBatch], dv[kSlotBatch], af[kSlotBatch];
+#pragma unroll
+        for (int q = 0; q < kSlotBatch; q++) {
+            const int k = slotAt(k0, ns, q);
+            o[q] = m.slotNb[k];
+            area[q] = m.sArea[k];
+            nz[q] = m.sNZ[k];
+            nr[q] = m.sNR[k];
+            w[q] = m.sW[k];
+            inv[q] = m.sInvDPN[k];
+        }
+#pragma unroll
+        for (int q = 0; q < kSlotBatch; q++) {
+            const int oc = o[q] >= 0 ? o[q] : n;
+            du[q] = s.DU[oc];
+            dv[q] = s.DV[oc];
+        }
+#pragma unroll
+        for (int q = 0; q < kSlotBatch; q++)
+            if (q < ns) af[q] = step(k0 + q, o[q], area[q], nz[q], nr[q], w[q], inv[q], du[q], dv[q]);
+#pragma unroll
+        for (int q = 0; q < kSlotBatch; q++)
+            if (q < ns) s.ppAF[k0 + q] = af[q];
+    } else {
+        for (int k = k0; k < k0 + ns; k++) {
+            const int o = m.slotNb[k];
+            const int oc = o >= 0 ? o : n;
+            s.ppAF[k] = step(k, o, m.sArea[k], m.sNZ[k], m.sNR[k], m.sW[k], m.sInvDPN[k], s.DU[oc], s.DV[oc]);
+        }
     }
     s.ppAC[n] = ac;
     s.ppInv[n] = (fabs(ac) < 1.0e-30) ? 0.0 : 1.0 / ac;
@@ -741,14 +878,54 @@ __global__ void __launch_bounds__(kBlock) kPPRhs(DMesh m, DState s, Physics ph, 
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= m.N) return;
     double imb = 0.0, cs = 0.0;
-    for (int k = m.faceStart[n]; k < m.faceStart[n + 1]; k++) {
-        const double md = s.mDot[m.faceIDs[k]];
-        if (slotOwns(m, k)) imb += md;
+    const int k0 = m.faceStart[n], ns = m.faceStart[n + 1] - k0;
+    const double DUn = cross ? s.DU[n] : 0.0, DVn = cross ? s.DV[n] : 0.0;
+    const double gzN = cross ? s.gPZ[n] : 0.0, grN = cross ? s.gPR[n] : 0.0;
+    auto step = [&](int o, uint8_t fl, double md, double nz, double nr, double w, double tz, double tr, double DUo,
+                    double DVo, double gzO, double grO) {
+        if (fl & 1) imb += md;
         else imb -= md;
         if (cross) {
-            const int o = m.slotNb[k];
-            const double Df = normalCorrSlot(m, s, n, k, o);
-            cs += nonOrthoSlot(m, n, k, o, s.gPZ, s.gPR, ph.rho * Df);
+            const double Df = normalCorrVal(DUn, DVn, DUo, DVo, nz, nr, w, o);
+            cs += (o >= 0) ? nonOrthoVal(tz, tr, w, gzN, gzO, grN, grO, ph.rho * Df) : 0.0;
+        }
+    };
+    if (ns > 0 && ns <= kSlotBatch) {
+        int o[kSlotBatch], fid[kSlotBatch];
+        uint8_t fl[kSlotBatch];
+        double md[kSlotBatch], nz[kSlotBatch], nr[kSlotBatch], w[kSlotBatch], tz[kSlotBatch], tr[kSlotBatch];
+        double du[kSlotBatch], dv[kSlotBatch], gz[kSlotBatch], gr[kSlotBatch];
+#pragma unroll
+        for (int q = 0; q < kSlotBatch; q++) {
+            const int k = slotAt(k0, ns, q);
+            fid[q] = m.faceIDs[k];
+            fl[q] = m.sFlags[k];
+            o[q] = cross ? m.slotNb[k] : -1;
+            nz[q] = cross ? m.sNZ[k] : 0.0;
+            nr[q] = cross ? m.sNR[k] : 0.0;
+            w[q] = cross ? m.sW[k] : 0.0;
+            tz[q] = cross ? m.sTZ[k] : 0.0;
+            tr[q] = cross ? m.sTR[k] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < kSlotBatch; q++) {
+            const int oc = o[q] >= 0 ? o[q] : n;
+            md[q] = s.mDot[fid[q]];
+            du[q] = cross ? s.DU[oc] : 0.0;
+            dv[q] = cross ? s.DV[oc] : 0.0;
+            gz[q] = cross ? s.gPZ[oc] : 0.0;
+            gr[q] = cross ? s.gPR[oc] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < kSlotBatch; q++)
+            if (q < ns) step(o[q], fl[q], md[q], nz[q], nr[q], w[q], tz[q], tr[q], du[q], dv[q], gz[q], gr[q]);
+    } else {
+        for (int k = k0; k < k0 + ns; k++) {
+            const int o = cross ? m.slotNb[k] : -1;
+            const int oc = o >= 0 ? o : n;
+            step(o, m.sFlags[k], s.mDot[m.faceIDs[k]], cross ? m.sNZ[k] : 0.0, cross ? m.sNR[k] : 0.0,
+                 cross ? m.sW[k] : 0.0, cross ? m.sTZ[k] : 0.0, cross ? m.sTR[k] : 0.0, cross ? s.DU[oc] : 0.0,
+                 cross ? s.DV[oc] : 0.0, cross ? s.gPZ[oc] : 0.0, cross ? s.gPR[oc] : 0.0);
         }
     }
     s.ppb[n] = -imb + cs;
@@ -812,16 +989,43 @@ __global__ void __launch_bounds__(kBlock) kResidual(DMesh m, DState s) {
         v[3] = fabs(axu);
         v[7] = fabs(axv);
         double imb = 0.0;
-        for (int k = m.faceStart[n]; k < m.faceStart[n + 1]; k++) {
-            const int nb = m.slotNb[k];
-            if (nb >= 0) {
-                const double a = s.AFuv[k];
-                axu += a * s.u[nb];
-                axv += a * s.v[nb];
+        const int k0 = m.faceStart[n], ns = m.faceStart[n + 1] - k0;
+        auto step = [&](int o, double a, double uo, double vo, uint8_t fl, double md) {
+            if (o >= 0) {
+                axu += a * uo;
+                axv += a * vo;
             }
-            const double md = s.mDot[m.faceIDs[k]];
-            if (slotOwns(m, k)) imb += md;
+            if (fl & 1) imb += md;
             else imb -= md;
+        };
+        if (ns > 0 && ns <= kSlotBatch) {
+            int o[kSlotBatch], fid[kSlotBatch];
+            uint8_t fl[kSlotBatch];
+            double a[kSlotBatch], uo[kSlotBatch], vo[kSlotBatch], md[kSlotBatch];
+#pragma unroll
+            for (int q = 0; q < kSlotBatch; q++) {
+                const int k = slotAt(k0, ns, q);
+                o[q] = m.slotNb[k];
+                a[q] = s.AFuv[k];
+                fid[q] = m.faceIDs[k];
+                fl[q] = m.sFlags[k];
+            }
+#pragma unroll
+            for (int q = 0; q < kSlotBatch; q++) {
+                const int oc = o[q] >= 0 ? o[q] : n;
+                uo[q] = s.u[oc];
+                vo[q] = s.v[oc];
+                md[q] = s.mDot[fid[q]];
+            }
+#pragma unroll
+            for (int q = 0; q < kSlotBatch; q++)
+                if (q < ns) step(o[q], a[q], uo[q], vo[q], fl[q], md[q]);
+        } else {
+            for (int k = k0; k < k0 + ns; k++) {
+                const int o = m.slotNb[k];
+                const int oc = o >= 0 ? o : n;
+                step(o, s.AFuv[k], s.u[oc], s.v[oc], m.sFlags[k], s.mDot[m.faceIDs[k]]);
+            }
         }
         const double ru = fabs(s.ub[n] - axu), rv = fabs(s.vb[n] - axv);
         v[0] = ru;
@@ -987,14 +1191,35 @@ __global__ void __launch_bounds__(kBlock) kMGCoarseGather(DLevel f, DLevel c, co
 
 // computeResidual (residualAll sign = true, residuals.cuh:17-60) into f.res,
 // one thread per fine cell; kMGRestrictSum then gathers it per coarse cell.
+// B > 0: rows of at most B slots load their slots, then their neighbours, in
+// one round each (indices clamped into the row); B = 0 or longer rows: the
+// plain loop.  The arithmetic is the same in either case.
+template <int B>
 __global__ void __launch_bounds__(kBlock) kMGResidual(DLevel f, const Ctl* ctl) {
     if (finished(ctl)) return;
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= f.n) return;
+    const int k0 = f.start[n], ns = f.start[n + 1] - k0;
     double Ax = f.AC[n] * f.x[n];
-    for (int k = f.start[n]; k < f.start[n + 1]; k++) {
-        const int nb = f.nb[k];
-        if (nb >= 0) Ax += f.AF[k] * f.x[nb];
+    if (B > 0 && ns > 0 && ns <= B) {
+        int o[B > 0 ? B : 1];
+        double a[B > 0 ? B : 1], xo[B > 0 ? B : 1];
+#pragma unroll
+        for (int q = 0; q < B; q++) {
+            const int k = slotAt(k0, ns, q);
+            o[q] = f.nb[k];
+            a[q] = f.AF[k];
+        }
+#pragma unroll
+        for (int q = 0; q < B; q++) xo[q] = f.x[o[q] >= 0 ? o[q] : n];
+#pragma unroll
+        for (int q = 0; q < B; q++)
+            if (q < ns && o[q] >= 0) Ax += a[q] * xo[q];
+    } else {
+        for (int k = k0; k < k0 + ns; k++) {
+            const int nb = f.nb[k];
+            if (nb >= 0) Ax += f.AF[k] * f.x[nb];
+        }
     }
     f.res[n] = f.b[n] - Ax;
 }
@@ -1011,12 +1236,37 @@ __device__ __forceinline__ double jacobiRow(const DLevel& L, const double* x, in
     return x[n] + w * (L.b[n] - Ax) / AC;
 }
 
-// jacobiFused, multigrid.cu:486-493
+// jacobiFused, multigrid.cu:486-493 (B as in kMGResidual)
+template <int B>
 __global__ void __launch_bounds__(kBlock) kMGJacobi(DLevel L, double w, const Ctl* ctl) {
     if (finished(ctl)) return;
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= L.n) return;
-    L.xNew[n] = jacobiRow(L, L.x, n, w);
+    const int k0 = L.start[n], ns = L.start[n + 1] - k0;
+    if (B > 0 && ns > 0 && ns <= B) {
+        int o[B > 0 ? B : 1];
+        double a[B > 0 ? B : 1], xo[B > 0 ? B : 1];
+#pragma unroll
+        for (int q = 0; q < B; q++) {
+            const int k = slotAt(k0, ns, q);
+            o[q] = L.nb[k];
+            a[q] = L.AF[k];
+        }
+        const double AC = L.AC[n], xn = L.x[n], bn = L.b[n];
+#pragma unroll
+        for (int q = 0; q < B; q++) xo[q] = L.x[o[q] >= 0 ? o[q] : n];
+        double r = xn;
+        if (!(fabs(AC) < 1.0e-30)) {
+            double Ax = AC * xn;
+#pragma unroll
+            for (int q = 0; q < B; q++)
+                if (q < ns && o[q] != -1) Ax += a[q] * xo[q];
+            r = xn + w * (bn - Ax) / AC;
+        }
+        L.xNew[n] = r;
+    } else {
+        L.xNew[n] = jacobiRow(L, L.x, n, w);
+    }
 }
 
 // jacobiSingleBlock, multigrid.cu:501-541: all sweeps of a small level in one
@@ -1055,10 +1305,44 @@ __global__ void __launch_bounds__(1024) kMGSmoothBlock(DLevel L, double w, int s
 // buildProlongationKernel (multigrid.cu:415-427) fused into the first
 // post-smoothing sweep: x + P x_c is formed on the fly for the cell and its
 // neighbours (the same sums the prolongation would store).
+template <int B>
 __global__ void __launch_bounds__(kBlock) kMGProlongSmooth(DLevel f, DLevel c, double w, const Ctl* ctl) {
     if (finished(ctl)) return;
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= f.n) return;
+    const int k0 = f.start[n], ns = f.start[n + 1] - k0;
+    if (B > 0 && ns > 0 && ns <= B) {
+        int o[B > 0 ? B : 1], tc[B > 0 ? B : 1];
+        double a[B > 0 ? B : 1], xo[B > 0 ? B : 1], co[B > 0 ? B : 1];
+#pragma unroll
+        for (int q = 0; q < B; q++) {
+            const int k = slotAt(k0, ns, q);
+            o[q] = f.nb[k];
+            a[q] = f.AF[k];
+        }
+        const int tn = f.toCoarse[n];
+        const double AC = f.AC[n], bn = f.b[n], xf = f.x[n];
+#pragma unroll
+        for (int q = 0; q < B; q++) {
+            const int oc = o[q] >= 0 ? o[q] : n;
+            xo[q] = f.x[oc];
+            tc[q] = f.toCoarse[oc];
+        }
+        const double xc = c.x[tn];
+#pragma unroll
+        for (int q = 0; q < B; q++) co[q] = c.x[tc[q]];
+        const double xn = xf + xc;
+        if (fabs(AC) < 1.0e-30) {
+            f.xNew[n] = xn;
+            return;
+        }
+        double Ax = AC * xn;
+#pragma unroll
+        for (int q = 0; q < B; q++)
+            if (q < ns && o[q] != -1) Ax += a[q] * (xo[q] + co[q]);
+        f.xNew[n] = xn + w * (bn - Ax) / AC;
+        return;
+    }
     const double xn = f.x[n] + c.x[f.toCoarse[n]];
     const double AC = f.AC[n];
     if (fabs(AC) < 1.0e-30) {
@@ -1066,7 +1350,7 @@ __global__ void __launch_bounds__(kBlock) kMGProlongSmooth(DLevel f, DLevel c, d
         return;
     }
     double Ax = AC * xn;
-    for (int k = f.start[n]; k < f.start[n + 1]; k++) {
+    for (int k = k0; k < k0 + ns; k++) {
         const int nb = f.nb[k];
         if (nb != -1) Ax += f.AF[k] * (f.x[nb] + c.x[f.toCoarse[nb]]);
     }
@@ -1117,7 +1401,7 @@ __device__ __forceinline__ T sweepFromZero(T b, T AC, T w) {
 // order of the residuals kMGResidual left in f.res, then the coarse x memset
 // (multigrid.cu:650-655) -- or, with sweep1, the coarse level's first
 // pre-smoothing sweep, which starts from that zero.
-template <class T>
+template <class T, int B>
 __global__ void __launch_bounds__(kBlock) kMGRestrictSum(DLevel f, DLevel c, double w, int sweep1, const Ctl* ctl) {
     if (finished(ctl)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1125,7 +1409,21 @@ __global__ void __launch_bounds__(kBlock) kMGRestrictSum(DLevel f, DLevel c, dou
     const LV<T> fv = view<T>(f);
     const LV<T> cv = view<T>(c);
     T acc = T(0);
-    for (int j = f.memberStart[i]; j < f.memberStart[i + 1]; j++) acc += fv.res[f.members[j]];
+    const int j0 = f.memberStart[i], nm = f.memberStart[i + 1] - j0;
+    if (B > 0 && nm > 0 && nm <= B) {
+        // members, then their residuals, in one round each; summed in order
+        int mb[B > 0 ? B : 1];
+        T r[B > 0 ? B : 1];
+#pragma unroll
+        for (int q = 0; q < B; q++) mb[q] = f.members[slotAt(j0, nm, q)];
+#pragma unroll
+        for (int q = 0; q < B; q++) r[q] = fv.res[mb[q]];
+#pragma unroll
+        for (int q = 0; q < B; q++)
+            if (q < nm) acc += r[q];
+    } else {
+        for (int j = j0; j < j0 + nm; j++) acc += fv.res[f.members[j]];
+    }
     cv.b[i] = acc;
     cv.x[i] = sweep1 ? sweepFromZero(acc, cv.AC[i], (T)w) : T(0);
 }
@@ -1288,6 +1586,35 @@ __global__ void __launch_bounds__(1024) kMGTail(const TailLevel* __restrict__ tl
         const bool own = t < S.n;
         TRow<T> r;
         if (own) loadRow(r, sT + S.AC, sT + S.b, AF, sI + S.start, nbs, t);
+        if (S.n <= 64) {
+            // a level of at most one wave's cells (the coarsest, usually):
+            // wave 0 sweeps alone; a wave's LDS accesses complete in program
+            // order, so its sweeps need no workgroup barrier in between
+            if (t < 64) {
+                unsigned f = flip;
+                for (int it = 0; it < count; it++) {
+                    const T* x = sT + (((f >> q) & 1u) ? S.x1 : S.x0);
+                    T* y = sT + (((f >> q) & 1u) ? S.x0 : S.x1);
+                    if (own) {
+                        const T xn = x[t];
+                        T res;
+                        if (tabs(r.ac) < T(1.0e-30)) {
+                            res = xn;
+                        } else {
+                            const T Ax = rowAx(r, xn, [&](int j) { return x[j]; }, AF, nbs);
+                            res = xn + w * (r.b - Ax) / r.ac;
+                        }
+                        y[t] = res;
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    f ^= 1u << q;
+                }
+            }
+            if (count & 1) flip ^= 1u << q;
+            __syncthreads();
+            return;
+        }
         for (int it = 0; it < count; it++) {
             const T* x = X(S, q);
             T* y = Y(S, q);
@@ -1451,12 +1778,24 @@ void launchMGCoarseGather(hipStream_t s, const DLevel& f, const DLevel& c, const
     kMGCoarseGather<<<blocksFor((long)c.refs + c.n), kBlock, 0, s>>>(f, c, ctl);
 }
 void launchMGResidual(hipStream_t s, const DLevel& f, const Ctl* ctl) {
-    kMGResidual<<<blocksFor(f.n, f.block), f.block, 0, s>>>(f, ctl);
+    const dim3 g(blocksFor(f.n, f.block)), b(f.block);
+    if (f.maxRow <= 4) kMGResidual<4><<<g, b, 0, s>>>(f, ctl);
+    else if (f.maxRow <= 8) kMGResidual<8><<<g, b, 0, s>>>(f, ctl);
+    else if (f.maxRow <= 16) kMGResidual<16><<<g, b, 0, s>>>(f, ctl);
+    else kMGResidual<0><<<g, b, 0, s>>>(f, ctl);
 }
 void launchMGRestrictSum(hipStream_t s, const DLevel& f, const DLevel& c, double w, int sweep1, bool fp32,
                          const Ctl* ctl) {
-    if (fp32) kMGRestrictSum<float><<<blocksFor(c.n, c.block), c.block, 0, s>>>(f, c, w, sweep1, ctl);
-    else kMGRestrictSum<double><<<blocksFor(c.n, c.block), c.block, 0, s>>>(f, c, w, sweep1, ctl);
+    const dim3 g(blocksFor(c.n, c.block)), b(c.block);
+    if (fp32) {
+        if (f.maxMembers <= 4) kMGRestrictSum<float, 4><<<g, b, 0, s>>>(f, c, w, sweep1, ctl);
+        else if (f.maxMembers <= 8) kMGRestrictSum<float, 8><<<g, b, 0, s>>>(f, c, w, sweep1, ctl);
+        else kMGRestrictSum<float, 0><<<g, b, 0, s>>>(f, c, w, sweep1, ctl);
+    } else {
+        if (f.maxMembers <= 4) kMGRestrictSum<double, 4><<<g, b, 0, s>>>(f, c, w, sweep1, ctl);
+        else if (f.maxMembers <= 8) kMGRestrictSum<double, 8><<<g, b, 0, s>>>(f, c, w, sweep1, ctl);
+        else kMGRestrictSum<double, 0><<<g, b, 0, s>>>(f, c, w, sweep1, ctl);
+    }
 }
 void launchGradUV(hipStream_t s, const DMesh& m, DBC bu, DBC bv, DState st, int scheme) {
     kGradUV<<<blocksFor(m.N), kBlock, 0, s>>>(m, bu, bv, st, scheme);
@@ -1537,14 +1876,22 @@ void launchFill(hipStream_t s, double* dst, double value, int n, const Ctl* ctl)
     kFill<<<blocksFor(n), kBlock, 0, s>>>(dst, value, n, ctl);
 }
 void launchMGJacobi(hipStream_t s, const DLevel& L, double w, const Ctl* ctl) {
-    kMGJacobi<<<blocksFor(L.n, L.block), L.block, 0, s>>>(L, w, ctl);
+    const dim3 g(blocksFor(L.n, L.block)), b(L.block);
+    if (L.maxRow <= 4) kMGJacobi<4><<<g, b, 0, s>>>(L, w, ctl);
+    else if (L.maxRow <= 8) kMGJacobi<8><<<g, b, 0, s>>>(L, w, ctl);
+    else if (L.maxRow <= 16) kMGJacobi<16><<<g, b, 0, s>>>(L, w, ctl);
+    else kMGJacobi<0><<<g, b, 0, s>>>(L, w, ctl);
 }
 void launchMGSmoothBlock(hipStream_t s, const DLevel& L, double w, int sweeps, const Ctl* ctl) {
     const int threads = L.n >= 1024 ? 1024 : ((L.n + 63) / 64) * 64;
     kMGSmoothBlock<<<1, threads, 2 * (size_t)L.n * sizeof(double), s>>>(L, w, sweeps, ctl);
 }
 void launchMGProlongSmooth(hipStream_t s, const DLevel& f, const DLevel& c, double w, const Ctl* ctl) {
-    kMGProlongSmooth<<<blocksFor(f.n, f.block), f.block, 0, s>>>(f, c, w, ctl);
+    const dim3 g(blocksFor(f.n, f.block)), b(f.block);
+    if (f.maxRow <= 4) kMGProlongSmooth<4><<<g, b, 0, s>>>(f, c, w, ctl);
+    else if (f.maxRow <= 8) kMGProlongSmooth<8><<<g, b, 0, s>>>(f, c, w, ctl);
+    else if (f.maxRow <= 16) kMGProlongSmooth<16><<<g, b, 0, s>>>(f, c, w, ctl);
+    else kMGProlongSmooth<0><<<g, b, 0, s>>>(f, c, w, ctl);
 }
 void launchMGProlong(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl) {
     kMGProlong<<<blocksFor(f.n, f.block), f.block, 0, s>>>(f, c, ctl);

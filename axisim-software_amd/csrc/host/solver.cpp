@@ -53,6 +53,10 @@ void MGDevice::build(int n, const std::vector<int>& start, const std::vector<int
         DLevel& d = dev[l];
         d.n = h.n;
         d.refs = h.refs();
+        for (int i = 0; i < h.n; i++) d.maxRow = std::max(d.maxRow, h.start[i + 1] - h.start[i]);
+        if (!h.coarsest())
+            for (size_t i = 0; i + 1 < h.memberStart.size(); i++)
+                d.maxMembers = std::max(d.maxMembers, h.memberStart[i + 1] - h.memberStart[i]);
         if (tail64_.holds((int)l)) {
             // operator, iterates, CSR and maps inside the tail's image
             const TailLevel& e = tail64_.lv[l - tail64_.lt];
