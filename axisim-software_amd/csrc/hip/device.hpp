@@ -13,6 +13,7 @@ constexpr int kBlock = 256;   // threads per workgroup (4 wave64)
 struct DMesh {
     int N = 0, F = 0, S = 0;
     int sweepBlock = kBlock;  // workgroup size of the momentum sweep (autotuned)
+    int maxSlots = 0;         // most slots of a cell (the cell loops batch up to 3 or 4)
     const double *cZ, *cR, *vol, *invA2D, *lsqInvZZ, *lsqInvZR, *lsqInvRR;
     const int *faceStart, *faceIDs;
     const int *slotNb;      // neighbour cell of each slot, -1 on a boundary face
@@ -161,6 +162,13 @@ void launchMGJacobi(hipStream_t s, const DLevel& L, double w, const Ctl* ctl);
 void launchMGSmoothBlock(hipStream_t s, const DLevel& L, double w, int sweeps, const Ctl* ctl);
 void launchMGProlongSmooth(hipStream_t s, const DLevel& f, const DLevel& c, double w, const Ctl* ctl);
 void launchMGProlong(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl);
+// level-0 sweeps fused per launch over the blocks sb (512 cells, halo rings):
+// `sweeps` weighted-Jacobi sweeps (after the prolongation when `prolong`), then
+// the residual into f.res; the final x in f.xNew.  Supported: 1 sweep without
+// the prolongation (a cycle's start), 2 sweeps after it (one cycle's end and
+// the next one's start).
+void launchMGLevel0Block(hipStream_t s, const DLevel& f, const DLevel& c, const DSweepBlocks& sb, int sweeps,
+                         bool prolong, double w, const Ctl* ctl);
 // multi-colour Gauss-Seidel smoothing: one colour (order[begin .. begin+count)),
 // or every sweep and colour of a level of at most kSmoothBlockGSMax cells in one
 // workgroup (colorStart: device copy, nColors + 1 entries)

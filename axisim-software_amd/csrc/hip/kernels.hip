@@ -24,6 +24,18 @@ namespace {
 
 __device__ __forceinline__ bool finished(const Ctl* ctl) { return ctl != nullptr && ctl->done != 0; }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b runs on XCD
+// b % 8), and each XCD has its own L2.  Every kernel here numbers its blocks
+// so that XCD x runs one contiguous range of them: the neighbour gathers
+// across a block's edge then mostly hit lines its neighbour block, on the same
+// XCD, has just brought in.  (Only the placement changes; each block still
+// covers the same cells.)
+__device__ __forceinline__ int xcdBlock() {
+    const int g = gridDim.x, b = blockIdx.x;
+    const int q = g >> 3, r = g & 7, x = b & 7;
+    return x * q + min(x, r) + (b >> 3);
+}
+
 // ---- face helpers (solver/header/solver_util.cuh:56-430) ----
 
 __device__ __forceinline__ double faceWeight(const DMesh& m, int c, int f) {
@@ -112,12 +124,12 @@ __device__ __forceinline__ double nonOrthoFlux(const DMesh& m, int c, int f, con
 // bit-identical to the face form (the squares of the normal in
 // normalCorrToFace do not see its orientation); only boundary slots still read
 // face-indexed data.
-// Cells with at most kSlotBatch slots (triangles, quads) walk their slots in
-// three dependent rounds of loads instead of one round per slot: every slot's
-// data (indices clamped into the cell's range, so every load is in bounds and
-// can issue at once), then every neighbour gather, then the arithmetic slot by
-// slot in the reference's order.  Larger cells take the plain loop.
-constexpr int kSlotBatch = 4;
+// Cells with at most B slots (B = 3 on triangle meshes, 4 otherwise) walk
+// their slots in three dependent rounds of loads instead of one round per slot:
+// every slot's data (indices clamped into the cell's range, so every load is in
+// bounds and can issue at once), then every neighbour gather, then the
+// arithmetic slot by slot in the reference's order.  Larger cells take the
+// plain loop.
 __device__ __forceinline__ int slotAt(int k0, int ns, int q) { return k0 + (q < ns ? q : ns - 1); }
 
 // nonOrthoFlux from loaded values (T, w, the gradients at c and o)
@@ -142,11 +154,11 @@ __device__ __forceinline__ double normalCorrVal(double DUc, double DVc, double D
 // for all of them; each field's sums run in the reference's order).  In the
 // least-squares form dphi = phi_o - phi_c with the weight negated where c is
 // the face's neighbour: (-w)(a - b) == w(b - a) exactly.
-template <int NF>
+template <int NF, int B>
 __device__ void cellGradients(const DMesh& m, const DBC* bc, const double* const* phi, int c, int scheme,
                               double* gZ, double* gR) {
     const int k0 = m.faceStart[c], ns = m.faceStart[c + 1] - k0;
-    const bool batch = ns > 0 && ns <= kSlotBatch;
+    const bool batch = ns > 0 && ns <= B;
     double phiP[NF];
 #pragma unroll
     for (int i = 0; i < NF; i++) phiP[i] = phi[i][c];
@@ -181,11 +193,11 @@ __device__ void cellGradients(const DMesh& m, const DBC* bc, const double* const
             }
         };
         if (batch) {
-            int o[kSlotBatch];
-            uint8_t fl[kSlotBatch];
-            double nz[kSlotBatch], nr[kSlotBatch], w[kSlotBatch], L[kSlotBatch], po[kSlotBatch][NF];
+            int o[B];
+            uint8_t fl[B];
+            double nz[B], nr[B], w[B], L[B], po[B][NF];
 #pragma unroll
-            for (int q = 0; q < kSlotBatch; q++) {
+            for (int q = 0; q < B; q++) {
                 const int k = slotAt(k0, ns, q);
                 o[q] = m.slotNb[k];
                 nz[q] = m.sNZ[k];
@@ -195,13 +207,13 @@ __device__ void cellGradients(const DMesh& m, const DBC* bc, const double* const
                 L[q] = m.sLen[k];
             }
 #pragma unroll
-            for (int q = 0; q < kSlotBatch; q++) {
+            for (int q = 0; q < B; q++) {
                 const int oc = o[q] >= 0 ? o[q] : c;
 #pragma unroll
                 for (int i = 0; i < NF; i++) po[q][i] = phi[i][oc];
             }
 #pragma unroll
-            for (int q = 0; q < kSlotBatch; q++)
+            for (int q = 0; q < B; q++)
                 if (q < ns) step(k0 + q, o[q], nz[q], nr[q], w[q], fl[q], L[q], po[q]);
         } else {
             for (int k = k0; k < k0 + ns; k++) {
@@ -244,23 +256,23 @@ __device__ void cellGradients(const DMesh& m, const DBC* bc, const double* const
             }
         };
         if (batch) {
-            int o[kSlotBatch];
-            double wz[kSlotBatch], wr[kSlotBatch], po[kSlotBatch][NF];
+            int o[B];
+            double wz[B], wr[B], po[B][NF];
 #pragma unroll
-            for (int q = 0; q < kSlotBatch; q++) {
+            for (int q = 0; q < B; q++) {
                 const int k = slotAt(k0, ns, q);
                 o[q] = m.slotNb[k];
                 wz[q] = m.sLsqWZ[k];
                 wr[q] = m.sLsqWR[k];
             }
 #pragma unroll
-            for (int q = 0; q < kSlotBatch; q++) {
+            for (int q = 0; q < B; q++) {
                 const int oc = o[q] >= 0 ? o[q] : c;
 #pragma unroll
                 for (int i = 0; i < NF; i++) po[q][i] = phi[i][oc];
             }
 #pragma unroll
-            for (int q = 0; q < kSlotBatch; q++)
+            for (int q = 0; q < B; q++)
                 if (q < ns) step(k0 + q, o[q], wz[q], wr[q], po[q]);
         } else {
             for (int k = k0; k < k0 + ns; k++) {
@@ -281,10 +293,11 @@ __device__ void cellGradients(const DMesh& m, const DBC* bc, const double* const
     }
 }
 
+template <int B>
 __device__ __forceinline__ void cellGradient(const DMesh& m, const DBC& bc, const double* phi, int c, int scheme,
                                              double& gZ, double& gR) {
     double z[1], r[1];
-    cellGradients<1>(m, &bc, &phi, c, scheme, z, r);
+    cellGradients<1, B>(m, &bc, &phi, c, scheme, z, r);
     gZ = z[0];
     gR = r[0];
 }
@@ -389,27 +402,29 @@ __device__ __forceinline__ double waveMax(double v) {
 // ===========================================================================
 // Gradients (computeGradient, solver_util.cu:146-169)
 // ===========================================================================
+template <int B>
 __global__ void __launch_bounds__(kBlock) kGradUV(DMesh m, DBC bu, DBC bv, DState s, int scheme) {
     if (finished(s.ctl)) return;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = xcdBlock() * blockDim.x + threadIdx.x;
     if (n >= m.N) return;
     const DBC bc[2] = {bu, bv};
     const double* phi[2] = {s.u, s.v};
     double gz[2], gr[2];
-    cellGradients<2>(m, bc, phi, n, scheme, gz, gr);
+    cellGradients<2, B>(m, bc, phi, n, scheme, gz, gr);
     s.gUZ[n] = gz[0];
     s.gUR[n] = gr[0];
     s.gVZ[n] = gz[1];
     s.gVR[n] = gr[1];
 }
 
+template <int B>
 __global__ void __launch_bounds__(kBlock) kGradField(DMesh m, DBC bc, const double* phi, double* gzo,
                                                      double* gro, int scheme, const Ctl* ctl) {
     if (finished(ctl)) return;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = xcdBlock() * blockDim.x + threadIdx.x;
     if (n >= m.N) return;
     double gz, gr;
-    cellGradient(m, bc, phi, n, scheme, gz, gr);
+    cellGradient<B>(m, bc, phi, n, scheme, gz, gr);
     gzo[n] = gz;
     gro[n] = gr;
 }
@@ -427,15 +442,18 @@ __global__ void __launch_bounds__(kBlock) kGradField(DMesh m, DBC bc, const doub
 //   getCorrectionCoefficients       solver_util.cu:127-144
 // Every step is per-cell, so one launch replaces nine.
 // ===========================================================================
+// HO: higher-order convection (the neighbour's u, v are read); the upwind
+// instance holds fewer registers.
+template <int B, bool HO>
 __global__ void __launch_bounds__(kBlock) kMomentum(DMesh m, DBC bu, DBC bv, DBC bp, DState s, Physics ph) {
     if (finished(s.ctl)) return;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = xcdBlock() * blockDim.x + threadIdx.x;
     if (n >= m.N) return;
     const double mu = ph.mu;
     double acU = 0.0, bU = 0.0, acV = 0.0, bV = 0.0, netF = 0.0;
     const int k0 = m.faceStart[n], ns = m.faceStart[n + 1] - k0;
     const bool needW = ph.nonOrtho || ph.scheme == 1;
-    const bool needUV = ph.scheme != 0;
+    const bool needUV = HO;
     const double uN = s.u[n], vN = s.v[n];
     const double guzN = ph.nonOrtho ? s.gUZ[n] : 0.0, gurN = ph.nonOrtho ? s.gUR[n] : 0.0;
     const double gvzN = ph.nonOrtho ? s.gVZ[n] : 0.0, gvrN = ph.nonOrtho ? s.gVR[n] : 0.0;
@@ -459,7 +477,7 @@ __global__ void __launch_bounds__(kBlock) kMomentum(DMesh m, DBC bu, DBC bv, DBC
             if (F != 0.0) {
                 acU += fmax(F, 0.0);
                 af += fmin(F, 0.0);
-                if (ph.scheme != 0) {
+                if (HO) {
                     const double ud = (F > 0.0) ? uN : uO;
                     const double ho = higherOrderVal(m, s.u, s.gUZ, s.gUR, ph.scheme, n, o, k, F, ud, w, uN, uO);
                     bU -= F * (ho - ud);
@@ -469,7 +487,7 @@ __global__ void __launch_bounds__(kBlock) kMomentum(DMesh m, DBC bu, DBC bv, DBC
             if (ph.nonOrtho) bV += nonOrthoVal(tz, tr, w, gvzN, gvzO, gvrN, gvrO, mu);
             if (F != 0.0) {
                 acV += fmax(F, 0.0);
-                if (ph.scheme != 0) {
+                if (HO) {
                     const double ud = (F > 0.0) ? vN : vO;
                     const double ho = higherOrderVal(m, s.v, s.gVZ, s.gVR, ph.scheme, n, o, k, F, ud, w, vN, vO);
                     bV -= F * (ho - ud);
@@ -482,14 +500,14 @@ __global__ void __launch_bounds__(kBlock) kMomentum(DMesh m, DBC bu, DBC bv, DBC
         boundaryTransport(m, bv, n, f, area, mu, F, acV, bV);
         return 0.0;
     };
-    if (ns > 0 && ns <= kSlotBatch) {
-        int o[kSlotBatch], fid[kSlotBatch];
-        uint8_t fl[kSlotBatch];
-        double area[kSlotBatch], inv[kSlotBatch], tz[kSlotBatch], tr[kSlotBatch], w[kSlotBatch];
-        double md[kSlotBatch], guz[kSlotBatch], gur[kSlotBatch], gvz[kSlotBatch], gvr[kSlotBatch];
-        double uo[kSlotBatch], vo[kSlotBatch], af[kSlotBatch];
+    if (ns > 0 && ns <= B) {
+        int o[B], fid[B];
+        uint8_t fl[B];
+        double area[B], inv[B], tz[B], tr[B], w[B];
+        double md[B], guz[B], gur[B], gvz[B], gvr[B];
+        double uo[B], vo[B], af[B];
 #pragma unroll
-        for (int q = 0; q < kSlotBatch; q++) {
+        for (int q = 0; q < B; q++) {
             const int k = slotAt(k0, ns, q);
             o[q] = m.slotNb[k];
             fid[q] = m.faceIDs[k];
@@ -501,7 +519,7 @@ __global__ void __launch_bounds__(kBlock) kMomentum(DMesh m, DBC bu, DBC bv, DBC
             w[q] = needW ? m.sW[k] : 0.0;
         }
 #pragma unroll
-        for (int q = 0; q < kSlotBatch; q++) {
+        for (int q = 0; q < B; q++) {
             const int oc = o[q] >= 0 ? o[q] : n;
             md[q] = ph.conv ? s.mDot[fid[q]] : 0.0;
             guz[q] = ph.nonOrtho ? s.gUZ[oc] : 0.0;
@@ -512,12 +530,12 @@ __global__ void __launch_bounds__(kBlock) kMomentum(DMesh m, DBC bu, DBC bv, DBC
             vo[q] = needUV ? s.v[oc] : 0.0;
         }
 #pragma unroll
-        for (int q = 0; q < kSlotBatch; q++)
+        for (int q = 0; q < B; q++)
             if (q < ns)
                 af[q] = step(k0 + q, o[q], fl[q], area[q], inv[q], md[q], tz[q], tr[q], w[q], guz[q], gur[q], gvz[q],
                              gvr[q], uo[q], vo[q]);
 #pragma unroll
-        for (int q = 0; q < kSlotBatch; q++)
+        for (int q = 0; q < B; q++)
             if (q < ns) {
                 s.AFuv[k0 + q] = af[q];
                 if (s.AF32) s.AF32[k0 + q] = (float)af[q];
@@ -559,7 +577,7 @@ __global__ void __launch_bounds__(kBlock) kMomentum(DMesh m, DBC bu, DBC bv, DBC
         }
     }
     double gz, gr;
-    cellGradient(m, bp, s.p, n, ph.gradScheme, gz, gr);
+    cellGradient<B>(m, bp, s.p, n, ph.gradScheme, gz, gr);
     s.gPZ[n] = gz;
     s.gPR[n] = gr;
     bU += -V * gz;
@@ -616,7 +634,7 @@ struct UVOut {
 
 __global__ void __launch_bounds__(kBlock) kJacobiUV(DMesh m, DState s, UVIn xo, UVOut xn) {
     if (finished(s.ctl)) return;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = xcdBlock() * blockDim.x + threadIdx.x;
     if (n >= m.N) return;
     double au = s.ub[n], av = s.vb[n];
     for (int k = m.faceStart[n]; k < m.faceStart[n + 1]; k++) {
@@ -701,7 +719,7 @@ __global__ void __launch_bounds__(C) kJacobiUVBlock(DMesh m, DState s, DSweepBlo
     __shared__ double2 buf[kMask + 1][C + H];
     __shared__ int2 keys[H];
     if (finished(s.ctl)) return;
-    const int t = threadIdx.x, blk = blockIdx.x;
+    const int t = threadIdx.x, blk = xcdBlock();
     const int a = blk * C, nOwn = min(C, m.N - a);
     const int h0 = sb.haloStart[blk], nh = sb.haloStart[blk + 1] - h0;
     // halo entries of rings 1 .. S - 1: the ones sweep 1 updates
@@ -743,7 +761,7 @@ __global__ void __launch_bounds__(C) kJacobiUVBlock(DMesh m, DState s, DSweepBlo
 // gaussSeidelColorSweep, linear_solver.cu:42-72, u and v together
 __global__ void __launch_bounds__(kBlock) kGSColorUV(DMesh m, DState s, const int* order, int begin, int count) {
     if (finished(s.ctl)) return;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = xcdBlock() * blockDim.x + threadIdx.x;
     if (t >= count) return;
     const int n = order[begin + t];
     double au = s.ub[n], av = s.vb[n];
@@ -765,7 +783,7 @@ __global__ void __launch_bounds__(kBlock) kJacobiSys(DMesh m, const double* b, c
                                                      const double* __restrict__ xo, double* __restrict__ xn,
                                                      const Ctl* ctl) {
     if (finished(ctl)) return;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = xcdBlock() * blockDim.x + threadIdx.x;
     if (n >= m.N) return;
     double a = b[n];
     for (int k = m.faceStart[n]; k < m.faceStart[n + 1]; k++) {
@@ -780,7 +798,7 @@ __global__ void __launch_bounds__(kBlock) kGSColorSys(DMesh m, const double* b, 
                                                       double* x, const int* order, int begin, int count,
                                                       const Ctl* ctl) {
     if (finished(ctl)) return;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = xcdBlock() * blockDim.x + threadIdx.x;
     if (t >= count) return;
     const int n = order[begin + t];
     double a = b[n];
@@ -796,9 +814,10 @@ __global__ void __launch_bounds__(kBlock) kGSColorSys(DMesh m, const double* b, 
 // Pressure correction matrix (createPPCoeff, simple.cu:7-88) + its inverse
 // diagonal (buildInverseDiagonal on ppCoeff, solver.cpp:1178)
 // ===========================================================================
+template <int B>
 __global__ void __launch_bounds__(kBlock) kPPCoeff(DMesh m, DBC bp, DState s, Physics ph) {
     if (finished(s.ctl)) return;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = xcdBlock() * blockDim.x + threadIdx.x;
     if (n >= m.N) return;
     double ac = 0.0;
     const int k0 = m.faceStart[n], ns = m.faceStart[n + 1] - k0;
@@ -821,12 +840,12 @@ __global__ void __launch_bounds__(kBlock) kPPCoeff(DMesh m, DBC bp, DState s, Ph
         }
         return af;
     };
-    if (ns > 0 && ns <= kSlotBatch) {
-        int o[kSlotBatch];
-        double area[kSlotBatch], nz[kSlotBatch], nr[kSlotBatch], w[kSlotBatch], inv[kSlotBatch];
-        double du[kSlotBatch], dv[kSlotBatch], af[kSlotBatch];
+    if (ns > 0 && ns <= B) {
+        int o[B];
+        double area[B], nz[B], nr[B], w[B], inv[B];
+        double du[B], dv[B], af[B];
 #pragma unroll
-        for (int q = 0; q < kSlotBatch; q++) {
+        for (int q = 0; q < B; q++) {
             const int k = slotAt(k0, ns, q);
             o[q] = m.slotNb[k];
             area[q] = m.sArea[k];
@@ -836,16 +855,16 @@ __global__ void __launch_bounds__(kBlock) kPPCoeff(DMesh m, DBC bp, DState s, Ph
             inv[q] = m.sInvDPN[k];
         }
 #pragma unroll
-        for (int q = 0; q < kSlotBatch; q++) {
+        for (int q = 0; q < B; q++) {
             const int oc = o[q] >= 0 ? o[q] : n;
             du[q] = s.DU[oc];
             dv[q] = s.DV[oc];
         }
 #pragma unroll
-        for (int q = 0; q < kSlotBatch; q++)
+        for (int q = 0; q < B; q++)
             if (q < ns) af[q] = step(k0 + q, o[q], area[q], nz[q], nr[q], w[q], inv[q], du[q], dv[q]);
 #pragma unroll
-        for (int q = 0; q < kSlotBatch; q++)
+        for (int q = 0; q < B; q++)
             if (q < ns) s.ppAF[k0 + q] = af[q];
     } else {
         for (int k = k0; k < k0 + ns; k++) {
@@ -861,7 +880,7 @@ __global__ void __launch_bounds__(kBlock) kPPCoeff(DMesh m, DBC bp, DState s, Ph
 // computeFaceMassFluxRhieChow, solver_util.cu:171-203
 __global__ void __launch_bounds__(kBlock) kFaceFlux(DMesh m, DBC bu, DBC bv, DBC bp, DState s, Physics ph) {
     if (finished(s.ctl)) return;
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = xcdBlock() * blockDim.x + threadIdx.x;
     if (f >= m.F) return;
     const double area = m.area[f];
     if (area <= 1.0e-30) {
@@ -873,9 +892,12 @@ __global__ void __launch_bounds__(kBlock) kFaceFlux(DMesh m, DBC bu, DBC bv, DBC
 }
 
 // createPPRhs, simple.cu:90-142
-__global__ void __launch_bounds__(kBlock) kPPRhs(DMesh m, DState s, Physics ph, int cross) {
+// CROSS: with the non-orthogonal cross term (the corrector pass)
+template <int B, bool CROSS>
+__global__ void __launch_bounds__(kBlock) kPPRhs(DMesh m, DState s, Physics ph) {
+    constexpr bool cross = CROSS;
     if (finished(s.ctl)) return;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = xcdBlock() * blockDim.x + threadIdx.x;
     if (n >= m.N) return;
     double imb = 0.0, cs = 0.0;
     const int k0 = m.faceStart[n], ns = m.faceStart[n + 1] - k0;
@@ -890,13 +912,13 @@ __global__ void __launch_bounds__(kBlock) kPPRhs(DMesh m, DState s, Physics ph, 
             cs += (o >= 0) ? nonOrthoVal(tz, tr, w, gzN, gzO, grN, grO, ph.rho * Df) : 0.0;
         }
     };
-    if (ns > 0 && ns <= kSlotBatch) {
-        int o[kSlotBatch], fid[kSlotBatch];
-        uint8_t fl[kSlotBatch];
-        double md[kSlotBatch], nz[kSlotBatch], nr[kSlotBatch], w[kSlotBatch], tz[kSlotBatch], tr[kSlotBatch];
-        double du[kSlotBatch], dv[kSlotBatch], gz[kSlotBatch], gr[kSlotBatch];
+    if (ns > 0 && ns <= B) {
+        int o[B], fid[B];
+        uint8_t fl[B];
+        double md[B], nz[B], nr[B], w[B], tz[B], tr[B];
+        double du[B], dv[B], gz[B], gr[B];
 #pragma unroll
-        for (int q = 0; q < kSlotBatch; q++) {
+        for (int q = 0; q < B; q++) {
             const int k = slotAt(k0, ns, q);
             fid[q] = m.faceIDs[k];
             fl[q] = m.sFlags[k];
@@ -908,7 +930,7 @@ __global__ void __launch_bounds__(kBlock) kPPRhs(DMesh m, DState s, Physics ph, 
             tr[q] = cross ? m.sTR[k] : 0.0;
         }
 #pragma unroll
-        for (int q = 0; q < kSlotBatch; q++) {
+        for (int q = 0; q < B; q++) {
             const int oc = o[q] >= 0 ? o[q] : n;
             md[q] = s.mDot[fid[q]];
             du[q] = cross ? s.DU[oc] : 0.0;
@@ -917,7 +939,7 @@ __global__ void __launch_bounds__(kBlock) kPPRhs(DMesh m, DState s, Physics ph, 
             gr[q] = cross ? s.gPR[oc] : 0.0;
         }
 #pragma unroll
-        for (int q = 0; q < kSlotBatch; q++)
+        for (int q = 0; q < B; q++)
             if (q < ns) step(o[q], fl[q], md[q], nz[q], nr[q], w[q], tz[q], tr[q], du[q], dv[q], gz[q], gr[q]);
     } else {
         for (int k = k0; k < k0 + ns; k++) {
@@ -932,12 +954,13 @@ __global__ void __launch_bounds__(kBlock) kPPRhs(DMesh m, DState s, Physics ph, 
 }
 
 // computeGradient(p', ppBC) fused with updateMomentumPressure (simple.cu:166-179)
+template <int B>
 __global__ void __launch_bounds__(kBlock) kGradPPUpdate(DMesh m, DBC ppbc, DState s, Physics ph) {
     if (finished(s.ctl)) return;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = xcdBlock() * blockDim.x + threadIdx.x;
     if (n >= m.N) return;
     double gz, gr;
-    cellGradient(m, ppbc, s.pp, n, ph.gradScheme, gz, gr);
+    cellGradient<B>(m, ppbc, s.pp, n, ph.gradScheme, gz, gr);
     s.gPZ[n] = gz;
     s.gPR[n] = gr;
     s.u[n] -= s.DU[n] * gz;
@@ -948,7 +971,7 @@ __global__ void __launch_bounds__(kBlock) kGradPPUpdate(DMesh m, DBC ppbc, DStat
 // updateMassFlux, simple.cu:181-258
 __global__ void __launch_bounds__(kBlock) kMassFluxUpdate(DMesh m, DBC bp, DState s, Physics ph) {
     if (finished(s.ctl)) return;
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = xcdBlock() * blockDim.x + threadIdx.x;
     if (f >= m.F) return;
     const int ow = m.owner[f], nb = m.neighbor[f];
     const double area = m.area[f];
@@ -978,10 +1001,11 @@ __global__ void __launch_bounds__(kBlock) kMassFluxUpdate(DMesh m, DBC bp, DStat
 // partial[b*9 + j]: 0 sum|ru| 1 sum ru^2 2 max|ru| 3 sum|AC_u u|
 //                   4 sum|rv| 5 sum rv^2 6 max|rv| 7 sum|AC_v v|  8 sum|imbalance|
 // ===========================================================================
+template <int B>
 __global__ void __launch_bounds__(kBlock) kResidual(DMesh m, DState s) {
     if (finished(s.ctl)) return;
     if (s.ctl->k % s.ctl->checkConv != 0) return;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = xcdBlock() * blockDim.x + threadIdx.x;
     double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (n < m.N) {
         double axu = s.uAC[n] * s.u[n];
@@ -998,12 +1022,12 @@ __global__ void __launch_bounds__(kBlock) kResidual(DMesh m, DState s) {
             if (fl & 1) imb += md;
             else imb -= md;
         };
-        if (ns > 0 && ns <= kSlotBatch) {
-            int o[kSlotBatch], fid[kSlotBatch];
-            uint8_t fl[kSlotBatch];
-            double a[kSlotBatch], uo[kSlotBatch], vo[kSlotBatch], md[kSlotBatch];
+        if (ns > 0 && ns <= B) {
+            int o[B], fid[B];
+            uint8_t fl[B];
+            double a[B], uo[B], vo[B], md[B];
 #pragma unroll
-            for (int q = 0; q < kSlotBatch; q++) {
+            for (int q = 0; q < B; q++) {
                 const int k = slotAt(k0, ns, q);
                 o[q] = m.slotNb[k];
                 a[q] = s.AFuv[k];
@@ -1011,14 +1035,14 @@ __global__ void __launch_bounds__(kBlock) kResidual(DMesh m, DState s) {
                 fl[q] = m.sFlags[k];
             }
 #pragma unroll
-            for (int q = 0; q < kSlotBatch; q++) {
+            for (int q = 0; q < B; q++) {
                 const int oc = o[q] >= 0 ? o[q] : n;
                 uo[q] = s.u[oc];
                 vo[q] = s.v[oc];
                 md[q] = s.mDot[fid[q]];
             }
 #pragma unroll
-            for (int q = 0; q < kSlotBatch; q++)
+            for (int q = 0; q < B; q++)
                 if (q < ns) step(o[q], a[q], uo[q], vo[q], fl[q], md[q]);
         } else {
             for (int k = k0; k < k0 + ns; k++) {
@@ -1047,7 +1071,7 @@ __global__ void __launch_bounds__(kBlock) kResidual(DMesh m, DState s) {
         const int j = threadIdx.x;
         double a = red[0][j];
         for (int w = 1; w < kBlock / 64; w++) a = (j == 2 || j == 6) ? fmax(a, red[w][j]) : a + red[w][j];
-        s.partial[blockIdx.x * 9 + j] = a;
+        s.partial[xcdBlock() * 9 + j] = a;
     }
 }
 
@@ -1125,7 +1149,7 @@ __global__ void __launch_bounds__(kBlock) kFinish(DState s, int blocks) {
 // the per-step loop state.  Copies instead of the reference's pointer swap
 // keep the graph's addresses fixed; the values are the same.
 __global__ void __launch_bounds__(kBlock) kStepBegin(DMesh m, DState s, int bdf2) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = xcdBlock() * blockDim.x + threadIdx.x;
     if (n < m.N) {
         if (bdf2) {
             s.uOld2[n] = s.uOld[n];
@@ -1138,13 +1162,13 @@ __global__ void __launch_bounds__(kBlock) kStepBegin(DMesh m, DState s, int bdf2
 
 __global__ void __launch_bounds__(kBlock) kCopy(double* dst, const double* src, int n, const Ctl* ctl) {
     if (finished(ctl)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = xcdBlock() * blockDim.x + threadIdx.x;
     if (i < n) dst[i] = src[i];
 }
 
 __global__ void __launch_bounds__(kBlock) kFill(double* dst, double value, int n, const Ctl* ctl) {
     if (finished(ctl)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = xcdBlock() * blockDim.x + threadIdx.x;
     if (i < n) dst[i] = value;
 }
 
@@ -1162,7 +1186,7 @@ __global__ void __launch_bounds__(kBlock) kFill(double* dst, double value, int n
 // per member and runs several times slower even on small levels.)
 __global__ void __launch_bounds__(kBlock) kMGCoarseDiag(DLevel f, const Ctl* ctl) {
     if (finished(ctl)) return;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = xcdBlock() * blockDim.x + threadIdx.x;
     if (n >= f.n) return;
     double diag = f.AC[n];
     for (int k = f.start[n]; k < f.start[n + 1]; k++) {
@@ -1174,7 +1198,7 @@ __global__ void __launch_bounds__(kBlock) kMGCoarseDiag(DLevel f, const Ctl* ctl
 
 __global__ void __launch_bounds__(kBlock) kMGCoarseGather(DLevel f, DLevel c, const Ctl* ctl) {
     if (finished(ctl)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;  // gathers by coarse slot: no XCD ranges
     if (i < c.refs) {
         double a = 0.0;
         for (int j = f.cslotStart[i]; j < f.cslotStart[i + 1]; j++) a += f.AF[f.cslotFine[j]];
@@ -1197,7 +1221,7 @@ __global__ void __launch_bounds__(kBlock) kMGCoarseGather(DLevel f, DLevel c, co
 template <int B>
 __global__ void __launch_bounds__(kBlock) kMGResidual(DLevel f, const Ctl* ctl) {
     if (finished(ctl)) return;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = xcdBlock() * blockDim.x + threadIdx.x;
     if (n >= f.n) return;
     const int k0 = f.start[n], ns = f.start[n + 1] - k0;
     double Ax = f.AC[n] * f.x[n];
@@ -1240,7 +1264,7 @@ __device__ __forceinline__ double jacobiRow(const DLevel& L, const double* x, in
 template <int B>
 __global__ void __launch_bounds__(kBlock) kMGJacobi(DLevel L, double w, const Ctl* ctl) {
     if (finished(ctl)) return;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = xcdBlock() * blockDim.x + threadIdx.x;
     if (n >= L.n) return;
     const int k0 = L.start[n], ns = L.start[n + 1] - k0;
     if (B > 0 && ns > 0 && ns <= B) {
@@ -1308,7 +1332,7 @@ __global__ void __launch_bounds__(1024) kMGSmoothBlock(DLevel L, double w, int s
 template <int B>
 __global__ void __launch_bounds__(kBlock) kMGProlongSmooth(DLevel f, DLevel c, double w, const Ctl* ctl) {
     if (finished(ctl)) return;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = xcdBlock() * blockDim.x + threadIdx.x;
     if (n >= f.n) return;
     const int k0 = f.start[n], ns = f.start[n + 1] - k0;
     if (B > 0 && ns > 0 && ns <= B) {
@@ -1357,9 +1381,124 @@ __global__ void __launch_bounds__(kBlock) kMGProlongSmooth(DLevel f, DLevel c, d
     f.xNew[n] = xn + w * (f.b[n] - Ax) / AC;
 }
 
+// Level-0 sweeps of the V-cycles fused per launch (temporal blocking, as in
+// kJacobiUVBlock): workgroup b owns the C cells [bC, bC + C) of level 0 and
+// runs, with the halo rings of the block recomputed and the iterates in LDS,
+//   P: the prolongation x + P x_c as the first sweep's input (the post-sweep
+//      that ends a cycle, as in kMGProlongSmooth),
+//   S weighted-Jacobi sweeps (jacobiRow, multigrid.cu:447-484),
+//   R: then the residual b - A x of the block's cells into f.res (kMGResidual).
+// The final x goes to f.xNew.  One launch replaces the post-sweep of cycle c,
+// the pre-sweep of cycle c + 1 and its residual, each row evaluated as in the
+// one-pass kernels, so the result is bit-identical.
+struct MGRowReg {
+    double af[kSweepRegSlots];
+    int nb[kSweepRegSlots];
+    int loc[kSweepRegSlots];
+    double ac, b;
+};
+
+__device__ __forceinline__ void loadMGRow(MGRowReg& r, const DLevel& L, int c) {
+    const int k0 = L.start[c], ns = L.start[c + 1] - k0;
+#pragma unroll
+    for (int q = 0; q < kSweepRegSlots; q++) {
+        r.nb[q] = (q < ns) ? L.nb[k0 + q] : -1;
+        r.af[q] = (q < ns) ? L.AF[k0 + q] : 0.0;
+    }
+    r.ac = L.AC[c];
+    r.b = L.b[c];
+}
+
+template <typename XAt>
+__device__ __forceinline__ double mgJacobiReg(const MGRowReg& r, XAt xat, double xn, double w) {
+    if (fabs(r.ac) < 1.0e-30) return xn;
+    double xv[kSweepRegSlots];
+#pragma unroll
+    for (int q = 0; q < kSweepRegSlots; q++) xv[q] = (r.nb[q] != -1) ? xat(q) : 0.0;
+    double Ax = r.ac * xn;
+#pragma unroll
+    for (int q = 0; q < kSweepRegSlots; q++)
+        if (r.nb[q] != -1) Ax += r.af[q] * xv[q];
+    return xn + w * (r.b - Ax) / r.ac;
+}
+
+template <int C, int S, bool P, bool R>
+__global__ void __launch_bounds__(C) kMGLevel0Block(DLevel f, DLevel c, DSweepBlocks sb, double w, const Ctl* ctl) {
+    constexpr int H = sweepHalo<C>();
+    constexpr int kSets = (S - 1 + (R ? 1 : 0)) > 1 ? 2 : 1;  // LDS iterate buffers in use
+    __shared__ double buf[kSets][C + H];
+    __shared__ int2 keys[H];
+    if (finished(ctl)) return;
+    const int t = threadIdx.x, blk = xcdBlock();
+    const int a = blk * C, nOwn = min(C, f.n - a);
+    const int h0 = sb.haloStart[blk], nh = sb.haloStart[blk + 1] - h0;
+    // halo entries the first sweep updates: rings 1 .. S - 1 + R
+    constexpr int kRings1 = S - 1 + (R ? 1 : 0);
+    const int nHalo1 = kRings1 > 0 ? sb.ringEnd[blk * sb.rings + kRings1 - 1] : 0;
+    const bool mine = t < nOwn, haloRow = t < nHalo1;
+    const int n = a + t;
+    const int hc = haloRow ? sb.halo[h0 + t] : 0;
+    MGRowReg own, hal;
+    if (mine) loadMGRow(own, f, n);
+    if (haloRow) loadMGRow(hal, f, hc);
+    for (int i = t; i < nh; i += C) keys[i] = sb.sorted[h0 + i];
+    // the first sweep's input at any cell: x, plus the prolonged correction
+    auto x0 = [&](int cell) -> double { return P ? f.x[cell] + c.x[f.toCoarse[cell]] : f.x[cell]; };
+    // sweep j (1-based) writes LDS buffer (j - 1) % kSets; the last one without
+    // a residual writes f.xNew directly
+    double lastOwn = 0.0;
+    {
+        const double v = mine ? mgJacobiReg(own, [&](int q) { return x0(own.nb[q]); }, x0(n), w) : 0.0;
+        const double vh = haloRow ? mgJacobiReg(hal, [&](int q) { return x0(hal.nb[q]); }, x0(hc), w) : 0.0;
+        if (S == 1 && !R) {
+            if (mine) f.xNew[n] = v;
+            return;
+        }
+        if (mine) buf[0][t] = v;
+        if (haloRow) buf[0][C + t] = vh;
+        lastOwn = v;
+    }
+    __syncthreads();
+    const int nHaloLds = (kRings1 > 1) ? sb.ringEnd[blk * sb.rings + kRings1 - 2] : 0;
+#pragma unroll
+    for (int q = 0; q < kSweepRegSlots; q++) {
+        if (mine) own.loc[q] = (own.nb[q] >= 0) ? sweepLocal<C>(own.nb[q], a, nOwn, keys, nh) : 0;
+        if (t < nHaloLds) hal.loc[q] = (hal.nb[q] >= 0) ? sweepLocal<C>(hal.nb[q], a, nOwn, keys, nh) : 0;
+    }
+#pragma unroll
+    for (int j = 2; j <= S; j++) {
+        const double* in = buf[(j - 2) % kSets];
+        double* out = buf[(j - 1) % kSets];
+        const double v = mine ? mgJacobiReg(own, [&](int q) { return in[own.loc[q]]; }, in[t], w) : 0.0;
+        const int nHaloJ = (S - j + (R ? 1 : 0)) > 0 ? sb.ringEnd[blk * sb.rings + (S - j + (R ? 1 : 0)) - 1] : 0;
+        const double vh = (t < nHaloJ) ? mgJacobiReg(hal, [&](int q) { return in[hal.loc[q]]; }, in[C + t], w) : 0.0;
+        if (j == S && !R) {
+            if (mine) f.xNew[n] = v;
+            return;
+        }
+        if (mine) out[t] = v;
+        if (t < nHaloJ) out[C + t] = vh;
+        lastOwn = v;
+        __syncthreads();
+    }
+    // residual (kMGResidual) from the last sweep's iterate
+    if (R && mine) {
+        const double* xs = buf[(S - 1) % kSets];
+        f.xNew[n] = lastOwn;
+        double xv[kSweepRegSlots];
+#pragma unroll
+        for (int q = 0; q < kSweepRegSlots; q++) xv[q] = (own.nb[q] >= 0) ? xs[own.loc[q]] : 0.0;
+        double Ax = own.ac * lastOwn;
+#pragma unroll
+        for (int q = 0; q < kSweepRegSlots; q++)
+            if (own.nb[q] >= 0) Ax += own.af[q] * xv[q];
+        f.res[n] = own.b - Ax;
+    }
+}
+
 __global__ void __launch_bounds__(kBlock) kMGProlong(DLevel f, DLevel c, const Ctl* ctl) {
     if (finished(ctl)) return;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = xcdBlock() * blockDim.x + threadIdx.x;
     if (n >= f.n) return;
     f.x[n] += c.x[f.toCoarse[n]];
 }
@@ -1404,7 +1543,7 @@ __device__ __forceinline__ T sweepFromZero(T b, T AC, T w) {
 template <class T, int B>
 __global__ void __launch_bounds__(kBlock) kMGRestrictSum(DLevel f, DLevel c, double w, int sweep1, const Ctl* ctl) {
     if (finished(ctl)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = xcdBlock() * blockDim.x + threadIdx.x;
     if (i >= c.n) return;
     const LV<T> fv = view<T>(f);
     const LV<T> cv = view<T>(c);
@@ -1451,7 +1590,7 @@ template <class T>
 __global__ void __launch_bounds__(kBlock) kMGGSColor(DLevel L, const int* __restrict__ order, int begin, int count,
                                                      const Ctl* ctl) {
     if (finished(ctl)) return;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = xcdBlock() * blockDim.x + threadIdx.x;
     if (t >= count) return;
     const LV<T> v = view<T>(L);
     const int n = order[begin + t];
@@ -1798,14 +1937,24 @@ void launchMGRestrictSum(hipStream_t s, const DLevel& f, const DLevel& c, double
     }
 }
 void launchGradUV(hipStream_t s, const DMesh& m, DBC bu, DBC bv, DState st, int scheme) {
-    kGradUV<<<blocksFor(m.N), kBlock, 0, s>>>(m, bu, bv, st, scheme);
+    if (m.maxSlots <= 3) kGradUV<3><<<blocksFor(m.N), kBlock, 0, s>>>(m, bu, bv, st, scheme);
+    else kGradUV<4><<<blocksFor(m.N), kBlock, 0, s>>>(m, bu, bv, st, scheme);
 }
 void launchGradField(hipStream_t s, const DMesh& m, DBC bc, const double* phi, double* gz, double* gr,
                      int scheme, const Ctl* ctl) {
-    kGradField<<<blocksFor(m.N), kBlock, 0, s>>>(m, bc, phi, gz, gr, scheme, ctl);
+    if (m.maxSlots <= 3) kGradField<3><<<blocksFor(m.N), kBlock, 0, s>>>(m, bc, phi, gz, gr, scheme, ctl);
+    else kGradField<4><<<blocksFor(m.N), kBlock, 0, s>>>(m, bc, phi, gz, gr, scheme, ctl);
 }
 void launchMomentum(hipStream_t s, const DMesh& m, DBC bu, DBC bv, DBC bp, DState st, Physics ph) {
-    kMomentum<<<blocksFor(m.N), kBlock, 0, s>>>(m, bu, bv, bp, st, ph);
+    const dim3 g(blocksFor(m.N)), b(kBlock);
+    const bool ho = ph.scheme != 0;
+    if (m.maxSlots <= 3) {
+        if (ho) kMomentum<3, true><<<g, b, 0, s>>>(m, bu, bv, bp, st, ph);
+        else kMomentum<3, false><<<g, b, 0, s>>>(m, bu, bv, bp, st, ph);
+    } else {
+        if (ho) kMomentum<4, true><<<g, b, 0, s>>>(m, bu, bv, bp, st, ph);
+        else kMomentum<4, false><<<g, b, 0, s>>>(m, bu, bv, bp, st, ph);
+    }
 }
 void launchJacobiUV(hipStream_t s, const DMesh& m, DState st, const double* uo, const double* vo, double* un,
                     double* vn) {
@@ -1848,22 +1997,32 @@ void launchGSColorSys(hipStream_t s, const DMesh& m, const double* b, const doub
     kGSColorSys<<<blocksFor(count), kBlock, 0, s>>>(m, b, inv, AF, x, order, begin, count, ctl);
 }
 void launchPPCoeff(hipStream_t s, const DMesh& m, DBC bp, DState st, Physics ph) {
-    kPPCoeff<<<blocksFor(m.N), kBlock, 0, s>>>(m, bp, st, ph);
+    if (m.maxSlots <= 3) kPPCoeff<3><<<blocksFor(m.N), kBlock, 0, s>>>(m, bp, st, ph);
+    else kPPCoeff<4><<<blocksFor(m.N), kBlock, 0, s>>>(m, bp, st, ph);
 }
 void launchFaceFlux(hipStream_t s, const DMesh& m, DBC bu, DBC bv, DBC bp, DState st, Physics ph) {
     kFaceFlux<<<blocksFor(m.F), kBlock, 0, s>>>(m, bu, bv, bp, st, ph);
 }
 void launchPPRhs(hipStream_t s, const DMesh& m, DState st, Physics ph, int cross) {
-    kPPRhs<<<blocksFor(m.N), kBlock, 0, s>>>(m, st, ph, cross);
+    const dim3 g(blocksFor(m.N)), b(kBlock);
+    if (m.maxSlots <= 3) {
+        if (cross) kPPRhs<3, true><<<g, b, 0, s>>>(m, st, ph);
+        else kPPRhs<3, false><<<g, b, 0, s>>>(m, st, ph);
+    } else {
+        if (cross) kPPRhs<4, true><<<g, b, 0, s>>>(m, st, ph);
+        else kPPRhs<4, false><<<g, b, 0, s>>>(m, st, ph);
+    }
 }
 void launchGradPPUpdate(hipStream_t s, const DMesh& m, DBC ppbc, DState st, Physics ph) {
-    kGradPPUpdate<<<blocksFor(m.N), kBlock, 0, s>>>(m, ppbc, st, ph);
+    if (m.maxSlots <= 3) kGradPPUpdate<3><<<blocksFor(m.N), kBlock, 0, s>>>(m, ppbc, st, ph);
+    else kGradPPUpdate<4><<<blocksFor(m.N), kBlock, 0, s>>>(m, ppbc, st, ph);
 }
 void launchMassFluxUpdate(hipStream_t s, const DMesh& m, DBC bp, DState st, Physics ph) {
     kMassFluxUpdate<<<blocksFor(m.F), kBlock, 0, s>>>(m, bp, st, ph);
 }
 void launchResidual(hipStream_t s, const DMesh& m, DState st, int blocks) {
-    kResidual<<<blocks, kBlock, 0, s>>>(m, st);
+    if (m.maxSlots <= 3) kResidual<3><<<blocks, kBlock, 0, s>>>(m, st);
+    else kResidual<4><<<blocks, kBlock, 0, s>>>(m, st);
 }
 void launchFinish(hipStream_t s, DState st, int blocks) { kFinish<<<1, kBlock, 0, s>>>(st, blocks); }
 void launchStepBegin(hipStream_t s, const DMesh& m, DState st, int bdf2) {
@@ -1892,6 +2051,19 @@ void launchMGProlongSmooth(hipStream_t s, const DLevel& f, const DLevel& c, doub
     else if (f.maxRow <= 8) kMGProlongSmooth<8><<<g, b, 0, s>>>(f, c, w, ctl);
     else if (f.maxRow <= 16) kMGProlongSmooth<16><<<g, b, 0, s>>>(f, c, w, ctl);
     else kMGProlongSmooth<0><<<g, b, 0, s>>>(f, c, w, ctl);
+}
+void launchMGLevel0Block(hipStream_t s, const DLevel& f, const DLevel& c, const DSweepBlocks& sb, int sweeps,
+                         bool prolong, double w, const Ctl* ctl) {
+    if (sb.cells != 512) throw std::invalid_argument("fused level-0 sweeps take blocks of 512 cells");
+    if (sweeps == 1 && !prolong) {
+        if (sb.rings < 1) throw std::invalid_argument("fused level-0 sweeps need one halo ring");
+        kMGLevel0Block<512, 1, false, true><<<sb.nBlocks, 512, 0, s>>>(f, c, sb, w, ctl);
+    } else if (sweeps == 2 && prolong) {
+        if (sb.rings < 2) throw std::invalid_argument("fused level-0 sweeps need two halo rings");
+        kMGLevel0Block<512, 2, true, true><<<sb.nBlocks, 512, 0, s>>>(f, c, sb, w, ctl);
+    } else {
+        throw std::invalid_argument("unsupported fused level-0 sweep combination");
+    }
 }
 void launchMGProlong(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl) {
     kMGProlong<<<blocksFor(f.n, f.block), f.block, 0, s>>>(f, c, ctl);

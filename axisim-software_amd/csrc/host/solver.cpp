@@ -96,6 +96,24 @@ void MGDevice::build(int n, const std::vector<int>& start, const std::vector<int
         AXS_HIP(hipMemcpy(tail64_.dev, tail64_.lv.data(), tail64_.lv.size() * sizeof(TailLevel),
                           hipMemcpyHostToDevice));
     }
+    // fused level-0 sweeps: blocks of 512 cells with two halo rings, when
+    // level 0 is large, its rows fit the kernel's registers and every halo its LDS
+    fuseL0_ = false;
+    dl0_ = DSweepBlocks{};
+    if (host.size() >= 2 && host[0].n >= kFuseL0MinCells) {
+        buildSweepBlocks(host[0].n, host[0].start, host[0].nb, 512, 2, l0Blocks_);
+        if (l0Blocks_.maxSlots <= kSweepRegSlots && l0Blocks_.maxHalo <= sweepHaloMax(512)) {
+            dl0_.cells = l0Blocks_.cells;
+            dl0_.rings = l0Blocks_.rings;
+            dl0_.nBlocks = l0Blocks_.nBlocks;
+            dl0_.haloStart = arena.upload(l0Blocks_.haloStart);
+            dl0_.ringEnd = arena.upload(l0Blocks_.ringEnd);
+            dl0_.halo = arena.upload(l0Blocks_.halo.empty() ? std::vector<int>{0} : l0Blocks_.halo);
+            dl0_.sorted = reinterpret_cast<const int2*>(
+                arena.upload(l0Blocks_.sorted.empty() ? std::vector<int>{0, 0} : l0Blocks_.sorted));
+            fuseL0_ = true;
+        }
+    }
 }
 
 // Upload level l's CSR and transfer maps into the int part of the tail image
@@ -250,6 +268,26 @@ void MGDevice::vcycle(hipStream_t s, int l, int preDone, const MGParams& p, cons
 }
 
 void MGDevice::swapX32(int l) { std::swap(dev[l].x32, dev[l].xNew32); }
+
+bool MGDevice::fusesLevel0(const MGParams& p) const {
+    return fuseL0_ && host.size() >= 2 && p.pre == 1 && p.post == 1 && p.smoother == AXS_MG_SMOOTH_JACOBI;
+}
+
+// The V-cycles of one solve with the level-0 work fused across cycles: each
+// cycle's pre-sweep and residual run in one launch together with the previous
+// cycle's prolongation and post-sweep; the last cycle's post-sweep runs alone.
+// The same sweeps and residuals as vcycle(0) repeated, bit for bit.
+void MGDevice::cyclesFused(hipStream_t s, int cycles, const MGParams& p, const Ctl* ctl) {
+    const int s1 = host[1].coarsest() ? (p.coarseSweeps >= 1) : (p.pre >= 1);
+    for (int c = 0; c < cycles; c++) {
+        launchMGLevel0Block(s, dev[0], dev[1], dl0_, c == 0 ? 1 : 2, c > 0, p.weight, ctl);
+        swapX(0);
+        launchMGRestrictSum(s, dev[0], dev[1], p.weight, s1, false, ctl);
+        vcycle(s, 1, s1, p, ctl);
+    }
+    launchMGProlongSmooth(s, dev[0], dev[1], p.weight, ctl);
+    swapX(0);
+}
 
 // Greedy colouring of every level's graph (buildMeshColoring order), built on
 // first use of the Gauss-Seidel smoother.
@@ -437,7 +475,9 @@ void MGDevice::enqueueSolve(hipStream_t s, const MGParams& p, const Ctl* ctl, bo
         for (int c = 0; c < cycles; c++) vcycleGS(s, 0, false, p, ctl);
         return;  // x never leaves the level-0 buffer bound as x
     }
-    for (int c = 0; c < cycles; c++) vcycle(s, 0, 0, p, ctl);
+    if (fusesLevel0(p)) cyclesFused(s, cycles, p, ctl);
+    else
+        for (int c = 0; c < cycles; c++) vcycle(s, 0, 0, p, ctl);
     if (dev[0].x != x0Home_) {
         launchCopy(s, x0Home_, dev[0].x, dev[0].n, ctl);
         dev[0].x = x0Home_;
@@ -495,6 +535,47 @@ double MGDevice::bytesPerSolve(const MGParams& p, bool buildOperators) const {
         if (sweeps > 1 && host[l].n <= kSmoothBlockMax) return (4.0 * v + 4.0) * host[l].n + (v + 4.0) * host[l].refs();
         return sweeps * jac(l);
     };
+    // one cycle from level `from` down, its pre-smoothing partly done
+    auto cycleFrom = [&](int from, int preDone) {
+        double cyc = 0.0;
+        for (int l = from; l < L; l++) {
+            if (l == tail.lt) {
+                cyc += tailTrafficBytes(tail, (size_t)v);
+                break;
+            }
+            if (host[l].coarsest()) {
+                cyc += (mixed && p.coarseDirect) ? 8.0 * host[l].n * host[l].n + 2.0 * v * host[l].n
+                                                 : sm(l, p.coarseSweeps - preDone);
+                break;
+            }
+            const double n = host[l].n, S = host[l].refs(), nc = host[l + 1].n;
+            cyc += sm(l, p.pre - preDone);
+            const int s1 =
+                host[l + 1].coarsest() ? (!(mixed && p.coarseDirect) && p.coarseSweeps >= 1) : (p.pre >= 1);
+            // restriction: fine rows (AC, b, x, start, members) + the residual written and
+            // re-read + memberStart, coarse b and x (+ coarse AC for the folded sweep)
+            cyc += (5.0 * v + 8.0) * n + (v + 4.0) * S + (2.0 * v + 4.0) * nc + (s1 ? v * nc : 0.0);
+            if (p.post >= 1) cyc += (4.0 * v + 8.0) * n + (v + 4.0) * S + v * nc + sm(l, p.post - 1);
+            else cyc += (2.0 * v + 4.0) * n + v * nc;
+            preDone = s1;
+        }
+        return cyc;
+    };
+    const int cycles = p.cycles > 1 ? p.cycles : 1;
+    if (!mixed && fusesLevel0(p)) {
+        // level 0 as cyclesFused launches it: (sweep + residual), then per later
+        // cycle (prolongation + 2 sweeps + residual) -- rows AC, b, x, start,
+        // AF/nb (+ toCoarse, coarse x), x and the residual written -- the
+        // restriction per cycle (residual, members; coarse b, x, AC), and the
+        // last post-sweep
+        const double n = host[0].n, S = host[0].refs(), nc = host[1].n;
+        const int s1 = host[1].coarsest() ? (p.coarseSweeps >= 1) : (p.pre >= 1);
+        double l0 = (5.0 * v + 4.0) * n + (v + 4.0) * S;
+        l0 += (cycles - 1) * ((5.0 * v + 8.0) * n + (v + 4.0) * S + v * nc);
+        l0 += cycles * ((v + 4.0) * n + (2.0 * v + 4.0) * nc + (s1 ? v * nc : 0.0));
+        l0 += (4.0 * v + 8.0) * n + (v + 4.0) * S + v * nc;
+        return bytes + l0 + cycles * cycleFrom(1, s1);
+    }
     double cyc = 0.0;
     int preDone = 0;
     if (mixed) {
@@ -502,28 +583,9 @@ double MGDevice::bytesPerSolve(const MGParams& p, bool buildOperators) const {
         cyc += 32.0 * host[0].n + 12.0 * host[0].refs() + 8.0 * host[0].n + (p.pre >= 1 ? 4.0 * host[0].n : 0.0);
         preDone = (L == 1 ? (!p.coarseDirect && p.coarseSweeps >= 1) : (p.pre >= 1)) ? 1 : 0;
     }
-    for (int l = 0; l < L; l++) {
-        if (l == tail.lt) {
-            cyc += tailTrafficBytes(tail, (size_t)v);
-            break;
-        }
-        if (host[l].coarsest()) {
-            cyc += (mixed && p.coarseDirect) ? 8.0 * host[l].n * host[l].n + 2.0 * v * host[l].n
-                                             : sm(l, p.coarseSweeps - preDone);
-            break;
-        }
-        const double n = host[l].n, S = host[l].refs(), nc = host[l + 1].n;
-        cyc += sm(l, p.pre - preDone);
-        const int s1 = host[l + 1].coarsest() ? (!(mixed && p.coarseDirect) && p.coarseSweeps >= 1) : (p.pre >= 1);
-        // restriction: fine rows (AC, b, x, start, members) + the residual written and
-        // re-read + memberStart, coarse b and x (+ coarse AC for the folded sweep)
-        cyc += (5.0 * v + 8.0) * n + (v + 4.0) * S + (2.0 * v + 4.0) * nc + (s1 ? v * nc : 0.0);
-        if (p.post >= 1) cyc += (4.0 * v + 8.0) * n + (v + 4.0) * S + v * nc + sm(l, p.post - 1);
-        else cyc += (2.0 * v + 4.0) * n + v * nc;
-        preDone = s1;
-    }
+    cyc += cycleFrom(0, preDone);
     if (mixed) cyc += 20.0 * host[0].n;  // x += e (FP64 x read + write, FP32 e)
-    return bytes + (p.cycles > 1 ? p.cycles : 1) * cyc;
+    return bytes + cycles * cyc;
 }
 
 // ---------------------------------------------------------------------------
@@ -658,6 +720,8 @@ void Solver::uploadMesh() {
     d.faceStart = meshArena_.upload(m.cellFaceStart);
     d.faceIDs = meshArena_.upload(m.cellFaceIDs);
     d.slotNb = meshArena_.upload(csrNb_);
+    d.maxSlots = 0;
+    for (int c = 0; c < m.nCells; c++) d.maxSlots = std::max(d.maxSlots, m.cellFaceStart[c + 1] - m.cellFaceStart[c]);
     d.owner = meshArena_.upload(m.faceOwner);
     d.neighbor = meshArena_.upload(m.faceNeighbor);
     d.group = meshArena_.upload(m.faceGroup);

@@ -135,6 +135,14 @@ private:
     double* x0Other_ = nullptr;
     TailPlan tail64_, tail32_;
     int ldsLimit_ = 0;
+    // level 0's blocks and halo rings for the fused level-0 sweeps (FP64
+    // Jacobi V-cycles with one pre- and one post-sweep)
+    SweepBlocks l0Blocks_;
+    DSweepBlocks dl0_{};
+    bool fuseL0_ = false;
+    static constexpr int kFuseL0MinCells = 16384;
+    bool fusesLevel0(const MGParams& p) const;
+    void cyclesFused(hipStream_t s, int cycles, const MGParams& p, const Ctl* ctl);
 };
 
 class Solver {
