@@ -14,6 +14,7 @@ struct DMesh {
     int N = 0, F = 0, S = 0;
     int sweepBlock = kBlock;  // workgroup size of the momentum sweep (autotuned)
     int maxSlots = 0;         // most slots of a cell (the cell loops batch up to 3 or 4)
+    int uniformSlots = 0;     // > 0: every cell has this many slots (faceStart[n] = uniformSlots * n)
     const double *cZ, *cR, *vol, *invA2D, *lsqInvZZ, *lsqInvZR, *lsqInvRR;
     const int *faceStart, *faceIDs;
     const int *slotNb;      // neighbour cell of each slot, -1 on a boundary face
@@ -90,6 +91,7 @@ struct DLevel {
     int block = kBlock;  // workgroup size of this level's per-cell kernels (autotuned)
     int maxRow = 0;      // most off-diagonal slots of a row
     int maxMembers = 0;  // most fine cells of one coarse cell (levels with a coarser one)
+    int uniformRow = 0;  // > 0: every row has this many slots (start[n] = uniformRow * n)
     int *start = nullptr, *nb = nullptr;
     double *AC = nullptr, *b = nullptr, *AF = nullptr;
     double *x = nullptr, *xNew = nullptr;

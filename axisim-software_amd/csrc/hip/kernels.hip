@@ -17,24 +17,14 @@
 #include <stdexcept>
 
 #include "device.hpp"
+#include "mixed.hpp"
+#include "rows.hpp"
 
 namespace axs {
 
 namespace {
 
-__device__ __forceinline__ bool finished(const Ctl* ctl) { return ctl != nullptr && ctl->done != 0; }
 
-// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b runs on XCD
-// b % 8), and each XCD has its own L2.  Every kernel here numbers its blocks
-// so that XCD x runs one contiguous range of them: the neighbour gathers
-// across a block's edge then mostly hit lines its neighbour block, on the same
-// XCD, has just brought in.  (Only the placement changes; each block still
-// covers the same cells.)
-__device__ __forceinline__ int xcdBlock() {
-    const int g = gridDim.x, b = blockIdx.x;
-    const int q = g >> 3, r = g & 7, x = b & 7;
-    return x * q + min(x, r) + (b >> 3);
-}
 
 // ---- face helpers (solver/header/solver_util.cuh:56-430) ----
 
@@ -124,13 +114,7 @@ __device__ __forceinline__ double nonOrthoFlux(const DMesh& m, int c, int f, con
 // bit-identical to the face form (the squares of the normal in
 // normalCorrToFace do not see its orientation); only boundary slots still read
 // face-indexed data.
-// Cells with at most B slots (B = 3 on triangle meshes, 4 otherwise) walk
-// their slots in three dependent rounds of loads instead of one round per slot:
-// every slot's data (indices clamped into the cell's range, so every load is in
-// bounds and can issue at once), then every neighbour gather, then the
-// arithmetic slot by slot in the reference's order.  Larger cells take the
-// plain loop.
-__device__ __forceinline__ int slotAt(int k0, int ns, int q) { return k0 + (q < ns ? q : ns - 1); }
+
 
 // nonOrthoFlux from loaded values (T, w, the gradients at c and o)
 __device__ __forceinline__ double nonOrthoVal(double Tz, double Tr, double w, double gzc, double gzo, double grc,
@@ -157,7 +141,8 @@ __device__ __forceinline__ double normalCorrVal(double DUc, double DVc, double D
 template <int NF, int B>
 __device__ void cellGradients(const DMesh& m, const DBC* bc, const double* const* phi, int c, int scheme,
                               double* gZ, double* gR) {
-    const int k0 = m.faceStart[c], ns = m.faceStart[c + 1] - k0;
+    int k0, ns;
+    rowRange(m.faceStart, m.uniformSlots, c, k0, ns);
     const bool batch = ns > 0 && ns <= B;
     double phiP[NF];
 #pragma unroll
@@ -451,7 +436,8 @@ __global__ void __launch_bounds__(kBlock) kMomentum(DMesh m, DBC bu, DBC bv, DBC
     if (n >= m.N) return;
     const double mu = ph.mu;
     double acU = 0.0, bU = 0.0, acV = 0.0, bV = 0.0, netF = 0.0;
-    const int k0 = m.faceStart[n], ns = m.faceStart[n + 1] - k0;
+    int k0, ns;
+    rowRange(m.faceStart, m.uniformSlots, n, k0, ns);
     const bool needW = ph.nonOrtho || ph.scheme == 1;
     const bool needUV = HO;
     const double uN = s.u[n], vN = s.v[n];
@@ -632,18 +618,41 @@ struct UVOut {
     __device__ __forceinline__ void put(int i, double2 x) const { put(i, x.x, x.y); }
 };
 
+template <int B>
 __global__ void __launch_bounds__(kBlock) kJacobiUV(DMesh m, DState s, UVIn xo, UVOut xn) {
     if (finished(s.ctl)) return;
     const int n = xcdBlock() * blockDim.x + threadIdx.x;
     if (n >= m.N) return;
     double au = s.ub[n], av = s.vb[n];
-    for (int k = m.faceStart[n]; k < m.faceStart[n + 1]; k++) {
-        const int nb = m.slotNb[k];
-        if (nb >= 0) {
-            const double a = s.AFuv[k];
-            const double2 x = xo.at(nb);
-            au -= a * x.x;
-            av -= a * x.y;
+    int k0, ns;
+    rowRange(m.faceStart, m.uniformSlots, n, k0, ns);
+    if (ns > 0 && ns <= B) {
+        int o[B];
+        double a[B];
+        double2 xv[B];
+#pragma unroll
+        for (int q = 0; q < B; q++) {
+            const int k = slotAt(k0, ns, q);
+            o[q] = m.slotNb[k];
+            a[q] = s.AFuv[k];
+        }
+#pragma unroll
+        for (int q = 0; q < B; q++) xv[q] = xo.at(o[q] >= 0 ? o[q] : n);
+#pragma unroll
+        for (int q = 0; q < B; q++)
+            if (q < ns && o[q] >= 0) {
+                au -= a[q] * xv[q].x;
+                av -= a[q] * xv[q].y;
+            }
+    } else {
+        for (int k = k0; k < k0 + ns; k++) {
+            const int nb = m.slotNb[k];
+            if (nb >= 0) {
+                const double a = s.AFuv[k];
+                const double2 x = xo.at(nb);
+                au -= a * x.x;
+                av -= a * x.y;
+            }
         }
     }
     const double iu = s.uInv[n], iv = s.vInv[n];
@@ -670,7 +679,8 @@ struct SweepRow {
 };
 
 __device__ __forceinline__ void loadSweepRow(SweepRow& r, const DMesh& m, const DState& s, int c) {
-    const int k0 = m.faceStart[c], ns = m.faceStart[c + 1] - k0;
+    int k0, ns;
+    rowRange(m.faceStart, m.uniformSlots, c, k0, ns);
 #pragma unroll
     for (int q = 0; q < kSweepRegSlots; q++) {
         r.nb[q] = (q < ns) ? m.slotNb[k0 + q] : -1;
@@ -820,7 +830,8 @@ __global__ void __launch_bounds__(kBlock) kPPCoeff(DMesh m, DBC bp, DState s, Ph
     const int n = xcdBlock() * blockDim.x + threadIdx.x;
     if (n >= m.N) return;
     double ac = 0.0;
-    const int k0 = m.faceStart[n], ns = m.faceStart[n + 1] - k0;
+    int k0, ns;
+    rowRange(m.faceStart, m.uniformSlots, n, k0, ns);
     const double DUn = s.DU[n], DVn = s.DV[n];
     auto step = [&](int k, int o, double area, double nz, double nr, double w, double invDPN, double DUo,
                     double DVo) -> double {
@@ -900,7 +911,8 @@ __global__ void __launch_bounds__(kBlock) kPPRhs(DMesh m, DState s, Physics ph) 
     const int n = xcdBlock() * blockDim.x + threadIdx.x;
     if (n >= m.N) return;
     double imb = 0.0, cs = 0.0;
-    const int k0 = m.faceStart[n], ns = m.faceStart[n + 1] - k0;
+    int k0, ns;
+    rowRange(m.faceStart, m.uniformSlots, n, k0, ns);
     const double DUn = cross ? s.DU[n] : 0.0, DVn = cross ? s.DV[n] : 0.0;
     const double gzN = cross ? s.gPZ[n] : 0.0, grN = cross ? s.gPR[n] : 0.0;
     auto step = [&](int o, uint8_t fl, double md, double nz, double nr, double w, double tz, double tr, double DUo,
@@ -1013,7 +1025,8 @@ __global__ void __launch_bounds__(kBlock) kResidual(DMesh m, DState s) {
         v[3] = fabs(axu);
         v[7] = fabs(axv);
         double imb = 0.0;
-        const int k0 = m.faceStart[n], ns = m.faceStart[n + 1] - k0;
+        int k0, ns;
+    rowRange(m.faceStart, m.uniformSlots, n, k0, ns);
         auto step = [&](int o, double a, double uo, double vo, uint8_t fl, double md) {
             if (o >= 0) {
                 axu += a * uo;
@@ -1213,84 +1226,113 @@ __global__ void __launch_bounds__(kBlock) kMGCoarseGather(DLevel f, DLevel c, co
     }
 }
 
+// A level's values in FP64 or in its FP32 copies (the mixed-precision cycle):
+// the per-cell multigrid kernels below are templates on the value type.
+namespace {
+template <class T>
+struct LV {
+    const T *AC, *AF;
+    T *b, *x, *xNew, *res;
+};
+template <class T>
+__device__ __forceinline__ LV<T> view(const DLevel& L);
+template <>
+__device__ __forceinline__ LV<double> view<double>(const DLevel& L) {
+    return LV<double>{L.AC, L.AF, L.b, L.x, L.xNew, L.res};
+}
+template <>
+__device__ __forceinline__ LV<float> view<float>(const DLevel& L) {
+    return LV<float>{L.AC32, L.AF32, L.b32, L.x32, L.xNew32, L.res32};
+}
+template <class T>
+__device__ __forceinline__ T tabs(T v) {
+    return v < T(0) ? -v : v;
+}
+// jacobiRow (multigrid.cu:447-484) applied to x = 0: every product with x is a
+// (signed) zero, b - Ax == b, and 0 + w*b/AC is the reference's value bit for
+// bit (a zero result comes out +0 either way).
+template <class T>
+__device__ __forceinline__ T sweepFromZero(T b, T AC, T w) {
+    if (tabs(AC) < T(1.0e-30)) return T(0);
+    return T(0) + w * b / AC;
+}
+}  // namespace
+
 // computeResidual (residualAll sign = true, residuals.cuh:17-60) into f.res,
 // one thread per fine cell; kMGRestrictSum then gathers it per coarse cell.
 // B > 0: rows of at most B slots load their slots, then their neighbours, in
 // one round each (indices clamped into the row); B = 0 or longer rows: the
 // plain loop.  The arithmetic is the same in either case.
-template <int B>
+template <class T, int B>
 __global__ void __launch_bounds__(kBlock) kMGResidual(DLevel f, const Ctl* ctl) {
     if (finished(ctl)) return;
     const int n = xcdBlock() * blockDim.x + threadIdx.x;
     if (n >= f.n) return;
-    const int k0 = f.start[n], ns = f.start[n + 1] - k0;
-    double Ax = f.AC[n] * f.x[n];
+    const LV<T> v = view<T>(f);
+    int k0, ns;
+    rowRange(f.start, f.uniformRow, n, k0, ns);
+    T Ax = v.AC[n] * v.x[n];
     if (B > 0 && ns > 0 && ns <= B) {
         int o[B > 0 ? B : 1];
-        double a[B > 0 ? B : 1], xo[B > 0 ? B : 1];
+        T a[B > 0 ? B : 1], xo[B > 0 ? B : 1];
 #pragma unroll
         for (int q = 0; q < B; q++) {
             const int k = slotAt(k0, ns, q);
             o[q] = f.nb[k];
-            a[q] = f.AF[k];
+            a[q] = v.AF[k];
         }
 #pragma unroll
-        for (int q = 0; q < B; q++) xo[q] = f.x[o[q] >= 0 ? o[q] : n];
+        for (int q = 0; q < B; q++) xo[q] = v.x[o[q] >= 0 ? o[q] : n];
 #pragma unroll
         for (int q = 0; q < B; q++)
             if (q < ns && o[q] >= 0) Ax += a[q] * xo[q];
     } else {
         for (int k = k0; k < k0 + ns; k++) {
             const int nb = f.nb[k];
-            if (nb >= 0) Ax += f.AF[k] * f.x[nb];
+            if (nb >= 0) Ax += v.AF[k] * v.x[nb];
         }
     }
-    f.res[n] = f.b[n] - Ax;
-}
-
-// jacobiRow, multigrid.cu:447-484
-__device__ __forceinline__ double jacobiRow(const DLevel& L, const double* x, int n, double w) {
-    const double AC = L.AC[n];
-    if (fabs(AC) < 1.0e-30) return x[n];
-    double Ax = AC * x[n];
-    for (int k = L.start[n]; k < L.start[n + 1]; k++) {
-        const int nb = L.nb[k];
-        if (nb != -1) Ax += L.AF[k] * x[nb];
-    }
-    return x[n] + w * (L.b[n] - Ax) / AC;
+    v.res[n] = v.b[n] - Ax;
 }
 
 // jacobiFused, multigrid.cu:486-493 (B as in kMGResidual)
-template <int B>
-__global__ void __launch_bounds__(kBlock) kMGJacobi(DLevel L, double w, const Ctl* ctl) {
+template <class T, int B>
+__global__ void __launch_bounds__(kBlock) kMGJacobi(DLevel L, T w, const Ctl* ctl) {
     if (finished(ctl)) return;
     const int n = xcdBlock() * blockDim.x + threadIdx.x;
     if (n >= L.n) return;
-    const int k0 = L.start[n], ns = L.start[n + 1] - k0;
+    const LV<T> v = view<T>(L);
+    int k0, ns;
+    rowRange(L.start, L.uniformRow, n, k0, ns);
+    const T AC = v.AC[n], xn = v.x[n], bn = v.b[n];
+    T r = xn;
     if (B > 0 && ns > 0 && ns <= B) {
         int o[B > 0 ? B : 1];
-        double a[B > 0 ? B : 1], xo[B > 0 ? B : 1];
+        T a[B > 0 ? B : 1], xo[B > 0 ? B : 1];
 #pragma unroll
         for (int q = 0; q < B; q++) {
             const int k = slotAt(k0, ns, q);
             o[q] = L.nb[k];
-            a[q] = L.AF[k];
+            a[q] = v.AF[k];
         }
-        const double AC = L.AC[n], xn = L.x[n], bn = L.b[n];
 #pragma unroll
-        for (int q = 0; q < B; q++) xo[q] = L.x[o[q] >= 0 ? o[q] : n];
-        double r = xn;
-        if (!(fabs(AC) < 1.0e-30)) {
-            double Ax = AC * xn;
+        for (int q = 0; q < B; q++) xo[q] = v.x[o[q] >= 0 ? o[q] : n];
+        if (!(tabs(AC) < T(1.0e-30))) {
+            T Ax = AC * xn;
 #pragma unroll
             for (int q = 0; q < B; q++)
                 if (q < ns && o[q] != -1) Ax += a[q] * xo[q];
             r = xn + w * (bn - Ax) / AC;
         }
-        L.xNew[n] = r;
-    } else {
-        L.xNew[n] = jacobiRow(L, L.x, n, w);
+    } else if (!(tabs(AC) < T(1.0e-30))) {
+        T Ax = AC * xn;
+        for (int k = k0; k < k0 + ns; k++) {
+            const int nb = L.nb[k];
+            if (nb != -1) Ax += v.AF[k] * v.x[nb];
+        }
+        r = xn + w * (bn - Ax) / AC;
     }
+    v.xNew[n] = r;
 }
 
 // jacobiSingleBlock, multigrid.cu:501-541: all sweeps of a small level in one
@@ -1329,56 +1371,59 @@ __global__ void __launch_bounds__(1024) kMGSmoothBlock(DLevel L, double w, int s
 // buildProlongationKernel (multigrid.cu:415-427) fused into the first
 // post-smoothing sweep: x + P x_c is formed on the fly for the cell and its
 // neighbours (the same sums the prolongation would store).
-template <int B>
-__global__ void __launch_bounds__(kBlock) kMGProlongSmooth(DLevel f, DLevel c, double w, const Ctl* ctl) {
+template <class T, int B>
+__global__ void __launch_bounds__(kBlock) kMGProlongSmooth(DLevel f, DLevel c, T w, const Ctl* ctl) {
     if (finished(ctl)) return;
     const int n = xcdBlock() * blockDim.x + threadIdx.x;
     if (n >= f.n) return;
-    const int k0 = f.start[n], ns = f.start[n + 1] - k0;
+    const LV<T> v = view<T>(f);
+    const LV<T> cv = view<T>(c);
+    int k0, ns;
+    rowRange(f.start, f.uniformRow, n, k0, ns);
     if (B > 0 && ns > 0 && ns <= B) {
         int o[B > 0 ? B : 1], tc[B > 0 ? B : 1];
-        double a[B > 0 ? B : 1], xo[B > 0 ? B : 1], co[B > 0 ? B : 1];
+        T a[B > 0 ? B : 1], xo[B > 0 ? B : 1], co[B > 0 ? B : 1];
 #pragma unroll
         for (int q = 0; q < B; q++) {
             const int k = slotAt(k0, ns, q);
             o[q] = f.nb[k];
-            a[q] = f.AF[k];
+            a[q] = v.AF[k];
         }
         const int tn = f.toCoarse[n];
-        const double AC = f.AC[n], bn = f.b[n], xf = f.x[n];
+        const T AC = v.AC[n], bn = v.b[n], xf = v.x[n];
 #pragma unroll
         for (int q = 0; q < B; q++) {
             const int oc = o[q] >= 0 ? o[q] : n;
-            xo[q] = f.x[oc];
+            xo[q] = v.x[oc];
             tc[q] = f.toCoarse[oc];
         }
-        const double xc = c.x[tn];
+        const T xc = cv.x[tn];
 #pragma unroll
-        for (int q = 0; q < B; q++) co[q] = c.x[tc[q]];
-        const double xn = xf + xc;
-        if (fabs(AC) < 1.0e-30) {
-            f.xNew[n] = xn;
+        for (int q = 0; q < B; q++) co[q] = cv.x[tc[q]];
+        const T xn = xf + xc;
+        if (tabs(AC) < T(1.0e-30)) {
+            v.xNew[n] = xn;
             return;
         }
-        double Ax = AC * xn;
+        T Ax = AC * xn;
 #pragma unroll
         for (int q = 0; q < B; q++)
             if (q < ns && o[q] != -1) Ax += a[q] * (xo[q] + co[q]);
-        f.xNew[n] = xn + w * (bn - Ax) / AC;
+        v.xNew[n] = xn + w * (bn - Ax) / AC;
         return;
     }
-    const double xn = f.x[n] + c.x[f.toCoarse[n]];
-    const double AC = f.AC[n];
-    if (fabs(AC) < 1.0e-30) {
-        f.xNew[n] = xn;
+    const T xn = v.x[n] + cv.x[f.toCoarse[n]];
+    const T AC = v.AC[n];
+    if (tabs(AC) < T(1.0e-30)) {
+        v.xNew[n] = xn;
         return;
     }
-    double Ax = AC * xn;
+    T Ax = AC * xn;
     for (int k = k0; k < k0 + ns; k++) {
         const int nb = f.nb[k];
-        if (nb != -1) Ax += f.AF[k] * (f.x[nb] + c.x[f.toCoarse[nb]]);
+        if (nb != -1) Ax += v.AF[k] * (v.x[nb] + cv.x[f.toCoarse[nb]]);
     }
-    f.xNew[n] = xn + w * (f.b[n] - Ax) / AC;
+    v.xNew[n] = xn + w * (v.b[n] - Ax) / AC;
 }
 
 // Level-0 sweeps of the V-cycles fused per launch (temporal blocking, as in
@@ -1399,7 +1444,8 @@ struct MGRowReg {
 };
 
 __device__ __forceinline__ void loadMGRow(MGRowReg& r, const DLevel& L, int c) {
-    const int k0 = L.start[c], ns = L.start[c + 1] - k0;
+    int k0, ns;
+    rowRange(L.start, L.uniformRow, c, k0, ns);
 #pragma unroll
     for (int q = 0; q < kSweepRegSlots; q++) {
         r.nb[q] = (q < ns) ? L.nb[k0 + q] : -1;
@@ -1506,35 +1552,6 @@ __global__ void __launch_bounds__(kBlock) kMGProlong(DLevel f, DLevel c, const C
 // ===========================================================================
 // Restriction, FP64 and FP32 (the mixed-precision cycle) from one template.
 // ===========================================================================
-namespace {
-template <class T>
-struct LV {
-    const T *AC, *AF;
-    T *b, *x, *xNew, *res;
-};
-template <class T>
-__device__ __forceinline__ LV<T> view(const DLevel& L);
-template <>
-__device__ __forceinline__ LV<double> view<double>(const DLevel& L) {
-    return LV<double>{L.AC, L.AF, L.b, L.x, L.xNew, L.res};
-}
-template <>
-__device__ __forceinline__ LV<float> view<float>(const DLevel& L) {
-    return LV<float>{L.AC32, L.AF32, L.b32, L.x32, L.xNew32, L.res32};
-}
-template <class T>
-__device__ __forceinline__ T tabs(T v) {
-    return v < T(0) ? -v : v;
-}
-// jacobiRow (multigrid.cu:447-484) applied to x = 0: every product with x is a
-// (signed) zero, b - Ax == b, and 0 + w*b/AC is the reference's value bit for
-// bit (a zero result comes out +0 either way).
-template <class T>
-__device__ __forceinline__ T sweepFromZero(T b, T AC, T w) {
-    if (tabs(AC) < T(1.0e-30)) return T(0);
-    return T(0) + w * b / AC;
-}
-}  // namespace
 
 // buildRestrictionKernel (multigrid.cu:398-410) as a gather in ascending fine
 // order of the residuals kMGResidual left in f.res, then the coarse x memset
@@ -1916,12 +1933,36 @@ void launchMGCoarseDiag(hipStream_t s, const DLevel& f, const Ctl* ctl) {
 void launchMGCoarseGather(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl) {
     kMGCoarseGather<<<blocksFor((long)c.refs + c.n), kBlock, 0, s>>>(f, c, ctl);
 }
-void launchMGResidual(hipStream_t s, const DLevel& f, const Ctl* ctl) {
+// the row batch of a level: its longest row rounded up to 4, 8 or 16 (0: loop)
+template <class T>
+static void launchMGResidualT(hipStream_t s, const DLevel& f, const Ctl* ctl) {
     const dim3 g(blocksFor(f.n, f.block)), b(f.block);
-    if (f.maxRow <= 4) kMGResidual<4><<<g, b, 0, s>>>(f, ctl);
-    else if (f.maxRow <= 8) kMGResidual<8><<<g, b, 0, s>>>(f, ctl);
-    else if (f.maxRow <= 16) kMGResidual<16><<<g, b, 0, s>>>(f, ctl);
-    else kMGResidual<0><<<g, b, 0, s>>>(f, ctl);
+    if (f.maxRow <= 4) kMGResidual<T, 4><<<g, b, 0, s>>>(f, ctl);
+    else if (f.maxRow <= 8) kMGResidual<T, 8><<<g, b, 0, s>>>(f, ctl);
+    else if (f.maxRow <= 16) kMGResidual<T, 16><<<g, b, 0, s>>>(f, ctl);
+    else kMGResidual<T, 0><<<g, b, 0, s>>>(f, ctl);
+}
+template <class T>
+static void launchMGJacobiT(hipStream_t s, const DLevel& L, T w, const Ctl* ctl) {
+    const dim3 g(blocksFor(L.n, L.block)), b(L.block);
+    if (L.maxRow <= 4) kMGJacobi<T, 4><<<g, b, 0, s>>>(L, w, ctl);
+    else if (L.maxRow <= 8) kMGJacobi<T, 8><<<g, b, 0, s>>>(L, w, ctl);
+    else if (L.maxRow <= 16) kMGJacobi<T, 16><<<g, b, 0, s>>>(L, w, ctl);
+    else kMGJacobi<T, 0><<<g, b, 0, s>>>(L, w, ctl);
+}
+template <class T>
+static void launchMGProlongSmoothT(hipStream_t s, const DLevel& f, const DLevel& c, T w, const Ctl* ctl) {
+    const dim3 g(blocksFor(f.n, f.block)), b(f.block);
+    if (f.maxRow <= 4) kMGProlongSmooth<T, 4><<<g, b, 0, s>>>(f, c, w, ctl);
+    else if (f.maxRow <= 8) kMGProlongSmooth<T, 8><<<g, b, 0, s>>>(f, c, w, ctl);
+    else if (f.maxRow <= 16) kMGProlongSmooth<T, 16><<<g, b, 0, s>>>(f, c, w, ctl);
+    else kMGProlongSmooth<T, 0><<<g, b, 0, s>>>(f, c, w, ctl);
+}
+void launchMGResidual(hipStream_t s, const DLevel& f, const Ctl* ctl) { launchMGResidualT<double>(s, f, ctl); }
+void launchMGResidual32(hipStream_t s, const DLevel& f, const Ctl* ctl) { launchMGResidualT<float>(s, f, ctl); }
+void launchMGJacobi32(hipStream_t s, const DLevel& L, float w, const Ctl* ctl) { launchMGJacobiT<float>(s, L, w, ctl); }
+void launchMGProlongSmooth32(hipStream_t s, const DLevel& f, const DLevel& c, float w, const Ctl* ctl) {
+    launchMGProlongSmoothT<float>(s, f, c, w, ctl);
 }
 void launchMGRestrictSum(hipStream_t s, const DLevel& f, const DLevel& c, double w, int sweep1, bool fp32,
                          const Ctl* ctl) {
@@ -1958,11 +1999,13 @@ void launchMomentum(hipStream_t s, const DMesh& m, DBC bu, DBC bv, DBC bp, DStat
 }
 void launchJacobiUV(hipStream_t s, const DMesh& m, DState st, const double* uo, const double* vo, double* un,
                     double* vn) {
-    kJacobiUV<<<blocksFor(m.N, m.sweepBlock), m.sweepBlock, 0, s>>>(m, st, UVIn{uo, vo, nullptr}, UVOut{un, vn, nullptr});
+    launchJacobiUVPacked(s, m, st, uo, vo, nullptr, un, vn, nullptr);
 }
 void launchJacobiUVPacked(hipStream_t s, const DMesh& m, DState st, const double* uo, const double* vo,
                           const double2* uvo, double* un, double* vn, double2* uvn) {
-    kJacobiUV<<<blocksFor(m.N, m.sweepBlock), m.sweepBlock, 0, s>>>(m, st, UVIn{uo, vo, uvo}, UVOut{un, vn, uvn});
+    const dim3 g(blocksFor(m.N, m.sweepBlock)), b(m.sweepBlock);
+    if (m.maxSlots <= 3) kJacobiUV<3><<<g, b, 0, s>>>(m, st, UVIn{uo, vo, uvo}, UVOut{un, vn, uvn});
+    else kJacobiUV<4><<<g, b, 0, s>>>(m, st, UVIn{uo, vo, uvo}, UVOut{un, vn, uvn});
 }
 int sweepHaloMax(int cells) {
     if (cells == 512) return sweepHalo<512>();
@@ -2034,23 +2077,13 @@ void launchCopy(hipStream_t s, double* dst, const double* src, int n, const Ctl*
 void launchFill(hipStream_t s, double* dst, double value, int n, const Ctl* ctl) {
     kFill<<<blocksFor(n), kBlock, 0, s>>>(dst, value, n, ctl);
 }
-void launchMGJacobi(hipStream_t s, const DLevel& L, double w, const Ctl* ctl) {
-    const dim3 g(blocksFor(L.n, L.block)), b(L.block);
-    if (L.maxRow <= 4) kMGJacobi<4><<<g, b, 0, s>>>(L, w, ctl);
-    else if (L.maxRow <= 8) kMGJacobi<8><<<g, b, 0, s>>>(L, w, ctl);
-    else if (L.maxRow <= 16) kMGJacobi<16><<<g, b, 0, s>>>(L, w, ctl);
-    else kMGJacobi<0><<<g, b, 0, s>>>(L, w, ctl);
-}
+void launchMGJacobi(hipStream_t s, const DLevel& L, double w, const Ctl* ctl) { launchMGJacobiT<double>(s, L, w, ctl); }
 void launchMGSmoothBlock(hipStream_t s, const DLevel& L, double w, int sweeps, const Ctl* ctl) {
     const int threads = L.n >= 1024 ? 1024 : ((L.n + 63) / 64) * 64;
     kMGSmoothBlock<<<1, threads, 2 * (size_t)L.n * sizeof(double), s>>>(L, w, sweeps, ctl);
 }
 void launchMGProlongSmooth(hipStream_t s, const DLevel& f, const DLevel& c, double w, const Ctl* ctl) {
-    const dim3 g(blocksFor(f.n, f.block)), b(f.block);
-    if (f.maxRow <= 4) kMGProlongSmooth<4><<<g, b, 0, s>>>(f, c, w, ctl);
-    else if (f.maxRow <= 8) kMGProlongSmooth<8><<<g, b, 0, s>>>(f, c, w, ctl);
-    else if (f.maxRow <= 16) kMGProlongSmooth<16><<<g, b, 0, s>>>(f, c, w, ctl);
-    else kMGProlongSmooth<0><<<g, b, 0, s>>>(f, c, w, ctl);
+    launchMGProlongSmoothT<double>(s, f, c, w, ctl);
 }
 void launchMGLevel0Block(hipStream_t s, const DLevel& f, const DLevel& c, const DSweepBlocks& sb, int sweeps,
                          bool prolong, double w, const Ctl* ctl) {

@@ -19,30 +19,56 @@
 
 #include "device.hpp"
 #include "mixed.hpp"
+#include "rows.hpp"
 
 namespace axs {
-
-namespace {
-__device__ __forceinline__ bool finished(const Ctl* ctl) { return ctl != nullptr && ctl->done != 0; }
-}  // namespace
 
 // ---------------------------------------------------------------------------
 // momentum
 // ---------------------------------------------------------------------------
 
+// The row walks below batch a cell's slots as the FP64 kernels do (rows.hpp):
+// B = 3 or 4 slots loaded in one round, then the neighbours in one round.
+
 // r0 = b - A u0 for u and v (FP64), stored FP32, and the first sweep from zero:
 // delta_1 = invAC * r0.
+template <int B>
 __global__ void __launch_bounds__(kBlock) kMomDefect(DMesh m, DState s, MixedState x) {
     if (finished(s.ctl)) return;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = xcdBlock() * blockDim.x + threadIdx.x;
     if (n >= m.N) return;
     double au = s.uAC[n] * s.u[n], av = s.vAC[n] * s.v[n];
-    for (int k = m.faceStart[n]; k < m.faceStart[n + 1]; k++) {
-        const int nb = m.slotNb[k];
-        if (nb >= 0) {
-            const double a = s.AFuv[k];
-            au += a * s.u[nb];
-            av += a * s.v[nb];
+    int k0, ns;
+    rowRange(m.faceStart, m.uniformSlots, n, k0, ns);
+    if (ns > 0 && ns <= B) {
+        int o[B];
+        double a[B], uo[B], vo[B];
+#pragma unroll
+        for (int q = 0; q < B; q++) {
+            const int k = slotAt(k0, ns, q);
+            o[q] = m.slotNb[k];
+            a[q] = s.AFuv[k];
+        }
+#pragma unroll
+        for (int q = 0; q < B; q++) {
+            const int oc = o[q] >= 0 ? o[q] : n;
+            uo[q] = s.u[oc];
+            vo[q] = s.v[oc];
+        }
+#pragma unroll
+        for (int q = 0; q < B; q++)
+            if (q < ns && o[q] >= 0) {
+                au += a[q] * uo[q];
+                av += a[q] * vo[q];
+            }
+    } else {
+        for (int k = k0; k < k0 + ns; k++) {
+            const int nb = m.slotNb[k];
+            if (nb >= 0) {
+                const double a = s.AFuv[k];
+                au += a * s.u[nb];
+                av += a * s.v[nb];
+            }
         }
     }
     const float ru = (float)(s.ub[n] - au), rv = (float)(s.vb[n] - av);
@@ -52,17 +78,40 @@ __global__ void __launch_bounds__(kBlock) kMomDefect(DMesh m, DState s, MixedSta
 }
 
 // one FP32 Jacobi sweep of the (u, v) defect equations
+template <int B>
 __device__ __forceinline__ float2 momSweep32(const DMesh& m, const MixedState& x, const float2* __restrict__ dO,
                                              int n) {
     const float2 r = x.r[n];
     float au = r.x, av = r.y;
-    for (int k = m.faceStart[n]; k < m.faceStart[n + 1]; k++) {
-        const int nb = m.slotNb[k];
-        if (nb >= 0) {
-            const float a = x.AF32[k];
-            const float2 d = dO[nb];
-            au -= a * d.x;
-            av -= a * d.y;
+    int k0, ns;
+    rowRange(m.faceStart, m.uniformSlots, n, k0, ns);
+    if (ns > 0 && ns <= B) {
+        int o[B];
+        float a[B];
+        float2 dv[B];
+#pragma unroll
+        for (int q = 0; q < B; q++) {
+            const int k = slotAt(k0, ns, q);
+            o[q] = m.slotNb[k];
+            a[q] = x.AF32[k];
+        }
+#pragma unroll
+        for (int q = 0; q < B; q++) dv[q] = dO[o[q] >= 0 ? o[q] : n];
+#pragma unroll
+        for (int q = 0; q < B; q++)
+            if (q < ns && o[q] >= 0) {
+                au -= a[q] * dv[q].x;
+                av -= a[q] * dv[q].y;
+            }
+    } else {
+        for (int k = k0; k < k0 + ns; k++) {
+            const int nb = m.slotNb[k];
+            if (nb >= 0) {
+                const float a = x.AF32[k];
+                const float2 d = dO[nb];
+                au -= a * d.x;
+                av -= a * d.y;
+            }
         }
     }
     const float2 iv = x.inv[n];
@@ -70,21 +119,23 @@ __device__ __forceinline__ float2 momSweep32(const DMesh& m, const MixedState& x
     return make_float2((iv.x != 0.0f) ? au * iv.x : d.x, (iv.y != 0.0f) ? av * iv.y : d.y);
 }
 
+template <int B>
 __global__ void __launch_bounds__(kBlock) kMomSweep32(DMesh m, const Ctl* ctl, MixedState x,
                                                      const float2* __restrict__ dO, float2* __restrict__ dN) {
     if (finished(ctl)) return;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = xcdBlock() * blockDim.x + threadIdx.x;
     if (n >= m.N) return;
-    dN[n] = momSweep32(m, x, dO, n);
+    dN[n] = momSweep32<B>(m, x, dO, n);
 }
 
 // last FP32 sweep fused with u += delta, v += delta (FP64)
+template <int B>
 __global__ void __launch_bounds__(kBlock) kMomSweepApply(DMesh m, DState s, MixedState x,
                                                         const float2* __restrict__ dO, int sweep) {
     if (finished(s.ctl)) return;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = xcdBlock() * blockDim.x + threadIdx.x;
     if (n >= m.N) return;
-    const float2 d = sweep ? momSweep32(m, x, dO, n) : dO[n];
+    const float2 d = sweep ? momSweep32<B>(m, x, dO, n) : dO[n];
     s.u[n] += (double)d.x;
     s.v[n] += (double)d.y;
 }
@@ -95,14 +146,33 @@ __global__ void __launch_bounds__(kBlock) kMomSweepApply(DMesh m, DState s, Mixe
 
 // Level-0 defect r = b - A x (FP64) -> FP32 right-hand side of the correction
 // cycle; with pre >= 1 also the first weighted-Jacobi sweep from e = 0.
+template <int B>
 __global__ void __launch_bounds__(kBlock) kMGDefect(DLevel L, double w, int firstSweep, const Ctl* ctl) {
     if (finished(ctl)) return;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = xcdBlock() * blockDim.x + threadIdx.x;
     if (n >= L.n) return;
     double Ax = L.AC[n] * L.x[n];
-    for (int k = L.start[n]; k < L.start[n + 1]; k++) {
-        const int nb = L.nb[k];
-        if (nb >= 0) Ax += L.AF[k] * L.x[nb];
+    int k0, ns;
+    rowRange(L.start, L.uniformRow, n, k0, ns);
+    if (ns > 0 && ns <= B) {
+        int o[B];
+        double a[B], xo[B];
+#pragma unroll
+        for (int q = 0; q < B; q++) {
+            const int k = slotAt(k0, ns, q);
+            o[q] = L.nb[k];
+            a[q] = L.AF[k];
+        }
+#pragma unroll
+        for (int q = 0; q < B; q++) xo[q] = L.x[o[q] >= 0 ? o[q] : n];
+#pragma unroll
+        for (int q = 0; q < B; q++)
+            if (q < ns && o[q] >= 0) Ax += a[q] * xo[q];
+    } else {
+        for (int k = k0; k < k0 + ns; k++) {
+            const int nb = L.nb[k];
+            if (nb >= 0) Ax += L.AF[k] * L.x[nb];
+        }
     }
     const float r = (float)(L.b[n] - Ax);
     L.b32[n] = r;
@@ -112,24 +182,6 @@ __global__ void __launch_bounds__(kBlock) kMGDefect(DLevel L, double w, int firs
     } else {
         L.x32[n] = 0.0f;
     }
-}
-
-__device__ __forceinline__ float jacobiRow32(const DLevel& L, const float* x, int n, float w) {
-    const float AC = L.AC32[n];
-    if (fabsf(AC) < 1.0e-30f) return x[n];
-    float Ax = AC * x[n];
-    for (int k = L.start[n]; k < L.start[n + 1]; k++) {
-        const int nb = L.nb[k];
-        if (nb != -1) Ax += L.AF32[k] * x[nb];
-    }
-    return x[n] + w * (L.b32[n] - Ax) / AC;
-}
-
-__global__ void __launch_bounds__(kBlock) kMGJacobi32(DLevel L, float w, const Ctl* ctl) {
-    if (finished(ctl)) return;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= L.n) return;
-    L.xNew32[n] = jacobiRow32(L, L.x32, n, w);
 }
 
 __global__ void __launch_bounds__(1024) kMGSmoothBlock32(DLevel L, float w, int sweeps, const Ctl* ctl) {
@@ -163,39 +215,9 @@ __global__ void __launch_bounds__(1024) kMGSmoothBlock32(DLevel L, float w, int 
     for (int n = threadIdx.x; n < L.n; n += blockDim.x) L.x32[n] = xa[n];
 }
 
-__global__ void __launch_bounds__(kBlock) kMGResidual32(DLevel f, const Ctl* ctl) {
-    if (finished(ctl)) return;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= f.n) return;
-    float Ax = f.AC32[n] * f.x32[n];
-    for (int k = f.start[n]; k < f.start[n + 1]; k++) {
-        const int nb = f.nb[k];
-        if (nb >= 0) Ax += f.AF32[k] * f.x32[nb];
-    }
-    f.res32[n] = f.b32[n] - Ax;
-}
-
-__global__ void __launch_bounds__(kBlock) kMGProlongSmooth32(DLevel f, DLevel c, float w, const Ctl* ctl) {
-    if (finished(ctl)) return;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= f.n) return;
-    const float xn = f.x32[n] + c.x32[f.toCoarse[n]];
-    const float AC = f.AC32[n];
-    if (fabsf(AC) < 1.0e-30f) {
-        f.xNew32[n] = xn;
-        return;
-    }
-    float Ax = AC * xn;
-    for (int k = f.start[n]; k < f.start[n + 1]; k++) {
-        const int nb = f.nb[k];
-        if (nb != -1) Ax += f.AF32[k] * (f.x32[nb] + c.x32[f.toCoarse[nb]]);
-    }
-    f.xNew32[n] = xn + w * (f.b32[n] - Ax) / AC;
-}
-
 __global__ void __launch_bounds__(kBlock) kMGProlong32(DLevel f, DLevel c, const Ctl* ctl) {
     if (finished(ctl)) return;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = xcdBlock() * blockDim.x + threadIdx.x;
     if (n >= f.n) return;
     f.x32[n] += c.x32[f.toCoarse[n]];
 }
@@ -203,7 +225,7 @@ __global__ void __launch_bounds__(kBlock) kMGProlong32(DLevel f, DLevel c, const
 // x (FP64) += e (FP32): the refinement step closing each correction cycle
 __global__ void __launch_bounds__(kBlock) kMGApply(DLevel L, const Ctl* ctl) {
     if (finished(ctl)) return;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = xcdBlock() * blockDim.x + threadIdx.x;
     if (n >= L.n) return;
     L.x[n] += (double)L.x32[n];
 }
@@ -212,7 +234,7 @@ __global__ void __launch_bounds__(kBlock) kMGApply(DLevel L, const Ctl* ctl) {
 // cast inside kMGCoarseOp64)
 __global__ void __launch_bounds__(kBlock) kMGCast(DLevel L, const Ctl* ctl) {
     if (finished(ctl)) return;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = xcdBlock() * blockDim.x + threadIdx.x;
     if (n < L.n) L.AC32[n] = (float)L.AC[n];
     for (int k = n; k < L.refs; k += gridDim.x * blockDim.x) L.AF32[k] = (float)L.AF[k];
 }
@@ -297,7 +319,7 @@ __global__ void __launch_bounds__(1024) kCoarseInvert(DLevel L, double* aug, dou
 // x_c = A_c^-1 b_c (FP64 product, FP32 in / out)
 __global__ void __launch_bounds__(kBlock) kCoarseApply(DLevel L, const double* inv, const Ctl* ctl) {
     if (finished(ctl)) return;
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = xcdBlock() * blockDim.x + threadIdx.x;
     if (r >= L.n) return;
     double acc = 0.0;
     for (int c = 0; c < L.n; c++) acc += inv[r * L.n + c] * (double)L.b32[c];
@@ -308,29 +330,25 @@ __global__ void __launch_bounds__(kBlock) kCoarseApply(DLevel L, const double* i
 // launchers
 // ---------------------------------------------------------------------------
 void launchMomDefect(hipStream_t s, const DMesh& m, DState st, MixedState x) {
-    kMomDefect<<<blocksFor(m.N), kBlock, 0, s>>>(m, st, x);
+    if (m.maxSlots <= 3) kMomDefect<3><<<blocksFor(m.N), kBlock, 0, s>>>(m, st, x);
+    else kMomDefect<4><<<blocksFor(m.N), kBlock, 0, s>>>(m, st, x);
 }
 void launchMomSweep32(hipStream_t s, const DMesh& m, const Ctl* ctl, MixedState x, const float2* dO, float2* dN) {
-    kMomSweep32<<<blocksFor(m.N), kBlock, 0, s>>>(m, ctl, x, dO, dN);
+    if (m.maxSlots <= 3) kMomSweep32<3><<<blocksFor(m.N), kBlock, 0, s>>>(m, ctl, x, dO, dN);
+    else kMomSweep32<4><<<blocksFor(m.N), kBlock, 0, s>>>(m, ctl, x, dO, dN);
 }
 void launchMomSweepApply(hipStream_t s, const DMesh& m, DState st, MixedState x, const float2* dO, int sweep) {
-    kMomSweepApply<<<blocksFor(m.N), kBlock, 0, s>>>(m, st, x, dO, sweep);
+    if (m.maxSlots <= 3) kMomSweepApply<3><<<blocksFor(m.N), kBlock, 0, s>>>(m, st, x, dO, sweep);
+    else kMomSweepApply<4><<<blocksFor(m.N), kBlock, 0, s>>>(m, st, x, dO, sweep);
 }
 void launchMGDefect(hipStream_t s, const DLevel& L, double w, int firstSweep, const Ctl* ctl) {
-    kMGDefect<<<blocksFor(L.n, L.block), L.block, 0, s>>>(L, w, firstSweep, ctl);
-}
-void launchMGJacobi32(hipStream_t s, const DLevel& L, float w, const Ctl* ctl) {
-    kMGJacobi32<<<blocksFor(L.n, L.block), L.block, 0, s>>>(L, w, ctl);
+    const dim3 g(blocksFor(L.n, L.block)), b(L.block);
+    if (L.maxRow <= 4) kMGDefect<4><<<g, b, 0, s>>>(L, w, firstSweep, ctl);
+    else kMGDefect<8><<<g, b, 0, s>>>(L, w, firstSweep, ctl);
 }
 void launchMGSmoothBlock32(hipStream_t s, const DLevel& L, float w, int sweeps, const Ctl* ctl) {
     const int threads = L.n >= 1024 ? 1024 : ((L.n + 63) / 64) * 64;
     kMGSmoothBlock32<<<1, threads, 2 * (size_t)L.n * sizeof(float), s>>>(L, w, sweeps, ctl);
-}
-void launchMGResidual32(hipStream_t s, const DLevel& f, const Ctl* ctl) {
-    kMGResidual32<<<blocksFor(f.n, f.block), f.block, 0, s>>>(f, ctl);
-}
-void launchMGProlongSmooth32(hipStream_t s, const DLevel& f, const DLevel& c, float w, const Ctl* ctl) {
-    kMGProlongSmooth32<<<blocksFor(f.n, f.block), f.block, 0, s>>>(f, c, w, ctl);
 }
 void launchMGProlong32(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl) {
     kMGProlong32<<<blocksFor(f.n, f.block), f.block, 0, s>>>(f, c, ctl);

@@ -53,7 +53,12 @@ void MGDevice::build(int n, const std::vector<int>& start, const std::vector<int
         DLevel& d = dev[l];
         d.n = h.n;
         d.refs = h.refs();
-        for (int i = 0; i < h.n; i++) d.maxRow = std::max(d.maxRow, h.start[i + 1] - h.start[i]);
+        int minRow = 1 << 30;
+        for (int i = 0; i < h.n; i++) {
+            d.maxRow = std::max(d.maxRow, h.start[i + 1] - h.start[i]);
+            minRow = std::min(minRow, h.start[i + 1] - h.start[i]);
+        }
+        d.uniformRow = (h.n > 0 && minRow == d.maxRow && h.start[0] == 0) ? d.maxRow : 0;
         if (!h.coarsest())
             for (size_t i = 0; i + 1 < h.memberStart.size(); i++)
                 d.maxMembers = std::max(d.maxMembers, h.memberStart[i + 1] - h.memberStart[i]);
@@ -721,7 +726,13 @@ void Solver::uploadMesh() {
     d.faceIDs = meshArena_.upload(m.cellFaceIDs);
     d.slotNb = meshArena_.upload(csrNb_);
     d.maxSlots = 0;
-    for (int c = 0; c < m.nCells; c++) d.maxSlots = std::max(d.maxSlots, m.cellFaceStart[c + 1] - m.cellFaceStart[c]);
+    int minSlots = 1 << 30;
+    for (int c = 0; c < m.nCells; c++) {
+        const int k = m.cellFaceStart[c + 1] - m.cellFaceStart[c];
+        d.maxSlots = std::max(d.maxSlots, k);
+        minSlots = std::min(minSlots, k);
+    }
+    d.uniformSlots = (minSlots == d.maxSlots && m.cellFaceStart[0] == 0) ? d.maxSlots : 0;
     d.owner = meshArena_.upload(m.faceOwner);
     d.neighbor = meshArena_.upload(m.faceNeighbor);
     d.group = meshArena_.upload(m.faceGroup);
