@@ -33,7 +33,8 @@ struct DSweepBlocks {
     const int* haloStart = nullptr;
     const int* ringEnd = nullptr;
     const int* halo = nullptr;
-    const int2* sorted = nullptr;
+    const int16_t* locOwn = nullptr;   // LDS slot of each slot's neighbour (block rows)
+    const int16_t* locHalo = nullptr;  // the same for halo rows, kSweepRegSlots per entry
 };
 constexpr int kSweepRegSlots = 4;   // slots of a block cell kept in registers across the sweeps
 

@@ -667,22 +667,28 @@ __global__ void __launch_bounds__(kBlock) kJacobiUV(DMesh m, DState s, UVIn xo, 
 // launch: block cell bC + t and halo entry t (a halo has at most C / 2 cells),
 // each as b, 1/AC and up to kSweepRegSlots off-diagonals with their
 // neighbours, all loaded up front, so the CSR and coefficients are read once
-// per S sweeps and no sweep waits on a chain of global loads.  A neighbour's
-// LDS slot is found once (block cells directly, halo cells by binary search in
-// the block's sorted halo).  Every row is the same expression as in kJacobiUV,
-// so the result is bit-identical to S launches of it.
+// per S sweeps and no sweep waits on a chain of global loads.  Each slot's
+// neighbour LDS slot comes from the host-built tables (int16, loaded with the
+// row).  Every row is the same expression as in kJacobiUV, so the result is
+// bit-identical to S launches of it.
+// R: register slots per row (3 on triangle meshes, else kSweepRegSlots)
+template <int R>
 struct SweepRow {
-    double af[kSweepRegSlots];
-    int nb[kSweepRegSlots];   // neighbour cell, -1: none (boundary or past the row's slots)
-    int loc[kSweepRegSlots];  // the neighbour's LDS slot
+    double af[R];
+    int nb[R];   // neighbour cell, -1: none (boundary or past the row's slots)
+    int loc[R];  // the neighbour's LDS slot, -1: none (loaded after the first sweep)
+    int k0, ns;  // the row's slots
     double bu, bv, iu, iv;
 };
 
-__device__ __forceinline__ void loadSweepRow(SweepRow& r, const DMesh& m, const DState& s, int c) {
+template <int R>
+__device__ __forceinline__ void loadSweepRow(SweepRow<R>& r, const DMesh& m, const DState& s, int c) {
     int k0, ns;
     rowRange(m.faceStart, m.uniformSlots, c, k0, ns);
+    r.k0 = k0;
+    r.ns = ns;
 #pragma unroll
-    for (int q = 0; q < kSweepRegSlots; q++) {
+    for (int q = 0; q < R; q++) {
         r.nb[q] = (q < ns) ? m.slotNb[k0 + q] : -1;
         r.af[q] = (q < ns) ? s.AFuv[k0 + q] : 0.0;
     }
@@ -692,13 +698,20 @@ __device__ __forceinline__ void loadSweepRow(SweepRow& r, const DMesh& m, const 
     r.iv = s.vInv[c];
 }
 
-// jacobi's row (kJacobiUV) from registers; xat(q) = the neighbour of slot q
-template <typename XAt>
-__device__ __forceinline__ double2 sweepRow(const SweepRow& r, XAt xat, double2 self) {
+// the neighbours' LDS slots of a row from its table entries l[0 .. ns)
+template <class Row>
+__device__ __forceinline__ void loadLoc(Row& r, const int16_t* l, int R) {
+    for (int q = 0; q < R; q++) r.loc[q] = (q < r.ns) ? (int)l[q] : -1;
+}
+
+// jacobi's row (kJacobiUV) from registers; xat(q) = the neighbour of slot q;
+// LDS: the neighbours are the ones with an LDS slot (the same ones)
+template <int R, bool LDS = false, typename XAt>
+__device__ __forceinline__ double2 sweepRow(const SweepRow<R>& r, XAt xat, double2 self) {
     double au = r.bu, av = r.bv;
 #pragma unroll
-    for (int q = 0; q < kSweepRegSlots; q++) {
-        if (r.nb[q] >= 0) {
+    for (int q = 0; q < R; q++) {
+        if (LDS ? r.loc[q] >= 0 : r.nb[q] >= 0) {
             const double2 x = xat(q);
             au -= r.af[q] * x.x;
             av -= r.af[q] * x.y;
@@ -708,61 +721,45 @@ __device__ __forceinline__ double2 sweepRow(const SweepRow& r, XAt xat, double2 
 }
 
 template <int C>
-__device__ __forceinline__ int sweepLocal(int c, int a, int nOwn, const int2* keys, int nh) {
-    if (c >= a && c < a + nOwn) return c - a;
-    int lo = 0, hi = nh;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (keys[mid].x < c) lo = mid + 1;
-        else hi = mid;
-    }
-    return (lo < nh) ? keys[lo].y : 0;  // always found for the cells a sweep reads
-}
-
-template <int C>
 constexpr int sweepHalo() { return C / 2; }
 
-template <int C, int S>
+template <int C, int S, int R>
 __global__ void __launch_bounds__(C) kJacobiUVBlock(DMesh m, DState s, DSweepBlocks sb, UVIn xo, UVOut xn) {
     constexpr int H = sweepHalo<C>();
     constexpr int kMask = S > 2 ? 1 : 0;  // sweep j < S writes LDS buffer (j - 1) & kMask
     __shared__ double2 buf[kMask + 1][C + H];
-    __shared__ int2 keys[H];
     if (finished(s.ctl)) return;
     const int t = threadIdx.x, blk = xcdBlock();
     const int a = blk * C, nOwn = min(C, m.N - a);
-    const int h0 = sb.haloStart[blk], nh = sb.haloStart[blk + 1] - h0;
-    // halo entries of rings 1 .. S - 1: the ones sweep 1 updates
+    const int h0 = sb.haloStart[blk];
+    // halo entries of rings 1 .. S - 1: the ones sweep 1 updates; of rings
+    // 1 .. S - 2: the ones sweeps 2 .. S - 1 update again, from LDS
     const int nHalo1 = (S > 1) ? sb.ringEnd[blk * sb.rings + S - 2] : 0;
+    const int nHaloLds = (S > 2) ? sb.ringEnd[blk * sb.rings + (S > 2 ? S - 3 : 0)] : 0;
     const bool mine = t < nOwn, haloRow = t < nHalo1;
     const int n = a + t;
     const int hc = haloRow ? sb.halo[h0 + t] : 0;
-    SweepRow own, hal;
+    SweepRow<R> own, hal;
     if (mine) loadSweepRow(own, m, s, n);
     if (haloRow) loadSweepRow(hal, m, s, hc);
-    for (int i = t; i < nh; i += C) keys[i] = sb.sorted[h0 + i];
     // sweep 1: global iterate in, LDS out
     double2* out = buf[0];
     if (mine) out[t] = sweepRow(own, [&](int q) { return xo.at(own.nb[q]); }, xo.at(n));
     if (haloRow) out[C + t] = sweepRow(hal, [&](int q) { return xo.at(hal.nb[q]); }, xo.at(hc));
+    // the LDS slots the later sweeps read, loaded while the barrier waits
+    if (mine) loadLoc(own, sb.locOwn + own.k0, R);
+    if (t < nHaloLds) loadLoc(hal, sb.locHalo + (size_t)(h0 + t) * kSweepRegSlots, R);
     __syncthreads();
-    // halo entries updated again by sweeps 2 .. S - 1 (rings 1 .. S - 2)
-    const bool haloLds = (S > 2) && t < sb.ringEnd[blk * sb.rings + (S > 2 ? S - 3 : 0)];
-#pragma unroll
-    for (int q = 0; q < kSweepRegSlots; q++) {
-        if (mine) own.loc[q] = (own.nb[q] >= 0) ? sweepLocal<C>(own.nb[q], a, nOwn, keys, nh) : 0;
-        if (haloLds) hal.loc[q] = (hal.nb[q] >= 0) ? sweepLocal<C>(hal.nb[q], a, nOwn, keys, nh) : 0;
-    }
 #pragma unroll
     for (int j = 2; j <= S; j++) {
         const double2* in = buf[(j - 2) & kMask];
         if (j == S) {
-            if (mine) xn.put(n, sweepRow(own, [&](int q) { return in[own.loc[q]]; }, in[t]));
+            if (mine) xn.put(n, sweepRow<R, true>(own, [&](int q) { return in[own.loc[q]]; }, in[t]));
         } else {
             double2* o = buf[(j - 1) & kMask];
             const int nHaloJ = sb.ringEnd[blk * sb.rings + (S - j) - 1];
-            if (mine) o[t] = sweepRow(own, [&](int q) { return in[own.loc[q]]; }, in[t]);
-            if (t < nHaloJ) o[C + t] = sweepRow(hal, [&](int q) { return in[hal.loc[q]]; }, in[C + t]);
+            if (mine) o[t] = sweepRow<R, true>(own, [&](int q) { return in[own.loc[q]]; }, in[t]);
+            if (t < nHaloJ) o[C + t] = sweepRow<R, true>(hal, [&](int q) { return in[hal.loc[q]]; }, in[C + t]);
             __syncthreads();
         }
     }
@@ -1436,18 +1433,23 @@ __global__ void __launch_bounds__(kBlock) kMGProlongSmooth(DLevel f, DLevel c, T
 // The final x goes to f.xNew.  One launch replaces the post-sweep of cycle c,
 // the pre-sweep of cycle c + 1 and its residual, each row evaluated as in the
 // one-pass kernels, so the result is bit-identical.
+template <int R>
 struct MGRowReg {
-    double af[kSweepRegSlots];
-    int nb[kSweepRegSlots];
-    int loc[kSweepRegSlots];
+    double af[R];
+    int nb[R];
+    int loc[R];  // LDS slots as in SweepRow
+    int k0, ns;
     double ac, b;
 };
 
-__device__ __forceinline__ void loadMGRow(MGRowReg& r, const DLevel& L, int c) {
+template <int R>
+__device__ __forceinline__ void loadMGRow(MGRowReg<R>& r, const DLevel& L, int c) {
     int k0, ns;
     rowRange(L.start, L.uniformRow, c, k0, ns);
+    r.k0 = k0;
+    r.ns = ns;
 #pragma unroll
-    for (int q = 0; q < kSweepRegSlots; q++) {
+    for (int q = 0; q < R; q++) {
         r.nb[q] = (q < ns) ? L.nb[k0 + q] : -1;
         r.af[q] = (q < ns) ? L.AF[k0 + q] : 0.0;
     }
@@ -1455,39 +1457,39 @@ __device__ __forceinline__ void loadMGRow(MGRowReg& r, const DLevel& L, int c) {
     r.b = L.b[c];
 }
 
-template <typename XAt>
-__device__ __forceinline__ double mgJacobiReg(const MGRowReg& r, XAt xat, double xn, double w) {
+template <int R, bool LDS = false, typename XAt>
+__device__ __forceinline__ double mgJacobiReg(const MGRowReg<R>& r, XAt xat, double xn, double w) {
     if (fabs(r.ac) < 1.0e-30) return xn;
-    double xv[kSweepRegSlots];
+    double xv[R];
 #pragma unroll
-    for (int q = 0; q < kSweepRegSlots; q++) xv[q] = (r.nb[q] != -1) ? xat(q) : 0.0;
+    for (int q = 0; q < R; q++) xv[q] = (LDS ? r.loc[q] >= 0 : r.nb[q] != -1) ? xat(q) : 0.0;
     double Ax = r.ac * xn;
 #pragma unroll
-    for (int q = 0; q < kSweepRegSlots; q++)
-        if (r.nb[q] != -1) Ax += r.af[q] * xv[q];
+    for (int q = 0; q < R; q++)
+        if (LDS ? r.loc[q] >= 0 : r.nb[q] != -1) Ax += r.af[q] * xv[q];
     return xn + w * (r.b - Ax) / r.ac;
 }
 
-template <int C, int S, bool P, bool R>
+template <int C, int S, bool P, bool RES, int R>
 __global__ void __launch_bounds__(C) kMGLevel0Block(DLevel f, DLevel c, DSweepBlocks sb, double w, const Ctl* ctl) {
     constexpr int H = sweepHalo<C>();
-    constexpr int kSets = (S - 1 + (R ? 1 : 0)) > 1 ? 2 : 1;  // LDS iterate buffers in use
+    constexpr int kSets = (S - 1 + (RES ? 1 : 0)) > 1 ? 2 : 1;  // LDS iterate buffers in use
     __shared__ double buf[kSets][C + H];
-    __shared__ int2 keys[H];
     if (finished(ctl)) return;
     const int t = threadIdx.x, blk = xcdBlock();
     const int a = blk * C, nOwn = min(C, f.n - a);
-    const int h0 = sb.haloStart[blk], nh = sb.haloStart[blk + 1] - h0;
-    // halo entries the first sweep updates: rings 1 .. S - 1 + R
-    constexpr int kRings1 = S - 1 + (R ? 1 : 0);
+    const int h0 = sb.haloStart[blk];
+    // halo entries the first sweep updates: rings 1 .. S - 1 + RES; of those,
+    // rings 1 .. S - 2 + RES are read from LDS by the later sweeps
+    constexpr int kRings1 = S - 1 + (RES ? 1 : 0);
     const int nHalo1 = kRings1 > 0 ? sb.ringEnd[blk * sb.rings + kRings1 - 1] : 0;
+    const int nHaloLds = (kRings1 > 1) ? sb.ringEnd[blk * sb.rings + (kRings1 > 1 ? kRings1 - 2 : 0)] : 0;
     const bool mine = t < nOwn, haloRow = t < nHalo1;
     const int n = a + t;
     const int hc = haloRow ? sb.halo[h0 + t] : 0;
-    MGRowReg own, hal;
+    MGRowReg<R> own, hal;
     if (mine) loadMGRow(own, f, n);
     if (haloRow) loadMGRow(hal, f, hc);
-    for (int i = t; i < nh; i += C) keys[i] = sb.sorted[h0 + i];
     // the first sweep's input at any cell: x, plus the prolonged correction
     auto x0 = [&](int cell) -> double { return P ? f.x[cell] + c.x[f.toCoarse[cell]] : f.x[cell]; };
     // sweep j (1-based) writes LDS buffer (j - 1) % kSets; the last one without
@@ -1496,7 +1498,7 @@ __global__ void __launch_bounds__(C) kMGLevel0Block(DLevel f, DLevel c, DSweepBl
     {
         const double v = mine ? mgJacobiReg(own, [&](int q) { return x0(own.nb[q]); }, x0(n), w) : 0.0;
         const double vh = haloRow ? mgJacobiReg(hal, [&](int q) { return x0(hal.nb[q]); }, x0(hc), w) : 0.0;
-        if (S == 1 && !R) {
+        if (S == 1 && !RES) {
             if (mine) f.xNew[n] = v;
             return;
         }
@@ -1504,21 +1506,20 @@ __global__ void __launch_bounds__(C) kMGLevel0Block(DLevel f, DLevel c, DSweepBl
         if (haloRow) buf[0][C + t] = vh;
         lastOwn = v;
     }
+    // the LDS slots the later sweeps and the residual read, loaded while the
+    // barrier waits
+    if (mine) loadLoc(own, sb.locOwn + own.k0, R);
+    if (t < nHaloLds) loadLoc(hal, sb.locHalo + (size_t)(h0 + t) * kSweepRegSlots, R);
     __syncthreads();
-    const int nHaloLds = (kRings1 > 1) ? sb.ringEnd[blk * sb.rings + kRings1 - 2] : 0;
-#pragma unroll
-    for (int q = 0; q < kSweepRegSlots; q++) {
-        if (mine) own.loc[q] = (own.nb[q] >= 0) ? sweepLocal<C>(own.nb[q], a, nOwn, keys, nh) : 0;
-        if (t < nHaloLds) hal.loc[q] = (hal.nb[q] >= 0) ? sweepLocal<C>(hal.nb[q], a, nOwn, keys, nh) : 0;
-    }
 #pragma unroll
     for (int j = 2; j <= S; j++) {
         const double* in = buf[(j - 2) % kSets];
         double* out = buf[(j - 1) % kSets];
-        const double v = mine ? mgJacobiReg(own, [&](int q) { return in[own.loc[q]]; }, in[t], w) : 0.0;
-        const int nHaloJ = (S - j + (R ? 1 : 0)) > 0 ? sb.ringEnd[blk * sb.rings + (S - j + (R ? 1 : 0)) - 1] : 0;
-        const double vh = (t < nHaloJ) ? mgJacobiReg(hal, [&](int q) { return in[hal.loc[q]]; }, in[C + t], w) : 0.0;
-        if (j == S && !R) {
+        const double v = mine ? mgJacobiReg<R, true>(own, [&](int q) { return in[own.loc[q]]; }, in[t], w) : 0.0;
+        const int nHaloJ = (S - j + (RES ? 1 : 0)) > 0 ? sb.ringEnd[blk * sb.rings + (S - j + (RES ? 1 : 0)) - 1] : 0;
+        const double vh =
+            (t < nHaloJ) ? mgJacobiReg<R, true>(hal, [&](int q) { return in[hal.loc[q]]; }, in[C + t], w) : 0.0;
+        if (j == S && !RES) {
             if (mine) f.xNew[n] = v;
             return;
         }
@@ -1528,16 +1529,16 @@ __global__ void __launch_bounds__(C) kMGLevel0Block(DLevel f, DLevel c, DSweepBl
         __syncthreads();
     }
     // residual (kMGResidual) from the last sweep's iterate
-    if (R && mine) {
+    if (RES && mine) {
         const double* xs = buf[(S - 1) % kSets];
         f.xNew[n] = lastOwn;
-        double xv[kSweepRegSlots];
+        double xv[R];
 #pragma unroll
-        for (int q = 0; q < kSweepRegSlots; q++) xv[q] = (own.nb[q] >= 0) ? xs[own.loc[q]] : 0.0;
+        for (int q = 0; q < R; q++) xv[q] = (own.loc[q] >= 0) ? xs[own.loc[q]] : 0.0;
         double Ax = own.ac * lastOwn;
 #pragma unroll
-        for (int q = 0; q < kSweepRegSlots; q++)
-            if (own.nb[q] >= 0) Ax += own.af[q] * xv[q];
+        for (int q = 0; q < R; q++)
+            if (own.loc[q] >= 0) Ax += own.af[q] * xv[q];
         f.res[n] = own.b - Ax;
     }
 }
@@ -2018,12 +2019,23 @@ void launchJacobiUVBlock(hipStream_t s, const DMesh& m, DState st, const DSweepB
     if (sweeps < 2 || sweeps > 3 || sweeps > sb.rings + 1) throw std::invalid_argument("bad fused sweep count");
     const UVIn in{uo, vo, uvo};
     const UVOut out{un, vn, uvn};
+    const bool tri = m.maxSlots <= 3;
     if (sb.cells == 512) {
-        if (sweeps == 2) kJacobiUVBlock<512, 2><<<sb.nBlocks, 512, 0, s>>>(m, st, sb, in, out);
-        else kJacobiUVBlock<512, 3><<<sb.nBlocks, 512, 0, s>>>(m, st, sb, in, out);
+        if (sweeps == 2) {
+            if (tri) kJacobiUVBlock<512, 2, 3><<<sb.nBlocks, 512, 0, s>>>(m, st, sb, in, out);
+            else kJacobiUVBlock<512, 2, kSweepRegSlots><<<sb.nBlocks, 512, 0, s>>>(m, st, sb, in, out);
+        } else {
+            if (tri) kJacobiUVBlock<512, 3, 3><<<sb.nBlocks, 512, 0, s>>>(m, st, sb, in, out);
+            else kJacobiUVBlock<512, 3, kSweepRegSlots><<<sb.nBlocks, 512, 0, s>>>(m, st, sb, in, out);
+        }
     } else if (sb.cells == 1024) {
-        if (sweeps == 2) kJacobiUVBlock<1024, 2><<<sb.nBlocks, 1024, 0, s>>>(m, st, sb, in, out);
-        else kJacobiUVBlock<1024, 3><<<sb.nBlocks, 1024, 0, s>>>(m, st, sb, in, out);
+        if (sweeps == 2) {
+            if (tri) kJacobiUVBlock<1024, 2, 3><<<sb.nBlocks, 1024, 0, s>>>(m, st, sb, in, out);
+            else kJacobiUVBlock<1024, 2, kSweepRegSlots><<<sb.nBlocks, 1024, 0, s>>>(m, st, sb, in, out);
+        } else {
+            if (tri) kJacobiUVBlock<1024, 3, 3><<<sb.nBlocks, 1024, 0, s>>>(m, st, sb, in, out);
+            else kJacobiUVBlock<1024, 3, kSweepRegSlots><<<sb.nBlocks, 1024, 0, s>>>(m, st, sb, in, out);
+        }
     } else {
         throw std::invalid_argument("fused sweeps take blocks of 512 or 1024 cells");
     }
@@ -2088,12 +2100,15 @@ void launchMGProlongSmooth(hipStream_t s, const DLevel& f, const DLevel& c, doub
 void launchMGLevel0Block(hipStream_t s, const DLevel& f, const DLevel& c, const DSweepBlocks& sb, int sweeps,
                          bool prolong, double w, const Ctl* ctl) {
     if (sb.cells != 512) throw std::invalid_argument("fused level-0 sweeps take blocks of 512 cells");
+    const bool tri = f.maxRow <= 3;
     if (sweeps == 1 && !prolong) {
         if (sb.rings < 1) throw std::invalid_argument("fused level-0 sweeps need one halo ring");
-        kMGLevel0Block<512, 1, false, true><<<sb.nBlocks, 512, 0, s>>>(f, c, sb, w, ctl);
+        if (tri) kMGLevel0Block<512, 1, false, true, 3><<<sb.nBlocks, 512, 0, s>>>(f, c, sb, w, ctl);
+        else kMGLevel0Block<512, 1, false, true, kSweepRegSlots><<<sb.nBlocks, 512, 0, s>>>(f, c, sb, w, ctl);
     } else if (sweeps == 2 && prolong) {
         if (sb.rings < 2) throw std::invalid_argument("fused level-0 sweeps need two halo rings");
-        kMGLevel0Block<512, 2, true, true><<<sb.nBlocks, 512, 0, s>>>(f, c, sb, w, ctl);
+        if (tri) kMGLevel0Block<512, 2, true, true, 3><<<sb.nBlocks, 512, 0, s>>>(f, c, sb, w, ctl);
+        else kMGLevel0Block<512, 2, true, true, kSweepRegSlots><<<sb.nBlocks, 512, 0, s>>>(f, c, sb, w, ctl);
     } else {
         throw std::invalid_argument("unsupported fused level-0 sweep combination");
     }

@@ -106,7 +106,7 @@ void MGDevice::build(int n, const std::vector<int>& start, const std::vector<int
     fuseL0_ = false;
     dl0_ = DSweepBlocks{};
     if (host.size() >= 2 && host[0].n >= kFuseL0MinCells) {
-        buildSweepBlocks(host[0].n, host[0].start, host[0].nb, 512, 2, l0Blocks_);
+        buildSweepBlocks(host[0].n, host[0].start, host[0].nb, 512, 2, kSweepRegSlots, l0Blocks_);
         if (l0Blocks_.maxSlots <= kSweepRegSlots && l0Blocks_.maxHalo <= sweepHaloMax(512)) {
             dl0_.cells = l0Blocks_.cells;
             dl0_.rings = l0Blocks_.rings;
@@ -114,8 +114,8 @@ void MGDevice::build(int n, const std::vector<int>& start, const std::vector<int
             dl0_.haloStart = arena.upload(l0Blocks_.haloStart);
             dl0_.ringEnd = arena.upload(l0Blocks_.ringEnd);
             dl0_.halo = arena.upload(l0Blocks_.halo.empty() ? std::vector<int>{0} : l0Blocks_.halo);
-            dl0_.sorted = reinterpret_cast<const int2*>(
-                arena.upload(l0Blocks_.sorted.empty() ? std::vector<int>{0, 0} : l0Blocks_.sorted));
+            dl0_.locOwn = arena.upload(l0Blocks_.locOwn);
+            dl0_.locHalo = arena.upload(l0Blocks_.locHalo.empty() ? std::vector<int16_t>{-1} : l0Blocks_.locHalo);
             fuseL0_ = true;
         }
     }
@@ -1427,7 +1427,7 @@ void Solver::prepareSweepBlocks() {
     for (int s = fuseReq_; s >= 2; s--) {
         if (sweepBlocks_.cells != fuseCellsReq_ || sweepBlocks_.rings != s - 1) {
             SweepBlocks sb;
-            buildSweepBlocks(mesh_.nCells, csrStart_, csrNb_, fuseCellsReq_, s - 1, sb);
+            buildSweepBlocks(mesh_.nCells, csrStart_, csrNb_, fuseCellsReq_, s - 1, kSweepRegSlots, sb);
             if (sb.maxSlots > kSweepRegSlots || sb.maxHalo > hmax) continue;
             sweepBlocks_ = std::move(sb);
             dropGraph();  // it holds the old blocks' addresses
@@ -1439,8 +1439,9 @@ void Solver::prepareSweepBlocks() {
             dsb_.haloStart = sweepArena_.upload(sweepBlocks_.haloStart);
             dsb_.ringEnd = sweepArena_.upload(sweepBlocks_.ringEnd);
             dsb_.halo = sweepArena_.upload(sweepBlocks_.halo.empty() ? std::vector<int>{0} : sweepBlocks_.halo);
-            dsb_.sorted = reinterpret_cast<const int2*>(
-                sweepArena_.upload(sweepBlocks_.sorted.empty() ? std::vector<int>{0, 0} : sweepBlocks_.sorted));
+            dsb_.locOwn = sweepArena_.upload(sweepBlocks_.locOwn);
+            dsb_.locHalo = sweepArena_.upload(sweepBlocks_.locHalo.empty() ? std::vector<int16_t>{-1}
+                                                                           : sweepBlocks_.locHalo);
         }
         fuse_ = s;
         return;

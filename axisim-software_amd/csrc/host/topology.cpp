@@ -126,16 +126,18 @@ void cellNeighbourCSR(const HostMesh& m, std::vector<int>& start, std::vector<in
 }
 
 void buildSweepBlocks(int nCells, const std::vector<int>& start, const std::vector<int>& nb, int cells, int rings,
-                      SweepBlocks& out) {
-    if (cells <= 0 || rings < 1) throw std::invalid_argument("sweep blocks need cells > 0 and rings >= 1");
+                      int rowSlots, SweepBlocks& out) {
+    if (cells <= 0 || rings < 1 || rowSlots < 1) throw std::invalid_argument("bad sweep block parameters");
     out = SweepBlocks{};
     out.cells = cells;
     out.rings = rings;
+    out.rowSlots = rowSlots;
     out.nBlocks = (nCells + cells - 1) / cells;
     out.haloStart.assign(1, 0);
     out.ringEnd.reserve((size_t)out.nBlocks * rings);
+    out.locOwn.assign(start[nCells], (int16_t)-1);
     for (int c = 0; c < nCells; c++) out.maxSlots = std::max(out.maxSlots, start[c + 1] - start[c]);
-    std::vector<int> stamp(nCells, -1), frontier, next, ring;
+    std::vector<int> stamp(nCells, -1), where(nCells, -1), frontier, ring;
     for (int b = 0; b < out.nBlocks; b++) {
         const int a = b * cells, e = std::min(nCells, a + cells);
         const size_t h0 = out.halo.size();
@@ -161,13 +163,24 @@ void buildSweepBlocks(int nCells, const std::vector<int>& start, const std::vect
         }
         const int nh = (int)(out.halo.size() - h0);
         out.maxHalo = std::max(out.maxHalo, nh);
-        next.resize(nh);
-        for (int i = 0; i < nh; i++) next[i] = i;
-        std::sort(next.begin(), next.end(), [&](int x, int y) { return out.halo[h0 + x] < out.halo[h0 + y]; });
-        for (int i : next) {
-            out.sorted.push_back(out.halo[h0 + i]);
-            out.sorted.push_back(cells + i);
+        for (int i = 0; i < nh; i++) where[out.halo[h0 + i]] = cells + i;
+        auto local = [&](int o) -> int16_t {
+            if (o < 0) return -1;
+            if (o >= a && o < e) return (int16_t)(o - a);
+            return (int16_t)where[o];  // >= cells: every neighbour read from LDS is listed
+        };
+        for (int c = a; c < e; c++)
+            for (int k = start[c]; k < start[c + 1]; k++) out.locOwn[k] = local(nb[k]);
+        // rows of rings 1 .. rings - 1, which the sweeps after the first read from LDS
+        const int inner = rings >= 2 ? out.ringEnd[(size_t)b * rings + rings - 2] : 0;
+        for (int i = 0; i < nh; i++) {
+            const int c = out.halo[h0 + i];
+            for (int q = 0; q < rowSlots; q++) {
+                const int k = start[c] + q;
+                out.locHalo.push_back((i < inner && k < start[c + 1]) ? local(nb[k]) : (int16_t)-1);
+            }
         }
+        for (int i = 0; i < nh; i++) where[out.halo[h0 + i]] = -1;
         out.haloStart.push_back((int)out.halo.size());
     }
 }

@@ -38,20 +38,24 @@ void cellNeighbourCSR(const HostMesh& m, std::vector<int>& start, std::vector<in
 // Jacobi sweeps in one workgroup: ring r of a block holds the cells at graph
 // distance r from it.  Sweep j of s fused sweeps updates the block and rings
 // 1 .. s - j, so s sweeps need rings 1 .. s - 1.  Per block, the halo lists
-// ring 1's cells, then ring 2's, ..., each ring ascending; `sorted` lists the
-// same cells ascending as (cell, local index) pairs, local index = cells + the
-// cell's position in the halo.
+// ring 1's cells, then ring 2's, ..., each ring ascending.  In the workgroup's
+// LDS, block cell a + i sits at slot i and halo entry i at slot cells + i;
+// the tables give, per slot of a row that a later sweep reads from LDS, its
+// neighbour's LDS slot (-1: boundary slot or past the row's slots).
 struct SweepBlocks {
-    int cells = 0, rings = 0, nBlocks = 0;
+    int cells = 0, rings = 0, nBlocks = 0, rowSlots = 0;
     int maxHalo = 0;   // largest halo of any block
     int maxSlots = 0;  // most slots of any cell
-    std::vector<int> haloStart;  // nBlocks + 1 offsets into halo / sorted
+    std::vector<int> haloStart;  // nBlocks + 1 offsets into halo
     std::vector<int> ringEnd;    // [block * rings + r - 1]: halo entries of rings 1..r
     std::vector<int> halo;
-    std::vector<int> sorted;     // interleaved (cell, local index)
+    std::vector<int16_t> locOwn;   // per slot of every cell (the mesh's slot order)
+    std::vector<int16_t> locHalo;  // per halo entry, rowSlots slots (rings 1 .. rings - 1)
 };
+// rowSlots: slots per halo row in locHalo (cells with more slots leave the
+// fused path; maxSlots tells).
 void buildSweepBlocks(int nCells, const std::vector<int>& start, const std::vector<int>& nb, int cells, int rings,
-                      SweepBlocks& out);
+                      int rowSlots, SweepBlocks& out);
 
 // Greedy smallest-free colouring in cell order, colour-major order
 // (buildMeshColoring, memory_manger.cpp:334-409).  Returns the colour count.

@@ -97,7 +97,9 @@ def main() -> int:
     log(f"jacobi_uv (one sweep): {k1_ms * 1000:.1f} us/launch, {k1_gbs:.0f} GB/s")
     if fused >= 2:
         k_ms, k_bytes = s.bench_kernel(axs.KERNEL_JACOBI_UV_FUSED, 200)
-        kname = f"kJacobiUVBlock<{fused_cells},{fused}>"
+        # template arguments as rocprofv3 names the kernel: cells, sweeps, slots held per row
+        regs = 3 if int(np.diff(m.arrays["cell_face_start"]).max()) <= 3 else 4
+        kname = f"kJacobiUVBlock<{fused_cells},{fused},{regs}>"
     else:
         k_ms, k_bytes, kname = k1_ms, k1_bytes, "kJacobiUV"
     achieved = k_bytes / (k_ms * 1e-3) / 1e9
