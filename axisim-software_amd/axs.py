@@ -473,9 +473,10 @@ class Solver:
         _check(lib().axs_autotune(self._h, repeats, C.byref(a), C.byref(b)), self._h)
         return a.value, b.value
 
-    def set_sweep_fusion(self, sweeps: int, block_cells: int = 1024) -> None:
+    def set_sweep_fusion(self, sweeps: int, block_cells: int = 0) -> None:
         """Momentum Jacobi sweeps per launch (1, 2 or 3) and cells per fused
-        block (512 or 1024) for the next run; bit-identical results."""
+        block (512 or 1024; 0: the larger one whose halos fit) for the next
+        run; bit-identical results."""
         _check(lib().axs_set_sweep_fusion(self._h, sweeps, block_cells), self._h)
 
     def set_learned_guess(self, net: "Net | None", level: int = -1) -> None:

@@ -1413,22 +1413,25 @@ int Solver::residualHistory(int res, double* out, int maxN) const {
 void Solver::setSweepFusion(int sweeps, int blockCells) {
     if (running_) throw std::logic_error("solver is running");
     if (sweeps < 1 || sweeps > 3) throw std::invalid_argument("sweeps per launch must be 1, 2 or 3");
-    if (blockCells != 512 && blockCells != 1024) throw std::invalid_argument("fused blocks hold 512 or 1024 cells");
+    if (blockCells != 0 && blockCells != 512 && blockCells != 1024)
+        throw std::invalid_argument("fused blocks hold 512 or 1024 cells (0: either)");
     fuseReq_ = sweeps;
     fuseCellsReq_ = blockCells;
 }
 
 // Build (or reuse) the blocks and halo rings for the largest sweep count
-// <= fuseReq_ whose halos fit the kernel's LDS and whose cells' rows fit its
-// registers; fuse_ = 1 when none does.
+// <= fuseReq_ (and, unless fuseCellsReq_ fixes it, the larger block size: its
+// halo is the smaller share) whose halos fit the kernel's LDS and whose cells'
+// rows fit its registers; fuse_ = 1 when none does.
 void Solver::prepareSweepBlocks() {
     fuse_ = 1;
-    const int hmax = sweepHaloMax(fuseCellsReq_);
-    for (int s = fuseReq_; s >= 2; s--) {
-        if (sweepBlocks_.cells != fuseCellsReq_ || sweepBlocks_.rings != s - 1) {
+    for (int s = fuseReq_; s >= 2; s--)
+        for (int cells : {1024, 512}) {
+        if (fuseCellsReq_ != 0 && cells != fuseCellsReq_) continue;
+        if (sweepBlocks_.cells != cells || sweepBlocks_.rings != s - 1) {
             SweepBlocks sb;
-            buildSweepBlocks(mesh_.nCells, csrStart_, csrNb_, fuseCellsReq_, s - 1, kSweepRegSlots, sb);
-            if (sb.maxSlots > kSweepRegSlots || sb.maxHalo > hmax) continue;
+            buildSweepBlocks(mesh_.nCells, csrStart_, csrNb_, cells, s - 1, kSweepRegSlots, sb);
+            if (sb.maxSlots > kSweepRegSlots || sb.maxHalo > sweepHaloMax(cells)) continue;
             sweepBlocks_ = std::move(sb);
             dropGraph();  // it holds the old blocks' addresses
             AXS_HIP(hipStreamSynchronize(stream_));
@@ -1445,7 +1448,7 @@ void Solver::prepareSweepBlocks() {
         }
         fuse_ = s;
         return;
-    }
+        }
 }
 
 // Sweeps per launch for a momentum solve of `sweeps` Jacobi sweeps: chunks of
@@ -1558,9 +1561,9 @@ void Solver::autotune(int repeats, double* msDefault, double* msTuned) {
         };
         // the default request first (whatever count its halos allow), then
         // every count and block size that fits
-        const int options[][2] = {{3, 512}, {1, 512}, {2, 512}, {2, 1024}, {3, 1024}};
+        const int options[][2] = {{3, 0}, {1, 0}, {2, 512}, {2, 1024}, {3, 512}, {3, 1024}};
         double bt = 1e300;
-        int bs = 1, bcells = 512;
+        int bs = 1, bcells = 0;
         for (size_t i = 0; i < sizeof(options) / sizeof(options[0]); i++) {
             fuseReq_ = options[i][0];
             fuseCellsReq_ = options[i][1];

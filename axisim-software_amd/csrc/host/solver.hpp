@@ -174,8 +174,9 @@ public:
     void setLearnedGuess(class Mpnn* net, int level);
     static constexpr int kGuessCells = 16384;
     // Momentum Jacobi sweeps fused per launch (1 = one sweep per launch, 2 or
-    // 3) and the cells per fused block (512 or 1024).  The next run uses the
-    // largest count <= sweeps whose halos fit; results are unchanged.
+    // 3) and the cells per fused block (512 or 1024; 0: 1024 where its halos
+    // fit, else 512).  The next run uses the largest count <= sweeps whose
+    // halos fit; results are unchanged.
     void setSweepFusion(int sweeps, int blockCells);
     int sweepsPerLaunch() const { return fuse_; }
 
@@ -238,7 +239,7 @@ private:
     // fused momentum sweeps: requested and in-use sweeps per launch, the blocks
     // and halo rings (host and device copies, built for fuseCellsReq_ and
     // fuse_ - 1 rings)
-    int fuseReq_ = 3, fuseCellsReq_ = 512, fuse_ = 1;
+    int fuseReq_ = 3, fuseCellsReq_ = 0, fuse_ = 1;
     SweepBlocks sweepBlocks_;
     DSweepBlocks dsb_{};
     Arena sweepArena_;
