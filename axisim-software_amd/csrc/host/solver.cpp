@@ -1244,7 +1244,9 @@ void Solver::benchKernel(int kernel, int repeats, double* ms, double* bytes) {
         break;
     }
     case AXS_KERNEL_FACE_FLUX:
-        *bytes = 68.0 * F + 56.0 * N;
+        // faces: owner, neighbour, normal, area, 1/|d|, wP, mDot out (the
+        // boundary faces' group and d_PB aside); cells: u, v, p, grad p, DU, DV
+        *bytes = 56.0 * F + 56.0 * N;
         launch = [&] { launchFaceFlux(stream_, dm_, dbc_[0], dbc_[1], dbc_[2], st, ph); };
         break;
     case AXS_KERNEL_PP_COEFF:
