@@ -63,3 +63,14 @@ def test_fused_sweeps_bit_exact_full_size(require_gpu):
     assert s.stats().sweeps_per_launch == 3
     for name in FIELDS:
         np.testing.assert_array_equal(s.field(name), ref.field(name), err_msg=name)
+
+
+def test_fused_sweeps_quad_mesh_bit_exact(require_gpu):
+    """A structured quad pipe (4 slots per cell, Green-Gauss gradients) numbered
+    in rows of 32 cells, so its halos fit: fused against one sweep per launch."""
+    case = axs.pipe_case(nz=32, nr=640, max_iter=6, tol=0.0)
+    ref = run(case, 1)
+    s = run(case, 3, 0)
+    assert s.stats().sweeps_per_launch == 3
+    for name in FIELDS:
+        np.testing.assert_array_equal(s.field(name), ref.field(name), err_msg=name)

@@ -128,3 +128,15 @@ def test_fused_level0_cycles_bit_exact(require_gpu, nozzle_system, cycles):
     xg2 = g.solve(AC, AF, b, x0, cycles=cycles, pre=2, post=1)
     xr2 = OracleMG(fs, fn).solve(AC, AF, b, x0, cycles=cycles, pre=2, post=1)
     np.testing.assert_array_equal(xg2, xr2)
+
+
+def test_fused_level0_quad_mesh_bit_exact(require_gpu):
+    """The fused level-0 cycles on a quad mesh (4 slots per row) against the
+    oracle, bit for bit."""
+    case = axs.pipe_case(nz=32, nr=640)
+    fs, fn, AC, AF, b = pp_system(case, iters=2)
+    assert fs.size - 1 >= 16384
+    x0 = np.linspace(-1.0, 1.0, b.size) * 1e-3
+    xg = axs.Multigrid(fs, fn).solve(AC, AF, b, x0, cycles=3)
+    xr = OracleMG(fs, fn).solve(AC, AF, b, x0, cycles=3)
+    np.testing.assert_array_equal(xg, xr)
