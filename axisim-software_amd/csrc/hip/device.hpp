@@ -93,6 +93,10 @@ struct DLevel {
     int maxRow = 0;      // most off-diagonal slots of a row
     int maxMembers = 0;  // most fine cells of one coarse cell (levels with a coarser one)
     int uniformRow = 0;  // > 0: every row has this many slots (start[n] = uniformRow * n)
+    // workgroups of kMGResidualRestrict: coarse-cell ranges whose members fit
+    // one workgroup (nAggBlocks + 1 entries; 0 blocks: not built)
+    const int* aggBlocks = nullptr;
+    int nAggBlocks = 0;
     int *start = nullptr, *nb = nullptr;
     double *AC = nullptr, *b = nullptr, *AF = nullptr;
     double *x = nullptr, *xNew = nullptr;
@@ -194,5 +198,8 @@ void launchMGCoarseGather(hipStream_t s, const DLevel& f, const DLevel& c, const
 void launchMGResidual(hipStream_t s, const DLevel& f, const Ctl* ctl);
 void launchMGRestrictSum(hipStream_t s, const DLevel& f, const DLevel& c, double w, int sweep1, bool fp32,
                          const Ctl* ctl);
+// the two above in one launch over f.aggBlocks (the fine residual stays in LDS)
+void launchMGResidualRestrict(hipStream_t s, const DLevel& f, const DLevel& c, double w, int sweep1, bool fp32,
+                              const Ctl* ctl);
 
 }  // namespace axs

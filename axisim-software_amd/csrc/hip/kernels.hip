@@ -1261,11 +1261,7 @@ __device__ __forceinline__ T sweepFromZero(T b, T AC, T w) {
 // one round each (indices clamped into the row); B = 0 or longer rows: the
 // plain loop.  The arithmetic is the same in either case.
 template <class T, int B>
-__global__ void __launch_bounds__(kBlock) kMGResidual(DLevel f, const Ctl* ctl) {
-    if (finished(ctl)) return;
-    const int n = xcdBlock() * blockDim.x + threadIdx.x;
-    if (n >= f.n) return;
-    const LV<T> v = view<T>(f);
+__device__ __forceinline__ T mgResidualRow(const DLevel& f, const LV<T>& v, int n) {
     int k0, ns;
     rowRange(f.start, f.uniformRow, n, k0, ns);
     T Ax = v.AC[n] * v.x[n];
@@ -1289,7 +1285,16 @@ __global__ void __launch_bounds__(kBlock) kMGResidual(DLevel f, const Ctl* ctl) 
             if (nb >= 0) Ax += v.AF[k] * v.x[nb];
         }
     }
-    v.res[n] = v.b[n] - Ax;
+    return v.b[n] - Ax;
+}
+
+template <class T, int B>
+__global__ void __launch_bounds__(kBlock) kMGResidual(DLevel f, const Ctl* ctl) {
+    if (finished(ctl)) return;
+    const int n = xcdBlock() * blockDim.x + threadIdx.x;
+    if (n >= f.n) return;
+    const LV<T> v = view<T>(f);
+    v.res[n] = mgResidualRow<T, B>(f, v, n);
 }
 
 // jacobiFused, multigrid.cu:486-493 (B as in kMGResidual)
@@ -1583,6 +1588,34 @@ __global__ void __launch_bounds__(kBlock) kMGRestrictSum(DLevel f, DLevel c, dou
     }
     cv.b[i] = acc;
     cv.x[i] = sweep1 ? sweepFromZero(acc, cv.AC[i], (T)w) : T(0);
+}
+
+// kMGResidual and kMGRestrictSum in one launch.  Workgroup b covers the coarse
+// cells [aggStart[b], aggStart[b + 1]); their members (at most kBlock, one run
+// of the member list) are its fine cells.  Thread t computes the residual of
+// member p0 + t with kMGResidual's expression into LDS; after the barrier
+// thread t sums coarse cell aggStart[b] + t's members from LDS in list order,
+// as kMGRestrictSum does.  The fine residual never goes to memory.
+template <class T, int B>
+__global__ void __launch_bounds__(kBlock) kMGResidualRestrict(DLevel f, DLevel c, const int* __restrict__ aggStart,
+                                                              double w, int sweep1, const Ctl* ctl) {
+    __shared__ T r[kBlock];
+    if (finished(ctl)) return;
+    const int blk = xcdBlock(), t = threadIdx.x;
+    const int j0 = aggStart[blk], j1 = aggStart[blk + 1];
+    const int p0 = f.memberStart[j0], np = f.memberStart[j1] - p0;
+    const LV<T> fv = view<T>(f);
+    const LV<T> cv = view<T>(c);
+    if (t < np) r[t] = mgResidualRow<T, B>(f, fv, f.members[p0 + t]);
+    __syncthreads();
+    if (t < j1 - j0) {
+        const int j = j0 + t;
+        const int a = f.memberStart[j] - p0, e = f.memberStart[j + 1] - p0;
+        T acc = T(0);
+        for (int q = a; q < e; q++) acc += r[q];
+        cv.b[j] = acc;
+        cv.x[j] = sweep1 ? sweepFromZero(acc, cv.AC[j], (T)w) : T(0);
+    }
 }
 
 // ===========================================================================
@@ -1964,6 +1997,21 @@ void launchMGResidual32(hipStream_t s, const DLevel& f, const Ctl* ctl) { launch
 void launchMGJacobi32(hipStream_t s, const DLevel& L, float w, const Ctl* ctl) { launchMGJacobiT<float>(s, L, w, ctl); }
 void launchMGProlongSmooth32(hipStream_t s, const DLevel& f, const DLevel& c, float w, const Ctl* ctl) {
     launchMGProlongSmoothT<float>(s, f, c, w, ctl);
+}
+template <class T>
+static void launchMGResidualRestrictT(hipStream_t s, const DLevel& f, const DLevel& c, double w, int sweep1,
+                                      const Ctl* ctl) {
+    const dim3 g(f.nAggBlocks), b(kBlock);
+    if (f.maxRow <= 4) kMGResidualRestrict<T, 4><<<g, b, 0, s>>>(f, c, f.aggBlocks, w, sweep1, ctl);
+    else if (f.maxRow <= 8) kMGResidualRestrict<T, 8><<<g, b, 0, s>>>(f, c, f.aggBlocks, w, sweep1, ctl);
+    else if (f.maxRow <= 16) kMGResidualRestrict<T, 16><<<g, b, 0, s>>>(f, c, f.aggBlocks, w, sweep1, ctl);
+    else kMGResidualRestrict<T, 0><<<g, b, 0, s>>>(f, c, f.aggBlocks, w, sweep1, ctl);
+}
+void launchMGResidualRestrict(hipStream_t s, const DLevel& f, const DLevel& c, double w, int sweep1, bool fp32,
+                              const Ctl* ctl) {
+    if (f.nAggBlocks <= 0) throw std::logic_error("level has no aggregate blocks");
+    if (fp32) launchMGResidualRestrictT<float>(s, f, c, w, sweep1, ctl);
+    else launchMGResidualRestrictT<double>(s, f, c, w, sweep1, ctl);
 }
 void launchMGRestrictSum(hipStream_t s, const DLevel& f, const DLevel& c, double w, int sweep1, bool fp32,
                          const Ctl* ctl) {

@@ -89,6 +89,20 @@ void MGDevice::build(int n, const std::vector<int>& start, const std::vector<int
             }
         }
         if (!h.coarsest()) {
+            // workgroups of the fused residual + restriction: runs of whole
+            // aggregates with at most kBlock members between them
+            std::vector<int> agg{0};
+            bool fits = true;
+            for (int j = 0; j < host[l + 1].n; j++) {
+                const int m = h.memberStart[j + 1] - h.memberStart[j];
+                if (m > kBlock) fits = false;
+                if (h.memberStart[j + 1] - h.memberStart[agg.back()] > kBlock) agg.push_back(j);
+            }
+            agg.push_back(host[l + 1].n);
+            if (fits && host[l + 1].n > 0) {
+                d.aggBlocks = arena.upload(agg);
+                d.nAggBlocks = (int)agg.size() - 1;
+            }
             d.nCoarse = host[l + 1].n;
             d.slotToCoarseSlot = arena.upload(h.slotToCoarseSlot);
             d.cslotStart = arena.upload(h.cslotStart);
@@ -243,6 +257,10 @@ void MGDevice::smooth(hipStream_t s, int l, int sweeps, const MGParams& p, const
 // (multigrid.cu:641-655), with the coarse level's first sweep from that zero
 // folded in when sweep1 is set.
 void MGDevice::restrictTo(hipStream_t s, int l, int sweep1, bool fp32, const MGParams& p, const Ctl* ctl) {
+    if (dev[l].nAggBlocks > 0) {
+        launchMGResidualRestrict(s, dev[l], dev[l + 1], p.weight, sweep1, fp32, ctl);
+        return;
+    }
     if (fp32) launchMGResidual32(s, dev[l], ctl);
     else launchMGResidual(s, dev[l], ctl);
     launchMGRestrictSum(s, dev[l], dev[l + 1], p.weight, sweep1, fp32, ctl);
@@ -529,7 +547,8 @@ double MGDevice::bytesPerSolve(const MGParams& p, bool buildOperators) const {
             }
             const double nc = host[l + 1].n;
             cyc += gs(l, p.pre) + gs(l, p.post);
-            cyc += (5.0 * v + 8.0) * n + (v + 4.0) * S + (2.0 * v + 4.0) * nc;  // residual + restriction
+            // residual + restriction (the residual written and re-read unless fused)
+            cyc += ((dev[l].nAggBlocks > 0 ? 3.0 : 5.0) * v + 8.0) * n + (v + 4.0) * S + (2.0 * v + 4.0) * nc;
             cyc += (2.0 * v + 4.0) * n + v * nc;                                  // prolongation
         }
         return bytes + (p.cycles > 1 ? p.cycles : 1) * cyc;
@@ -558,8 +577,10 @@ double MGDevice::bytesPerSolve(const MGParams& p, bool buildOperators) const {
             const int s1 =
                 host[l + 1].coarsest() ? (!(mixed && p.coarseDirect) && p.coarseSweeps >= 1) : (p.pre >= 1);
             // restriction: fine rows (AC, b, x, start, members) + the residual written and
-            // re-read + memberStart, coarse b and x (+ coarse AC for the folded sweep)
-            cyc += (5.0 * v + 8.0) * n + (v + 4.0) * S + (2.0 * v + 4.0) * nc + (s1 ? v * nc : 0.0);
+            // re-read (unless fused) + memberStart, coarse b and x (+ coarse AC for the
+            // folded sweep)
+            cyc += ((dev[l].nAggBlocks > 0 ? 3.0 : 5.0) * v + 8.0) * n + (v + 4.0) * S + (2.0 * v + 4.0) * nc +
+                   (s1 ? v * nc : 0.0);
             if (p.post >= 1) cyc += (4.0 * v + 8.0) * n + (v + 4.0) * S + v * nc + sm(l, p.post - 1);
             else cyc += (2.0 * v + 4.0) * n + v * nc;
             preDone = s1;
