@@ -1782,14 +1782,29 @@ __global__ void __launch_bounds__(1024) kMGTail(const TailLevel* __restrict__ tl
             // order, so its sweeps need no workgroup barrier in between
             if (t < 64) {
                 unsigned f = flip;
+                // rows of at most kWaveK slots (the coarsest level's are) sweep
+                // with a shorter register row: fewer dependent steps per sweep
+                constexpr int kWaveK = 8;
+                static_assert(kWaveK <= kRowK, "a short row is a prefix of the register row");
+                const bool shortRows = __all(!own || r.deg <= kWaveK);
+                const bool keep = own && tabs(r.ac) < T(1.0e-30);
                 for (int it = 0; it < count; it++) {
                     const T* x = sT + (((f >> q) & 1u) ? S.x1 : S.x0);
                     T* y = sT + (((f >> q) & 1u) ? S.x0 : S.x1);
                     if (own) {
                         const T xn = x[t];
                         T res;
-                        if (tabs(r.ac) < T(1.0e-30)) {
+                        if (keep) {
                             res = xn;
+                        } else if (shortRows) {
+                            T xv[kWaveK];
+#pragma unroll
+                            for (int k = 0; k < kWaveK; k++) xv[k] = r.nb[k] != -1 ? x[r.nb[k]] : T(0);
+                            T Ax = r.ac * xn;
+#pragma unroll
+                            for (int k = 0; k < kWaveK; k++)
+                                if (r.nb[k] != -1) Ax += r.af[k] * xv[k];
+                            res = xn + w * (r.b - Ax) / r.ac;
                         } else {
                             const T Ax = rowAx(r, xn, [&](int j) { return x[j]; }, AF, nbs);
                             res = xn + w * (r.b - Ax) / r.ac;

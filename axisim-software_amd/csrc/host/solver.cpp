@@ -164,6 +164,10 @@ MGDevice::TailPlan MGDevice::planTail(int lt, size_t vsize) const {
     const int L = (int)host.size();
     t.lt = lt;
     t.nl = L - lt;
+    // Values the kernel reads before writing them (every level's AC and AF,
+    // level lt's b and x0) come first, then the integer CSR and maps, then
+    // the values it only writes before reading (x1, the deeper levels' b and
+    // x0): a launch stages the image up to the end of the integers only.
     long tv = 0, ti = 0;
     int maxN = 0;
     for (int l = lt; l < L; l++) {
@@ -174,9 +178,10 @@ MGDevice::TailPlan MGDevice::planTail(int lt, size_t vsize) const {
         e.refs = h.refs();
         e.nc = last ? 0 : host[l + 1].n;
         e.AC = (int)tv; tv += h.n;
-        e.b = (int)tv; tv += h.n;
-        e.x0 = (int)tv; tv += h.n;
-        e.x1 = (int)tv; tv += h.n;
+        if (l == lt) {
+            e.b = (int)tv; tv += h.n;
+            e.x0 = (int)tv; tv += h.n;
+        }
         e.AF = (int)tv; tv += e.refs;
         e.start = (int)ti; ti += h.n + 1;
         e.nb = (int)ti; ti += e.refs;
@@ -189,7 +194,18 @@ MGDevice::TailPlan MGDevice::planTail(int lt, size_t vsize) const {
         maxN = std::max(maxN, h.n);
     }
     t.intBase = (int)(((size_t)tv * vsize + 15) / 16 * 16);
-    t.bytes = ((size_t)t.intBase + (size_t)ti * sizeof(int) + 15) / 16 * 16;
+    t.staged = ((size_t)t.intBase + (size_t)ti * sizeof(int) + 15) / 16 * 16;
+    // the write-first values after the integers, in value units from the start
+    long tw = (long)(t.staged / vsize);
+    for (int l = lt; l < L; l++) {
+        TailLevel& e = t.lv[l - lt];
+        if (l != lt) {
+            e.b = (int)tw; tw += e.n;
+            e.x0 = (int)tw; tw += e.n;
+        }
+        e.x1 = (int)tw; tw += e.n;
+    }
+    t.bytes = ((size_t)tw * vsize + 15) / 16 * 16;
     t.threads = std::min(1024, std::max(64, (maxN + 63) / 64 * 64));
     return t;
 }
@@ -212,7 +228,7 @@ TailLaunch MGDevice::TailPlan::launch() const {
     TailLaunch l;
     l.plan = dev;
     l.image = image;
-    l.vecs = (int)(bytes / 16);
+    l.vecs = (int)(staged / 16);
     l.lt = lt;
     l.nl = nl;
     l.intBase = intBase;
@@ -224,7 +240,7 @@ TailLaunch MGDevice::TailPlan::launch() const {
 // bytes the tail kernel moves between HBM and LDS: the image in, level lt's x out
 double MGDevice::tailTrafficBytes(const TailPlan& t, size_t vsize) const {
     if (t.lt < 0) return 0.0;
-    return (double)t.bytes + (double)vsize * t.lv[0].n;
+    return (double)t.staged + (double)vsize * t.lv[0].n;
 }
 
 void MGDevice::bindLevel0(double* AC, double* b, double* AF, double* x, double* xNew) {

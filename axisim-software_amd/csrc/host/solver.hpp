@@ -106,7 +106,8 @@ private:
     // The tail kernel's LDS image of levels lt..lt+nl-1 (value type of size vsize)
     struct TailPlan {
         int lt = -1, nl = 0, intBase = 0, threads = 0;
-        size_t bytes = 0;
+        size_t bytes = 0;   // the whole image (the kernel's LDS)
+        size_t staged = 0;  // its prefix a launch copies in (values read first, integers)
         std::vector<TailLevel> lv;
         TailLevel* dev = nullptr;        // device copy of lv
         unsigned char* image = nullptr;  // the tail levels' arrays, laid out as in LDS
