@@ -2162,16 +2162,27 @@ void launchMGProlongSmooth(hipStream_t s, const DLevel& f, const DLevel& c, doub
 }
 void launchMGLevel0Block(hipStream_t s, const DLevel& f, const DLevel& c, const DSweepBlocks& sb, int sweeps,
                          bool prolong, double w, const Ctl* ctl) {
-    if (sb.cells != 512) throw std::invalid_argument("fused level-0 sweeps take blocks of 512 cells");
+    if (sb.cells != 512 && sb.cells != 1024) throw std::invalid_argument("fused level-0 sweeps take 512 or 1024 cells");
     const bool tri = f.maxRow <= 3;
+    const int C = sb.cells;
     if (sweeps == 1 && !prolong) {
         if (sb.rings < 1) throw std::invalid_argument("fused level-0 sweeps need one halo ring");
-        if (tri) kMGLevel0Block<512, 1, false, true, 3><<<sb.nBlocks, 512, 0, s>>>(f, c, sb, w, ctl);
-        else kMGLevel0Block<512, 1, false, true, kSweepRegSlots><<<sb.nBlocks, 512, 0, s>>>(f, c, sb, w, ctl);
+        if (C == 512) {
+            if (tri) kMGLevel0Block<512, 1, false, true, 3><<<sb.nBlocks, C, 0, s>>>(f, c, sb, w, ctl);
+            else kMGLevel0Block<512, 1, false, true, kSweepRegSlots><<<sb.nBlocks, C, 0, s>>>(f, c, sb, w, ctl);
+        } else {
+            if (tri) kMGLevel0Block<1024, 1, false, true, 3><<<sb.nBlocks, C, 0, s>>>(f, c, sb, w, ctl);
+            else kMGLevel0Block<1024, 1, false, true, kSweepRegSlots><<<sb.nBlocks, C, 0, s>>>(f, c, sb, w, ctl);
+        }
     } else if (sweeps == 2 && prolong) {
         if (sb.rings < 2) throw std::invalid_argument("fused level-0 sweeps need two halo rings");
-        if (tri) kMGLevel0Block<512, 2, true, true, 3><<<sb.nBlocks, 512, 0, s>>>(f, c, sb, w, ctl);
-        else kMGLevel0Block<512, 2, true, true, kSweepRegSlots><<<sb.nBlocks, 512, 0, s>>>(f, c, sb, w, ctl);
+        if (C == 512) {
+            if (tri) kMGLevel0Block<512, 2, true, true, 3><<<sb.nBlocks, C, 0, s>>>(f, c, sb, w, ctl);
+            else kMGLevel0Block<512, 2, true, true, kSweepRegSlots><<<sb.nBlocks, C, 0, s>>>(f, c, sb, w, ctl);
+        } else {
+            if (tri) kMGLevel0Block<1024, 2, true, true, 3><<<sb.nBlocks, C, 0, s>>>(f, c, sb, w, ctl);
+            else kMGLevel0Block<1024, 2, true, true, kSweepRegSlots><<<sb.nBlocks, C, 0, s>>>(f, c, sb, w, ctl);
+        }
     } else {
         throw std::invalid_argument("unsupported fused level-0 sweep combination");
     }

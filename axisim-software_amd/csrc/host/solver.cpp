@@ -115,13 +115,15 @@ void MGDevice::build(int n, const std::vector<int>& start, const std::vector<int
         AXS_HIP(hipMemcpy(tail64_.dev, tail64_.lv.data(), tail64_.lv.size() * sizeof(TailLevel),
                           hipMemcpyHostToDevice));
     }
-    // fused level-0 sweeps: blocks of 512 cells with two halo rings, when
-    // level 0 is large, its rows fit the kernel's registers and every halo its LDS
+    // fused level-0 sweeps: blocks of 1024 (else 512) cells with two halo
+    // rings, when level 0 is large, its rows fit the kernel's registers and
+    // every halo its LDS
     fuseL0_ = false;
     dl0_ = DSweepBlocks{};
-    if (host.size() >= 2 && host[0].n >= kFuseL0MinCells) {
-        buildSweepBlocks(host[0].n, host[0].start, host[0].nb, 512, 2, kSweepRegSlots, l0Blocks_);
-        if (l0Blocks_.maxSlots <= kSweepRegSlots && l0Blocks_.maxHalo <= sweepHaloMax(512)) {
+    for (int cells : {1024, 512}) {
+        if (fuseL0_ || host.size() < 2 || host[0].n < kFuseL0MinCells) break;
+        buildSweepBlocks(host[0].n, host[0].start, host[0].nb, cells, 2, kSweepRegSlots, l0Blocks_);
+        if (l0Blocks_.maxSlots <= kSweepRegSlots && l0Blocks_.maxHalo <= sweepHaloMax(cells)) {
             dl0_.cells = l0Blocks_.cells;
             dl0_.rings = l0Blocks_.rings;
             dl0_.nBlocks = l0Blocks_.nBlocks;
