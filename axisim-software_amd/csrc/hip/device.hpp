@@ -131,10 +131,11 @@ struct TailLaunch {
 inline int blocksFor(long n, int t = kBlock) { return (int)((n + t - 1) / t); }
 
 // ---- launchers (kernels.hip) ----
-void launchGradUV(hipStream_t s, const DMesh& m, DBC bu, DBC bv, DState st, int scheme);
+// withP: also the pressure gradient (then launchMomentum with gradPDone)
+void launchGradUV(hipStream_t s, const DMesh& m, DBC bu, DBC bv, DBC bp, DState st, int scheme, bool withP);
 void launchGradField(hipStream_t s, const DMesh& m, DBC bc, const double* phi, double* gz, double* gr,
                      int scheme, const Ctl* ctl);
-void launchMomentum(hipStream_t s, const DMesh& m, DBC bu, DBC bv, DBC bp, DState st, Physics ph);
+void launchMomentum(hipStream_t s, const DMesh& m, DBC bu, DBC bv, DBC bp, DState st, Physics ph, bool gradPDone);
 void launchJacobiUV(hipStream_t s, const DMesh& m, DState st, const double* uo, const double* vo, double* un,
                     double* vn);
 // the same sweep with either side interleaved as (u, v) pairs: a non-null uvo

@@ -387,19 +387,26 @@ __device__ __forceinline__ double waveMax(double v) {
 // ===========================================================================
 // Gradients (computeGradient, solver_util.cu:146-169)
 // ===========================================================================
-template <int B>
-__global__ void __launch_bounds__(kBlock) kGradUV(DMesh m, DBC bu, DBC bv, DState s, int scheme) {
+// P: the pressure gradient too (then kMomentum reads it instead of computing
+// it); the three fields share one walk over the slots and their geometry.
+template <int B, bool P>
+__global__ void __launch_bounds__(kBlock) kGradUV(DMesh m, DBC bu, DBC bv, DBC bp, DState s, int scheme) {
     if (finished(s.ctl)) return;
     const int n = xcdBlock() * blockDim.x + threadIdx.x;
     if (n >= m.N) return;
-    const DBC bc[2] = {bu, bv};
-    const double* phi[2] = {s.u, s.v};
-    double gz[2], gr[2];
-    cellGradients<2, B>(m, bc, phi, n, scheme, gz, gr);
+    constexpr int NF = P ? 3 : 2;
+    const DBC bc[3] = {bu, bv, bp};
+    const double* phi[3] = {s.u, s.v, s.p};
+    double gz[NF], gr[NF];
+    cellGradients<NF, B>(m, bc, phi, n, scheme, gz, gr);
     s.gUZ[n] = gz[0];
     s.gUR[n] = gr[0];
     s.gVZ[n] = gz[1];
     s.gVR[n] = gr[1];
+    if (P) {
+        s.gPZ[n] = gz[NF - 1];
+        s.gPR[n] = gr[NF - 1];
+    }
 }
 
 template <int B>
@@ -428,8 +435,9 @@ __global__ void __launch_bounds__(kBlock) kGradField(DMesh m, DBC bc, const doub
 // Every step is per-cell, so one launch replaces nine.
 // ===========================================================================
 // HO: higher-order convection (the neighbour's u, v are read); the upwind
-// instance holds fewer registers.
-template <int B, bool HO>
+// instance holds fewer registers.  PG: the pressure gradient is computed
+// here (else kGradUV has already written it, from the same p).
+template <int B, bool HO, bool PG>
 __global__ void __launch_bounds__(kBlock) kMomentum(DMesh m, DBC bu, DBC bv, DBC bp, DState s, Physics ph) {
     if (finished(s.ctl)) return;
     const int n = xcdBlock() * blockDim.x + threadIdx.x;
@@ -563,9 +571,14 @@ __global__ void __launch_bounds__(kBlock) kMomentum(DMesh m, DBC bu, DBC bv, DBC
         }
     }
     double gz, gr;
-    cellGradient<B>(m, bp, s.p, n, ph.gradScheme, gz, gr);
-    s.gPZ[n] = gz;
-    s.gPR[n] = gr;
+    if (PG) {
+        cellGradient<B>(m, bp, s.p, n, ph.gradScheme, gz, gr);
+        s.gPZ[n] = gz;
+        s.gPR[n] = gr;
+    } else {
+        gz = s.gPZ[n];
+        gr = s.gPR[n];
+    }
     bU += -V * gz;
     bV += -V * gr;
     const double al = ph.alphaU;
@@ -2041,25 +2054,33 @@ void launchMGRestrictSum(hipStream_t s, const DLevel& f, const DLevel& c, double
         else kMGRestrictSum<double, 0><<<g, b, 0, s>>>(f, c, w, sweep1, ctl);
     }
 }
-void launchGradUV(hipStream_t s, const DMesh& m, DBC bu, DBC bv, DState st, int scheme) {
-    if (m.maxSlots <= 3) kGradUV<3><<<blocksFor(m.N), kBlock, 0, s>>>(m, bu, bv, st, scheme);
-    else kGradUV<4><<<blocksFor(m.N), kBlock, 0, s>>>(m, bu, bv, st, scheme);
+void launchGradUV(hipStream_t s, const DMesh& m, DBC bu, DBC bv, DBC bp, DState st, int scheme, bool withP) {
+    const dim3 g(blocksFor(m.N)), b(kBlock);
+    if (m.maxSlots <= 3) {
+        if (withP) kGradUV<3, true><<<g, b, 0, s>>>(m, bu, bv, bp, st, scheme);
+        else kGradUV<3, false><<<g, b, 0, s>>>(m, bu, bv, bp, st, scheme);
+    } else {
+        if (withP) kGradUV<4, true><<<g, b, 0, s>>>(m, bu, bv, bp, st, scheme);
+        else kGradUV<4, false><<<g, b, 0, s>>>(m, bu, bv, bp, st, scheme);
+    }
 }
 void launchGradField(hipStream_t s, const DMesh& m, DBC bc, const double* phi, double* gz, double* gr,
                      int scheme, const Ctl* ctl) {
     if (m.maxSlots <= 3) kGradField<3><<<blocksFor(m.N), kBlock, 0, s>>>(m, bc, phi, gz, gr, scheme, ctl);
     else kGradField<4><<<blocksFor(m.N), kBlock, 0, s>>>(m, bc, phi, gz, gr, scheme, ctl);
 }
-void launchMomentum(hipStream_t s, const DMesh& m, DBC bu, DBC bv, DBC bp, DState st, Physics ph) {
+void launchMomentum(hipStream_t s, const DMesh& m, DBC bu, DBC bv, DBC bp, DState st, Physics ph, bool gradPDone) {
     const dim3 g(blocksFor(m.N)), b(kBlock);
     const bool ho = ph.scheme != 0;
+#define AXS_MOMENTUM(BB, HO, PG) kMomentum<BB, HO, PG><<<g, b, 0, s>>>(m, bu, bv, bp, st, ph)
     if (m.maxSlots <= 3) {
-        if (ho) kMomentum<3, true><<<g, b, 0, s>>>(m, bu, bv, bp, st, ph);
-        else kMomentum<3, false><<<g, b, 0, s>>>(m, bu, bv, bp, st, ph);
+        if (ho) gradPDone ? AXS_MOMENTUM(3, true, false) : AXS_MOMENTUM(3, true, true);
+        else gradPDone ? AXS_MOMENTUM(3, false, false) : AXS_MOMENTUM(3, false, true);
     } else {
-        if (ho) kMomentum<4, true><<<g, b, 0, s>>>(m, bu, bv, bp, st, ph);
-        else kMomentum<4, false><<<g, b, 0, s>>>(m, bu, bv, bp, st, ph);
+        if (ho) gradPDone ? AXS_MOMENTUM(4, true, false) : AXS_MOMENTUM(4, true, true);
+        else gradPDone ? AXS_MOMENTUM(4, false, false) : AXS_MOMENTUM(4, false, true);
     }
+#undef AXS_MOMENTUM
 }
 void launchJacobiUV(hipStream_t s, const DMesh& m, DState st, const double* uo, const double* vo, double* un,
                     double* vn) {

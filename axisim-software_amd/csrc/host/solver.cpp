@@ -904,8 +904,11 @@ void Solver::enqueueIteration(hipStream_t s) {
     const DBC &bu = dbc_[AXS_VAR_U], &bv = dbc_[AXS_VAR_V], &bp = dbc_[AXS_VAR_P];
     DState st = ds_;
 
-    if (ph.nonOrtho || needGrad) launchGradUV(s, dm_, bu, bv, st, ph.gradScheme);
-    launchMomentum(s, dm_, bu, bv, bp, st, ph);
+    // the pressure gradient rides along with the velocity gradients when those
+    // are needed (one walk over the geometry instead of two)
+    const bool gradUV = ph.nonOrtho || needGrad;
+    if (gradUV) launchGradUV(s, dm_, bu, bv, bp, st, ph.gradScheme, true);
+    launchMomentum(s, dm_, bu, bv, bp, st, ph, gradUV);
 
     // momentum solves (LinearSolver::run, linear_solver.cu:91-148, 209-271)
     if (c.precision == AXS_PREC_MIXED) {
@@ -1272,7 +1275,7 @@ void Solver::benchKernel(int kernel, int repeats, double* ms, double* bytes) {
         // the oriented normal and length)
         *bytes = 4.0 * N + 40.0 * N + (lsq ? 24.0 * N : 8.0 * N) + 80.0 * N + (needGrad ? 32.0 * N : 0.0) +
                  65.0 * S + (lsq ? 16.0 * S : 25.0 * S);
-        launch = [&] { launchMomentum(stream_, dm_, dbc_[0], dbc_[1], dbc_[2], st, ph); };
+        launch = [&] { launchMomentum(stream_, dm_, dbc_[0], dbc_[1], dbc_[2], st, ph, false); };
         break;
     }
     case AXS_KERNEL_MG_SMOOTH: {
@@ -1520,10 +1523,11 @@ double Solver::bytesPerIteration() const {
     // face arrays in the per-face kernels.
     const double gradGeom = lsq ? (28.0 * N + 20.0 * S) : (12.0 * N + 37.0 * S);
     double b = 0.0;
-    if (c.use_non_orth || needGrad) b += gradGeom + 2 * 8.0 * N + 4 * 8.0 * N;              // grad u, v (one pass)
-    b += gradGeom + 8.0 * N                                                                 // grad p
-         + 65.0 * S + 8.0 * 4 * N + (c.use_non_orth || needGrad ? 32.0 * N : 0.0)            // slots, cells
-         + 8.0 * 10 * N;                                                                    // 10 outputs
+    const bool gradUV = c.use_non_orth || needGrad;
+    if (gradUV) b += gradGeom + 3 * 8.0 * N + 6 * 8.0 * N;                                 // grad u, v, p (one pass)
+    b += (gradUV ? 16.0 * N : gradGeom + 8.0 * N)                                          // grad p read, or computed
+         + 65.0 * S + 8.0 * 4 * N + (gradUV ? 32.0 * N : 0.0)                                // slots, cells
+         + 8.0 * (gradUV ? 8 : 10) * N;                                                     // outputs
     const bool fused = c.linear_solver == AXS_LINEAR_JACOBI && c.precision != AXS_PREC_MIXED;
     b += (fused ? sweepChunks(c.linear_max_iter).size() : c.linear_max_iter) * (68.0 * N + 12.0 * S);  // u, v sweeps
     b += 16.0 * N + 44.0 * S + 8.0 * S + 16.0 * N;                                          // pp coeff
