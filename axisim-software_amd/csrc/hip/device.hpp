@@ -194,6 +194,8 @@ int tailLdsLimit();
 // Galerkin product in two coalesced passes (fine diagonals, then gathers)
 void launchMGCoarseDiag(hipStream_t s, const DLevel& f, const Ctl* ctl);
 void launchMGCoarseGather(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl);
+// both in one launch over f.aggBlocks (the fine diagonals stay in LDS)
+void launchMGCoarseOperator(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl);
 // residual per fine cell + ordered gather per coarse cell (sweep1: also the
 // coarse level's first smoothing sweep from zero); fp32: the FP32 copies
 void launchMGResidual(hipStream_t s, const DLevel& f, const Ctl* ctl);

@@ -1268,6 +1268,75 @@ __device__ __forceinline__ T sweepFromZero(T b, T AC, T w) {
 }
 }  // namespace
 
+// kMGCoarseDiag and kMGCoarseGather in one launch, over the same workgroups
+// of whole aggregates as kMGResidualRestrict: thread t forms the diagonal of
+// member p0 + t (kMGCoarseDiag's sum) into LDS; after the barrier thread t sums
+// coarse cell aggStart[b] + t's members from LDS, and the workgroup's threads
+// stride over the coarse slots of its coarse cells (kMGCoarseGather's sums).
+template <int B>
+__global__ void __launch_bounds__(kBlock) kMGCoarseOperator(DLevel f, DLevel c, const Ctl* ctl) {
+    __shared__ double diag[kBlock];
+    if (finished(ctl)) return;
+    const int blk = xcdBlock(), t = threadIdx.x;
+    const int j0 = f.aggBlocks[blk], j1 = f.aggBlocks[blk + 1];
+    const int p0 = f.memberStart[j0], np = f.memberStart[j1] - p0;
+    if (t < np) {
+        const int n = f.members[p0 + t];
+        double d = f.AC[n];
+        int k0, ns;
+        rowRange(f.start, f.uniformRow, n, k0, ns);
+        if (B > 0 && ns > 0 && ns <= B) {
+            // the row's slots in one round (clamped), summed in slot order
+            int o[B > 0 ? B : 1], cs[B > 0 ? B : 1];
+            double a[B > 0 ? B : 1];
+#pragma unroll
+            for (int q = 0; q < B; q++) {
+                const int k = slotAt(k0, ns, q);
+                o[q] = f.nb[k];
+                cs[q] = f.slotToCoarseSlot[k];
+                a[q] = f.AF[k];
+            }
+#pragma unroll
+            for (int q = 0; q < B; q++)
+                if (q < ns && o[q] >= 0 && cs[q] < 0) d += a[q];
+        } else {
+            for (int k = k0; k < k0 + ns; k++) {
+                if (f.nb[k] < 0) continue;
+                if (f.slotToCoarseSlot[k] < 0) d += f.AF[k];
+            }
+        }
+        diag[t] = d;
+    }
+    __syncthreads();
+    if (t < j1 - j0) {
+        const int j = j0 + t;
+        const int a = f.memberStart[j] - p0, e = f.memberStart[j + 1] - p0;
+        double acc = 0.0;
+        for (int q = a; q < e; q++) acc += diag[q];
+        c.AC[j] = acc;
+        if (c.AC32) c.AC32[j] = (float)acc;
+    }
+    for (int i = c.start[j0] + t; i < c.start[j1]; i += blockDim.x) {
+        const int q0 = f.cslotStart[i], nq = f.cslotStart[i + 1] - q0;
+        double a = 0.0;
+        if (nq > 0 && nq <= 4) {
+            int fs[4];
+            double v[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) fs[q] = f.cslotFine[slotAt(q0, nq, q)];
+#pragma unroll
+            for (int q = 0; q < 4; q++) v[q] = f.AF[fs[q]];
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if (q < nq) a += v[q];
+        } else {
+            for (int q = q0; q < q0 + nq; q++) a += f.AF[f.cslotFine[q]];
+        }
+        c.AF[i] = a;
+        if (c.AF32) c.AF32[i] = (float)a;
+    }
+}
+
 // computeResidual (residualAll sign = true, residuals.cuh:17-60) into f.res,
 // one thread per fine cell; kMGRestrictSum then gathers it per coarse cell.
 // B > 0: rows of at most B slots load their slots, then their neighbours, in
@@ -1991,6 +2060,13 @@ void launchMGTail(hipStream_t s, const TailLaunch& t, double w, int preDone, int
 }
 void launchMGCoarseDiag(hipStream_t s, const DLevel& f, const Ctl* ctl) {
     kMGCoarseDiag<<<blocksFor(f.n), kBlock, 0, s>>>(f, ctl);
+}
+void launchMGCoarseOperator(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl) {
+    if (f.nAggBlocks <= 0) throw std::logic_error("level has no aggregate blocks");
+    if (f.maxRow <= 4) kMGCoarseOperator<4><<<f.nAggBlocks, kBlock, 0, s>>>(f, c, ctl);
+    else if (f.maxRow <= 8) kMGCoarseOperator<8><<<f.nAggBlocks, kBlock, 0, s>>>(f, c, ctl);
+    else if (f.maxRow <= 16) kMGCoarseOperator<16><<<f.nAggBlocks, kBlock, 0, s>>>(f, c, ctl);
+    else kMGCoarseOperator<0><<<f.nAggBlocks, kBlock, 0, s>>>(f, c, ctl);
 }
 void launchMGCoarseGather(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl) {
     kMGCoarseGather<<<blocksFor((long)c.refs + c.n), kBlock, 0, s>>>(f, c, ctl);

@@ -498,6 +498,10 @@ void MGDevice::enqueueOperators(hipStream_t s, const MGParams& p, const Ctl* ctl
 // multigrid.cu:350-389), each in two coalesced passes.
 void MGDevice::buildOperators(hipStream_t s, const Ctl* ctl) {
     for (size_t l = 1; l < dev.size(); l++) {
+        if (dev[l - 1].nAggBlocks > 0) {
+            launchMGCoarseOperator(s, dev[l - 1], dev[l], ctl);
+            continue;
+        }
         launchMGCoarseDiag(s, dev[l - 1], ctl);
         launchMGCoarseGather(s, dev[l - 1], dev[l], ctl);
     }
@@ -543,8 +547,9 @@ double MGDevice::bytesPerSolve(const MGParams& p, bool buildOperators) const {
             const Level& c = host[l];
             const double X = (double)f.cslotFine.size();
             // diagonals (AC, start, nb/AF/slot map; one diagonal per fine cell written
-            // and re-read) + ordered gathers + coarse writes
-            bytes += 32.0 * f.n + 16.0 * f.refs() + 12.0 * X + (12.0 + w32) * c.n + (12.0 + w32) * c.refs();
+            // and re-read unless fused) + ordered gathers + coarse writes
+            bytes += (dev[l - 1].nAggBlocks > 0 ? 16.0 : 32.0) * f.n + 16.0 * f.refs() + 12.0 * X +
+                     (12.0 + w32) * c.n + (12.0 + w32) * c.refs();
         }
         if (mixed && p.coarseDirect) bytes += 8.0 * 3.0 * host.back().n * host.back().n;
     }
