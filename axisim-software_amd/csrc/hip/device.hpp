@@ -33,6 +33,7 @@ struct DSweepBlocks {
     const int* haloStart = nullptr;
     const int* ringEnd = nullptr;
     const int* halo = nullptr;
+    const int* own = nullptr;          // non-null: block b's cells are own[b*cells ..] (-1: none)
     const int16_t* locOwn = nullptr;   // LDS slot of each slot's neighbour (block rows)
     const int16_t* locHalo = nullptr;  // the same for halo rows, kSweepRegSlots per entry
 };

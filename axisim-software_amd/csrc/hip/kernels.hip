@@ -749,8 +749,9 @@ __global__ void __launch_bounds__(C) kJacobiUVBlock(DMesh m, DState s, DSweepBlo
     // 1 .. S - 2: the ones sweeps 2 .. S - 1 update again, from LDS
     const int nHalo1 = (S > 1) ? sb.ringEnd[blk * sb.rings + S - 2] : 0;
     const int nHaloLds = (S > 2) ? sb.ringEnd[blk * sb.rings + (S > 2 ? S - 3 : 0)] : 0;
-    const bool mine = t < nOwn, haloRow = t < nHalo1;
-    const int n = a + t;
+    // block cell t: a + t, or the t-th cell of the block's patch
+    const int n = sb.own ? sb.own[(size_t)blk * C + t] : a + t;
+    const bool mine = sb.own ? n >= 0 : t < nOwn, haloRow = t < nHalo1;
     const int hc = haloRow ? sb.halo[h0 + t] : 0;
     SweepRow<R> own, hal;
     if (mine) loadSweepRow(own, m, s, n);
@@ -1571,8 +1572,8 @@ __global__ void __launch_bounds__(C) kMGLevel0Block(DLevel f, DLevel c, DSweepBl
     constexpr int kRings1 = S - 1 + (RES ? 1 : 0);
     const int nHalo1 = kRings1 > 0 ? sb.ringEnd[blk * sb.rings + kRings1 - 1] : 0;
     const int nHaloLds = (kRings1 > 1) ? sb.ringEnd[blk * sb.rings + (kRings1 > 1 ? kRings1 - 2 : 0)] : 0;
-    const bool mine = t < nOwn, haloRow = t < nHalo1;
-    const int n = a + t;
+    const int n = sb.own ? sb.own[(size_t)blk * C + t] : a + t;
+    const bool mine = sb.own ? n >= 0 : t < nOwn, haloRow = t < nHalo1;
     const int hc = haloRow ? sb.halo[h0 + t] : 0;
     MGRowReg<R> own, hal;
     if (mine) loadMGRow(own, f, n);

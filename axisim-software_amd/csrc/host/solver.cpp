@@ -123,6 +123,8 @@ void MGDevice::build(int n, const std::vector<int>& start, const std::vector<int
     for (int cells : {1024, 512}) {
         if (fuseL0_ || host.size() < 2 || host[0].n < kFuseL0MinCells) break;
         buildSweepBlocks(host[0].n, host[0].start, host[0].nb, cells, 2, kSweepRegSlots, l0Blocks_);
+        if (l0Blocks_.maxSlots <= kSweepRegSlots && l0Blocks_.maxHalo > sweepHaloMax(cells))
+            buildSweepBlocks(host[0].n, host[0].start, host[0].nb, cells, 2, kSweepRegSlots, l0Blocks_, true);
         if (l0Blocks_.maxSlots <= kSweepRegSlots && l0Blocks_.maxHalo <= sweepHaloMax(cells)) {
             dl0_.cells = l0Blocks_.cells;
             dl0_.rings = l0Blocks_.rings;
@@ -130,6 +132,7 @@ void MGDevice::build(int n, const std::vector<int>& start, const std::vector<int
             dl0_.haloStart = arena.upload(l0Blocks_.haloStart);
             dl0_.ringEnd = arena.upload(l0Blocks_.ringEnd);
             dl0_.halo = arena.upload(l0Blocks_.halo.empty() ? std::vector<int>{0} : l0Blocks_.halo);
+            dl0_.own = l0Blocks_.patches ? arena.upload(l0Blocks_.own) : nullptr;
             dl0_.locOwn = arena.upload(l0Blocks_.locOwn);
             dl0_.locHalo = arena.upload(l0Blocks_.locHalo.empty() ? std::vector<int16_t>{-1} : l0Blocks_.locHalo);
             fuseL0_ = true;
@@ -1478,8 +1481,12 @@ void Solver::prepareSweepBlocks() {
         for (int cells : {1024, 512}) {
         if (fuseCellsReq_ != 0 && cells != fuseCellsReq_) continue;
         if (sweepBlocks_.cells != cells || sweepBlocks_.rings != s - 1) {
+            // index ranges first (they keep each block's rows contiguous), else
+            // breadth-first patches
             SweepBlocks sb;
             buildSweepBlocks(mesh_.nCells, csrStart_, csrNb_, cells, s - 1, kSweepRegSlots, sb);
+            if (sb.maxSlots <= kSweepRegSlots && sb.maxHalo > sweepHaloMax(cells))
+                buildSweepBlocks(mesh_.nCells, csrStart_, csrNb_, cells, s - 1, kSweepRegSlots, sb, true);
             if (sb.maxSlots > kSweepRegSlots || sb.maxHalo > sweepHaloMax(cells)) continue;
             sweepBlocks_ = std::move(sb);
             dropGraph();  // it holds the old blocks' addresses
@@ -1491,6 +1498,7 @@ void Solver::prepareSweepBlocks() {
             dsb_.haloStart = sweepArena_.upload(sweepBlocks_.haloStart);
             dsb_.ringEnd = sweepArena_.upload(sweepBlocks_.ringEnd);
             dsb_.halo = sweepArena_.upload(sweepBlocks_.halo.empty() ? std::vector<int>{0} : sweepBlocks_.halo);
+            dsb_.own = sweepBlocks_.patches ? sweepArena_.upload(sweepBlocks_.own) : nullptr;
             dsb_.locOwn = sweepArena_.upload(sweepBlocks_.locOwn);
             dsb_.locHalo = sweepArena_.upload(sweepBlocks_.locHalo.empty() ? std::vector<int16_t>{-1}
                                                                            : sweepBlocks_.locHalo);

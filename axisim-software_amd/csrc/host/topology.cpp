@@ -126,24 +126,75 @@ void cellNeighbourCSR(const HostMesh& m, std::vector<int>& start, std::vector<in
 }
 
 void buildSweepBlocks(int nCells, const std::vector<int>& start, const std::vector<int>& nb, int cells, int rings,
-                      int rowSlots, SweepBlocks& out) {
+                      int rowSlots, SweepBlocks& out, bool patches) {
     if (cells <= 0 || rings < 1 || rowSlots < 1) throw std::invalid_argument("bad sweep block parameters");
     out = SweepBlocks{};
     out.cells = cells;
     out.rings = rings;
     out.rowSlots = rowSlots;
+    out.patches = patches;
     out.nBlocks = (nCells + cells - 1) / cells;
     out.haloStart.assign(1, 0);
     out.ringEnd.reserve((size_t)out.nBlocks * rings);
     out.locOwn.assign(start[nCells], (int16_t)-1);
     for (int c = 0; c < nCells; c++) out.maxSlots = std::max(out.maxSlots, start[c + 1] - start[c]);
+    // the blocks' cells: index ranges, or breadth-first patches of `cells`
+    // cells each (every patch full but the last; a patch continues from the
+    // next unassigned cell when its region runs out)
+    std::vector<int> blockOf(nCells, -1), members;
+    if (patches) {
+        out.own.assign((size_t)out.nBlocks * cells, -1);
+        std::vector<int> queue;
+        int seed = 0, b = 0, filled = 0;
+        size_t qh = 0;
+        queue.reserve(nCells);
+        while (filled < nCells) {
+            if (qh == queue.size()) {
+                while (blockOf[seed] >= 0) seed++;
+                blockOf[seed] = b;
+                queue.push_back(seed);
+            }
+            const int c = queue[qh++];
+            out.own[(size_t)b * cells + (filled - b * cells)] = c;
+            if (++filled % cells == 0 || filled == nCells) {
+                // the patch is full: the rest of the queue goes back to the pool
+                for (size_t i = qh; i < queue.size(); i++) {
+                    blockOf[queue[i]] = -1;
+                    seed = std::min(seed, queue[i]);
+                }
+                queue.clear();
+                qh = 0;
+                std::sort(out.own.begin() + (size_t)b * cells, out.own.begin() + (size_t)b * cells + (filled - b * cells));
+                b++;
+                continue;
+            }
+            for (int k = start[c]; k < start[c + 1]; k++) {
+                const int o = nb[k];
+                if (o >= 0 && blockOf[o] < 0) {
+                    blockOf[o] = b;
+                    queue.push_back(o);
+                }
+            }
+        }
+    } else {
+        for (int c = 0; c < nCells; c++) blockOf[c] = c / cells;
+    }
     std::vector<int> stamp(nCells, -1), where(nCells, -1), frontier, ring;
     for (int b = 0; b < out.nBlocks; b++) {
         const int a = b * cells, e = std::min(nCells, a + cells);
         const size_t h0 = out.halo.size();
+        members.clear();
+        if (patches) {
+            for (int i = 0; i < cells; i++)
+                if (out.own[(size_t)b * cells + i] >= 0) members.push_back(out.own[(size_t)b * cells + i]);
+        } else {
+            for (int c = a; c < e; c++) members.push_back(c);
+        }
         frontier.clear();
-        for (int c = a; c < e; c++) {
+        for (size_t i = 0; i < members.size(); i++) {
+            const int c = members[i];
             stamp[c] = b;
+            where[c] = (int)i;  // LDS slot of a block cell: its position in the block
             frontier.push_back(c);
         }
         for (int r = 1; r <= rings; r++) {
@@ -164,12 +215,10 @@ void buildSweepBlocks(int nCells, const std::vector<int>& start, const std::vect
         const int nh = (int)(out.halo.size() - h0);
         out.maxHalo = std::max(out.maxHalo, nh);
         for (int i = 0; i < nh; i++) where[out.halo[h0 + i]] = cells + i;
-        auto local = [&](int o) -> int16_t {
-            if (o < 0) return -1;
-            if (o >= a && o < e) return (int16_t)(o - a);
-            return (int16_t)where[o];  // >= cells: every neighbour read from LDS is listed
-        };
-        for (int c = a; c < e; c++)
+        // block cells sit at their position in the block, halo cells after
+        // the block: every neighbour a later sweep reads from LDS is listed
+        auto local = [&](int o) -> int16_t { return o < 0 ? (int16_t)-1 : (int16_t)where[o]; };
+        for (int c : members)
             for (int k = start[c]; k < start[c + 1]; k++) out.locOwn[k] = local(nb[k]);
         // rows of rings 1 .. rings - 1, which the sweeps after the first read from LDS
         const int inner = rings >= 2 ? out.ringEnd[(size_t)b * rings + rings - 2] : 0;
@@ -181,6 +230,7 @@ void buildSweepBlocks(int nCells, const std::vector<int>& start, const std::vect
             }
         }
         for (int i = 0; i < nh; i++) where[out.halo[h0 + i]] = -1;
+        for (int c : members) where[c] = -1;
         out.haloStart.push_back((int)out.halo.size());
     }
 }

@@ -44,6 +44,10 @@ void cellNeighbourCSR(const HostMesh& m, std::vector<int>& start, std::vector<in
 // neighbour's LDS slot (-1: boundary slot or past the row's slots).
 struct SweepBlocks {
     int cells = 0, rings = 0, nBlocks = 0, rowSlots = 0;
+    // patches: the blocks are breadth-first patches, not index ranges; own
+    // lists each block's cells (cells entries per block, ascending, -1 padded)
+    bool patches = false;
+    std::vector<int> own;
     int maxHalo = 0;   // largest halo of any block
     int maxSlots = 0;  // most slots of any cell
     std::vector<int> haloStart;  // nBlocks + 1 offsets into halo
@@ -53,9 +57,12 @@ struct SweepBlocks {
     std::vector<int16_t> locHalo;  // per halo entry, rowSlots slots (rings 1 .. rings - 1)
 };
 // rowSlots: slots per halo row in locHalo (cells with more slots leave the
-// fused path; maxSlots tells).
+// fused path; maxSlots tells).  patches: grow each block breadth-first from
+// the lowest-numbered cell not yet in a block (for meshes whose index ranges
+// are long thin strips, e.g. structured grids numbered row by row), instead
+// of taking consecutive cells.
 void buildSweepBlocks(int nCells, const std::vector<int>& start, const std::vector<int>& nb, int cells, int rings,
-                      int rowSlots, SweepBlocks& out);
+                      int rowSlots, SweepBlocks& out, bool patches = false);
 
 // Greedy smallest-free colouring in cell order, colour-major order
 // (buildMeshColoring, memory_manger.cpp:334-409).  Returns the colour count.

@@ -74,3 +74,15 @@ def test_fused_sweeps_quad_mesh_bit_exact(require_gpu):
     assert s.stats().sweeps_per_launch == 3
     for name in FIELDS:
         np.testing.assert_array_equal(s.field(name), ref.field(name), err_msg=name)
+
+
+def test_fused_sweeps_patch_blocks_bit_exact(require_gpu):
+    """Rows of 640 cells: consecutive-index blocks are strips whose halos do
+    not fit, so the blocks are breadth-first patches; fused against one sweep
+    per launch, bit for bit."""
+    case = axs.pipe_case(nz=640, nr=40, max_iter=6, tol=0.0)
+    ref = run(case, 1)
+    s = run(case, 3, 0)
+    assert s.stats().sweeps_per_launch == 3
+    for name in FIELDS:
+        np.testing.assert_array_equal(s.field(name), ref.field(name), err_msg=name)

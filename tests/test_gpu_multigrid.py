@@ -140,3 +140,14 @@ def test_fused_level0_quad_mesh_bit_exact(require_gpu):
     xg = axs.Multigrid(fs, fn).solve(AC, AF, b, x0, cycles=3)
     xr = OracleMG(fs, fn).solve(AC, AF, b, x0, cycles=3)
     np.testing.assert_array_equal(xg, xr)
+
+
+def test_fused_level0_patch_blocks_bit_exact(require_gpu):
+    """Level 0 numbered in rows of 640 cells (breadth-first patch blocks) against
+    the oracle, bit for bit."""
+    case = axs.pipe_case(nz=640, nr=40)
+    fs, fn, AC, AF, b = pp_system(case, iters=2)
+    x0 = np.linspace(-1.0, 1.0, b.size) * 1e-3
+    xg = axs.Multigrid(fs, fn).solve(AC, AF, b, x0, cycles=3)
+    xr = OracleMG(fs, fn).solve(AC, AF, b, x0, cycles=3)
+    np.testing.assert_array_equal(xg, xr)
