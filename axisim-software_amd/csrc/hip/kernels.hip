@@ -674,11 +674,12 @@ __global__ void __launch_bounds__(kBlock) kJacobiUV(DMesh m, DState s, UVIn xo, 
 }
 
 // S Jacobi sweeps of u and v in one launch (temporal blocking).  Workgroup b
-// owns the C cells [bC, bC + C); sweep j updates them and the halo rings
-// 1 .. S - j, the iterates of sweeps 1 .. S - 1 live in LDS, and only sweep S
-// writes global memory.  Thread t holds two rows in registers for the whole
-// launch: block cell bC + t and halo entry t (a halo has at most C / 2 cells),
-// each as b, 1/AC and up to kSweepRegSlots off-diagonals with their
+// owns C cells (the index range [bC, bC + C), or a breadth-first patch the
+// host lists); sweep j updates them and the halo rings 1 .. S - j, the
+// iterates of sweeps 1 .. S - 1 live in LDS, and only sweep S writes global
+// memory.  Thread t holds two rows in registers for the whole launch: block
+// cell t and halo entry t (a halo has at most sweepHalo<C>() cells), each as
+// b, 1/AC and up to kSweepRegSlots off-diagonals with their
 // neighbours, all loaded up front, so the CSR and coefficients are read once
 // per S sweeps and no sweep waits on a chain of global loads.  Each slot's
 // neighbour LDS slot comes from the host-built tables (int16, loaded with the
@@ -733,8 +734,10 @@ __device__ __forceinline__ double2 sweepRow(const SweepRow<R>& r, XAt xat, doubl
     return make_double2((r.iu != 0.0) ? au * r.iu : self.x, (r.iv != 0.0) ? av * r.iv : self.y);
 }
 
+// halo entries a block may have: at most one per thread (thread t holds halo
+// row t), and the LDS of two iterate buffers within the static 64 KiB
 template <int C>
-constexpr int sweepHalo() { return C / 2; }
+constexpr int sweepHalo() { return C <= 512 ? C : C / 2; }
 
 template <int C, int S, int R>
 __global__ void __launch_bounds__(C) kJacobiUVBlock(DMesh m, DState s, DSweepBlocks sb, UVIn xo, UVOut xn) {
