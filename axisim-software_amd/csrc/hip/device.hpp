@@ -88,15 +88,30 @@ struct Physics {
 };
 
 // One multigrid level on the device (MultigridLevel, multigrid.cuh:60-106)
+// Per-thread entry of the aggregate workgroups below: member m of the run
+// (-1: none) with its row's slot range, and, for coarse cell j0 + t of the
+// run, where its members' values sit in the workgroup's LDS (sum = a | e << 16,
+// the range [a, e); -1: none).  Built once on the host so that a thread reaches
+// its row in one load instead of a chain through the run and member lists.
+struct alignas(16) AggThread {
+    int m, k0, ns, sum;
+};
+// Per-workgroup entry: coarse cells [j0, j1), their coarse slots [cs0, cs1).
+struct alignas(16) AggBlock {
+    int j0, j1, cs0, cs1;
+};
+
 struct DLevel {
     int n = 0, refs = 0, nCoarse = 0;
     int block = kBlock;  // workgroup size of this level's per-cell kernels (autotuned)
     int maxRow = 0;      // most off-diagonal slots of a row
     int maxMembers = 0;  // most fine cells of one coarse cell (levels with a coarser one)
     int uniformRow = 0;  // > 0: every row has this many slots (start[n] = uniformRow * n)
-    // workgroups of kMGResidualRestrict: coarse-cell ranges whose members fit
-    // one workgroup (nAggBlocks + 1 entries; 0 blocks: not built)
-    const int* aggBlocks = nullptr;
+    // workgroups of kMGResidualRestrict and kMGCoarseOperator: runs of whole
+    // coarse cells whose members fit one workgroup of kBlock threads
+    // (nAggBlocks entries, nAggBlocks * kBlock thread entries; 0: not built)
+    const AggBlock* aggBlocks = nullptr;
+    const AggThread* aggThreads = nullptr;
     int nAggBlocks = 0;
     int *start = nullptr, *nb = nullptr;
     double *AC = nullptr, *b = nullptr, *AF = nullptr;

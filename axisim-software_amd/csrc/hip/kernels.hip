@@ -1273,22 +1273,59 @@ __device__ __forceinline__ T sweepFromZero(T b, T AC, T w) {
 }  // namespace
 
 // kMGCoarseDiag and kMGCoarseGather in one launch, over the same workgroups
-// of whole aggregates as kMGResidualRestrict: thread t forms the diagonal of
-// member p0 + t (kMGCoarseDiag's sum) into LDS; after the barrier thread t sums
-// coarse cell aggStart[b] + t's members from LDS, and the workgroup's threads
-// stride over the coarse slots of its coarse cells (kMGCoarseGather's sums).
+// of whole aggregates as kMGResidualRestrict (f.aggBlocks / f.aggThreads):
+// the workgroup's threads stride over the coarse slots [cs0, cs1) of its
+// coarse cells (kMGCoarseGather's sums); thread t forms the diagonal of the
+// run's member t (kMGCoarseDiag's sum) into LDS, and after the barrier sums
+// coarse cell j0 + t's members from LDS.
 template <int B>
 __global__ void __launch_bounds__(kBlock) kMGCoarseOperator(DLevel f, DLevel c, const Ctl* ctl) {
     __shared__ double diag[kBlock];
     if (finished(ctl)) return;
     const int blk = xcdBlock(), t = threadIdx.x;
-    const int j0 = f.aggBlocks[blk], j1 = f.aggBlocks[blk + 1];
-    const int p0 = f.memberStart[j0], np = f.memberStart[j1] - p0;
-    if (t < np) {
-        const int n = f.members[p0 + t];
+    const AggThread me = f.aggThreads[(size_t)blk * kBlock + t];
+    const AggBlock ab = f.aggBlocks[blk];
+    // The coarse off-diagonals first: they do not read the diagonals, so their
+    // loads overlap the diagonal phase below.  Two coarse slots per thread and
+    // round, the first four fine slots of each loaded at once (indices clamped
+    // into range), summed in ascending fine slot order, longer sums continued
+    // by the loop after.
+    for (int i = ab.cs0 + t; i < ab.cs1; i += 2 * kBlock) {
+        const bool two = i + kBlock < ab.cs1;
+        const int ic[2] = {i, two ? i + kBlock : i};
+        int q0[2], nq[2], fs[2][4];
+        double v[2][4], a[2];
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+            q0[p] = f.cslotStart[ic[p]];
+            nq[p] = f.cslotStart[ic[p] + 1] - q0[p];
+        }
+#pragma unroll
+        for (int p = 0; p < 2; p++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) fs[p][q] = f.cslotFine[nq[p] > 0 ? slotAt(q0[p], nq[p], q) : 0];
+#pragma unroll
+        for (int p = 0; p < 2; p++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) v[p][q] = f.AF[fs[p][q]];
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+            a[p] = 0.0;
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if (q < nq[p]) a[p] += v[p][q];
+            for (int q = q0[p] + 4; q < q0[p] + nq[p]; q++) a[p] += f.AF[f.cslotFine[q]];
+        }
+        c.AF[i] = a[0];
+        if (c.AF32) c.AF32[i] = (float)a[0];
+        if (two) {
+            c.AF[i + kBlock] = a[1];
+            if (c.AF32) c.AF32[i + kBlock] = (float)a[1];
+        }
+    }
+    if (me.m >= 0) {
+        const int n = me.m, k0 = me.k0, ns = me.ns;
         double d = f.AC[n];
-        int k0, ns;
-        rowRange(f.start, f.uniformRow, n, k0, ns);
         if (B > 0 && ns > 0 && ns <= B) {
             // the row's slots in one round (clamped), summed in slot order
             int o[B > 0 ? B : 1], cs[B > 0 ? B : 1];
@@ -1312,32 +1349,13 @@ __global__ void __launch_bounds__(kBlock) kMGCoarseOperator(DLevel f, DLevel c, 
         diag[t] = d;
     }
     __syncthreads();
-    if (t < j1 - j0) {
-        const int j = j0 + t;
-        const int a = f.memberStart[j] - p0, e = f.memberStart[j + 1] - p0;
+    if (me.sum >= 0) {
+        const int j = ab.j0 + t;
+        const int a = me.sum & 0xffff, e = me.sum >> 16;
         double acc = 0.0;
         for (int q = a; q < e; q++) acc += diag[q];
         c.AC[j] = acc;
         if (c.AC32) c.AC32[j] = (float)acc;
-    }
-    for (int i = c.start[j0] + t; i < c.start[j1]; i += blockDim.x) {
-        const int q0 = f.cslotStart[i], nq = f.cslotStart[i + 1] - q0;
-        double a = 0.0;
-        if (nq > 0 && nq <= 4) {
-            int fs[4];
-            double v[4];
-#pragma unroll
-            for (int q = 0; q < 4; q++) fs[q] = f.cslotFine[slotAt(q0, nq, q)];
-#pragma unroll
-            for (int q = 0; q < 4; q++) v[q] = f.AF[fs[q]];
-#pragma unroll
-            for (int q = 0; q < 4; q++)
-                if (q < nq) a += v[q];
-        } else {
-            for (int q = q0; q < q0 + nq; q++) a += f.AF[f.cslotFine[q]];
-        }
-        c.AF[i] = a;
-        if (c.AF32) c.AF32[i] = (float)a;
     }
 }
 
@@ -1347,9 +1365,7 @@ __global__ void __launch_bounds__(kBlock) kMGCoarseOperator(DLevel f, DLevel c, 
 // one round each (indices clamped into the row); B = 0 or longer rows: the
 // plain loop.  The arithmetic is the same in either case.
 template <class T, int B>
-__device__ __forceinline__ T mgResidualRow(const DLevel& f, const LV<T>& v, int n) {
-    int k0, ns;
-    rowRange(f.start, f.uniformRow, n, k0, ns);
+__device__ __forceinline__ T mgResidualRowAt(const DLevel& f, const LV<T>& v, int n, int k0, int ns) {
     T Ax = v.AC[n] * v.x[n];
     if (B > 0 && ns > 0 && ns <= B) {
         int o[B > 0 ? B : 1];
@@ -1372,6 +1388,12 @@ __device__ __forceinline__ T mgResidualRow(const DLevel& f, const LV<T>& v, int 
         }
     }
     return v.b[n] - Ax;
+}
+template <class T, int B>
+__device__ __forceinline__ T mgResidualRow(const DLevel& f, const LV<T>& v, int n) {
+    int k0, ns;
+    rowRange(f.start, f.uniformRow, n, k0, ns);
+    return mgResidualRowAt<T, B>(f, v, n, k0, ns);
 }
 
 template <class T, int B>
@@ -1677,30 +1699,31 @@ __global__ void __launch_bounds__(kBlock) kMGRestrictSum(DLevel f, DLevel c, dou
 }
 
 // kMGResidual and kMGRestrictSum in one launch.  Workgroup b covers the coarse
-// cells [aggStart[b], aggStart[b + 1]); their members (at most kBlock, one run
-// of the member list) are its fine cells.  Thread t computes the residual of
-// member p0 + t with kMGResidual's expression into LDS; after the barrier
-// thread t sums coarse cell aggStart[b] + t's members from LDS in list order,
-// as kMGRestrictSum does.  The fine residual never goes to memory.
+// cells [j0, j1) of f.aggBlocks[b]; their members (at most kBlock, one run of
+// the member list) are its fine cells.  Thread t computes the residual of the
+// run's member t (its row range from f.aggThreads) with kMGResidual's
+// expression into LDS; after the barrier thread t sums coarse cell j0 + t's
+// members from LDS in list order, as kMGRestrictSum does.  The fine residual
+// never goes to memory.
 template <class T, int B>
-__global__ void __launch_bounds__(kBlock) kMGResidualRestrict(DLevel f, DLevel c, const int* __restrict__ aggStart,
-                                                              double w, int sweep1, const Ctl* ctl) {
+__global__ void __launch_bounds__(kBlock) kMGResidualRestrict(DLevel f, DLevel c, double w, int sweep1,
+                                                              const Ctl* ctl) {
     __shared__ T r[kBlock];
     if (finished(ctl)) return;
     const int blk = xcdBlock(), t = threadIdx.x;
-    const int j0 = aggStart[blk], j1 = aggStart[blk + 1];
-    const int p0 = f.memberStart[j0], np = f.memberStart[j1] - p0;
+    const AggThread me = f.aggThreads[(size_t)blk * kBlock + t];
+    const int j = f.aggBlocks[blk].j0 + t;
     const LV<T> fv = view<T>(f);
     const LV<T> cv = view<T>(c);
-    if (t < np) r[t] = mgResidualRow<T, B>(f, fv, f.members[p0 + t]);
+    const T acj = (sweep1 && me.sum >= 0) ? cv.AC[j] : T(0);  // read ahead of the barrier
+    if (me.m >= 0) r[t] = mgResidualRowAt<T, B>(f, fv, me.m, me.k0, me.ns);
     __syncthreads();
-    if (t < j1 - j0) {
-        const int j = j0 + t;
-        const int a = f.memberStart[j] - p0, e = f.memberStart[j + 1] - p0;
+    if (me.sum >= 0) {
+        const int a = me.sum & 0xffff, e = me.sum >> 16;
         T acc = T(0);
         for (int q = a; q < e; q++) acc += r[q];
         cv.b[j] = acc;
-        cv.x[j] = sweep1 ? sweepFromZero(acc, cv.AC[j], (T)w) : T(0);
+        cv.x[j] = sweep1 ? sweepFromZero(acc, acj, (T)w) : T(0);
     }
 }
 
@@ -2110,10 +2133,10 @@ template <class T>
 static void launchMGResidualRestrictT(hipStream_t s, const DLevel& f, const DLevel& c, double w, int sweep1,
                                       const Ctl* ctl) {
     const dim3 g(f.nAggBlocks), b(kBlock);
-    if (f.maxRow <= 4) kMGResidualRestrict<T, 4><<<g, b, 0, s>>>(f, c, f.aggBlocks, w, sweep1, ctl);
-    else if (f.maxRow <= 8) kMGResidualRestrict<T, 8><<<g, b, 0, s>>>(f, c, f.aggBlocks, w, sweep1, ctl);
-    else if (f.maxRow <= 16) kMGResidualRestrict<T, 16><<<g, b, 0, s>>>(f, c, f.aggBlocks, w, sweep1, ctl);
-    else kMGResidualRestrict<T, 0><<<g, b, 0, s>>>(f, c, f.aggBlocks, w, sweep1, ctl);
+    if (f.maxRow <= 4) kMGResidualRestrict<T, 4><<<g, b, 0, s>>>(f, c, w, sweep1, ctl);
+    else if (f.maxRow <= 8) kMGResidualRestrict<T, 8><<<g, b, 0, s>>>(f, c, w, sweep1, ctl);
+    else if (f.maxRow <= 16) kMGResidualRestrict<T, 16><<<g, b, 0, s>>>(f, c, w, sweep1, ctl);
+    else kMGResidualRestrict<T, 0><<<g, b, 0, s>>>(f, c, w, sweep1, ctl);
 }
 void launchMGResidualRestrict(hipStream_t s, const DLevel& f, const DLevel& c, double w, int sweep1, bool fp32,
                               const Ctl* ctl) {

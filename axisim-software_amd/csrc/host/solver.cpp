@@ -100,8 +100,27 @@ void MGDevice::build(int n, const std::vector<int>& start, const std::vector<int
             }
             agg.push_back(host[l + 1].n);
             if (fits && host[l + 1].n > 0) {
-                d.aggBlocks = arena.upload(agg);
-                d.nAggBlocks = (int)agg.size() - 1;
+                const Level& hc = host[l + 1];
+                const int nB = (int)agg.size() - 1;
+                std::vector<AggBlock> blocks(nB);
+                std::vector<AggThread> threads((size_t)nB * kBlock, AggThread{-1, 0, 0, -1});
+                for (int b = 0; b < nB; b++) {
+                    const int j0 = agg[b], j1 = agg[b + 1];
+                    const int p0 = h.memberStart[j0], np = h.memberStart[j1] - p0;
+                    blocks[b] = AggBlock{j0, j1, hc.start[j0], hc.start[j1]};
+                    AggThread* th = threads.data() + (size_t)b * kBlock;
+                    for (int t = 0; t < np; t++) {
+                        const int m = h.members[p0 + t];
+                        th[t].m = m;
+                        th[t].k0 = h.start[m];
+                        th[t].ns = h.start[m + 1] - h.start[m];
+                    }
+                    for (int t = 0; t < j1 - j0; t++)
+                        th[t].sum = (h.memberStart[j0 + t] - p0) | ((h.memberStart[j0 + t + 1] - p0) << 16);
+                }
+                d.aggBlocks = arena.upload(blocks);
+                d.aggThreads = arena.upload(threads);
+                d.nAggBlocks = nB;
             }
             d.nCoarse = host[l + 1].n;
             d.slotToCoarseSlot = arena.upload(h.slotToCoarseSlot);
@@ -536,6 +555,16 @@ void MGDevice::enqueueSolve(hipStream_t s, const MGParams& p, const Ctl* ctl, bo
 // Algorithmic bytes of one solve (distinct elements read + written, FP64 / int32).
 // Per launch: distinct elements read + written, values of size v (8 or 4),
 // int32 indices 4 B.  Mirrors the launches enqueueSolve makes.
+// Residual + restriction of level l: fine rows (AC, b, x, AF/nb) and their
+// range (start + members, or the fused kernel's 16-byte thread entry), the
+// residual written and re-read unless fused, memberStart unless fused, coarse
+// b and x.
+double MGDevice::restrictBytes(int l, double v) const {
+    const double n = host[l].n, S = host[l].refs(), nc = host[l + 1].n;
+    if (dev[l].nAggBlocks > 0) return (3.0 * v + 16.0) * n + (v + 4.0) * S + 2.0 * v * nc;
+    return (5.0 * v + 8.0) * n + (v + 4.0) * S + (2.0 * v + 4.0) * nc;
+}
+
 double MGDevice::bytesPerSolve(const MGParams& p, bool buildOperators) const {
     const bool mixed = p.precision == AXS_PREC_MIXED;
     const double v = mixed ? 4.0 : 8.0;
@@ -549,9 +578,10 @@ double MGDevice::bytesPerSolve(const MGParams& p, bool buildOperators) const {
             const Level& f = host[l - 1];
             const Level& c = host[l];
             const double X = (double)f.cslotFine.size();
-            // diagonals (AC, start, nb/AF/slot map; one diagonal per fine cell written
-            // and re-read unless fused) + ordered gathers + coarse writes
-            bytes += (dev[l - 1].nAggBlocks > 0 ? 16.0 : 32.0) * f.n + 16.0 * f.refs() + 12.0 * X +
+            // diagonals (AC and the row range -- start, or the fused kernel's
+            // 16-byte thread entry -- nb/AF/slot map; one diagonal per fine cell
+            // written and re-read unless fused) + ordered gathers + coarse writes
+            bytes += (dev[l - 1].nAggBlocks > 0 ? 24.0 : 32.0) * f.n + 16.0 * f.refs() + 12.0 * X +
                      (12.0 + w32) * c.n + (12.0 + w32) * c.refs();
         }
         if (mixed && p.coarseDirect) bytes += 8.0 * 3.0 * host.back().n * host.back().n;
@@ -574,7 +604,7 @@ double MGDevice::bytesPerSolve(const MGParams& p, bool buildOperators) const {
             const double nc = host[l + 1].n;
             cyc += gs(l, p.pre) + gs(l, p.post);
             // residual + restriction (the residual written and re-read unless fused)
-            cyc += ((dev[l].nAggBlocks > 0 ? 3.0 : 5.0) * v + 8.0) * n + (v + 4.0) * S + (2.0 * v + 4.0) * nc;
+            cyc += restrictBytes(l, v);
             cyc += (2.0 * v + 4.0) * n + v * nc;                                  // prolongation
         }
         return bytes + (p.cycles > 1 ? p.cycles : 1) * cyc;
@@ -602,11 +632,8 @@ double MGDevice::bytesPerSolve(const MGParams& p, bool buildOperators) const {
             cyc += sm(l, p.pre - preDone);
             const int s1 =
                 host[l + 1].coarsest() ? (!(mixed && p.coarseDirect) && p.coarseSweeps >= 1) : (p.pre >= 1);
-            // restriction: fine rows (AC, b, x, start, members) + the residual written and
-            // re-read (unless fused) + memberStart, coarse b and x (+ coarse AC for the
-            // folded sweep)
-            cyc += ((dev[l].nAggBlocks > 0 ? 3.0 : 5.0) * v + 8.0) * n + (v + 4.0) * S + (2.0 * v + 4.0) * nc +
-                   (s1 ? v * nc : 0.0);
+            // restriction (+ coarse AC for the folded sweep)
+            cyc += restrictBytes(l, v) + (s1 ? v * nc : 0.0);
             if (p.post >= 1) cyc += (4.0 * v + 8.0) * n + (v + 4.0) * S + v * nc + sm(l, p.post - 1);
             else cyc += (2.0 * v + 4.0) * n + v * nc;
             preDone = s1;

@@ -97,6 +97,7 @@ public:
     // their FP32 copies and the coarsest inverse in mixed precision).
     void enqueueOperators(hipStream_t s, const MGParams& p, const Ctl* ctl);
     double bytesPerSolve(const MGParams& p, bool buildOperators) const;
+    double restrictBytes(int l, double v) const;
     int coarsestCells() const { return host.empty() ? 0 : host.back().n; }
     // first level of the LDS-resident tail (-1: no tail) and its LDS bytes
     int tailLevel(bool fp32) const { return (fp32 ? tail32_ : tail64_).lt; }
