@@ -172,7 +172,8 @@ void launchGSColorSys(hipStream_t s, const DMesh& m, const double* b, const doub
                       const int* order, int begin, int count, const Ctl* ctl);
 void launchPPCoeff(hipStream_t s, const DMesh& m, DBC bp, DState st, Physics ph);
 void launchFaceFlux(hipStream_t s, const DMesh& m, DBC bu, DBC bv, DBC bp, DState st, Physics ph);
-void launchPPRhs(hipStream_t s, const DMesh& m, DState st, Physics ph, int cross);
+// zero: when non-null, also zero[n] = 0 for every cell (p' before the solve)
+void launchPPRhs(hipStream_t s, const DMesh& m, DState st, Physics ph, int cross, double* zero = nullptr);
 void launchGradPPUpdate(hipStream_t s, const DMesh& m, DBC ppbc, DState st, Physics ph);
 void launchMassFluxUpdate(hipStream_t s, const DMesh& m, DBC bp, DState st, Physics ph);
 void launchResidual(hipStream_t s, const DMesh& m, DState st, int blocks);

@@ -917,13 +917,16 @@ __global__ void __launch_bounds__(kBlock) kFaceFlux(DMesh m, DBC bu, DBC bv, DBC
 }
 
 // createPPRhs, simple.cu:90-142
-// CROSS: with the non-orthogonal cross term (the corrector pass)
+// CROSS: with the non-orthogonal cross term (the corrector pass).
+// zero: when set, also p' = 0 there (the solve's initial guess,
+// the memset at solver.cpp:1193), saving a fill launch.
 template <int B, bool CROSS>
-__global__ void __launch_bounds__(kBlock) kPPRhs(DMesh m, DState s, Physics ph) {
+__global__ void __launch_bounds__(kBlock) kPPRhs(DMesh m, DState s, Physics ph, double* __restrict__ zero) {
     constexpr bool cross = CROSS;
     if (finished(s.ctl)) return;
     const int n = xcdBlock() * blockDim.x + threadIdx.x;
     if (n >= m.N) return;
+    if (zero) zero[n] = 0.0;
     double imb = 0.0, cs = 0.0;
     int k0, ns;
     rowRange(m.faceStart, m.uniformSlots, n, k0, ns);
@@ -2245,14 +2248,14 @@ void launchPPCoeff(hipStream_t s, const DMesh& m, DBC bp, DState st, Physics ph)
 void launchFaceFlux(hipStream_t s, const DMesh& m, DBC bu, DBC bv, DBC bp, DState st, Physics ph) {
     kFaceFlux<<<blocksFor(m.F), kBlock, 0, s>>>(m, bu, bv, bp, st, ph);
 }
-void launchPPRhs(hipStream_t s, const DMesh& m, DState st, Physics ph, int cross) {
+void launchPPRhs(hipStream_t s, const DMesh& m, DState st, Physics ph, int cross, double* zero) {
     const dim3 g(blocksFor(m.N)), b(kBlock);
     if (m.maxSlots <= 3) {
-        if (cross) kPPRhs<3, true><<<g, b, 0, s>>>(m, st, ph);
-        else kPPRhs<3, false><<<g, b, 0, s>>>(m, st, ph);
+        if (cross) kPPRhs<3, true><<<g, b, 0, s>>>(m, st, ph, zero);
+        else kPPRhs<3, false><<<g, b, 0, s>>>(m, st, ph, zero);
     } else {
-        if (cross) kPPRhs<4, true><<<g, b, 0, s>>>(m, st, ph);
-        else kPPRhs<4, false><<<g, b, 0, s>>>(m, st, ph);
+        if (cross) kPPRhs<4, true><<<g, b, 0, s>>>(m, st, ph, zero);
+        else kPPRhs<4, false><<<g, b, 0, s>>>(m, st, ph, zero);
     }
 }
 void launchGradPPUpdate(hipStream_t s, const DMesh& m, DBC ppbc, DState st, Physics ph) {

@@ -340,17 +340,37 @@ axs_status axs_mg_bench(axs_mg* h, const double* AC, const double* AF, const dou
         AXS_HIP(hipSetDevice(h->device));
         mgUpload(h, AC, AF, b, nullptr);
         h->mg.enqueueSolve(h->stream, p, nullptr, true);  // warm-up
-        hipEvent_t e0, e1;
-        AXS_HIP(hipEventCreate(&e0));
-        AXS_HIP(hipEventCreate(&e1));
-        AXS_HIP(hipEventRecord(e0, h->stream));
-        for (int r = 0; r < repeats; r++) h->mg.enqueueSolve(h->stream, p, nullptr, true);
-        AXS_HIP(hipEventRecord(e1, h->stream));
-        AXS_HIP(hipEventSynchronize(e1));
+        // timed as the solver runs it: one solve (operators + cycles) captured
+        // in a graph and replayed
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        AXS_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        try {
+            h->mg.enqueueSolve(h->stream, p, nullptr, true);
+        } catch (...) {
+            (void)hipStreamEndCapture(h->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            (void)hipGetLastError();
+            throw;
+        }
+        AXS_HIP(hipStreamEndCapture(h->stream, &g));
+        const hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        AXS_HIP(ie);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
         float f = 0.0f;
-        AXS_HIP(hipEventElapsedTime(&f, e0, e1));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
+        hipError_t err = hipGraphLaunch(ge, h->stream);  // warm-up of the graph
+        if (err == hipSuccess) err = hipEventCreate(&e0);
+        if (err == hipSuccess) err = hipEventCreate(&e1);
+        if (err == hipSuccess) err = hipEventRecord(e0, h->stream);
+        for (int r = 0; r < repeats && err == hipSuccess; r++) err = hipGraphLaunch(ge, h->stream);
+        if (err == hipSuccess) err = hipEventRecord(e1, h->stream);
+        if (err == hipSuccess) err = hipEventSynchronize(e1);
+        if (err == hipSuccess) err = hipEventElapsedTime(&f, e0, e1);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        (void)hipGraphExecDestroy(ge);
+        AXS_HIP(err);
         AXS_HIP(hipGetLastError());
         *ms = f / repeats;
         *bytes = h->mg.bytesPerSolve(p, true);

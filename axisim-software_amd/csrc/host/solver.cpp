@@ -989,14 +989,15 @@ void Solver::enqueueIteration(hipStream_t s) {
 
     launchPPCoeff(s, dm_, bp, st, ph);
     launchFaceFlux(s, dm_, bu, bv, bp, st, ph);
+    // p' = 0 before the solve (unless the learned guess supplies it): written
+    // by the first right-hand-side launch
     const bool guess = c.use_multigrid && net_ != nullptr && netLevel_ > 0;
-    if (!guess) launchFill(s, st.pp, 0.0, dm_.N, st.ctl);
 
     const MGParams mp = mgParamsOf(c);
     for (int corr = 0; corr <= ph.nonOrtho; corr++) {
         const int cross = corr > 0 ? 1 : 0;
         if (cross) launchGradField(s, dm_, ppbc_, st.pp, st.gPZ, st.gPR, ph.gradScheme, st.ctl);
-        launchPPRhs(s, dm_, st, ph, cross);
+        launchPPRhs(s, dm_, st, ph, cross, (corr == 0 && !guess) ? st.pp : nullptr);
         if (c.use_multigrid && corr == 0 && guess) {
             // operators, then the learned guess in place of p' = 0, then the cycles
             enqueueGuess(s, mp);

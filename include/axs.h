@@ -274,9 +274,12 @@ axs_status axs_mg_get_maps(const axs_mg* mg, int level, int* face_start, int* fa
 axs_status axs_mg_solve(axs_mg* mg, const double* AC, const double* AF, const double* b,
                         double* x, int cycles, double weight, int pre_sweeps, int post_sweeps,
                         int coarse_sweeps, int precision, int coarse_direct, int smoother);
-/* Benchmark form: the system is uploaded once, then `repeats` solves of
- * `cycles` V-cycles are timed with HIP events (x restarts from zero each solve).
- * *ms = average time of one solve; *bytes = algorithmic bytes of one solve. */
+/* Benchmark form: the system is uploaded once (x = 0), one solve of `cycles`
+ * V-cycles (coarse operators included) is captured in a HIP graph, as the
+ * solver's iteration graph runs it, and `repeats` replays are timed with HIP
+ * events; each solve continues from the x the previous one left (the work does
+ * not depend on x).  *ms = average time of one solve; *bytes = algorithmic
+ * bytes of one solve. */
 axs_status axs_mg_bench(axs_mg* mg, const double* AC, const double* AF, const double* b,
                         int cycles, double weight, int pre_sweeps, int post_sweeps,
                         int coarse_sweeps, int precision, int coarse_direct, int smoother,
