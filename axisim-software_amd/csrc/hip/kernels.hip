@@ -701,10 +701,15 @@ __device__ __forceinline__ void loadSweepRow(SweepRow<R>& r, const DMesh& m, con
     rowRange(m.faceStart, m.uniformSlots, c, k0, ns);
     r.k0 = k0;
     r.ns = ns;
+    // slots read at indices clamped into the row (index 0 for an empty row),
+    // then masked: unconditional loads go out together
 #pragma unroll
     for (int q = 0; q < R; q++) {
-        r.nb[q] = (q < ns) ? m.slotNb[k0 + q] : -1;
-        r.af[q] = (q < ns) ? s.AFuv[k0 + q] : 0.0;
+        const int k = ns > 0 ? k0 + min(q, ns - 1) : 0;
+        const int nb = m.slotNb[k];
+        const double af = s.AFuv[k];
+        r.nb[q] = (q < ns) ? nb : -1;
+        r.af[q] = (q < ns) ? af : 0.0;
     }
     r.bu = s.ub[c];
     r.bv = s.vb[c];
@@ -718,17 +723,21 @@ __device__ __forceinline__ void loadLoc(Row& r, const int16_t* l, int R) {
     for (int q = 0; q < R; q++) r.loc[q] = (q < r.ns) ? (int)l[q] : -1;
 }
 
-// jacobi's row (kJacobiUV) from registers; xat(q) = the neighbour of slot q;
-// LDS: the neighbours are the ones with an LDS slot (the same ones)
+// jacobi's row (kJacobiUV) from registers; xat(q) = the neighbour of slot q
+// (for a slot without one, any valid entry: it is read and not used, so that
+// every read is unconditional and they all go out together); LDS: the
+// neighbours are the ones with an LDS slot (the same ones)
 template <int R, bool LDS = false, typename XAt>
 __device__ __forceinline__ double2 sweepRow(const SweepRow<R>& r, XAt xat, double2 self) {
+    double2 xv[R];
+#pragma unroll
+    for (int q = 0; q < R; q++) xv[q] = xat(q);
     double au = r.bu, av = r.bv;
 #pragma unroll
     for (int q = 0; q < R; q++) {
         if (LDS ? r.loc[q] >= 0 : r.nb[q] >= 0) {
-            const double2 x = xat(q);
-            au -= r.af[q] * x.x;
-            av -= r.af[q] * x.y;
+            au -= r.af[q] * xv[q].x;
+            av -= r.af[q] * xv[q].y;
         }
     }
     return make_double2((r.iu != 0.0) ? au * r.iu : self.x, (r.iv != 0.0) ? av * r.iv : self.y);
@@ -739,8 +748,11 @@ __device__ __forceinline__ double2 sweepRow(const SweepRow<R>& r, XAt xat, doubl
 template <int C>
 constexpr int sweepHalo() { return C <= 512 ? C : C / 2; }
 
+// Triangle rows (R = 3) fit 64 VGPRs, i.e. two 1024-thread blocks per CU;
+// the attribute holds the compiler to that (longer rows take what they need).
 template <int C, int S, int R>
-__global__ void __launch_bounds__(C) kJacobiUVBlock(DMesh m, DState s, DSweepBlocks sb, UVIn xo, UVOut xn) {
+__global__ void __launch_bounds__(C) __attribute__((amdgpu_waves_per_eu(R <= 3 ? 8 : 1)))
+kJacobiUVBlock(DMesh m, DState s, DSweepBlocks sb, UVIn xo, UVOut xn) {
     constexpr int H = sweepHalo<C>();
     constexpr int kMask = S > 2 ? 1 : 0;  // sweep j < S writes LDS buffer (j - 1) & kMask
     __shared__ double2 buf[kMask + 1][C + H];
@@ -761,8 +773,9 @@ __global__ void __launch_bounds__(C) kJacobiUVBlock(DMesh m, DState s, DSweepBlo
     if (haloRow) loadSweepRow(hal, m, s, hc);
     // sweep 1: global iterate in, LDS out
     double2* out = buf[0];
-    if (mine) out[t] = sweepRow(own, [&](int q) { return xo.at(own.nb[q]); }, xo.at(n));
-    if (haloRow) out[C + t] = sweepRow(hal, [&](int q) { return xo.at(hal.nb[q]); }, xo.at(hc));
+    if (mine) out[t] = sweepRow(own, [&](int q) { return xo.at(own.nb[q] >= 0 ? own.nb[q] : n); }, xo.at(n));
+    if (haloRow)
+        out[C + t] = sweepRow(hal, [&](int q) { return xo.at(hal.nb[q] >= 0 ? hal.nb[q] : hc); }, xo.at(hc));
     // the LDS slots the later sweeps read, loaded while the barrier waits
     if (mine) loadLoc(own, sb.locOwn + own.k0, R);
     if (t < nHaloLds) loadLoc(hal, sb.locHalo + (size_t)(h0 + t) * kSweepRegSlots, R);
@@ -771,12 +784,14 @@ __global__ void __launch_bounds__(C) kJacobiUVBlock(DMesh m, DState s, DSweepBlo
     for (int j = 2; j <= S; j++) {
         const double2* in = buf[(j - 2) & kMask];
         if (j == S) {
-            if (mine) xn.put(n, sweepRow<R, true>(own, [&](int q) { return in[own.loc[q]]; }, in[t]));
+            if (mine) xn.put(n, sweepRow<R, true>(own, [&](int q) { return in[own.loc[q] >= 0 ? own.loc[q] : t]; }, in[t]));
         } else {
             double2* o = buf[(j - 1) & kMask];
             const int nHaloJ = sb.ringEnd[blk * sb.rings + (S - j) - 1];
-            if (mine) o[t] = sweepRow<R, true>(own, [&](int q) { return in[own.loc[q]]; }, in[t]);
-            if (t < nHaloJ) o[C + t] = sweepRow<R, true>(hal, [&](int q) { return in[hal.loc[q]]; }, in[C + t]);
+            if (mine) o[t] = sweepRow<R, true>(own, [&](int q) { return in[own.loc[q] >= 0 ? own.loc[q] : t]; }, in[t]);
+            if (t < nHaloJ)
+                o[C + t] = sweepRow<R, true>(
+                    hal, [&](int q) { return in[hal.loc[q] >= 0 ? hal.loc[q] : C + t]; }, in[C + t]);
             __syncthreads();
         }
     }
@@ -1564,10 +1579,14 @@ __device__ __forceinline__ void loadMGRow(MGRowReg<R>& r, const DLevel& L, int c
     rowRange(L.start, L.uniformRow, c, k0, ns);
     r.k0 = k0;
     r.ns = ns;
+    // clamped, unconditional reads as in loadSweepRow
 #pragma unroll
     for (int q = 0; q < R; q++) {
-        r.nb[q] = (q < ns) ? L.nb[k0 + q] : -1;
-        r.af[q] = (q < ns) ? L.AF[k0 + q] : 0.0;
+        const int k = ns > 0 ? k0 + min(q, ns - 1) : 0;
+        const int nb = L.nb[k];
+        const double af = L.AF[k];
+        r.nb[q] = (q < ns) ? nb : -1;
+        r.af[q] = (q < ns) ? af : 0.0;
     }
     r.ac = L.AC[c];
     r.b = L.b[c];
@@ -1578,7 +1597,7 @@ __device__ __forceinline__ double mgJacobiReg(const MGRowReg<R>& r, XAt xat, dou
     if (fabs(r.ac) < 1.0e-30) return xn;
     double xv[R];
 #pragma unroll
-    for (int q = 0; q < R; q++) xv[q] = (LDS ? r.loc[q] >= 0 : r.nb[q] != -1) ? xat(q) : 0.0;
+    for (int q = 0; q < R; q++) xv[q] = xat(q);  // slots without a neighbour: read, not used (as in sweepRow)
     double Ax = r.ac * xn;
 #pragma unroll
     for (int q = 0; q < R; q++)
@@ -1586,8 +1605,11 @@ __device__ __forceinline__ double mgJacobiReg(const MGRowReg<R>& r, XAt xat, dou
     return xn + w * (r.b - Ax) / r.ac;
 }
 
+// held to 64 VGPRs (two 1024-thread blocks per CU), which these rows fit
+// without spilling
 template <int C, int S, bool P, bool RES, int R>
-__global__ void __launch_bounds__(C) kMGLevel0Block(DLevel f, DLevel c, DSweepBlocks sb, double w, const Ctl* ctl) {
+__global__ void __launch_bounds__(C) __attribute__((amdgpu_waves_per_eu(8)))
+kMGLevel0Block(DLevel f, DLevel c, DSweepBlocks sb, double w, const Ctl* ctl) {
     constexpr int H = sweepHalo<C>();
     constexpr int kSets = (S - 1 + (RES ? 1 : 0)) > 1 ? 2 : 1;  // LDS iterate buffers in use
     __shared__ double buf[kSets][C + H];
@@ -1612,8 +1634,11 @@ __global__ void __launch_bounds__(C) kMGLevel0Block(DLevel f, DLevel c, DSweepBl
     // a residual writes f.xNew directly
     double lastOwn = 0.0;
     {
-        const double v = mine ? mgJacobiReg(own, [&](int q) { return x0(own.nb[q]); }, x0(n), w) : 0.0;
-        const double vh = haloRow ? mgJacobiReg(hal, [&](int q) { return x0(hal.nb[q]); }, x0(hc), w) : 0.0;
+        const double v =
+            mine ? mgJacobiReg(own, [&](int q) { return x0(own.nb[q] != -1 ? own.nb[q] : n); }, x0(n), w) : 0.0;
+        const double vh =
+            haloRow ? mgJacobiReg(hal, [&](int q) { return x0(hal.nb[q] != -1 ? hal.nb[q] : hc); }, x0(hc), w)
+                    : 0.0;
         if (S == 1 && !RES) {
             if (mine) f.xNew[n] = v;
             return;
@@ -1631,10 +1656,14 @@ __global__ void __launch_bounds__(C) kMGLevel0Block(DLevel f, DLevel c, DSweepBl
     for (int j = 2; j <= S; j++) {
         const double* in = buf[(j - 2) % kSets];
         double* out = buf[(j - 1) % kSets];
-        const double v = mine ? mgJacobiReg<R, true>(own, [&](int q) { return in[own.loc[q]]; }, in[t], w) : 0.0;
+        const double v =
+            mine ? mgJacobiReg<R, true>(own, [&](int q) { return in[own.loc[q] >= 0 ? own.loc[q] : t]; }, in[t], w)
+                 : 0.0;
         const int nHaloJ = (S - j + (RES ? 1 : 0)) > 0 ? sb.ringEnd[blk * sb.rings + (S - j + (RES ? 1 : 0)) - 1] : 0;
         const double vh =
-            (t < nHaloJ) ? mgJacobiReg<R, true>(hal, [&](int q) { return in[hal.loc[q]]; }, in[C + t], w) : 0.0;
+            (t < nHaloJ)
+                ? mgJacobiReg<R, true>(hal, [&](int q) { return in[hal.loc[q] >= 0 ? hal.loc[q] : C + t]; }, in[C + t], w)
+                : 0.0;
         if (j == S && !RES) {
             if (mine) f.xNew[n] = v;
             return;
@@ -1650,7 +1679,7 @@ __global__ void __launch_bounds__(C) kMGLevel0Block(DLevel f, DLevel c, DSweepBl
         f.xNew[n] = lastOwn;
         double xv[R];
 #pragma unroll
-        for (int q = 0; q < R; q++) xv[q] = (own.loc[q] >= 0) ? xs[own.loc[q]] : 0.0;
+        for (int q = 0; q < R; q++) xv[q] = xs[own.loc[q] >= 0 ? own.loc[q] : t];
         double Ax = own.ac * lastOwn;
 #pragma unroll
         for (int q = 0; q < R; q++)
@@ -1817,23 +1846,35 @@ __device__ __forceinline__ void loadRow(TRow<T>& r, const T* AC, const T* b, con
     r.b = b[n];
     r.k0 = st[n];
     r.deg = st[n + 1] - r.k0;
+    // every slot read at an index clamped into the row, then masked: the reads
+    // stay unconditional and go out together instead of one branch each
+    int nv[kRowK];
+    T av[kRowK];
+#pragma unroll
+    for (int k = 0; k < kRowK; k++) {
+        const int kk = r.k0 + (k < r.deg ? k : 0);
+        nv[k] = nbs[kk];
+        av[k] = AF[kk];
+    }
 #pragma unroll
     for (int k = 0; k < kRowK; k++) {
         const bool in = k < r.deg;
-        r.nb[k] = in ? nbs[r.k0 + k] : -1;
-        r.af[k] = in ? AF[r.k0 + k] : T(0);
+        r.nb[k] = in ? nv[k] : -1;
+        r.af[k] = in ? av[k] : T(0);
     }
 }
 
 // Ax of the row: AC*xn + AF*x(nb) over the slots with a neighbour, in slot
-// order.  xat(j) gives the (possibly prolonged) x of cell j.
+// order.  xat(j) gives the (possibly prolonged) x of cell j; self is the row's
+// own cell, read (and not used) in place of missing neighbours so that every
+// read is unconditional.
 template <class T, class XAt>
-__device__ __forceinline__ T rowAx(const TRow<T>& r, T xn, XAt xat, const T* AF, const int* nbs) {
+__device__ __forceinline__ T rowAx(const TRow<T>& r, T xn, XAt xat, const T* AF, const int* nbs, int self) {
     T Ax = r.ac * xn;
     if (r.deg <= kRowK) {
         T xv[kRowK];
 #pragma unroll
-        for (int k = 0; k < kRowK; k++) xv[k] = r.nb[k] != -1 ? xat(r.nb[k]) : T(0);
+        for (int k = 0; k < kRowK; k++) xv[k] = xat(r.nb[k] != -1 ? r.nb[k] : self);
 #pragma unroll
         for (int k = 0; k < kRowK; k++)
             if (r.nb[k] != -1) Ax += r.af[k] * xv[k];
@@ -1863,16 +1904,14 @@ __global__ void __launch_bounds__(1024) kMGTail(const TailLevel* __restrict__ tl
     // vectors before storing any, so they are in flight together
     for (int base = 0; base < vecs; base += kCopyR * NT) {
         uint4 v[kCopyR];
+        // indices clamped into the image, for the loads and the stores alike:
+        // both are unconditional, so the loads issue back to back (a guarded
+        // load waits before the next one).  Lanes past the end store the last
+        // vector again, with the value it already has.
 #pragma unroll
-        for (int r = 0; r < kCopyR; r++) {
-            const int i = base + r * NT + t;
-            if (i < vecs) v[r] = image[i];
-        }
+        for (int r = 0; r < kCopyR; r++) v[r] = image[min(base + r * NT + t, vecs - 1)];
 #pragma unroll
-        for (int r = 0; r < kCopyR; r++) {
-            const int i = base + r * NT + t;
-            if (i < vecs) sV[i] = v[r];
-        }
+        for (int r = 0; r < kCopyR; r++) sV[min(base + r * NT + t, vecs - 1)] = v[r];
     }
     __syncthreads();
 
@@ -1911,14 +1950,14 @@ __global__ void __launch_bounds__(1024) kMGTail(const TailLevel* __restrict__ tl
                         } else if (shortRows) {
                             T xv[kWaveK];
 #pragma unroll
-                            for (int k = 0; k < kWaveK; k++) xv[k] = r.nb[k] != -1 ? x[r.nb[k]] : T(0);
+                            for (int k = 0; k < kWaveK; k++) xv[k] = x[r.nb[k] != -1 ? r.nb[k] : t];
                             T Ax = r.ac * xn;
 #pragma unroll
                             for (int k = 0; k < kWaveK; k++)
                                 if (r.nb[k] != -1) Ax += r.af[k] * xv[k];
                             res = xn + w * (r.b - Ax) / r.ac;
                         } else {
-                            const T Ax = rowAx(r, xn, [&](int j) { return x[j]; }, AF, nbs);
+                            const T Ax = rowAx(r, xn, [&](int j) { return x[j]; }, AF, nbs, t);
                             res = xn + w * (r.b - Ax) / r.ac;
                         }
                         y[t] = res;
@@ -1941,7 +1980,7 @@ __global__ void __launch_bounds__(1024) kMGTail(const TailLevel* __restrict__ tl
                 if (tabs(r.ac) < T(1.0e-30)) {
                     res = xn;
                 } else {
-                    const T Ax = rowAx(r, xn, [&](int j) { return x[j]; }, AF, nbs);
+                    const T Ax = rowAx(r, xn, [&](int j) { return x[j]; }, AF, nbs, t);
                     res = xn + w * (r.b - Ax) / r.ac;
                 }
                 y[t] = res;
@@ -1968,7 +2007,7 @@ __global__ void __launch_bounds__(1024) kMGTail(const TailLevel* __restrict__ tl
             if (t < F.n) {
                 TRow<T> r;
                 loadRow(r, sT + F.AC, sT + F.b, AF, sI + F.start, nbs, t);
-                const T Ax = rowAx(r, x[t], [&](int j) { return x[j]; }, AF, nbs);
+                const T Ax = rowAx(r, x[t], [&](int j) { return x[j]; }, AF, nbs, t);
                 res[t] = r.b - Ax;
             }
         }
@@ -1984,8 +2023,11 @@ __global__ void __launch_bounds__(1024) kMGTail(const TailLevel* __restrict__ tl
                 T acc = T(0);
                 if (cnt <= kMemberK) {
                     T rv[kMemberK];
+                    int mv[kMemberK];
 #pragma unroll
-                    for (int j = 0; j < kMemberK; j++) rv[j] = j < cnt ? res[mem[j0 + j]] : T(0);
+                    for (int j = 0; j < kMemberK; j++) mv[j] = mem[j0 + (j < cnt ? j : 0)];
+#pragma unroll
+                    for (int j = 0; j < kMemberK; j++) rv[j] = res[cnt > 0 ? mv[j] : t];
 #pragma unroll
                     for (int j = 0; j < kMemberK; j++)
                         if (j < cnt) acc += rv[j];
@@ -2036,7 +2078,7 @@ __global__ void __launch_bounds__(1024) kMGTail(const TailLevel* __restrict__ tl
                 if (tabs(r.ac) < T(1.0e-30)) {
                     y[t] = xn;
                 } else {
-                    const T Ax = rowAx(r, xn, [&](int j) { return x[j] + xc[toC[j]]; }, AF, nbs);
+                    const T Ax = rowAx(r, xn, [&](int j) { return x[j] + xc[toC[j]]; }, AF, nbs, t);
                     y[t] = xn + w * (r.b - Ax) / r.ac;
                 }
             }
