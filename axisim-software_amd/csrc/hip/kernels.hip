@@ -2245,32 +2245,30 @@ int sweepHaloMax(int cells) {
     if (cells == 1024) return sweepHalo<1024>();
     return 0;
 }
+namespace {
+template <int C, int S>
+void launchJacobiUVBlockCS(hipStream_t s, const DMesh& m, DState st, const DSweepBlocks& sb, UVIn in, UVOut out) {
+    if (m.maxSlots <= 3) kJacobiUVBlock<C, S, 3><<<sb.nBlocks, C, 0, s>>>(m, st, sb, in, out);
+    else kJacobiUVBlock<C, S, kSweepRegSlots><<<sb.nBlocks, C, 0, s>>>(m, st, sb, in, out);
+}
+template <int C>
+void launchJacobiUVBlockC(hipStream_t s, const DMesh& m, DState st, const DSweepBlocks& sb, int sweeps, UVIn in,
+                          UVOut out) {
+    if (sweeps == 2) launchJacobiUVBlockCS<C, 2>(s, m, st, sb, in, out);
+    else if (sweeps == 3) launchJacobiUVBlockCS<C, 3>(s, m, st, sb, in, out);
+    else launchJacobiUVBlockCS<C, 4>(s, m, st, sb, in, out);
+}
+}  // namespace
 void launchJacobiUVBlock(hipStream_t s, const DMesh& m, DState st, const DSweepBlocks& sb, int sweeps,
                          const double* uo, const double* vo, const double2* uvo, double* un, double* vn,
                          double2* uvn) {
-    if (sweeps < 2 || sweeps > 3 || sweeps > sb.rings + 1) throw std::invalid_argument("bad fused sweep count");
+    if (sweeps < 2 || sweeps > kMaxFusedSweeps || sweeps > sb.rings + 1)
+        throw std::invalid_argument("bad fused sweep count");
     const UVIn in{uo, vo, uvo};
     const UVOut out{un, vn, uvn};
-    const bool tri = m.maxSlots <= 3;
-    if (sb.cells == 512) {
-        if (sweeps == 2) {
-            if (tri) kJacobiUVBlock<512, 2, 3><<<sb.nBlocks, 512, 0, s>>>(m, st, sb, in, out);
-            else kJacobiUVBlock<512, 2, kSweepRegSlots><<<sb.nBlocks, 512, 0, s>>>(m, st, sb, in, out);
-        } else {
-            if (tri) kJacobiUVBlock<512, 3, 3><<<sb.nBlocks, 512, 0, s>>>(m, st, sb, in, out);
-            else kJacobiUVBlock<512, 3, kSweepRegSlots><<<sb.nBlocks, 512, 0, s>>>(m, st, sb, in, out);
-        }
-    } else if (sb.cells == 1024) {
-        if (sweeps == 2) {
-            if (tri) kJacobiUVBlock<1024, 2, 3><<<sb.nBlocks, 1024, 0, s>>>(m, st, sb, in, out);
-            else kJacobiUVBlock<1024, 2, kSweepRegSlots><<<sb.nBlocks, 1024, 0, s>>>(m, st, sb, in, out);
-        } else {
-            if (tri) kJacobiUVBlock<1024, 3, 3><<<sb.nBlocks, 1024, 0, s>>>(m, st, sb, in, out);
-            else kJacobiUVBlock<1024, 3, kSweepRegSlots><<<sb.nBlocks, 1024, 0, s>>>(m, st, sb, in, out);
-        }
-    } else {
-        throw std::invalid_argument("fused sweeps take blocks of 512 or 1024 cells");
-    }
+    if (sb.cells == 512) launchJacobiUVBlockC<512>(s, m, st, sb, sweeps, in, out);
+    else if (sb.cells == 1024) launchJacobiUVBlockC<1024>(s, m, st, sb, sweeps, in, out);
+    else throw std::invalid_argument("fused sweeps take blocks of 512 or 1024 cells");
 }
 void launchGSColorUV(hipStream_t s, const DMesh& m, DState st, const int* order, int begin, int count) {
     kGSColorUV<<<blocksFor(count), kBlock, 0, s>>>(m, st, order, begin, count);

@@ -241,7 +241,7 @@ private:
     // fused momentum sweeps: requested and in-use sweeps per launch, the blocks
     // and halo rings (host and device copies, built for fuseCellsReq_ and
     // fuse_ - 1 rings)
-    int fuseReq_ = 3, fuseCellsReq_ = 0, fuse_ = 1;
+    int fuseReq_ = kMaxFusedSweeps, fuseCellsReq_ = 0, fuse_ = 1;
     SweepBlocks sweepBlocks_;
     DSweepBlocks dsb_{};
     Arena sweepArena_;
