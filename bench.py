@@ -19,6 +19,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 from pathlib import Path
@@ -28,6 +29,18 @@ sys.path.insert(0, str(ROOT / "axisim-software_amd"))
 sys.path.insert(0, str(ROOT / "tests"))
 
 HBM_PEAK_GBS = 8000.0
+
+
+def _cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo), or the platform's string."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
 
 
 def log(msg: str) -> None:
@@ -184,7 +197,8 @@ def main() -> int:
         dt = time.time() - tc
         cpu = {"value": round(o.iterations / dt, 4), "unit": "outer_iter/s", "cores": 1, "kind": "port",
                "sample": f"{o.iterations} outer iterations of the same {m.n_cells}-cell nozzle case from a cold "
-                         f"start, FP64 oracle (oracle/oracle.cpp), one thread, {dt:.1f} s"}
+                         f"start, FP64 oracle (oracle/oracle.cpp), one thread, {dt:.1f} s",
+               "host_cpu": _cpu_model(), "host_threads": os.cpu_count()}
         log(f"cpu baseline: {cpu['value']} it/s")
 
     out = {
