@@ -717,12 +717,6 @@ __device__ __forceinline__ void loadSweepRow(SweepRow<R>& r, const DMesh& m, con
     r.iv = s.vInv[c];
 }
 
-// the neighbours' LDS slots of a row from its table entries l[0 .. ns)
-template <class Row>
-__device__ __forceinline__ void loadLoc(Row& r, const int16_t* l, int R) {
-    for (int q = 0; q < R; q++) r.loc[q] = (q < r.ns) ? (int)l[q] : -1;
-}
-
 // jacobi's row (kJacobiUV) from registers; xat(q) = the neighbour of slot q
 // (for a slot without one, any valid entry: it is read and not used, so that
 // every read is unconditional and they all go out together); LDS: the
@@ -742,11 +736,6 @@ __device__ __forceinline__ double2 sweepRow(const SweepRow<R>& r, XAt xat, doubl
     }
     return make_double2((r.iu != 0.0) ? au * r.iu : self.x, (r.iv != 0.0) ? av * r.iv : self.y);
 }
-
-// halo entries a block may have: at most one per thread (thread t holds halo
-// row t), and the LDS of two iterate buffers within the static 64 KiB
-template <int C>
-constexpr int sweepHalo() { return C <= 512 ? C : C / 2; }
 
 // Triangle rows (R = 3) fit 64 VGPRs, i.e. two 1024-thread blocks per CU;
 // the attribute holds the compiler to that (longer rows take what they need).

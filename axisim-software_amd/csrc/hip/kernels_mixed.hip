@@ -16,6 +16,7 @@
 // Rounding errors of the FP32 parts scale with the defect, which goes to zero
 // as SIMPLE converges, so the converged fields keep FP64 accuracy.
 #include <cmath>
+#include <stdexcept>
 
 #include "device.hpp"
 #include "mixed.hpp"
@@ -129,6 +130,96 @@ __global__ void __launch_bounds__(kBlock) kMomSweep32(DMesh m, const Ctl* ctl, M
 }
 
 // last FP32 sweep fused with u += delta, v += delta (FP64)
+// Several FP32 sweeps in one launch: kJacobiUVBlock's temporal blocking
+// (the same blocks, halo rings and LDS-slot tables) on the defect iterates,
+// each row evaluated as momSweep32 evaluates it, so S sweeps here are
+// bit-identical to S launches of kMomSweep32.
+template <int R>
+struct SweepRow32 {
+    float af[R];
+    int nb[R];   // neighbour cell, -1: none (boundary or past the row's slots)
+    int loc[R];  // the neighbour's LDS slot, -1: none (loaded after the first sweep)
+    int k0, ns;
+    float2 r, inv;
+};
+
+template <int R>
+__device__ __forceinline__ void loadSweepRow32(SweepRow32<R>& w, const DMesh& m, const MixedState& x, int c) {
+    int k0, ns;
+    rowRange(m.faceStart, m.uniformSlots, c, k0, ns);
+    w.k0 = k0;
+    w.ns = ns;
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+        const int k = ns > 0 ? k0 + min(q, ns - 1) : 0;  // clamped, unconditional reads
+        const int nb = m.slotNb[k];
+        const float af = x.AF32[k];
+        w.nb[q] = (q < ns) ? nb : -1;
+        w.af[q] = (q < ns) ? af : 0.0f;
+    }
+    w.r = x.r[c];
+    w.inv = x.inv[c];
+}
+
+// momSweep32's row from registers; dat(q) = the neighbour of slot q (any valid
+// entry where there is none: read, not used); LDS: the neighbours with an LDS slot
+template <int R, bool LDS = false, typename DAt>
+__device__ __forceinline__ float2 sweepRow32(const SweepRow32<R>& w, DAt dat, float2 self) {
+    float2 dv[R];
+#pragma unroll
+    for (int q = 0; q < R; q++) dv[q] = dat(q);
+    float au = w.r.x, av = w.r.y;
+#pragma unroll
+    for (int q = 0; q < R; q++)
+        if (LDS ? w.loc[q] >= 0 : w.nb[q] >= 0) {
+            au -= w.af[q] * dv[q].x;
+            av -= w.af[q] * dv[q].y;
+        }
+    return make_float2((w.inv.x != 0.0f) ? au * w.inv.x : self.x, (w.inv.y != 0.0f) ? av * w.inv.y : self.y);
+}
+
+template <int C, int S, int R>
+__global__ void __launch_bounds__(C) kMomSweep32Block(DMesh m, const Ctl* ctl, MixedState x, DSweepBlocks sb,
+                                                      const float2* __restrict__ dO, float2* __restrict__ dN) {
+    constexpr int H = sweepHalo<C>();
+    constexpr int kMask = S > 2 ? 1 : 0;  // sweep j < S writes LDS buffer (j - 1) & kMask
+    __shared__ float2 buf[kMask + 1][C + H];
+    if (finished(ctl)) return;
+    const int t = threadIdx.x, blk = xcdBlock();
+    const int a = blk * C, nOwn = min(C, m.N - a);
+    const int h0 = sb.haloStart[blk];
+    // halo rings 1 .. S - 1 are swept first; rings 1 .. S - 2 again from LDS
+    const int nHalo1 = sb.ringEnd[blk * sb.rings + S - 2];
+    const int nHaloLds = (S > 2) ? sb.ringEnd[blk * sb.rings + (S > 2 ? S - 3 : 0)] : 0;
+    const int n = sb.own ? sb.own[(size_t)blk * C + t] : a + t;
+    const bool mine = sb.own ? n >= 0 : t < nOwn, haloRow = t < nHalo1;
+    const int hc = haloRow ? sb.halo[h0 + t] : 0;
+    SweepRow32<R> own, hal;
+    if (mine) loadSweepRow32(own, m, x, n);
+    if (haloRow) loadSweepRow32(hal, m, x, hc);
+    float2* out = buf[0];
+    if (mine) out[t] = sweepRow32(own, [&](int q) { return dO[own.nb[q] >= 0 ? own.nb[q] : n]; }, dO[n]);
+    if (haloRow) out[C + t] = sweepRow32(hal, [&](int q) { return dO[hal.nb[q] >= 0 ? hal.nb[q] : hc]; }, dO[hc]);
+    if (mine) loadLoc(own, sb.locOwn + own.k0, R);
+    if (t < nHaloLds) loadLoc(hal, sb.locHalo + (size_t)(h0 + t) * kSweepRegSlots, R);
+    __syncthreads();
+#pragma unroll
+    for (int j = 2; j <= S; j++) {
+        const float2* in = buf[(j - 2) & kMask];
+        if (j == S) {
+            if (mine) dN[n] = sweepRow32<R, true>(own, [&](int q) { return in[own.loc[q] >= 0 ? own.loc[q] : t]; }, in[t]);
+        } else {
+            float2* o = buf[(j - 1) & kMask];
+            const int nHaloJ = sb.ringEnd[blk * sb.rings + (S - j) - 1];
+            if (mine) o[t] = sweepRow32<R, true>(own, [&](int q) { return in[own.loc[q] >= 0 ? own.loc[q] : t]; }, in[t]);
+            if (t < nHaloJ)
+                o[C + t] = sweepRow32<R, true>(
+                    hal, [&](int q) { return in[hal.loc[q] >= 0 ? hal.loc[q] : C + t]; }, in[C + t]);
+            __syncthreads();
+        }
+    }
+}
+
 template <int B>
 __global__ void __launch_bounds__(kBlock) kMomSweepApply(DMesh m, DState s, MixedState x,
                                                         const float2* __restrict__ dO, int sweep) {
@@ -364,6 +455,30 @@ void launchCoarseInvert(hipStream_t s, const DLevel& L, double* aug, double* inv
 }
 void launchCoarseApply(hipStream_t s, const DLevel& L, const double* inv, const Ctl* ctl) {
     kCoarseApply<<<blocksFor(L.n), kBlock, 0, s>>>(L, inv, ctl);
+}
+
+namespace {
+template <int C, int S>
+void launchMomSweep32BlockCS(hipStream_t s, const DMesh& m, const Ctl* ctl, MixedState x, const DSweepBlocks& sb,
+                             const float2* dO, float2* dN) {
+    if (m.maxSlots <= 3) kMomSweep32Block<C, S, 3><<<sb.nBlocks, C, 0, s>>>(m, ctl, x, sb, dO, dN);
+    else kMomSweep32Block<C, S, kSweepRegSlots><<<sb.nBlocks, C, 0, s>>>(m, ctl, x, sb, dO, dN);
+}
+template <int C>
+void launchMomSweep32BlockC(hipStream_t s, const DMesh& m, const Ctl* ctl, MixedState x, const DSweepBlocks& sb,
+                            int sweeps, const float2* dO, float2* dN) {
+    if (sweeps == 2) launchMomSweep32BlockCS<C, 2>(s, m, ctl, x, sb, dO, dN);
+    else if (sweeps == 3) launchMomSweep32BlockCS<C, 3>(s, m, ctl, x, sb, dO, dN);
+    else launchMomSweep32BlockCS<C, 4>(s, m, ctl, x, sb, dO, dN);
+}
+}  // namespace
+void launchMomSweep32Block(hipStream_t s, const DMesh& m, const Ctl* ctl, MixedState x, const DSweepBlocks& sb,
+                           int sweeps, const float2* dO, float2* dN) {
+    if (sweeps < 2 || sweeps > kMaxFusedSweeps || sweeps > sb.rings + 1)
+        throw std::invalid_argument("bad fused sweep count");
+    if (sb.cells == 512) launchMomSweep32BlockC<512>(s, m, ctl, x, sb, sweeps, dO, dN);
+    else if (sb.cells == 1024) launchMomSweep32BlockC<1024>(s, m, ctl, x, sb, sweeps, dO, dN);
+    else throw std::invalid_argument("fused sweeps take blocks of 512 or 1024 cells");
 }
 
 }  // namespace axs

@@ -41,5 +41,18 @@ __device__ __forceinline__ void rowRange(const int* start, int uniform, int n, i
     }
 }
 
+// ---- the fused (temporally blocked) sweep kernels' shared pieces ----
+
+// halo entries a block may have: at most one per thread (thread t holds halo
+// row t), and the LDS of two iterate buffers within the static 64 KiB
+template <int C>
+constexpr int sweepHalo() { return C <= 512 ? C : C / 2; }
+
+// the neighbours' LDS slots of a row from its table entries l[0 .. ns)
+template <class Row>
+__device__ __forceinline__ void loadLoc(Row& r, const int16_t* l, int R) {
+    for (int q = 0; q < R; q++) r.loc[q] = (q < r.ns) ? (int)l[q] : -1;
+}
+
 }  // namespace
 }  // namespace axs

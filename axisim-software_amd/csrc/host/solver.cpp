@@ -952,9 +952,15 @@ void Solver::enqueueIteration(hipStream_t s) {
         const int L = c.linear_max_iter;
         launchMomDefect(s, dm_, st, mx_);
         float2 *d = mx_.d, *d2 = mx_.d2;
-        for (int it = 2; it < L; it++) {
-            launchMomSweep32(s, dm_, st.ctl, mx_, d, d2);
+        // sweeps 2 .. L-1 in chunks of up to fuse_ per launch (the FP64 path's
+        // blocks), avoiding a single sweep at the end where possible
+        for (int left = L - 2; left > 0;) {
+            int n = std::min(fuse_, left);
+            if (n > 2 && left - n == 1) n--;
+            if (n == 1) launchMomSweep32(s, dm_, st.ctl, mx_, d, d2);
+            else launchMomSweep32Block(s, dm_, st.ctl, mx_, dsb_, n, d, d2);
             std::swap(d, d2);
+            left -= n;
         }
         launchMomSweepApply(s, dm_, st, mx_, d, L >= 2 ? 1 : 0);
     } else if (c.linear_solver == AXS_LINEAR_GS_RB && nColors_ > 0) {

@@ -3,7 +3,8 @@ and v in one workgroup per block of consecutive cells, the intermediate
 iterates in LDS, the halo rings around each block recomputed by the block.
 Each row is evaluated exactly as by the one-sweep kernel, so a run is
 bit-identical to the unfused one for every block size and sweep count, also
-when the sweep count does not divide the solve's sweeps."""
+when the sweep count does not divide the solve's sweeps -- for the FP64 sweeps
+and for the mixed path's FP32 defect sweeps alike."""
 import numpy as np
 import pytest
 
@@ -86,5 +87,19 @@ def test_fused_sweeps_patch_blocks_bit_exact(require_gpu, sweeps):
     ref = run(case, 1)
     s = run(case, sweeps, 0)
     assert s.stats().sweeps_per_launch == sweeps
+    for name in FIELDS:
+        np.testing.assert_array_equal(s.field(name), ref.field(name), err_msg=name)
+
+
+@pytest.mark.parametrize("sweeps,cells", [(2, 512), (3, 1024), (4, 1024)])
+def test_fused_fp32_sweeps_bit_exact(require_gpu, nozzle_mesh, sweeps, cells):
+    """The mixed path's FP32 defect sweeps fused over the same blocks: S sweeps
+    per launch against one per launch, bit for bit (18 FP32 sweeps per solve,
+    so the last chunk is shorter)."""
+    case = axs.nozzle_case(mesh=nozzle_mesh, re_throat=100, max_iter=8, tol=0.0, precision=axs.PREC_MIXED)
+    ref = run(case, 1)
+    s = run(case, sweeps, cells)
+    assert s.stats().sweeps_per_launch == sweeps
+    assert s.stats().iterations == ref.stats().iterations == 8
     for name in FIELDS:
         np.testing.assert_array_equal(s.field(name), ref.field(name), err_msg=name)
