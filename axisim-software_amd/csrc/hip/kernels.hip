@@ -1489,12 +1489,18 @@ __global__ void __launch_bounds__(1024) kMGSmoothBlock(DLevel L, double w, int s
 // post-smoothing sweep: x + P x_c is formed on the fly for the cell and its
 // neighbours (the same sums the prolongation would store).
 template <class T, int B>
-__global__ void __launch_bounds__(kBlock) kMGProlongSmooth(DLevel f, DLevel c, T w, const Ctl* ctl) {
+__global__ void __launch_bounds__(kBlock) kMGProlongSmooth(DLevel f, DLevel c, T w, double* __restrict__ apply,
+                                                           const Ctl* ctl) {
     if (finished(ctl)) return;
     const int n = xcdBlock() * blockDim.x + threadIdx.x;
     if (n >= f.n) return;
     const LV<T> v = view<T>(f);
     const LV<T> cv = view<T>(c);
+    // the new iterate; with apply (the mixed cycle's level 0), also x += it in FP64
+    auto put = [&](T r) {
+        v.xNew[n] = r;
+        if (apply) apply[n] += (double)r;
+    };
     int k0, ns;
     rowRange(f.start, f.uniformRow, n, k0, ns);
     if (B > 0 && ns > 0 && ns <= B) {
@@ -1519,20 +1525,20 @@ __global__ void __launch_bounds__(kBlock) kMGProlongSmooth(DLevel f, DLevel c, T
         for (int q = 0; q < B; q++) co[q] = cv.x[tc[q]];
         const T xn = xf + xc;
         if (tabs(AC) < T(1.0e-30)) {
-            v.xNew[n] = xn;
+            put(xn);
             return;
         }
         T Ax = AC * xn;
 #pragma unroll
         for (int q = 0; q < B; q++)
             if (q < ns && o[q] != -1) Ax += a[q] * (xo[q] + co[q]);
-        v.xNew[n] = xn + w * (bn - Ax) / AC;
+        put(xn + w * (bn - Ax) / AC);
         return;
     }
     const T xn = v.x[n] + cv.x[f.toCoarse[n]];
     const T AC = v.AC[n];
     if (tabs(AC) < T(1.0e-30)) {
-        v.xNew[n] = xn;
+        put(xn);
         return;
     }
     T Ax = AC * xn;
@@ -1540,7 +1546,7 @@ __global__ void __launch_bounds__(kBlock) kMGProlongSmooth(DLevel f, DLevel c, T
         const int nb = f.nb[k];
         if (nb != -1) Ax += v.AF[k] * (v.x[nb] + cv.x[f.toCoarse[nb]]);
     }
-    v.xNew[n] = xn + w * (v.b[n] - Ax) / AC;
+    put(xn + w * (v.b[n] - Ax) / AC);
 }
 
 // Level-0 sweeps of the V-cycles fused per launch (temporal blocking, as in
@@ -2150,18 +2156,20 @@ static void launchMGJacobiT(hipStream_t s, const DLevel& L, T w, const Ctl* ctl)
     else kMGJacobi<T, 0><<<g, b, 0, s>>>(L, w, ctl);
 }
 template <class T>
-static void launchMGProlongSmoothT(hipStream_t s, const DLevel& f, const DLevel& c, T w, const Ctl* ctl) {
+static void launchMGProlongSmoothT(hipStream_t s, const DLevel& f, const DLevel& c, T w, const Ctl* ctl,
+                                   double* apply = nullptr) {
     const dim3 g(blocksFor(f.n, f.block)), b(f.block);
-    if (f.maxRow <= 4) kMGProlongSmooth<T, 4><<<g, b, 0, s>>>(f, c, w, ctl);
-    else if (f.maxRow <= 8) kMGProlongSmooth<T, 8><<<g, b, 0, s>>>(f, c, w, ctl);
-    else if (f.maxRow <= 16) kMGProlongSmooth<T, 16><<<g, b, 0, s>>>(f, c, w, ctl);
-    else kMGProlongSmooth<T, 0><<<g, b, 0, s>>>(f, c, w, ctl);
+    if (f.maxRow <= 4) kMGProlongSmooth<T, 4><<<g, b, 0, s>>>(f, c, w, apply, ctl);
+    else if (f.maxRow <= 8) kMGProlongSmooth<T, 8><<<g, b, 0, s>>>(f, c, w, apply, ctl);
+    else if (f.maxRow <= 16) kMGProlongSmooth<T, 16><<<g, b, 0, s>>>(f, c, w, apply, ctl);
+    else kMGProlongSmooth<T, 0><<<g, b, 0, s>>>(f, c, w, apply, ctl);
 }
 void launchMGResidual(hipStream_t s, const DLevel& f, const Ctl* ctl) { launchMGResidualT<double>(s, f, ctl); }
 void launchMGResidual32(hipStream_t s, const DLevel& f, const Ctl* ctl) { launchMGResidualT<float>(s, f, ctl); }
 void launchMGJacobi32(hipStream_t s, const DLevel& L, float w, const Ctl* ctl) { launchMGJacobiT<float>(s, L, w, ctl); }
-void launchMGProlongSmooth32(hipStream_t s, const DLevel& f, const DLevel& c, float w, const Ctl* ctl) {
-    launchMGProlongSmoothT<float>(s, f, c, w, ctl);
+void launchMGProlongSmooth32(hipStream_t s, const DLevel& f, const DLevel& c, float w, const Ctl* ctl,
+                             double* apply) {
+    launchMGProlongSmoothT<float>(s, f, c, w, ctl, apply);
 }
 template <class T>
 static void launchMGResidualRestrictT(hipStream_t s, const DLevel& f, const DLevel& c, double w, int sweep1,

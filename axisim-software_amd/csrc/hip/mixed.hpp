@@ -28,7 +28,10 @@ void launchMGDefect(hipStream_t s, const DLevel& L, double w, int firstSweep, co
 void launchMGJacobi32(hipStream_t s, const DLevel& L, float w, const Ctl* ctl);
 void launchMGSmoothBlock32(hipStream_t s, const DLevel& L, float w, int sweeps, const Ctl* ctl);
 void launchMGResidual32(hipStream_t s, const DLevel& f, const Ctl* ctl);
-void launchMGProlongSmooth32(hipStream_t s, const DLevel& f, const DLevel& c, float w, const Ctl* ctl);
+// apply: when set, also apply[n] += the new iterate (in FP64) -- the mixed
+// cycle's x += e at level 0, fused into its last sweep
+void launchMGProlongSmooth32(hipStream_t s, const DLevel& f, const DLevel& c, float w, const Ctl* ctl,
+                             double* apply = nullptr);
 void launchMGProlong32(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl);
 void launchMGApply(hipStream_t s, const DLevel& L, const Ctl* ctl);
 void launchMGCast(hipStream_t s, const DLevel& L, const Ctl* ctl);

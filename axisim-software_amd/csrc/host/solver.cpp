@@ -453,7 +453,7 @@ void MGDevice::smooth32(hipStream_t s, int l, int sweeps, const MGParams& p, con
     }
 }
 
-void MGDevice::vcycle32(hipStream_t s, int l, int preDone, const MGParams& p, const Ctl* ctl) {
+void MGDevice::vcycle32(hipStream_t s, int l, int preDone, const MGParams& p, const Ctl* ctl, double* apply) {
     if (l == tail32_.lt) {
         launchMGTail(s, tail32_.launch(), p.weight, preDone, p.pre, p.post, p.coarseSweeps, true,
                      p.coarseDirect ? inv : nullptr, ctl);
@@ -469,7 +469,7 @@ void MGDevice::vcycle32(hipStream_t s, int l, int preDone, const MGParams& p, co
     restrictTo(s, l, s1, true, p, ctl);
     vcycle32(s, l + 1, s1, p, ctl);
     if (p.post >= 1) {
-        launchMGProlongSmooth32(s, dev[l], dev[l + 1], (float)p.weight, ctl);
+        launchMGProlongSmooth32(s, dev[l], dev[l + 1], (float)p.weight, ctl, apply);
         swapX32(l);
         smooth32(s, l, p.post - 1, p, ctl);
     } else {
@@ -498,10 +498,12 @@ void MGDevice::enqueueSolveMixed(hipStream_t s, const MGParams& p, const Ctl* ct
         return;
     }
     const int first = (single ? (!p.coarseDirect && p.coarseSweeps >= 1) : (p.pre >= 1)) ? 1 : 0;
+    // x += e rides on level 0's post-sweep when that sweep is the cycle's last
+    const bool fuseApply = !single && tail32_.lt != 0 && p.post == 1;
     for (int c = 0; c < cycles; c++) {
         launchMGDefect(s, dev[0], p.weight, first, ctl);
-        vcycle32(s, 0, first, p, ctl);
-        launchMGApply(s, dev[0], ctl);
+        vcycle32(s, 0, first, p, ctl, fuseApply ? dev[0].x : nullptr);
+        if (!fuseApply) launchMGApply(s, dev[0], ctl);
     }
 }
 

@@ -123,7 +123,8 @@ private:
     void smooth(hipStream_t s, int l, int sweeps, const MGParams& p, const Ctl* ctl);
     void restrictTo(hipStream_t s, int l, int sweep1, bool fp32, const MGParams& p, const Ctl* ctl);
     void enqueueSolveMixed(hipStream_t s, const MGParams& p, const Ctl* ctl, bool buildOperators);
-    void vcycle32(hipStream_t s, int l, int preDone, const MGParams& p, const Ctl* ctl);
+    void vcycle32(hipStream_t s, int l, int preDone, const MGParams& p, const Ctl* ctl,
+                  double* apply = nullptr);
     void smooth32(hipStream_t s, int l, int sweeps, const MGParams& p, const Ctl* ctl);
     void buildOperators(hipStream_t s, const Ctl* ctl);
     // multi-colour Gauss-Seidel smoother (AXS_MG_SMOOTH_RBGS)
