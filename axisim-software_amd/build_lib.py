@@ -97,10 +97,11 @@ def build_compat() -> Path:
 
 
 def build_oracle() -> Path:
-    """g++ build of the CPU oracle (FP64, no contraction)."""
+    """g++ build of the CPU oracle (FP64, no contraction; OpenMP for the
+    per-cell loops of the multi-thread CPU baseline)."""
     tmp = ORACLE_LIB.with_suffix(".so.tmp")
     _run([
-        "g++", "-std=c++17", "-O2", "-ffp-contract=off", "-fPIC", "-shared",
+        "g++", "-std=c++17", "-O2", "-ffp-contract=off", "-fopenmp", "-fPIC", "-shared",
         f"-I{ROOT / 'include'}", str(ORACLE_DIR / "oracle.cpp"), "-o", str(tmp),
     ])
     os.replace(tmp, ORACLE_LIB)

@@ -9,6 +9,10 @@
 // C++, in the reference's operation order, and cites the file:line it follows
 // (paths relative to the reference repository root).  Built with
 // -ffp-contract=off so that every multiply-add rounds exactly as written.
+// Loops whose iterations each write only their own cell, face or row slots
+// carry an OpenMP `parallel for` (the CPU baseline's multi-thread sample); the
+// results do not depend on the thread count.  Reductions and scatters stay
+// serial.  orc_set_threads chooses the count (the binding sets 1 at load).
 //
 // Pinning (see tests/test_oracle_*.py): geometry closed forms, gradient
 // exactness for linear fields, dense P^T A P, dense Jacobi / Gauss-Seidel
@@ -24,6 +28,8 @@
 #include <cstring>
 #include <string>
 #include <vector>
+
+#include <omp.h>
 
 #include "../include/axs.h"
 
@@ -328,6 +334,7 @@ void smoothen(MGLevel& L, double weight, int sweeps) {
     if (N <= 0 || sweeps <= 0) return;
     std::vector<double> xNew(N);
     for (int s = 0; s < sweeps; s++) {
+#pragma omp parallel for schedule(static)
         for (int n = 0; n < N; n++) jacobiRow(L.coeff, L.x, xNew, n, weight);
         L.x.swap(xNew);
     }
@@ -362,6 +369,7 @@ void smoothenGS(MGLevel& L, int sweeps) {
 
 // residualRaw(sign = true, no scaling), residuals.cuh:17-48
 void residualSigned(const Coeff& c, const std::vector<double>& x, std::vector<double>& res) {
+#pragma omp parallel for schedule(static)
     for (int n = 0; n < c.N; n++) {
         double Ax = c.AC[n] * x[n];
         for (int k = c.faceStart[n]; k < c.faceStart[n + 1]; k++) {
@@ -606,6 +614,7 @@ void gradLSQ(int c, const Mesh& m, const BC& bc, const std::vector<double>& phi,
 // computeGradient, solver_util.cu:146-169
 void computeGradient(const Mesh& m, const BC& bc, const std::vector<double>& phi,
                      std::vector<double>& gZ, std::vector<double>& gR, int scheme) {
+#pragma omp parallel for schedule(static)
     for (int n = 0; n < m.N; n++) {
         gZ[n] = 0.0;
         gR[n] = 0.0;
@@ -637,6 +646,7 @@ void addTransport(const Mesh& m, const Fields& s, Coeff& co, const BC& bc,
                   const std::vector<double>& phi, const std::vector<double>& gz,
                   const std::vector<double>& gr, int nonOrtho, double gamma, int conv, int scheme,
                   double fluxScale) {
+#pragma omp parallel for schedule(static)
     for (int n = 0; n < m.N; n++) {
         double netF = 0.0;
         for (int k = m.faceStart[n]; k < m.faceStart[n + 1]; k++) {
@@ -709,6 +719,7 @@ void addTransport(const Mesh& m, const Fields& s, Coeff& co, const BC& bc,
 
 // addRadialMomentumCylindricalSource, solver_util.cu:454-470
 void radialSource(const Mesh& m, Coeff& co, double scalar) {
+#pragma omp parallel for schedule(static)
     for (int n = 0; n < m.N; n++) {
         const double r = m.cR[n];
         co.AC[n] += scalar * m.vol[n] / (r * r);
@@ -735,6 +746,7 @@ void transientTerm(const Mesh& m, Coeff& co, const std::vector<double>* old,
 
 // underRelaxEquation, solver_util.cu:582-600
 void underRelax(Coeff& co, const std::vector<double>& x, double alpha) {
+#pragma omp parallel for schedule(static)
     for (int n = 0; n < co.N; n++) {
         const double AC0 = co.AC[n];
         if (std::fabs(AC0) < 1.0e-30) continue;
@@ -745,6 +757,7 @@ void underRelax(Coeff& co, const std::vector<double>& x, double alpha) {
 
 // buildInverseDiagonal, solver_util.cu:602-612
 void inverseDiagonal(Coeff& co) {
+#pragma omp parallel for schedule(static)
     for (int n = 0; n < co.N; n++) {
         const double AC = co.AC[n];
         co.invAC[n] = (std::fabs(AC) < 1.0e-30) ? 0.0 : 1.0 / AC;
@@ -753,6 +766,7 @@ void inverseDiagonal(Coeff& co) {
 
 // createPPCoeff, simple.cu:7-88
 void createPPCoeff(double rho, const Mesh& m, Coeff& co, const Fields& s, const BC& pbc) {
+#pragma omp parallel for schedule(static)
     for (int n = 0; n < m.N; n++) {
         for (int k = m.faceStart[n]; k < m.faceStart[n + 1]; k++) {
             const int f = m.faceIDs[k];
@@ -807,6 +821,7 @@ double rhieChow(int c, int f, const Mesh& m, const Fields& s, const BC& bu, cons
 
 // computeFaceMassFluxRhieChow, solver_util.cu:171-203
 void faceMassFlux(double rho, const Mesh& m, Fields& s, const BC& bu, const BC& bv, const BC& bp) {
+#pragma omp parallel for schedule(static)
     for (int f = 0; f < m.F; f++) {
         const double area = m.area[f];
         if (area <= 1.0e-30) {
@@ -820,6 +835,7 @@ void faceMassFlux(double rho, const Mesh& m, Fields& s, const BC& bu, const BC& 
 
 // createPPRhs, simple.cu:90-142
 void createPPRhs(double rho, const Mesh& m, Coeff& co, const Fields& s, int nonOrtho) {
+#pragma omp parallel for schedule(static)
     for (int n = 0; n < m.N; n++) {
         co.b[n] = 0.0;
         double imb = 0.0, cross = 0.0;
@@ -839,6 +855,7 @@ void createPPRhs(double rho, const Mesh& m, Coeff& co, const Fields& s, int nonO
 
 // createMomentumPressureRhs, simple.cu:144-164
 void momentumPressureRhs(const Mesh& m, Coeff& uc, Coeff& vc, const Fields& s) {
+#pragma omp parallel for schedule(static)
     for (int n = 0; n < m.N; n++) {
         const double V = m.vol[n];
         uc.b[n] += -V * s.gradPZ[n];
@@ -848,6 +865,7 @@ void momentumPressureRhs(const Mesh& m, Coeff& uc, Coeff& vc, const Fields& s) {
 
 // getCorrectionCoefficients, solver_util.cu:127-144
 void correctionCoeffs(const Mesh& m, const Coeff& uc, const Coeff& vc, Fields& s) {
+#pragma omp parallel for schedule(static)
     for (int n = 0; n < m.N; n++) {
         const double V = m.vol[n];
         s.DU[n] = V * uc.invAC[n];
@@ -857,6 +875,7 @@ void correctionCoeffs(const Mesh& m, const Coeff& uc, const Coeff& vc, Fields& s
 
 // updateMomentumPressure, simple.cu:166-179
 void updateMomentumPressure(const Mesh& m, Fields& s) {
+#pragma omp parallel for schedule(static)
     for (int n = 0; n < m.N; n++) {
         s.u[n] -= s.DU[n] * s.gradPZ[n];
         s.v[n] -= s.DV[n] * s.gradPR[n];
@@ -866,6 +885,7 @@ void updateMomentumPressure(const Mesh& m, Fields& s) {
 
 // updateMassFlux, simple.cu:181-258
 void updateMassFlux(double rho, const Mesh& m, Fields& s, const BC& pbc, int nonOrtho) {
+#pragma omp parallel for schedule(static)
     for (int f = 0; f < m.F; f++) {
         const int ow = m.owner[f], nb = m.neighbor[f];
         const double area = m.area[f];
@@ -897,6 +917,7 @@ void updateMassFlux(double rho, const Mesh& m, Fields& s, const BC& pbc, int non
 
 // jacobi, linear_solver.cu:9-33
 void jacobiSweep(const Coeff& co, const std::vector<double>& xOld, std::vector<double>& xNew) {
+#pragma omp parallel for schedule(static)
     for (int n = 0; n < co.N; n++) {
         const double inv = co.invAC[n];
         double val = co.b[n];
@@ -926,6 +947,7 @@ void gsColorSweep(const Coeff& co, std::vector<double>& x, const std::vector<int
 // residualRaw (sign = false), residuals.cuh:17-48
 void residualAbs(const Coeff& co, const std::vector<double>& x, int scaleType,
                  std::vector<double>& res, std::vector<double>& scale) {
+#pragma omp parallel for schedule(static)
     for (int n = 0; n < co.N; n++) {
         double Ax = co.AC[n] * x[n];
         if (scaleType == AXS_SCALE_DIAGONAL) scale[n] = Ax;
@@ -940,6 +962,7 @@ void residualAbs(const Coeff& co, const std::vector<double>& x, int scaleType,
 
 // continuityResidual, residuals.cu:5-35
 void continuity(const Mesh& m, const Fields& s, std::vector<double>& res) {
+#pragma omp parallel for schedule(static)
     for (int n = 0; n < m.N; n++) {
         double imb = 0.0;
         for (int k = m.faceStart[n]; k < m.faceStart[n + 1]; k++) {
@@ -1283,6 +1306,9 @@ orc_solver* orc_create(const axs_mesh_view* v, const axs_config* cfg) {
 }
 
 void orc_destroy(orc_solver* h) { delete h; }
+// threads of the OpenMP loops (n >= 1); the results do not depend on it
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int orc_get_threads() { return omp_get_max_threads(); }
 
 int orc_set_bcs(orc_solver* h, int var, const axs_bc_view* v) {
     if (var == AXS_VAR_U) loadBC(h->o.bcU, v);

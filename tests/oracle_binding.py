@@ -47,11 +47,14 @@ def olib() -> C.CDLL:
                                        C.c_int]),
             "orc_mg_get_operator": (C.c_int, [V, C.c_int, _dp, _dp]),
             "orc_residual_value": (None, [C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_int, _dp, _dp]),
+            "orc_set_threads": (None, [C.c_int]),
+            "orc_get_threads": (C.c_int, []),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
+        L.orc_set_threads(1)  # one thread unless a caller asks for more (set_threads)
         _L = L
     return _L
 
@@ -62,6 +65,13 @@ def _d(a):
 
 def _i(a):
     return a.ctypes.data_as(_ip)
+
+
+def set_threads(n: int) -> int:
+    """Threads of the oracle's per-cell / per-face loops (results do not depend
+    on it); returns the count in effect."""
+    olib().orc_set_threads(int(n))
+    return olib().orc_get_threads()
 
 
 class Oracle:

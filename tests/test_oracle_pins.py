@@ -532,3 +532,23 @@ def test_poiseuille_pipe_converges():
     gz = o.field("grad_pz")
     dev = sel & (z < 0.19)
     assert np.mean(gz[dev]) == pytest.approx(-0.4, rel=0.03)
+
+
+def test_oracle_threads_do_not_change_results():
+    """The oracle's OpenMP loops each write only their own cell, face or row
+    slots, so a run on several threads is bit-identical to one thread (the
+    multi-thread CPU baseline measures the same computation)."""
+    from oracle_binding import set_threads
+
+    case = axs.nozzle_case(cells=2.0e4, max_iter=4, tol=0.0)
+    fields = {}
+    try:
+        for th in (1, 4):
+            set_threads(th)
+            o = Oracle(case.mesh, case.bcs, case.config)
+            o.run()
+            fields[th] = {k: o.field(k).copy() for k in ("u", "v", "p", "pp", "mdot")}
+    finally:
+        set_threads(1)
+    for k in fields[1]:
+        np.testing.assert_array_equal(fields[4][k], fields[1][k], err_msg=k)
