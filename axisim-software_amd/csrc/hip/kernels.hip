@@ -452,8 +452,10 @@ __global__ void __launch_bounds__(kBlock) kMomentum(DMesh m, DBC bu, DBC bv, DBC
     const double guzN = ph.nonOrtho ? s.gUZ[n] : 0.0, gurN = ph.nonOrtho ? s.gUR[n] : 0.0;
     const double gvzN = ph.nonOrtho ? s.gVZ[n] : 0.0, gvrN = ph.nonOrtho ? s.gVR[n] : 0.0;
     // one slot (the values at the neighbour o already loaded); returns AF
-    auto step = [&](int k, int o, uint8_t fl, double area, double invDPN, double md, double tz, double tr,
-                    double w, double guzO, double gurO, double gvzO, double gvrO, double uO, double vO) -> double {
+    // noU / noV: the slot's non-orthogonal diffusion terms (nonOrthoVal),
+    // evaluated before the walk and added at the same point of it
+    auto step = [&](int k, int o, uint8_t fl, double area, double invDPN, double md, double noU, double noV,
+                    double w, double uO, double vO) -> double {
         double F = 0.0;
         if (ph.conv) {
             // a boundary face is always owned by its one cell
@@ -467,7 +469,7 @@ __global__ void __launch_bounds__(kBlock) kMomentum(DMesh m, DBC bu, DBC bv, DBC
             double af = 0.0;
             acU += K;
             af -= K;
-            if (ph.nonOrtho) bU += nonOrthoVal(tz, tr, w, guzN, guzO, gurN, gurO, mu);
+            if (ph.nonOrtho) bU += noU;
             if (F != 0.0) {
                 acU += fmax(F, 0.0);
                 af += fmin(F, 0.0);
@@ -478,7 +480,7 @@ __global__ void __launch_bounds__(kBlock) kMomentum(DMesh m, DBC bu, DBC bv, DBC
                 }
             }
             acV += K;
-            if (ph.nonOrtho) bV += nonOrthoVal(tz, tr, w, gvzN, gvzO, gvrN, gvrO, mu);
+            if (ph.nonOrtho) bV += noV;
             if (F != 0.0) {
                 acV += fmax(F, 0.0);
                 if (HO) {
@@ -497,37 +499,54 @@ __global__ void __launch_bounds__(kBlock) kMomentum(DMesh m, DBC bu, DBC bv, DBC
     if (ns > 0 && ns <= B) {
         int o[B], fid[B];
         uint8_t fl[B];
-        double area[B], inv[B], tz[B], tr[B], w[B];
-        double md[B], guz[B], gur[B], gvz[B], gvr[B];
-        double uo[B], vo[B], af[B];
+        double area[B], inv[B], w[B], md[B], uo[B], vo[B], af[B];
+        double noU[B], noV[B];
+#pragma unroll
+        for (int q = 0; q < B; q++) o[q] = m.slotNb[slotAt(k0, ns, q)];
+        if (ph.nonOrtho) {
+            // first pass: the non-orthogonal terms, so that their geometry and
+            // the neighbours' gradients are not held through the main walk
+            double tz[B], tr[B], wn[B], guz[B], gur[B], gvz[B], gvr[B];
+#pragma unroll
+            for (int q = 0; q < B; q++) {
+                const int k = slotAt(k0, ns, q);
+                const int oc = o[q] >= 0 ? o[q] : n;
+                tz[q] = m.sTZ[k];
+                tr[q] = m.sTR[k];
+                wn[q] = m.sW[k];
+                guz[q] = s.gUZ[oc];
+                gur[q] = s.gUR[oc];
+                gvz[q] = s.gVZ[oc];
+                gvr[q] = s.gVR[oc];
+            }
+#pragma unroll
+            for (int q = 0; q < B; q++) {
+                noU[q] = nonOrthoVal(tz[q], tr[q], wn[q], guzN, guz[q], gurN, gur[q], mu);
+                noV[q] = nonOrthoVal(tz[q], tr[q], wn[q], gvzN, gvz[q], gvrN, gvr[q], mu);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < B; q++) noU[q] = noV[q] = 0.0;
+        }
 #pragma unroll
         for (int q = 0; q < B; q++) {
             const int k = slotAt(k0, ns, q);
-            o[q] = m.slotNb[k];
             fid[q] = m.faceIDs[k];
             fl[q] = m.sFlags[k];
             area[q] = m.sArea[k];
             inv[q] = m.sInvDPN[k];
-            tz[q] = ph.nonOrtho ? m.sTZ[k] : 0.0;
-            tr[q] = ph.nonOrtho ? m.sTR[k] : 0.0;
-            w[q] = needW ? m.sW[k] : 0.0;
+            w[q] = HO ? m.sW[k] : 0.0;
         }
 #pragma unroll
         for (int q = 0; q < B; q++) {
             const int oc = o[q] >= 0 ? o[q] : n;
             md[q] = ph.conv ? s.mDot[fid[q]] : 0.0;
-            guz[q] = ph.nonOrtho ? s.gUZ[oc] : 0.0;
-            gur[q] = ph.nonOrtho ? s.gUR[oc] : 0.0;
-            gvz[q] = ph.nonOrtho ? s.gVZ[oc] : 0.0;
-            gvr[q] = ph.nonOrtho ? s.gVR[oc] : 0.0;
             uo[q] = needUV ? s.u[oc] : 0.0;
             vo[q] = needUV ? s.v[oc] : 0.0;
         }
 #pragma unroll
         for (int q = 0; q < B; q++)
-            if (q < ns)
-                af[q] = step(k0 + q, o[q], fl[q], area[q], inv[q], md[q], tz[q], tr[q], w[q], guz[q], gur[q], gvz[q],
-                             gvr[q], uo[q], vo[q]);
+            if (q < ns) af[q] = step(k0 + q, o[q], fl[q], area[q], inv[q], md[q], noU[q], noV[q], w[q], uo[q], vo[q]);
 #pragma unroll
         for (int q = 0; q < B; q++)
             if (q < ns) {
@@ -538,11 +557,11 @@ __global__ void __launch_bounds__(kBlock) kMomentum(DMesh m, DBC bu, DBC bv, DBC
         for (int k = k0; k < k0 + ns; k++) {
             const int o = m.slotNb[k];
             const int oc = o >= 0 ? o : n;
+            const double w = needW ? m.sW[k] : 0.0;
+            const double noU = ph.nonOrtho ? nonOrthoVal(m.sTZ[k], m.sTR[k], w, guzN, s.gUZ[oc], gurN, s.gUR[oc], mu) : 0.0;
+            const double noV = ph.nonOrtho ? nonOrthoVal(m.sTZ[k], m.sTR[k], w, gvzN, s.gVZ[oc], gvrN, s.gVR[oc], mu) : 0.0;
             const double a = step(k, o, m.sFlags[k], m.sArea[k], m.sInvDPN[k], ph.conv ? s.mDot[m.faceIDs[k]] : 0.0,
-                                  ph.nonOrtho ? m.sTZ[k] : 0.0, ph.nonOrtho ? m.sTR[k] : 0.0, needW ? m.sW[k] : 0.0,
-                                  ph.nonOrtho ? s.gUZ[oc] : 0.0, ph.nonOrtho ? s.gUR[oc] : 0.0,
-                                  ph.nonOrtho ? s.gVZ[oc] : 0.0, ph.nonOrtho ? s.gVR[oc] : 0.0,
-                                  needUV ? s.u[oc] : 0.0, needUV ? s.v[oc] : 0.0);
+                                  noU, noV, w, needUV ? s.u[oc] : 0.0, needUV ? s.v[oc] : 0.0);
             s.AFuv[k] = a;
             if (s.AF32) s.AF32[k] = (float)a;
         }
