@@ -1352,8 +1352,9 @@ __global__ void __launch_bounds__(kBlock) kMGCoarseOperator(DLevel f, DLevel c, 
     if (me.m >= 0) {
         const int n = me.m, k0 = me.k0, ns = me.ns;
         double d = f.AC[n];
-        if (B > 0 && ns > 0 && ns <= B) {
-            // the row's slots in one round (clamped), summed in slot order
+        if (B > 0 && ns > 0) {
+            // the row's first B slots in one round (clamped), summed in slot
+            // order; a longer row continues slot by slot
             int o[B > 0 ? B : 1], cs[B > 0 ? B : 1];
             double a[B > 0 ? B : 1];
 #pragma unroll
@@ -1366,6 +1367,8 @@ __global__ void __launch_bounds__(kBlock) kMGCoarseOperator(DLevel f, DLevel c, 
 #pragma unroll
             for (int q = 0; q < B; q++)
                 if (q < ns && o[q] >= 0 && cs[q] < 0) d += a[q];
+            for (int k = k0 + B; k < k0 + ns; k++)
+                if (f.nb[k] >= 0 && f.slotToCoarseSlot[k] < 0) d += f.AF[k];
         } else {
             for (int k = k0; k < k0 + ns; k++) {
                 if (f.nb[k] < 0) continue;
@@ -1387,13 +1390,14 @@ __global__ void __launch_bounds__(kBlock) kMGCoarseOperator(DLevel f, DLevel c, 
 
 // computeResidual (residualAll sign = true, residuals.cuh:17-60) into f.res,
 // one thread per fine cell; kMGRestrictSum then gathers it per coarse cell.
-// B > 0: rows of at most B slots load their slots, then their neighbours, in
-// one round each (indices clamped into the row); B = 0 or longer rows: the
-// plain loop.  The arithmetic is the same in either case.
+// B > 0: a row's first B slots, then their neighbours, load in one round each
+// (indices clamped into the row), and a longer row continues slot by slot
+// (the multigrid kernels take B = 4 or 8: coarse rows average 5-6 slots and
+// rarely exceed 8); B = 0: the plain loop.  The sums run in slot order either way.
 template <class T, int B>
 __device__ __forceinline__ T mgResidualRowAt(const DLevel& f, const LV<T>& v, int n, int k0, int ns) {
     T Ax = v.AC[n] * v.x[n];
-    if (B > 0 && ns > 0 && ns <= B) {
+    if (B > 0 && ns > 0) {
         int o[B > 0 ? B : 1];
         T a[B > 0 ? B : 1], xo[B > 0 ? B : 1];
 #pragma unroll
@@ -1407,6 +1411,10 @@ __device__ __forceinline__ T mgResidualRowAt(const DLevel& f, const LV<T>& v, in
 #pragma unroll
         for (int q = 0; q < B; q++)
             if (q < ns && o[q] >= 0) Ax += a[q] * xo[q];
+        for (int k = k0 + B; k < k0 + ns; k++) {  // slots past the batch, in order
+            const int nb = f.nb[k];
+            if (nb >= 0) Ax += v.AF[k] * v.x[nb];
+        }
     } else {
         for (int k = k0; k < k0 + ns; k++) {
             const int nb = f.nb[k];
@@ -1442,7 +1450,7 @@ __global__ void __launch_bounds__(kBlock) kMGJacobi(DLevel L, T w, const Ctl* ct
     rowRange(L.start, L.uniformRow, n, k0, ns);
     const T AC = v.AC[n], xn = v.x[n], bn = v.b[n];
     T r = xn;
-    if (B > 0 && ns > 0 && ns <= B) {
+    if (B > 0 && ns > 0) {
         int o[B > 0 ? B : 1];
         T a[B > 0 ? B : 1], xo[B > 0 ? B : 1];
 #pragma unroll
@@ -1458,6 +1466,10 @@ __global__ void __launch_bounds__(kBlock) kMGJacobi(DLevel L, T w, const Ctl* ct
 #pragma unroll
             for (int q = 0; q < B; q++)
                 if (q < ns && o[q] != -1) Ax += a[q] * xo[q];
+            for (int k = k0 + B; k < k0 + ns; k++) {  // slots past the batch, in order
+                const int nb = L.nb[k];
+                if (nb != -1) Ax += v.AF[k] * v.x[nb];
+            }
             r = xn + w * (bn - Ax) / AC;
         }
     } else if (!(tabs(AC) < T(1.0e-30))) {
@@ -1522,7 +1534,7 @@ __global__ void __launch_bounds__(kBlock) kMGProlongSmooth(DLevel f, DLevel c, T
     };
     int k0, ns;
     rowRange(f.start, f.uniformRow, n, k0, ns);
-    if (B > 0 && ns > 0 && ns <= B) {
+    if (B > 0 && ns > 0) {
         int o[B > 0 ? B : 1], tc[B > 0 ? B : 1];
         T a[B > 0 ? B : 1], xo[B > 0 ? B : 1], co[B > 0 ? B : 1];
 #pragma unroll
@@ -1551,6 +1563,10 @@ __global__ void __launch_bounds__(kBlock) kMGProlongSmooth(DLevel f, DLevel c, T
 #pragma unroll
         for (int q = 0; q < B; q++)
             if (q < ns && o[q] != -1) Ax += a[q] * (xo[q] + co[q]);
+        for (int k = k0 + B; k < k0 + ns; k++) {  // slots past the batch, in order
+            const int nb = f.nb[k];
+            if (nb != -1) Ax += v.AF[k] * (v.x[nb] + cv.x[f.toCoarse[nb]]);
+        }
         put(xn + w * (bn - Ax) / AC);
         return;
     }
@@ -2150,9 +2166,7 @@ void launchMGCoarseDiag(hipStream_t s, const DLevel& f, const Ctl* ctl) {
 void launchMGCoarseOperator(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl) {
     if (f.nAggBlocks <= 0) throw std::logic_error("level has no aggregate blocks");
     if (f.maxRow <= 4) kMGCoarseOperator<4><<<f.nAggBlocks, kBlock, 0, s>>>(f, c, ctl);
-    else if (f.maxRow <= 8) kMGCoarseOperator<8><<<f.nAggBlocks, kBlock, 0, s>>>(f, c, ctl);
-    else if (f.maxRow <= 16) kMGCoarseOperator<16><<<f.nAggBlocks, kBlock, 0, s>>>(f, c, ctl);
-    else kMGCoarseOperator<0><<<f.nAggBlocks, kBlock, 0, s>>>(f, c, ctl);
+    else kMGCoarseOperator<8><<<f.nAggBlocks, kBlock, 0, s>>>(f, c, ctl);
 }
 void launchMGCoarseGather(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl) {
     kMGCoarseGather<<<blocksFor((long)c.refs + c.n), kBlock, 0, s>>>(f, c, ctl);
@@ -2162,26 +2176,20 @@ template <class T>
 static void launchMGResidualT(hipStream_t s, const DLevel& f, const Ctl* ctl) {
     const dim3 g(blocksFor(f.n, f.block)), b(f.block);
     if (f.maxRow <= 4) kMGResidual<T, 4><<<g, b, 0, s>>>(f, ctl);
-    else if (f.maxRow <= 8) kMGResidual<T, 8><<<g, b, 0, s>>>(f, ctl);
-    else if (f.maxRow <= 16) kMGResidual<T, 16><<<g, b, 0, s>>>(f, ctl);
-    else kMGResidual<T, 0><<<g, b, 0, s>>>(f, ctl);
+    else kMGResidual<T, 8><<<g, b, 0, s>>>(f, ctl);
 }
 template <class T>
 static void launchMGJacobiT(hipStream_t s, const DLevel& L, T w, const Ctl* ctl) {
     const dim3 g(blocksFor(L.n, L.block)), b(L.block);
     if (L.maxRow <= 4) kMGJacobi<T, 4><<<g, b, 0, s>>>(L, w, ctl);
-    else if (L.maxRow <= 8) kMGJacobi<T, 8><<<g, b, 0, s>>>(L, w, ctl);
-    else if (L.maxRow <= 16) kMGJacobi<T, 16><<<g, b, 0, s>>>(L, w, ctl);
-    else kMGJacobi<T, 0><<<g, b, 0, s>>>(L, w, ctl);
+    else kMGJacobi<T, 8><<<g, b, 0, s>>>(L, w, ctl);
 }
 template <class T>
 static void launchMGProlongSmoothT(hipStream_t s, const DLevel& f, const DLevel& c, T w, const Ctl* ctl,
                                    double* apply = nullptr) {
     const dim3 g(blocksFor(f.n, f.block)), b(f.block);
     if (f.maxRow <= 4) kMGProlongSmooth<T, 4><<<g, b, 0, s>>>(f, c, w, apply, ctl);
-    else if (f.maxRow <= 8) kMGProlongSmooth<T, 8><<<g, b, 0, s>>>(f, c, w, apply, ctl);
-    else if (f.maxRow <= 16) kMGProlongSmooth<T, 16><<<g, b, 0, s>>>(f, c, w, apply, ctl);
-    else kMGProlongSmooth<T, 0><<<g, b, 0, s>>>(f, c, w, apply, ctl);
+    else kMGProlongSmooth<T, 8><<<g, b, 0, s>>>(f, c, w, apply, ctl);
 }
 void launchMGResidual(hipStream_t s, const DLevel& f, const Ctl* ctl) { launchMGResidualT<double>(s, f, ctl); }
 void launchMGResidual32(hipStream_t s, const DLevel& f, const Ctl* ctl) { launchMGResidualT<float>(s, f, ctl); }
@@ -2195,9 +2203,7 @@ static void launchMGResidualRestrictT(hipStream_t s, const DLevel& f, const DLev
                                       const Ctl* ctl) {
     const dim3 g(f.nAggBlocks), b(kBlock);
     if (f.maxRow <= 4) kMGResidualRestrict<T, 4><<<g, b, 0, s>>>(f, c, w, sweep1, ctl);
-    else if (f.maxRow <= 8) kMGResidualRestrict<T, 8><<<g, b, 0, s>>>(f, c, w, sweep1, ctl);
-    else if (f.maxRow <= 16) kMGResidualRestrict<T, 16><<<g, b, 0, s>>>(f, c, w, sweep1, ctl);
-    else kMGResidualRestrict<T, 0><<<g, b, 0, s>>>(f, c, w, sweep1, ctl);
+    else kMGResidualRestrict<T, 8><<<g, b, 0, s>>>(f, c, w, sweep1, ctl);
 }
 void launchMGResidualRestrict(hipStream_t s, const DLevel& f, const DLevel& c, double w, int sweep1, bool fp32,
                               const Ctl* ctl) {
