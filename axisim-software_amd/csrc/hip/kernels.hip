@@ -1392,8 +1392,8 @@ __global__ void __launch_bounds__(kBlock) kMGCoarseOperator(DLevel f, DLevel c, 
 // one thread per fine cell; kMGRestrictSum then gathers it per coarse cell.
 // B > 0: a row's first B slots, then their neighbours, load in one round each
 // (indices clamped into the row), and a longer row continues slot by slot
-// (the multigrid kernels take B = 4 or 8: coarse rows average 5-6 slots and
-// rarely exceed 8); B = 0: the plain loop.  The sums run in slot order either way.
+// (the multigrid kernels take B = 3, 4 or 8: coarse rows average 5-6 slots
+// and rarely exceed 8); B = 0: the plain loop.  The sums run in slot order either way.
 template <class T, int B>
 __device__ __forceinline__ T mgResidualRowAt(const DLevel& f, const LV<T>& v, int n, int k0, int ns) {
     T Ax = v.AC[n] * v.x[n];
@@ -2165,7 +2165,8 @@ void launchMGCoarseDiag(hipStream_t s, const DLevel& f, const Ctl* ctl) {
 }
 void launchMGCoarseOperator(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl) {
     if (f.nAggBlocks <= 0) throw std::logic_error("level has no aggregate blocks");
-    if (f.maxRow <= 4) kMGCoarseOperator<4><<<f.nAggBlocks, kBlock, 0, s>>>(f, c, ctl);
+    if (f.maxRow <= 3) kMGCoarseOperator<3><<<f.nAggBlocks, kBlock, 0, s>>>(f, c, ctl);
+    else if (f.maxRow <= 4) kMGCoarseOperator<4><<<f.nAggBlocks, kBlock, 0, s>>>(f, c, ctl);
     else kMGCoarseOperator<8><<<f.nAggBlocks, kBlock, 0, s>>>(f, c, ctl);
 }
 void launchMGCoarseGather(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl) {
@@ -2188,7 +2189,8 @@ template <class T>
 static void launchMGProlongSmoothT(hipStream_t s, const DLevel& f, const DLevel& c, T w, const Ctl* ctl,
                                    double* apply = nullptr) {
     const dim3 g(blocksFor(f.n, f.block)), b(f.block);
-    if (f.maxRow <= 4) kMGProlongSmooth<T, 4><<<g, b, 0, s>>>(f, c, w, apply, ctl);
+    if (f.maxRow <= 3) kMGProlongSmooth<T, 3><<<g, b, 0, s>>>(f, c, w, apply, ctl);
+    else if (f.maxRow <= 4) kMGProlongSmooth<T, 4><<<g, b, 0, s>>>(f, c, w, apply, ctl);
     else kMGProlongSmooth<T, 8><<<g, b, 0, s>>>(f, c, w, apply, ctl);
 }
 void launchMGResidual(hipStream_t s, const DLevel& f, const Ctl* ctl) { launchMGResidualT<double>(s, f, ctl); }
