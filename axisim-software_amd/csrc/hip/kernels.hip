@@ -2218,10 +2218,12 @@ void launchMGRestrictSum(hipStream_t s, const DLevel& f, const DLevel& c, double
     const dim3 g(blocksFor(c.n, c.block)), b(c.block);
     if (fp32) {
         if (f.maxMembers <= 4) kMGRestrictSum<float, 4><<<g, b, 0, s>>>(f, c, w, sweep1, ctl);
+        else if (f.maxMembers <= 6) kMGRestrictSum<float, 6><<<g, b, 0, s>>>(f, c, w, sweep1, ctl);
         else if (f.maxMembers <= 8) kMGRestrictSum<float, 8><<<g, b, 0, s>>>(f, c, w, sweep1, ctl);
         else kMGRestrictSum<float, 0><<<g, b, 0, s>>>(f, c, w, sweep1, ctl);
     } else {
         if (f.maxMembers <= 4) kMGRestrictSum<double, 4><<<g, b, 0, s>>>(f, c, w, sweep1, ctl);
+        else if (f.maxMembers <= 6) kMGRestrictSum<double, 6><<<g, b, 0, s>>>(f, c, w, sweep1, ctl);
         else if (f.maxMembers <= 8) kMGRestrictSum<double, 8><<<g, b, 0, s>>>(f, c, w, sweep1, ctl);
         else kMGRestrictSum<double, 0><<<g, b, 0, s>>>(f, c, w, sweep1, ctl);
     }
