@@ -1392,8 +1392,8 @@ __global__ void __launch_bounds__(kBlock) kMGCoarseOperator(DLevel f, DLevel c, 
 // one thread per fine cell; kMGRestrictSum then gathers it per coarse cell.
 // B > 0: a row's first B slots, then their neighbours, load in one round each
 // (indices clamped into the row), and a longer row continues slot by slot
-// (the multigrid kernels take B = 3, 4 or 8: coarse rows average 5-6 slots
-// and rarely exceed 8); B = 0: the plain loop.  The sums run in slot order either way.
+// (the multigrid kernels take B = 3, 4, 7 or 8 by the level's longest row:
+// coarse rows average 5-6 slots and rarely exceed 8); B = 0: the plain loop.  The sums run in slot order either way.
 template <class T, int B>
 __device__ __forceinline__ T mgResidualRowAt(const DLevel& f, const LV<T>& v, int n, int k0, int ns) {
     T Ax = v.AC[n] * v.x[n];
@@ -1857,7 +1857,7 @@ __global__ void __launch_bounds__(1024) kMGSmoothBlockGS(DLevel L, const int* __
 // goes back to its x buffer.
 // ===========================================================================
 namespace {
-constexpr int kRowK = 12;     // slots a row keeps in registers (longer rows loop over LDS)
+constexpr int kRowK = 10;     // slots a row keeps in registers (longer rows loop over LDS)
 constexpr int kMemberK = 8;   // aggregate members gathered without a loop
 constexpr int kCopyR = 8;     // staged 16-byte vectors each thread holds per round
 
@@ -2167,6 +2167,7 @@ void launchMGCoarseOperator(hipStream_t s, const DLevel& f, const DLevel& c, con
     if (f.nAggBlocks <= 0) throw std::logic_error("level has no aggregate blocks");
     if (f.maxRow <= 3) kMGCoarseOperator<3><<<f.nAggBlocks, kBlock, 0, s>>>(f, c, ctl);
     else if (f.maxRow <= 4) kMGCoarseOperator<4><<<f.nAggBlocks, kBlock, 0, s>>>(f, c, ctl);
+    else if (f.maxRow <= 7) kMGCoarseOperator<7><<<f.nAggBlocks, kBlock, 0, s>>>(f, c, ctl);
     else kMGCoarseOperator<8><<<f.nAggBlocks, kBlock, 0, s>>>(f, c, ctl);
 }
 void launchMGCoarseGather(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl) {
@@ -2191,6 +2192,7 @@ static void launchMGProlongSmoothT(hipStream_t s, const DLevel& f, const DLevel&
     const dim3 g(blocksFor(f.n, f.block)), b(f.block);
     if (f.maxRow <= 3) kMGProlongSmooth<T, 3><<<g, b, 0, s>>>(f, c, w, apply, ctl);
     else if (f.maxRow <= 4) kMGProlongSmooth<T, 4><<<g, b, 0, s>>>(f, c, w, apply, ctl);
+    else if (f.maxRow <= 7) kMGProlongSmooth<T, 7><<<g, b, 0, s>>>(f, c, w, apply, ctl);
     else kMGProlongSmooth<T, 8><<<g, b, 0, s>>>(f, c, w, apply, ctl);
 }
 void launchMGResidual(hipStream_t s, const DLevel& f, const Ctl* ctl) { launchMGResidualT<double>(s, f, ctl); }
@@ -2205,6 +2207,7 @@ static void launchMGResidualRestrictT(hipStream_t s, const DLevel& f, const DLev
                                       const Ctl* ctl) {
     const dim3 g(f.nAggBlocks), b(kBlock);
     if (f.maxRow <= 4) kMGResidualRestrict<T, 4><<<g, b, 0, s>>>(f, c, w, sweep1, ctl);
+    else if (f.maxRow <= 7) kMGResidualRestrict<T, 7><<<g, b, 0, s>>>(f, c, w, sweep1, ctl);
     else kMGResidualRestrict<T, 8><<<g, b, 0, s>>>(f, c, w, sweep1, ctl);
 }
 void launchMGResidualRestrict(hipStream_t s, const DLevel& f, const DLevel& c, double w, int sweep1, bool fp32,
