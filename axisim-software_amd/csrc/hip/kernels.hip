@@ -1918,6 +1918,20 @@ __device__ __forceinline__ T rowAx(const TRow<T>& r, T xn, XAt xat, const T* AF,
 }
 }  // namespace
 
+// jacobiRow of a register row with at most K slots, its neighbours read from
+// x in one round (missing ones read the row's own entry and are not used)
+template <int K, class T>
+__device__ __forceinline__ T waveRowSweep(const TRow<T>& r, const T* x, T xn, T w, int self) {
+    T xv[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) xv[k] = x[r.nb[k] != -1 ? r.nb[k] : self];
+    T Ax = r.ac * xn;
+#pragma unroll
+    for (int k = 0; k < K; k++)
+        if (r.nb[k] != -1) Ax += r.af[k] * xv[k];
+    return xn + w * (r.b - Ax) / r.ac;
+}
+
 template <class T>
 __global__ void __launch_bounds__(1024) kMGTail(const TailLevel* __restrict__ tl, uint4* __restrict__ image,
                                                 int vecs, int nl, int intBase, double wd, int preDone, int pre,
@@ -1965,9 +1979,11 @@ __global__ void __launch_bounds__(1024) kMGTail(const TailLevel* __restrict__ tl
                 unsigned f = flip;
                 // rows of at most kWaveK slots (the coarsest level's are) sweep
                 // with a shorter register row: fewer dependent steps per sweep
+                // (6 or 8 slots, whichever covers the wave's longest row)
                 constexpr int kWaveK = 8;
                 static_assert(kWaveK <= kRowK, "a short row is a prefix of the register row");
                 const bool shortRows = __all(!own || r.deg <= kWaveK);
+                const bool shorterRows = __all(!own || r.deg <= 6);
                 const bool keep = own && tabs(r.ac) < T(1.0e-30);
                 for (int it = 0; it < count; it++) {
                     const T* x = sT + (((f >> q) & 1u) ? S.x1 : S.x0);
@@ -1977,15 +1993,10 @@ __global__ void __launch_bounds__(1024) kMGTail(const TailLevel* __restrict__ tl
                         T res;
                         if (keep) {
                             res = xn;
+                        } else if (shorterRows) {
+                            res = waveRowSweep<6>(r, x, xn, w, t);
                         } else if (shortRows) {
-                            T xv[kWaveK];
-#pragma unroll
-                            for (int k = 0; k < kWaveK; k++) xv[k] = x[r.nb[k] != -1 ? r.nb[k] : t];
-                            T Ax = r.ac * xn;
-#pragma unroll
-                            for (int k = 0; k < kWaveK; k++)
-                                if (r.nb[k] != -1) Ax += r.af[k] * xv[k];
-                            res = xn + w * (r.b - Ax) / r.ac;
+                            res = waveRowSweep<kWaveK>(r, x, xn, w, t);
                         } else {
                             const T Ax = rowAx(r, xn, [&](int j) { return x[j]; }, AF, nbs, t);
                             res = xn + w * (r.b - Ax) / r.ac;
