@@ -214,6 +214,16 @@ void launchMGCoarseDiag(hipStream_t s, const DLevel& f, const Ctl* ctl);
 void launchMGCoarseGather(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl);
 // both in one launch over f.aggBlocks (the fine diagonals stay in LDS)
 void launchMGCoarseOperator(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl);
+// kMGRestrictSum into level 1 followed by level 1's kMGResidualRestrict into
+// level 2, in one launch over level 1's aggregate workgroups: each workgroup
+// forms b and x of its level-1 cells and of the halo cells their rows read.
+constexpr int kL1Slots = 8;  // row slots of a level-1 cell the LDS-index table holds
+struct L1Fused {
+    const int* halo = nullptr;     // nAggBlocks * kBlock: halo cell of (block, thread), -1: none
+    const int16_t* loc = nullptr;  // nAggBlocks * kBlock * kL1Slots: LDS index of each slot's neighbour, -1: none
+};
+void launchMGRestrictFused01(hipStream_t s, const DLevel& l0, const DLevel& l1, const DLevel& l2, L1Fused t,
+                             double w, int sweep1, int sweep2, const Ctl* ctl);
 // residual per fine cell + ordered gather per coarse cell (sweep1: also the
 // coarse level's first smoothing sweep from zero); fp32: the FP32 copies
 void launchMGResidual(hipStream_t s, const DLevel& f, const Ctl* ctl);

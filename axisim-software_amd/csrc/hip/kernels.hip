@@ -1789,6 +1789,84 @@ __global__ void __launch_bounds__(kBlock) kMGResidualRestrict(DLevel f, DLevel c
     }
 }
 
+// kMGRestrictSum into level 1 (the level-0 residuals gathered per level-1
+// cell in member order, then the first level-1 sweep from zero) and level 1's
+// kMGResidualRestrict into level 2, in one launch over level 1's aggregate
+// workgroups (f = level 1).  Thread t forms b and x of its level-1 cell (as
+// kMGRestrictSum does, and writes them) and of its halo cell, one of the
+// level-1 cells the workgroup's rows read (t.halo); after the barrier the
+// residual of its row reads the x it needs from LDS through the host-built
+// slot table (t.loc), in slot order as in kMGResidual; after the next barrier
+// the level-2 sums run as in kMGResidualRestrict.  Same values, bit for bit.
+template <int B>
+__device__ __forceinline__ double memberSum(const DLevel& g, int i) {
+    const int j0 = g.memberStart[i], nm = g.memberStart[i + 1] - j0;
+    int mb[B];
+    double r[B];
+#pragma unroll
+    for (int q = 0; q < B; q++) mb[q] = g.members[nm > 0 ? slotAt(j0, nm, q) : 0];
+#pragma unroll
+    for (int q = 0; q < B; q++) r[q] = g.res[mb[q]];
+    double acc = 0.0;
+#pragma unroll
+    for (int q = 0; q < B; q++)
+        if (q < nm) acc += r[q];
+    for (int j = j0 + B; j < j0 + nm; j++) acc += g.res[g.members[j]];
+    return acc;
+}
+
+__global__ void __launch_bounds__(kBlock) kMGRestrictFused01(DLevel g, DLevel f, DLevel c, L1Fused tb, double w,
+                                                             int sweep1, int sweep2, const Ctl* ctl) {
+    __shared__ double xs[2 * kBlock];
+    __shared__ double r[kBlock];
+    if (finished(ctl)) return;
+    const int blk = xcdBlock(), t = threadIdx.x;
+    const AggThread me = f.aggThreads[(size_t)blk * kBlock + t];
+    const int hc = tb.halo[(size_t)blk * kBlock + t];
+    const int j = f.aggBlocks[blk].j0 + t;
+    double bOwn = 0.0, xOwn = 0.0;
+    if (me.m >= 0) {
+        bOwn = memberSum<6>(g, me.m);
+        xOwn = sweep1 ? sweepFromZero(bOwn, f.AC[me.m], w) : 0.0;
+        f.b[me.m] = bOwn;
+        f.x[me.m] = xOwn;
+        xs[t] = xOwn;
+    }
+    if (hc >= 0) {
+        const double bh = memberSum<6>(g, hc);
+        xs[kBlock + t] = sweep1 ? sweepFromZero(bh, f.AC[hc], w) : 0.0;
+    }
+    const double acj = (sweep2 && me.sum >= 0) ? c.AC[j] : 0.0;
+    __syncthreads();
+    if (me.m >= 0) {
+        constexpr int K = kL1Slots;
+        const int k0 = me.k0, ns = me.ns;
+        int lc[K];
+        double a[K], xv[K];
+        const int16_t* l = tb.loc + ((size_t)blk * kBlock + t) * K;
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            lc[q] = l[q];
+            a[q] = f.AF[ns > 0 ? slotAt(k0, ns, q) : 0];
+        }
+#pragma unroll
+        for (int q = 0; q < K; q++) xv[q] = xs[lc[q] >= 0 ? lc[q] : t];
+        double Ax = f.AC[me.m] * xOwn;
+#pragma unroll
+        for (int q = 0; q < K; q++)
+            if (q < ns && lc[q] >= 0) Ax += a[q] * xv[q];
+        r[t] = bOwn - Ax;
+    }
+    __syncthreads();
+    if (me.sum >= 0) {
+        const int a = me.sum & 0xffff, e = me.sum >> 16;
+        double acc = 0.0;
+        for (int q = a; q < e; q++) acc += r[q];
+        c.b[j] = acc;
+        c.x[j] = sweep2 ? sweepFromZero(acc, acj, w) : 0.0;
+    }
+}
+
 // ===========================================================================
 // Multi-colour Gauss-Seidel smoothing of a level (build extension; the
 // reference smooths with weighted Jacobi only).  Cells of one colour share no
@@ -2226,6 +2304,12 @@ void launchMGResidualRestrict(hipStream_t s, const DLevel& f, const DLevel& c, d
     if (f.nAggBlocks <= 0) throw std::logic_error("level has no aggregate blocks");
     if (fp32) launchMGResidualRestrictT<float>(s, f, c, w, sweep1, ctl);
     else launchMGResidualRestrictT<double>(s, f, c, w, sweep1, ctl);
+}
+void launchMGRestrictFused01(hipStream_t s, const DLevel& l0, const DLevel& l1, const DLevel& l2, L1Fused t,
+                             double w, int sweep1, int sweep2, const Ctl* ctl) {
+    if (l1.nAggBlocks <= 0 || !t.halo || !t.loc || l1.maxRow > kL1Slots)
+        throw std::logic_error("level 1 has no fused restriction tables");
+    kMGRestrictFused01<<<l1.nAggBlocks, kBlock, 0, s>>>(l0, l1, l2, t, w, sweep1, sweep2, ctl);
 }
 void launchMGRestrictSum(hipStream_t s, const DLevel& f, const DLevel& c, double w, int sweep1, bool fp32,
                          const Ctl* ctl) {

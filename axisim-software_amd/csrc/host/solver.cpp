@@ -48,6 +48,7 @@ void MGDevice::build(int n, const std::vector<int>& start, const std::vector<int
     tail32_ = TailPlan{};
     if (tail64_.lt >= 0) tail64_.image = arena.zeros<unsigned char>(tail64_.bytes);
     dev.assign(host.size(), DLevel{});
+    aggThreadsHost_.assign(host.size(), {});
     for (size_t l = 0; l < host.size(); l++) {
         const Level& h = host[l];
         DLevel& d = dev[l];
@@ -120,6 +121,7 @@ void MGDevice::build(int n, const std::vector<int>& start, const std::vector<int
                 }
                 d.aggBlocks = arena.upload(blocks);
                 d.aggThreads = arena.upload(threads);
+                aggThreadsHost_[l] = std::move(threads);
                 d.nAggBlocks = nB;
             }
             d.nCoarse = host[l + 1].n;
@@ -155,6 +157,56 @@ void MGDevice::build(int n, const std::vector<int>& start, const std::vector<int
             dl0_.locOwn = arena.upload(l0Blocks_.locOwn);
             dl0_.locHalo = arena.upload(l0Blocks_.locHalo.empty() ? std::vector<int16_t>{-1} : l0Blocks_.locHalo);
             fuseL0_ = true;
+        }
+    }
+    // the level-1 fused restriction: every row's neighbours within its
+    // workgroup's cells or a halo of at most kBlock cells, rows of at most
+    // kL1Slots slots, members of at most 8
+    fuseL1_ = false;
+    l1f_ = L1Fused{};
+    if (fuseL0_ && host.size() >= 3 && !host[1].coarsest() && dev[1].nAggBlocks > 0 && !tail64_.holds(1) &&
+        dev[1].maxRow <= kL1Slots && dev[0].maxMembers <= 8) {
+        const Level& h1 = host[1];
+        const int nB = dev[1].nAggBlocks;
+        const std::vector<AggThread>& th = aggThreadsHost_[1];
+        std::vector<int> halo((size_t)nB * kBlock, -1);
+        std::vector<int16_t> loc((size_t)nB * kBlock * kL1Slots, -1);
+        std::vector<int> local(h1.n, -1), touched;
+        bool ok = true;
+        for (int b = 0; b < nB && ok; b++) {
+            touched.clear();
+            for (int t = 0; t < kBlock; t++) {
+                const int m = th[(size_t)b * kBlock + t].m;
+                if (m >= 0) {
+                    local[m] = t;
+                    touched.push_back(m);
+                }
+            }
+            int nh = 0;
+            for (int t = 0; t < kBlock && ok; t++) {
+                const int m = th[(size_t)b * kBlock + t].m;
+                if (m < 0) continue;
+                for (int k = h1.start[m], q = 0; k < h1.start[m + 1]; k++, q++) {
+                    const int o = h1.nb[k];
+                    if (o < 0) continue;
+                    if (local[o] < 0) {
+                        if (nh == kBlock) {
+                            ok = false;
+                            break;
+                        }
+                        local[o] = kBlock + nh;
+                        halo[(size_t)b * kBlock + nh++] = o;
+                        touched.push_back(o);
+                    }
+                    loc[((size_t)b * kBlock + t) * kL1Slots + q] = (int16_t)local[o];
+                }
+            }
+            for (int c : touched) local[c] = -1;
+        }
+        if (ok) {
+            l1f_.halo = arena.upload(halo);
+            l1f_.loc = arena.upload(loc);
+            fuseL1_ = true;
         }
     }
 }
@@ -342,9 +394,21 @@ bool MGDevice::fusesLevel0(const MGParams& p) const {
 // The same sweeps and residuals as vcycle(0) repeated, bit for bit.
 void MGDevice::cyclesFused(hipStream_t s, int cycles, const MGParams& p, const Ctl* ctl) {
     const int s1 = host[1].coarsest() ? (p.coarseSweeps >= 1) : (p.pre >= 1);
+    // level 1 folded into the restriction when no pre-sweep runs there between
+    const bool l1 = fuseL1_ && s1 == 1 && p.pre == 1;
+    const int s2 = host.size() < 3 ? 0 : host[2].coarsest() ? (p.coarseSweeps >= 1) : (p.pre >= 1);
     for (int c = 0; c < cycles; c++) {
         launchMGLevel0Block(s, dev[0], dev[1], dl0_, c == 0 ? 1 : 2, c > 0, p.weight, ctl);
         swapX(0);
+        if (l1) {
+            // vcycle(1) with its restriction taken by the fused launch
+            launchMGRestrictFused01(s, dev[0], dev[1], dev[2], l1f_, p.weight, s1, s2, ctl);
+            vcycle(s, 2, s2, p, ctl);
+            launchMGProlongSmooth(s, dev[1], dev[2], p.weight, ctl);
+            swapX(1);
+            smooth(s, 1, p.post - 1, p, ctl);
+            continue;
+        }
         launchMGRestrictSum(s, dev[0], dev[1], p.weight, s1, false, ctl);
         vcycle(s, 1, s1, p, ctl);
     }

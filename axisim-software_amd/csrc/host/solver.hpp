@@ -143,6 +143,12 @@ private:
     SweepBlocks l0Blocks_;
     DSweepBlocks dl0_{};
     bool fuseL0_ = false;
+    // the restriction into level 1 and level 1's residual + restriction in one
+    // launch (fused level-0 cycles): per level-1 aggregate workgroup, the halo
+    // of level-1 cells its rows read and each row slot's LDS index
+    L1Fused l1f_{};
+    bool fuseL1_ = false;
+    std::vector<std::vector<AggThread>> aggThreadsHost_;
     static constexpr int kFuseL0MinCells = 16384;
     bool fusesLevel0(const MGParams& p) const;
     void cyclesFused(hipStream_t s, int cycles, const MGParams& p, const Ctl* ctl);
