@@ -719,7 +719,10 @@ double MGDevice::bytesPerSolve(const MGParams& p, bool buildOperators) const {
         l0 += (cycles - 1) * ((5.0 * v + 8.0) * n + (v + 4.0) * S + v * nc);
         l0 += cycles * ((v + 4.0) * n + (2.0 * v + 4.0) * nc + (s1 ? v * nc : 0.0));
         l0 += (4.0 * v + 8.0) * n + (v + 4.0) * S + v * nc;
-        return bytes + l0 + cycles * cycleFrom(1, s1);
+        // with level 1 folded into the restriction, its b and x come from LDS,
+        // not from a second read
+        const double l1 = (fuseL1_ && s1 == 1 && p.pre == 1) ? 2.0 * v * nc : 0.0;
+        return bytes + l0 + cycles * (cycleFrom(1, s1) - l1);
     }
     double cyc = 0.0;
     int preDone = 0;
