@@ -1145,7 +1145,7 @@ __global__ void __launch_bounds__(kBlock) kFinish(DState s, int blocks) {
         for (int j = 0; j < 9; j++) red[threadIdx.x][j] = a[j];
         __syncthreads();
         for (int st = kBlock / 2; st > 0; st >>= 1) {
-            if (threadIdx.x < st)
+            if ((int)threadIdx.x < st)
                 for (int j = 0; j < 9; j++) {
                     const double x = red[threadIdx.x + st][j];
                     red[threadIdx.x][j] = (j == 2 || j == 6) ? fmax(red[threadIdx.x][j], x) : red[threadIdx.x][j] + x;

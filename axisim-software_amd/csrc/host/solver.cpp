@@ -662,7 +662,7 @@ double MGDevice::bytesPerSolve(const MGParams& p, bool buildOperators) const {
         };
         double cyc = mixed ? 32.0 * host[0].n + 12.0 * host[0].refs() + 8.0 * host[0].n + 20.0 * host[0].n : 0.0;
         for (int l = 0; l < L; l++) {
-            const double n = host[l].n, S = host[l].refs();
+            const double n = host[l].n;
             if (host[l].coarsest()) {
                 cyc += (mixed && p.coarseDirect) ? 8.0 * n * n + 2.0 * v * n : gs(l, p.coarseSweeps);
                 break;
@@ -746,6 +746,16 @@ Solver::Solver(int device) : device_(device) {
     std::memset(hctl_, 0, sizeof(Ctl));
     axs_config_defaults(&cfg_);
 }
+
+// the learned guess's device state (defined here, before the destructor that
+// deletes it)
+struct Solver::NetSetup {
+    NetGraphDev graph;
+    NetWork work;
+    Arena arena;
+    std::vector<double*> vec;  // per level 1..netLevel_: restricted b, later the prolonged guess
+    double* x = nullptr;       // the guess on netLevel_
+};
 
 Solver::~Solver() {
     (void)hipSetDevice(device_);
@@ -1783,14 +1793,6 @@ void Solver::autotune(int repeats, double* msDefault, double* msTuned) {
 // ---------------------------------------------------------------------------
 // learned initial guess (build extension)
 // ---------------------------------------------------------------------------
-struct Solver::NetSetup {
-    NetGraphDev graph;
-    NetWork work;
-    Arena arena;
-    std::vector<double*> vec;  // per level 1..netLevel_: restricted b, later the prolonged guess
-    double* x = nullptr;       // the guess on netLevel_
-};
-
 void Solver::setLearnedGuess(Mpnn* net, int level) {
     if (running_) throw std::logic_error("solver is running");
     if (level < -1 || level == 0) throw std::invalid_argument("guess level must be -1 or a coarse level >= 1");
