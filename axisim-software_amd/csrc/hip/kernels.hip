@@ -372,13 +372,30 @@ __device__ __forceinline__ double rhieChow(const DMesh& m, const DBC& bu, const 
     return unLin - Df * (gPN - gFN);
 }
 
-// ---- wave64 / workgroup reductions (deterministic for a fixed launch) ----
-__device__ __forceinline__ double waveSum(double v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+// ---- wave64 reductions by DPP (deterministic: a fixed tree) ----
+// v moved across lanes by the DPP control `ctrl` (both 32-bit halves); lanes
+// whose source is outside the enabled rows / banks, or outside the row, keep
+// `identity`
+template <int ctrl, int rowMask>
+__device__ __forceinline__ double dppMove(double v, double identity) {
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(identity), __double2loint(v), ctrl, rowMask, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(identity), __double2hiint(v), ctrl, rowMask, 0xf, false);
+    return __hiloint2double(hi, lo);
 }
-__device__ __forceinline__ double waveMax(double v) {
-    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+// Inclusive scan of a wave's values with VALU moves only (no LDS traffic):
+// row_shr 1, 2, 4, 8 within each 16-lane row, then row_bcast 15 (rows 1, 3
+// add the row before) and row_bcast 31 (rows 2, 3 add rows 0-1).  Lane 63
+// ends with the wave's sum (MAX: maximum; the values are >= 0, so 0 is the
+// identity of both).
+template <bool MAX>
+__device__ __forceinline__ double waveScan(double v) {
+    auto op = [](double a, double b) { return MAX ? fmax(a, b) : a + b; };
+    v = op(v, dppMove<0x111, 0xf>(v, 0.0));  // row_shr:1
+    v = op(v, dppMove<0x112, 0xf>(v, 0.0));  // row_shr:2
+    v = op(v, dppMove<0x114, 0xf>(v, 0.0));  // row_shr:4
+    v = op(v, dppMove<0x118, 0xf>(v, 0.0));  // row_shr:8
+    v = op(v, dppMove<0x142, 0xa>(v, 0.0));  // row_bcast:15
+    v = op(v, dppMove<0x143, 0xc>(v, 0.0));  // row_bcast:31
     return v;
 }
 
@@ -1116,8 +1133,8 @@ __global__ void __launch_bounds__(kBlock) kResidual(DMesh m, DState s) {
     __shared__ double red[kBlock / 64][9];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int j = 0; j < 9; j++) {
-        const double w = (j == 2 || j == 6) ? waveMax(v[j]) : waveSum(v[j]);
-        if (lane == 0) red[wave][j] = w;
+        const double w = (j == 2 || j == 6) ? waveScan<true>(v[j]) : waveScan<false>(v[j]);
+        if (lane == 63) red[wave][j] = w;
     }
     __syncthreads();
     if (threadIdx.x < 9) {
