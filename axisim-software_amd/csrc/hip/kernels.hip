@@ -1147,72 +1147,102 @@ __global__ void __launch_bounds__(kBlock) kResidual(DMesh m, DState s) {
 
 // Final reduction + residualAllHost (residuals.cu:37-100) + residualsConverged
 // (solver.cpp:59-72) + the loop bookkeeping of solver.cpp:1243-1283, on device.
-__global__ void __launch_bounds__(kBlock) kFinish(DState s, int blocks) {
+// One workgroup of kFinishThreads: each thread loads the partials of four
+// blocks in one round (indices clamped, extra loads unused), then each wave
+// reduces by DPP and one thread per quantity combines the waves.
+constexpr int kFinishThreads = 1024;
+__global__ void __launch_bounds__(kFinishThreads) kFinish(DState s, int blocks) {
     Ctl* c = s.ctl;
-    if (c->done) return;
-    const bool check = (c->k % c->checkConv) == 0;
-    __shared__ double red[kBlock][9];
+    // the whole control block in one round of loads; thread 0 works on this
+    // copy and stores back the fields it changes
+    Ctl cc = *c;
+    if (cc.done) return;
+    const bool check = (cc.k % cc.checkConv) == 0;
+    __shared__ double red[kFinishThreads / 64][9];
+    __shared__ double tot[9];
     if (check) {
+        const int t = threadIdx.x;
         double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        for (int b = threadIdx.x; b < blocks; b += kBlock)
-            for (int j = 0; j < 9; j++) {
-                const double x = s.partial[b * 9 + j];
-                a[j] = (j == 2 || j == 6) ? fmax(a[j], x) : a[j] + x;
+        for (int b0 = 0; b0 < blocks; b0 += 4 * kFinishThreads) {
+            double x[4][9];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int b = min(b0 + r * kFinishThreads + t, blocks - 1);
+#pragma unroll
+                for (int j = 0; j < 9; j++) x[r][j] = s.partial[(size_t)b * 9 + j];
             }
-        for (int j = 0; j < 9; j++) red[threadIdx.x][j] = a[j];
-        __syncthreads();
-        for (int st = kBlock / 2; st > 0; st >>= 1) {
-            if ((int)threadIdx.x < st)
-                for (int j = 0; j < 9; j++) {
-                    const double x = red[threadIdx.x + st][j];
-                    red[threadIdx.x][j] = (j == 2 || j == 6) ? fmax(red[threadIdx.x][j], x) : red[threadIdx.x][j] + x;
-                }
-            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                if (b0 + r * kFinishThreads + t < blocks)
+#pragma unroll
+                    for (int j = 0; j < 9; j++) a[j] = (j == 2 || j == 6) ? fmax(a[j], x[r][j]) : a[j] + x[r][j];
         }
+        const int lane = t & 63, wave = t >> 6;
+#pragma unroll
+        for (int j = 0; j < 9; j++) {
+            const double w = (j == 2 || j == 6) ? waveScan<true>(a[j]) : waveScan<false>(a[j]);
+            if (lane == 63) red[wave][j] = w;
+        }
+        __syncthreads();
+        if (t < 9) {
+            double v = red[0][t];
+            for (int w = 1; w < kFinishThreads / 64; w++) v = (t == 2 || t == 6) ? fmax(v, red[w][t]) : v + red[w][t];
+            tot[t] = v;
+        }
+        __syncthreads();
     }
     if (threadIdx.x != 0) return;
     if (check) {
-        const int scaleIter = c->transient ? c->k : c->iter;
+        const int scaleIter = cc.transient ? cc.k : cc.iter;
         for (int i = 0; i < 3; i++) {
-            if (!c->enabled[i]) continue;
-            double res, scale = c->scaleVal[i];
+            if (!cc.enabled[i]) continue;
+            double res, scale = cc.scaleVal[i];
             if (i == 2) {
-                res = red[0][8];
+                res = tot[8];
             } else {
                 const int o = i * 4;
-                res = c->norm[i] == 0 ? red[0][o] : c->norm[i] == 1 ? red[0][o + 1] : red[0][o + 2];
-                switch (c->scaling[i]) {
+                res = cc.norm[i] == 0 ? tot[o] : cc.norm[i] == 1 ? tot[o + 1] : tot[o + 2];
+                switch (cc.scaling[i]) {
                 case 0: scale = 1.0; break;
-                case 1: scale = (double)c->N; break;
-                case 2: scale = sqrt((double)c->N); break;
-                case 3: scale = red[0][o + 3]; break;
+                case 1: scale = (double)cc.N; break;
+                case 2: scale = sqrt((double)cc.N); break;
+                case 3: scale = tot[o + 3]; break;
                 default: break;
                 }
             }
             if (i == 2 && scaleIter < 5) scale = fmax(res, 0.0);
-            c->scaleVal[i] = scale;
+            cc.scaleVal[i] = scale;
             if (scale != 0.0) {
-                if (c->norm[i] == 1) res = sqrt(res);
+                if (cc.norm[i] == 1) res = sqrt(res);
                 res /= scale;
             }
-            c->resVal[i] = res;
+            cc.resVal[i] = res;
         }
-        if (c->nHist < c->histCap) {
-            for (int i = 0; i < 3; i++) s.hist[c->nHist * 4 + i] = c->resVal[i];
-            s.hist[c->nHist * 4 + 3] = (double)c->iter;
-            c->nHist++;
+        if (cc.nHist < cc.histCap) {
+            for (int i = 0; i < 3; i++) s.hist[cc.nHist * 4 + i] = cc.resVal[i];
+            s.hist[cc.nHist * 4 + 3] = (double)cc.iter;
+            cc.nHist++;
         }
         bool any = false, ok = true;
         for (int i = 0; i < 3; i++) {
-            if (!c->enabled[i]) continue;
+            if (!cc.enabled[i]) continue;
             any = true;
-            if (!(c->resVal[i] <= c->tol[i])) ok = false;
+            if (!(cc.resVal[i] <= cc.tol[i])) ok = false;
         }
-        c->converged = (any && ok) ? 1 : 0;
+        cc.converged = (any && ok) ? 1 : 0;
     }
-    c->iter++;
-    c->k++;
-    if (c->converged || c->k >= c->maxIter) c->done = 1;
+    cc.iter++;
+    cc.k++;
+    if (cc.converged || cc.k >= cc.maxIter) cc.done = 1;
+    for (int i = 0; i < 3; i++) {
+        c->resVal[i] = cc.resVal[i];
+        c->scaleVal[i] = cc.scaleVal[i];
+    }
+    c->nHist = cc.nHist;
+    c->converged = cc.converged;
+    c->iter = cc.iter;
+    c->k = cc.k;
+    c->done = cc.done;
 }
 
 // Start of a time step: freeze the old levels (solver.cpp:1058-1095) and reset
@@ -2450,7 +2480,7 @@ void launchResidual(hipStream_t s, const DMesh& m, DState st, int blocks) {
     if (m.maxSlots <= 3) kResidual<3><<<blocks, kBlock, 0, s>>>(m, st);
     else kResidual<4><<<blocks, kBlock, 0, s>>>(m, st);
 }
-void launchFinish(hipStream_t s, DState st, int blocks) { kFinish<<<1, kBlock, 0, s>>>(st, blocks); }
+void launchFinish(hipStream_t s, DState st, int blocks) { kFinish<<<1, kFinishThreads, 0, s>>>(st, blocks); }
 void launchStepBegin(hipStream_t s, const DMesh& m, DState st, int bdf2) {
     kStepBegin<<<blocksFor(m.N), kBlock, 0, s>>>(m, st, bdf2);
 }
