@@ -474,7 +474,7 @@ class Solver:
         return a.value, b.value
 
     def set_sweep_fusion(self, sweeps: int, block_cells: int = 0) -> None:
-        """Momentum Jacobi sweeps per launch (1 to 4) and cells per fused
+        """Momentum Jacobi sweeps per launch (1 to 5) and cells per fused
         block (512 or 1024; 0: the larger one whose halos fit) for the next
         run; bit-identical results."""
         _check(lib().axs_set_sweep_fusion(self._h, sweeps, block_cells), self._h)

@@ -38,7 +38,7 @@ struct DSweepBlocks {
     const int16_t* locHalo = nullptr;  // the same for halo rows, kSweepRegSlots per entry
 };
 constexpr int kSweepRegSlots = 4;   // slots of a block cell kept in registers across the sweeps
-constexpr int kMaxFusedSweeps = 4;  // momentum Jacobi sweeps one fused launch may run
+constexpr int kMaxFusedSweeps = 5;  // momentum Jacobi sweeps one fused launch may run
 
 // Boundary conditions of one variable (BoundaryFieldDevice, solver_struct.h:431-454).
 struct DBC {

@@ -2427,7 +2427,8 @@ void launchJacobiUVBlockC(hipStream_t s, const DMesh& m, DState st, const DSweep
                           UVOut out) {
     if (sweeps == 2) launchJacobiUVBlockCS<C, 2>(s, m, st, sb, in, out);
     else if (sweeps == 3) launchJacobiUVBlockCS<C, 3>(s, m, st, sb, in, out);
-    else launchJacobiUVBlockCS<C, 4>(s, m, st, sb, in, out);
+    else if (sweeps == 4) launchJacobiUVBlockCS<C, 4>(s, m, st, sb, in, out);
+    else launchJacobiUVBlockCS<C, 5>(s, m, st, sb, in, out);
 }
 }  // namespace
 void launchJacobiUVBlock(hipStream_t s, const DMesh& m, DState st, const DSweepBlocks& sb, int sweeps,

@@ -469,7 +469,8 @@ void launchMomSweep32BlockC(hipStream_t s, const DMesh& m, const Ctl* ctl, Mixed
                             int sweeps, const float2* dO, float2* dN) {
     if (sweeps == 2) launchMomSweep32BlockCS<C, 2>(s, m, ctl, x, sb, dO, dN);
     else if (sweeps == 3) launchMomSweep32BlockCS<C, 3>(s, m, ctl, x, sb, dO, dN);
-    else launchMomSweep32BlockCS<C, 4>(s, m, ctl, x, sb, dO, dN);
+    else if (sweeps == 4) launchMomSweep32BlockCS<C, 4>(s, m, ctl, x, sb, dO, dN);
+    else launchMomSweep32BlockCS<C, 5>(s, m, ctl, x, sb, dO, dN);
 }
 }  // namespace
 void launchMomSweep32Block(hipStream_t s, const DMesh& m, const Ctl* ctl, MixedState x, const DSweepBlocks& sb,

@@ -46,7 +46,7 @@ __device__ __forceinline__ void rowRange(const int* start, int uniform, int n, i
 // halo entries a block may have: at most one per thread (thread t holds halo
 // row t), and the LDS of two iterate buffers within the static 64 KiB
 template <int C>
-constexpr int sweepHalo() { return C <= 512 ? C : C / 2; }
+constexpr int sweepHalo() { return C <= 512 ? C : 5 * C / 8; }
 
 // the neighbours' LDS slots of a row from its table entries l[0 .. ns)
 template <class Row>

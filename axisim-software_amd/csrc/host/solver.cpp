@@ -1577,7 +1577,7 @@ int Solver::residualHistory(int res, double* out, int maxN) const {
 // elements read + written per launch; FP64 8 B, int32 4 B).
 void Solver::setSweepFusion(int sweeps, int blockCells) {
     if (running_) throw std::logic_error("solver is running");
-    if (sweeps < 1 || sweeps > kMaxFusedSweeps) throw std::invalid_argument("sweeps per launch must be 1 to 4");
+    if (sweeps < 1 || sweeps > kMaxFusedSweeps) throw std::invalid_argument("sweeps per launch must be 1 to 5");
     if (blockCells != 0 && blockCells != 512 && blockCells != 1024)
         throw std::invalid_argument("fused blocks hold 512 or 1024 cells (0: either)");
     fuseReq_ = sweeps;
@@ -1732,8 +1732,8 @@ void Solver::autotune(int repeats, double* msDefault, double* msTuned) {
         };
         // the default request first (whatever count its halos allow), then
         // every count and block size that fits
-        const int options[][2] = {{kMaxFusedSweeps, 0}, {1, 0},   {2, 512}, {2, 1024},
-                                  {3, 512},              {3, 1024}, {4, 512}, {4, 1024}};
+        const int options[][2] = {{kMaxFusedSweeps, 0}, {1, 0},    {2, 512}, {2, 1024}, {3, 512},
+                                  {3, 1024},             {4, 512}, {4, 1024}, {5, 512}, {5, 1024}};
         double bt = 1e300;
         int bs = 1, bcells = 0;
         for (size_t i = 0; i < sizeof(options) / sizeof(options[0]); i++) {

@@ -240,13 +240,13 @@ axs_status axs_bench_kernel(axs_solver* s, int kernel, int repeats, double* ms, 
  * uses the choice.  *ms_default / *ms_tuned (may be NULL): summed kernel times
  * of one pass at 256 threads and at the chosen sizes. */
 axs_status axs_autotune(axs_solver* s, int repeats, double* ms_default, double* ms_tuned);
-/* Momentum Jacobi sweeps per launch (build extension): `sweeps` (1 to 4)
+/* Momentum Jacobi sweeps per launch (build extension): `sweeps` (1 to 5)
  * sweeps of u and v run in one launch, one workgroup per block of
  * `block_cells` (512 or 1024; 0: 1024 where the halos fit, else 512)
  * consecutive cells with the intermediate iterates in LDS; the results are
  * bit-identical to one sweep per launch.  The next run uses the largest count
  * <= sweeps that the mesh's halos allow (axs_stats::sweeps_per_launch,
- * sweep_block_cells).  Default 4 sweeps, block size 0; axs_autotune also picks
+ * sweep_block_cells).  Default 5 sweeps, block size 0; axs_autotune also picks
  * them. */
 axs_status axs_set_sweep_fusion(axs_solver* s, int sweeps, int block_cells);
 

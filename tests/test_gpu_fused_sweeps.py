@@ -29,7 +29,7 @@ def nozzle_mesh():
 
 
 @pytest.mark.parametrize("linear_iters", [20, 5, 4])
-@pytest.mark.parametrize("sweeps,cells", [(2, 512), (2, 1024), (3, 512), (3, 1024), (4, 512), (4, 1024)])
+@pytest.mark.parametrize("sweeps,cells", [(2, 512), (2, 1024), (3, 512), (3, 1024), (4, 512), (4, 1024), (5, 512), (5, 1024)])
 def test_fused_sweeps_bit_exact(require_gpu, nozzle_mesh, sweeps, cells, linear_iters):
     case = axs.nozzle_case(mesh=nozzle_mesh, re_throat=100, max_iter=8, tol=0.0, linear_max_iter=linear_iters)
     ref = run(case, 1)
@@ -61,12 +61,12 @@ def test_fused_sweeps_bit_exact_full_size(require_gpu):
     ref = run(case, 1)
     s = axs.make_solver(case)
     s.run()
-    assert s.stats().sweeps_per_launch == 4
+    assert s.stats().sweeps_per_launch == 5
     for name in FIELDS:
         np.testing.assert_array_equal(s.field(name), ref.field(name), err_msg=name)
 
 
-@pytest.mark.parametrize("sweeps", [3, 4])
+@pytest.mark.parametrize("sweeps", [3, 4, 5])
 def test_fused_sweeps_quad_mesh_bit_exact(require_gpu, sweeps):
     """A structured quad pipe (4 slots per cell, Green-Gauss gradients) numbered
     in rows of 32 cells, so its halos fit: fused against one sweep per launch."""
@@ -78,7 +78,7 @@ def test_fused_sweeps_quad_mesh_bit_exact(require_gpu, sweeps):
         np.testing.assert_array_equal(s.field(name), ref.field(name), err_msg=name)
 
 
-@pytest.mark.parametrize("sweeps", [3, 4])
+@pytest.mark.parametrize("sweeps", [3, 4, 5])
 def test_fused_sweeps_patch_blocks_bit_exact(require_gpu, sweeps):
     """Rows of 640 cells: consecutive-index blocks are strips whose halos do
     not fit, so the blocks are breadth-first patches; fused against one sweep
@@ -91,7 +91,7 @@ def test_fused_sweeps_patch_blocks_bit_exact(require_gpu, sweeps):
         np.testing.assert_array_equal(s.field(name), ref.field(name), err_msg=name)
 
 
-@pytest.mark.parametrize("sweeps,cells", [(2, 512), (3, 1024), (4, 1024)])
+@pytest.mark.parametrize("sweeps,cells", [(2, 512), (3, 1024), (4, 1024), (5, 1024)])
 def test_fused_fp32_sweeps_bit_exact(require_gpu, nozzle_mesh, sweeps, cells):
     """The mixed path's FP32 defect sweeps fused over the same blocks: S sweeps
     per launch against one per launch, bit for bit (18 FP32 sweeps per solve,
