@@ -38,6 +38,7 @@ FIELDS = {
     "u_ac": 16, "u_b": 17, "u_invac": 18, "u_af": 19,
     "v_ac": 20, "v_b": 21, "v_invac": 22, "v_af": 23,
     "pp_ac": 24, "pp_b": 25, "pp_invac": 26, "pp_af": 27,
+    "u_old": 28, "v_old": 29, "u_old2": 30, "v_old2": 31,
     "inv_dpn": 40, "wp": 41, "dpb": 42, "inv_a2d": 43, "lsq_wz": 44, "lsq_wr": 45,
     "lsq_inv_zz": 46, "lsq_inv_zr": 47, "lsq_inv_rr": 48, "length2d": 49,
 }
@@ -84,6 +85,7 @@ class Config(C.Structure):
         ("structured_mesh", C.c_int),
         ("mg_max_levels", C.c_int), ("mg_smoother", C.c_int), ("precision", C.c_int),
         ("coarse_direct", C.c_int),
+        ("save_key_frame_iter", C.c_int), ("max_time_frames", C.c_int),
     ]
 
 
@@ -146,6 +148,8 @@ def load_library(path: os.PathLike | None = None) -> C.CDLL:
         "axs_field_length": (C.c_size_t, [S, C.c_int]),
         "axs_get_residual_history": (C.c_int, [S, C.c_int, _dp, C.c_int, _ip]),
         "axs_get_stats": (C.c_int, [S, C.POINTER(Stats)]),
+        "axs_frame_count": (C.c_int, [S]),
+        "axs_get_frame": (C.c_int, [S, C.c_int, C.c_int, _dp, _dp, C.c_size_t]),
         "axs_mg_create": (C.c_int, [C.POINTER(S), C.c_int, _ip, _ip, C.c_int]),
         "axs_mg_destroy": (None, [S]),
         "axs_mg_levels": (C.c_int, [S]),
@@ -512,6 +516,20 @@ class Solver:
         out = np.zeros(cnt.value)
         if cnt.value:
             _check(lib().axs_get_residual_history(self._h, res, _dptr(out), cnt.value, C.byref(cnt)), self._h)
+        return out
+
+    def frames(self) -> list:
+        """[(time, {"u", "v", "p"})] of the transient frames."""
+        out = []
+        N = self.mesh.n_cells
+        for i in range(lib().axs_frame_count(self._h)):
+            t = C.c_double()
+            fields = {}
+            for name, fid in (("u", 0), ("v", 1), ("p", 2)):
+                a = np.zeros(N)
+                _check(lib().axs_get_frame(self._h, i, fid, C.byref(t), _dptr(a), N), self._h)
+                fields[name] = a
+            out.append((t.value, fields))
         return out
 
     def stats(self) -> Stats:

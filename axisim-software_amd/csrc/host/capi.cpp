@@ -112,6 +112,8 @@ void axs_config_defaults(axs_config* c) {
     c->mg_smoother = AXS_MG_SMOOTH_JACOBI;
     c->precision = AXS_PREC_FP64;
     c->coarse_direct = 0;
+    c->save_key_frame_iter = 2;               // solver_struct.h:230
+    c->max_time_frames = 500;                 // solver/header/solver.h:108
 }
 
 axs_status axs_create(axs_solver** out, int device) {
@@ -216,6 +218,22 @@ axs_status axs_get_residual_history(axs_solver* h, int res, double* out, int max
 axs_status axs_get_stats(axs_solver* h, axs_stats* out) {
     if (!h || !out) return AXS_ERR_INVALID;
     return guarded(&h->s->lastError, [&] { h->s->stats(*out); });
+}
+
+int axs_frame_count(const axs_solver* h) { return h ? h->s->frameCount() : 0; }
+
+axs_status axs_get_frame(axs_solver* h, int index, int field, double* time, double* out, size_t n) {
+    if (!h) return AXS_ERR_INVALID;
+    return guarded(&h->s->lastError, [&] {
+        const Solver::Frame& f = h->s->frame(index);
+        if (time) *time = f.time;
+        if (!out) return;
+        const std::vector<double>* src = field == AXS_FIELD_U ? &f.u : field == AXS_FIELD_V ? &f.v
+                                       : field == AXS_FIELD_P ? &f.p : nullptr;
+        if (!src) throw std::invalid_argument("frames hold u, v and p");
+        if (n != src->size()) throw std::out_of_range("output length does not match the frame");
+        std::copy(src->begin(), src->end(), out);
+    });
 }
 
 // ---------------- multigrid, stand-alone ----------------

@@ -1182,6 +1182,7 @@ void Solver::run(axs_progress_cb cb, void* user, bool continueRun) {
     if (c.check_conv < 1) c.check_conv = 1;
     if (c.linear_max_iter < 1) c.linear_max_iter = 20;
     if (c.structured_mesh) c.use_non_orth = 0;
+    if (c.save_key_frame_iter < 1) c.save_key_frame_iter = 1;
     if (c.transient && (!std::isfinite(c.dt) || c.dt <= 0.0 || !std::isfinite(c.t_end) || c.t_end <= 0.0))
         throw std::invalid_argument("Transient run needs a positive time step and end time.");
     if (!std::isfinite(c.rho) || !std::isfinite(c.mu) || c.rho <= 0.0 || c.mu <= 0.0)
@@ -1291,14 +1292,22 @@ void Solver::run(axs_progress_cb cb, void* user, bool continueRun) {
             batch = std::min(batch * 2, 64);
         }
     };
+    // Frames are kept only by a continued transient run, which carries time
+    // and the pulsatile phase on from the last frame (solver.cpp:890-900); a
+    // transient run starting afresh first captures the state it starts from
+    // (solver.cpp:1028-1031).
+    if (!c.transient || !cont) frames_.clear();
+    const double timeOffset = frames_.empty() ? 0.0 : frames_.back().time;
     if (!c.transient) {
         iterateUntilDone();
     } else {
+        if (frames_.empty()) captureFrame(timeOffset);
         const int steps = (int)std::ceil(c.t_end / c.dt);
         const bool bdf2 = c.time_scheme == AXS_TIME_BDF2;
         for (int t = 0; t < steps && !stopped_; t++) {
             AXS_HIP(hipStreamSynchronize(stream_));
-            uploadBCs((double)(t + 1) * c.dt);
+            // implicit at t_(n+1) (solver.cpp:1036-1056)
+            uploadBCs(timeOffset + (double)(t + 1) * c.dt);
             launchStepBegin(stream_, dm_, ds_, bdf2 ? 1 : 0);
             AXS_HIP(hipMemcpyAsync(hctl_, ds_.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream_));
             AXS_HIP(hipStreamSynchronize(stream_));
@@ -1309,6 +1318,9 @@ void Solver::run(axs_progress_cb cb, void* user, bool continueRun) {
             hctl_->bdf2Step = (bdf2 && t > 0) ? 1 : 0;
             AXS_HIP(hipMemcpy(ds_.ctl, hctl_, sizeof(Ctl), hipMemcpyHostToDevice));
             iterateUntilDone();
+            // the keyframe interval and the last (or stopped) step, solver.cpp:1286-1294
+            const bool lastStep = (t == steps - 1) || stopped_;
+            if (t % c.save_key_frame_iter == 0 || lastStep) captureFrame(timeOffset + (double)(t + 1) * c.dt);
         }
     }
     currentIteration_ = hctl_->iter;
@@ -1359,8 +1371,11 @@ void Solver::runTimed(int n, double* ms) {
 }
 
 // Time `repeats` eager launches of one kernel from the current state.  The
-// launches run with no control block (they ignore ctl->done) and only
-// rewrite buffers the next outer iteration overwrites anyway.
+// launches run with no control block (they ignore ctl->done).  Every kernel
+// but the face flux rewrites only buffers the next outer iteration overwrites
+// (and from the same inputs, so with the same values); the face flux
+// overwrites the mass flux, which is state carried between iterations, so it
+// is saved before the launches and restored after them.
 void Solver::benchKernel(int kernel, int repeats, double* ms, double* bytes) {
     if (!ready_) throw std::logic_error("run the solver once before timing a kernel");
     if (repeats <= 0) throw std::invalid_argument("repeats must be positive");
@@ -1420,6 +1435,12 @@ void Solver::benchKernel(int kernel, int repeats, double* ms, double* bytes) {
         break;
     default: throw std::invalid_argument("unknown kernel id");
     }
+    Arena saved;
+    double* mDotSaved = nullptr;
+    if (kernel == AXS_KERNEL_FACE_FLUX) {
+        mDotSaved = saved.alloc<double>(dm_.F);
+        AXS_HIP(hipMemcpyAsync(mDotSaved, ds_.mDot, (size_t)dm_.F * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    }
     launch();  // warm-up
     hipEvent_t e0, e1;
     AXS_HIP(hipEventCreate(&e0));
@@ -1432,8 +1453,33 @@ void Solver::benchKernel(int kernel, int repeats, double* ms, double* bytes) {
     AXS_HIP(hipEventElapsedTime(&f, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    if (mDotSaved) {
+        AXS_HIP(hipMemcpyAsync(ds_.mDot, mDotSaved, (size_t)dm_.F * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+        AXS_HIP(hipStreamSynchronize(stream_));
+    }
     AXS_HIP(hipGetLastError());
     *ms = (double)f / repeats;
+}
+
+// captureTimeFrame, solver.cpp:265-296: u, v, p copied to the host; the cap
+// stops capture, not the solve.
+void Solver::captureFrame(double time) {
+    const int cap = cfg_.max_time_frames > 0 ? cfg_.max_time_frames : 500;
+    if ((int)frames_.size() >= cap) return;
+    AXS_HIP(hipStreamSynchronize(stream_));
+    const size_t N = mesh_.nCells;
+    Frame f;
+    f.time = time;
+    for (auto [dst, src] : {std::pair<std::vector<double>*, const double*>{&f.u, ds_.u}, {&f.v, ds_.v}, {&f.p, ds_.p}}) {
+        dst->resize(N);
+        AXS_HIP(hipMemcpy(dst->data(), src, N * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    frames_.push_back(std::move(f));
+}
+
+const Solver::Frame& Solver::frame(int index) const {
+    if (index < 0 || index >= (int)frames_.size()) throw std::out_of_range("no such frame");
+    return frames_[index];
 }
 
 bool Solver::canContinue(std::string& reason) const {  // solver.cpp:469-528
@@ -1468,6 +1514,7 @@ size_t Solver::fieldLength(int field) const {
     if (field == AXS_FIELD_DU || field == AXS_FIELD_DV) return N;
     if (field >= AXS_FIELD_U_AC && field <= AXS_FIELD_PP_AF)
         return ((field - AXS_FIELD_U_AC) % 4 == 3) ? S : N;
+    if (field >= AXS_FIELD_U_OLD && field <= AXS_FIELD_V_OLD2) return N;
     switch (field) {
     case AXS_GEOM_INV_A2D:
     case AXS_GEOM_LSQ_INV_ZZ:
@@ -1521,6 +1568,10 @@ void Solver::getField(int field, double* out, size_t n) {
     case AXS_FIELD_PP_B: down(ds_.ppb); break;
     case AXS_FIELD_PP_INVAC: down(ds_.ppInv); break;
     case AXS_FIELD_PP_AF: down(ds_.ppAF); break;
+    case AXS_FIELD_U_OLD: down(ds_.uOld); break;
+    case AXS_FIELD_V_OLD: down(ds_.vOld); break;
+    case AXS_FIELD_U_OLD2: down(ds_.uOld2); break;
+    case AXS_FIELD_V_OLD2: down(ds_.vOld2); break;
     case AXS_FIELD_CONTINUITY: {  // getMassImbalance, solver.cpp:298-331
         std::vector<double> md(mesh_.nFaces);
         AXS_HIP(hipMemcpy(md.data(), ds_.mDot, md.size() * sizeof(double), hipMemcpyDeviceToHost));

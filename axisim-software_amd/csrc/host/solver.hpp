@@ -177,6 +177,13 @@ public:
     size_t fieldLength(int field) const;
     int residualHistory(int res, double* out, int maxN) const;
     void stats(axs_stats& s) const;
+    // Transient frames (Solver::TimeFrame / timeFrames, solver/header/solver.h:96-108)
+    struct Frame {
+        double time = 0.0;
+        std::vector<double> u, v, p;
+    };
+    int frameCount() const { return (int)frames_.size(); }
+    const Frame& frame(int index) const;
     // Learned initial guess of the first p' solve of every outer iteration
     // (net null: off).  level: multigrid level the net runs on, -1 = the first
     // one with at most kGuessCells cells.
@@ -273,6 +280,8 @@ private:
     bool converged_ = false, stopped_ = false, ready_ = false;
     std::vector<double> history_[3];
     double setupMs_ = 0.0, solveMs_ = 0.0;
+    std::vector<Frame> frames_;
+    void captureFrame(double time);
     int contCells_ = -1, contFaces_ = -1, contRefs_ = -1;
 };
 

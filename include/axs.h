@@ -19,6 +19,7 @@
  *   axs_can_continue              Solver::canContinue(mesh, &reason)          solver/source/solver.cpp:469-528
  *   axs_get_field                 Solver::solutions[name].field               solver/source/solver.cpp:186-263
  *   axs_get_residual_history      ResidualPlot::add(iteration, cfg) feed      graphics/gui/source/residual_plot.cpp:113-130
+ *   axs_frame_count / get_frame   Solver::timeFrames                          solver/header/solver.h:96-108, solver/source/solver.cpp:265-296
  *   axs_get_stats                 console lines: hierarchy, colours, timing   solver/source/multigrid.cu:727-764, solver/source/solver.cpp:836-841, 1318-1323
  *   axs_mg_*                      MultigridSolver (stand-alone p' solve)      solver/source/multigrid.cu:23-31, 298-325, 766-811
  *   axs_build_coloring            buildMeshColoring                           manager/source/memory_manger.cpp:334-409
@@ -144,6 +145,11 @@ typedef struct axs_config {
     int mg_smoother;      /* AXS_MG_SMOOTH_*                                    */
     int precision;        /* AXS_PREC_*                                         */
     int coarse_direct;    /* 1 = FP64 LU solve on the coarsest level            */
+    /* Transient frames: ConfigSolver::saveKeyFrameIter (structs/solver_struct.h:230,
+     * clamped to >= 1 by runCheck, solver/source/solver.cpp:557-559) and
+     * Solver::maxTimeFrames (solver/header/solver.h:108) */
+    int save_key_frame_iter;
+    int max_time_frames;
 } axs_config;
 
 /* Reference defaults (structs/solver_struct.h, src/header/setting.cuh:61-71,
@@ -185,6 +191,9 @@ enum {
     AXS_FIELD_U_AC = 16, AXS_FIELD_U_B = 17, AXS_FIELD_U_INVAC = 18, AXS_FIELD_U_AF = 19,
     AXS_FIELD_V_AC = 20, AXS_FIELD_V_B = 21, AXS_FIELD_V_INVAC = 22, AXS_FIELD_V_AF = 23,
     AXS_FIELD_PP_AC = 24, AXS_FIELD_PP_B = 25, AXS_FIELD_PP_INVAC = 26, AXS_FIELD_PP_AF = 27,
+    /* transient old levels n and n-1 (VariablesSimple uOld/vOld/uOld2/vOld2,
+     * structs/solver_struct.h:265-321) */
+    AXS_FIELD_U_OLD = 28, AXS_FIELD_V_OLD = 29, AXS_FIELD_U_OLD2 = 30, AXS_FIELD_V_OLD2 = 31,
     /* precomputed geometry (preComputeVariables, memory_manger.cpp:512-617) */
     AXS_GEOM_INV_DPN = 40, AXS_GEOM_WP = 41, AXS_GEOM_DPB = 42, AXS_GEOM_INV_A2D = 43,
     AXS_GEOM_LSQ_WZ = 44, AXS_GEOM_LSQ_WR = 45, AXS_GEOM_LSQ_INV_ZZ = 46,
@@ -197,6 +206,17 @@ size_t axs_field_length(const axs_solver* s, int field);
 /* Normalised residual history of one residual (AXS_RES_*): one entry per
  * residual check.  Returns the number of entries written through *count. */
 axs_status axs_get_residual_history(axs_solver* s, int res, double* out, int max_n, int* count);
+
+/* Transient frames (Solver::TimeFrame / timeFrames, solver/header/solver.h:96-108;
+ * captureTimeFrame, solver/source/solver.cpp:265-296): a transient run captures
+ * the state it starts from, then every save_key_frame_iter steps and the last
+ * (or stopped) step, at most max_time_frames; a continued transient run appends
+ * and carries time and the pulsatile phase on from the last frame
+ * (solver.cpp:890-900, 1036-1056).  axs_get_frame gives frame `index`'s time
+ * (may be NULL) and copies its field AXS_FIELD_U, V or P into out (NULL: time
+ * only; n = cell count). */
+int axs_frame_count(const axs_solver* s);
+axs_status axs_get_frame(axs_solver* s, int index, int field, double* time, double* out, size_t n);
 
 typedef struct axs_stats {
     int iterations;        /* outer iterations done (currentIteration)           */

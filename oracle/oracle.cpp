@@ -18,9 +18,11 @@
 // exactness for linear fields, dense P^T A P, dense Jacobi / Gauss-Seidel
 // iteration matrices, M-matrix and symmetry identities of the assembled
 // systems, Hagen-Poiseuille on the pipe case, hand-computed agglomeration
-// maps, residual norm closed forms and the index-4 continuity scale capture.
-// Parts with no independent pin are listed as "parity unpinned" in the
-// module docstring of tests/test_oracle_pins.py.
+// maps, residual norm closed forms and the index-4 continuity scale capture;
+// each face term on its own through the orc_probe_* entry points (exactness
+// for linear / cubic fields and polynomials in time, crafted limiter cases,
+// closed-form boundary values).  What stays unpinned is listed as "parity
+// unpinned" in the module docstring of tests/test_oracle_pins.py.
 
 #include <algorithm>
 #include <cmath>
@@ -1039,6 +1041,19 @@ struct Oracle {
     bool stopRequested = false;
     int stopAtIteration = -1;   // test hook: request a stop before this iteration
     bool allocated = false;
+    // Solver::TimeFrame / timeFrames / maxTimeFrames (solver/header/solver.h:96-108)
+    struct Frame {
+        double time = 0.0;
+        std::vector<double> u, v, p;
+    };
+    std::vector<Frame> frames;
+
+    // captureTimeFrame, solver.cpp:265-296 (the cap stops capture, not the solve)
+    void captureFrame(double time) {
+        const int cap = cfg.max_time_frames > 0 ? cfg.max_time_frames : 500;
+        if ((int)frames.size() >= cap) return;
+        frames.push_back(Frame{time, s.u, s.v, s.p});
+    }
 
     void allocCoeff(Coeff& c, const std::vector<int>& fs, const std::vector<int>& fn) {
         c.N = m.N;
@@ -1187,6 +1202,7 @@ struct Oracle {
         if (cfg.check_conv < 1) cfg.check_conv = 1;
         if (cfg.linear_max_iter < 1) cfg.linear_max_iter = 20;
         if (cfg.structured_mesh) cfg.use_non_orth = 0;
+        if (cfg.save_key_frame_iter < 1) cfg.save_key_frame_iter = 1;  // solver.cpp:557-559
         if (!(cfg.rho > 0.0) || !(cfg.mu > 0.0)) return AXS_ERR_INVALID;
         if (cfg.transient && (!(cfg.dt > 0.0) || !(cfg.t_end > 0.0))) return AXS_ERR_INVALID;
         s.alphaU = cfg.alpha_u;
@@ -1210,9 +1226,16 @@ struct Oracle {
         bcU.value = bcU.mean;  // valueAtTime(0.0), memory_manger.cpp:102-103
         bcV.value = bcV.mean;
         converged = false;
+        // frames are kept only by a continued transient run; the continued run
+        // carries the time (and the pulsatile phase) on from the last frame
+        // (solver.cpp:890-900); the first frame is the state the run starts from
+        // (solver.cpp:1028-1031)
+        if (!transient || !canCont) frames.clear();
+        const double timeOffset = frames.empty() ? 0.0 : frames.back().time;
+        if (transient && frames.empty()) captureFrame(timeOffset);
         for (int t = 0; t < numSteps; t++) {
             if (transient) {
-                const double stepTime = (double)(t + 1) * cfg.dt;
+                const double stepTime = timeOffset + (double)(t + 1) * cfg.dt;
                 updatePulsatile(bcU, stepTime);
                 updatePulsatile(bcV, stepTime);
                 if (bdf2) {
@@ -1235,6 +1258,10 @@ struct Oracle {
                 currentIteration++;
                 if (converged) break;
             }
+            // keyframe interval and the last (or stopped) step, solver.cpp:1286-1294
+            const bool lastStep = (t == numSteps - 1) || stopRequested;
+            if (transient && (t % cfg.save_key_frame_iter == 0 || lastStep))
+                captureFrame(timeOffset + (double)(t + 1) * cfg.dt);
             if (stopRequested) break;
         }
         // output gradients, solver.cpp:1330-1336
@@ -1390,6 +1417,10 @@ int orc_get_field(const orc_solver* h, int field, double* out, size_t n) {
     case AXS_FIELD_PP_B: src = &o.ppC.b; break;
     case AXS_FIELD_PP_INVAC: src = &o.ppC.invAC; break;
     case AXS_FIELD_PP_AF: src = &o.ppC.AF; break;
+    case AXS_FIELD_U_OLD: src = &o.s.uOld; break;
+    case AXS_FIELD_V_OLD: src = &o.s.vOld; break;
+    case AXS_FIELD_U_OLD2: src = &o.s.uOld2; break;
+    case AXS_FIELD_V_OLD2: src = &o.s.vOld2; break;
     case AXS_GEOM_INV_DPN: src = &m.invDPN; break;
     case AXS_GEOM_WP: src = &m.wP; break;
     case AXS_GEOM_DPB: src = &m.dPB; break;
@@ -1418,6 +1449,162 @@ int orc_gradient(const orc_solver* h, int var, int scheme, const double* phi, do
     std::memcpy(gz, z.data(), o.m.N * sizeof(double));
     std::memcpy(gr, r.data(), o.m.N * sizeof(double));
     return AXS_OK;
+}
+
+static const BC& bcOf(const Oracle& o, int var) {
+    return var == AXS_VAR_U ? o.bcU : var == AXS_VAR_V ? o.bcV : o.bcP;
+}
+static std::vector<double> vec(const double* a, int n) {
+    return a ? std::vector<double>(a, a + n) : std::vector<double>(n, 0.0);
+}
+
+// interpolateFieldToFace (solver_util.cuh:240-297) seen from each face's owner:
+// out[f] = phi_f for every face, boundary faces from the BCs of `var`.
+int orc_probe_face_values(const orc_solver* h, int var, const double* phi, double* out) {
+    const Oracle& o = h->o;
+    const std::vector<double> p = vec(phi, o.m.N);
+    for (int f = 0; f < o.m.F; f++) out[f] = interpToFace(o.m.owner[f], f, o.m, bcOf(o, var), p);
+    return AXS_OK;
+}
+
+// addTransportCoefficients (solver_util.cu:218-448) of one field into zeroed
+// coefficients: AC[N], AF[S], b[N].
+int orc_probe_transport(const orc_solver* h, int var, const double* phi, const double* gz, const double* gr,
+                        const double* mdot, int nonOrtho, double gamma, int conv, int scheme, double fluxScale,
+                        double* AC, double* AF, double* b) {
+    const Oracle& o = h->o;
+    const Mesh& m = o.m;
+    Fields s;
+    s.mDot = vec(mdot, m.F);
+    Coeff co;
+    co.N = m.N;
+    co.AC.assign(m.N, 0.0);
+    co.b.assign(m.N, 0.0);
+    co.AF.assign(m.faceIDs.size(), 0.0);
+    addTransport(m, s, co, bcOf(o, var), vec(phi, m.N), vec(gz, m.N), vec(gr, m.N), nonOrtho, gamma, conv, scheme,
+                 fluxScale);
+    std::memcpy(AC, co.AC.data(), m.N * sizeof(double));
+    std::memcpy(b, co.b.data(), m.N * sizeof(double));
+    std::memcpy(AF, co.AF.data(), co.AF.size() * sizeof(double));
+    return AXS_OK;
+}
+
+// addTransientCoefficient (solver_util.cu:536-578) into zeroed AC, b; old2 NULL = BDF1.
+int orc_probe_transient(const orc_solver* h, const double* old, const double* old2, double cap, double dt,
+                        double* AC, double* b) {
+    const Mesh& m = h->o.m;
+    Coeff co;
+    co.N = m.N;
+    co.AC.assign(m.N, 0.0);
+    co.b.assign(m.N, 0.0);
+    const std::vector<double> o1 = vec(old, m.N), o2 = vec(old2, m.N);
+    transientTerm(m, co, &o1, old2 ? &o2 : nullptr, cap, dt);
+    std::memcpy(AC, co.AC.data(), m.N * sizeof(double));
+    std::memcpy(b, co.b.data(), m.N * sizeof(double));
+    return AXS_OK;
+}
+
+// Fields of the pressure-correction kernels from caller arrays.
+static Fields probeFields(const Mesh& m, const double* u, const double* v, const double* p, const double* pp,
+                          const double* gpz, const double* gpr, const double* DU, const double* DV,
+                          const double* mdot) {
+    Fields s;
+    s.u = vec(u, m.N);
+    s.v = vec(v, m.N);
+    s.p = vec(p, m.N);
+    s.pp = vec(pp, m.N);
+    s.gradPZ = vec(gpz, m.N);
+    s.gradPR = vec(gpr, m.N);
+    s.DU = vec(DU, m.N);
+    s.DV = vec(DV, m.N);
+    s.mDot = vec(mdot, m.F);
+    return s;
+}
+
+// computeFaceMassFluxRhieChow (solver_util.cu:171-203) with the solver's u, v, p BCs.
+int orc_probe_face_flux(const orc_solver* h, double rho, const double* u, const double* v, const double* p,
+                        const double* gpz, const double* gpr, const double* DU, const double* DV, double* mdot) {
+    const Oracle& o = h->o;
+    Fields s = probeFields(o.m, u, v, p, nullptr, gpz, gpr, DU, DV, nullptr);
+    faceMassFlux(rho, o.m, s, o.bcU, o.bcV, o.bcP);
+    std::memcpy(mdot, s.mDot.data(), o.m.F * sizeof(double));
+    return AXS_OK;
+}
+
+// createPPCoeff (simple.cu:7-88) into zeroed AC[N], AF[S] with the solver's p BCs.
+int orc_probe_pp_coeff(const orc_solver* h, double rho, const double* DU, const double* DV, double* AC,
+                       double* AF) {
+    const Oracle& o = h->o;
+    const Mesh& m = o.m;
+    Fields s = probeFields(m, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, DU, DV, nullptr);
+    Coeff co;
+    co.N = m.N;
+    co.AC.assign(m.N, 0.0);
+    co.b.assign(m.N, 0.0);
+    co.AF.assign(m.faceIDs.size(), 0.0);
+    createPPCoeff(rho, m, co, s, o.bcP);
+    std::memcpy(AC, co.AC.data(), m.N * sizeof(double));
+    std::memcpy(AF, co.AF.data(), co.AF.size() * sizeof(double));
+    return AXS_OK;
+}
+
+// createPPRhs (simple.cu:90-142); gpz/gpr hold grad p' for the cross term.
+int orc_probe_pp_rhs(const orc_solver* h, double rho, const double* mdot, const double* DU, const double* DV,
+                     const double* gpz, const double* gpr, int cross, double* b) {
+    const Oracle& o = h->o;
+    const Mesh& m = o.m;
+    Fields s = probeFields(m, nullptr, nullptr, nullptr, nullptr, gpz, gpr, DU, DV, mdot);
+    Coeff co;
+    co.N = m.N;
+    co.b.assign(m.N, 0.0);
+    createPPRhs(rho, m, co, s, cross);
+    std::memcpy(b, co.b.data(), m.N * sizeof(double));
+    return AXS_OK;
+}
+
+// updateMassFlux (simple.cu:181-258) of mdot in place, p BCs of the solver.
+int orc_probe_mass_flux_update(const orc_solver* h, double rho, const double* pp, const double* DU,
+                               const double* DV, const double* gpz, const double* gpr, int nonOrtho,
+                               double* mdot) {
+    const Oracle& o = h->o;
+    const Mesh& m = o.m;
+    Fields s = probeFields(m, nullptr, nullptr, nullptr, pp, gpz, gpr, DU, DV, mdot);
+    updateMassFlux(rho, m, s, o.bcP, nonOrtho);
+    std::memcpy(mdot, s.mDot.data(), m.F * sizeof(double));
+    return AXS_OK;
+}
+
+// Transient frames: count, and one frame's time and field (AXS_FIELD_U/V/P).
+int orc_frame_count(const orc_solver* h) { return (int)h->o.frames.size(); }
+int orc_get_frame(const orc_solver* h, int index, int field, double* time, double* out, size_t n) {
+    const auto& fr = h->o.frames;
+    if (index < 0 || index >= (int)fr.size()) return AXS_ERR_RANGE;
+    const auto& f = fr[index];
+    const std::vector<double>* src = field == AXS_FIELD_U ? &f.u : field == AXS_FIELD_V ? &f.v
+                                   : field == AXS_FIELD_P ? &f.p : nullptr;
+    if (time) *time = f.time;
+    if (!out) return AXS_OK;
+    if (!src) return AXS_ERR_INVALID;
+    if (src->size() != n) return AXS_ERR_RANGE;
+    std::memcpy(out, src->data(), n * sizeof(double));
+    return AXS_OK;
+}
+
+// Value of group `group` of `var` as the last run left it on the device
+// (updatePulsatileBoundaryValues at the last step's time, solver.cpp:1039-1056).
+double orc_bc_current(const orc_solver* h, int var, int group) {
+    const BC& bc = bcOf(h->o, var);
+    if (group < 0 || group >= bc.nGroups) return NAN;
+    return bc.value[group];
+}
+
+// BoundaryCondition::valueAtTime (structs/boundary_struct.h:184-192) of one
+// group of `var` as the run loop prescribes it.
+double orc_probe_bc_value(const orc_solver* h, int var, int group, double t) {
+    BC bc = bcOf(h->o, var);
+    if (group < 0 || group >= bc.nGroups) return NAN;
+    Oracle::updatePulsatile(bc, t);
+    return bc.value[group];
 }
 
 // Cell -> face CSR of the mesh (buildCellFaceCSR).

@@ -47,6 +47,18 @@ def olib() -> C.CDLL:
                                        C.c_int]),
             "orc_mg_get_operator": (C.c_int, [V, C.c_int, _dp, _dp]),
             "orc_residual_value": (None, [C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_int, _dp, _dp]),
+            "orc_probe_face_values": (C.c_int, [V, C.c_int, _dp, _dp]),
+            "orc_probe_transport": (C.c_int, [V, C.c_int, _dp, _dp, _dp, _dp, C.c_int, C.c_double, C.c_int,
+                                              C.c_int, C.c_double, _dp, _dp, _dp]),
+            "orc_probe_transient": (C.c_int, [V, _dp, _dp, C.c_double, C.c_double, _dp, _dp]),
+            "orc_probe_face_flux": (C.c_int, [V, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+            "orc_probe_pp_coeff": (C.c_int, [V, C.c_double, _dp, _dp, _dp, _dp]),
+            "orc_probe_pp_rhs": (C.c_int, [V, C.c_double, _dp, _dp, _dp, _dp, _dp, C.c_int, _dp]),
+            "orc_probe_mass_flux_update": (C.c_int, [V, C.c_double, _dp, _dp, _dp, _dp, _dp, C.c_int, _dp]),
+            "orc_probe_bc_value": (C.c_double, [V, C.c_int, C.c_int, C.c_double]),
+            "orc_bc_current": (C.c_double, [V, C.c_int, C.c_int]),
+            "orc_frame_count": (C.c_int, [V]),
+            "orc_get_frame": (C.c_int, [V, C.c_int, C.c_int, _dp, _dp, C.c_size_t]),
             "orc_set_threads": (None, [C.c_int]),
             "orc_get_threads": (C.c_int, []),
         }
@@ -61,6 +73,15 @@ def olib() -> C.CDLL:
 
 def _d(a):
     return a.ctypes.data_as(_dp)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _dn(a):
+    """Pointer to a float64 copy of a, or NULL for None."""
+    return None if a is None else _d(_f64(a))
 
 
 def _i(a):
@@ -139,6 +160,75 @@ class Oracle:
         gr = np.zeros_like(phi)
         olib().orc_gradient(self._h, var, scheme, _d(phi), _d(gz), _d(gr))
         return gz, gr
+
+    # ---- single kernels on caller data (closed-form pins) ----
+    def _n(self):
+        return self.mesh.n_cells, self.mesh.n_faces, int(self.mesh.arrays["cell_face_start"][-1])
+
+    def face_values(self, var: int, phi) -> np.ndarray:
+        """phi at every face seen from its owner, boundary faces from var's BCs."""
+        N, F, S = self._n()
+        out = np.zeros(F)
+        olib().orc_probe_face_values(self._h, var, _d(_f64(phi)), _d(out))
+        return out
+
+    def transport(self, var, phi, gz, gr, mdot, non_ortho, gamma, conv, scheme, flux_scale=1.0):
+        """Transport coefficients (AC, AF, b) of one field from zero."""
+        N, F, S = self._n()
+        AC, AF, b = np.zeros(N), np.zeros(S), np.zeros(N)
+        olib().orc_probe_transport(self._h, var, _dn(phi), _dn(gz), _dn(gr), _dn(mdot),
+                                   int(non_ortho), gamma, int(conv), scheme, flux_scale, _d(AC), _d(AF), _d(b))
+        return AC, AF, b
+
+    def transient(self, old, old2, cap, dt):
+        N, F, S = self._n()
+        AC, b = np.zeros(N), np.zeros(N)
+        olib().orc_probe_transient(self._h, _dn(old), _dn(old2), cap, dt, _d(AC), _d(b))
+        return AC, b
+
+    def face_flux(self, rho, u, v, p, gpz, gpr, DU, DV) -> np.ndarray:
+        N, F, S = self._n()
+        out = np.zeros(F)
+        olib().orc_probe_face_flux(self._h, rho, *(_d(_f64(a)) for a in (u, v, p, gpz, gpr, DU, DV)), _d(out))
+        return out
+
+    def pp_coeff(self, rho, DU, DV):
+        N, F, S = self._n()
+        AC, AF = np.zeros(N), np.zeros(S)
+        olib().orc_probe_pp_coeff(self._h, rho, _d(_f64(DU)), _d(_f64(DV)), _d(AC), _d(AF))
+        return AC, AF
+
+    def pp_rhs(self, rho, mdot, DU, DV, gpz, gpr, cross) -> np.ndarray:
+        N, F, S = self._n()
+        b = np.zeros(N)
+        olib().orc_probe_pp_rhs(self._h, rho, *(_d(_f64(a)) for a in (mdot, DU, DV, gpz, gpr)), int(cross), _d(b))
+        return b
+
+    def mass_flux_update(self, rho, mdot, pp, DU, DV, gpz, gpr, non_ortho) -> np.ndarray:
+        out = np.array(mdot, dtype=np.float64, copy=True)
+        olib().orc_probe_mass_flux_update(self._h, rho, *(_d(_f64(a)) for a in (pp, DU, DV, gpz, gpr)),
+                                          int(non_ortho), _d(out))
+        return out
+
+    def bc_value(self, var: int, group: int, t: float) -> float:
+        return olib().orc_probe_bc_value(self._h, var, group, t)
+
+    def bc_current(self, var: int, group: int) -> float:
+        return olib().orc_bc_current(self._h, var, group)
+
+    def frames(self) -> list:
+        """[(time, {"u", "v", "p"})] of the transient frames."""
+        out = []
+        N = self.mesh.n_cells
+        for i in range(olib().orc_frame_count(self._h)):
+            t = C.c_double()
+            fields = {}
+            for name, fid in (("u", 0), ("v", 1), ("p", 2)):
+                a = np.zeros(N)
+                olib().orc_get_frame(self._h, i, fid, C.byref(t), _d(a), N)
+                fields[name] = a
+            out.append((t.value, fields))
+        return out
 
     def csr(self):
         m = self.mesh

@@ -138,3 +138,73 @@ def test_transient_bdf2_pulsatile(require_gpu):
     s, o = run_both(case)
     assert s.stats().iterations == o.iterations
     compare(s, o, ["u", "v", "p", "mdot"], 1e-12)
+
+
+def test_fully_developed_inlet_c1(require_gpu):
+    """C1 with a fully-developed (parabolic) inlet profile v (1 - (r/R)^2)."""
+    case = axs.pipe_case()
+    u = case.bcs[axs.VAR_U]
+    u.type[axs.G_INLET] = axs.BC_FULLY_DEVELOPED
+    u.value[axs.G_INLET] = 0.01
+    u.length = [0.0] * len(u.type)
+    u.length[axs.G_INLET] = 0.01
+    s, o = run_both(case, iters=30)
+    compare(s, o, STATE_FIELDS, 1e-12)
+
+
+def test_derived_outputs_c1(require_gpu):
+    """createSolutions' host-derived fields: continuity, |V|, cell Reynolds number."""
+    s, o = run_both(axs.pipe_case(), iters=25)
+    compare(s, o, ["continuity", "vmag", "cell_re", "grad_uz", "grad_ur", "grad_vz", "grad_vr"], 1e-12)
+
+
+def _pulsatile_pipe(t_end, dt=1e-2):
+    case = small_pipe(transient=1, time_scheme=axs.TIME_BDF2, dt=dt, t_end=t_end, max_iter=3, save_key_frame_iter=1)
+    case.bcs = axs.default_bcs(axs.GROUP_TYPES, {(axs.G_INLET, axs.VAR_U): (axs.BC_PULSATILE, 0.005, 0.3, 5.0)})
+    return case
+
+
+def test_transient_frames_and_continued_pulsatile(require_gpu):
+    """Frames every step (interval 1) from the start state; a continued run
+    appends frames and keeps the inlet's phase (steps at 4 dt and 5 dt)."""
+    dt = 0.0625
+    case = _pulsatile_pipe(3 * dt, dt)
+    s, o = run_both(case)
+    case.config.t_end = 2 * dt
+    s.configure(case.config)
+    s.run(continue_run=True)
+    o.configure(case.config)
+    o.run(continue_run=True)
+    fs, fo = s.frames(), o.frames()
+    assert [t for t, _ in fs] == [t for t, _ in fo]
+    assert [round(t / dt, 9) for t, _ in fs] == [0.0, 1.0, 2.0, 3.0, 4.0, 5.0]
+    for (_, a), (_, b) in zip(fs, fo):
+        for k in ("u", "v", "p"):
+            assert rel_l2(a[k], b[k]) <= 1e-12, k
+    compare(s, o, ["u", "v", "p", "mdot", "u_old", "v_old", "u_old2", "v_old2"], 1e-12)
+    # a fresh run drops the frames and starts again at t = 0
+    s.run()
+    assert [round(t / dt, 9) for t, _ in s.frames()] == [0.0, 1.0, 2.0]
+
+
+def test_bench_kernel_leaves_the_state_alone(require_gpu):
+    """Timing kernels between runs does not change what a continued run computes."""
+    case = small_pipe(max_iter=5)
+    s1, s2 = axs.make_solver(case), axs.make_solver(case)
+    s1.run()
+    s2.run()
+    before = {k: s1.field(k) for k in ("u", "v", "p", "mdot", "pp")}
+    kernels = [axs.KERNEL_JACOBI_UV, axs.KERNEL_MOMENTUM, axs.KERNEL_MG_SMOOTH, axs.KERNEL_FACE_FLUX,
+               axs.KERNEL_PP_COEFF]
+    if s1.stats().sweeps_per_launch >= 2:
+        kernels.append(axs.KERNEL_JACOBI_UV_FUSED)
+    for k in kernels:
+        s1.bench_kernel(k, 3)
+    for k, a in before.items():
+        np.testing.assert_array_equal(s1.field(k), a, err_msg=k)
+    case.config.max_iter = 2
+    for s in (s1, s2):
+        s.configure(case.config)
+        s.run(continue_run=True)
+    for k in ("u", "v", "p", "mdot", "pp"):
+        np.testing.assert_array_equal(s1.field(k), s2.field(k), err_msg=k)

@@ -17,9 +17,13 @@ Pinned here:
   residual norms (K17-K19, D4)    closed forms, index-4 continuity capture,
                                   NaN never converges
   whole SIMPLE loop (D1)          Hagen-Poiseuille on the pipe case
-Parity unpinned (no independent check available without running the
-reference): the Rhie-Chow face flux away from its converged-state identity,
-and the exact outer-iteration counts of the reference itself.
+The face terms (central / second-order upwind deferred correction and its
+limiter, non-orthogonal cross terms of momentum, p' right-hand side and mass-flux
+update, Rhie-Chow in isolation, BDF1/BDF2, pulsatile valueAtTime, fully-developed
+and Neumann boundary values, old-level rotation, transient frames and the
+continued run's time offset) are pinned in tests/test_oracle_pins_terms.py.
+Parity unpinned (needs the reference executable): the reference's own
+outer-iteration counts.
 """
 import math
 
