@@ -158,6 +158,8 @@ def load_library(path: os.PathLike | None = None) -> C.CDLL:
         "axs_mg_get_maps": (C.c_int, [S, C.c_int, _ip, _ip, _ip, _ip]),
         "axs_mg_solve": (C.c_int, [S, _dp, _dp, _dp, _dp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
                                    C.c_int, C.c_int, C.c_int]),
+        "axs_mg_solve_noalloc": (C.c_int, [S, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double,
+                                           C.c_int, C.c_int, C.c_int, C.c_void_p]),
         "axs_mg_bench": (C.c_int, [S, _dp, _dp, _dp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int,
                                    C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]),
         "axs_build_coloring": (C.c_int, [C.c_int, _ip, _ip, _ip, _ip, _ip]),
@@ -592,6 +594,12 @@ class Multigrid:
         _check(lib().axs_mg_solve(self._h, _dptr(AC), _dptr(AF), _dptr(b), _dptr(x), cycles, weight, pre, post,
                                   coarse, precision, coarse_direct, smoother))
         return x
+
+    def solve_noalloc(self, AC: int, AF: int, b: int, x: int, cycles=3, weight=0.6, pre=1, post=1, coarse=30,
+                      stream: int | None = None) -> None:
+        """axs_mg_solve_noalloc on device arrays given by address (x in/out);
+        enqueued on `stream` (None: the hierarchy's stream), not waited for."""
+        _check(lib().axs_mg_solve_noalloc(self._h, AC, AF, b, x, cycles, weight, pre, post, coarse, stream))
 
     def bench(self, AC, AF, b, cycles=3, weight=0.6, pre=1, post=1, coarse=30, precision=PREC_FP64,
               coarse_direct=0, smoother=MG_SMOOTH_JACOBI, repeats=20):

@@ -27,6 +27,7 @@ SOURCES = [
     "csrc/hip/kernels.hip",
     "csrc/hip/kernels_mixed.hip",
     "csrc/hip/mpnn.hip",
+    "csrc/hip/noalloc.hip",
     "csrc/host/solver.cpp",
     "csrc/host/net.cpp",
     "csrc/host/capi.cpp",

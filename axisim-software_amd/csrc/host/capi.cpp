@@ -16,6 +16,7 @@
 #include "net.hpp"
 #include "solver.hpp"
 #include "topology.hpp"
+#include "handles.hpp"
 
 using namespace axs;
 
@@ -23,14 +24,6 @@ struct axs_solver {
     Solver* s = nullptr;
 };
 
-struct axs_mg {
-    MGDevice mg;
-    int device = 0;
-    hipStream_t stream = nullptr;
-    Arena arena;
-    double *AC = nullptr, *b = nullptr, *AF = nullptr, *x = nullptr, *xNew = nullptr;
-    int n = 0, refs = 0;
-};
 
 struct axs_mesh {
     HostMesh m;
@@ -40,39 +33,13 @@ struct axs_net {
     Mpnn* net = nullptr;
 };
 
-namespace {
-
-thread_local std::string tlsError;
-
-template <class F>
-axs_status guarded(std::string* err, F&& f) {
-    try {
-        f();
-        if (err) err->clear();
-        return AXS_OK;
-    } catch (const HipError& e) {
-        (err ? *err : tlsError) = e.what();
-        return AXS_ERR_DEVICE;
-    } catch (const std::invalid_argument& e) {
-        (err ? *err : tlsError) = e.what();
-        return AXS_ERR_INVALID;
-    } catch (const std::out_of_range& e) {
-        (err ? *err : tlsError) = e.what();
-        return AXS_ERR_RANGE;
-    } catch (const std::logic_error& e) {
-        // domain_error (unsupported option) derives from logic_error
-        (err ? *err : tlsError) = e.what();
-        return dynamic_cast<const std::domain_error*>(&e) ? AXS_ERR_UNSUPPORTED : AXS_ERR_STATE;
-    } catch (const std::bad_alloc&) {
-        (err ? *err : tlsError) = "out of host memory";
-        return AXS_ERR_DEVICE;
-    } catch (const std::exception& e) {
-        (err ? *err : tlsError) = e.what();
-        return AXS_ERR_DEVICE;
-    }
+namespace axs {
+std::string& threadError() {
+    thread_local std::string text;
+    return text;
 }
+}  // namespace axs
 
-}  // namespace
 
 extern "C" {
 
@@ -141,7 +108,7 @@ void axs_destroy(axs_solver* h) {
 }
 
 const char* axs_last_error(const axs_solver* h) {
-    if (!h || !h->s) return tlsError.c_str();
+    if (!h || !h->s) return axs::threadError().c_str();
     return h->s->lastError.c_str();
 }
 

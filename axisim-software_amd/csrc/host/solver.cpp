@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <type_traits>
 
 namespace axs {
 
@@ -330,6 +331,89 @@ void MGDevice::bindLevel0(double* AC, double* b, double* AF, double* x, double* 
 }
 
 void MGDevice::swapX(int l) { std::swap(dev[l].x, dev[l].xNew); }
+
+namespace {
+size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+}  // namespace
+
+size_t MGDevice::workBytes() const {
+    size_t bytes = 0;
+    const int L = (int)host.size();
+    for (int l = 0; l < L; l++) {
+        const Level& h = host[l];
+        const size_t n = h.n, r = h.refs();
+        if (!h.coarsest()) bytes += align256(n * sizeof(double));  // res (tail levels' too)
+        if (tail64_.holds(l)) continue;
+        if (l > 0) bytes += 4 * align256(n * sizeof(double)) + align256(r * sizeof(double));  // AC b x xNew, AF
+        else bytes += align256(n * sizeof(double));                                          // xNew
+    }
+    if (tail64_.lt >= 0) bytes += align256(tail64_.bytes);
+    return bytes;
+}
+
+MGDevice::Bound MGDevice::bindWork(unsigned char* base, const double* AC, const double* b, const double* AF, double* x,
+                                   hipStream_t s) {
+    Bound saved{dev, tail64_.image, x0Home_, x0Other_};
+    size_t off = 0;
+    auto take = [&](size_t count) {
+        double* p = (double*)(base + off);
+        off += align256(count * sizeof(double));
+        return p;
+    };
+    unsigned char* tailBase = nullptr;
+    const int L = (int)host.size();
+    for (int l = 0; l < L; l++) {
+        const Level& h = host[l];
+        DLevel& d = dev[l];
+        if (!h.coarsest()) d.res = take(h.n);
+        if (tail64_.holds(l)) continue;
+        if (l > 0) {
+            d.AC = take(h.n);
+            d.b = take(h.n);
+            d.x = take(h.n);
+            d.xNew = take(h.n);
+            d.AF = take(h.refs());
+        } else {
+            d.AC = const_cast<double*>(AC);
+            d.b = const_cast<double*>(b);
+            d.AF = const_cast<double*>(AF);
+            d.x = x;
+            d.xNew = take(h.n);
+        }
+    }
+    if (tail64_.lt >= 0) {
+        tailBase = base + off;
+        off += align256(tail64_.bytes);
+        // the tail levels' arrays are offsets into the image: move every pointer
+        // that lies inside it by the same amount
+        unsigned char* img = tail64_.image;
+        const size_t n = tail64_.bytes;
+        auto move = [&](auto*& p) {
+            using T = std::remove_reference_t<decltype(*p)>;
+            unsigned char* q = (unsigned char*)p;
+            if (q >= img && q < img + n) p = (T*)(tailBase + (q - img));
+        };
+        for (int l = tail64_.lt; l < L; l++) {
+            DLevel& d = dev[l];
+            move(d.AC); move(d.b); move(d.AF); move(d.x); move(d.xNew);
+            move(d.start); move(d.nb); move(d.toCoarse); move(d.memberStart); move(d.members);
+        }
+        // the integer maps the kernel stages (the values are written before read)
+        AXS_HIP(hipMemcpyAsync(tailBase + tail64_.intBase, img + tail64_.intBase, tail64_.staged - tail64_.intBase,
+                               hipMemcpyDeviceToDevice, s));
+        tail64_.image = tailBase;
+    }
+    x0Home_ = x;
+    x0Other_ = dev[0].xNew;
+    return saved;
+}
+
+void MGDevice::unbindWork(const Bound& saved) {
+    dev = saved.dev;
+    tail64_.image = saved.tailImage;
+    x0Home_ = saved.x0Home;
+    x0Other_ = saved.x0Other;
+}
 
 // smoothen / smoothenRegular / smoothenSingleBlock, multigrid.cu:664-725
 void MGDevice::smooth(hipStream_t s, int l, int sweeps, const MGParams& p, const Ctl* ctl) {

@@ -99,6 +99,23 @@ public:
     double bytesPerSolve(const MGParams& p, bool buildOperators) const;
     double restrictBytes(int l, double v) const;
     int coarsestCells() const { return host.empty() ? 0 : host.back().n; }
+    // Work arrays of one FP64 weighted-Jacobi solve placed in an external
+    // region: level 0's second iterate and residual, every coarser level's
+    // operator, right-hand side, iterates and residual, and the LDS tail's
+    // image.  workBytes() is the region's size (256-byte aligned pieces);
+    // bindWork() points the hierarchy at the region (level 0 at the caller's
+    // AC, b, AF, x) and enqueues the copy of the tail image's integer maps
+    // into it; unbindWork() restores the hierarchy's own arrays.  Maps and
+    // tables stay where build() put them.
+    size_t workBytes() const;
+    struct Bound {
+        std::vector<DLevel> dev;
+        unsigned char* tailImage = nullptr;
+        double *x0Home = nullptr, *x0Other = nullptr;
+    };
+    Bound bindWork(unsigned char* base, const double* AC, const double* b, const double* AF, double* x,
+                   hipStream_t s);
+    void unbindWork(const Bound& saved);
     // first level of the LDS-resident tail (-1: no tail) and its LDS bytes
     int tailLevel(bool fp32) const { return (fp32 ? tail32_ : tail64_).lt; }
     size_t tailBytes(bool fp32) const { return (fp32 ? tail32_ : tail64_).bytes; }

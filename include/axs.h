@@ -294,6 +294,22 @@ axs_status axs_mg_get_maps(const axs_mg* mg, int level, int* face_start, int* fa
 axs_status axs_mg_solve(axs_mg* mg, const double* AC, const double* AF, const double* b,
                         double* x, int cycles, double weight, int pre_sweeps, int post_sweeps,
                         int coarse_sweeps, int precision, int coarse_direct, int smoother);
+/* The solve of axs_mg_solve (FP64, weighted-Jacobi smoothing, no coarsest
+ * direct solve) without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  AC, AF, b
+ * and x are DEVICE arrays of the current process in the level-0 CSR order of
+ * `mg` (x in: start, out: answer).  The V-cycles' work arrays live in one
+ * static device array of AXS_MG_NOALLOC_SCRATCH_BYTES per GPU, cleared at the
+ * start of every call; a hierarchy that needs more (axs_mg_create on more than
+ * about 1.4M triangle cells) is rejected with AXS_ERR_RANGE.  The solve is
+ * enqueued on `stream` (a hipStream_t; NULL: the hierarchy's own stream) and
+ * the call returns without waiting.  Calls that overlap on one GPU share that
+ * array and race.  Results are bit-identical to axs_mg_solve's. */
+#define AXS_MG_NOALLOC_SCRATCH_BYTES (64u << 20)
+axs_status axs_mg_solve_noalloc(axs_mg* mg, const double* AC, const double* AF, const double* b, double* x,
+                                int cycles, double weight, int pre_sweeps, int post_sweeps, int coarse_sweeps,
+                                void* stream);
 /* Benchmark form: the system is uploaded once (x = 0), one solve of `cycles`
  * V-cycles (coarse operators included) is captured in a HIP graph, as the
  * solver's iteration graph runs it, and `repeats` replays are timed with HIP
