@@ -520,6 +520,9 @@ class Solver:
             _check(lib().axs_get_residual_history(self._h, res, _dptr(out), cnt.value, C.byref(cnt)), self._h)
         return out
 
+    def frame_count(self) -> int:
+        return lib().axs_frame_count(self._h)
+
     def frames(self) -> list:
         """[(time, {"u", "v", "p"})] of the transient frames."""
         out = []

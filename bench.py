@@ -54,6 +54,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cells", type=float, default=1.0e6)
+    ap.add_argument("--mg-levels", type=int, default=6,
+                    help="multigrid levels of the nozzle case (its configuration: 4:1 coarsening, 6 levels)")
     ap.add_argument("--cpu-iters", type=int, default=10, help="oracle sample size (outer iterations)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
@@ -78,7 +80,7 @@ def main() -> int:
     import axs
 
     t0 = time.time()
-    case = axs.nozzle_case(cells=args.cells, max_iter=args.warmup)
+    case = axs.nozzle_case(cells=args.cells, max_iter=args.warmup, mg_max_levels=args.mg_levels)
     m = case.mesh
     log(f"mesh: {m.n_cells} cells, {m.n_faces} faces ({time.time() - t0:.1f} s)")
 
@@ -123,7 +125,7 @@ def main() -> int:
 
     # p' system of the last iteration -> stand-alone multigrid solve timing
     fs = m.arrays["cell_face_start"]
-    mg = axs.Multigrid(fs, _slot_neighbours(m))
+    mg = axs.Multigrid(fs, _slot_neighbours(m), max_levels=args.mg_levels)
     AC, AF, b = s.field("pp_ac"), s.field("pp_af"), s.field("pp_b")
     mg_ms, mg_bytes, _ = mg.bench(AC, AF, b, cycles=case.config.mg_max_iter, repeats=20)
     mg_gbs = mg_bytes / (mg_ms * 1e-3) / 1e9
@@ -138,7 +140,7 @@ def main() -> int:
     # same number of iterations from the same cold start
     mixed = None
     if not args.no_mixed:
-        mcase = axs.nozzle_case(mesh=m, max_iter=pre_iters, precision=axs.PREC_MIXED)
+        mcase = axs.nozzle_case(mesh=m, max_iter=pre_iters, precision=axs.PREC_MIXED, mg_max_levels=args.mg_levels)
         sm = axs.make_solver(mcase)
         sm.run()
         mms = sm.run_timed(args.steps)
@@ -163,7 +165,8 @@ def main() -> int:
         cells = sum(c.mesh.n_cells for c in cases)
         batch = axs.batch_case(cases, transient=1, time_scheme=axs.TIME_BDF2, dt=1e-3, t_end=2e-3,
                                max_iter=args.c4_iters, use_non_orth=1, structured_mesh=0,
-                               gradient_scheme=axs.GRAD_LSQ, convection_scheme=axs.CONV_UPWIND)
+                               gradient_scheme=axs.GRAD_LSQ, convection_scheme=axs.CONV_UPWIND,
+                               save_key_frame_iter=args.c4_steps)
         log(f"c4: {len(cases)} cases, {cells} cells ({time.time() - tc:.1f} s to generate)")
         sb = axs.make_solver(batch)
         sb.run()  # two time steps: hierarchy, graph, warm caches
@@ -180,6 +183,7 @@ def main() -> int:
         templates = [c.info["template"] for c in cases]
         c4 = {"cases": len(cases), "cells": cells, "templates": {t: templates.count(t) for t in set(templates)},
               "time_steps": args.c4_steps, "dt": 1e-3, "max_iter_per_step": args.c4_iters,
+              "frames_captured": sb.frame_count(), "save_key_frame_iter": args.c4_steps,
               "outer_iterations": sst.iterations, "solve_s": round(sst.solve_ms / 1e3, 3),
               "cell_updates_per_s": round(cups, 1), "per_case_latency_s": round(wall, 3),
               "mg_levels": sst.mg_levels,
