@@ -91,9 +91,13 @@ def build_compat() -> Path:
     if archive.exists():
         archive.unlink()
     _run(["ar", "rcs", str(archive), str(obj)])
+    # the demo plays the GUI: its console and residual plot are the stand-ins
+    # under tests/compat_stub, and it compiles the GUI-side hooks against them
     demo = LIB_DIR / "compat_demo"
-    _run(["g++", "-std=c++17", "-O2", "-Wall", *inc, str(PKG / "compat" / "compat_demo.cpp"), str(archive),
-          f"-L{LIB_DIR}", "-laxs", f"-Wl,-rpath,$ORIGIN", "-pthread", "-o", str(demo)])
+    stub = [f"-I{ROOT / 'tests' / 'compat_stub'}"]
+    _run(["g++", "-std=c++17", "-O2", "-Wall", *inc, *stub, str(PKG / "compat" / "compat_demo.cpp"),
+          str(PKG / "compat" / "gui_hooks.cpp"), str(archive), f"-L{LIB_DIR}", "-laxs", f"-Wl,-rpath,$ORIGIN",
+          "-pthread", "-o", str(demo)])
     return demo
 
 

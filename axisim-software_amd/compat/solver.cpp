@@ -3,6 +3,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <sstream>
+
+#include "gui_hooks.hpp"
 
 namespace axs::compat {
 
@@ -142,12 +145,45 @@ BoundaryCondition effectiveBC(const BoundarySegmentGroup& g, BoundaryVariable v)
     return bc;
 }
 
-Solver::Solver() {
+Solver::Solver(Config& cfgRef) : config(cfgRef), g(cfgRef.g), f(cfgRef.f), varUnits(cfgRef.varUnits) {
     initConfigResiduals(cfg);
+    bindResidualStorage();
     if (axs_create(&handle_, 0) != AXS_OK) {
         message_ = std::string("no GPU: ") + axs_last_error(nullptr);
         handle_ = nullptr;
     }
+}
+
+// resVal / scaleVal of every residual point at this Solver's host storage
+// (allocateResiduals gives each its pinned pair, memory_manger.cpp:270-285)
+void Solver::bindResidualStorage() {
+    const char* names[5] = {"U", "V", "Continuity", "Temperature", "Concentration"};
+    for (int i = 0; i < 5; i++) {
+        ConfigResidual& r = cfg.at(names[i]);
+        resStorage_[i] = scaleStorage_[i] = 0.0;
+        r.resVal = &resStorage_[i];
+        r.scaleVal = &scaleStorage_[i];
+    }
+}
+
+void Solver::say(const std::string& line) {
+    {
+        std::lock_guard<std::mutex> lock(msgMutex_);
+        message_ = line;
+    }
+    consoleAddLine(console, line);
+}
+
+// addFieldType, solver.cpp:333-362
+void Solver::addFieldType() {
+    fieldType.clear();
+    for (const char* n : {"Axial Velocity", "Radial Velocity", "Velocity Magnitude", "Pressure"}) fieldType.push_back(n);
+    if (fieldOption.solveEnergy)
+        for (const char* n : {"Temperature", "dT/dz", "dT/dr"}) fieldType.push_back(n);
+    if (fieldOption.solveConcentration)
+        for (const char* n : {"Concentration", "dC/dz", "dC/dr"}) fieldType.push_back(n);
+    for (const char* n : {"Continuity", "Cell Reynolds Number", "dU/dz", "dU/dr", "dV/dz", "dV/dr"})
+        fieldType.push_back(n);
 }
 
 Solver::~Solver() {
@@ -166,6 +202,8 @@ void Solver::reset() {
     continueSolver = false;
     isReady = false;
     solutions.clear();
+    timeFrames.clear();
+    fieldType.clear();
     fvMesh = FVMesh{};
     currentIteration_ = 0;
     fieldOption = SolverFieldOption{};
@@ -173,6 +211,7 @@ void Solver::reset() {
     configSimple = ConfigSimple{};
     configMultigrid = ConfigMultigrid{};
     initConfigResiduals(cfg);
+    bindResidualStorage();
 }
 
 std::string Solver::lastMessage() const {
@@ -206,8 +245,7 @@ bool Solver::canContinue(const Mesh& mesh, std::string* reason) const {  // solv
 
 bool Solver::runCheck(const Mesh& mesh) {  // solver.cpp:530-661
     auto fail = [&](const std::string& why) {
-        std::lock_guard<std::mutex> lock(msgMutex_);
-        message_ = why;
+        say(why);
         return false;
     };
     if (solverRunning) return fail("Solver still running");
@@ -240,14 +278,20 @@ bool Solver::runCheck(const Mesh& mesh) {  // solver.cpp:530-661
         return fail("Solver needs positive density and viscosity.\n");
     if (continueSolver) {
         std::string why;
-        if (!canContinue(mesh, &why)) continueSolver = false;
+        if (!canContinue(mesh, &why)) {
+            continueSolver = false;
+            say("Continue Solver disabled: " + why + " Starting from a fresh solve.\n");
+        }
     }
     return true;
 }
 
 void Solver::run(Mesh& mesh) {  // solver.cpp:146-173
+    mesh.refreshFVMesh();
     if (!runCheck(mesh)) return;
+    if (!continueSolver) residualPlotResetState(residualPlot);
     shutdown();
+    addFieldType();
     stopRequested = false;
     solverRunning = true;
     worker_ = std::thread([this, m = mesh]() mutable {
@@ -262,9 +306,17 @@ void Solver::progressThunk(void* user, int iteration, int n, const char* const* 
     for (int i = 0; i < n; i++) {
         vals[names[i]] = values[i];
         auto it = self->cfg.find(names[i]);
-        if (it != self->cfg.end()) it->second.resValue = values[i];
+        if (it != self->cfg.end() && it->second.resVal) *it->second.resVal = values[i];
     }
     self->currentIteration_ = iteration + 1;
+    // residualPlot->add + printResidualConsole (solver.cpp:1260-1261)
+    residualPlotAdd(self->residualPlot, iteration, self->cfg);
+    if (self->console) {
+        std::ostringstream line;
+        line << "Iteration " << iteration;
+        for (int i = 0; i < n; i++) line << "  " << names[i] << ": " << values[i];
+        consoleAddLine(self->console, line.str() + "\n");
+    }
     if (self->onResidual) self->onResidual(iteration, vals);
 }
 
@@ -275,8 +327,7 @@ void Solver::runSimple(Mesh mesh) {
     const axs_mesh_view mv = pm.view();
     auto report = [&](axs_status st) {
         if (st == AXS_OK) return true;
-        std::lock_guard<std::mutex> lock(msgMutex_);
-        message_ = axs_last_error(handle_);
+        say(std::string(axs_last_error(handle_)) + "\n");
         return false;
     };
     if (!sameMesh && !report(axs_set_mesh(handle_, &mv))) return;
@@ -300,6 +351,8 @@ void Solver::runSimple(Mesh mesh) {
     c.use_non_orth = configSolver.useNonOrthCorrector;
     c.dt = configSolver.dt;
     c.t_end = configSolver.tEnd;
+    c.save_key_frame_iter = configSolver.saveKeyFrameIter;
+    c.max_time_frames = maxTimeFrames;
     c.max_iter = configSimple.maxIter;
     c.check_conv = configSimple.checkConv;
     c.mg_max_iter = configMultigrid.maxIter;
@@ -337,7 +390,26 @@ void Solver::runSimple(Mesh mesh) {
     for (const auto& [name, id] : fields) {
         std::vector<double> v(axs_field_length(handle_, id));
         if (!report(axs_get_field(handle_, id, v.data(), v.size()))) return;
-        solutions[name] = SolutionField{std::move(v)};
+        BoundaryVariable var = BoundaryVariable::None;
+        if (id == AXS_FIELD_U || id == AXS_FIELD_GRAD_UZ || id == AXS_FIELD_GRAD_UR) var = BoundaryVariable::UVelocity;
+        if (id == AXS_FIELD_V || id == AXS_FIELD_GRAD_VZ || id == AXS_FIELD_GRAD_VR) var = BoundaryVariable::VVelocity;
+        if (id == AXS_FIELD_P || id == AXS_FIELD_GRAD_PZ || id == AXS_FIELD_GRAD_PR) var = BoundaryVariable::Pressure;
+        solutions[name] = SolutionField{std::move(v), g.dr, g.dz, var};
+    }
+    // timeFrames: the library keeps a transient run's frames (appending on a
+    // continued run, dropping them on a fresh or steady one); mirror its list
+    timeFrames.clear();
+    const int nFrames = axs_frame_count(handle_);
+    const std::pair<const char*, int> frameFields[] = {
+        {"Axial Velocity", AXS_FIELD_U}, {"Radial Velocity", AXS_FIELD_V}, {"Pressure", AXS_FIELD_P}};
+    for (int i = 0; i < nFrames; i++) {
+        TimeFrame fr;
+        for (const auto& [name, id] : frameFields) {
+            std::vector<double> v(fvMesh.numCells());
+            if (!report(axs_get_frame(handle_, i, id, &fr.time, v.data(), v.size()))) return;
+            fr.fields[name] = std::move(v);
+        }
+        timeFrames.push_back(std::move(fr));
     }
     axs_stats st;
     if (axs_get_stats(handle_, &st) == AXS_OK) currentIteration_ = st.iterations;

@@ -25,6 +25,13 @@
 
 #include "axs.h"
 
+// The GUI's console and residual plot (graphics/gui/header/console.h,
+// residual_plot.h).  The solver only holds pointers to them; it reaches their
+// methods through the hooks in gui_hooks.hpp, which the GUI's build compiles
+// against its own headers (compat/gui_hooks.cpp).
+class Console;
+class ResidualPlot;
+
 namespace axs::compat {
 
 enum class ResidualType : uint8_t { RESIDUAL_SCALED = 0, RESIDUAL_RAW = 1, RESIDUAL_RMS = 2 };
@@ -78,21 +85,57 @@ struct ConfigSolver {
     int saveKeyFrameIter = 2;
 };
 
+// ConfigResidual (structs/solver_struct.h:240-263): *resVal / *scaleVal are the
+// last check's normalised residual and scale, host memory the Solver owns (the
+// residual plot reads *resVal); res / scale stay null (the per-cell arrays
+// live on the device inside libaxs.so).
 struct ConfigResidual {
     ResidualType type = ResidualType::RESIDUAL_SCALED;
     ResidualNormType normType = ResidualNormType::RESIDUAL_L1;
     ResidualScalingType scaleType = ResidualScalingType::RESIDUAL_SCALING_DIAGONAL;
     bool enabled = false;
-    double resValue = 0.0;  // last normalised residual (the reference's *resVal)
+    double* res = nullptr;
+    double* scale = nullptr;
+    double* resVal = nullptr;
+    double* scaleVal = nullptr;
     double tol = 0.001;
 };
 
-struct FluidPropertyConfig {
+struct FluidPropertyConfig {  // src/header/setting.cuh:61-71
     double rho = 998.0;
     double mu = 0.0010518;
     double cp = 4180.0;
     double k = 0.6;
     double D = 3.0277e-9;
+};
+
+struct GridConfig {  // src/header/setting.cuh:14-37
+    int nr = 170;
+    int nz = 100;
+    double R = 0.0017;
+    double L = 0.01;
+    std::vector<double> dz, dr, r, z, rFace, zFace, Az, Ar, Vcell;
+    double zBias = 1.0;
+    double rBias = 1.0;
+    int n_cell = 0;
+    double A_tot = 0.0;
+};
+
+struct VariableUnits {  // manager/header/unit_manager.h:6-31 (display-unit indices)
+    std::uint8_t LUnit = 0, RUnit = 0;
+    std::uint8_t axialUnit = 0, radialUnit = 0, pressureUnit = 0, temperatureUnit = 0, concentrationUnit = 0;
+    std::uint8_t rhoUnit = 0, muUnit = 0, DUnit = 0;
+    std::uint8_t specificHeatUnit = 0, heatCondUnit = 0;
+    std::uint8_t VmaxUnit = 0;
+    std::uint8_t layerKUnit = 0, layerDUnit = 0;
+};
+
+// Config (structs/solver_struct.h:325-331): the project-owned configuration the
+// Solver is constructed on and keeps references into.
+struct Config {
+    FluidPropertyConfig f;
+    GridConfig g;
+    VariableUnits varUnits;
 };
 
 struct VariablesSimple {  // only the relaxation factors are configuration
@@ -158,10 +201,13 @@ struct Mesh {
     FVMesh fvMesh;
     std::vector<BoundarySegmentGroup> boundaryGroups;
     const FVMesh& getFVMesh() const { return fvMesh; }
+    void refreshFVMesh() {}  // the mesher rebuilds its cached FVMesh here
 };
 
-struct SolutionField {
+struct SolutionField {  // structs/boundary_struct.h:537-541
     std::vector<double> field;
+    std::vector<double> dr, dz;
+    BoundaryVariable boundaryVariable = BoundaryVariable::None;
 };
 
 // getDefaultBCType + getEffectiveBC (manager/source/boundary_func.cpp:104-197)
@@ -170,10 +216,29 @@ BoundaryCondition effectiveBC(const BoundarySegmentGroup& g, BoundaryVariable v)
 
 class Solver {
 public:
-    Solver();
+    explicit Solver(Config& config);  // solver/source/solver.cpp:104-114
     ~Solver();
     Solver(const Solver&) = delete;
     Solver& operator=(const Solver&) = delete;
+
+    // combo-box labels the settings panels index with the enum values
+    // (solver/header/solver.h:23-35)
+    const char* boundaryType[5] = {"Wall", "Velocity Inlet", "Pressure Outlet", "Symmetry", "Far Field"};
+    const char* residualType[4] = {"Scaled", "Raw Residual", "RMS", "Custom Residual"};
+    const char* residualNormType[3] = {"L1 Norm", "L2 Norm", "Linf Norm"};
+    const char* residualScalingType[4] = {"None", "N", "sqrt(N)", "Diagonal"};
+    const char* linearSolverType[2] = {"Jacobi", "Red Black Gauss Seidel"};
+    const char* velocitySolverType[1] = {"SIMPLE"};
+    const char* bcTypeNames[2] = {"Constant", "Flux"};
+    const char* bcInletTypeNames[3]{"Constant", "Flux", "Fully Developed"};
+    const char* convectionDiscretizationType[3] = {"First Order Upwind", "Central Difference", "Second Order Upwind"};
+    const char* gradientSchemeType[2] = {"Green-Gauss", "Least Squares"};
+    const char* timeSchemeType[2] = {"First Order Implicit", "Second Order Implicit"};
+    const char* residualPlotType[5] = {"U", "V", "Continuity", "Temperature", "Concentration"};
+
+    // the fields a run produces, in the results panel's order (addFieldType,
+    // solver.cpp:333-362)
+    std::vector<std::string> fieldType;
 
     bool continueSolver = false;
     bool isReady = false;
@@ -187,9 +252,17 @@ public:
     void shutdown();  // join the worker
     void reset();
 
-    // configuration, written directly by the settings panels
-    FluidPropertyConfig f;
+    // configuration, written directly by the settings panels; config / g / f /
+    // varUnits refer into the project's Config
+    Config& config;
+    GridConfig& g;
+    FluidPropertyConfig& f;
+    VariableUnits& varUnits;
     SolverFieldOption fieldOption;
+
+    // GUI classes (set by the window, graphics/window/source/gui.cpp:307-313)
+    Console* console = nullptr;
+    ResidualPlot* residualPlot = nullptr;
     ConfigSimple configSimple;
     ConfigSolver configSolver;
     ConfigMultigrid configMultigrid;
@@ -199,6 +272,16 @@ public:
     // results, read once solverRunning drops
     std::unordered_map<std::string, SolutionField> solutions;
     FVMesh fvMesh;
+
+    // transient frames (solver/header/solver.h:96-108): the start state, every
+    // saveKeyFrameIter steps and the last step, at most maxTimeFrames; a
+    // continued transient run appends
+    struct TimeFrame {
+        double time = 0.0;
+        std::unordered_map<std::string, std::vector<double>> fields;
+    };
+    std::vector<TimeFrame> timeFrames;
+    int maxTimeFrames = 500;
     int currentIteration() const { return currentIteration_; }
     std::string lastMessage() const;
 
@@ -209,7 +292,13 @@ public:
 private:
     void runSimple(Mesh mesh);
     bool runCheck(const Mesh& mesh);
+    void addFieldType();
+    void say(const std::string& line);
+    void bindResidualStorage();
     static void progressThunk(void* user, int iteration, int n, const char* const* names, const double* values);
+
+    // host storage behind cfg[...].resVal / scaleVal (five residuals)
+    double resStorage_[5] = {}, scaleStorage_[5] = {};
 
     axs_solver* handle_ = nullptr;
     std::thread worker_;
@@ -220,3 +309,35 @@ private:
 };
 
 }  // namespace axs::compat
+
+// The GUI names these types unqualified (global in the reference's headers,
+// which the drop-in replaces).
+using axs::compat::BCType;
+using axs::compat::BoundaryCondition;
+using axs::compat::BoundarySegmentGroup;
+using axs::compat::BoundaryType;
+using axs::compat::BoundaryVariable;
+using axs::compat::Config;
+using axs::compat::ConfigMultigrid;
+using axs::compat::ConfigResidual;
+using axs::compat::ConfigSimple;
+using axs::compat::ConfigSolver;
+using axs::compat::ConvectionScheme;
+using axs::compat::FluidPropertyConfig;
+using axs::compat::FVCell;
+using axs::compat::FVFace;
+using axs::compat::FVMesh;
+using axs::compat::GradientScheme;
+using axs::compat::GridConfig;
+using axs::compat::LinearSolverType;
+using axs::compat::Mesh;
+using axs::compat::MeshType;
+using axs::compat::ResidualNormType;
+using axs::compat::ResidualScalingType;
+using axs::compat::ResidualType;
+using axs::compat::SolutionField;
+using axs::compat::Solver;
+using axs::compat::SolverFieldOption;
+using axs::compat::TimeScheme;
+using axs::compat::VariableUnits;
+using axs::compat::VariablesSimple;
