@@ -27,8 +27,9 @@ NORM_L1, NORM_L2, NORM_LINF = range(3)
 SCALE_NONE, SCALE_N, SCALE_SQRT_N, SCALE_DIAGONAL, SCALE_CONTINUITY = range(5)
 PREC_FP64, PREC_MIXED = range(2)
 MG_SMOOTH_JACOBI, MG_SMOOTH_RBGS = range(2)
-VAR_U, VAR_V, VAR_P = range(3)
-RES_U, RES_V, RES_CONTINUITY = range(3)
+VAR_U, VAR_V, VAR_P, VAR_T, VAR_C = range(5)
+RES_U, RES_V, RES_CONTINUITY, RES_TEMPERATURE, RES_CONCENTRATION = range(5)
+RES_COUNT = 5
 KERNEL_JACOBI_UV, KERNEL_MOMENTUM, KERNEL_MG_SMOOTH, KERNEL_FACE_FLUX, KERNEL_PP_COEFF, KERNEL_JACOBI_UV_FUSED = range(6)
 
 FIELDS = {
@@ -39,6 +40,9 @@ FIELDS = {
     "v_ac": 20, "v_b": 21, "v_invac": 22, "v_af": 23,
     "pp_ac": 24, "pp_b": 25, "pp_invac": 26, "pp_af": 27,
     "u_old": 28, "v_old": 29, "u_old2": 30, "v_old2": 31,
+    "t": 32, "c": 33, "grad_tz": 34, "grad_tr": 35, "grad_cz": 36, "grad_cr": 37,
+    "t_ac": 50, "t_b": 51, "t_invac": 52, "t_af": 53, "c_ac": 54, "c_b": 55, "c_invac": 56, "c_af": 57,
+    "t_old": 58, "c_old": 59, "t_old2": 60, "c_old2": 61, "wall_cw": 62, "wall_ocr": 63,
     "inv_dpn": 40, "wp": 41, "dpb": 42, "inv_a2d": 43, "lsq_wz": 44, "lsq_wr": 45,
     "lsq_inv_zz": 46, "lsq_inv_zr": 47, "lsq_inv_rr": 48, "length2d": 49,
 }
@@ -64,6 +68,8 @@ class BCView(C.Structure):
     _fields_ = [
         ("n_groups", C.c_int), ("type", _u8p), ("boundary_type", _u8p), ("value", _dp),
         ("length", _dp), ("amplitude", _dp), ("frequency", _dp),
+        ("vmax", _dp), ("km", _dp), ("hill_n", _dp), ("inhib_m", _dp), ("inhib_k2", _dp), ("inhib_v2", _dp),
+        ("wall_resistance", _dp),
     ]
 
 
@@ -81,11 +87,13 @@ class Config(C.Structure):
         ("mg_max_iter", C.c_int), ("mg_weight", C.c_double), ("mg_coarse_sweeps", C.c_int),
         ("mg_pre_sweeps", C.c_int), ("mg_post_sweeps", C.c_int),
         ("alpha_u", C.c_double), ("alpha_p", C.c_double), ("rho", C.c_double), ("mu", C.c_double),
-        ("residual", ResidualConfig * 3),
+        ("residual", ResidualConfig * RES_COUNT),
         ("structured_mesh", C.c_int),
         ("mg_max_levels", C.c_int), ("mg_smoother", C.c_int), ("precision", C.c_int),
         ("coarse_direct", C.c_int),
         ("save_key_frame_iter", C.c_int), ("max_time_frames", C.c_int),
+        ("solve_energy", C.c_int), ("solve_concentration", C.c_int),
+        ("cp", C.c_double), ("k", C.c_double), ("diffusivity", C.c_double),
     ]
 
 
@@ -94,7 +102,7 @@ class Stats(C.Structure):
         ("iterations", C.c_int), ("converged", C.c_int), ("stopped", C.c_int),
         ("n_cells", C.c_int), ("n_faces", C.c_int), ("n_face_refs", C.c_int),
         ("n_colors", C.c_int), ("mg_levels", C.c_int), ("mg_level_cells", C.c_int * 32),
-        ("setup_ms", C.c_double), ("solve_ms", C.c_double), ("last_residual", C.c_double * 3),
+        ("setup_ms", C.c_double), ("solve_ms", C.c_double), ("last_residual", C.c_double * RES_COUNT),
         ("bytes_per_iteration", C.c_double), ("vcycle_bytes", C.c_double),
         ("sweeps_per_launch", C.c_int), ("sweep_block_cells", C.c_int),
     ]
@@ -149,6 +157,7 @@ def load_library(path: os.PathLike | None = None) -> C.CDLL:
         "axs_get_residual_history": (C.c_int, [S, C.c_int, _dp, C.c_int, _ip]),
         "axs_get_stats": (C.c_int, [S, C.POINTER(Stats)]),
         "axs_frame_count": (C.c_int, [S]),
+        "axs_get_wall_diagnostic": (C.c_int, [S, _dp]),
         "axs_get_frame": (C.c_int, [S, C.c_int, C.c_int, _dp, _dp, C.c_size_t]),
         "axs_mg_create": (C.c_int, [C.POINTER(S), C.c_int, _ip, _ip, C.c_int]),
         "axs_mg_destroy": (None, [S]),
@@ -358,6 +367,16 @@ class BC:
     boundary_type: list | None = None
     amplitude: list | None = None
     frequency: list | None = None
+    # Michaelis-Menten / Hill wall kinetics per group (None: defaults)
+    vmax: list | None = None
+    km: list | None = None
+    hill_n: list | None = None
+    inhib_m: list | None = None
+    inhib_k2: list | None = None
+    inhib_v2: list | None = None
+    wall_resistance: list | None = None
+
+    KINETICS = ("vmax", "km", "hill_n", "inhib_m", "inhib_k2", "inhib_v2", "wall_resistance")
 
     def view(self):
         n = len(self.type)
@@ -377,25 +396,30 @@ class BC:
         v.length = _dptr(keep["length"])
         v.amplitude = _dptr(keep["amp"])
         v.frequency = _dptr(keep["freq"])
+        for name in self.KINETICS:
+            vals = getattr(self, name)
+            if vals is not None:
+                keep[name] = np.asarray(vals, dtype=np.float64)
+                setattr(v, name, _dptr(keep[name]))
         return v, keep
 
 
 # Default BC type of each (boundary type, variable) (getDefaultBCType,
 # manager/source/boundary_func.cpp:117-197)
-_DEFAULT_BC = {
-    BOUNDARY_VELOCITY_INLET: (BC_DIRICHLET, BC_DIRICHLET, BC_NEUMANN),
-    BOUNDARY_PRESSURE_OUTLET: (BC_NEUMANN, BC_NEUMANN, BC_DIRICHLET),
-    BOUNDARY_WALL: (BC_DIRICHLET, BC_DIRICHLET, BC_NEUMANN),
-    BOUNDARY_SYMMETRY: (BC_NEUMANN, BC_DIRICHLET, BC_NEUMANN),
-    BOUNDARY_FAR_FIELD: (BC_DIRICHLET, BC_DIRICHLET, BC_NEUMANN),
+_DEFAULT_BC = {  # u, v, p, T, C
+    BOUNDARY_VELOCITY_INLET: (BC_DIRICHLET, BC_DIRICHLET, BC_NEUMANN, BC_DIRICHLET, BC_DIRICHLET),
+    BOUNDARY_PRESSURE_OUTLET: (BC_NEUMANN, BC_NEUMANN, BC_DIRICHLET, BC_NEUMANN, BC_NEUMANN),
+    BOUNDARY_WALL: (BC_DIRICHLET, BC_DIRICHLET, BC_NEUMANN, BC_NEUMANN, BC_NEUMANN),
+    BOUNDARY_SYMMETRY: (BC_NEUMANN, BC_DIRICHLET, BC_NEUMANN, BC_NEUMANN, BC_NEUMANN),
+    BOUNDARY_FAR_FIELD: (BC_DIRICHLET, BC_DIRICHLET, BC_NEUMANN, BC_NEUMANN, BC_NEUMANN),
 }
 
 
-def default_bcs(group_types: list[int], overrides: dict | None = None) -> dict:
-    """BCs of u, v, p for groups of the given boundary types, default values 0.
-    overrides: {(group, var): (type, value)}."""
+def default_bcs(group_types: list[int], overrides: dict | None = None, scalars: bool = False) -> dict:
+    """BCs of u, v, p (and T, C when `scalars`) for groups of the given boundary
+    types, default values 0.  overrides: {(group, var): (type, value)}."""
     out = {}
-    for var in (VAR_U, VAR_V, VAR_P):
+    for var in ((VAR_U, VAR_V, VAR_P, VAR_T, VAR_C) if scalars else (VAR_U, VAR_V, VAR_P)):
         types = [_DEFAULT_BC[t][var] for t in group_types]
         values = [0.0] * len(group_types)
         out[var] = BC(types, values, boundary_type=list(group_types))
@@ -415,8 +439,8 @@ def default_config(**kw) -> Config:
     lib().axs_config_defaults(C.byref(c))
     for k, v in kw.items():
         if k == "residual_tol":
-            for i in range(3):
-                c.residual[i].tol = v[i]
+            for i, t in enumerate(v):
+                c.residual[i].tol = t
         else:
             setattr(c, k, v)
     return c
@@ -524,17 +548,25 @@ class Solver:
         return lib().axs_frame_count(self._h)
 
     def frames(self) -> list:
-        """[(time, {"u", "v", "p"})] of the transient frames."""
+        """[(time, {"u", "v", "p"[, "t"][, "c"]})] of the transient frames."""
         out = []
         N = self.mesh.n_cells
+        solved = [("u", 0), ("v", 1), ("p", 2)]
+        if getattr(self, "cfg", None) is not None:
+            solved += [("t", 32)] * bool(self.cfg.solve_energy) + [("c", 33)] * bool(self.cfg.solve_concentration)
         for i in range(lib().axs_frame_count(self._h)):
             t = C.c_double()
             fields = {}
-            for name, fid in (("u", 0), ("v", 1), ("p", 2)):
+            for name, fid in solved:
                 a = np.zeros(N)
                 _check(lib().axs_get_frame(self._h, i, fid, C.byref(t), _dptr(a), N), self._h)
                 fields[name] = a
             out.append((t.value, fields))
+        return out
+
+    def wall_diagnostic(self) -> np.ndarray:
+        out = np.zeros(10)
+        _check(lib().axs_get_wall_diagnostic(self._h, _dptr(out)), self._h)
         return out
 
     def stats(self) -> Stats:
@@ -718,10 +750,41 @@ def pipe_case(nz: int = 128, nr: int = 32, L: float = 0.2, R: float = 0.01, u_in
                                  groups=(G_INLET, G_OUTLET, G_AXIS, G_WALL))])
     bcs = default_bcs(GROUP_TYPES, {(G_INLET, VAR_U): (BC_DIRICHLET, u_in)})
     c = default_config(rho=rho, mu=mu, gradient_scheme=GRAD_GREEN_GAUSS, convection_scheme=CONV_UPWIND,
-                       max_iter=max_iter, structured_mesh=1, residual_tol=(tol, tol, tol))
+                       max_iter=max_iter, structured_mesh=1, residual_tol=(tol,) * RES_COUNT)
     for k, v in cfg.items():
         setattr(c, k, v)
     return Case("pipe", mesh, bcs, c, dict(L=L, R=R, u_in=u_in, rho=rho, mu=mu))
+
+
+def scalar_pipe_case(nz: int = 64, nr: int = 16, energy: bool = True, concentration: bool = True,
+                     wall_kinetics: int = BC_MICHAELIS_MENTEN, t_in: float = 310.0, t_wall: float = 300.0,
+                     c_in: float = 0.2, vmax: float = 1e-6, km: float = 0.01, hill_n: float = 1.0,
+                     inhib_v2: float = 0.0, inhib_k2: float = 0.05, inhib_m: float = 2.0,
+                     wall_resistance: float = 0.0, **kw) -> Case:
+    """The pipe with heat and species transport: inlet temperature t_in into a
+    wall held at t_wall; inlet concentration c_in consumed at the wall by
+    Michaelis-Menten (or Hill) kinetics with Vmax, Km (n, optional substrate
+    inhibition V2, K2, m) through an extra wall resistance."""
+    case = pipe_case(nz=nz, nr=nr, **kw)
+    u_in = case.info["u_in"]
+    bcs = default_bcs(GROUP_TYPES, {(G_INLET, VAR_U): (BC_DIRICHLET, u_in), (G_INLET, VAR_T): (BC_DIRICHLET, t_in),
+                                    (G_WALL, VAR_T): (BC_DIRICHLET, t_wall), (G_INLET, VAR_C): (BC_DIRICHLET, c_in)},
+                      scalars=True)
+    cb = bcs[VAR_C]
+    n = len(cb.type)
+    cb.type[G_WALL] = wall_kinetics
+    cb.vmax, cb.km, cb.hill_n = [0.0] * n, [0.0] * n, [1.0] * n
+    cb.inhib_m, cb.inhib_k2, cb.inhib_v2, cb.wall_resistance = [1.0] * n, [0.0] * n, [0.0] * n, [0.0] * n
+    cb.vmax[G_WALL], cb.km[G_WALL], cb.hill_n[G_WALL] = vmax, km, hill_n
+    cb.inhib_m[G_WALL], cb.inhib_k2[G_WALL], cb.inhib_v2[G_WALL] = inhib_m, inhib_k2, inhib_v2
+    cb.wall_resistance[G_WALL] = wall_resistance
+    case.bcs = bcs
+    c = case.config
+    c.solve_energy, c.solve_concentration = int(energy), int(concentration)
+    c.residual[RES_TEMPERATURE].enabled = int(energy)
+    c.residual[RES_CONCENTRATION].enabled = int(concentration)
+    case.name = "scalar_pipe"
+    return case
 
 
 def expansion_case(nz=(137, 887), nr=(85, 171), bias: float = 1.05, re_throat: float = 500.0, rho=1000.0,
@@ -741,7 +804,7 @@ def expansion_case(nz=(137, 887), nr=(85, 171), bias: float = 1.05, re_throat: f
     mesh = multiblock_mesh(blocks)
     bcs = default_bcs(GROUP_TYPES, {(G_INLET, VAR_U): (BC_DIRICHLET, u_in)})
     c = default_config(rho=rho, mu=mu, gradient_scheme=GRAD_LSQ, convection_scheme=CONV_SOU, max_iter=max_iter,
-                       structured_mesh=1, residual_tol=(tol, tol, tol))
+                       structured_mesh=1, residual_tol=(tol,) * RES_COUNT)
     for k, v in cfg.items():
         setattr(c, k, v)
     return Case("expansion", mesh, bcs, c, dict(u_in=u_in, r1=r1, r2=r2))
@@ -771,7 +834,7 @@ def nozzle_case(cells: float = 1.0e6, spacing: float | None = None, re_throat=50
     u_throat = re_throat * mu / (rho * 2 * g["r_throat"])
     u_in = u_throat * (g["r_throat"] / g["r_in"]) ** 2
     bcs = default_bcs(GROUP_TYPES, {(G_INLET, VAR_U): (BC_DIRICHLET, u_in)})
-    c = default_config(rho=rho, mu=mu, use_non_orth=1, max_iter=max_iter, residual_tol=(tol, tol, tol))
+    c = default_config(rho=rho, mu=mu, use_non_orth=1, max_iter=max_iter, residual_tol=(tol,) * RES_COUNT)
     for k, v in cfg.items():
         setattr(c, k, v)
     return Case("nozzle", mesh, bcs, c, dict(u_in=u_in, u_throat=u_throat))
@@ -834,20 +897,21 @@ def batch_case(cases: list[Case], groups_per_mesh: int = 4, **cfg) -> Case:
     each case's (first cell, cell count) in the union."""
     mesh = concat_meshes([c.mesh for c in cases], groups_per_mesh)
     bcs = {}
-    for var in (VAR_U, VAR_V, VAR_P):
-        cols = {k: [] for k in ("type", "value", "length", "boundary_type", "amplitude", "frequency")}
+    defaults = (("type", BC_NONE), ("value", 0.0), ("length", 0.0), ("boundary_type", 0), ("amplitude", 0.0),
+                ("frequency", 0.0), ("vmax", 0.0), ("km", 0.0), ("hill_n", 1.0), ("inhib_m", 1.0),
+                ("inhib_k2", 0.0), ("inhib_v2", 0.0), ("wall_resistance", 0.0))
+    for var in sorted(cases[0].bcs):
+        cols = {k: [] for k, _ in defaults}
         for c in cases:
             bc = c.bcs[var]
             n = len(bc.type)
             if n > groups_per_mesh:
                 raise ValueError("a case has more boundary groups than groups_per_mesh")
             pad = groups_per_mesh - n
-            for k, default in (("type", BC_NONE), ("value", 0.0), ("length", 0.0), ("boundary_type", 0),
-                               ("amplitude", 0.0), ("frequency", 0.0)):
+            for k, default in defaults:
                 vals = getattr(bc, k)
                 cols[k] += (list(vals) if vals is not None else [default] * n) + [default] * pad
-        bcs[var] = BC(cols["type"], cols["value"], length=cols["length"], boundary_type=cols["boundary_type"],
-                      amplitude=cols["amplitude"], frequency=cols["frequency"])
+        bcs[var] = BC(**cols)
     c0 = cases[0].config
     config = Config()
     C.pointer(config)[0] = c0

@@ -72,7 +72,7 @@ PackedMesh pack(const FVMesh& m) {
 
 struct BCArrays {  // createBoundaryFieldHost, memory_manger.cpp:51-152
     std::vector<uint8_t> type, btype;
-    std::vector<double> value, length, amp, freq;
+    std::vector<double> value, length, amp, freq, vmax, km, n, m, k2, v2, rtot;
     axs_bc_view view() const {
         axs_bc_view v{};
         v.n_groups = (int)type.size();
@@ -82,6 +82,13 @@ struct BCArrays {  // createBoundaryFieldHost, memory_manger.cpp:51-152
         v.length = length.data();
         v.amplitude = amp.data();
         v.frequency = freq.data();
+        v.vmax = vmax.data();
+        v.km = km.data();
+        v.hill_n = n.data();
+        v.inhib_m = m.data();
+        v.inhib_k2 = k2.data();
+        v.inhib_v2 = v2.data();
+        v.wall_resistance = rtot.data();
         return v;
     }
 };
@@ -97,6 +104,13 @@ BCArrays bcArrays(const std::vector<BoundarySegmentGroup>& groups, BoundaryVaria
     a.length.assign(n, 0.0);
     a.amp.assign(n, 0.0);
     a.freq.assign(n, 0.0);
+    a.vmax.assign(n, 0.0);
+    a.km.assign(n, 0.0);
+    a.n.assign(n, 1.0);
+    a.m.assign(n, 1.0);
+    a.k2.assign(n, 0.0);
+    a.v2.assign(n, 0.0);
+    a.rtot.assign(n, 0.0);
     for (const auto& g : groups) {
         if (g.id < 0) continue;
         const BoundaryCondition bc = effectiveBC(g, var);
@@ -107,6 +121,22 @@ BCArrays bcArrays(const std::vector<BoundarySegmentGroup>& groups, BoundaryVaria
         if (bc.type == BCType::PULSATILE) {
             a.amp[g.id] = bc.amplitude;
             a.freq[g.id] = bc.frequency;
+        }
+        if (bc.type == BCType::MICHAELIS_MENTEN || bc.type == BCType::HILL) {
+            a.vmax[g.id] = bc.Vmax;
+            a.km[g.id] = bc.Km;
+            if (bc.type == BCType::HILL) a.n[g.id] = bc.n;
+            if (bc.inhibition) {
+                a.m[g.id] = bc.m;
+                a.k2[g.id] = bc.K2;
+                a.v2[g.id] = bc.V2;
+            }
+        }
+        auto layers = g.layers.find((int)var);
+        if (layers != g.layers.end()) {
+            double r = 0.0;
+            for (const Layer& l : layers->second) r += l.resistance();
+            a.rtot[g.id] = r;
         }
     }
     return a;
@@ -331,9 +361,10 @@ void Solver::runSimple(Mesh mesh) {
         return false;
     };
     if (!sameMesh && !report(axs_set_mesh(handle_, &mv))) return;
-    const BoundaryVariable vars[3] = {BoundaryVariable::UVelocity, BoundaryVariable::VVelocity,
-                                      BoundaryVariable::Pressure};
-    for (int i = 0; i < 3; i++) {
+    const BoundaryVariable vars[5] = {BoundaryVariable::UVelocity, BoundaryVariable::VVelocity,
+                                      BoundaryVariable::Pressure, BoundaryVariable::StaticTemperature,
+                                      BoundaryVariable::Concentration};
+    for (int i = 0; i < 5; i++) {
         const BCArrays a = bcArrays(mesh.boundaryGroups, vars[i]);
         const axs_bc_view v = a.view();
         if (!report(axs_set_bcs(handle_, i, &v))) return;
@@ -368,8 +399,13 @@ void Solver::runSimple(Mesh mesh) {
     c.rho = f.rho;
     c.mu = f.mu;
     c.structured_mesh = mesh.currentMeshType == MeshType::Structured;
-    const char* names[3] = {"U", "V", "Continuity"};
-    for (int i = 0; i < 3; i++) {
+    c.solve_energy = fieldOption.solveEnergy;
+    c.solve_concentration = fieldOption.solveConcentration;
+    c.cp = f.cp;
+    c.k = f.k;
+    c.diffusivity = f.D;
+    const char* names[5] = {"U", "V", "Continuity", "Temperature", "Concentration"};
+    for (int i = 0; i < 5; i++) {
         const ConfigResidual& r = cfg.at(names[i]);
         c.residual[i].enabled = r.enabled;
         c.residual[i].norm = (int)r.normType;
@@ -385,26 +421,38 @@ void Solver::runSimple(Mesh mesh) {
         {"dU/dz", AXS_FIELD_GRAD_UZ},      {"dU/dr", AXS_FIELD_GRAD_UR},     {"dV/dz", AXS_FIELD_GRAD_VZ},
         {"dV/dr", AXS_FIELD_GRAD_VR},      {"dP/dz", AXS_FIELD_GRAD_PZ},     {"dP/dr", AXS_FIELD_GRAD_PR},
         {"Continuity", AXS_FIELD_CONTINUITY}, {"Velocity Magnitude", AXS_FIELD_VMAG},
-        {"Cell Reynolds Number", AXS_FIELD_CELL_RE}};
+        {"Cell Reynolds Number", AXS_FIELD_CELL_RE}, {"Temperature", AXS_FIELD_T}, {"dT/dz", AXS_FIELD_GRAD_TZ},
+        {"dT/dr", AXS_FIELD_GRAD_TR}, {"Concentration", AXS_FIELD_C}, {"dC/dz", AXS_FIELD_GRAD_CZ},
+        {"dC/dr", AXS_FIELD_GRAD_CR}};
     solutions.clear();
     for (const auto& [name, id] : fields) {
+        const bool isT = id == AXS_FIELD_T || id == AXS_FIELD_GRAD_TZ || id == AXS_FIELD_GRAD_TR;
+        const bool isC = id == AXS_FIELD_C || id == AXS_FIELD_GRAD_CZ || id == AXS_FIELD_GRAD_CR;
+        if ((isT && !fieldOption.solveEnergy) || (isC && !fieldOption.solveConcentration)) continue;
         std::vector<double> v(axs_field_length(handle_, id));
         if (!report(axs_get_field(handle_, id, v.data(), v.size()))) return;
         BoundaryVariable var = BoundaryVariable::None;
         if (id == AXS_FIELD_U || id == AXS_FIELD_GRAD_UZ || id == AXS_FIELD_GRAD_UR) var = BoundaryVariable::UVelocity;
         if (id == AXS_FIELD_V || id == AXS_FIELD_GRAD_VZ || id == AXS_FIELD_GRAD_VR) var = BoundaryVariable::VVelocity;
         if (id == AXS_FIELD_P || id == AXS_FIELD_GRAD_PZ || id == AXS_FIELD_GRAD_PR) var = BoundaryVariable::Pressure;
+        if (isT) var = BoundaryVariable::StaticTemperature;
+        if (isC) var = BoundaryVariable::Concentration;
         solutions[name] = SolutionField{std::move(v), g.dr, g.dz, var};
     }
     // timeFrames: the library keeps a transient run's frames (appending on a
     // continued run, dropping them on a fresh or steady one); mirror its list
     timeFrames.clear();
     const int nFrames = axs_frame_count(handle_);
-    const std::pair<const char*, int> frameFields[] = {
-        {"Axial Velocity", AXS_FIELD_U}, {"Radial Velocity", AXS_FIELD_V}, {"Pressure", AXS_FIELD_P}};
+    const std::pair<const char*, int> frameFields[] = {{"Axial Velocity", AXS_FIELD_U},
+                                                       {"Radial Velocity", AXS_FIELD_V},
+                                                       {"Pressure", AXS_FIELD_P},
+                                                       {"Temperature", AXS_FIELD_T},
+                                                       {"Concentration", AXS_FIELD_C}};
     for (int i = 0; i < nFrames; i++) {
         TimeFrame fr;
         for (const auto& [name, id] : frameFields) {
+            if ((id == AXS_FIELD_T && !fieldOption.solveEnergy) || (id == AXS_FIELD_C && !fieldOption.solveConcentration))
+                continue;
             std::vector<double> v(fvMesh.numCells());
             if (!report(axs_get_frame(handle_, i, id, &fr.time, v.data(), v.size()))) return;
             fr.fields[name] = std::move(v);

@@ -184,6 +184,20 @@ struct BoundaryCondition {
     double value = 0.0;
     double amplitude = 0.1;  // pulsatile only
     double frequency = 1.0;  // pulsatile only (Hz)
+    // Michaelis-Menten / Hill kinetics (MichaelisMentenParams, HillParams,
+    // boundary_struct.h:112-113): n is the Hill exponent; the substrate
+    // inhibition m, K2, V2 applies when `inhibition` is set
+    double Vmax = 0.0, Km = 0.0, n = 1.0;
+    bool inhibition = false;
+    double m = 1.0, K2 = 0.0, V2 = 0.0;
+};
+
+// One wall layer (Layer, boundary_struct.h:24-41): conductivity k, thickness d.
+struct Layer {
+    double k = 0.0;
+    double d = 0.0;
+    double R = 0.0;
+    double resistance() const { return (k != 0.0) ? d / k : 0.0; }
 };
 
 struct BoundarySegmentGroup {
@@ -192,6 +206,8 @@ struct BoundarySegmentGroup {
     BoundaryType type = BoundaryType::WALL;
     float totalLength = 0.0f;
     std::unordered_map<int, BoundaryCondition> bcs;  // keyed by (int)BoundaryVariable
+    // wall layers per variable (temperature, concentration), a series resistance
+    std::unordered_map<int, std::vector<Layer>> layers;  // keyed by (int)BoundaryVariable
 };
 
 // What Solver::run reads from Mesh (mesh.h:226-245).

@@ -46,6 +46,30 @@ struct DBC {
     const uint8_t* type = nullptr;
     const double* value = nullptr;
     const double* length = nullptr;
+    // Michaelis-Menten / Hill kinetics per group (scalars only; null otherwise):
+    // Vmax, Km, Hill n, inhibition m and V2, wall-layer resistance, Km^n, K2^m
+    const double *vmax = nullptr, *km = nullptr, *hillN = nullptr, *inhibM = nullptr, *v2 = nullptr;
+    const double *rtot = nullptr, *kmN = nullptr, *k2M = nullptr;
+};
+
+// One transported scalar (temperature or concentration): field, Jacobi
+// partner, old levels, gradient and its linear system (VariablesSimple temp /
+// conc, Coefficients "Temperature" / "Concentration").
+struct DScalar {
+    double *phi = nullptr, *tmp = nullptr, *old = nullptr, *old2 = nullptr, *gZ = nullptr, *gR = nullptr;
+    double *AC = nullptr, *b = nullptr, *inv = nullptr, *AF = nullptr;
+};
+// Reactive walls (FVFaceDevice cw / ocrWall, per face).
+struct DWall {
+    double *cw = nullptr, *ocr = nullptr;
+};
+// What the scalar assembly needs besides the mesh and the BCs.
+struct ScalarPhysics {
+    double gamma;      // k / (rho cp) or D
+    double fluxScale;  // 1 / rho: the volumetric flux
+    double alpha;      // under-relaxation (alpha_u for temperature, 1 for species)
+    double dt;
+    int conv, scheme, nonOrtho, transient;
 };
 
 // Run control shared by the host and the per-iteration graph.
@@ -61,10 +85,11 @@ struct Ctl {
     int histCap;       // capacity of DState::hist in checks
     int converged;
     int N;
-    int enabled[3], norm[3], scaling[3];
-    double tol[3];
-    double resVal[3], scaleVal[3];
+    int enabled[5], norm[5], scaling[5];   // U, V, Continuity, Temperature, Concentration
+    double tol[5];
+    double resVal[5], scaleVal[5];
 };
+constexpr int kHistStride = 6;  // per residual check in DState::hist: five residuals, then the iteration
 
 // Everything the SIMPLE kernels touch (VariablesSimple + Coefficients).
 struct DState {
@@ -77,8 +102,8 @@ struct DState {
     // FP32 copies written by kMomentum in mixed precision (null otherwise)
     float* AF32;
     float2* uvInv32;  // (1/AC_u, 1/AC_v) interleaved
-    double *partial;   // per-block residual partial sums
-    double *hist;      // residual history: U, V, Continuity, iteration per check
+    double *partial;   // per-block residual partial sums (9 per block, then 8 of the scalars)
+    double *hist;      // residual history: kHistStride values per check
     Ctl* ctl;
 };
 
@@ -178,7 +203,16 @@ void launchPPRhs(hipStream_t s, const DMesh& m, DState st, Physics ph, int cross
 void launchGradPPUpdate(hipStream_t s, const DMesh& m, DBC ppbc, DState st, Physics ph);
 void launchMassFluxUpdate(hipStream_t s, const DMesh& m, DBC bp, DState st, Physics ph);
 void launchResidual(hipStream_t s, const DMesh& m, DState st, int blocks);
-void launchFinish(hipStream_t s, DState st, int blocks);
+// scalars: non-null phi of each solved scalar; their residual partials go
+// after the momentum ones (8 per block from st.partial + 9 * blocks)
+void launchResidualScalars(hipStream_t s, const DMesh& m, DState st, DScalar t, DScalar c, int blocks);
+void launchFinish(hipStream_t s, DState st, int blocks, bool scalars);
+// the scalar equation of one field: transport, wall kinetics, unsteady term,
+// under-relaxation and inverse diagonal (solver.cpp:1222-1235)
+void launchScalar(hipStream_t s, const DMesh& m, DBC bc, DScalar sc, DWall wall, const double* mDot, ScalarPhysics ph,
+                  const Ctl* ctl);
+// old levels of a scalar at the start of a time step (old2 <- old first when bdf2)
+void launchRotate(hipStream_t s, int n, double* old2, double* old, const double* phi, int bdf2);
 void launchStepBegin(hipStream_t s, const DMesh& m, DState st, int bdf2);
 void launchCopy(hipStream_t s, double* dst, const double* src, int n, const Ctl* ctl);
 void launchFill(hipStream_t s, double* dst, double value, int n, const Ctl* ctl);

@@ -1150,57 +1150,69 @@ __global__ void __launch_bounds__(kBlock) kResidual(DMesh m, DState s) {
 // One workgroup of kFinishThreads: each thread loads the partials of four
 // blocks in one round (indices clamped, extra loads unused), then each wave
 // reduces by DPP and one thread per quantity combines the waves.
+// SC: the scalars' partials too (8 more per block after the 9 * blocks of
+// momentum and continuity: 9 sum|rT| 10 sum rT^2 11 max|rT| 12 sum|AC_T T|,
+// 13-16 the same of the concentration).
 constexpr int kFinishThreads = 1024;
+__device__ __forceinline__ constexpr bool isMaxQuantity(int j) { return j == 2 || j == 6 || j == 11 || j == 15; }
+template <bool SC>
 __global__ void __launch_bounds__(kFinishThreads) kFinish(DState s, int blocks) {
+    constexpr int Q = SC ? 17 : 9;
     Ctl* c = s.ctl;
     // the whole control block in one round of loads; thread 0 works on this
     // copy and stores back the fields it changes
     Ctl cc = *c;
     if (cc.done) return;
     const bool check = (cc.k % cc.checkConv) == 0;
-    __shared__ double red[kFinishThreads / 64][9];
-    __shared__ double tot[9];
+    __shared__ double red[kFinishThreads / 64][Q];
+    __shared__ double tot[17];
     if (check) {
         const int t = threadIdx.x;
-        double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        double a[Q];
+#pragma unroll
+        for (int j = 0; j < Q; j++) a[j] = 0.0;
+        const double* ps = s.partial + (size_t)blocks * 9;
         for (int b0 = 0; b0 < blocks; b0 += 4 * kFinishThreads) {
-            double x[4][9];
+            double x[4][Q];
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int b = min(b0 + r * kFinishThreads + t, blocks - 1);
 #pragma unroll
                 for (int j = 0; j < 9; j++) x[r][j] = s.partial[(size_t)b * 9 + j];
+#pragma unroll
+                for (int j = 9; j < Q; j++) x[r][j] = ps[(size_t)b * 8 + (j - 9)];
             }
 #pragma unroll
             for (int r = 0; r < 4; r++)
                 if (b0 + r * kFinishThreads + t < blocks)
 #pragma unroll
-                    for (int j = 0; j < 9; j++) a[j] = (j == 2 || j == 6) ? fmax(a[j], x[r][j]) : a[j] + x[r][j];
+                    for (int j = 0; j < Q; j++) a[j] = isMaxQuantity(j) ? fmax(a[j], x[r][j]) : a[j] + x[r][j];
         }
         const int lane = t & 63, wave = t >> 6;
 #pragma unroll
-        for (int j = 0; j < 9; j++) {
-            const double w = (j == 2 || j == 6) ? waveScan<true>(a[j]) : waveScan<false>(a[j]);
+        for (int j = 0; j < Q; j++) {
+            const double w = isMaxQuantity(j) ? waveScan<true>(a[j]) : waveScan<false>(a[j]);
             if (lane == 63) red[wave][j] = w;
         }
         __syncthreads();
-        if (t < 9) {
+        if (t < Q) {
             double v = red[0][t];
-            for (int w = 1; w < kFinishThreads / 64; w++) v = (t == 2 || t == 6) ? fmax(v, red[w][t]) : v + red[w][t];
+            for (int w = 1; w < kFinishThreads / 64; w++) v = isMaxQuantity(t) ? fmax(v, red[w][t]) : v + red[w][t];
             tot[t] = v;
         }
+        if (!SC && t >= 9 && t < 17) tot[t] = 0.0;
         __syncthreads();
     }
     if (threadIdx.x != 0) return;
     if (check) {
         const int scaleIter = cc.transient ? cc.k : cc.iter;
-        for (int i = 0; i < 3; i++) {
+        for (int i = 0; i < 5; i++) {
             if (!cc.enabled[i]) continue;
             double res, scale = cc.scaleVal[i];
             if (i == 2) {
                 res = tot[8];
             } else {
-                const int o = i * 4;
+                const int o = i < 2 ? i * 4 : 9 + (i - 3) * 4;
                 res = cc.norm[i] == 0 ? tot[o] : cc.norm[i] == 1 ? tot[o + 1] : tot[o + 2];
                 switch (cc.scaling[i]) {
                 case 0: scale = 1.0; break;
@@ -1219,12 +1231,12 @@ __global__ void __launch_bounds__(kFinishThreads) kFinish(DState s, int blocks) 
             cc.resVal[i] = res;
         }
         if (cc.nHist < cc.histCap) {
-            for (int i = 0; i < 3; i++) s.hist[cc.nHist * 4 + i] = cc.resVal[i];
-            s.hist[cc.nHist * 4 + 3] = (double)cc.iter;
+            for (int i = 0; i < 5; i++) s.hist[cc.nHist * kHistStride + i] = cc.resVal[i];
+            s.hist[cc.nHist * kHistStride + 5] = (double)cc.iter;
             cc.nHist++;
         }
         bool any = false, ok = true;
-        for (int i = 0; i < 3; i++) {
+        for (int i = 0; i < 5; i++) {
             if (!cc.enabled[i]) continue;
             any = true;
             if (!(cc.resVal[i] <= cc.tol[i])) ok = false;
@@ -1234,7 +1246,7 @@ __global__ void __launch_bounds__(kFinishThreads) kFinish(DState s, int blocks) 
     cc.iter++;
     cc.k++;
     if (cc.converged || cc.k >= cc.maxIter) cc.done = 1;
-    for (int i = 0; i < 3; i++) {
+    for (int i = 0; i < 5; i++) {
         c->resVal[i] = cc.resVal[i];
         c->scaleVal[i] = cc.scaleVal[i];
     }
@@ -1257,6 +1269,251 @@ __global__ void __launch_bounds__(kBlock) kStepBegin(DMesh m, DState s, int bdf2
         }
         s.uOld[n] = s.u[n];
         s.vOld[n] = s.v[n];
+    }
+}
+
+// ===========================================================================
+// Scalar transport (energy, concentration): one pass per cell doing, in the
+// reference's order, for one field,
+//   addTransportCoefficients     solver_util.cu:218-448 (+ the wall kinetics
+//                                of the Michaelis-Menten / Hill faces)
+//   addTransientCoefficient      solver_util.cu:536-578 (capacity 1)
+//   underRelaxEquation           solver_util.cu:582-600
+//   buildInverseDiagonal         solver_util.cu:602-612
+// (solver.cpp:1222-1235).  The system is the field's own (its diffusivity and
+// the volumetric flux differ from the momentum's).
+// ===========================================================================
+namespace {
+
+// concentration_equation.cu:6-88: rate laws of group g and their derivatives
+struct WallKinetics {
+    double vmax, km, n, m, v2, kmN, k2M;
+    __device__ WallKinetics(const DBC& bc, int g)
+        : vmax(bc.vmax[g]), km(bc.km[g]), n(bc.hillN[g]), m(bc.inhibM[g]), v2(bc.v2[g]), kmN(bc.kmN[g]),
+          k2M(bc.k2M[g]) {}
+    __device__ double mm(double c) const { return (vmax * c) / (km + c); }
+    __device__ double dmm(double c) const {
+        const double inv = 1.0 / (km + c);
+        return vmax * km * inv * inv;
+    }
+    __device__ double inhib(double c) const {
+        if (v2 == 0.0) return 1.0;
+        const double cm = pow(c, m);
+        return (1.0 - (v2 * cm / (k2M + cm)));
+    }
+    __device__ double dinhib(double c) const {
+        if (v2 == 0.0) return 0.0;
+        const double cm = pow(c, m), cm1 = pow(c, m - 1.0);
+        return (-(m * v2 * k2M) * (cm1) / ((k2M + cm) * (k2M + cm)));
+    }
+    __device__ double hill(double c) const {
+        const double cn = pow(c, n);
+        return (vmax * cn) / (kmN + cn);
+    }
+    __device__ double dhill(double c) const {
+        const double cn = pow(c, n), cn1 = pow(c, n - 1.0);
+        return ((n * vmax * kmN) * (cn1) / ((kmN + cn) * (kmN + cn)));
+    }
+};
+
+// wallConcentrationMichaelisMenten / Hill, concentration_equation.cu:91-160:
+// Newton on h (c_P - c_w) = J(c_w) from c_w = c_P, at most 100 steps
+__device__ double wallNewton(const WallKinetics& k, bool hill, double cp, double h) {
+    double cw = cp;
+#pragma unroll 1
+    for (int i = 0; i < 100; i++) {
+        const double rate = hill ? k.hill(cw) : k.mm(cw);
+        const double inh = k.inhib(cw);
+        const double J = rate * inh;
+        const double dJ = rate * k.dinhib(cw) + (hill ? k.dhill(cw) : k.dmm(cw)) * inh;
+        const double F = h * (cp - cw) - J;
+        const double dF = -h - dJ;
+        const double step = F / dF;
+        cw -= step;
+        cw = fmax(0.0, cw);
+        if (fabs(step) < 1e-14 + 1e-10 * fabs(cw)) break;
+    }
+    return cw;
+}
+
+// boundary half of addTransportCoefficients for a scalar: boundaryTransport's
+// types plus the reactive walls (solver_util.cu:383-412)
+__device__ void boundaryTransportScalar(const DMesh& m, const DBC& bc, DWall wall, int n, int f, double area,
+                                        double gamma, double F, double phiP, double& AC, double& b) {
+    const int g = m.group[f];
+    if (g >= 0 && g < bc.nGroups && (bc.type[g] == 3 || bc.type[g] == 4)) {
+        const double dPF = fabs((m.fZ[f] - m.cZ[n]) * m.nZ[f] + (m.fR[f] - m.cR[n]) * m.nR[f]);
+        const double Rtot = (dPF / gamma) + bc.rtot[g];
+        const double h = 1 / Rtot;
+        const WallKinetics kin(bc, g);
+        const bool hill = bc.type[g] == 4;
+        const double cw = wallNewton(kin, hill, phiP, h);
+        wall.cw[f] = cw;
+        wall.ocr[f] = area * (hill ? kin.hill(cw) : kin.mm(cw)) * kin.inhib(cw);
+        AC += area * h;
+        b += area * h * cw;
+        return;  // kinetics walls add no convection (addConvectionContribution, solver_util.cuh:817-864)
+    }
+    boundaryTransport(m, bc, n, f, area, gamma, F, AC, b);
+}
+
+}  // namespace
+
+template <int B, bool HO>
+__global__ void __launch_bounds__(kBlock) kScalar(DMesh m, DBC bc, DScalar sc, DWall wall, const double* mDot,
+                                                  ScalarPhysics ph, const Ctl* ctl) {
+    if (finished(ctl)) return;
+    const int n = xcdBlock() * blockDim.x + threadIdx.x;
+    if (n >= m.N) return;
+    const double gamma = ph.gamma;
+    double ac = 0.0, b = 0.0, netF = 0.0;
+    int k0, ns;
+    rowRange(m.faceStart, m.uniformSlots, n, k0, ns);
+    const double pN = sc.phi[n];
+    const double gzN = ph.nonOrtho ? sc.gZ[n] : 0.0, grN = ph.nonOrtho ? sc.gR[n] : 0.0;
+    auto step = [&](int k, int o, uint8_t fl, double area, double invDPN, double md, double no, double w,
+                    double pO) -> double {
+        double F = 0.0;
+        if (ph.conv) {
+            const double Fo = md * ph.fluxScale;
+            F = (fl & 1) ? Fo : -Fo;
+            if (fabs(F) <= 1.0e-30) F = 0.0;
+            netF += F;
+        }
+        if (o >= 0) {
+            const double K = gamma * area * invDPN;
+            double af = 0.0;
+            ac += K;
+            af -= K;
+            if (ph.nonOrtho) b += no;
+            if (F != 0.0) {
+                ac += fmax(F, 0.0);
+                af += fmin(F, 0.0);
+                if (HO) {
+                    const double ud = (F > 0.0) ? pN : pO;
+                    const double ho = higherOrderVal(m, sc.phi, sc.gZ, sc.gR, ph.scheme, n, o, k, F, ud, w, pN, pO);
+                    b -= F * (ho - ud);
+                }
+            }
+            return af;
+        }
+        boundaryTransportScalar(m, bc, wall, n, m.faceIDs[k], area, gamma, F, pN, ac, b);
+        return 0.0;
+    };
+    if (ns > 0 && ns <= B) {
+        int o[B], fid[B];
+        uint8_t fl[B];
+        double area[B], inv[B], w[B], md[B], po[B], af[B], no[B];
+#pragma unroll
+        for (int q = 0; q < B; q++) {
+            const int k = slotAt(k0, ns, q);
+            o[q] = m.slotNb[k];
+            fid[q] = m.faceIDs[k];
+            fl[q] = m.sFlags[k];
+            area[q] = m.sArea[k];
+            inv[q] = m.sInvDPN[k];
+            w[q] = (HO || ph.nonOrtho) ? m.sW[k] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < B; q++) {
+            const int k = slotAt(k0, ns, q);
+            const int oc = o[q] >= 0 ? o[q] : n;
+            md[q] = ph.conv ? mDot[fid[q]] : 0.0;
+            po[q] = HO ? sc.phi[oc] : 0.0;
+            no[q] = ph.nonOrtho ? nonOrthoVal(m.sTZ[k], m.sTR[k], w[q], gzN, sc.gZ[oc], grN, sc.gR[oc], gamma) : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < B; q++)
+            if (q < ns) af[q] = step(k0 + q, o[q], fl[q], area[q], inv[q], md[q], no[q], w[q], po[q]);
+#pragma unroll
+        for (int q = 0; q < B; q++)
+            if (q < ns) sc.AF[k0 + q] = af[q];
+    } else {
+        for (int k = k0; k < k0 + ns; k++) {
+            const int o = m.slotNb[k];
+            const int oc = o >= 0 ? o : n;
+            const double w = (HO || ph.nonOrtho) ? m.sW[k] : 0.0;
+            const double no = ph.nonOrtho ? nonOrthoVal(m.sTZ[k], m.sTR[k], w, gzN, sc.gZ[oc], grN, sc.gR[oc], gamma)
+                                          : 0.0;
+            sc.AF[k] = step(k, o, m.sFlags[k], m.sArea[k], m.sInvDPN[k], ph.conv ? mDot[m.faceIDs[k]] : 0.0, no, w,
+                            HO ? sc.phi[oc] : 0.0);
+        }
+    }
+    if (ph.conv) ac -= netF;
+    if (ph.transient && ph.dt > 0.0) {
+        const double cV = 1.0 * m.vol[n];
+        if (ctl != nullptr && ctl->bdf2Step) {
+            const double a = cV / (2.0 * ph.dt);
+            ac += 3.0 * a;
+            b += a * (4.0 * sc.old[n] - sc.old2[n]);
+        } else {
+            const double a = cV / ph.dt;
+            ac += a;
+            b += a * sc.old[n];
+        }
+    }
+    const double al = ph.alpha;
+    if (!(fabs(ac) < 1.0e-30)) {
+        const double a0 = ac;
+        ac = a0 / al;
+        b += ((1.0 - al) / al) * a0 * pN;
+    }
+    sc.AC[n] = ac;
+    sc.b[n] = b;
+    sc.inv[n] = (fabs(ac) < 1.0e-30) ? 0.0 : 1.0 / ac;
+}
+
+// residualRaw (residuals.cuh:17-48) of the solved scalars' systems and the first
+// pass of their norms: per workgroup 0 sum|r| 1 sum r^2 2 max|r| 3 sum|AC x|, of
+// the temperature, then 4-7 of the concentration (zero for a field not solved)
+template <int B>
+__global__ void __launch_bounds__(kBlock) kResidualScalars(DMesh m, DScalar t, DScalar c, double* partial,
+                                                           const Ctl* ctl) {
+    if (finished(ctl)) return;
+    if (ctl->k % ctl->checkConv != 0) return;
+    const int n = xcdBlock() * blockDim.x + threadIdx.x;
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (n < m.N) {
+        int k0, ns;
+        rowRange(m.faceStart, m.uniformSlots, n, k0, ns);
+        const DScalar* sys[2] = {&t, &c};
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            const DScalar& x = *sys[i];
+            if (!x.phi) continue;
+            double ax = x.AC[n] * x.phi[n];
+            v[i * 4 + 3] = fabs(ax);
+            for (int k = k0; k < k0 + ns; k++) {
+                const int o = m.slotNb[k];
+                if (o >= 0) ax += x.AF[k] * x.phi[o];
+            }
+            const double r = fabs(x.b[n] - ax);
+            v[i * 4 + 0] = r;
+            v[i * 4 + 1] = r * r;
+            v[i * 4 + 2] = r;
+        }
+    }
+    __shared__ double red[kBlock / 64][8];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int j = 0; j < 8; j++) {
+        const double w = (j == 2 || j == 6) ? waveScan<true>(v[j]) : waveScan<false>(v[j]);
+        if (lane == 63) red[wave][j] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x < 8) {
+        const int j = threadIdx.x;
+        double a = red[0][j];
+        for (int w = 1; w < kBlock / 64; w++) a = (j == 2 || j == 6) ? fmax(a, red[w][j]) : a + red[w][j];
+        partial[xcdBlock() * 8 + j] = a;
+    }
+}
+
+// Old levels of a scalar at the start of a time step (solver.cpp:1058-1095).
+__global__ void __launch_bounds__(kBlock) kRotate(int n, double* old2, double* old, const double* phi, int bdf2) {
+    const int i = xcdBlock() * blockDim.x + threadIdx.x;
+    if (i < n) {
+        if (bdf2) old2[i] = old[i];
+        old[i] = phi[i];
     }
 }
 
@@ -2481,7 +2738,30 @@ void launchResidual(hipStream_t s, const DMesh& m, DState st, int blocks) {
     if (m.maxSlots <= 3) kResidual<3><<<blocks, kBlock, 0, s>>>(m, st);
     else kResidual<4><<<blocks, kBlock, 0, s>>>(m, st);
 }
-void launchFinish(hipStream_t s, DState st, int blocks) { kFinish<<<1, kFinishThreads, 0, s>>>(st, blocks); }
+void launchFinish(hipStream_t s, DState st, int blocks, bool scalars) {
+    if (scalars) kFinish<true><<<1, kFinishThreads, 0, s>>>(st, blocks);
+    else kFinish<false><<<1, kFinishThreads, 0, s>>>(st, blocks);
+}
+void launchResidualScalars(hipStream_t s, const DMesh& m, DState st, DScalar t, DScalar c, int blocks) {
+    double* part = st.partial + (size_t)blocks * 9;
+    if (m.maxSlots <= 3) kResidualScalars<3><<<blocks, kBlock, 0, s>>>(m, t, c, part, st.ctl);
+    else kResidualScalars<4><<<blocks, kBlock, 0, s>>>(m, t, c, part, st.ctl);
+}
+void launchScalar(hipStream_t s, const DMesh& m, DBC bc, DScalar sc, DWall wall, const double* mDot, ScalarPhysics ph,
+                  const Ctl* ctl) {
+    const int blocks = blocksFor(m.N);
+    const bool ho = ph.conv && ph.scheme != 0;
+    if (m.maxSlots <= 3) {
+        if (ho) kScalar<3, true><<<blocks, kBlock, 0, s>>>(m, bc, sc, wall, mDot, ph, ctl);
+        else kScalar<3, false><<<blocks, kBlock, 0, s>>>(m, bc, sc, wall, mDot, ph, ctl);
+    } else {
+        if (ho) kScalar<4, true><<<blocks, kBlock, 0, s>>>(m, bc, sc, wall, mDot, ph, ctl);
+        else kScalar<4, false><<<blocks, kBlock, 0, s>>>(m, bc, sc, wall, mDot, ph, ctl);
+    }
+}
+void launchRotate(hipStream_t s, int n, double* old2, double* old, const double* phi, int bdf2) {
+    kRotate<<<blocksFor(n), kBlock, 0, s>>>(n, old2, old, phi, bdf2);
+}
 void launchStepBegin(hipStream_t s, const DMesh& m, DState st, int bdf2) {
     kStepBegin<<<blocksFor(m.N), kBlock, 0, s>>>(m, st, bdf2);
 }

@@ -69,7 +69,7 @@ void axs_config_defaults(axs_config* c) {
     c->rho = 998.0;                           // src/header/setting.cuh:63
     c->mu = 0.0010518;                        // src/header/setting.cuh:64
     for (int i = 0; i < AXS_RES_COUNT; i++) { // initConfigResiduals, solver.cpp:87-101
-        c->residual[i].enabled = 1;
+        c->residual[i].enabled = i < AXS_RES_TEMPERATURE;  // the scalars' only when switched on
         c->residual[i].norm = AXS_NORM_L1;
         c->residual[i].scaling = AXS_SCALE_DIAGONAL;
         c->residual[i].tol = 0.001;
@@ -81,6 +81,11 @@ void axs_config_defaults(axs_config* c) {
     c->coarse_direct = 0;
     c->save_key_frame_iter = 2;               // solver_struct.h:230
     c->max_time_frames = 500;                 // solver/header/solver.h:108
+    c->solve_energy = 0;                      // solver_struct.h:10
+    c->solve_concentration = 0;               // solver_struct.h:11
+    c->cp = 4180.0;                           // src/header/setting.cuh:66
+    c->k = 0.6;                               // :67
+    c->diffusivity = 3.0277e-9;               // :69
 }
 
 axs_status axs_create(axs_solver** out, int device) {
@@ -196,11 +201,18 @@ axs_status axs_get_frame(axs_solver* h, int index, int field, double* time, doub
         if (time) *time = f.time;
         if (!out) return;
         const std::vector<double>* src = field == AXS_FIELD_U ? &f.u : field == AXS_FIELD_V ? &f.v
-                                       : field == AXS_FIELD_P ? &f.p : nullptr;
-        if (!src) throw std::invalid_argument("frames hold u, v and p");
+                                       : field == AXS_FIELD_P ? &f.p : field == AXS_FIELD_T ? &f.t
+                                       : field == AXS_FIELD_C ? &f.c : nullptr;
+        if (!src) throw std::invalid_argument("frames hold u, v, p, T and C");
+        if (src->empty()) throw std::logic_error("that field was not solved");
         if (n != src->size()) throw std::out_of_range("output length does not match the frame");
         std::copy(src->begin(), src->end(), out);
     });
+}
+
+axs_status axs_get_wall_diagnostic(axs_solver* h, double out[10]) {
+    if (!h || !out) return AXS_ERR_INVALID;
+    return guarded(&h->s->lastError, [&] { h->s->wallDiagnostic(out); });
 }
 
 // ---------------- multigrid, stand-alone ----------------

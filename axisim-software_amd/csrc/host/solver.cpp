@@ -903,21 +903,33 @@ void Solver::setMesh(const axs_mesh_view& v) {
 
 void Solver::setBC(int var, const axs_bc_view& v) {
     if (running_) throw std::logic_error("solver is running");
-    if (var < 0 || var > 2) throw std::invalid_argument("bad variable");
+    if (var < 0 || var >= AXS_VAR_COUNT) throw std::invalid_argument("bad variable");
     if (v.n_groups < 0 || (v.n_groups > 0 && (!v.type || !v.value))) throw std::invalid_argument("bad bc view");
     HostBC h;
     h.nGroups = v.n_groups;
     for (int g = 0; g < v.n_groups; g++) {
         const uint8_t t = v.type[g];
         if (t > AXS_BC_PULSATILE) throw std::invalid_argument("bad bc type");
-        if (t == AXS_BC_MICHAELIS_MENTEN || t == AXS_BC_HILL)
-            throw std::domain_error("reactive-wall kinetics are not part of the momentum / pressure path");
+        if ((t == AXS_BC_MICHAELIS_MENTEN || t == AXS_BC_HILL) && var < AXS_VAR_T)
+            throw std::domain_error("reactive-wall kinetics apply to temperature and concentration only");
+        if (t == AXS_BC_MICHAELIS_MENTEN || t == AXS_BC_HILL) h.kinetics = true;
         h.pulsatile.push_back(t == AXS_BC_PULSATILE);
         h.type.push_back(t == AXS_BC_PULSATILE ? (uint8_t)AXS_BC_DIRICHLET : t);  // memory_manger.cpp:96-100
         h.value.push_back(v.value[g]);
         h.length.push_back(v.length ? v.length[g] : 0.0);
         h.amplitude.push_back(v.amplitude ? v.amplitude[g] : 0.0);
         h.frequency.push_back(v.frequency ? v.frequency[g] : 0.0);
+        // createBoundaryFieldHost defaults: rates 0, exponents 1 (memory_manger.cpp:70-82)
+        h.vmax.push_back(v.vmax ? v.vmax[g] : 0.0);
+        h.km.push_back(v.km ? v.km[g] : 0.0);
+        h.hillN.push_back(v.hill_n ? v.hill_n[g] : 1.0);
+        h.inhibM.push_back(v.inhib_m ? v.inhib_m[g] : 1.0);
+        h.k2.push_back(v.inhib_k2 ? v.inhib_k2[g] : 0.0);
+        h.v2.push_back(v.inhib_v2 ? v.inhib_v2[g] : 0.0);
+        h.rtot.push_back(v.wall_resistance ? v.wall_resistance[g] : 0.0);
+        // Km^n and K2^m once per group (createBoundaryFieldDevice, memory_manger.cpp:175-184)
+        h.kmN.push_back(std::pow(h.km.back(), h.hillN.back()));
+        h.k2M.push_back(std::pow(h.k2.back(), h.inhibM.back()));
     }
     bc_[var] = std::move(h);
 }
@@ -1019,11 +1031,28 @@ void Solver::allocateState() {
     s.uvA = stateArena_.zeros<double2>(N);
     s.uvB = stateArena_.zeros<double2>(N);
     partialBlocks_ = blocksFor(N);
-    s.partial = stateArena_.zeros<double>((size_t)partialBlocks_ * 9);
+    s.partial = stateArena_.zeros<double>((size_t)partialBlocks_ * (9 + 8));
     s.ctl = stateArena_.zeros<Ctl>(1);
     s.hist = nullptr;
     histCap_ = 0;
     ds_ = s;
+    // temperature / concentration (allocateSimple allocates the solved ones,
+    // memory_manger.cpp:691-773)
+    const bool want[2] = {cfg_.solve_energy != 0, cfg_.solve_concentration != 0};
+    for (int i = 0; i < 2; i++) {
+        sc_[i] = DScalar{};
+        scAlloc_[i] = want[i];
+        if (!want[i]) continue;
+        DScalar& x = sc_[i];
+        for (double** p : {&x.phi, &x.tmp, &x.old, &x.old2, &x.gZ, &x.gR, &x.AC, &x.b, &x.inv})
+            *p = stateArena_.zeros<double>(N);
+        x.AF = stateArena_.zeros<double>(S);
+    }
+    wall_ = DWall{};
+    if (want[0] || want[1]) {
+        wall_.cw = stateArena_.zeros<double>(F);
+        wall_.ocr = stateArena_.zeros<double>(F);
+    }
     std::memset(hctl_, 0, sizeof(Ctl));
     stateAllocated_ = true;
 }
@@ -1054,7 +1083,7 @@ void Solver::uploadBCs(double time) {
     };
     if (bcValueDev_[0] == nullptr) {
         bcArena_.release();
-        for (int var = 0; var < 3; var++) {
+        for (int var = 0; var < AXS_VAR_COUNT; var++) {
             const HostBC& h = bc_[var];
             DBC d{};
             d.nGroups = h.nGroups;
@@ -1062,6 +1091,16 @@ void Solver::uploadBCs(double time) {
             bcValueDev_[var] = bcArena_.upload(values(h));
             d.value = bcValueDev_[var];
             d.length = bcArena_.upload(h.length);
+            if (h.kinetics) {
+                d.vmax = bcArena_.upload(h.vmax);
+                d.km = bcArena_.upload(h.km);
+                d.hillN = bcArena_.upload(h.hillN);
+                d.inhibM = bcArena_.upload(h.inhibM);
+                d.v2 = bcArena_.upload(h.v2);
+                d.rtot = bcArena_.upload(h.rtot);
+                d.kmN = bcArena_.upload(h.kmN);
+                d.k2M = bcArena_.upload(h.k2M);
+            }
             dbc_[var] = d;
         }
         // ppBC: the p types with every value zeroed (solver.cpp:788-795)
@@ -1195,8 +1234,56 @@ void Solver::enqueueIteration(hipStream_t s) {
     }
     launchGradPPUpdate(s, dm_, ppbc_, st, ph);
     launchMassFluxUpdate(s, dm_, bp, st, ph);
+    if (scalarsOn()) enqueueScalars(s);
     launchResidual(s, dm_, st, partialBlocks_);
-    launchFinish(s, st, partialBlocks_);
+    if (scalarsOn()) launchResidualScalars(s, dm_, st, sc_[0], sc_[1], partialBlocks_);
+    launchFinish(s, st, partialBlocks_, scalarsOn());
+}
+
+// The scalar equation of field `which` (0 temperature, 1 concentration):
+// diffusivity k / (rho cp) or D, the volumetric flux, under-relaxation with
+// alpha_u for temperature only (solver.cpp:912, 993-1009)
+ScalarPhysics Solver::scalarPhysics(int which) const {
+    const Physics ph = physics();
+    ScalarPhysics sp{};
+    sp.gamma = which == 0 ? cfg_.k / (cfg_.rho * cfg_.cp) : cfg_.diffusivity;
+    sp.fluxScale = 1.0 / cfg_.rho;
+    sp.alpha = which == 0 ? cfg_.alpha_u : 1.0;
+    sp.dt = ph.dt;
+    sp.conv = ph.conv;
+    sp.scheme = ph.scheme;
+    sp.nonOrtho = ph.nonOrtho;
+    sp.transient = ph.transient;
+    return sp;
+}
+
+// Energy, then concentration, after the mass-flux update (solver.cpp:1222-1237)
+void Solver::enqueueScalars(hipStream_t s) {
+    const axs_config& c = cfg_;
+    const bool needGrad = c.add_convection && (c.convection_scheme == AXS_CONV_SOU || c.convection_scheme == AXS_CONV_QUICK);
+    for (int i = 0; i < 2; i++) {
+        if (!scAlloc_[i]) continue;
+        const DScalar& x = sc_[i];
+        const DBC& bc = dbc_[AXS_VAR_T + i];
+        const ScalarPhysics sp = scalarPhysics(i);
+        if (sp.nonOrtho || needGrad) launchGradField(s, dm_, bc, x.phi, x.gZ, x.gR, c.gradient_scheme, ds_.ctl);
+        launchScalar(s, dm_, bc, x, wall_, ds_.mDot, sp, ds_.ctl);
+        // the field's linear solver (LinearSolver::run, linear_solver.cu:91-148)
+        if (c.linear_solver == AXS_LINEAR_GS_RB && nColors_ > 0) {
+            for (int it = 0; it < c.linear_max_iter; it++)
+                for (int col = 0; col < nColors_; col++) {
+                    const int begin = colorStart_[col], count = colorStart_[col + 1] - begin;
+                    if (count > 0) launchGSColorSys(s, dm_, x.b, x.inv, x.AF, x.phi, dOrder_, begin, count, ds_.ctl);
+                }
+        } else {
+            double *a = x.phi, *t = x.tmp;
+            for (int it = 0; it < c.linear_max_iter; it++) {
+                launchJacobiSys(s, dm_, x.b, x.inv, x.AF, a, t, ds_.ctl);
+                std::swap(a, t);
+            }
+            if (a != x.phi) launchCopy(s, x.phi, a, dm_.N, ds_.ctl);
+        }
+    }
 }
 
 void Solver::dropGraph() {
@@ -1239,13 +1326,13 @@ void Solver::pullControl(axs_progress_cb cb, void* user) {
     const int after = hctl_->nHist;
     if (after > before) {
         const int cnt = after - before;
-        AXS_HIP(hipMemcpy(hhist_, ds_.hist + (size_t)before * 4, (size_t)cnt * 4 * sizeof(double),
+        AXS_HIP(hipMemcpy(hhist_, ds_.hist + (size_t)before * kHistStride, (size_t)cnt * kHistStride * sizeof(double),
                           hipMemcpyDeviceToHost));
-        static const char* names[3] = {"U", "V", "Continuity"};
+        static const char* names[AXS_RES_COUNT] = {"U", "V", "Continuity", "Temperature", "Concentration"};
         for (int i = 0; i < cnt; i++) {
-            const double* r = hhist_ + (size_t)i * 4;
-            for (int j = 0; j < 3; j++) history_[j].push_back(r[j]);
-            if (cb) cb(user, (int)r[3], 3, names, r);
+            const double* r = hhist_ + (size_t)i * kHistStride;
+            for (int j = 0; j < AXS_RES_COUNT; j++) history_[j].push_back(r[j]);
+            if (cb) cb(user, (int)r[kHistStride - 1], AXS_RES_COUNT, names, r);
         }
     }
 }
@@ -1271,6 +1358,11 @@ void Solver::run(axs_progress_cb cb, void* user, bool continueRun) {
         throw std::invalid_argument("Transient run needs a positive time step and end time.");
     if (!std::isfinite(c.rho) || !std::isfinite(c.mu) || c.rho <= 0.0 || c.mu <= 0.0)
         throw std::invalid_argument("Solver needs positive density and viscosity.");
+    // the fluid properties the scalars read (file_manager.cpp:287-291 rejects the same)
+    if (c.solve_energy && (!std::isfinite(c.cp) || c.cp <= 0.0 || !std::isfinite(c.k) || c.k < 0.0))
+        throw std::invalid_argument("Energy equation needs a positive heat capacity and a non-negative conductivity.");
+    if (c.solve_concentration && (!std::isfinite(c.diffusivity) || c.diffusivity < 0.0))
+        throw std::invalid_argument("Concentration equation needs a non-negative diffusion coefficient.");
     for (int v = 0; v < 2; v++)
         for (int g = 0; g < bc_[v].nGroups; g++)
             if (bc_[v].pulsatile[g]) {
@@ -1324,12 +1416,12 @@ void Solver::run(axs_progress_cb cb, void* user, bool continueRun) {
     // residual history buffer for this run
     const int cap = c.max_iter + 1;
     if (cap > histCap_) {
-        ds_.hist = stateArena_.zeros<double>((size_t)cap * 4);
+        ds_.hist = stateArena_.zeros<double>((size_t)cap * kHistStride);
         histCap_ = cap;
     }
     if (cap > hhistCap_) {
         if (hhist_) AXS_HIP(hipHostFree(hhist_));
-        AXS_HIP(hipHostMalloc((void**)&hhist_, (size_t)cap * 4 * sizeof(double), hipHostMallocDefault));
+        AXS_HIP(hipHostMalloc((void**)&hhist_, (size_t)cap * kHistStride * sizeof(double), hipHostMallocDefault));
         hhistCap_ = cap;
     }
     buildGraph();
@@ -1349,7 +1441,7 @@ void Solver::run(axs_progress_cb cb, void* user, bool continueRun) {
     k.histCap = histCap_;
     k.converged = 0;
     k.N = mesh_.nCells;
-    for (int i = 0; i < 3; i++) {
+    for (int i = 0; i < AXS_RES_COUNT; i++) {
         k.enabled[i] = c.residual[i].enabled;
         k.norm[i] = c.residual[i].norm;
         k.scaling[i] = c.residual[i].scaling;
@@ -1393,6 +1485,8 @@ void Solver::run(axs_progress_cb cb, void* user, bool continueRun) {
             // implicit at t_(n+1) (solver.cpp:1036-1056)
             uploadBCs(timeOffset + (double)(t + 1) * c.dt);
             launchStepBegin(stream_, dm_, ds_, bdf2 ? 1 : 0);
+            for (int i = 0; i < 2; i++)
+                if (scAlloc_[i]) launchRotate(stream_, dm_.N, sc_[i].old2, sc_[i].old, sc_[i].phi, bdf2 ? 1 : 0);
             AXS_HIP(hipMemcpyAsync(hctl_, ds_.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream_));
             AXS_HIP(hipStreamSynchronize(stream_));
             hctl_->done = 0;
@@ -1413,6 +1507,10 @@ void Solver::run(axs_progress_cb cb, void* user, bool continueRun) {
     launchGradField(stream_, dm_, dbc_[AXS_VAR_U], ds_.u, ds_.gUZ, ds_.gUR, c.gradient_scheme, nullptr);
     launchGradField(stream_, dm_, dbc_[AXS_VAR_V], ds_.v, ds_.gVZ, ds_.gVR, c.gradient_scheme, nullptr);
     launchGradField(stream_, dm_, dbc_[AXS_VAR_P], ds_.p, ds_.gPZ, ds_.gPR, c.gradient_scheme, nullptr);
+    for (int i = 0; i < 2; i++)  // solver.cpp:1338-1340
+        if (scAlloc_[i])
+            launchGradField(stream_, dm_, dbc_[AXS_VAR_T + i], sc_[i].phi, sc_[i].gZ, sc_[i].gR, c.gradient_scheme,
+                            nullptr);
     AXS_HIP(hipStreamSynchronize(stream_));
     AXS_HIP(hipGetLastError());
     solveMs_ = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
@@ -1432,7 +1530,7 @@ void Solver::runTimed(int n, double* ms) {
     hctl_->maxIter = n;
     hctl_->converged = 0;
     hctl_->nHist = 0;
-    for (int i = 0; i < 3; i++) hctl_->enabled[i] = 0;
+    for (int i = 0; i < AXS_RES_COUNT; i++) hctl_->enabled[i] = 0;
     AXS_HIP(hipMemcpy(ds_.ctl, hctl_, sizeof(Ctl), hipMemcpyHostToDevice));
     hipEvent_t e0, e1;
     AXS_HIP(hipEventCreate(&e0));
@@ -1554,11 +1652,50 @@ void Solver::captureFrame(double time) {
     const size_t N = mesh_.nCells;
     Frame f;
     f.time = time;
-    for (auto [dst, src] : {std::pair<std::vector<double>*, const double*>{&f.u, ds_.u}, {&f.v, ds_.v}, {&f.p, ds_.p}}) {
+    for (auto [dst, src] : {std::pair<std::vector<double>*, const double*>{&f.u, ds_.u}, {&f.v, ds_.v}, {&f.p, ds_.p},
+                            {&f.t, sc_[0].phi}, {&f.c, sc_[1].phi}}) {
+        if (!src) continue;  // a scalar not solved
         dst->resize(N);
         AXS_HIP(hipMemcpy(dst->data(), src, N * sizeof(double), hipMemcpyDeviceToHost));
     }
     frames_.push_back(std::move(f));
+}
+
+// wallConsumptionDiagnostic, solver_util.cu:480-531, on host copies of the
+// concentration, the wall state and the mass flux (summed in face order).
+void Solver::wallDiagnostic(double out[10]) {
+    for (int i = 0; i < 10; i++) out[i] = 0.0;
+    if (!scAlloc_[1]) throw std::logic_error("concentration is not solved");
+    AXS_HIP(hipSetDevice(device_));
+    AXS_HIP(hipStreamSynchronize(stream_));
+    const int N = mesh_.nCells, F = mesh_.nFaces;
+    std::vector<double> conc(N), cw(F), ocr(F), md(F);
+    AXS_HIP(hipMemcpy(conc.data(), sc_[1].phi, N * sizeof(double), hipMemcpyDeviceToHost));
+    AXS_HIP(hipMemcpy(cw.data(), wall_.cw, F * sizeof(double), hipMemcpyDeviceToHost));
+    AXS_HIP(hipMemcpy(ocr.data(), wall_.ocr, F * sizeof(double), hipMemcpyDeviceToHost));
+    AXS_HIP(hipMemcpy(md.data(), ds_.mDot, F * sizeof(double), hipMemcpyDeviceToHost));
+    const HostBC& bc = bc_[AXS_VAR_C];
+    for (int f = 0; f < F; f++) {
+        if (mesh_.faceNeighbor[f] >= 0) continue;
+        const int ow = mesh_.faceOwner[f], g = mesh_.faceGroup[f];
+        if (g < 0 || g >= bc.nGroups) continue;
+        const uint8_t t = bc.type[g];
+        if (t == AXS_BC_DIRICHLET && md[f] < 0.0) out[1] += -md[f] * bc.value[g];
+        if (!(t == AXS_BC_MICHAELIS_MENTEN || t == AXS_BC_HILL)) continue;
+        const double dPF = std::fabs((mesh_.faceCenterZ[f] - mesh_.cellCenterZ[ow]) * mesh_.faceNormalZ[f] +
+                                     (mesh_.faceCenterR[f] - mesh_.cellCenterR[ow]) * mesh_.faceNormalR[f]);
+        const double h = 1.0 / (dPF / cfg_.diffusivity + bc.rtot[g]);
+        const double area = mesh_.faceArea[f], cp = conc[ow];
+        out[0] += ocr[f];
+        out[2] += area * h * cp;
+        out[3] += 1.0;
+        out[4] += cw[f];
+        out[5] += dPF;
+        out[6] += h;
+        out[7] += cp;
+        out[8] += bc.vmax[g];
+        out[9] += bc.km[g];
+    }
 }
 
 const Solver::Frame& Solver::frame(int index) const {
@@ -1587,6 +1724,14 @@ bool Solver::canContinue(std::string& reason) const {  // solver.cpp:469-528
         reason = "Mesh connectivity changed.";
         return false;
     }
+    if ((cfg_.solve_energy != 0) != scAlloc_[0]) {  // solver.cpp:486-504
+        reason = "Temperature field selection changed.";
+        return false;
+    }
+    if ((cfg_.solve_concentration != 0) != scAlloc_[1]) {
+        reason = "Concentration field selection changed.";
+        return false;
+    }
     return true;
 }
 
@@ -1598,7 +1743,9 @@ size_t Solver::fieldLength(int field) const {
     if (field == AXS_FIELD_DU || field == AXS_FIELD_DV) return N;
     if (field >= AXS_FIELD_U_AC && field <= AXS_FIELD_PP_AF)
         return ((field - AXS_FIELD_U_AC) % 4 == 3) ? S : N;
-    if (field >= AXS_FIELD_U_OLD && field <= AXS_FIELD_V_OLD2) return N;
+    if (field >= AXS_FIELD_U_OLD && field <= AXS_FIELD_GRAD_CR) return N;
+    if (field >= AXS_FIELD_T_AC && field <= AXS_FIELD_C_OLD2) return (field == AXS_FIELD_T_AF || field == AXS_FIELD_C_AF) ? S : N;
+    if (field == AXS_FIELD_WALL_CW || field == AXS_FIELD_WALL_OCR) return F;
     switch (field) {
     case AXS_GEOM_INV_A2D:
     case AXS_GEOM_LSQ_INV_ZZ:
@@ -1656,6 +1803,35 @@ void Solver::getField(int field, double* out, size_t n) {
     case AXS_FIELD_V_OLD: down(ds_.vOld); break;
     case AXS_FIELD_U_OLD2: down(ds_.uOld2); break;
     case AXS_FIELD_V_OLD2: down(ds_.vOld2); break;
+    case AXS_FIELD_T: case AXS_FIELD_C: case AXS_FIELD_GRAD_TZ: case AXS_FIELD_GRAD_TR: case AXS_FIELD_GRAD_CZ:
+    case AXS_FIELD_GRAD_CR: case AXS_FIELD_T_AC: case AXS_FIELD_T_B: case AXS_FIELD_T_INVAC: case AXS_FIELD_T_AF:
+    case AXS_FIELD_C_AC: case AXS_FIELD_C_B: case AXS_FIELD_C_INVAC: case AXS_FIELD_C_AF: case AXS_FIELD_T_OLD:
+    case AXS_FIELD_C_OLD: case AXS_FIELD_T_OLD2: case AXS_FIELD_C_OLD2: {
+        const bool isT = field == AXS_FIELD_T || field == AXS_FIELD_GRAD_TZ || field == AXS_FIELD_GRAD_TR ||
+                         (field >= AXS_FIELD_T_AC && field <= AXS_FIELD_T_AF) || field == AXS_FIELD_T_OLD ||
+                         field == AXS_FIELD_T_OLD2;
+        const DScalar& x = sc_[isT ? 0 : 1];
+        if (!scAlloc_[isT ? 0 : 1]) throw std::logic_error(isT ? "temperature is not solved" : "concentration is not solved");
+        const double* src = nullptr;
+        switch (field) {
+        case AXS_FIELD_T: case AXS_FIELD_C: src = x.phi; break;
+        case AXS_FIELD_GRAD_TZ: case AXS_FIELD_GRAD_CZ: src = x.gZ; break;
+        case AXS_FIELD_GRAD_TR: case AXS_FIELD_GRAD_CR: src = x.gR; break;
+        case AXS_FIELD_T_AC: case AXS_FIELD_C_AC: src = x.AC; break;
+        case AXS_FIELD_T_B: case AXS_FIELD_C_B: src = x.b; break;
+        case AXS_FIELD_T_INVAC: case AXS_FIELD_C_INVAC: src = x.inv; break;
+        case AXS_FIELD_T_AF: case AXS_FIELD_C_AF: src = x.AF; break;
+        case AXS_FIELD_T_OLD: case AXS_FIELD_C_OLD: src = x.old; break;
+        default: src = x.old2; break;
+        }
+        down(src);
+        break;
+    }
+    case AXS_FIELD_WALL_CW:
+    case AXS_FIELD_WALL_OCR:
+        if (!wall_.cw) throw std::logic_error("no scalar is solved");
+        down(field == AXS_FIELD_WALL_CW ? wall_.cw : wall_.ocr);
+        break;
     case AXS_FIELD_CONTINUITY: {  // getMassImbalance, solver.cpp:298-331
         std::vector<double> md(mesh_.nFaces);
         AXS_HIP(hipMemcpy(md.data(), ds_.mDot, md.size() * sizeof(double), hipMemcpyDeviceToHost));
@@ -1701,7 +1877,7 @@ void Solver::getField(int field, double* out, size_t n) {
 }
 
 int Solver::residualHistory(int res, double* out, int maxN) const {
-    if (res < 0 || res > 2) throw std::invalid_argument("bad residual index");
+    if (res < 0 || res >= AXS_RES_COUNT) throw std::invalid_argument("bad residual index");
     const auto& h = history_[res];
     const int n = std::min<int>(maxN, (int)h.size());
     if (out) std::copy(h.begin(), h.begin() + n, out);
@@ -1999,7 +2175,7 @@ void Solver::stats(axs_stats& s) const {
     for (int l = 0; l < (int)mg_.host.size() && l < 32; l++) s.mg_level_cells[l] = mg_.host[l].n;
     s.setup_ms = setupMs_;
     s.solve_ms = solveMs_;
-    for (int i = 0; i < 3; i++) s.last_residual[i] = history_[i].empty() ? 0.0 : history_[i].back();
+    for (int i = 0; i < AXS_RES_COUNT; i++) s.last_residual[i] = history_[i].empty() ? 0.0 : history_[i].back();
     s.bytes_per_iteration = haveMesh_ ? bytesPerIteration() : 0.0;
     s.vcycle_bytes = mg_.host.empty() ? 0.0 : mg_.bytesPerSolve(mgParamsOf(cfg_), true);
     s.sweeps_per_launch = fuse_;

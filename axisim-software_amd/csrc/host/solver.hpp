@@ -197,8 +197,10 @@ public:
     // Transient frames (Solver::TimeFrame / timeFrames, solver/header/solver.h:96-108)
     struct Frame {
         double time = 0.0;
-        std::vector<double> u, v, p;
+        std::vector<double> u, v, p, t, c;  // t, c: when those fields are solved
     };
+    // wallConsumptionDiagnostic of the last run (axs_get_wall_diagnostic)
+    void wallDiagnostic(double out[10]);
     int frameCount() const { return (int)frames_.size(); }
     const Frame& frame(int index) const;
     // Learned initial guess of the first p' solve of every outer iteration
@@ -221,6 +223,9 @@ private:
         std::vector<uint8_t> type;        // device type (PULSATILE -> DIRICHLET)
         std::vector<uint8_t> pulsatile;
         std::vector<double> value, length, amplitude, frequency;
+        // kinetics (scalars): Vmax, Km, n, m, K2, V2, wall resistance, Km^n, K2^m
+        std::vector<double> vmax, km, hillN, inhibM, k2, v2, rtot, kmN, k2M;
+        bool kinetics = false;
     };
 
     void uploadMesh();
@@ -241,13 +246,13 @@ private:
     std::vector<int> csrStart_, csrNb_;
     axs_config cfg_{};
     bool haveCfg_ = false;
-    HostBC bc_[3];
+    HostBC bc_[AXS_VAR_COUNT];
 
     Arena meshArena_, stateArena_;
     DMesh dm_{};
     DState ds_{};
-    DBC dbc_[3]{}, ppbc_{};
-    double* bcValueDev_[3] = {nullptr, nullptr, nullptr};
+    DBC dbc_[AXS_VAR_COUNT]{}, ppbc_{};
+    double* bcValueDev_[AXS_VAR_COUNT] = {};
     Arena bcArena_;
     bool meshUploaded_ = false, stateAllocated_ = false;
     int partialBlocks_ = 0;
@@ -295,7 +300,15 @@ private:
     std::atomic<bool> running_{false};
     int currentIteration_ = 0;
     bool converged_ = false, stopped_ = false, ready_ = false;
-    std::vector<double> history_[3];
+    std::vector<double> history_[AXS_RES_COUNT];
+    // scalar transport: temperature and concentration state (allocated when
+    // solved), the reactive walls' per-face state
+    DScalar sc_[2]{};
+    DWall wall_{};
+    bool scAlloc_[2] = {false, false};
+    bool scalarsOn() const { return cfg_.solve_energy || cfg_.solve_concentration; }
+    void enqueueScalars(hipStream_t s);
+    ScalarPhysics scalarPhysics(int which) const;
     double setupMs_ = 0.0, solveMs_ = 0.0;
     std::vector<Frame> frames_;
     void captureFrame(double time);

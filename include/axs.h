@@ -63,8 +63,12 @@ enum { AXS_SCALE_NONE = 0, AXS_SCALE_N = 1, AXS_SCALE_SQRT_N = 2, AXS_SCALE_DIAG
 enum { AXS_PREC_FP64 = 0,      /* reference arithmetic and schedule                  */
        AXS_PREC_MIXED = 1 };   /* FP32 smoothers + V-cycles, FP64 residual/flux/coarse */
 enum { AXS_MG_SMOOTH_JACOBI = 0, AXS_MG_SMOOTH_RBGS = 1 };
-enum { AXS_VAR_U = 0, AXS_VAR_V = 1, AXS_VAR_P = 2 };
-enum { AXS_RES_U = 0, AXS_RES_V = 1, AXS_RES_CONTINUITY = 2, AXS_RES_COUNT = 3 };
+/* Variables with boundary conditions (BoundaryVariable, structs/boundary_struct.h:88-97
+ * order U, V, P, StaticTemperature, Concentration). */
+enum { AXS_VAR_U = 0, AXS_VAR_V = 1, AXS_VAR_P = 2, AXS_VAR_T = 3, AXS_VAR_C = 4, AXS_VAR_COUNT = 5 };
+/* Residuals in the order of Solver::residualPlotType (solver/header/solver.h:35). */
+enum { AXS_RES_U = 0, AXS_RES_V = 1, AXS_RES_CONTINUITY = 2, AXS_RES_TEMPERATURE = 3, AXS_RES_CONCENTRATION = 4,
+       AXS_RES_COUNT = 5 };
 
 /* Face-based mesh, structure of arrays (FVMeshHostPacked, structs/solver_struct.h:400-429).
  * Slot k of cell c is cell_face_ids[cell_face_start[c] + k]. */
@@ -99,6 +103,19 @@ typedef struct axs_bc_view {
     const double* length;          /* group length, fully-developed profile       */
     const double* amplitude;       /* pulsatile only, may be NULL                 */
     const double* frequency;       /* pulsatile only (Hz), may be NULL            */
+    /* Michaelis-Menten / Hill wall kinetics (MichaelisMentenParams, HillParams,
+     * structs/boundary_struct.h:112-113; createBoundaryFieldHost,
+     * manager/source/memory_manger.cpp:51-152): per group Vmax, Km, the Hill
+     * exponent n (1 for Michaelis-Menten), the substrate-inhibition m, K2, V2
+     * (V2 = 0: no inhibition) and the wall layers' total resistance sum d/k.
+     * Each may be NULL (defaults 0, except n = m = 1). */
+    const double* vmax;
+    const double* km;
+    const double* hill_n;
+    const double* inhib_m;
+    const double* inhib_k2;
+    const double* inhib_v2;
+    const double* wall_resistance;
 } axs_bc_view;
 
 typedef struct axs_residual_config {
@@ -135,7 +152,8 @@ typedef struct axs_config {
     double alpha_p;
     double rho;
     double mu;
-    /* ConfigResidual for U, V, Continuity (solver_struct.h:240-263) */
+    /* ConfigResidual for U, V, Continuity, Temperature, Concentration
+     * (solver_struct.h:240-263; initConfigResiduals, solver.cpp:87-101) */
     axs_residual_config residual[AXS_RES_COUNT];
     /* mesh type: a structured mesh forces the non-orthogonal corrector off
      * (Solver::runCheck, solver/source/solver.cpp:553-555) */
@@ -150,6 +168,15 @@ typedef struct axs_config {
      * Solver::maxTimeFrames (solver/header/solver.h:108) */
     int save_key_frame_iter;
     int max_time_frames;
+    /* Scalar transport: SolverFieldOption (structs/solver_struct.h:9-12) and
+     * FluidPropertyConfig cp, k, D (src/header/setting.cuh:61-71).  Energy
+     * diffuses with k / (rho cp), concentration with D; both convect with the
+     * volumetric flux (solver/source/solver.cpp:976-1009). */
+    int solve_energy;
+    int solve_concentration;
+    double cp;
+    double k;
+    double diffusivity;
 } axs_config;
 
 /* Reference defaults (structs/solver_struct.h, src/header/setting.cuh:61-71,
@@ -194,10 +221,20 @@ enum {
     /* transient old levels n and n-1 (VariablesSimple uOld/vOld/uOld2/vOld2,
      * structs/solver_struct.h:265-321) */
     AXS_FIELD_U_OLD = 28, AXS_FIELD_V_OLD = 29, AXS_FIELD_U_OLD2 = 30, AXS_FIELD_V_OLD2 = 31,
+    /* scalar transport (VariablesSimple temp / conc and their gradients) */
+    AXS_FIELD_T = 32, AXS_FIELD_C = 33, AXS_FIELD_GRAD_TZ = 34, AXS_FIELD_GRAD_TR = 35,
+    AXS_FIELD_GRAD_CZ = 36, AXS_FIELD_GRAD_CR = 37,
     /* precomputed geometry (preComputeVariables, memory_manger.cpp:512-617) */
     AXS_GEOM_INV_DPN = 40, AXS_GEOM_WP = 41, AXS_GEOM_DPB = 42, AXS_GEOM_INV_A2D = 43,
     AXS_GEOM_LSQ_WZ = 44, AXS_GEOM_LSQ_WR = 45, AXS_GEOM_LSQ_INV_ZZ = 46,
-    AXS_GEOM_LSQ_INV_ZR = 47, AXS_GEOM_LSQ_INV_RR = 48, AXS_GEOM_LENGTH2D = 49
+    AXS_GEOM_LSQ_INV_ZR = 47, AXS_GEOM_LSQ_INV_RR = 48, AXS_GEOM_LENGTH2D = 49,
+    /* scalar systems and old levels after the last iteration, for parity checks */
+    AXS_FIELD_T_AC = 50, AXS_FIELD_T_B = 51, AXS_FIELD_T_INVAC = 52, AXS_FIELD_T_AF = 53,
+    AXS_FIELD_C_AC = 54, AXS_FIELD_C_B = 55, AXS_FIELD_C_INVAC = 56, AXS_FIELD_C_AF = 57,
+    AXS_FIELD_T_OLD = 58, AXS_FIELD_C_OLD = 59, AXS_FIELD_T_OLD2 = 60, AXS_FIELD_C_OLD2 = 61,
+    /* reactive walls, length n_faces (FVFaceDevice cw / ocrWall): the wall
+     * concentration of the last Newton solve and area * rate there */
+    AXS_FIELD_WALL_CW = 62, AXS_FIELD_WALL_OCR = 63
 };
 axs_status axs_get_field(axs_solver* s, int field, double* out, size_t n);
 /* Number of elements field `field` holds (0 if unknown). */
@@ -216,7 +253,16 @@ axs_status axs_get_residual_history(axs_solver* s, int res, double* out, int max
  * (may be NULL) and copies its field AXS_FIELD_U, V or P into out (NULL: time
  * only; n = cell count). */
 int axs_frame_count(const axs_solver* s);
+/* Frames also hold AXS_FIELD_T / AXS_FIELD_C when those are solved. */
 axs_status axs_get_frame(axs_solver* s, int index, int field, double* time, double* out, size_t n);
+
+/* Reactive-wall consumption diagnostic of the last run's concentration field
+ * (wallConsumptionDiagnostic, solver/source/solver_util.cu:480-531; reported by
+ * solver.cpp:1338-1385): out[0] total wall OCR, [1] substrate carried in through
+ * Dirichlet inflow faces, [2] mass-transfer ceiling sum A h c_P, [3] reactive
+ * face count, then sums over the reactive faces of [4] c_w, [5] d_PF, [6] h,
+ * [7] c_P, [8] Vmax, [9] Km.  Summed in face order. */
+axs_status axs_get_wall_diagnostic(axs_solver* s, double out[10]);
 
 typedef struct axs_stats {
     int iterations;        /* outer iterations done (currentIteration)           */

@@ -57,6 +57,16 @@ struct BC {
     std::vector<uint8_t> type;
     std::vector<double> value, length, amplitude, frequency, mean;
     std::vector<uint8_t> pulsatile;
+    // Michaelis-Menten / Hill kinetics and wall-layer resistance per group, with
+    // Km^n and K2^m evaluated once (createBoundaryFieldHost/Device,
+    // memory_manger.cpp:51-189)
+    std::vector<double> vmax, km, hillN, inhibM, k2, v2, rtot, kmN, k2M;
+};
+
+// Reactive-wall state per face (FVFaceDevice cw / ocrWall): the wall
+// concentration of the last Newton solve and the face's consumption area*J.
+struct Wall {
+    std::vector<double> cw, ocr;
 };
 
 struct Coeff {
@@ -477,6 +487,8 @@ struct Multigrid {
 // ---------------------------------------------------------------------------
 struct Fields {
     std::vector<double> DU, DV, p, pp, u, v;
+    // scalar transport: temperature and concentration, old levels, gradients
+    std::vector<double> temp, conc, tempOld, concOld, tempOld2, concOld2, gradTZ, gradTR, gradCZ, gradCR;
     std::vector<double> uOld, vOld, uOld2, vOld2;
     std::vector<double> gradPZ, gradPR, gradUZ, gradUR, gradVZ, gradVR;
     std::vector<double> mDot;
@@ -642,12 +654,67 @@ double higherOrder(int n, int nb, int f, const Mesh& m, const BC& bc, const std:
     return phiUD;
 }
 
+// ---- wall kinetics, solver/source/concentration_equation.cu:6-160 ----
+// Rate laws of group g at concentration c and their derivatives: Michaelis-
+// Menten Vmax c / (Km + c), Hill Vmax c^n / (Km^n + c^n), and the optional
+// substrate-inhibition factor 1 - V2 c^m / (K2^m + c^m) (V2 = 0: factor 1).
+struct Kinetics {
+    double vmax, km, n, m, v2, kmN, k2M;
+    Kinetics(const BC& bc, int g)
+        : vmax(bc.vmax[g]), km(bc.km[g]), n(bc.hillN[g]), m(bc.inhibM[g]), v2(bc.v2[g]), kmN(bc.kmN[g]),
+          k2M(bc.k2M[g]) {}
+    double mm(double c) const { return (vmax * c) / (km + c); }                        // :6-14
+    double dmm(double c) const {                                                       // :16-26
+        const double inv = 1.0 / (km + c);
+        return vmax * km * inv * inv;
+    }
+    double inhib(double c) const {                                                     // :28-41
+        if (v2 == 0.0) return 1.0;
+        const double cm = std::pow(c, m);
+        return (1.0 - (v2 * cm / (k2M + cm)));
+    }
+    double dinhib(double c) const {                                                    // :43-61
+        if (v2 == 0.0) return 0.0;
+        const double cm = std::pow(c, m), cm1 = std::pow(c, m - 1.0);
+        return (-(m * v2 * k2M) * (cm1) / ((k2M + cm) * (k2M + cm)));
+    }
+    double hill(double c) const {                                                      // :63-74
+        const double cn = std::pow(c, n);
+        return (vmax * cn) / (kmN + cn);
+    }
+    double dhill(double c) const {                                                     // :76-88
+        const double cn = std::pow(c, n), cn1 = std::pow(c, n - 1.0);
+        return ((n * vmax * kmN) * (cn1) / ((kmN + cn) * (kmN + cn)));
+    }
+};
+
+// wallConcentrationMichaelisMenten / wallConcentrationHill (:91-160): Newton
+// on h (c_P - c_w) = J(c_w) from c_w = c_P, c_w kept >= 0, at most 100 steps,
+// stopping once |step| < 1e-14 + 1e-10 |c_w|.
+double wallConcentration(const Kinetics& k, bool hill, double cp, double h) {
+    double cw = cp;
+    for (int i = 0; i < 100; i++) {
+        const double rate = hill ? k.hill(cw) : k.mm(cw);
+        const double inh = k.inhib(cw);
+        const double J = rate * inh;
+        const double dJ = rate * k.dinhib(cw) + (hill ? k.dhill(cw) : k.dmm(cw)) * inh;
+        const double F = h * (cp - cw) - J;
+        const double dF = -h - dJ;
+        const double step = F / dF;
+        cw -= step;
+        cw = std::fmax(0.0, cw);
+        if (std::fabs(step) < 1e-14 + 1e-10 * std::fabs(cw)) break;
+    }
+    return cw;
+}
+
 // addTransportCoefficients, solver_util.cu:218-448 (+ addConvectionContribution,
-// solver_util.cuh:770-865).  Reactive-wall kinetics (MM / Hill) are out of scope.
+// solver_util.cuh:770-865).  wall: the reactive faces' state (null when the
+// BCs have no kinetics type).
 void addTransport(const Mesh& m, const Fields& s, Coeff& co, const BC& bc,
                   const std::vector<double>& phi, const std::vector<double>& gz,
                   const std::vector<double>& gr, int nonOrtho, double gamma, int conv, int scheme,
-                  double fluxScale) {
+                  double fluxScale, Wall* wall = nullptr) {
 #pragma omp parallel for schedule(static)
     for (int n = 0; n < m.N; n++) {
         double netF = 0.0;
@@ -694,6 +761,20 @@ void addTransport(const Mesh& m, const Fields& s, Coeff& co, const BC& bc,
                     } else if (t == AXS_BC_FULLY_DEVELOPED) {
                         co.AC[n] += K;
                         co.b[n] += K * prescribed(m, f, t, v, bc.length[g]);
+                    } else if (t == AXS_BC_MICHAELIS_MENTEN || t == AXS_BC_HILL) {
+                        // reactive wall: conductance h through the near-wall
+                        // half cell and the wall layers, wall value by Newton
+                        const double Rtot = (dPF / gamma) + bc.rtot[g];
+                        const double h = 1 / Rtot;
+                        const Kinetics kin(bc, g);
+                        const bool hill = t == AXS_BC_HILL;
+                        const double cw = wallConcentration(kin, hill, phi[n], h);
+                        if (wall) {
+                            wall->cw[f] = cw;
+                            wall->ocr[f] = area * (hill ? kin.hill(cw) : kin.mm(cw)) * kin.inhib(cw);
+                        }
+                        co.AC[n] += area * h;
+                        co.b[n] += area * h * cw;
                     }
                 }
                 if (F != 0.0) {
@@ -1028,8 +1109,9 @@ namespace {
 struct Oracle {
     Mesh m;
     axs_config cfg{};
-    BC bcU, bcV, bcP, ppBC;
-    Coeff uC, vC, ppC;
+    BC bcU, bcV, bcP, ppBC, bcT, bcC;
+    Coeff uC, vC, ppC, tC, cC;
+    Wall wall;
     Fields s;
     Multigrid mg;
     std::vector<int> colorStart, cellOrder;
@@ -1044,7 +1126,7 @@ struct Oracle {
     // Solver::TimeFrame / timeFrames / maxTimeFrames (solver/header/solver.h:96-108)
     struct Frame {
         double time = 0.0;
-        std::vector<double> u, v, p;
+        std::vector<double> u, v, p, t, c;
     };
     std::vector<Frame> frames;
 
@@ -1052,7 +1134,8 @@ struct Oracle {
     void captureFrame(double time) {
         const int cap = cfg.max_time_frames > 0 ? cfg.max_time_frames : 500;
         if ((int)frames.size() >= cap) return;
-        frames.push_back(Frame{time, s.u, s.v, s.p});
+        frames.push_back(Frame{time, s.u, s.v, s.p, cfg.solve_energy ? s.temp : std::vector<double>{},
+                               cfg.solve_concentration ? s.conc : std::vector<double>{}});
     }
 
     void allocCoeff(Coeff& c, const std::vector<int>& fs, const std::vector<int>& fn) {
@@ -1072,9 +1155,15 @@ struct Oracle {
         allocCoeff(uC, fs, fn);
         allocCoeff(vC, fs, fn);
         allocCoeff(ppC, fs, fn);
+        allocCoeff(tC, fs, fn);
+        allocCoeff(cC, fs, fn);
+        wall.cw.assign(m.F, 0.0);
+        wall.ocr.assign(m.F, 0.0);
         const int N = m.N;
         for (auto* a : {&s.DU, &s.DV, &s.p, &s.pp, &s.u, &s.v, &s.uOld, &s.vOld, &s.uOld2, &s.vOld2,
-                        &s.gradPZ, &s.gradPR, &s.gradUZ, &s.gradUR, &s.gradVZ, &s.gradVR})
+                        &s.gradPZ, &s.gradPR, &s.gradUZ, &s.gradUR, &s.gradVZ, &s.gradVR, &s.temp, &s.conc,
+                        &s.tempOld, &s.concOld, &s.tempOld2, &s.concOld2, &s.gradTZ, &s.gradTR, &s.gradCZ,
+                        &s.gradCR})
             a->assign(N, 0.0);
         s.mDot.assign(m.F, 0.0);
         for (auto& r : rc) {
@@ -1125,6 +1214,12 @@ struct Oracle {
         residualAbs(uC, s.u, types[0], rc[AXS_RES_U].res, rc[AXS_RES_U].scale);
         residualAbs(vC, s.v, types[1], rc[AXS_RES_V].res, rc[AXS_RES_V].scale);
         continuity(m, s, rc[AXS_RES_CONTINUITY].res);
+        if (cfg.solve_energy)
+            residualAbs(tC, s.temp, rc[AXS_RES_TEMPERATURE].cfg.scaling, rc[AXS_RES_TEMPERATURE].res,
+                        rc[AXS_RES_TEMPERATURE].scale);
+        if (cfg.solve_concentration)
+            residualAbs(cC, s.conc, rc[AXS_RES_CONCENTRATION].cfg.scaling, rc[AXS_RES_CONCENTRATION].res,
+                        rc[AXS_RES_CONCENTRATION].scale);
         for (int i = 0; i < AXS_RES_COUNT; i++) {
             ResCfg& r = rc[i];
             if (!r.cfg.enabled) continue;
@@ -1152,7 +1247,7 @@ struct Oracle {
         // QUICK is clamped to upwind on load (file_manager.cpp:273-275) and
         // falls to upwind inside the kernel (solver_util.cuh:765-766).
         const int scheme = cfg.convection_scheme == AXS_CONV_QUICK ? AXS_CONV_UPWIND : cfg.convection_scheme;
-        for (Coeff* c : {&uC, &vC, &ppC}) {  // clearAllCoefficients, solver_util.cuh:976-997
+        for (Coeff* c : {&uC, &vC, &ppC, &tC, &cC}) {  // clearAllCoefficients, solver_util.cuh:976-997
             std::fill(c->AC.begin(), c->AC.end(), 0.0);
             std::fill(c->b.begin(), c->b.end(), 0.0);
             std::fill(c->AF.begin(), c->AF.end(), 0.0);
@@ -1193,6 +1288,27 @@ struct Oracle {
         computeGradient(m, ppBC, s.pp, s.gradPZ, s.gradPR, cfg.gradient_scheme);
         updateMomentumPressure(m, s);
         updateMassFlux(cfg.rho, m, s, bcP, nonOrtho);
+        // energy and concentration (solver.cpp:976-1009, 1222-1237): the
+        // momentum assembly with diffusivity k / (rho cp) or D, the volumetric
+        // flux, capacity 1; temperature under-relaxes with alpha_u, species not
+        for (int sc = 0; sc < 2; sc++) {
+            if (sc == 0 ? !cfg.solve_energy : !cfg.solve_concentration) continue;
+            Coeff& co = sc == 0 ? tC : cC;
+            const BC& bc = sc == 0 ? bcT : bcC;
+            std::vector<double>& phi = sc == 0 ? s.temp : s.conc;
+            std::vector<double>& gz = sc == 0 ? s.gradTZ : s.gradCZ;
+            std::vector<double>& gr = sc == 0 ? s.gradTR : s.gradCR;
+            const double gamma = sc == 0 ? cfg.k / (cfg.rho * cfg.cp) : cfg.diffusivity;
+            if (nonOrtho || needGrad) computeGradient(m, bc, phi, gz, gr, cfg.gradient_scheme);
+            addTransport(m, s, co, bc, phi, gz, gr, nonOrtho, gamma, cfg.add_convection ? 1 : 0, scheme,
+                         1.0 / cfg.rho, &wall);
+            if (transient)
+                transientTerm(m, co, sc == 0 ? &s.tempOld : &s.concOld,
+                              uOld2 ? (sc == 0 ? &s.tempOld2 : &s.concOld2) : nullptr, 1.0, cfg.dt);
+            underRelax(co, phi, sc == 0 ? s.alphaU : 1.0);
+            inverseDiagonal(co);
+            linearSolve(co, phi);
+        }
     }
 
     int run(int continueRun) {
@@ -1243,6 +1359,12 @@ struct Oracle {
                     s.vOld2.swap(s.vOld);
                 }
                 s.uOld = s.u;
+                if (bdf2) {
+                    s.tempOld2.swap(s.tempOld);
+                    s.concOld2.swap(s.concOld);
+                }
+                s.tempOld = s.temp;
+                s.concOld = s.conc;
                 s.vOld = s.v;
             }
             const bool bdf2Step = bdf2 && t > 0;
@@ -1268,6 +1390,8 @@ struct Oracle {
         computeGradient(m, bcU, s.u, s.gradUZ, s.gradUR, cfg.gradient_scheme);
         computeGradient(m, bcV, s.v, s.gradVZ, s.gradVR, cfg.gradient_scheme);
         computeGradient(m, bcP, s.p, s.gradPZ, s.gradPR, cfg.gradient_scheme);
+        if (cfg.solve_energy) computeGradient(m, bcT, s.temp, s.gradTZ, s.gradTR, cfg.gradient_scheme);
+        if (cfg.solve_concentration) computeGradient(m, bcC, s.conc, s.gradCZ, s.gradCR, cfg.gradient_scheme);
         return AXS_OK;
     }
 };
@@ -1292,6 +1416,25 @@ void loadBC(BC& bc, const axs_bc_view* v) {
         if (v->frequency) bc.frequency[g] = v->frequency[g];
     }
     bc.mean = bc.value;
+    auto col = [&](const double* src, double dflt) {
+        std::vector<double> a(bc.nGroups, dflt);
+        if (src)
+            for (int g = 0; g < bc.nGroups; g++) a[g] = src[g];
+        return a;
+    };
+    bc.vmax = col(v->vmax, 0.0);
+    bc.km = col(v->km, 0.0);
+    bc.hillN = col(v->hill_n, 1.0);
+    bc.inhibM = col(v->inhib_m, 1.0);
+    bc.k2 = col(v->inhib_k2, 0.0);
+    bc.v2 = col(v->inhib_v2, 0.0);
+    bc.rtot = col(v->wall_resistance, 0.0);
+    bc.kmN.assign(bc.nGroups, 0.0);
+    bc.k2M.assign(bc.nGroups, 0.0);
+    for (int g = 0; g < bc.nGroups; g++) {  // memory_manger.cpp:175-184
+        bc.kmN[g] = std::pow(bc.km[g], bc.hillN[g]);
+        bc.k2M[g] = std::pow(bc.k2[g], bc.inhibM[g]);
+    }
 }
 
 }  // namespace
@@ -1341,6 +1484,8 @@ int orc_set_bcs(orc_solver* h, int var, const axs_bc_view* v) {
     if (var == AXS_VAR_U) loadBC(h->o.bcU, v);
     else if (var == AXS_VAR_V) loadBC(h->o.bcV, v);
     else if (var == AXS_VAR_P) loadBC(h->o.bcP, v);
+    else if (var == AXS_VAR_T) loadBC(h->o.bcT, v);
+    else if (var == AXS_VAR_C) loadBC(h->o.bcC, v);
     else return AXS_ERR_INVALID;
     return AXS_OK;
 }
@@ -1421,6 +1566,26 @@ int orc_get_field(const orc_solver* h, int field, double* out, size_t n) {
     case AXS_FIELD_V_OLD: src = &o.s.vOld; break;
     case AXS_FIELD_U_OLD2: src = &o.s.uOld2; break;
     case AXS_FIELD_V_OLD2: src = &o.s.vOld2; break;
+    case AXS_FIELD_T: src = &o.s.temp; break;
+    case AXS_FIELD_C: src = &o.s.conc; break;
+    case AXS_FIELD_GRAD_TZ: src = &o.s.gradTZ; break;
+    case AXS_FIELD_GRAD_TR: src = &o.s.gradTR; break;
+    case AXS_FIELD_GRAD_CZ: src = &o.s.gradCZ; break;
+    case AXS_FIELD_GRAD_CR: src = &o.s.gradCR; break;
+    case AXS_FIELD_T_AC: src = &o.tC.AC; break;
+    case AXS_FIELD_T_B: src = &o.tC.b; break;
+    case AXS_FIELD_T_INVAC: src = &o.tC.invAC; break;
+    case AXS_FIELD_T_AF: src = &o.tC.AF; break;
+    case AXS_FIELD_C_AC: src = &o.cC.AC; break;
+    case AXS_FIELD_C_B: src = &o.cC.b; break;
+    case AXS_FIELD_C_INVAC: src = &o.cC.invAC; break;
+    case AXS_FIELD_C_AF: src = &o.cC.AF; break;
+    case AXS_FIELD_T_OLD: src = &o.s.tempOld; break;
+    case AXS_FIELD_C_OLD: src = &o.s.concOld; break;
+    case AXS_FIELD_T_OLD2: src = &o.s.tempOld2; break;
+    case AXS_FIELD_C_OLD2: src = &o.s.concOld2; break;
+    case AXS_FIELD_WALL_CW: src = &o.wall.cw; break;
+    case AXS_FIELD_WALL_OCR: src = &o.wall.ocr; break;
     case AXS_GEOM_INV_DPN: src = &m.invDPN; break;
     case AXS_GEOM_WP: src = &m.wP; break;
     case AXS_GEOM_DPB: src = &m.dPB; break;
@@ -1452,7 +1617,8 @@ int orc_gradient(const orc_solver* h, int var, int scheme, const double* phi, do
 }
 
 static const BC& bcOf(const Oracle& o, int var) {
-    return var == AXS_VAR_U ? o.bcU : var == AXS_VAR_V ? o.bcV : o.bcP;
+    return var == AXS_VAR_U ? o.bcU : var == AXS_VAR_V ? o.bcV : var == AXS_VAR_T ? o.bcT : var == AXS_VAR_C ? o.bcC
+                                                                                                       : o.bcP;
 }
 static std::vector<double> vec(const double* a, int n) {
     return a ? std::vector<double>(a, a + n) : std::vector<double>(n, 0.0);
@@ -1574,14 +1740,66 @@ int orc_probe_mass_flux_update(const orc_solver* h, double rho, const double* pp
     return AXS_OK;
 }
 
-// Transient frames: count, and one frame's time and field (AXS_FIELD_U/V/P).
+// wallConsumptionDiagnostic (solver_util.cu:480-531) over the last run's
+// concentration, summed in face order (out[10], as axs_get_wall_diagnostic).
+int orc_wall_diagnostic(const orc_solver* h, double* out) {
+    const Oracle& o = h->o;
+    const Mesh& m = o.m;
+    const BC& bc = o.bcC;
+    for (int i = 0; i < 10; i++) out[i] = 0.0;
+    if (!o.allocated) return AXS_ERR_STATE;
+    for (int f = 0; f < m.F; f++) {
+        if (m.neighbor[f] >= 0) continue;
+        const int ow = m.owner[f], g = m.group[f];
+        if (g < 0 || g >= bc.nGroups) continue;
+        const uint8_t t = bc.type[g];
+        const double F = o.s.mDot[f];
+        if (t == AXS_BC_DIRICHLET && F < 0.0) out[1] += -F * bc.value[g];
+        if (!(t == AXS_BC_MICHAELIS_MENTEN || t == AXS_BC_HILL)) continue;
+        double nz, nr;
+        outwardNormal(m, ow, f, nz, nr);
+        const double dPF = distCellToFace(m, ow, f, nz, nr);
+        const double hh = 1.0 / (dPF / o.cfg.diffusivity + bc.rtot[g]);
+        const double area = m.area[f], cp = o.s.conc[ow], cw = o.wall.cw[f];
+        out[0] += o.wall.ocr[f];
+        out[2] += area * hh * cp;
+        out[3] += 1.0;
+        out[4] += cw;
+        out[5] += dPF;
+        out[6] += hh;
+        out[7] += cp;
+        out[8] += bc.vmax[g];
+        out[9] += bc.km[g];
+    }
+    return AXS_OK;
+}
+
+// The wall Newton solve alone for given kinetics (hill: Hill law, else
+// Michaelis-Menten), cell value cp and conductance h.
+double orc_probe_wall_concentration(int hill, double vmax, double km, double n, double m, double k2, double v2,
+                                    double cp, double h) {
+    BC bc;
+    bc.nGroups = 1;
+    bc.vmax = {vmax};
+    bc.km = {km};
+    bc.hillN = {n};
+    bc.inhibM = {m};
+    bc.k2 = {k2};
+    bc.v2 = {v2};
+    bc.kmN = {std::pow(km, n)};
+    bc.k2M = {std::pow(k2, m)};
+    return wallConcentration(Kinetics(bc, 0), hill != 0, cp, h);
+}
+
+// Transient frames: count, and one frame's time and field (AXS_FIELD_U/V/P/T/C).
 int orc_frame_count(const orc_solver* h) { return (int)h->o.frames.size(); }
 int orc_get_frame(const orc_solver* h, int index, int field, double* time, double* out, size_t n) {
     const auto& fr = h->o.frames;
     if (index < 0 || index >= (int)fr.size()) return AXS_ERR_RANGE;
     const auto& f = fr[index];
     const std::vector<double>* src = field == AXS_FIELD_U ? &f.u : field == AXS_FIELD_V ? &f.v
-                                   : field == AXS_FIELD_P ? &f.p : nullptr;
+                                   : field == AXS_FIELD_P ? &f.p : field == AXS_FIELD_T ? &f.t
+                                   : field == AXS_FIELD_C ? &f.c : nullptr;
     if (time) *time = f.time;
     if (!out) return AXS_OK;
     if (!src) return AXS_ERR_INVALID;

@@ -58,6 +58,8 @@ def olib() -> C.CDLL:
             "orc_probe_bc_value": (C.c_double, [V, C.c_int, C.c_int, C.c_double]),
             "orc_bc_current": (C.c_double, [V, C.c_int, C.c_int]),
             "orc_frame_count": (C.c_int, [V]),
+            "orc_wall_diagnostic": (C.c_int, [V, _dp]),
+            "orc_probe_wall_concentration": (C.c_double, [C.c_int] + [C.c_double] * 8),
             "orc_get_frame": (C.c_int, [V, C.c_int, C.c_int, _dp, _dp, C.c_size_t]),
             "orc_set_threads": (None, [C.c_int]),
             "orc_get_threads": (C.c_int, []),
@@ -142,7 +144,7 @@ class Oracle:
         fid = axs.FIELDS[name]
         m = self.mesh
         S = int(m.arrays["cell_face_start"][-1])
-        if name in ("mdot", "inv_dpn", "wp", "dpb", "lsq_wz", "lsq_wr", "length2d"):
+        if name in ("mdot", "inv_dpn", "wp", "dpb", "lsq_wz", "lsq_wr", "length2d", "wall_cw", "wall_ocr"):
             n = m.n_faces
         elif name.endswith("_af"):
             n = S
@@ -217,17 +219,22 @@ class Oracle:
         return olib().orc_bc_current(self._h, var, group)
 
     def frames(self) -> list:
-        """[(time, {"u", "v", "p"})] of the transient frames."""
+        """[(time, {"u", "v", "p"[, "t"][, "c"]})] of the transient frames."""
         out = []
         N = self.mesh.n_cells
         for i in range(olib().orc_frame_count(self._h)):
             t = C.c_double()
             fields = {}
-            for name, fid in (("u", 0), ("v", 1), ("p", 2)):
+            for name, fid in (("u", 0), ("v", 1), ("p", 2), ("t", 32), ("c", 33)):
                 a = np.zeros(N)
-                olib().orc_get_frame(self._h, i, fid, C.byref(t), _d(a), N)
-                fields[name] = a
+                if olib().orc_get_frame(self._h, i, fid, C.byref(t), _d(a), N) == 0:
+                    fields[name] = a
             out.append((t.value, fields))
+        return out
+
+    def wall_diagnostic(self) -> np.ndarray:
+        out = np.zeros(10)
+        olib().orc_wall_diagnostic(self._h, _d(out))
         return out
 
     def csr(self):
@@ -236,6 +243,11 @@ class Oracle:
         fn = np.zeros(int(m.arrays["cell_face_start"][-1]), np.int32)
         olib().orc_build_csr(self._h, _i(fs), _i(fn))
         return fs, fn
+
+
+def wall_concentration(hill, vmax, km, n, m, k2, v2, cp, h) -> float:
+    """The oracle's wall Newton solve alone."""
+    return olib().orc_probe_wall_concentration(int(hill), vmax, km, n, m, k2, v2, cp, h)
 
 
 def oracle_coloring(fs: np.ndarray, fn: np.ndarray):
