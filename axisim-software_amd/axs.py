@@ -128,6 +128,21 @@ class NozzleSpec(C.Structure):
     ]
 
 
+class CellPolygons(C.Structure):
+    _fields_ = [
+        ("corners", C.c_int), ("quad_z", _dp), ("quad_r", _dp), ("n_points", C.c_int), ("point_z", _dp),
+        ("point_r", _dp), ("tris", _ip), ("structured", C.c_int), ("multiblock", C.c_int),
+    ]
+
+
+class SolverFile(C.Structure):
+    """What the .axislv solver payload holds beyond Config."""
+    _fields_ = [
+        ("var_units", C.c_uint8 * 15), ("velocity_solver", C.c_int), ("mom_tol", C.c_double),
+        ("pp_tol", C.c_double), ("residual_type", C.c_int * RES_COUNT),
+    ]
+
+
 PROGRESS_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_char_p), _dp)
 
 
@@ -158,6 +173,10 @@ def load_library(path: os.PathLike | None = None) -> C.CDLL:
         "axs_get_stats": (C.c_int, [S, C.POINTER(Stats)]),
         "axs_frame_count": (C.c_int, [S]),
         "axs_get_wall_diagnostic": (C.c_int, [S, _dp]),
+        "axs_mesh_get_quads": (C.c_int, [S, _ip, _dp, _dp]),
+        "axs_save_solution_npy": (C.c_int, [S, C.POINTER(CellPolygons), C.c_char_p]),
+        "axs_save_solver_file": (C.c_int, [C.c_char_p, C.POINTER(Config), C.POINTER(SolverFile)]),
+        "axs_load_solver_file": (C.c_int, [C.c_char_p, C.POINTER(Config), C.POINTER(SolverFile)]),
         "axs_get_frame": (C.c_int, [S, C.c_int, C.c_int, _dp, _dp, C.c_size_t]),
         "axs_mg_create": (C.c_int, [C.POINTER(S), C.c_int, _ip, _ip, C.c_int]),
         "axs_mg_destroy": (None, [S]),
@@ -246,6 +265,27 @@ class Mesh:
     arrays: dict
     points: np.ndarray | None = None  # (n, 2) for triangle meshes
     tris: np.ndarray | None = None
+    quads: np.ndarray | None = None   # (cells, 4, 2) corner (z, r) of multiblock cells
+
+    def polygons(self) -> "CellPolygons | None":
+        """The cell outlines for a solution export (None: unknown)."""
+        p = CellPolygons()
+        keep = {}
+        if self.quads is not None:
+            keep["qz"] = np.ascontiguousarray(self.quads[:, :, 0]).ravel()
+            keep["qr"] = np.ascontiguousarray(self.quads[:, :, 1]).ravel()
+            p.corners, p.structured, p.multiblock = 4, 1, 1
+            p.quad_z, p.quad_r = _dptr(keep["qz"]), _dptr(keep["qr"])
+        elif self.points is not None and self.tris is not None:
+            keep["pz"] = np.ascontiguousarray(self.points[:, 0])
+            keep["pr"] = np.ascontiguousarray(self.points[:, 1])
+            keep["t"] = np.ascontiguousarray(self.tris, dtype=np.int32).ravel()
+            p.corners, p.n_points = 3, self.points.shape[0]
+            p.point_z, p.point_r, p.tris = _dptr(keep["pz"]), _dptr(keep["pr"]), _iptr(keep["t"])
+        else:
+            return None
+        p._keep = keep
+        return p
 
     @property
     def n_cells(self) -> int:
@@ -270,6 +310,8 @@ class Mesh:
         extra = {}
         if self.points is not None:
             extra = {"points": self.points, "tris": self.tris}
+        if self.quads is not None:
+            extra["quads"] = self.quads
         np.savez_compressed(path, **self.arrays, **extra)
 
     @staticmethod
@@ -278,7 +320,8 @@ class Mesh:
             arrays = {k: np.ascontiguousarray(z[k]) for k in MESH_KEYS_D + MESH_KEYS_I}
             pts = z["points"] if "points" in z.files else None
             tris = z["tris"] if "tris" in z.files else None
-        return Mesh(arrays, pts, tris)
+            quads = z["quads"] if "quads" in z.files else None
+        return Mesh(arrays, pts, tris, quads)
 
 
 def _mesh_from_handle(h) -> Mesh:
@@ -304,8 +347,16 @@ def _mesh_from_handle(h) -> Mesh:
         _check(lib().axs_mesh_get_triangles(h, None, _dptr(pz), _dptr(pr), None, _iptr(tr)))
         pts = np.stack([pz, pr], axis=1)
         tris = tr.reshape(-1, 3)
+    nq = C.c_int()
+    _check(lib().axs_mesh_get_quads(h, C.byref(nq), None, None))
+    quads = None
+    if nq.value > 0:
+        qz = np.zeros(4 * nq.value)
+        qr = np.zeros(4 * nq.value)
+        _check(lib().axs_mesh_get_quads(h, None, _dptr(qz), _dptr(qr)))
+        quads = np.stack([qz.reshape(-1, 4), qr.reshape(-1, 4)], axis=2)
     lib().axs_mesh_destroy(h)
-    return Mesh(arrays, pts, tris)
+    return Mesh(arrays, pts, tris, quads)
 
 
 def multiblock_mesh(blocks: list[dict]) -> Mesh:
@@ -432,6 +483,23 @@ def default_bcs(group_types: list[int], overrides: dict | None = None, scalars: 
             bc.frequency = bc.frequency or [0.0] * len(group_types)
             bc.amplitude[g], bc.frequency[g] = spec[2], spec[3]
     return out
+
+
+def save_solver_file(path: os.PathLike, cfg: Config, extra: SolverFile | None = None) -> None:
+    """Write the .axislv solver payload (format 5) of cfg."""
+    _check(lib().axs_save_solver_file(os.fsencode(path), C.byref(cfg), C.byref(extra) if extra is not None else None))
+
+
+def load_solver_file(path: os.PathLike, cfg: Config | None = None) -> tuple[Config, SolverFile]:
+    """Read a .axislv solver payload over cfg (default: the defaults)."""
+    out = Config()
+    if cfg is None:
+        lib().axs_config_defaults(C.byref(out))
+    else:
+        C.pointer(out)[0] = cfg
+    extra = SolverFile()
+    _check(lib().axs_load_solver_file(os.fsencode(path), C.byref(out), C.byref(extra)))
+    return out, extra
 
 
 def default_config(**kw) -> Config:
@@ -568,6 +636,14 @@ class Solver:
         out = np.zeros(10)
         _check(lib().axs_get_wall_diagnostic(self._h, _dptr(out)), self._h)
         return out
+
+    def save_solution_npy(self, directory: os.PathLike, mesh: "Mesh | None" = None) -> None:
+        """NumPy export of the last solve (solution.npy, points/cells.npy,
+        frame_NNNN.npy, meta.json) into `directory`."""
+        mesh = mesh if mesh is not None else self.mesh
+        poly = mesh.polygons() if mesh is not None else None
+        _check(lib().axs_save_solution_npy(self._h, C.byref(poly) if poly is not None else None,
+                                           os.fsencode(directory)), self._h)
 
     def stats(self) -> Stats:
         s = Stats()

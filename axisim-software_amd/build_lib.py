@@ -33,6 +33,7 @@ SOURCES = [
     "csrc/host/capi.cpp",
     "csrc/host/topology.cpp",
     "csrc/host/casegen.cpp",
+    "csrc/host/export.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -20,9 +20,6 @@
 
 using namespace axs;
 
-struct axs_solver {
-    Solver* s = nullptr;
-};
 
 
 struct axs_mesh {
@@ -460,6 +457,14 @@ axs_status axs_mesh_get_triangles(const axs_mesh* h, int* n_points, double* pz, 
     if (pz) std::copy(h->m.pointZ.begin(), h->m.pointZ.end(), pz);
     if (pr) std::copy(h->m.pointR.begin(), h->m.pointR.end(), pr);
     if (tris) std::copy(h->m.tris.begin(), h->m.tris.end(), tris);
+    return AXS_OK;
+}
+
+axs_status axs_mesh_get_quads(const axs_mesh* h, int* n_cells, double* z4, double* r4) {
+    if (!h) return AXS_ERR_INVALID;
+    if (n_cells) *n_cells = (int)h->m.quadZ.size() / 4;
+    if (z4) std::copy(h->m.quadZ.begin(), h->m.quadZ.end(), z4);
+    if (r4) std::copy(h->m.quadR.begin(), h->m.quadR.end(), r4);
     return AXS_OK;
 }
 

@@ -156,11 +156,17 @@ std::string buildMultiblock(HostMesh& out, int nBlocks, const axs_block_spec* sp
     m.cellCenterR.assign(total, 0.0);
     m.cellArea2D.assign(total, 0.0);
     m.cellVolume.assign(total, 0.0);
+    m.quadZ.assign((size_t)total * 4, 0.0);
+    m.quadR.assign((size_t)total * 4, 0.0);
     for (const Block& b : blocks)
         for (int i = 0; i < b.nr; i++)
             for (int j = 0; j < b.nz; j++) {
                 const int g = b.cell(i, j);
                 const Node q[4] = {b.node(i, j), b.node(i, j + 1), b.node(i + 1, j + 1), b.node(i + 1, j)};
+                for (int k = 0; k < 4; k++) {
+                    m.quadZ[(size_t)g * 4 + k] = q[k].z;
+                    m.quadR[(size_t)g * 4 + k] = q[k].r;
+                }
                 double cz, cr, area;
                 quadCentroid(q, cz, cr, area);
                 m.cellCenterZ[g] = cz;

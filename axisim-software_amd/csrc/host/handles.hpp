@@ -50,6 +50,11 @@ axs_status guarded(std::string* err, F&& f) {
 
 }  // namespace axs
 
+// One solver instance (axs_create).
+struct axs_solver {
+    axs::Solver* s = nullptr;
+};
+
 // A stand-alone multigrid hierarchy on one device (axs_mg_*): the levels and
 // their tables, the level-0 system and iterates the host-array entry points
 // copy into, and the stream every call runs on.

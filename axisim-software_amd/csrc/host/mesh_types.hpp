@@ -22,6 +22,9 @@ struct HostMesh {
     // triangle meshes keep their node list (for export / invariants)
     std::vector<double> pointZ, pointR;
     std::vector<int> tris;
+    // multiblock meshes keep each cell's corner quad, 4 per cell in cell order
+    // (Mesh::multiBlockCellCorners, solver/source/mesh.cpp:2514-2537)
+    std::vector<double> quadZ, quadR;
 
     axs_mesh_view view() const {
         axs_mesh_view v{};

@@ -202,6 +202,8 @@ public:
     // wallConsumptionDiagnostic of the last run (axs_get_wall_diagnostic)
     void wallDiagnostic(double out[10]);
     int frameCount() const { return (int)frames_.size(); }
+    const HostMesh& hostMesh() const { return mesh_; }
+    const axs_config& config() const { return cfg_; }
     const Frame& frame(int index) const;
     // Learned initial guess of the first p' solve of every outer iteration
     // (net null: off).  level: multigrid level the net runs on, -1 = the first

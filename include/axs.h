@@ -460,7 +460,53 @@ axs_status axs_mesh_get_view(const axs_mesh* m, axs_mesh_view* view);
 /* Triangle mesh only: point and triangle arrays (NULL to query counts). */
 axs_status axs_mesh_get_triangles(const axs_mesh* m, int* n_points, double* point_z,
                                   double* point_r, int* n_tris, int* tris);
+/* Multiblock mesh only: each cell's corner quad in cell order, 4 (z, r) per cell
+ * (Mesh::multiBlockCellCorners, solver/source/mesh.cpp:2514-2537); *n_cells = 0
+ * for a mesh without them.  NULL arrays query the count. */
+axs_status axs_mesh_get_quads(const axs_mesh* m, int* n_cells, double* z4, double* r4);
 void axs_mesh_destroy(axs_mesh* m);
+
+/* ---------------- Output formats (manager/source/file_manager.cpp) ---------------- */
+/* The cell polygons a solution export writes beside the fields: corners 4 (quads
+ * in cell order, welded into one point list as the exporter does), 3 (triangles
+ * indexing points, cell c = triangle c) or 0 (none). */
+typedef struct axs_cell_polygons {
+    int corners;
+    const double* quad_z;      /* corners == 4: 4 per cell                       */
+    const double* quad_r;
+    int n_points;              /* corners == 3                                    */
+    const double* point_z;
+    const double* point_r;
+    const int* tris;           /* corners == 3: 3 per cell                        */
+    int structured;            /* meta "meshType": 1 Structured, 0 Unstructured   */
+    int multiblock;            /* meta "multiBlock"                               */
+} axs_cell_polygons;
+/* saveSolutionNpy (file_manager.cpp:2074-2309): into directory `dir` (created),
+ * solution.npy (cells x [z, r, volume, area2D, then the solved fields in the
+ * results panel's order]), points.npy + cells.npy when `poly` has corners,
+ * frame_NNNN.npy per transient frame (the same field columns; only the frame's
+ * primary fields are non-zero) and meta.json; float64 / int32, little-endian,
+ * NPY format 1.0. */
+axs_status axs_save_solution_npy(axs_solver* s, const axs_cell_polygons* poly, const char* dir);
+
+/* The solver settings file (.axislv solver payload, format version 5;
+ * saveFromPathSolver / loadFromPathSolver, file_manager.cpp:1482-1549): magic
+ * "AXVS", version, VariableUnits, SolverFieldOption, ConfigSolver, the fluid
+ * properties, ConfigSimple, then type / norm / scaling / enabled / tolerance of
+ * the residuals U, V, Continuity, Temperature, Concentration.  What the file
+ * holds beyond axs_config: */
+typedef struct axs_solver_file {
+    uint8_t var_units[15];            /* VariableUnits display-unit indices        */
+    int velocity_solver;              /* VelocitySolverType (0: SIMPLE)            */
+    double mom_tol, pp_tol;           /* ConfigSimple tolerances                   */
+    int residual_type[AXS_RES_COUNT]; /* ResidualType per residual                 */
+} axs_solver_file;
+/* Write cfg's settings (and `extra`, NULL: defaults) after the reference's
+ * sanitizeSolverConfig clamps; load reads a version-5 file into cfg (its other
+ * members untouched) and `extra` (may be NULL), clamped the same way.  Files of
+ * another magic or version: AXS_ERR_UNSUPPORTED; short files: AXS_ERR_INVALID. */
+axs_status axs_save_solver_file(const char* path, const axs_config* cfg, const axs_solver_file* extra);
+axs_status axs_load_solver_file(const char* path, axs_config* cfg, axs_solver_file* extra);
 
 #ifdef __cplusplus
 }
