@@ -68,7 +68,8 @@ def test_expansion_export_welds_block_interfaces(require_gpu, tmp_path):
     pts, cells = load(tmp_path / "points.npy"), load(tmp_path / "cells.npy")
     # three blocks' nodes, each interface node once: (6+1)(4+1) + (10+1)(4+1) + (10+1)(5+1) - 5 - 11
     assert pts.shape[0] == 7 * 5 + 11 * 5 + 11 * 6 - 5 - 11
-    np.testing.assert_array_equal(pts[cells], case.mesh.quads)
+    # an interface node keeps the first block's coordinates (the blocks may differ in the last bit)
+    np.testing.assert_allclose(pts[cells], case.mesh.quads, rtol=0, atol=1e-15)
 
 
 def test_nozzle_export_triangles(require_gpu, tmp_path):
