@@ -108,6 +108,10 @@ class Stats(C.Structure):
     ]
 
 
+class CaseStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("last_residual", C.c_double * RES_COUNT)]
+
+
 class BlockSpec(C.Structure):
     _fields_ = [
         ("z0", C.c_double), ("r0", C.c_double),
@@ -174,6 +178,8 @@ def load_library(path: os.PathLike | None = None) -> C.CDLL:
         "axs_frame_count": (C.c_int, [S]),
         "axs_get_wall_diagnostic": (C.c_int, [S, _dp]),
         "axs_mesh_get_quads": (C.c_int, [S, _ip, _dp, _dp]),
+        "axs_set_cases": (C.c_int, [S, C.c_int, _ip, _ip]),
+        "axs_get_case_stats": (C.c_int, [S, C.c_int, C.POINTER(CaseStats)]),
         "axs_save_solution_npy": (C.c_int, [S, C.POINTER(CellPolygons), C.c_char_p]),
         "axs_save_solver_file": (C.c_int, [C.c_char_p, C.POINTER(Config), C.POINTER(SolverFile)]),
         "axs_load_solver_file": (C.c_int, [C.c_char_p, C.POINTER(Config), C.POINTER(SolverFile)]),
@@ -612,6 +618,17 @@ class Solver:
             _check(lib().axs_get_residual_history(self._h, res, _dptr(out), cnt.value, C.byref(cnt)), self._h)
         return out
 
+    def set_cases(self, cell_starts, face_starts) -> None:
+        """Batched run: cases as contiguous cell / face ranges (starts incl. the end)."""
+        cs = np.ascontiguousarray(cell_starts, dtype=np.int32)
+        fs = np.ascontiguousarray(face_starts, dtype=np.int32)
+        _check(lib().axs_set_cases(self._h, cs.size - 1, _iptr(cs), _iptr(fs)), self._h)
+
+    def case_stats(self, c: int) -> CaseStats:
+        out = CaseStats()
+        _check(lib().axs_get_case_stats(self._h, c, C.byref(out)), self._h)
+        return out
+
     def frame_count(self) -> int:
         return lib().axs_frame_count(self._h)
 
@@ -994,15 +1011,23 @@ def batch_case(cases: list[Case], groups_per_mesh: int = 4, **cfg) -> Case:
     for k, v in cfg.items():
         setattr(config, k, v)
     ranges, off = [], 0
+    cell_starts, face_starts = [0], [0]
     for c in cases:
         ranges.append((off, c.mesh.n_cells))
         off += c.mesh.n_cells
-    return Case("batch", mesh, bcs, config, dict(ranges=ranges, names=[c.name for c in cases]))
+        cell_starts.append(cell_starts[-1] + c.mesh.n_cells)
+        face_starts.append(face_starts[-1] + c.mesh.n_faces)
+    return Case("batch", mesh, bcs, config, dict(ranges=ranges, names=[c.name for c in cases],
+                                                 cell_starts=cell_starts, face_starts=face_starts))
 
 
-def make_solver(case: Case, device: int = 0) -> Solver:
+def make_solver(case: Case, device: int = 0, per_case: bool = False) -> Solver:
+    """A solver on the case; per_case (batched cases): each case converges on
+    its own residuals (axs_set_cases)."""
     s = Solver(device)
     s.set_mesh(case.mesh)
     s.set_bcs(case.bcs)
     s.configure(case.config)
+    if per_case:
+        s.set_cases(case.info["cell_starts"], case.info["face_starts"])
     return s

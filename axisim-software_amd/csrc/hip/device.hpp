@@ -91,6 +91,27 @@ struct Ctl {
 };
 constexpr int kHistStride = 6;  // per residual check in DState::hist: five residuals, then the iteration
 
+// One case of a batched run (many cases solved as one union mesh): its own
+// convergence state, as a run of that case alone keeps it (solver.cpp:1243-1283).
+struct CaseCtl {
+    int done;       // converged, or out of iterations, in the current time step
+    int frozen;     // its fields are restored from the snapshot after every iteration
+    int converged;
+    int iters;      // outer iterations the case has run (all time steps)
+    double resVal[5], scaleVal[5];
+};
+// Device view of the cases: residual blocks per case and the cell / face maps
+// the freeze kernels read.
+struct DCases {
+    int n = 0;                      // cases (0: not a batched run)
+    const int* blockStart = nullptr;  // n + 1: the case's residual blocks
+    const int2* blockCells = nullptr;  // per residual block: [first cell, end)
+    const int* cellCount = nullptr;  // n: cells of each case (N of its scalings)
+    const int* caseOfCell = nullptr;
+    const int* caseOfFace = nullptr;
+    CaseCtl* ctl = nullptr;          // n
+};
+
 // Everything the SIMPLE kernels touch (VariablesSimple + Coefficients).
 struct DState {
     double *u, *v, *p, *pp, *uTmp, *vTmp, *ppTmp;
@@ -202,11 +223,25 @@ void launchFaceFlux(hipStream_t s, const DMesh& m, DBC bu, DBC bv, DBC bp, DStat
 void launchPPRhs(hipStream_t s, const DMesh& m, DState st, Physics ph, int cross, double* zero = nullptr);
 void launchGradPPUpdate(hipStream_t s, const DMesh& m, DBC ppbc, DState st, Physics ph);
 void launchMassFluxUpdate(hipStream_t s, const DMesh& m, DBC bp, DState st, Physics ph);
-void launchResidual(hipStream_t s, const DMesh& m, DState st, int blocks);
+void launchResidual(hipStream_t s, const DMesh& m, DState st, int blocks, const int2* blockCells = nullptr);
 // scalars: non-null phi of each solved scalar; their residual partials go
 // after the momentum ones (8 per block from st.partial + 9 * blocks)
-void launchResidualScalars(hipStream_t s, const DMesh& m, DState st, DScalar t, DScalar c, int blocks);
+void launchResidualScalars(hipStream_t s, const DMesh& m, DState st, DScalar t, DScalar c, int blocks,
+                           const int2* blockCells = nullptr);
 void launchFinish(hipStream_t s, DState st, int blocks, bool scalars);
+// batched run: each case's residuals, scale capture and convergence in one
+// workgroup per case (blocks: residual blocks of all cases), then the run's
+// bookkeeping; then the cases done keep their fields (restore the frozen ones
+// from the snapshot, snapshot the ones just done).  snapCells / fieldsCells: nc
+// cell arrays; snapFaces / fieldsFaces: nf face arrays.
+void launchFinishCases(hipStream_t s, DState st, DCases cs, int blocks, bool scalars);
+constexpr int kFreezeMax = 8;
+struct FreezeSet {
+    int n = 0;
+    double* field[kFreezeMax] = {};
+    double* snap[kFreezeMax] = {};
+};
+void launchFreeze(hipStream_t s, DCases cs, int nCells, FreezeSet cells, int nFaces, FreezeSet faces);
 // the scalar equation of one field: transport, wall kinetics, unsteady term,
 // under-relaxation and inverse diagonal (solver.cpp:1222-1235)
 void launchScalar(hipStream_t s, const DMesh& m, DBC bc, DScalar sc, DWall wall, const double* mDot, ScalarPhysics ph,

@@ -1070,13 +1070,20 @@ __global__ void __launch_bounds__(kBlock) kMassFluxUpdate(DMesh m, DBC bp, DStat
 // partial[b*9 + j]: 0 sum|ru| 1 sum ru^2 2 max|ru| 3 sum|AC_u u|
 //                   4 sum|rv| 5 sum rv^2 6 max|rv| 7 sum|AC_v v|  8 sum|imbalance|
 // ===========================================================================
+// blockCells (batched runs): the cell range of each block, so that a block never
+// straddles two cases and each case's blocks start at its first cell.
 template <int B>
-__global__ void __launch_bounds__(kBlock) kResidual(DMesh m, DState s) {
+__global__ void __launch_bounds__(kBlock) kResidual(DMesh m, DState s, const int2* blockCells) {
     if (finished(s.ctl)) return;
     if (s.ctl->k % s.ctl->checkConv != 0) return;
-    const int n = xcdBlock() * blockDim.x + threadIdx.x;
+    int n = xcdBlock() * blockDim.x + threadIdx.x, nEnd = m.N;
+    if (blockCells) {
+        const int2 r = blockCells[xcdBlock()];
+        n = r.x + threadIdx.x;
+        nEnd = r.y;
+    }
     double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (n < m.N) {
+    if (n < nEnd) {
         double axu = s.uAC[n] * s.u[n];
         double axv = s.vAC[n] * s.v[n];
         v[3] = fabs(axu);
@@ -1155,93 +1162,107 @@ __global__ void __launch_bounds__(kBlock) kResidual(DMesh m, DState s) {
 // 13-16 the same of the concentration).
 constexpr int kFinishThreads = 1024;
 __device__ __forceinline__ constexpr bool isMaxQuantity(int j) { return j == 2 || j == 6 || j == 11 || j == 15; }
+// The partials of blocks [b0 .. b0 + blocks) reduced to tot[0 .. 16] by one
+// workgroup of kFinishThreads: each thread loads the partials of four blocks in
+// one round (indices clamped, extra loads unused), each wave reduces by DPP and
+// one thread per quantity combines the waves.  Every workgroup thread calls it.
+template <bool SC>
+__device__ void reducePartials(const double* partial, const double* ps, int b0, int blocks, double* tot) {
+    constexpr int Q = SC ? 17 : 9;
+    __shared__ double red[kFinishThreads / 64][Q];
+    const int t = threadIdx.x;
+    double a[Q];
+#pragma unroll
+    for (int j = 0; j < Q; j++) a[j] = 0.0;
+    for (int bb = 0; bb < blocks; bb += 4 * kFinishThreads) {
+        double x[4][Q];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int b = b0 + min(bb + r * kFinishThreads + t, blocks - 1);
+#pragma unroll
+            for (int j = 0; j < 9; j++) x[r][j] = partial[(size_t)b * 9 + j];
+#pragma unroll
+            for (int j = 9; j < Q; j++) x[r][j] = ps[(size_t)b * 8 + (j - 9)];
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+            if (bb + r * kFinishThreads + t < blocks)
+#pragma unroll
+                for (int j = 0; j < Q; j++) a[j] = isMaxQuantity(j) ? fmax(a[j], x[r][j]) : a[j] + x[r][j];
+    }
+    const int lane = t & 63, wave = t >> 6;
+#pragma unroll
+    for (int j = 0; j < Q; j++) {
+        const double w = isMaxQuantity(j) ? waveScan<true>(a[j]) : waveScan<false>(a[j]);
+        if (lane == 63) red[wave][j] = w;
+    }
+    __syncthreads();
+    if (t < Q) {
+        double v = red[0][t];
+        for (int w = 1; w < kFinishThreads / 64; w++) v = isMaxQuantity(t) ? fmax(v, red[w][t]) : v + red[w][t];
+        tot[t] = v;
+    }
+    if (!SC && t >= 9 && t < 17) tot[t] = 0.0;
+    __syncthreads();
+}
+
+// residualAllHost (residuals.cu:37-100) for the enabled residuals from the
+// reduced sums, N cells; scaleVal keeps the continuity scale captured while
+// scaleIter < 5; then residualsConverged (solver.cpp:59-72)
+__device__ bool normaliseResiduals(const Ctl& cc, const double* tot, int N, int scaleIter, double* resVal,
+                                   double* scaleVal) {
+    for (int i = 0; i < 5; i++) {
+        if (!cc.enabled[i]) continue;
+        double res, scale = scaleVal[i];
+        if (i == 2) {
+            res = tot[8];
+        } else {
+            const int o = i < 2 ? i * 4 : 9 + (i - 3) * 4;
+            res = cc.norm[i] == 0 ? tot[o] : cc.norm[i] == 1 ? tot[o + 1] : tot[o + 2];
+            switch (cc.scaling[i]) {
+            case 0: scale = 1.0; break;
+            case 1: scale = (double)N; break;
+            case 2: scale = sqrt((double)N); break;
+            case 3: scale = tot[o + 3]; break;
+            default: break;
+            }
+        }
+        if (i == 2 && scaleIter < 5) scale = fmax(res, 0.0);
+        scaleVal[i] = scale;
+        if (scale != 0.0) {
+            if (cc.norm[i] == 1) res = sqrt(res);
+            res /= scale;
+        }
+        resVal[i] = res;
+    }
+    bool any = false, ok = true;
+    for (int i = 0; i < 5; i++) {
+        if (!cc.enabled[i]) continue;
+        any = true;
+        if (!(resVal[i] <= cc.tol[i])) ok = false;
+    }
+    return any && ok;
+}
+
 template <bool SC>
 __global__ void __launch_bounds__(kFinishThreads) kFinish(DState s, int blocks) {
-    constexpr int Q = SC ? 17 : 9;
     Ctl* c = s.ctl;
     // the whole control block in one round of loads; thread 0 works on this
     // copy and stores back the fields it changes
     Ctl cc = *c;
     if (cc.done) return;
     const bool check = (cc.k % cc.checkConv) == 0;
-    __shared__ double red[kFinishThreads / 64][Q];
     __shared__ double tot[17];
-    if (check) {
-        const int t = threadIdx.x;
-        double a[Q];
-#pragma unroll
-        for (int j = 0; j < Q; j++) a[j] = 0.0;
-        const double* ps = s.partial + (size_t)blocks * 9;
-        for (int b0 = 0; b0 < blocks; b0 += 4 * kFinishThreads) {
-            double x[4][Q];
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int b = min(b0 + r * kFinishThreads + t, blocks - 1);
-#pragma unroll
-                for (int j = 0; j < 9; j++) x[r][j] = s.partial[(size_t)b * 9 + j];
-#pragma unroll
-                for (int j = 9; j < Q; j++) x[r][j] = ps[(size_t)b * 8 + (j - 9)];
-            }
-#pragma unroll
-            for (int r = 0; r < 4; r++)
-                if (b0 + r * kFinishThreads + t < blocks)
-#pragma unroll
-                    for (int j = 0; j < Q; j++) a[j] = isMaxQuantity(j) ? fmax(a[j], x[r][j]) : a[j] + x[r][j];
-        }
-        const int lane = t & 63, wave = t >> 6;
-#pragma unroll
-        for (int j = 0; j < Q; j++) {
-            const double w = isMaxQuantity(j) ? waveScan<true>(a[j]) : waveScan<false>(a[j]);
-            if (lane == 63) red[wave][j] = w;
-        }
-        __syncthreads();
-        if (t < Q) {
-            double v = red[0][t];
-            for (int w = 1; w < kFinishThreads / 64; w++) v = isMaxQuantity(t) ? fmax(v, red[w][t]) : v + red[w][t];
-            tot[t] = v;
-        }
-        if (!SC && t >= 9 && t < 17) tot[t] = 0.0;
-        __syncthreads();
-    }
+    if (check) reducePartials<SC>(s.partial, s.partial + (size_t)blocks * 9, 0, blocks, tot);
     if (threadIdx.x != 0) return;
     if (check) {
         const int scaleIter = cc.transient ? cc.k : cc.iter;
-        for (int i = 0; i < 5; i++) {
-            if (!cc.enabled[i]) continue;
-            double res, scale = cc.scaleVal[i];
-            if (i == 2) {
-                res = tot[8];
-            } else {
-                const int o = i < 2 ? i * 4 : 9 + (i - 3) * 4;
-                res = cc.norm[i] == 0 ? tot[o] : cc.norm[i] == 1 ? tot[o + 1] : tot[o + 2];
-                switch (cc.scaling[i]) {
-                case 0: scale = 1.0; break;
-                case 1: scale = (double)cc.N; break;
-                case 2: scale = sqrt((double)cc.N); break;
-                case 3: scale = tot[o + 3]; break;
-                default: break;
-                }
-            }
-            if (i == 2 && scaleIter < 5) scale = fmax(res, 0.0);
-            cc.scaleVal[i] = scale;
-            if (scale != 0.0) {
-                if (cc.norm[i] == 1) res = sqrt(res);
-                res /= scale;
-            }
-            cc.resVal[i] = res;
-        }
+        cc.converged = normaliseResiduals(cc, tot, cc.N, scaleIter, cc.resVal, cc.scaleVal) ? 1 : 0;
         if (cc.nHist < cc.histCap) {
             for (int i = 0; i < 5; i++) s.hist[cc.nHist * kHistStride + i] = cc.resVal[i];
             s.hist[cc.nHist * kHistStride + 5] = (double)cc.iter;
             cc.nHist++;
         }
-        bool any = false, ok = true;
-        for (int i = 0; i < 5; i++) {
-            if (!cc.enabled[i]) continue;
-            any = true;
-            if (!(cc.resVal[i] <= cc.tol[i])) ok = false;
-        }
-        cc.converged = (any && ok) ? 1 : 0;
     }
     cc.iter++;
     cc.k++;
@@ -1255,6 +1276,86 @@ __global__ void __launch_bounds__(kFinishThreads) kFinish(DState s, int blocks) 
     c->iter = cc.iter;
     c->k = cc.k;
     c->done = cc.done;
+}
+
+// Batched runs, one workgroup per case: the case's residual blocks reduced as
+// a run of that case alone reduces them (its blocks start at its first cell),
+// normalised with its own cell count and its own continuity scale, and its
+// own convergence test.  A case already done this time step is left as it is.
+template <bool SC>
+__global__ void __launch_bounds__(kFinishThreads) kFinishCases(DState s, DCases cs, int blocks) {
+    const Ctl cc = *s.ctl;
+    if (cc.done) return;
+    const int ci = blockIdx.x;
+    CaseCtl& k = cs.ctl[ci];
+    if (k.done) return;  // uniform over the workgroup
+    const bool check = (cc.k % cc.checkConv) == 0;
+    __shared__ double tot[17];
+    const int b0 = cs.blockStart[ci], nb = cs.blockStart[ci + 1] - b0;
+    if (check) reducePartials<SC>(s.partial, s.partial + (size_t)blocks * 9, b0, nb, tot);
+    if (threadIdx.x != 0) return;
+    if (check) {
+        const int scaleIter = cc.transient ? cc.k : cc.iter;
+        double res[5], scale[5];
+        for (int i = 0; i < 5; i++) {
+            res[i] = k.resVal[i];
+            scale[i] = k.scaleVal[i];
+        }
+        k.converged = normaliseResiduals(cc, tot, cs.cellCount[ci], scaleIter, res, scale) ? 1 : 0;
+        for (int i = 0; i < 5; i++) {
+            k.resVal[i] = res[i];
+            k.scaleVal[i] = scale[i];
+        }
+    }
+    k.iters++;
+    if (k.converged || cc.k + 1 >= cc.maxIter) k.done = 1;
+}
+
+// After kFinishCases: the run's bookkeeping.  The history records, per check,
+// the largest of each residual over the cases that ran this iteration.
+__global__ void kFinishRun(DState s, DCases cs) {
+    Ctl* c = s.ctl;
+    Ctl cc = *c;
+    if (cc.done) return;
+    const bool check = (cc.k % cc.checkConv) == 0;
+    bool all = true, allConverged = true;
+    double worst[5] = {0, 0, 0, 0, 0};
+    for (int i = 0; i < cs.n; i++) {
+        const CaseCtl& k = cs.ctl[i];
+        all = all && k.done;
+        allConverged = allConverged && k.converged;
+        if (!k.frozen)
+            for (int j = 0; j < 5; j++) worst[j] = fmax(worst[j], k.resVal[j]);
+    }
+    if (check && cc.nHist < cc.histCap) {
+        for (int j = 0; j < 5; j++) s.hist[cc.nHist * kHistStride + j] = worst[j];
+        s.hist[cc.nHist * kHistStride + 5] = (double)cc.iter;
+        c->nHist = cc.nHist + 1;
+    }
+    for (int j = 0; j < 5; j++) c->resVal[j] = worst[j];
+    c->converged = allConverged ? 1 : 0;
+    c->iter = cc.iter + 1;
+    c->k = cc.k + 1;
+    c->done = (all || cc.k + 1 >= cc.maxIter) ? 1 : 0;
+}
+
+// Cases done keep their fields: the frozen ones get them back from the
+// snapshot, the ones done this iteration have them copied into it.  Runs in
+// every iteration (also after the run's last one, where it changes nothing).
+__global__ void __launch_bounds__(kBlock) kFreeze(const int* caseOf, const CaseCtl* ctl, int n, FreezeSet set) {
+    const int i = xcdBlock() * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const CaseCtl& k = ctl[caseOf[i]];
+    if (k.frozen) {
+        for (int a = 0; a < set.n; a++) set.field[a][i] = set.snap[a][i];
+    } else if (k.done) {
+        for (int a = 0; a < set.n; a++) set.snap[a][i] = set.field[a][i];
+    }
+}
+
+__global__ void kMarkFrozen(CaseCtl* ctl, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && ctl[i].done) ctl[i].frozen = 1;
 }
 
 // Start of a time step: freeze the old levels (solver.cpp:1058-1095) and reset
@@ -1468,12 +1569,17 @@ __global__ void __launch_bounds__(kBlock) kScalar(DMesh m, DBC bc, DScalar sc, D
 // the temperature, then 4-7 of the concentration (zero for a field not solved)
 template <int B>
 __global__ void __launch_bounds__(kBlock) kResidualScalars(DMesh m, DScalar t, DScalar c, double* partial,
-                                                           const Ctl* ctl) {
+                                                           const Ctl* ctl, const int2* blockCells) {
     if (finished(ctl)) return;
     if (ctl->k % ctl->checkConv != 0) return;
-    const int n = xcdBlock() * blockDim.x + threadIdx.x;
+    int n = xcdBlock() * blockDim.x + threadIdx.x, nEnd = m.N;
+    if (blockCells) {
+        const int2 r = blockCells[xcdBlock()];
+        n = r.x + threadIdx.x;
+        nEnd = r.y;
+    }
     double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (n < m.N) {
+    if (n < nEnd) {
         int k0, ns;
         rowRange(m.faceStart, m.uniformSlots, n, k0, ns);
         const DScalar* sys[2] = {&t, &c};
@@ -2734,18 +2840,29 @@ void launchGradPPUpdate(hipStream_t s, const DMesh& m, DBC ppbc, DState st, Phys
 void launchMassFluxUpdate(hipStream_t s, const DMesh& m, DBC bp, DState st, Physics ph) {
     kMassFluxUpdate<<<blocksFor(m.F), kBlock, 0, s>>>(m, bp, st, ph);
 }
-void launchResidual(hipStream_t s, const DMesh& m, DState st, int blocks) {
-    if (m.maxSlots <= 3) kResidual<3><<<blocks, kBlock, 0, s>>>(m, st);
-    else kResidual<4><<<blocks, kBlock, 0, s>>>(m, st);
+void launchResidual(hipStream_t s, const DMesh& m, DState st, int blocks, const int2* blockCells) {
+    if (m.maxSlots <= 3) kResidual<3><<<blocks, kBlock, 0, s>>>(m, st, blockCells);
+    else kResidual<4><<<blocks, kBlock, 0, s>>>(m, st, blockCells);
 }
 void launchFinish(hipStream_t s, DState st, int blocks, bool scalars) {
     if (scalars) kFinish<true><<<1, kFinishThreads, 0, s>>>(st, blocks);
     else kFinish<false><<<1, kFinishThreads, 0, s>>>(st, blocks);
 }
-void launchResidualScalars(hipStream_t s, const DMesh& m, DState st, DScalar t, DScalar c, int blocks) {
+void launchFinishCases(hipStream_t s, DState st, DCases cs, int blocks, bool scalars) {
+    if (scalars) kFinishCases<true><<<cs.n, kFinishThreads, 0, s>>>(st, cs, blocks);
+    else kFinishCases<false><<<cs.n, kFinishThreads, 0, s>>>(st, cs, blocks);
+    kFinishRun<<<1, 1, 0, s>>>(st, cs);
+}
+void launchFreeze(hipStream_t s, DCases cs, int nCells, FreezeSet cells, int nFaces, FreezeSet faces) {
+    if (cells.n > 0) kFreeze<<<blocksFor(nCells), kBlock, 0, s>>>(cs.caseOfCell, cs.ctl, nCells, cells);
+    if (faces.n > 0) kFreeze<<<blocksFor(nFaces), kBlock, 0, s>>>(cs.caseOfFace, cs.ctl, nFaces, faces);
+    kMarkFrozen<<<blocksFor(cs.n), kBlock, 0, s>>>(cs.ctl, cs.n);
+}
+void launchResidualScalars(hipStream_t s, const DMesh& m, DState st, DScalar t, DScalar c, int blocks,
+                           const int2* blockCells) {
     double* part = st.partial + (size_t)blocks * 9;
-    if (m.maxSlots <= 3) kResidualScalars<3><<<blocks, kBlock, 0, s>>>(m, t, c, part, st.ctl);
-    else kResidualScalars<4><<<blocks, kBlock, 0, s>>>(m, t, c, part, st.ctl);
+    if (m.maxSlots <= 3) kResidualScalars<3><<<blocks, kBlock, 0, s>>>(m, t, c, part, st.ctl, blockCells);
+    else kResidualScalars<4><<<blocks, kBlock, 0, s>>>(m, t, c, part, st.ctl, blockCells);
 }
 void launchScalar(hipStream_t s, const DMesh& m, DBC bc, DScalar sc, DWall wall, const double* mDot, ScalarPhysics ph,
                   const Ctl* ctl) {

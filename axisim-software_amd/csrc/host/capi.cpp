@@ -207,6 +207,19 @@ axs_status axs_get_frame(axs_solver* h, int index, int field, double* time, doub
     });
 }
 
+axs_status axs_set_cases(axs_solver* h, int count, const int* cell_start, const int* face_start) {
+    if (!h) return AXS_ERR_INVALID;
+    return guarded(&h->s->lastError, [&] { h->s->setCases(count, cell_start, face_start); });
+}
+
+axs_status axs_get_case_stats(axs_solver* h, int c, axs_case_stats* out) {
+    if (!h || !out) return AXS_ERR_INVALID;
+    return guarded(&h->s->lastError, [&] {
+        std::memset(out, 0, sizeof(*out));
+        h->s->caseStats(c, out->iterations, out->converged, out->last_residual);
+    });
+}
+
 axs_status axs_get_wall_diagnostic(axs_solver* h, double out[10]) {
     if (!h || !out) return AXS_ERR_INVALID;
     return guarded(&h->s->lastError, [&] { h->s->wallDiagnostic(out); });

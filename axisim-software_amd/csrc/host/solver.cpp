@@ -899,6 +899,8 @@ void Solver::setMesh(const axs_mesh_view& v) {
     nColors_ = 0;
     colorStart_.clear();
     sweepBlocks_ = SweepBlocks{};
+    nCases_ = 0;  // case ranges refer to the previous mesh
+    casesReady_ = false;
 }
 
 void Solver::setBC(int var, const axs_bc_view& v) {
@@ -1235,9 +1237,114 @@ void Solver::enqueueIteration(hipStream_t s) {
     launchGradPPUpdate(s, dm_, ppbc_, st, ph);
     launchMassFluxUpdate(s, dm_, bp, st, ph);
     if (scalarsOn()) enqueueScalars(s);
+    if (nCases_ > 0) {
+        // each case's residuals over blocks that start at its first cell, its
+        // own convergence, then the cases done keep their fields
+        DState sc = st;
+        sc.partial = casePartial_;
+        launchResidual(s, dm_, sc, caseBlocks_, dcases_.blockCells);
+        if (scalarsOn()) launchResidualScalars(s, dm_, sc, sc_[0], sc_[1], caseBlocks_, dcases_.blockCells);
+        launchFinishCases(s, sc, dcases_, caseBlocks_, scalarsOn());
+        launchFreeze(s, dcases_, dm_.N, freezeCells_, dm_.F, freezeFaces_);
+        return;
+    }
     launchResidual(s, dm_, st, partialBlocks_);
     if (scalarsOn()) launchResidualScalars(s, dm_, st, sc_[0], sc_[1], partialBlocks_);
     launchFinish(s, st, partialBlocks_, scalarsOn());
+}
+
+void Solver::setCases(int count, const int* cellStart, const int* faceStart) {
+    if (running_) throw std::logic_error("solver is running");
+    if (!haveMesh_) throw std::logic_error("set the mesh before its cases");
+    if (count < 0) throw std::invalid_argument("negative case count");
+    caseCellStart_.clear();
+    caseFaceStart_.clear();
+    nCases_ = 0;
+    casesReady_ = false;
+    if (count == 0) return;
+    if (!cellStart || !faceStart) throw std::invalid_argument("case ranges missing");
+    if (cellStart[0] != 0 || faceStart[0] != 0 || cellStart[count] != mesh_.nCells || faceStart[count] != mesh_.nFaces)
+        throw std::invalid_argument("case ranges must cover the mesh's cells and faces from 0");
+    for (int c = 0; c < count; c++)
+        if (cellStart[c + 1] <= cellStart[c] || faceStart[c + 1] <= faceStart[c])
+            throw std::invalid_argument("every case needs cells and faces");
+    // a case's faces connect only its own cells
+    for (int c = 0; c < count; c++)
+        for (int f = faceStart[c]; f < faceStart[c + 1]; f++) {
+            const int o = mesh_.faceOwner[f], nb = mesh_.faceNeighbor[f];
+            if (o < cellStart[c] || o >= cellStart[c + 1] || (nb >= 0 && (nb < cellStart[c] || nb >= cellStart[c + 1])))
+                throw std::invalid_argument("case " + std::to_string(c) + " has a face outside its cells");
+        }
+    caseCellStart_.assign(cellStart, cellStart + count + 1);
+    caseFaceStart_.assign(faceStart, faceStart + count + 1);
+    nCases_ = count;
+}
+
+// Device tables of the cases (after the state is allocated: the snapshots
+// cover the fields allocated for this run).
+void Solver::prepareCases() {
+    if (nCases_ == 0 || casesReady_) return;
+    caseArena_.release();
+    const int n = nCases_;
+    std::vector<int> blockStart(n + 1, 0), cellCount(n), caseOfCell(mesh_.nCells), caseOfFace(mesh_.nFaces);
+    std::vector<int2> blockCells;
+    for (int c = 0; c < n; c++) {
+        const int c0 = caseCellStart_[c], c1 = caseCellStart_[c + 1];
+        cellCount[c] = c1 - c0;
+        for (int b = c0; b < c1; b += kBlock) blockCells.push_back(make_int2(b, std::min(b + kBlock, c1)));
+        blockStart[c + 1] = (int)blockCells.size();
+        for (int i = c0; i < c1; i++) caseOfCell[i] = c;
+        for (int f = caseFaceStart_[c]; f < caseFaceStart_[c + 1]; f++) caseOfFace[f] = c;
+    }
+    caseBlocks_ = (int)blockCells.size();
+    DCases d{};
+    d.n = n;
+    d.blockStart = caseArena_.upload(blockStart);
+    d.blockCells = caseArena_.upload(blockCells);
+    d.cellCount = caseArena_.upload(cellCount);
+    d.caseOfCell = caseArena_.upload(caseOfCell);
+    d.caseOfFace = caseArena_.upload(caseOfFace);
+    d.ctl = caseArena_.zeros<CaseCtl>(n);
+    dcases_ = d;
+    casePartial_ = caseArena_.zeros<double>((size_t)caseBlocks_ * (9 + 8));
+    caseHost_.assign(n, CaseCtl{});
+    // the fields a case done keeps (everything an iteration changes that a
+    // later one or the output reads)
+    FreezeSet fc{}, ff{};
+    auto add = [&](FreezeSet& set, double* field, int len) {
+        if (!field) return;
+        set.field[set.n] = field;
+        set.snap[set.n] = caseArena_.zeros<double>(len);
+        set.n++;
+    };
+    for (double* f : {ds_.u, ds_.v, ds_.p, ds_.pp, sc_[0].phi, sc_[1].phi}) add(fc, f, mesh_.nCells);
+    for (double* f : {ds_.mDot, wall_.cw, wall_.ocr}) add(ff, f, mesh_.nFaces);
+    freezeCells_ = fc;
+    freezeFaces_ = ff;
+    casesReady_ = true;
+}
+
+// Every case runs again (a new run or time step): done / frozen / converged
+// cleared; residual and scale values kept (the continuity scale is captured
+// again at the first checks, as in a run of the case alone).
+void Solver::resetCases() {
+    if (nCases_ == 0) return;
+    AXS_HIP(hipMemcpy(caseHost_.data(), dcases_.ctl, (size_t)nCases_ * sizeof(CaseCtl), hipMemcpyDeviceToHost));
+    for (CaseCtl& k : caseHost_) {  // iterations keep counting over steps and continued runs
+        k.done = 0;
+        k.frozen = 0;
+        k.converged = 0;
+    }
+    AXS_HIP(hipMemcpy(dcases_.ctl, caseHost_.data(), (size_t)nCases_ * sizeof(CaseCtl), hipMemcpyHostToDevice));
+}
+
+void Solver::caseStats(int c, int& iterations, int& converged, double* lastResidual) const {
+    if (c < 0 || c >= nCases_) throw std::out_of_range("no such case");
+    if (!casesReady_) throw std::logic_error("run the batch first");
+    const CaseCtl& k = caseHost_[c];
+    iterations = k.iters;
+    converged = k.converged;
+    for (int i = 0; i < AXS_RES_COUNT; i++) lastResidual[i] = k.resVal[i];
 }
 
 // The scalar equation of field `which` (0 temperature, 1 concentration):
@@ -1381,6 +1488,7 @@ void Solver::run(axs_progress_cb cb, void* user, bool continueRun) {
     const bool cont = continueRun && canContinue(why);
     if (!cont) {
         allocateState();
+        casesReady_ = false;  // the snapshots cover this run's fields
         currentIteration_ = 0;
         for (auto& h : history_) h.clear();
         contCells_ = mesh_.nCells;
@@ -1424,6 +1532,8 @@ void Solver::run(axs_progress_cb cb, void* user, bool continueRun) {
         AXS_HIP(hipHostMalloc((void**)&hhist_, (size_t)cap * kHistStride * sizeof(double), hipHostMallocDefault));
         hhistCap_ = cap;
     }
+    prepareCases();
+    resetCases();
     buildGraph();
 
     // control block (scaleVal persists across continued runs like the pinned
@@ -1495,6 +1605,7 @@ void Solver::run(axs_progress_cb cb, void* user, bool continueRun) {
             hctl_->nHist = 0;
             hctl_->bdf2Step = (bdf2 && t > 0) ? 1 : 0;
             AXS_HIP(hipMemcpy(ds_.ctl, hctl_, sizeof(Ctl), hipMemcpyHostToDevice));
+            resetCases();
             iterateUntilDone();
             // the keyframe interval and the last (or stopped) step, solver.cpp:1286-1294
             const bool lastStep = (t == steps - 1) || stopped_;
@@ -1503,6 +1614,8 @@ void Solver::run(axs_progress_cb cb, void* user, bool continueRun) {
     }
     currentIteration_ = hctl_->iter;
     converged_ = hctl_->converged != 0;
+    if (nCases_ > 0)
+        AXS_HIP(hipMemcpy(caseHost_.data(), dcases_.ctl, (size_t)nCases_ * sizeof(CaseCtl), hipMemcpyDeviceToHost));
     // output gradients with the final fields (solver.cpp:1330-1336)
     launchGradField(stream_, dm_, dbc_[AXS_VAR_U], ds_.u, ds_.gUZ, ds_.gUR, c.gradient_scheme, nullptr);
     launchGradField(stream_, dm_, dbc_[AXS_VAR_V], ds_.v, ds_.gVZ, ds_.gVR, c.gradient_scheme, nullptr);

@@ -203,6 +203,12 @@ public:
     void wallDiagnostic(double out[10]);
     int frameCount() const { return (int)frames_.size(); }
     const HostMesh& hostMesh() const { return mesh_; }
+    // Batched runs: the union mesh's cases as contiguous cell / face ranges
+    // (count 0: one case).  Each case then keeps its own residuals, continuity
+    // scale and convergence, and stops on them.
+    void setCases(int count, const int* cellStart, const int* faceStart);
+    int caseCount() const { return nCases_; }
+    void caseStats(int c, int& iterations, int& converged, double* lastResidual) const;
     const axs_config& config() const { return cfg_; }
     const Frame& frame(int index) const;
     // Learned initial guess of the first p' solve of every outer iteration
@@ -305,6 +311,20 @@ private:
     std::vector<double> history_[AXS_RES_COUNT];
     // scalar transport: temperature and concentration state (allocated when
     // solved), the reactive walls' per-face state
+    // batched cases (setCases): host tables, their device copies, the per-case
+    // control blocks with a host mirror, the residual partials and the
+    // snapshots of the fields of cases done
+    int nCases_ = 0;
+    std::vector<int> caseCellStart_, caseFaceStart_;
+    DCases dcases_{};
+    Arena caseArena_;
+    int caseBlocks_ = 0;
+    double* casePartial_ = nullptr;
+    std::vector<CaseCtl> caseHost_;
+    FreezeSet freezeCells_{}, freezeFaces_{};
+    bool casesReady_ = false;
+    void prepareCases();
+    void resetCases();
     DScalar sc_[2]{};
     DWall wall_{};
     bool scAlloc_[2] = {false, false};

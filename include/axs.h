@@ -256,6 +256,24 @@ int axs_frame_count(const axs_solver* s);
 /* Frames also hold AXS_FIELD_T / AXS_FIELD_C when those are solved. */
 axs_status axs_get_frame(axs_solver* s, int index, int field, double* time, double* out, size_t n);
 
+/* Batched runs (build extension; the reference solves one case at a time,
+ * solver.cpp:1243-1283): the union mesh of axs_mesh_concat as `count` cases,
+ * case c = cells cell_start[c] .. cell_start[c+1]-1 and faces face_start[c] ..
+ * face_start[c+1]-1 (each case's faces connect only its cells).  Every case then
+ * keeps its own residual norms (summed over blocks from its first cell, its own
+ * cell count in the scalings), continuity scale and convergence test, and stops
+ * on them: a case done keeps its fields while the others iterate, and the run
+ * ends when every case is done (or at max_iter), per time step in a transient
+ * run.  Without multigrid a case then runs exactly as it runs alone.  count 0:
+ * one case (the default); set after axs_set_mesh. */
+axs_status axs_set_cases(axs_solver* s, int count, const int* cell_start, const int* face_start);
+typedef struct axs_case_stats {
+    int iterations;                       /* outer iterations the case ran (all steps) */
+    int converged;                        /* in its last time step                   */
+    double last_residual[AXS_RES_COUNT];  /* at its last residual check              */
+} axs_case_stats;
+axs_status axs_get_case_stats(axs_solver* s, int c, axs_case_stats* out);
+
 /* Reactive-wall consumption diagnostic of the last run's concentration field
  * (wallConsumptionDiagnostic, solver/source/solver_util.cu:480-531; reported by
  * solver.cpp:1338-1385): out[0] total wall OCR, [1] substrate carried in through
