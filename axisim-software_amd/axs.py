@@ -109,7 +109,8 @@ class Stats(C.Structure):
 
 
 class CaseStats(C.Structure):
-    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("last_residual", C.c_double * RES_COUNT)]
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("last_residual", C.c_double * RES_COUNT),
+                ("stop_iteration", C.c_int)]
 
 
 class BlockSpec(C.Structure):

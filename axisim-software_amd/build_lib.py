@@ -22,6 +22,8 @@ LIB_DIR = PKG / "lib"
 LIB = LIB_DIR / "libaxs.so"
 ORACLE_DIR = ROOT / "oracle"
 ORACLE_LIB = ORACLE_DIR / "liboracle.so"
+# FP64 as written (no contraction into fused multiply-adds), OpenMP loops
+ORACLE_FLAGS = ["-std=c++17", "-O3", "-ffp-contract=off", "-fopenmp"]
 
 SOURCES = [
     "csrc/hip/kernels.hip",
@@ -107,7 +109,7 @@ def build_oracle() -> Path:
     per-cell loops of the multi-thread CPU baseline)."""
     tmp = ORACLE_LIB.with_suffix(".so.tmp")
     _run([
-        "g++", "-std=c++17", "-O2", "-ffp-contract=off", "-fopenmp", "-fPIC", "-shared",
+        "g++", *ORACLE_FLAGS, "-fPIC", "-shared",
         f"-I{ROOT / 'include'}", str(ORACLE_DIR / "oracle.cpp"), "-o", str(tmp),
     ])
     os.replace(tmp, ORACLE_LIB)

@@ -98,6 +98,7 @@ struct CaseCtl {
     int frozen;     // its fields are restored from the snapshot after every iteration
     int converged;
     int iters;      // outer iterations the case has run (all time steps)
+    int stopIter;   // the run's iteration count when the case last stopped
     double resVal[5], scaleVal[5];
 };
 // Device view of the cases: residual blocks per case and the cell / face maps

@@ -1308,7 +1308,10 @@ __global__ void __launch_bounds__(kFinishThreads) kFinishCases(DState s, DCases 
         }
     }
     k.iters++;
-    if (k.converged || cc.k + 1 >= cc.maxIter) k.done = 1;
+    if (k.converged || cc.k + 1 >= cc.maxIter) {
+        k.done = 1;
+        k.stopIter = cc.iter + 1;
+    }
 }
 
 // After kFinishCases: the run's bookkeeping.  The history records, per check,

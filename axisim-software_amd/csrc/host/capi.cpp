@@ -216,7 +216,7 @@ axs_status axs_get_case_stats(axs_solver* h, int c, axs_case_stats* out) {
     if (!h || !out) return AXS_ERR_INVALID;
     return guarded(&h->s->lastError, [&] {
         std::memset(out, 0, sizeof(*out));
-        h->s->caseStats(c, out->iterations, out->converged, out->last_residual);
+        h->s->caseStats(c, out->iterations, out->converged, out->last_residual, out->stop_iteration);
     });
 }
 

@@ -1338,12 +1338,13 @@ void Solver::resetCases() {
     AXS_HIP(hipMemcpy(dcases_.ctl, caseHost_.data(), (size_t)nCases_ * sizeof(CaseCtl), hipMemcpyHostToDevice));
 }
 
-void Solver::caseStats(int c, int& iterations, int& converged, double* lastResidual) const {
+void Solver::caseStats(int c, int& iterations, int& converged, double* lastResidual, int& stopIteration) const {
     if (c < 0 || c >= nCases_) throw std::out_of_range("no such case");
     if (!casesReady_) throw std::logic_error("run the batch first");
     const CaseCtl& k = caseHost_[c];
     iterations = k.iters;
     converged = k.converged;
+    stopIteration = k.stopIter;
     for (int i = 0; i < AXS_RES_COUNT; i++) lastResidual[i] = k.resVal[i];
 }
 

@@ -208,7 +208,7 @@ public:
     // scale and convergence, and stops on them.
     void setCases(int count, const int* cellStart, const int* faceStart);
     int caseCount() const { return nCases_; }
-    void caseStats(int c, int& iterations, int& converged, double* lastResidual) const;
+    void caseStats(int c, int& iterations, int& converged, double* lastResidual, int& stopIteration) const;
     const axs_config& config() const { return cfg_; }
     const Frame& frame(int index) const;
     // Learned initial guess of the first p' solve of every outer iteration

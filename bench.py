@@ -11,9 +11,16 @@ Prints one JSON line (see the repository's benchmark contract):
               sweeps per launch) measured with HIP events on the solver's
               stream, against the 8 TB/s HBM peak; the one-sweep kernel beside it
   mg_vcycle   the p' multigrid solve (3 V-cycles) against the same peak
-  cpu_baseline the FP64 oracle on this host, a bounded sample of the same
-              workload on every thread this process may use (`cores`), with
-              the one-thread rate beside it (`one_thread`)
+  to_tolerance the nozzle, the 128 x 32 pipe and the 238,717-cell expansion
+              from a cold start to the 1e-6 residual, FP64 and mixed: iterations,
+              wall time, and the mixed fields' error against the FP64 fields
+  batched_sweep 64 cases of 20k-500k cells in one solver, BDF2 pulsatile, each
+              case stopping on its own residuals; aggregate cell-updates/s and
+              the cases' latency percentiles
+  cpu_baseline the FP64 oracle built on this host (-O3 -march=native), a bounded
+              sample (20 iterations) of the same workload on every thread this
+              process may use (`cores`), the one-thread rate beside it, and the
+              pipe run to 1e-6 on the host (`c1`)
 """
 from __future__ import annotations
 
@@ -56,7 +63,12 @@ def main() -> int:
     ap.add_argument("--cells", type=float, default=1.0e6)
     ap.add_argument("--mg-levels", type=int, default=6,
                     help="multigrid levels of the nozzle case (its configuration: 4:1 coarsening, 6 levels)")
-    ap.add_argument("--cpu-iters", type=int, default=10, help="oracle sample size (outer iterations)")
+    ap.add_argument("--cpu-iters", type=int, default=20, help="oracle sample size (outer iterations)")
+    ap.add_argument("--no-converge", action="store_true",
+                    help="skip the runs to the 1e-6 residual (nozzle, pipe and expansion, FP64 and mixed)")
+    ap.add_argument("--converge-cap", type=int, default=60000, help="iteration cap of the runs to 1e-6")
+    ap.add_argument("--expansion-cap", type=int, default=20000,
+                    help="iteration cap of the 238,717-cell expansion runs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default workgroup sizes")
@@ -157,7 +169,21 @@ def main() -> int:
                  "mg_solve_ms": round(mg32_ms, 4), "mg_solve_gbs": round(mg32_gbs, 1)}
         del sm
 
-    # batched sweep: many cases of uneven size in one solver, every launch covers all
+    # runs to the 1e-6 residual from a cold start: the nozzle (the metric's
+    # workload), the pipe and the expansion, each FP64 and mixed, with the
+    # per-field error of the mixed run against the FP64 run
+    to_tol = None
+    if not args.no_converge:
+        to_tol = {
+            "tolerance": 1e-6,
+            "nozzle_1M": converge_pair(lambda **kw: axs.nozzle_case(mesh=m, mg_max_levels=args.mg_levels, **kw),
+                                       args.converge_cap),
+            "pipe_128x32": converge_pair(lambda **kw: axs.pipe_case(**kw), args.converge_cap),
+            "expansion_238k": converge_pair(lambda **kw: axs.expansion_case(**kw), args.expansion_cap),
+        }
+
+    # batched sweep: many cases of uneven size in one solver, every launch
+    # covers all; each case stops on its own residuals in every time step
     c4 = None
     if not args.no_c4:
         tc = time.time()
@@ -168,7 +194,7 @@ def main() -> int:
                                gradient_scheme=axs.GRAD_LSQ, convection_scheme=axs.CONV_UPWIND,
                                save_key_frame_iter=args.c4_steps)
         log(f"c4: {len(cases)} cases, {cells} cells ({time.time() - tc:.1f} s to generate)")
-        sb = axs.make_solver(batch)
+        sb = axs.make_solver(batch, per_case=True)
         sb.run()  # two time steps: hierarchy, graph, warm caches
         c4tune = None if args.no_autotune else sb.autotune(10)
         batch.config.t_end = args.c4_steps * 1e-3
@@ -178,22 +204,42 @@ def main() -> int:
         wall = time.time() - t2
         sst = sb.stats()
         cups = cells * sst.iterations / (sst.solve_ms / 1e3)
+        # a case's latency: the run's iterations up to the one where it stopped
+        # (in the last time step) times the mean iteration time of the run
+        per_it = sst.solve_ms / 1e3 / max(1, sst.iterations)
+        cst = [sb.case_stats(i) for i in range(len(cases))]
+        lat = np.array([k.stop_iteration * per_it for k in cst])
+        active = np.array([k.iterations for k in cst])
         log(f"c4: {args.c4_steps} BDF2 steps, {sst.iterations} outer iterations in {wall:.2f} s -> "
-            f"{cups:.3e} cell-updates/s")
+            f"{cups:.3e} cell-updates/s; case latency p50 {np.percentile(lat, 50):.3f} s p95 "
+            f"{np.percentile(lat, 95):.3f} s")
         templates = [c.info["template"] for c in cases]
         c4 = {"cases": len(cases), "cells": cells, "templates": {t: templates.count(t) for t in set(templates)},
               "time_steps": args.c4_steps, "dt": 1e-3, "max_iter_per_step": args.c4_iters,
+              "per_case_convergence": True,
               "frames_captured": sb.frame_count(), "save_key_frame_iter": args.c4_steps,
               "outer_iterations": sst.iterations, "solve_s": round(sst.solve_ms / 1e3, 3),
-              "cell_updates_per_s": round(cups, 1), "per_case_latency_s": round(wall, 3),
+              "cell_updates_per_s": round(cups, 1),
+              "case_latency_s": {"p50": round(float(np.percentile(lat, 50)), 4),
+                                 "p95": round(float(np.percentile(lat, 95)), 4),
+                                 "max": round(float(lat.max()), 4),
+                                 "definition": "iterations of the run until the case stopped in its last time "
+                                               "step x the run's mean iteration time"},
+              "case_iterations": {"p50": float(np.percentile(active, 50)), "p95": float(np.percentile(active, 95)),
+                                  "min": int(active.min()), "max": int(active.max())},
+              "cases_converged_last_step": int(sum(k.converged for k in cst)),
               "mg_levels": sst.mg_levels,
+              "work_split": {"sweeps_per_launch": sst.sweeps_per_launch, "sweep_block_cells": sst.sweep_block_cells,
+                             "autotuned": c4tune is not None},
               "autotune_ms": None if c4tune is None else [round(c4tune[0], 4), round(c4tune[1], 4)]}
         del sb
 
     cpu = None
     if not args.no_cpu_baseline:
+        import oracle_binding
         from oracle_binding import Oracle, set_threads
 
+        flags = native_oracle(oracle_binding)
         cfg = case.config
         cfg.max_iter = args.cpu_iters
 
@@ -210,13 +256,28 @@ def main() -> int:
         itn, dtn, used = oracle_rate(avail)
         set_threads(1)
         what = (f"outer iterations of the same {m.n_cells}-cell nozzle case from a cold start, FP64 oracle "
-                f"(oracle/oracle.cpp; OpenMP over its per-cell and per-face loops, reductions and multigrid "
-                f"transfers serial)")
+                f"(oracle/oracle.cpp built for this host; OpenMP over its per-cell and per-face loops, the "
+                f"multigrid transfers per coarse cell and the residual sums in fixed chunks)")
+        # the pipe (128 x 32) to its 1e-6 residual on the host, one thread and all
+        c1 = {}
+        for th in (1, avail):
+            used1 = set_threads(th)
+            pc = axs.pipe_case()
+            o = Oracle(pc.mesh, pc.bcs, pc.config)
+            tc = time.time()
+            o.run()
+            d = time.time() - tc
+            c1[f"threads_{used1}"] = {"iterations": o.iterations, "converged": bool(o.converged),
+                                      "wall_s": round(d, 3), "value": round(o.iterations / d, 2)}
+        set_threads(1)
         cpu = {"value": round(itn / dtn, 4), "unit": "outer_iter/s", "cores": used, "kind": "port",
-               "sample": f"{itn} {what}, {used} threads, {dtn:.1f} s",
+               "sample": f"{itn} {what}, {used} threads, {dtn:.1f} s", "iterations": itn,
+               "compiler_flags": flags,
                "one_thread": {"value": round(it1 / dt1, 4), "cores": 1, "sample": f"{it1} of the same, {dt1:.1f} s"},
+               "c1": {"workload": "pipe 128x32 (4096 cells) from a cold start to the 1e-6 residual", **c1},
                "host_cpu": _cpu_model(), "host_threads": os.cpu_count()}
-        log(f"cpu baseline: {cpu['value']} it/s on {used} threads, {cpu['one_thread']['value']} it/s on one")
+        log(f"cpu baseline: {cpu['value']} it/s on {used} threads, {cpu['one_thread']['value']} it/s on one; "
+            f"pipe to 1e-6: {c1}")
 
     out = {
         "metric": "SIMPLE outer iter/s at 1M cells (unstructured nozzle, non-orthogonal correction, agglomeration MG)",
@@ -253,11 +314,67 @@ def main() -> int:
                       "bytes_per_solve": mg_bytes, "cycles": case.config.mg_max_iter},
         "autotune": tune,
         "mixed_precision": mixed,
+        "to_tolerance": to_tol,
         "batched_sweep": c4,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def converge_pair(make, cap: int) -> dict:
+    """One case from a cold start to its residual tolerance (at most `cap`
+    iterations), FP64 and mixed: iterations, wall time and rate of each, and
+    the mixed fields' relative L2 error against the FP64 fields."""
+    import numpy as np
+
+    import axs
+
+    out, fields = {}, {}
+    for name, prec in (("fp64", axs.PREC_FP64), ("mixed", axs.PREC_MIXED)):
+        case = make(max_iter=cap, precision=prec)
+        s = axs.make_solver(case)
+        t = time.time()
+        s.run()
+        wall = time.time() - t
+        st = s.stats()
+        out[name] = {"iterations": st.iterations, "converged": bool(st.converged), "cap_hit": st.iterations >= cap,
+                     "wall_s": round(wall, 3), "solve_s": round(st.solve_ms / 1e3, 3),
+                     "it_per_s": round(st.iterations / (st.solve_ms / 1e3), 2),
+                     "last_residual": [float(f"{x:.4g}") for x in st.last_residual[:3]]}
+        fields[name] = {f: s.field(f) for f in ("u", "v", "p")}
+        log(f"{case.name} {name}: {st.iterations} iterations, converged {bool(st.converged)}, {wall:.2f} s")
+        del s
+    out["cells"] = case.mesh.n_cells
+    out["mixed_rel_l2_vs_fp64"] = {
+        f: float(np.linalg.norm(fields["mixed"][f] - fields["fp64"][f]) / max(np.linalg.norm(fields["fp64"][f]), 1e-300))
+        for f in ("u", "v", "p")}
+    out["mixed_speedup"] = round(out["fp64"]["solve_s"] / out["mixed"]["solve_s"], 3)
+    return out
+
+
+def native_oracle(binding) -> str:
+    """Build oracle/oracle.cpp for this host's CPU (-O3 -march=native) into a
+    temporary directory and load it; the flags used (the prebuilt library's
+    when the compiler is missing or fails)."""
+    import shutil
+    import subprocess
+    import tempfile
+
+    sys.path.insert(0, str(ROOT / "axisim-software_amd"))
+    from build_lib import ORACLE_FLAGS
+
+    flags = ORACLE_FLAGS + ["-march=native"]
+    if shutil.which("g++") is None:
+        return " ".join(ORACLE_FLAGS) + " (prebuilt)"
+    out = Path(tempfile.mkdtemp(prefix="axs_oracle_")) / "liboracle_native.so"
+    r = subprocess.run(["g++", *flags, "-fPIC", "-shared", f"-I{ROOT / 'include'}", str(ROOT / "oracle" / "oracle.cpp"),
+                        "-o", str(out)], capture_output=True, text=True)
+    if r.returncode != 0:
+        log(f"native oracle build failed, using the prebuilt one: {r.stderr[-300:]}")
+        return " ".join(ORACLE_FLAGS) + " (prebuilt)"
+    binding.use_library(out)
+    return " ".join(flags)
 
 
 TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_nozzle1M_pmc.txt"

@@ -271,6 +271,8 @@ typedef struct axs_case_stats {
     int iterations;                       /* outer iterations the case ran (all steps) */
     int converged;                        /* in its last time step                   */
     double last_residual[AXS_RES_COUNT];  /* at its last residual check              */
+    int stop_iteration;                   /* the run's iteration count (axs_stats
+                                             iterations) when it last stopped    */
 } axs_case_stats;
 axs_status axs_get_case_stats(axs_solver* s, int c, axs_case_stats* out);
 

@@ -195,6 +195,10 @@ int buildColoring(int N, const int* fs, const int* fn, std::vector<int>& colorSt
 struct Grid {
     int nCells = 0;
     std::vector<int> faceStart, faceNeighbor, cellToCoarse, fineSlotToCoarseSlot;
+    // the fine cells of each coarse cell in ascending order (CSR over the
+    // coarse cells): restriction and the coarse operator gather per coarse
+    // cell in that order, which is the order of a serial scatter over fine cells
+    std::vector<int> memberStart, members;
     bool isCoarsest() const { return cellToCoarse.empty(); }
     int nRefs() const { return (int)faceNeighbor.size(); }
 };
@@ -394,21 +398,30 @@ void residualSigned(const Coeff& c, const std::vector<double>& x, std::vector<do
 
 // buildCoarseOperatorKernel, multigrid.cu:350-389 (serial scatter in fine-cell
 // order; the reference's atomicAdd order is unspecified)
+// Gathered per coarse cell over its fine cells in ascending order: every sum
+// is added in the serial scatter's order (a fine cell's slots map into its
+// coarse cell's row), so the result does not depend on the thread count.
 void buildCoarseOperator(const MGLevel& fine, MGLevel& coarse) {
-    std::fill(coarse.coeff.AC.begin(), coarse.coeff.AC.end(), 0.0);
-    std::fill(coarse.coeff.AF.begin(), coarse.coeff.AF.end(), 0.0);
     const Coeff& f = fine.coeff;
-    for (int n = 0; n < f.N; n++) {
-        const int cn = fine.grid.cellToCoarse[n];
-        double diag = f.AC[n];
-        for (int k = f.faceStart[n]; k < f.faceStart[n + 1]; k++) {
-            const int nb = f.faceNeighbor[k];
-            if (nb < 0) continue;
-            const int slot = fine.grid.fineSlotToCoarseSlot[k];
-            if (slot >= 0) coarse.coeff.AF[slot] += f.AF[k];
-            else diag += f.AF[k];
+    const Grid& g = fine.grid;
+    Coeff& cc = coarse.coeff;
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < cc.N; c++) {
+        for (int k = cc.faceStart[c]; k < cc.faceStart[c + 1]; k++) cc.AF[k] = 0.0;
+        double ac = 0.0;
+        for (int j = g.memberStart[c]; j < g.memberStart[c + 1]; j++) {
+            const int n = g.members[j];
+            double diag = f.AC[n];
+            for (int k = f.faceStart[n]; k < f.faceStart[n + 1]; k++) {
+                const int nb = f.faceNeighbor[k];
+                if (nb < 0) continue;
+                const int slot = g.fineSlotToCoarseSlot[k];
+                if (slot >= 0) cc.AF[slot] += f.AF[k];
+                else diag += f.AF[k];
+            }
+            ac += diag;
         }
-        coarse.coeff.AC[cn] += diag;
+        cc.AC[c] = ac;
     }
 }
 
@@ -426,6 +439,17 @@ struct Multigrid {
         for (Grid& gr : grids) {
             MGLevel L;
             L.grid = std::move(gr);
+            if (!L.grid.isCoarsest()) {
+                Grid& q = L.grid;
+                int nc = 0;
+                for (int c : q.cellToCoarse) nc = std::max(nc, c + 1);
+                q.memberStart.assign(nc + 1, 0);
+                for (int c : q.cellToCoarse) q.memberStart[c + 1]++;
+                for (int c = 0; c < nc; c++) q.memberStart[c + 1] += q.memberStart[c];
+                q.members.resize(q.nCells);
+                std::vector<int> fill(q.memberStart.begin(), q.memberStart.end() - 1);
+                for (int n = 0; n < q.nCells; n++) q.members[fill[q.cellToCoarse[n]]++] = n;
+            }
             const int n = L.grid.nCells;
             L.coeff.N = n;
             L.coeff.nRefs = L.grid.nRefs();
@@ -458,12 +482,19 @@ struct Multigrid {
         residualSigned(L.coeff, L.x, L.res);
         MGLevel& C = levels[l + 1];
         // buildRestriction, multigrid.cu:398-410, 568-582
-        std::fill(C.coeff.b.begin(), C.coeff.b.end(), 0.0);
-        for (int n = 0; n < L.grid.nCells; n++) C.coeff.b[L.grid.cellToCoarse[n]] += L.res[n];
-        std::fill(C.x.begin(), C.x.end(), 0.0);
+        // (gathered per coarse cell in ascending fine order, as a serial scatter adds)
+        const Grid& g = L.grid;
+#pragma omp parallel for schedule(static)
+        for (int c = 0; c < C.grid.nCells; c++) {
+            double b = 0.0;
+            for (int j = g.memberStart[c]; j < g.memberStart[c + 1]; j++) b += L.res[g.members[j]];
+            C.coeff.b[c] = b;
+            C.x[c] = 0.0;
+        }
         vCycle(l + 1);
         // buildProlongationKernel, multigrid.cu:415-427
-        for (int n = 0; n < L.grid.nCells; n++) L.x[n] += C.x[L.grid.cellToCoarse[n]];
+#pragma omp parallel for schedule(static)
+        for (int n = 0; n < g.nCells; n++) L.x[n] += C.x[g.cellToCoarse[n]];
         smooth(cfg.post);
     }
 
@@ -1066,25 +1097,40 @@ struct ResCfg {
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// residualAllHost for one residual, residuals.cu:37-100.  The sums run
-// sequentially (the reference reduces with a 256-wide tree).
+// residualAllHost for one residual, residuals.cu:37-100.  The sums run in a
+// fixed order of chunks (the reference reduces with a 256-wide tree).
 // ---------------------------------------------------------------------------
+
+// Sums of term(i) over [0, N): sequential within chunks of 32768 terms, the
+// chunks' partial sums (computed in parallel) added in chunk order, so the
+// value is the same for every thread count.
+template <class Term>
+double orderedSum(int N, Term term) {
+    constexpr int kChunk = 1 << 15;
+    const int nChunks = (N + kChunk - 1) / kChunk;
+    std::vector<double> part(nChunks > 0 ? nChunks : 1, 0.0);
+#pragma omp parallel for schedule(static) if (nChunks > 1)
+    for (int c = 0; c < nChunks; c++) {
+        double s = 0.0;
+        const int end = std::min(N, (c + 1) * kChunk);
+        for (int i = c * kChunk; i < end; i++) s += term(i);
+        part[c] = s;
+    }
+    double s = 0.0;
+    for (int c = 0; c < nChunks; c++) s += part[c];
+    return s;
+}
+
 extern "C" void orc_residual_value(int isContinuity, int norm, int scaling, const double* res,
                                    const double* scale, int N, int scaleIteration,
                                    double* scaleVal, double* resVal) {
-    auto sumAbs = [&](const double* a) {
-        double s = 0.0;
-        for (int i = 0; i < N; i++) s += std::fabs(a[i]);
-        return s;
-    };
+    auto sumAbs = [&](const double* a) { return orderedSum(N, [a](int i) { return std::fabs(a[i]); }); };
     if (isContinuity) {
         *resVal = sumAbs(res);
     } else {
         if (norm == AXS_NORM_L1) *resVal = sumAbs(res);
         else if (norm == AXS_NORM_L2) {
-            double s = 0.0;
-            for (int i = 0; i < N; i++) s += res[i] * res[i];
-            *resVal = s;
+            *resVal = orderedSum(N, [res](int i) { return res[i] * res[i]; });
         } else {
             double mx = -INFINITY;
             for (int i = 0; i < N; i++) mx = std::fmax(mx, std::fabs(res[i]));

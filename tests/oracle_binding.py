@@ -90,6 +90,15 @@ def _i(a):
     return a.ctypes.data_as(_ip)
 
 
+def use_library(path) -> None:
+    """Load the oracle from another build of oracle/oracle.cpp (bench.py's
+    host-tuned CPU baseline); before the first call into the oracle."""
+    global ORACLE_LIB
+    if _L is not None:
+        raise RuntimeError("the oracle library is already loaded")
+    ORACLE_LIB = Path(path)
+
+
 def set_threads(n: int) -> int:
     """Threads of the oracle's per-cell / per-face loops (results do not depend
     on it); returns the count in effect."""

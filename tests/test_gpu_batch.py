@@ -98,6 +98,7 @@ def test_per_case_convergence_matches_each_case_alone(require_gpu):
         st = sb.case_stats(i)
         iters.append((st.iterations, s.stats().iterations))
         assert st.converged and st.iterations == s.stats().iterations, (i, st.iterations, s.stats().iterations)
+        assert st.stop_iteration == st.iterations  # a steady run from iteration 0
         np.testing.assert_allclose(np.array(st.last_residual), s.history(0)[-1:].tolist() + [
             s.history(r)[-1] for r in range(1, axs.RES_COUNT)], rtol=0, atol=0)
         for name in ("u", "v", "p", "pp"):
@@ -148,5 +149,7 @@ def test_per_case_transient_steps(require_gpu):
     for i, (c, (first, n)) in enumerate(zip(cases, batch.info["ranges"])):
         s = _alone(c, batch.config)
         assert sb.case_stats(i).iterations == s.stats().iterations, i
+        # it stopped within the last time step
+        assert sb.stats().iterations - 40 < sb.case_stats(i).stop_iteration <= sb.stats().iterations
         for name in ("u", "v", "p", "u_old", "u_old2"):
             np.testing.assert_array_equal(sb.field(name)[first:first + n], s.field(name), err_msg=f"case {i} {name}")
