@@ -168,6 +168,7 @@ def load_library(path: os.PathLike | None = None) -> C.CDLL:
         "axs_run": (C.c_int, [S, PROGRESS_CB, C.c_void_p, C.c_int]),
         "axs_run_timed": (C.c_int, [S, C.c_int, _dp]),
         "axs_bench_kernel": (C.c_int, [S, C.c_int, C.c_int, _dp, _dp]),
+        "axs_bench_kernel_cold": (C.c_int, [S, C.c_int, C.c_int, _dp, _dp]),
         "axs_autotune": (C.c_int, [S, C.c_int, _dp, _dp]),
         "axs_set_sweep_fusion": (C.c_int, [S, C.c_int, C.c_int]),
         "axs_request_stop": (None, [S]),
@@ -590,10 +591,12 @@ class Solver:
         self._net = net
         _check(lib().axs_set_learned_guess(self._h, net._h if net is not None else None, level), self._h)
 
-    def bench_kernel(self, kernel: int, repeats: int = 50) -> tuple[float, float]:
-        """(average launch ms, algorithmic bytes per launch) of one kernel."""
+    def bench_kernel(self, kernel: int, repeats: int = 50, cold: bool = False) -> tuple[float, float]:
+        """(average launch ms, algorithmic bytes per launch) of one kernel;
+        cold: each launch after 512 MiB of writes have flushed the caches."""
         ms, by = C.c_double(), C.c_double()
-        _check(lib().axs_bench_kernel(self._h, kernel, repeats, C.byref(ms), C.byref(by)), self._h)
+        fn = lib().axs_bench_kernel_cold if cold else lib().axs_bench_kernel
+        _check(fn(self._h, kernel, repeats, C.byref(ms), C.byref(by)), self._h)
         return ms.value, by.value
 
     def request_stop(self) -> None:

@@ -252,6 +252,9 @@ void launchRotate(hipStream_t s, int n, double* old2, double* old, const double*
 void launchStepBegin(hipStream_t s, const DMesh& m, DState st, int bdf2);
 void launchCopy(hipStream_t s, double* dst, const double* src, int n, const Ctl* ctl);
 void launchFill(hipStream_t s, double* dst, double value, int n, const Ctl* ctl);
+// streams src[0 .. n) through the caches; out[b] gets workgroup b's partial
+// sum only where it equals `mark`
+void launchStreamRead(hipStream_t s, const double* src, size_t n, double* out, int blocks, double mark);
 
 // multigrid (FP64)
 void launchMGJacobi(hipStream_t s, const DLevel& L, double w, const Ctl* ctl);

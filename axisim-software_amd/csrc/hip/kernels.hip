@@ -1638,6 +1638,17 @@ __global__ void __launch_bounds__(kBlock) kFill(double* dst, double value, int n
     if (i < n) dst[i] = value;
 }
 
+// Streams n doubles through the caches (a benchmark's cache flush): each
+// thread sums a grid-stride share; a sum equal to `mark` (never, for the
+// filled buffer the caller passes) is stored, so the loads stay.
+__global__ void __launch_bounds__(kBlock) kStreamRead(const double* __restrict__ src, size_t n, double* out,
+                                                      double mark) {
+    double acc = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        acc += src[i];
+    if (acc == mark) out[blockIdx.x] = acc;
+}
+
 // ===========================================================================
 // Agglomeration multigrid, FP64 (multigrid.cu:350-541)
 // ===========================================================================
@@ -2890,6 +2901,9 @@ void launchCopy(hipStream_t s, double* dst, const double* src, int n, const Ctl*
 }
 void launchFill(hipStream_t s, double* dst, double value, int n, const Ctl* ctl) {
     kFill<<<blocksFor(n), kBlock, 0, s>>>(dst, value, n, ctl);
+}
+void launchStreamRead(hipStream_t s, const double* src, size_t n, double* out, int blocks, double mark) {
+    kStreamRead<<<blocks, kBlock, 0, s>>>(src, n, out, mark);
 }
 void launchMGJacobi(hipStream_t s, const DLevel& L, double w, const Ctl* ctl) { launchMGJacobiT<double>(s, L, w, ctl); }
 void launchMGSmoothBlock(hipStream_t s, const DLevel& L, double w, int sweeps, const Ctl* ctl) {

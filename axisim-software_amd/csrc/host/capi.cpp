@@ -144,6 +144,11 @@ axs_status axs_bench_kernel(axs_solver* h, int kernel, int repeats, double* ms, 
     return guarded(&h->s->lastError, [&] { h->s->benchKernel(kernel, repeats, ms, bytes); });
 }
 
+axs_status axs_bench_kernel_cold(axs_solver* h, int kernel, int repeats, double* ms, double* bytes) {
+    if (!h || !ms || !bytes) return AXS_ERR_INVALID;
+    return guarded(&h->s->lastError, [&] { h->s->benchKernel(kernel, repeats, ms, bytes, true); });
+}
+
 axs_status axs_autotune(axs_solver* h, int repeats, double* ms_default, double* ms_tuned) {
     if (!h) return AXS_ERR_INVALID;
     return guarded(&h->s->lastError, [&] { h->s->autotune(repeats, ms_default, ms_tuned); });

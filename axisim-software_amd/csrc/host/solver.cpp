@@ -1672,7 +1672,7 @@ void Solver::runTimed(int n, double* ms) {
 // (and from the same inputs, so with the same values); the face flux
 // overwrites the mass flux, which is state carried between iterations, so it
 // is saved before the launches and restored after them.
-void Solver::benchKernel(int kernel, int repeats, double* ms, double* bytes) {
+void Solver::benchKernel(int kernel, int repeats, double* ms, double* bytes, bool cold) {
     if (!ready_) throw std::logic_error("run the solver once before timing a kernel");
     if (repeats <= 0) throw std::invalid_argument("repeats must be positive");
     AXS_HIP(hipSetDevice(device_));
@@ -1738,17 +1738,49 @@ void Solver::benchKernel(int kernel, int repeats, double* ms, double* bytes) {
         AXS_HIP(hipMemcpyAsync(mDotSaved, ds_.mDot, (size_t)dm_.F * sizeof(double), hipMemcpyDeviceToDevice, stream_));
     }
     launch();  // warm-up
-    hipEvent_t e0, e1;
-    AXS_HIP(hipEventCreate(&e0));
-    AXS_HIP(hipEventCreate(&e1));
-    AXS_HIP(hipEventRecord(e0, stream_));
-    for (int i = 0; i < repeats; i++) launch();
-    AXS_HIP(hipEventRecord(e1, stream_));
-    AXS_HIP(hipEventSynchronize(e1));
     float f = 0.0f;
-    AXS_HIP(hipEventElapsedTime(&f, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    if (!cold) {
+        hipEvent_t e0, e1;
+        AXS_HIP(hipEventCreate(&e0));
+        AXS_HIP(hipEventCreate(&e1));
+        AXS_HIP(hipEventRecord(e0, stream_));
+        for (int i = 0; i < repeats; i++) launch();
+        AXS_HIP(hipEventRecord(e1, stream_));
+        AXS_HIP(hipEventSynchronize(e1));
+        AXS_HIP(hipEventElapsedTime(&f, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    } else {
+        // cold caches: a kernel reads 512 MiB (more than the 4 MB L2 of each
+        // XCD and the 256 MB last-level cache hold) before each launch, so
+        // the launch finds its inputs in HBM only; the launch's time is what
+        // `repeats` (flush, launch) pairs take beyond `repeats` flushes alone,
+        // both back to back on the stream
+        const size_t flushN = (size_t)64 << 20;  // doubles
+        double* flush = saved.alloc<double>(flushN);
+        double* sink = saved.alloc<double>(2048);
+        launchFill(stream_, flush, 0.0, (int)flushN, nullptr);
+        auto timed = [&](bool withKernel) {
+            hipEvent_t e0, e1;
+            AXS_HIP(hipEventCreate(&e0));
+            AXS_HIP(hipEventCreate(&e1));
+            launchStreamRead(stream_, flush, flushN, sink, 2048, -1.0);
+            AXS_HIP(hipEventRecord(e0, stream_));
+            for (int i = 0; i < repeats; i++) {
+                launchStreamRead(stream_, flush, flushN, sink, 2048, -1.0);
+                if (withKernel) launch();
+            }
+            AXS_HIP(hipEventRecord(e1, stream_));
+            AXS_HIP(hipEventSynchronize(e1));
+            float t = 0.0f;
+            AXS_HIP(hipEventElapsedTime(&t, e0, e1));
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            return t;
+        };
+        const float flushOnly = timed(false);
+        f = std::max(0.0f, timed(true) - flushOnly);
+    }
     if (mDotSaved) {
         AXS_HIP(hipMemcpyAsync(ds_.mDot, mDotSaved, (size_t)dm_.F * sizeof(double), hipMemcpyDeviceToDevice, stream_));
         AXS_HIP(hipStreamSynchronize(stream_));

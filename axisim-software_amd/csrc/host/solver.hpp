@@ -181,7 +181,7 @@ public:
     void configure(const axs_config& c);
     void run(axs_progress_cb cb, void* user, bool continueRun);
     void runTimed(int n, double* ms);
-    void benchKernel(int kernel, int repeats, double* ms, double* bytes);
+    void benchKernel(int kernel, int repeats, double* ms, double* bytes, bool cold = false);
     // Workgroup sizes of the momentum sweep and of every multigrid level's
     // per-cell kernels, each chosen from 64 / 128 / 256 threads by timing them on
     // the current state (after a run).  *msDefault / *msTuned: summed kernel

@@ -320,6 +320,11 @@ enum {
     AXS_KERNEL_JACOBI_UV_FUSED = 5  /* the fused launch of several u+v sweeps  */
 };
 axs_status axs_bench_kernel(axs_solver* s, int kernel, int repeats, double* ms, double* bytes);
+/* The same kernel timed with cold caches: before each launch a kernel reads
+ * 512 MiB (more than the L2s and the last-level cache hold); *ms = the time
+ * `repeats` (flush, launch) pairs take beyond `repeats` flushes, divided by
+ * `repeats`.  Allocates the 512 MiB for the call. */
+axs_status axs_bench_kernel_cold(axs_solver* s, int kernel, int repeats, double* ms, double* bytes);
 /* Autotuner (build extension): choose the workgroup size (64, 128 or 256
  * threads) of the momentum sweep and of each multigrid level's per-cell kernels
  * by timing them on the solver's current state; call after a run, the next run
