@@ -67,6 +67,7 @@ def main() -> int:
     ap.add_argument("--no-converge", action="store_true",
                     help="skip the runs to the 1e-6 residual (nozzle, pipe and expansion, FP64 and mixed)")
     ap.add_argument("--converge-cap", type=int, default=60000, help="iteration cap of the runs to 1e-6")
+    ap.add_argument("--no-guess", action="store_true", help="skip the learned-guess run to 1e-6")
     ap.add_argument("--expansion-cap", type=int, default=20000,
                     help="iteration cap of the 238,717-cell expansion runs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -179,14 +180,50 @@ def main() -> int:
     # workload), the pipe and the expansion, each FP64 and mixed, with the
     # per-field error of the mixed run against the FP64 run
     to_tol = None
+    nozzle_fp64 = {}  # the nozzle's converged FP64 fields
     if not args.no_converge:
         to_tol = {
             "tolerance": 1e-6,
             "nozzle_1M": converge_pair(lambda **kw: axs.nozzle_case(mesh=m, mg_max_levels=args.mg_levels, **kw),
-                                       args.converge_cap),
+                                       args.converge_cap, keep=nozzle_fp64),
             "pipe_128x32": converge_pair(lambda **kw: axs.pipe_case(**kw), args.converge_cap),
             "expansion_238k": converge_pair(lambda **kw: axs.expansion_case(**kw), args.expansion_cap),
         }
+
+    # the learned initial guess (weights trained by train_guess.py on other
+    # nozzles and the expansion): the nozzle to 1e-6 on the mixed path with the
+    # guess before each outer iteration's first p' solve
+    guess = None
+    weights = ROOT / "axisim-software_amd" / "data" / "guess_weights.npy"
+    if not args.no_guess and not args.no_converge and weights.exists():
+        net = axs.Net(layers=4)
+        net.set_params(np.load(weights))
+        gcase = axs.nozzle_case(mesh=m, mg_max_levels=args.mg_levels, max_iter=args.converge_cap,
+                                precision=axs.PREC_MIXED)
+        sg = axs.make_solver(gcase)
+        sg.set_learned_guess(net, -1)
+        t = time.time()
+        sg.run()
+        wall = time.time() - t
+        gst = sg.stats()
+        plain, mixed = to_tol["nozzle_1M"]["fp64"], to_tol["nozzle_1M"]["mixed"]
+        err = {}
+        for f in ("u", "v", "p"):
+            ref = nozzle_fp64[f]
+            err[f] = float(np.linalg.norm(sg.field(f) - ref) / max(np.linalg.norm(ref), 1e-300))
+        guess = {"workload": "the 1M nozzle from a cold start to 1e-6, mixed precision with the learned guess",
+                 "weights": "axisim-software_amd/data/guess_weights.npy (train_guess.py)",
+                 "iterations": gst.iterations, "converged": bool(gst.converged), "wall_s": round(wall, 3),
+                 "solve_s": round(gst.solve_ms / 1e3, 3),
+                 "iterations_saved_vs_mixed": mixed["iterations"] - gst.iterations,
+                 "wall_vs_fp64": round(gst.solve_ms / 1e3 / plain["solve_s"], 4),
+                 "wall_vs_mixed": round(gst.solve_ms / 1e3 / mixed["solve_s"], 4),
+                 "plain_fp64": {"iterations": plain["iterations"], "solve_s": plain["solve_s"]},
+                 "mixed": {"iterations": mixed["iterations"], "solve_s": mixed["solve_s"]},
+                 "rel_l2_vs_fp64": err}
+        log(f"learned guess: {gst.iterations} iterations to 1e-6 in {wall:.2f} s (mixed {mixed['iterations']}, "
+            f"fp64 {plain['iterations']})")
+        del sg, net
 
     # batched sweep: many cases of uneven size in one solver, every launch
     # covers all; each case stops on its own residuals in every time step
@@ -328,6 +365,7 @@ def main() -> int:
         "autotune": tune,
         "mixed_precision": mixed,
         "to_tolerance": to_tol,
+        "learned_guess": guess,
         "batched_sweep": c4,
         "cpu_baseline": cpu,
     }
@@ -335,10 +373,11 @@ def main() -> int:
     return 0
 
 
-def converge_pair(make, cap: int) -> dict:
+def converge_pair(make, cap: int, keep: dict | None = None) -> dict:
     """One case from a cold start to its residual tolerance (at most `cap`
     iterations), FP64 and mixed: iterations, wall time and rate of each, and
-    the mixed fields' relative L2 error against the FP64 fields."""
+    the mixed fields' relative L2 error against the FP64 fields (kept in
+    `keep` when given)."""
     import numpy as np
 
     import axs
@@ -358,6 +397,8 @@ def converge_pair(make, cap: int) -> dict:
         fields[name] = {f: s.field(f) for f in ("u", "v", "p")}
         log(f"{case.name} {name}: {st.iterations} iterations, converged {bool(st.converged)}, {wall:.2f} s")
         del s
+    if keep is not None:
+        keep.update(fields["fp64"])
     out["cells"] = case.mesh.n_cells
     out["mixed_rel_l2_vs_fp64"] = {
         f: float(np.linalg.norm(fields["mixed"][f] - fields["fp64"][f]) / max(np.linalg.norm(fields["fp64"][f]), 1e-300))

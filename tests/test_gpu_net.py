@@ -107,18 +107,26 @@ def test_training_gain_on_held_out_systems(require_gpu):
     assert after > before
 
 
-def test_guess_changes_iterations_not_the_answer(require_gpu):
-    """C1 with the learned guess converges to the same fields as without it."""
+def test_trained_guess_changes_iterations_not_the_answer(require_gpu):
+    """A network trained on seeded nozzles' p' systems, then used as the initial
+    guess of every outer iteration's first p' solve on a nozzle it has not seen:
+    both runs converge to 1e-6 and end on the same fields (to that tolerance);
+    the iteration counts are reported."""
     net = axs.Net(layers=4, seed=3)
-    case = axs.pipe_case()
-    s0 = axs.make_solver(case)
+    for q in _pairs(range(5000, 5006)):
+        net.add_sample(*q)
+    losses = net.train(150, batch=8, lr=1e-3, seed=0)
+    assert losses[-1] < losses[0]
+    mesh = axs.nozzle_mesh(axs.nozzle_spacing_for(6000), seed_points=5301, seed_jitter=5302)
+    s0 = axs.make_solver(axs.nozzle_case(mesh=mesh))
     s0.run()
-    s1 = axs.make_solver(axs.pipe_case())
+    s1 = axs.make_solver(axs.nozzle_case(mesh=mesh))
     s1.set_learned_guess(net, -1)
     s1.run()
     st0, st1 = s0.stats(), s1.stats()
-    print("iterations plain", st0.iterations, "with guess", st1.iterations)
+    print(f"train loss {losses[0]:.4f} -> {losses[-1]:.4f}; outer iterations to 1e-6: plain {st0.iterations}, "
+          f"with the trained guess {st1.iterations}")
     assert st0.converged and st1.converged
-    for name in ("u", "v", "p"):
-        a, r = s1.field(name), s0.field(name)
-        assert np.linalg.norm(a - r) / np.linalg.norm(r) < 1e-4, name
+    uscale = np.linalg.norm(s0.field("u"))
+    for name, scale in (("u", uscale), ("v", uscale), ("p", np.linalg.norm(s0.field("p")))):
+        assert np.linalg.norm(s1.field(name) - s0.field(name)) <= 1e-5 * scale, name
