@@ -180,6 +180,7 @@ struct TailLevel {
     int n, refs, nc;
     int AC, b, x0, x1, AF;
     int start, nb, toC, mS, mem;
+    int perm;  // the coarsest level: cells in ascending slot count (thread t sweeps cell perm[t]); -1: none
 };
 constexpr int kTailLevels = 8;         // levels the tail kernel may hold
 constexpr int kTailStaticLds = 2048;   // LDS reserved next to the tail's dynamic image

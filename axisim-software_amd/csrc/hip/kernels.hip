@@ -2471,8 +2471,11 @@ __global__ void __launch_bounds__(1024) kMGTail(const TailLevel* __restrict__ tl
         const T* AF = sT + S.AF;
         const int* nbs = sI + S.nb;
         const bool own = t < S.n;
+        // thread t sweeps cell perm[t] where the level has an order (the
+        // coarsest, above one wave: cells by slot count), else cell t
+        const int cell = (own && S.perm >= 0) ? sI[S.perm + t] : t;
         TRow<T> r;
-        if (own) loadRow(r, sT + S.AC, sT + S.b, AF, sI + S.start, nbs, t);
+        if (own) loadRow(r, sT + S.AC, sT + S.b, AF, sI + S.start, nbs, cell);
         if (S.n <= 64) {
             // a level of at most one wave's cells (the coarsest, usually):
             // wave 0 sweeps alone; a wave's LDS accesses complete in program
@@ -2514,19 +2517,34 @@ __global__ void __launch_bounds__(1024) kMGTail(const TailLevel* __restrict__ tl
             __syncthreads();
             return;
         }
+        // Each wave reads as many slots as its longest row has (4, 6, 8 or
+        // kRowK; a longer row loops), so short rows do not pay for long ones;
+        // the row's arithmetic is the same whichever thread does it.
+        const TRow<T>& rc = r;
+        const bool keep = own && tabs(rc.ac) < T(1.0e-30);
+        const bool le4 = __all(!own || rc.deg <= 4), le6 = __all(!own || rc.deg <= 6);
+        const bool le8 = __all(!own || rc.deg <= 8), leK = __all(!own || rc.deg <= kRowK);
         for (int it = 0; it < count; it++) {
             const T* x = X(S, q);
             T* y = Y(S, q);
             if (own) {
-                const T xn = x[t];
+                const T xn = x[cell];
                 T res;
-                if (tabs(r.ac) < T(1.0e-30)) {
+                if (keep) {
                     res = xn;
+                } else if (le4) {
+                    res = waveRowSweep<4>(rc, x, xn, w, cell);
+                } else if (le6) {
+                    res = waveRowSweep<6>(rc, x, xn, w, cell);
+                } else if (le8) {
+                    res = waveRowSweep<8>(rc, x, xn, w, cell);
+                } else if (leK) {
+                    res = waveRowSweep<kRowK>(rc, x, xn, w, cell);
                 } else {
-                    const T Ax = rowAx(r, xn, [&](int j) { return x[j]; }, AF, nbs, t);
-                    res = xn + w * (r.b - Ax) / r.ac;
+                    const T Ax = rowAx(rc, xn, [&](int j) { return x[j]; }, AF, nbs, cell);
+                    res = xn + w * (rc.b - Ax) / rc.ac;
                 }
-                y[t] = res;
+                y[cell] = res;
             }
             __syncthreads();
             flip ^= 1u << q;

@@ -13,6 +13,7 @@
 #include "net.hpp"
 
 #include <algorithm>
+#include <numeric>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -225,6 +226,15 @@ void MGDevice::placeInts(const TailPlan& t, int l, DLevel& d) {
     };
     d.start = put(e.start, h.start);
     d.nb = put(e.nb, h.nb);
+    if (e.perm >= 0) {
+        // the coarsest level's sweep order of threads: cells by slot count
+        // (stable), so each wave's rows are about equally long
+        std::vector<int> perm(h.n);
+        std::iota(perm.begin(), perm.end(), 0);
+        std::stable_sort(perm.begin(), perm.end(),
+                         [&](int a, int b) { return h.start[a + 1] - h.start[a] < h.start[b + 1] - h.start[b]; });
+        put(e.perm, perm);
+    }
     if (!h.coarsest()) {
         d.toCoarse = put(e.toC, h.toCoarse);
         d.memberStart = put(e.mS, h.memberStart);
@@ -266,6 +276,11 @@ MGDevice::TailPlan MGDevice::planTail(int lt, size_t vsize) const {
             e.toC = (int)ti; ti += h.n;
             e.mS = (int)ti; ti += e.nc + 1;
             e.mem = (int)ti; ti += h.n;
+        }
+        e.perm = -1;
+        if (last && h.n > 64) {
+            e.perm = (int)ti;
+            ti += h.n;
         }
         t.lv.push_back(e);
         maxN = std::max(maxN, h.n);
