@@ -1169,15 +1169,16 @@ __device__ __forceinline__ constexpr bool isMaxQuantity(int j) { return j == 2 |
 template <bool SC>
 __device__ void reducePartials(const double* partial, const double* ps, int b0, int blocks, double* tot) {
     constexpr int Q = SC ? 17 : 9;
+    constexpr int R = SC ? 1 : 2;  // blocks per thread and round (R * Q doubles in registers, no spill)
     __shared__ double red[kFinishThreads / 64][Q];
     const int t = threadIdx.x;
     double a[Q];
 #pragma unroll
     for (int j = 0; j < Q; j++) a[j] = 0.0;
-    for (int bb = 0; bb < blocks; bb += 4 * kFinishThreads) {
-        double x[4][Q];
+    for (int bb = 0; bb < blocks; bb += R * kFinishThreads) {
+        double x[R][Q];
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
+        for (int r = 0; r < R; r++) {
             const int b = b0 + min(bb + r * kFinishThreads + t, blocks - 1);
 #pragma unroll
             for (int j = 0; j < 9; j++) x[r][j] = partial[(size_t)b * 9 + j];
@@ -1185,7 +1186,7 @@ __device__ void reducePartials(const double* partial, const double* ps, int b0, 
             for (int j = 9; j < Q; j++) x[r][j] = ps[(size_t)b * 8 + (j - 9)];
         }
 #pragma unroll
-        for (int r = 0; r < 4; r++)
+        for (int r = 0; r < R; r++)
             if (bb + r * kFinishThreads + t < blocks)
 #pragma unroll
                 for (int j = 0; j < Q; j++) a[j] = isMaxQuantity(j) ? fmax(a[j], x[r][j]) : a[j] + x[r][j];
@@ -1244,17 +1245,29 @@ __device__ bool normaliseResiduals(const Ctl& cc, const double* tot, int N, int 
     return any && ok;
 }
 
+// The control block into LDS, one 32-bit word per thread (visible after the
+// caller's next barrier).
+__device__ __forceinline__ void copyCtl(Ctl& dst, const Ctl* src) {
+    static_assert(sizeof(Ctl) % 4 == 0, "the control block is copied in 32-bit words");
+    constexpr int W = (int)(sizeof(Ctl) / 4);
+    for (int i = threadIdx.x; i < W; i += blockDim.x) reinterpret_cast<int*>(&dst)[i] = reinterpret_cast<const int*>(src)[i];
+}
+
 template <bool SC>
 __global__ void __launch_bounds__(kFinishThreads) kFinish(DState s, int blocks) {
     Ctl* c = s.ctl;
-    // the whole control block in one round of loads; thread 0 works on this
-    // copy and stores back the fields it changes
-    Ctl cc = *c;
-    if (cc.done) return;
-    const bool check = (cc.k % cc.checkConv) == 0;
+    if (c->done) return;
+    const bool check = (c->k % c->checkConv) == 0;
+    // the control block copied into LDS by the workgroup (a copy in every
+    // thread's registers spilled); thread 0 works on it after the reduction's
+    // barriers and stores back the fields it changes
+    __shared__ Ctl sc;
     __shared__ double tot[17];
+    copyCtl(sc, c);
     if (check) reducePartials<SC>(s.partial, s.partial + (size_t)blocks * 9, 0, blocks, tot);
+    else __syncthreads();
     if (threadIdx.x != 0) return;
+    Ctl& cc = sc;
     if (check) {
         const int scaleIter = cc.transient ? cc.k : cc.iter;
         cc.converged = normaliseResiduals(cc, tot, cc.N, scaleIter, cc.resVal, cc.scaleVal) ? 1 : 0;
@@ -1284,16 +1297,19 @@ __global__ void __launch_bounds__(kFinishThreads) kFinish(DState s, int blocks) 
 // own convergence test.  A case already done this time step is left as it is.
 template <bool SC>
 __global__ void __launch_bounds__(kFinishThreads) kFinishCases(DState s, DCases cs, int blocks) {
-    const Ctl cc = *s.ctl;
-    if (cc.done) return;
+    if (s.ctl->done) return;
     const int ci = blockIdx.x;
     CaseCtl& k = cs.ctl[ci];
     if (k.done) return;  // uniform over the workgroup
-    const bool check = (cc.k % cc.checkConv) == 0;
+    const bool check = (s.ctl->k % s.ctl->checkConv) == 0;
+    __shared__ Ctl sc;  // as in kFinish
     __shared__ double tot[17];
+    copyCtl(sc, s.ctl);
     const int b0 = cs.blockStart[ci], nb = cs.blockStart[ci + 1] - b0;
     if (check) reducePartials<SC>(s.partial, s.partial + (size_t)blocks * 9, b0, nb, tot);
+    else __syncthreads();
     if (threadIdx.x != 0) return;
+    const Ctl& cc = sc;
     if (check) {
         const int scaleIter = cc.transient ? cc.k : cc.iter;
         double res[5], scale[5];

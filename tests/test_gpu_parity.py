@@ -200,6 +200,7 @@ def test_bench_kernel_leaves_the_state_alone(require_gpu):
         kernels.append(axs.KERNEL_JACOBI_UV_FUSED)
     for k in kernels:
         s1.bench_kernel(k, 3)
+        s1.bench_kernel(k, 2, cold=True)
     for k, a in before.items():
         np.testing.assert_array_equal(s1.field(k), a, err_msg=k)
     case.config.max_iter = 2
