@@ -454,7 +454,7 @@ def traffic_probe() -> int:
 
     import axs
 
-    case = axs.nozzle_case(max_iter=5)
+    case = axs.nozzle_case(max_iter=5, mg_max_levels=6)
     s = axs.make_solver(case)
     s.run()
     for k in (axs.KERNEL_JACOBI_UV, axs.KERNEL_JACOBI_UV_FUSED, axs.KERNEL_MOMENTUM, axs.KERNEL_FACE_FLUX,
