@@ -104,7 +104,7 @@ class Stats(C.Structure):
         ("n_colors", C.c_int), ("mg_levels", C.c_int), ("mg_level_cells", C.c_int * 32),
         ("setup_ms", C.c_double), ("solve_ms", C.c_double), ("last_residual", C.c_double * RES_COUNT),
         ("bytes_per_iteration", C.c_double), ("vcycle_bytes", C.c_double),
-        ("sweeps_per_launch", C.c_int), ("sweep_block_cells", C.c_int),
+        ("sweeps_per_launch", C.c_int), ("sweep_block_cells", C.c_int), ("graph_nodes", C.c_int),
     ]
 
 

@@ -1433,6 +1433,9 @@ void Solver::buildGraph() {
         throw;
     }
     AXS_HIP(hipStreamEndCapture(stream_, &graph_));
+    size_t nodes = 0;
+    AXS_HIP(hipGraphGetNodes(graph_, nullptr, &nodes));
+    graphNodes_ = (int)nodes;
     AXS_HIP(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
     AXS_HIP(hipGraphUpload(exec_, stream_));
 }
@@ -2340,6 +2343,7 @@ void Solver::stats(axs_stats& s) const {
     s.bytes_per_iteration = haveMesh_ ? bytesPerIteration() : 0.0;
     s.vcycle_bytes = mg_.host.empty() ? 0.0 : mg_.bytesPerSolve(mgParamsOf(cfg_), true);
     s.sweeps_per_launch = fuse_;
+    s.graph_nodes = graphNodes_;
     s.sweep_block_cells = fuse_ >= 2 ? sweepBlocks_.cells : 0;
 }
 

@@ -286,6 +286,7 @@ private:
     // and halo rings (host and device copies, built for fuseCellsReq_ and
     // fuse_ - 1 rings)
     int fuseReq_ = kMaxFusedSweeps, fuseCellsReq_ = 0, fuse_ = 1;
+    int graphNodes_ = 0;  // nodes of the captured iteration graph
     SweepBlocks sweepBlocks_;
     DSweepBlocks dsb_{};
     Arena sweepArena_;

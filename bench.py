@@ -172,6 +172,7 @@ def main() -> int:
             f"{pre_iters + args.steps} iterations {err}")
         mixed = {"value": round(mit_s, 3), "unit": "outer_iter/s", "ms_per_step": round(mms / args.steps, 5),
                  "speedup_vs_fp64": round(mit_s / it_s, 3), "dtype": "f32 sweeps/V-cycles, f64 defects",
+                 "graph_nodes_per_iteration": {"fp64": s.stats().graph_nodes, "mixed": sm.stats().graph_nodes},
                  "rel_l2_vs_fp64": err, "iterations_compared": pre_iters + args.steps,
                  "mg_solve_ms": round(mg32_ms, 4), "mg_solve_gbs": round(mg32_gbs, 1)}
         del sm
@@ -206,7 +207,7 @@ def main() -> int:
         sg.run()
         wall = time.time() - t
         gst = sg.stats()
-        plain, mixed = to_tol["nozzle_1M"]["fp64"], to_tol["nozzle_1M"]["mixed"]
+        plain, plain32 = to_tol["nozzle_1M"]["fp64"], to_tol["nozzle_1M"]["mixed"]
         err = {}
         for f in ("u", "v", "p"):
             ref = nozzle_fp64[f]
@@ -215,13 +216,13 @@ def main() -> int:
                  "weights": "axisim-software_amd/data/guess_weights.npy (train_guess.py)",
                  "iterations": gst.iterations, "converged": bool(gst.converged), "wall_s": round(wall, 3),
                  "solve_s": round(gst.solve_ms / 1e3, 3),
-                 "iterations_saved_vs_mixed": mixed["iterations"] - gst.iterations,
+                 "iterations_saved_vs_mixed": plain32["iterations"] - gst.iterations,
                  "wall_vs_fp64": round(gst.solve_ms / 1e3 / plain["solve_s"], 4),
-                 "wall_vs_mixed": round(gst.solve_ms / 1e3 / mixed["solve_s"], 4),
+                 "wall_vs_mixed": round(gst.solve_ms / 1e3 / plain32["solve_s"], 4),
                  "plain_fp64": {"iterations": plain["iterations"], "solve_s": plain["solve_s"]},
-                 "mixed": {"iterations": mixed["iterations"], "solve_s": mixed["solve_s"]},
+                 "mixed": {"iterations": plain32["iterations"], "solve_s": plain32["solve_s"]},
                  "rel_l2_vs_fp64": err}
-        log(f"learned guess: {gst.iterations} iterations to 1e-6 in {wall:.2f} s (mixed {mixed['iterations']}, "
+        log(f"learned guess: {gst.iterations} iterations to 1e-6 in {wall:.2f} s (mixed {plain32['iterations']}, "
             f"fp64 {plain['iterations']})")
         del sg, net
 
@@ -346,6 +347,7 @@ def main() -> int:
             "momentum_sweeps": case.config.linear_max_iter, "mg_cycles": case.config.mg_max_iter,
             "cell_updates_per_s": round(it_s * m.n_cells, 1),
             "bytes_per_iteration": st.bytes_per_iteration,
+            "graph_nodes_per_iteration": st.graph_nodes,
             "iteration_hbm_frac": round(st.bytes_per_iteration * it_s / 1e9 / HBM_PEAK_GBS, 4),
         },
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

@@ -299,6 +299,8 @@ typedef struct axs_stats {
     double vcycle_bytes;         /* algorithmic bytes of one p' multigrid solve    */
     int sweeps_per_launch;       /* momentum Jacobi sweeps fused per launch        */
     int sweep_block_cells;       /* cells per fused block (0: not fused)          */
+    int graph_nodes;             /* nodes of the outer-iteration graph (launches
+                                    and copies per iteration; 0 before a run)     */
 } axs_stats;
 axs_status axs_get_stats(axs_solver* s, axs_stats* out);
 
