@@ -183,6 +183,7 @@ def load_library(path: os.PathLike | None = None) -> C.CDLL:
         "axs_set_cases": (C.c_int, [S, C.c_int, _ip, _ip]),
         "axs_get_case_stats": (C.c_int, [S, C.c_int, C.POINTER(CaseStats)]),
         "axs_save_solution_npy": (C.c_int, [S, C.POINTER(CellPolygons), C.c_char_p]),
+        "axs_mesh_quality": (C.c_int, [C.POINTER(MeshView), C.POINTER(CellPolygons), C.c_int, _dp, _dp, _dp]),
         "axs_save_solver_file": (C.c_int, [C.c_char_p, C.POINTER(Config), C.POINTER(SolverFile)]),
         "axs_load_solver_file": (C.c_int, [C.c_char_p, C.POINTER(Config), C.POINTER(SolverFile)]),
         "axs_get_frame": (C.c_int, [S, C.c_int, C.c_int, _dp, _dp, C.c_size_t]),
@@ -330,6 +331,20 @@ class Mesh:
             tris = z["tris"] if "tris" in z.files else None
             quads = z["quads"] if "quads" in z.files else None
         return Mesh(arrays, pts, tris, quads)
+
+
+def mesh_quality(mesh: Mesh, device: int = 0) -> dict:
+    """Quality::buildQuality (solver/source/quality.cpp:160-166) on the GPU:
+    per-cell aspect ratio, non-orthogonality and skewness (the last from the
+    mesh's quads or triangles; zeros without them)."""
+    n = mesh.n_cells
+    out = {k: np.zeros(n) for k in ("aspect_ratio", "non_orthogonality", "skewness")}
+    v = mesh.view()
+    poly = mesh.polygons()
+    _check(lib().axs_mesh_quality(C.byref(v), C.byref(poly) if poly is not None else None, device,
+                                  _dptr(out["aspect_ratio"]), _dptr(out["non_orthogonality"]),
+                                  _dptr(out["skewness"])))
+    return out
 
 
 def _mesh_from_handle(h) -> Mesh:

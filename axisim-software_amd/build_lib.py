@@ -30,6 +30,7 @@ SOURCES = [
     "csrc/hip/kernels_mixed.hip",
     "csrc/hip/mpnn.hip",
     "csrc/hip/noalloc.hip",
+    "csrc/hip/quality.hip",
     "csrc/host/solver.cpp",
     "csrc/host/net.cpp",
     "csrc/host/capi.cpp",

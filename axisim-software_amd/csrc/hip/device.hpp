@@ -252,6 +252,17 @@ void launchScalar(hipStream_t s, const DMesh& m, DBC bc, DScalar sc, DWall wall,
 void launchRotate(hipStream_t s, int n, double* old2, double* old, const double* phi, int bdf2);
 void launchStepBegin(hipStream_t s, const DMesh& m, DState st, int bdf2);
 void launchCopy(hipStream_t s, double* dst, const double* src, int n, const Ctl* ctl);
+// Mesh quality metrics (quality.hip): the mesh arrays one thread per cell reads.
+struct QualityMesh {
+    int n;
+    const double *cz, *cr;            // cell centres
+    const int *faceStart, *faceIds;   // the cell's faces
+    const int *owner, *neighbor;
+    const double *fz, *fr, *nz, *nr;  // face centres, unit normals
+    const int* cornerStart;           // n + 1: the cell's corners, in ring order
+    const double *pz, *pr;            // corner coordinates, one entry per corner
+};
+void launchMeshQuality(hipStream_t s, const QualityMesh& m, double* aspect, double* orth, double* skew);
 void launchFill(hipStream_t s, double* dst, double value, int n, const Ctl* ctl);
 // streams src[0 .. n) through the caches; out[b] gets workgroup b's partial
 // sum only where it equals `mark`

@@ -488,6 +488,85 @@ axs_status axs_mesh_get_quads(const axs_mesh* h, int* n_cells, double* z4, doubl
 
 void axs_mesh_destroy(axs_mesh* h) { delete h; }
 
+// Quality::buildQuality (solver/source/quality.cpp:160-166) on the GPU: the mesh
+// arrays and the cells' corner rings go up, one thread per cell computes the
+// three metrics, the results come back.
+axs_status axs_mesh_quality(const axs_mesh_view* v, const axs_cell_polygons* poly, int device, double* aspect,
+                            double* non_orth, double* skew) {
+    if (!v || !aspect || !non_orth || !skew || v->n_cells < 0 || v->n_faces < 0) return AXS_ERR_INVALID;
+    if (v->n_cells > 0 && (!v->cell_center_z || !v->cell_center_r || !v->cell_face_start || !v->cell_face_ids ||
+                           !v->face_owner || !v->face_neighbor || !v->face_center_z || !v->face_center_r ||
+                           !v->face_normal_z || !v->face_normal_r))
+        return AXS_ERR_INVALID;
+    return guarded(nullptr, [&] {
+        int count = 0;
+        AXS_HIP(hipGetDeviceCount(&count));
+        if (device < 0 || device >= count) throw std::invalid_argument("no such device");
+        AXS_HIP(hipSetDevice(device));
+        const int N = v->n_cells, F = v->n_faces;
+        const int S = N > 0 ? v->cell_face_start[N] : 0;
+        // the corner rings as one coordinate list (quads: 4 per cell in ring
+        // order; triangles: their three points); no polygons: empty rings
+        std::vector<int> cs(N + 1, 0);
+        std::vector<double> pz, pr;
+        const int corners = poly ? poly->corners : 0;
+        if (corners == 4) {
+            if (!poly->quad_z || !poly->quad_r) throw std::invalid_argument("quad corners missing");
+            pz.assign(poly->quad_z, poly->quad_z + 4 * (size_t)N);
+            pr.assign(poly->quad_r, poly->quad_r + 4 * (size_t)N);
+            for (int c = 0; c <= N; c++) cs[c] = 4 * c;
+        } else if (corners == 3) {
+            if (!poly->tris || !poly->point_z || !poly->point_r) throw std::invalid_argument("triangles missing");
+            pz.resize(3 * (size_t)N);
+            pr.resize(3 * (size_t)N);
+            for (int c = 0; c < N; c++)
+                for (int j = 0; j < 3; j++) {
+                    const int p = poly->tris[3 * (size_t)c + j];
+                    if (p < 0 || p >= poly->n_points) throw std::out_of_range("triangle corner out of range");
+                    pz[3 * (size_t)c + j] = poly->point_z[p];
+                    pr[3 * (size_t)c + j] = poly->point_r[p];
+                }
+            for (int c = 0; c <= N; c++) cs[c] = 3 * c;
+        } else if (corners != 0) {
+            throw std::invalid_argument("corners must be 0, 3 or 4");
+        }
+        for (int k = 0; k < S; k++)
+            if (v->cell_face_ids[k] < 0 || v->cell_face_ids[k] >= F) throw std::out_of_range("face id out of range");
+        for (int f = 0; f < F; f++)
+            if (v->face_owner[f] < 0 || v->face_owner[f] >= N || v->face_neighbor[f] >= N)
+                throw std::out_of_range("face cell out of range");
+        axs::Arena a;
+        auto up = [&](const auto* src, size_t n) {
+            using T = std::remove_const_t<std::remove_pointer_t<decltype(src)>>;
+            T* d = a.alloc<T>(n);
+            if (n) AXS_HIP(hipMemcpy(d, src, n * sizeof(T), hipMemcpyHostToDevice));
+            return (const T*)d;
+        };
+        axs::QualityMesh m{};
+        m.n = N;
+        m.cz = up(v->cell_center_z, N);
+        m.cr = up(v->cell_center_r, N);
+        m.faceStart = up(v->cell_face_start, (size_t)N + 1);
+        m.faceIds = up(v->cell_face_ids, S);
+        m.owner = up(v->face_owner, F);
+        m.neighbor = up(v->face_neighbor, F);
+        m.fz = up(v->face_center_z, F);
+        m.fr = up(v->face_center_r, F);
+        m.nz = up(v->face_normal_z, F);
+        m.nr = up(v->face_normal_r, F);
+        m.cornerStart = up(cs.data(), cs.size());
+        m.pz = up(pz.data(), pz.size());
+        m.pr = up(pr.data(), pr.size());
+        double* out = a.alloc<double>(3 * (size_t)N);
+        axs::launchMeshQuality(nullptr, m, out, out + N, out + 2 * (size_t)N);
+        AXS_HIP(hipGetLastError());
+        AXS_HIP(hipDeviceSynchronize());
+        AXS_HIP(hipMemcpy(aspect, out, (size_t)N * sizeof(double), hipMemcpyDeviceToHost));
+        AXS_HIP(hipMemcpy(non_orth, out + N, (size_t)N * sizeof(double), hipMemcpyDeviceToHost));
+        AXS_HIP(hipMemcpy(skew, out + 2 * (size_t)N, (size_t)N * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
 // ---------------- learned initial guess ----------------
 axs_status axs_net_create(axs_net** out, int layers, uint64_t seed, int device) {
     if (!out) return AXS_ERR_INVALID;

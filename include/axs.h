@@ -508,6 +508,17 @@ typedef struct axs_cell_polygons {
     int structured;            /* meta "meshType": 1 Structured, 0 Unstructured   */
     int multiblock;            /* meta "multiBlock"                               */
 } axs_cell_polygons;
+/* Mesh quality metrics (Quality::buildQuality, solver/source/quality.cpp:160-166),
+ * per cell, computed on GPU `device`: aspect_ratio = max/min over the cell's
+ * faces of |(x_f - x_c).n_f| (0 when the smallest is not above 1e-30,
+ * quality.cpp:129-158); non_orthogonality = the smallest of
+ * |(x_f - x_c).n_f|/|x_f - x_c| over the faces and |(x_N - x_P).n_f|/|x_N - x_P|
+ * over the interior ones, from 1 (quality.cpp:61-105); skewness from the corner
+ * ring's interior angles (quality.cpp:7-59; 0 without corners).  `poly` gives
+ * the corner rings (axs_cell_polygons, corners 4 or 3; NULL or 0: none).  Each
+ * output holds n_cells values.  Allocates device memory for the call. */
+axs_status axs_mesh_quality(const axs_mesh_view* mesh, const axs_cell_polygons* poly, int device,
+                            double* aspect_ratio, double* non_orthogonality, double* skewness);
 /* saveSolutionNpy (file_manager.cpp:2074-2309): into directory `dir` (created),
  * solution.npy (cells x [z, r, volume, area2D, then the solved fields in the
  * results panel's order]), points.npy + cells.npy when `poly` has corners,

@@ -28,6 +28,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -1944,6 +1945,70 @@ int orc_mg_get_operator(const orc_mg* h, int l, double* AC, double* AF) {
     const Coeff& c = h->mg.levels[l].coeff;
     std::copy(c.AC.begin(), c.AC.end(), AC);
     std::copy(c.AF.begin(), c.AF.end(), AF);
+    return AXS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Quality::buildQuality, solver/source/quality.cpp:160-166: calculateAspectRatio
+// (129-158), calculateOrthogonality (61-105), calculateSkewness (7-59), per
+// cell, on the mesh view and the cells' corner rings (cornerStart: n_cells + 1
+// offsets into the coordinate lists pz, pr).
+// ---------------------------------------------------------------------------
+int orc_mesh_quality(const axs_mesh_view* v, const int* cornerStart, const double* pz, const double* pr,
+                     double* aspect, double* orth, double* skew) {
+    const int N = v->n_cells;
+    for (int c = 0; c < N; c++) {
+        double dMin = std::numeric_limits<double>::max(), dMax = 0.0;
+        for (int k = v->cell_face_start[c]; k < v->cell_face_start[c + 1]; k++) {
+            const int f = v->cell_face_ids[k];
+            const double fz = v->face_center_z[f] - v->cell_center_z[c];
+            const double fr = v->face_center_r[f] - v->cell_center_r[c];
+            const double d = std::abs(fz * v->face_normal_z[f] + fr * v->face_normal_r[f]);
+            dMin = std::min(dMin, d);
+            dMax = std::max(dMax, d);
+        }
+        aspect[c] = (dMin > 1e-30) ? dMax / dMin : 0.0;
+    }
+    for (int c = 0; c < N; c++) {
+        double o = 1.0;
+        for (int k = v->cell_face_start[c]; k < v->cell_face_start[c + 1]; k++) {
+            const int f = v->cell_face_ids[k];
+            const double fz = v->face_center_z[f] - v->cell_center_z[c];
+            const double fr = v->face_center_r[f] - v->cell_center_r[c];
+            const double fDot = std::abs(fz * v->face_normal_z[f] + fr * v->face_normal_r[f]);
+            const double fMag = std::sqrt(fz * fz + fr * fr);
+            o = std::min(o, fDot / fMag);
+            if (v->face_neighbor[f] < 0) continue;
+            const int P = v->face_owner[f], Nb = v->face_neighbor[f];
+            const double dz = v->cell_center_z[Nb] - v->cell_center_z[P];
+            const double dr = v->cell_center_r[Nb] - v->cell_center_r[P];
+            const double dDot = std::abs(dz * v->face_normal_z[f] + dr * v->face_normal_r[f]);
+            const double dMag = std::sqrt(dz * dz + dr * dr);
+            o = std::min(o, dDot / dMag);
+        }
+        orth[c] = o;
+    }
+    constexpr double radToDeg = 57.29577951308232;
+    for (int c = 0; c < N; c++) {
+        skew[c] = 0.0;
+        const int begin = cornerStart[c], n = cornerStart[c + 1] - begin;
+        if (n < 3) continue;
+        double tMin = 180.0, tMax = 0.0;
+        for (int k = 0; k < n; ++k) {
+            const int ip = begin + (k + n - 1) % n, ih = begin + k, in = begin + (k + 1) % n;
+            const double az = pz[ip] - pz[ih], ar = pr[ip] - pr[ih];
+            const double bz = pz[in] - pz[ih], br = pr[in] - pr[ih];
+            const double aMag = std::sqrt(az * az + ar * ar);
+            const double bMag = std::sqrt(bz * bz + br * br);
+            if (aMag < 1e-30 || bMag < 1e-30) continue;
+            const double cosT = std::clamp((az * bz + ar * br) / (aMag * bMag), -1.0, 1.0);
+            const double theta = std::acos(cosT) * radToDeg;
+            tMin = std::min(tMin, theta);
+            tMax = std::max(tMax, theta);
+        }
+        const double tE = 180.0 - 360.0 / (double)n;
+        skew[c] = std::max((tMax - tE) / (180 - tE), (tE - tMin) / tE);
+    }
     return AXS_OK;
 }
 

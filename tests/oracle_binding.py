@@ -25,6 +25,7 @@ def olib() -> C.CDLL:
         L = C.CDLL(str(ORACLE_LIB))
         V = C.c_void_p
         sig = {
+            "orc_mesh_quality": (C.c_int, [C.POINTER(axs.MeshView), _ip, _dp, _dp, _dp, _dp, _dp]),
             "orc_create": (V, [C.POINTER(axs.MeshView), C.POINTER(axs.Config)]),
             "orc_destroy": (None, [V]),
             "orc_set_bcs": (C.c_int, [V, C.c_int, C.POINTER(axs.BCView)]),
@@ -321,3 +322,31 @@ def residual_value(is_continuity, norm, scaling, res, scale, scale_iteration, sc
     olib().orc_residual_value(int(is_continuity), norm, scaling, _d(res), _d(scale), res.size, scale_iteration,
                               C.byref(sv), C.byref(rv))
     return rv.value, sv.value
+
+
+def corner_rings(mesh):
+    """(cornerStart, z, r) of the cells' corner rings: 4 per multiblock cell, the
+    three points of each triangle; empty rings without either."""
+    n = mesh.n_cells
+    if mesh.quads is not None:
+        return (np.arange(n + 1, dtype=np.int32) * 4, np.ascontiguousarray(mesh.quads[:, :, 0]).ravel(),
+                np.ascontiguousarray(mesh.quads[:, :, 1]).ravel())
+    if mesh.points is not None and mesh.tris is not None:
+        t = np.asarray(mesh.tris).ravel()
+        return (np.arange(n + 1, dtype=np.int32) * 3, np.ascontiguousarray(mesh.points[t, 0]),
+                np.ascontiguousarray(mesh.points[t, 1]))
+    return np.zeros(n + 1, np.int32), np.zeros(0), np.zeros(0)
+
+
+def mesh_quality(mesh, rings=None):
+    """The oracle's Quality::buildQuality: aspect ratio, non-orthogonality and
+    skewness per cell; rings = (cornerStart, z, r) overrides the mesh's own."""
+    cs, pz, pr = rings if rings is not None else corner_rings(mesh)
+    cs = np.ascontiguousarray(cs, dtype=np.int32)
+    pz, pr = _f64(pz), _f64(pr)
+    n = mesh.n_cells
+    a, o, k = np.zeros(n), np.zeros(n), np.zeros(n)
+    v = mesh.view()
+    rc = olib().orc_mesh_quality(C.byref(v), _i(cs), _d(pz), _d(pr), _d(a), _d(o), _d(k))
+    assert rc == 0
+    return {"aspect_ratio": a, "non_orthogonality": o, "skewness": k}
