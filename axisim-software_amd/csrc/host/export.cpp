@@ -3,7 +3,7 @@
 //   axs_save_solution_npy   saveSolutionNpy (file_manager.cpp:2074-2309): the
 //                           cell-centred fields as a NumPy table, the cell
 //                           polygons, one table per transient frame, meta.json
-//   axs_save_solver_file /  the .axislv solver payload, format version 5
+//   axs_save_solver_file /  the .axislv solver payload, format version 5 (and 4 read)
 //   axs_load_solver_file    (saveFromPathSolver / loadFromPathSolver,
 //                           file_manager.cpp:1482-1549)
 //
@@ -231,6 +231,7 @@ void saveSolutionNpy(Solver& s, const axs_cell_polygons* poly, const std::filesy
 // ---------------- .axislv solver payload, version 5 ----------------
 constexpr uint32_t kSolverMagic = 0x53585641u;  // "AXVS" little-endian
 constexpr uint32_t kSolverVersion = 5u;
+constexpr uint32_t kSolverVersionSplitConfig = 4u;  // still read (readSolverPayloadLegacy)
 
 // Field-by-field little-endian byte image of the payload, each struct at its
 // reference size with zeroed padding: ConfigSolver 40 bytes (five one-byte enums,
@@ -349,16 +350,71 @@ std::vector<uint8_t> encodeSolverFile(axs_config c, axs_solver_file x) {
     return o.b;
 }
 
-// readSolverPayload (file_manager.cpp:304-320) into a copy of cfg; throws on a
-// foreign or short file
+// readSolverPayloadLegacy (file_manager.cpp:321-373): the version-4 payload,
+// whose ConfigSolver was 32 bytes of int-width enums (linear solver, linear
+// iterations, convection and transient flags, the time scheme, dt, tEnd) with
+// the velocity solver, convection scheme and keyframe interval as loose int32s
+// after it; ConfigSimple 32 bytes with the corrector as a trailing byte (a
+// pass count in old files: any non-zero is on); the gradient scheme as an
+// int32 after it; residual records with int32 enums.  The settings v4 did not
+// hold take the reference's ConfigSolver / ConfigSimple defaults (the
+// multigrid on).
+void decodeSolverFileV4(Reader& r, axs_config& c, axs_solver_file& x) {
+    for (int i = 0; i < 15; i++) x.var_units[i] = r.get<uint8_t>();
+    c.solve_energy = r.get<uint8_t>() != 0;
+    c.solve_concentration = r.get<uint8_t>() != 0;
+    c.linear_solver = r.get<int32_t>();
+    c.linear_max_iter = r.get<int32_t>();
+    c.add_convection = r.get<uint8_t>() != 0;
+    c.transient = r.get<uint8_t>() != 0;
+    c.time_scheme = r.get<uint8_t>();
+    r.skip(5);
+    c.dt = r.get<double>();
+    c.t_end = r.get<double>();
+    x.velocity_solver = r.get<int32_t>();
+    c.convection_scheme = r.get<int32_t>();
+    c.save_key_frame_iter = r.get<int32_t>();
+    c.rho = r.get<double>();
+    c.mu = r.get<double>();
+    c.cp = r.get<double>();
+    c.k = r.get<double>();
+    c.diffusivity = r.get<double>();
+    c.max_iter = r.get<int32_t>();
+    c.check_conv = r.get<int32_t>();
+    x.mom_tol = r.get<double>();
+    x.pp_tol = r.get<double>();
+    c.use_non_orth = r.get<uint8_t>() != 0;
+    r.skip(7);
+    c.gradient_scheme = r.get<int32_t>();
+    c.use_multigrid = 1;
+    for (int i = 0; i < AXS_RES_COUNT; i++) {  // readResidualConfigsLegacy, :195-219
+        x.residual_type[i] = r.get<int32_t>();
+        c.residual[i].norm = r.get<int32_t>();
+        c.residual[i].scaling = r.get<int32_t>();
+        c.residual[i].enabled = r.get<uint8_t>() != 0;
+        c.residual[i].tol = r.get<double>();
+    }
+}
+
+// loadFromPathSolver (file_manager.cpp:1518-1546): readSolverPayload
+// (file_manager.cpp:300-314) for version 5, readSolverPayloadLegacy for 4,
+// into a copy of cfg; throws on a foreign, other-version or short file
 void decodeSolverFile(const std::vector<uint8_t>& bytes, axs_config& cfg, axs_solver_file& x) {
     Reader r{bytes};
     const uint32_t magic = r.get<uint32_t>();
     const uint32_t version = r.get<uint32_t>();
     if (!r.ok) throw std::invalid_argument("solver file too short");
     if (magic != kSolverMagic) throw std::domain_error("not an AxiSim solver file (magic)");
+    if (version == kSolverVersionSplitConfig) {
+        axs_config c = cfg;
+        decodeSolverFileV4(r, c, x);
+        if (!r.ok) throw std::invalid_argument("solver file truncated");
+        sanitize(c, x);
+        cfg = c;
+        return;
+    }
     if (version != kSolverVersion)
-        throw std::domain_error("solver file version " + std::to_string(version) + " (this reads version 5)");
+        throw std::domain_error("solver file version " + std::to_string(version) + " (this reads versions 4 and 5)");
     axs_config c = cfg;
     for (int i = 0; i < 15; i++) x.var_units[i] = r.get<uint8_t>();
     c.solve_energy = r.get<uint8_t>() != 0;

@@ -527,7 +527,9 @@ axs_status axs_mesh_quality(const axs_mesh_view* mesh, const axs_cell_polygons* 
  * NPY format 1.0. */
 axs_status axs_save_solution_npy(axs_solver* s, const axs_cell_polygons* poly, const char* dir);
 
-/* The solver settings file (.axislv solver payload, format version 5;
+/* The solver settings file (.axislv solver payload, format version 5, and the
+ * version-4 payload loadFromPathSolver still reads (readSolverPayloadLegacy,
+ * file_manager.cpp:321-373);
  * saveFromPathSolver / loadFromPathSolver, file_manager.cpp:1482-1549): magic
  * "AXVS", version, VariableUnits, SolverFieldOption, ConfigSolver, the fluid
  * properties, ConfigSimple, then type / norm / scaling / enabled / tolerance of

@@ -89,8 +89,8 @@ def test_solver_file_rejects_other_files(tmp_path):
     path = tmp_path / "s.axislv"
     axs.save_solver_file(path, axs.default_config())
     data = bytearray(path.read_bytes())
-    bad = tmp_path / "v4.axislv"
-    bad.write_bytes(bytes(data[:4]) + struct.pack("<I", 4) + bytes(data[8:]))
+    bad = tmp_path / "v6.axislv"
+    bad.write_bytes(bytes(data[:4]) + struct.pack("<I", 6) + bytes(data[8:]))
     with pytest.raises(axs.AxsError) as e:
         axs.load_solver_file(bad)
     assert e.value.args and "version" in str(e.value)
@@ -102,6 +102,52 @@ def test_solver_file_rejects_other_files(tmp_path):
     other.write_bytes(b"\0" * 200)
     with pytest.raises(axs.AxsError):
         axs.load_solver_file(other)
+
+
+LAYOUT_V4 = "<II" + "15B" + "2B" + "iiBBB5xdd" + "iii" + "5d" + "iiddB7x" + "i" + "iiiBd" * 5
+
+
+def test_solver_file_reads_version_4(tmp_path):
+    """The version-4 payload (readSolverPayloadLegacy, file_manager.cpp:321-373):
+    int-width enums, the velocity solver / convection scheme / keyframe interval
+    as loose ints, the corrector as ConfigSimple's trailing byte (a pass count:
+    2 means on), the gradient scheme after it; the multigrid flag, which v4 did
+    not hold, comes back on whatever the caller had."""
+    res = []
+    for i in range(axs.RES_COUNT):
+        res += [7 if i == 0 else i % 3, i % 3, (i + 1) % 4, i % 2, 10.0 ** -(i + 3)]
+    data = struct.pack(LAYOUT_V4, 0x53585641, 4, *range(2, 17), 1, 1,
+                       axs.LINEAR_GS_RB, 17, 1, 1, axs.TIME_BDF2, 2.5e-3, 0.5,
+                       0, axs.CONV_SOU, 3,
+                       1060.0, 3.5e-3, 3600.0, 0.52, 1e-9,
+                       250, 2, 3e-8, 4e-5, 2,
+                       axs.GRAD_GREEN_GAUSS, *res)
+    assert len(data) == 8 + 15 + 2 + 32 + 12 + 40 + 32 + 4 + 5 * 21
+    path = tmp_path / "v4.axislv"
+    path.write_bytes(data)
+    got, extra = axs.load_solver_file(path, axs.default_config(use_multigrid=0, alpha_u=0.5))
+    assert list(extra.var_units) == list(range(2, 17))
+    assert (got.solve_energy, got.solve_concentration) == (1, 1)
+    assert (got.linear_solver, got.linear_max_iter, got.add_convection, got.transient) == (axs.LINEAR_GS_RB, 17, 1, 1)
+    assert (got.time_scheme, got.dt, got.t_end) == (axs.TIME_BDF2, 2.5e-3, 0.5)
+    assert (got.convection_scheme, got.save_key_frame_iter, got.gradient_scheme) == (axs.CONV_SOU, 3,
+                                                                                     axs.GRAD_GREEN_GAUSS)
+    assert (got.rho, got.mu, got.cp, got.k, got.diffusivity) == (1060.0, 3.5e-3, 3600.0, 0.52, 1e-9)
+    assert (got.max_iter, got.check_conv, extra.mom_tol, extra.pp_tol) == (250, 2, 3e-8, 4e-5)
+    assert got.use_non_orth == 1 and got.use_multigrid == 1 and got.alpha_u == 0.5
+    assert extra.residual_type[0] == 1  # out of range -> RESIDUAL_RAW
+    for i in range(axs.RES_COUNT):
+        assert (got.residual[i].norm, got.residual[i].scaling, got.residual[i].enabled) == (i % 3, (i + 1) % 4, i % 2)
+        assert got.residual[i].tol == 10.0 ** -(i + 3)
+    # a version-4 file cut short, and version 3, are refused
+    short = tmp_path / "short.axislv"
+    short.write_bytes(data[:-10])
+    with pytest.raises(axs.AxsError):
+        axs.load_solver_file(short)
+    v3 = tmp_path / "v3.axislv"
+    v3.write_bytes(data[:4] + struct.pack("<I", 3) + data[8:])
+    with pytest.raises(axs.AxsError):
+        axs.load_solver_file(v3)
 
 
 def test_multiblock_quads_match_cells():
