@@ -8,6 +8,9 @@ depend on the thread count):
   C4  the 64-case pulsatile BDF2 sweep (about 10M cells) as one batch, without
       multigrid, 2 time steps: sampled cases' slices bit-exact against the
       same cases run alone, and the smallest case against the oracle
+  C3 variants at full size: the colour Gauss-Seidel momentum solver with the
+      multi-colour Gauss-Seidel multigrid smoother; energy and Michaelis-Menten
+      species transport; BDF2 with a pulsatile inlet (observed: bit for bit)
 """
 import os
 
@@ -89,3 +92,49 @@ def test_c4_sweep_batch_sampled_cases(require_gpu, oracle_threads):
             o = Oracle(single.mesh, single.bcs, single.config)
             o.run()
             compare(s, o, ["u", "v", "p", "pp", "mdot", "u_old", "u_old2"], 1e-12)
+
+
+def test_c3_nozzle_gauss_seidel_paths(require_gpu, oracle_threads):
+    """The 1M nozzle with the colour Gauss-Seidel momentum solver and the
+    multi-colour Gauss-Seidel multigrid smoother, 2 iterations."""
+    case = axs.nozzle_case(mg_max_levels=6, max_iter=2, linear_solver=axs.LINEAR_GS_RB,
+                           mg_smoother=axs.MG_SMOOTH_RBGS)
+    s, o = _both(case)
+    worst = compare(s, o, STATE_FIELDS, 1e-12)
+    print("C3 Gauss-Seidel worst rel L2", max(worst.values()))
+
+
+def test_c3_nozzle_scalar_transport(require_gpu, oracle_threads):
+    """The 1M nozzle with energy and Michaelis-Menten species transport
+    (integer-exponent kinetics: no pow on either side), 2 iterations."""
+    case = axs.nozzle_case(mg_max_levels=6, max_iter=2)
+    sc = axs.scalar_pipe_case(nz=4, nr=2)
+    case.bcs = sc.bcs  # the same four groups: inlet, outlet, wall, axis
+    case.bcs[axs.VAR_U].value[axs.G_INLET] = case.info["u_in"]
+    c = case.config
+    c.solve_energy = c.solve_concentration = 1
+    c.residual[axs.RES_TEMPERATURE].enabled = c.residual[axs.RES_CONCENTRATION].enabled = 1
+    s, o = _both(case)
+    worst = compare(s, o, STATE_FIELDS + ["t", "c", "wall_cw", "wall_ocr"], 1e-12)
+    print("C3 scalars worst rel L2", max(worst.values()))
+
+
+def test_c3_nozzle_transient_pulsatile(require_gpu, oracle_threads):
+    """The 1M nozzle, BDF2 with a pulsatile inlet, 2 steps of 2 iterations."""
+    case = axs.nozzle_case(mg_max_levels=6, max_iter=2, transient=1, time_scheme=axs.TIME_BDF2, dt=1e-3,
+                           t_end=2e-3)
+    u = case.bcs[axs.VAR_U]
+    u.type[axs.G_INLET] = axs.BC_PULSATILE
+    u.amplitude = [0.0] * len(u.type)
+    u.frequency = [0.0] * len(u.type)
+    u.amplitude[axs.G_INLET], u.frequency[axs.G_INLET] = 0.3, 50.0
+    for i in range(3):
+        case.config.residual[i].tol = 0.0
+    print(f"{case.name}: {case.mesh.n_cells} cells, 2 BDF2 steps", flush=True)
+    s = axs.make_solver(case)
+    s.run()
+    o = Oracle(case.mesh, case.bcs, case.config)
+    o.run()
+    assert s.stats().iterations == o.iterations == 4
+    worst = compare(s, o, ["u", "v", "p", "pp", "mdot", "u_old", "u_old2", "v_old", "v_old2"], 1e-12)
+    print("C3 transient worst rel L2", max(worst.values()))
