@@ -1,6 +1,6 @@
 """Mixed precision (build extension): FP32 momentum sweeps and FP32 V-cycles
 inside FP64 defect correction, FP64 coarsest direct solve.  The plain FP64
-path is the judge: the mixed path must reach the same converged state."""
+path is the yardstick: the mixed path must reach the same converged state."""
 import numpy as np
 import pytest
 
