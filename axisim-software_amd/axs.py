@@ -140,6 +140,11 @@ class CellPolygons(C.Structure):
     ]
 
 
+class CdtParams(C.Structure):
+    _fields_ = [("B", C.c_double), ("beta", C.c_double), ("classify_tol", C.c_double),
+                ("max_points", C.c_longlong)]
+
+
 class SolverFile(C.Structure):
     """What the .axislv solver payload holds beyond Config."""
     _fields_ = [
@@ -219,6 +224,12 @@ def load_library(path: os.PathLike | None = None) -> C.CDLL:
         "axs_mesh_concat": (C.c_int, [C.POINTER(S), C.c_int, C.POINTER(MeshView), C.c_int]),
         "axs_mesh_get_triangles": (C.c_int, [S, _ip, _dp, _dp, _ip, _ip]),
         "axs_mesh_destroy": (None, [S]),
+        "axs_cdt_params_defaults": (None, [C.POINTER(CdtParams)]),
+        "axs_cdt_generate": (C.c_int, [C.POINTER(S), C.c_int, _dp, _dp, C.c_int, _ip, C.POINTER(CdtParams)]),
+        "axs_cdt_insert_points": (C.c_int, [C.POINTER(S), C.c_int, _dp, _dp]),
+        "axs_cdt_counts": (C.c_int, [S, _ip, _ip, _ip]),
+        "axs_cdt_get": (C.c_int, [S, _dp, _dp, _ip, _ip, _ip]),
+        "axs_cdt_destroy": (None, [S]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -427,6 +438,125 @@ def nozzle_mesh(spacing: float, jitter: float = 0.05, seed_points: int = 3001, s
     h = C.c_void_p()
     _check(lib().axs_mesh_nozzle(C.byref(h), C.byref(s)))
     return _mesh_from_handle(h)
+
+
+# ---------------------------------------------------------------------------
+# Unstructured mesher (AxiMesh::generateMesh, aximesh/source/aximesh.cpp)
+# ---------------------------------------------------------------------------
+@dataclass
+class CdtMesh:
+    """AxiMesh::Mesh (aximesh/include/aximesh/aximesh.h:80-86): points (n, 2)
+    in bin order then refinement points; triangles tri_v (t, 3) wound
+    counter-clockwise with tri_adj[t, i] the triangle across edge
+    v[i] -> v[i+1] (-1 on the boundary); segments (s, 2) renumbered."""
+    points: np.ndarray
+    tri_v: np.ndarray
+    tri_adj: np.ndarray
+    segments: np.ndarray
+
+
+def _cdt_result(h) -> CdtMesh:
+    n, t, s = C.c_int(), C.c_int(), C.c_int()
+    try:
+        _check(lib().axs_cdt_counts(h, C.byref(n), C.byref(t), C.byref(s)))
+        x, y = np.zeros(n.value), np.zeros(n.value)
+        v = np.zeros(3 * t.value, dtype=np.int32)
+        adj = np.zeros(3 * t.value, dtype=np.int32)
+        seg = np.zeros(2 * s.value, dtype=np.int32)
+        _check(lib().axs_cdt_get(h, _dptr(x), _dptr(y), _iptr(v), _iptr(adj), _iptr(seg)))
+    finally:
+        lib().axs_cdt_destroy(h)
+    return CdtMesh(np.stack([x, y], axis=1), v.reshape(-1, 3), adj.reshape(-1, 3), seg.reshape(-1, 2))
+
+
+def cdt_params(B: float | None = None, beta: float | None = None, classify_tol: float | None = None,
+               max_points: int = 0) -> CdtParams:
+    """AxiMesh::Params with the reference defaults for whatever is not given
+    (max_points: the build's point limit, 0 for the default)."""
+    p = CdtParams()
+    lib().axs_cdt_params_defaults(C.byref(p))
+    if B is not None:
+        p.B = B
+    if beta is not None:
+        p.beta = beta
+    if classify_tol is not None:
+        p.classify_tol = classify_tol
+    p.max_points = max_points
+    return p
+
+
+def cdt_generate(points: np.ndarray, segments: np.ndarray, params: CdtParams | None = None) -> CdtMesh:
+    """AxiMesh::generateMesh(points, segments, len(points), params):
+    constrained Delaunay triangulation, flood-fill removal of the outside,
+    frontal refinement (aximesh.cpp:1105-1175)."""
+    pts = np.asarray(points, dtype=np.float64).reshape(-1, 2)
+    x, y = np.ascontiguousarray(pts[:, 0]), np.ascontiguousarray(pts[:, 1])
+    seg = np.ascontiguousarray(np.asarray(segments, dtype=np.int32).reshape(-1, 2)).ravel()
+    h = C.c_void_p()
+    _check(lib().axs_cdt_generate(C.byref(h), x.size, _dptr(x), _dptr(y), seg.size // 2, _iptr(seg),
+                                  C.byref(params) if params is not None else None))
+    return _cdt_result(h)
+
+
+def cdt_insert_points(points: np.ndarray) -> CdtMesh:
+    """AxiMesh::insertPoints(points, len(points)) (aximesh.cpp:466-489): the
+    Delaunay triangulation of the points in the given order inside the super
+    triangle, whose three vertices follow the input points."""
+    pts = np.asarray(points, dtype=np.float64).reshape(-1, 2)
+    x, y = np.ascontiguousarray(pts[:, 0]), np.ascontiguousarray(pts[:, 1])
+    h = C.c_void_p()
+    _check(lib().axs_cdt_insert_points(C.byref(h), x.size, _dptr(x), _dptr(y)))
+    return _cdt_result(h)
+
+
+def nozzle_outline(**geometry) -> list[tuple]:
+    """The nozzle's boundary loop, counter-clockwise, as (z0, r0, z1, r1, group)
+    edges: axis (3), outlet (1), wall (2), inlet (0)."""
+    g = dict(NOZZLE_GEOMETRY)
+    g.update(geometry)
+    tc, td = np.tan(np.radians(g["cone_deg"])), np.tan(np.radians(g["diff_deg"]))
+    z = [0.0, g["l_in"]]
+    r = [g["r_in"], g["r_in"]]
+    z.append(z[-1] + abs(g["r_in"] - g["r_throat"]) / tc); r.append(g["r_throat"])
+    z.append(z[-1] + g["l_throat"]); r.append(g["r_throat"])
+    z.append(z[-1] + abs(g["r_out"] - g["r_throat"]) / td); r.append(g["r_out"])
+    z.append(z[-1] + g["l_out"]); r.append(g["r_out"])
+    L = z[-1]
+    edges = [(0.0, 0.0, L, 0.0, 3), (L, 0.0, L, r[-1], 1)]
+    edges += [(z[i], r[i], z[i - 1], r[i - 1], 2) for i in range(len(z) - 1, 0, -1)]
+    edges.append((0.0, r[0], 0.0, 0.0, 0))
+    return edges
+
+
+def outline_boundary(loops: list[list[tuple]], spacing: float):
+    """Boundary points and segments of closed loops of (z0, r0, z1, r1, group)
+    edges, each edge cut into ceil(length / spacing) equal pieces (the input
+    the GUI hands the mesher, Mesh::runAxiMeshTriangulation,
+    solver/source/mesh.cpp:1216-1223).  Returns points (n, 2), segments
+    (n, 2) and each segment's group."""
+    pts, segs, groups = [], [], []
+    for loop in loops:
+        first = len(pts)
+        for z0, r0, z1, r1, g in loop:
+            n = max(1, int(np.ceil(np.hypot(z1 - z0, r1 - r0) / spacing - 1e-9)))
+            for k in range(n):
+                t = k / n
+                pts.append((z0 + t * (z1 - z0), r0 + t * (r1 - r0)))
+                segs.append((len(pts) - 1, len(pts)))
+                groups.append(g)
+        segs[-1] = (segs[-1][0], first)
+    return np.array(pts), np.array(segs, dtype=np.int32), np.array(groups, dtype=np.int32)
+
+
+def cdt_mesh(loops: list[list[tuple]], spacing: float, params: CdtParams | None = None) -> tuple[Mesh, CdtMesh]:
+    """The unstructured front end end to end: loops discretised at `spacing`,
+    meshed by the constrained Delaunay mesher, and built into the
+    finite-volume mesh (createUnstructuredMesh), boundary faces taking the
+    group of the loop edge they lie on."""
+    pts, segs, _ = outline_boundary(loops, spacing)
+    m = cdt_generate(pts, segs, params)
+    edges = [e for loop in loops for e in loop]
+    return triangle_mesh(m.points, m.tri_v, edges, tol=1e-6 * spacing), m
 
 
 # ---------------------------------------------------------------------------

@@ -37,6 +37,7 @@ SOURCES = [
     "csrc/host/topology.cpp",
     "csrc/host/casegen.cpp",
     "csrc/host/export.cpp",
+    "csrc/host/cdt.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -84,17 +85,21 @@ def build_product(jobs: int = 8) -> Path:
 
 
 def build_compat() -> Path:
-    """The C++ Solver drop-in (compat/) as a static archive plus the demo that
-    drives it like the GUI does; plain g++ against include/axs.h + libaxs.so."""
+    """The C++ drop-ins (compat/: the Solver class and the AxiMesh mesher
+    interface) as a static archive plus the demo that drives the solver like
+    the GUI does; plain g++ against include/axs.h + libaxs.so."""
     obj_dir = PKG / "_build"
     obj_dir.mkdir(exist_ok=True)
     inc = [f"-I{ROOT / 'include'}", f"-I{PKG / 'compat'}"]
     obj = obj_dir / "compat_solver.o"
     _run(["g++", "-std=c++17", "-O2", "-fPIC", "-Wall", *inc, "-c", str(PKG / "compat" / "solver.cpp"), "-o", str(obj)])
+    mesher = obj_dir / "compat_aximesh.o"
+    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-Wall", *inc, "-c", str(PKG / "compat" / "aximesh.cpp"), "-o",
+          str(mesher)])
     archive = LIB_DIR / "libaxs_compat.a"
     if archive.exists():
         archive.unlink()
-    _run(["ar", "rcs", str(archive), str(obj)])
+    _run(["ar", "rcs", str(archive), str(obj), str(mesher)])
     # the demo plays the GUI: its console and residual plot are the stand-ins
     # under tests/compat_stub, and it compiles the GUI-side hooks against them
     demo = LIB_DIR / "compat_demo"
@@ -119,12 +124,16 @@ def build_oracle() -> Path:
 
 def build_reference_checkers() -> bool:
     """oracle/_ref: compile the reference's header-only multiblock packer into a
-    golden-vector writer.  Only where the reference sources are present."""
+    golden-vector writer and its mesher into a ctypes library.  Only where the
+    reference sources are present."""
     if not Path("/root/reference/structs/multiblock.h").exists():
         return False
     if shutil.which("make") is None:
         return False
     _run(["make", "-C", str(ORACLE_DIR), "ref"])
+    # the reference's mesher harness, once on the reference and once on the
+    # product's drop-in (needs libaxs.so and libaxs_compat.a built)
+    _run(["make", "-C", str(ORACLE_DIR), "harness"])
     return True
 
 

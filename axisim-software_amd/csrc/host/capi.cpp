@@ -12,6 +12,7 @@
 #include <string>
 
 #include "axs.h"
+#include "cdt.hpp"
 #include "mesh_types.hpp"
 #include "net.hpp"
 #include "solver.hpp"
@@ -28,6 +29,10 @@ struct axs_mesh {
 
 struct axs_net {
     Mpnn* net = nullptr;
+};
+
+struct axs_cdt {
+    axs::cdt::Result r;
 };
 
 namespace axs {
@@ -487,6 +492,93 @@ axs_status axs_mesh_get_quads(const axs_mesh* h, int* n_cells, double* z4, doubl
 }
 
 void axs_mesh_destroy(axs_mesh* h) { delete h; }
+
+// ---------------- unstructured mesher ----------------
+void axs_cdt_params_defaults(axs_cdt_params* p) {
+    if (!p) return;
+    const axs::cdt::Params d;
+    p->B = d.skinny;
+    p->beta = d.grading;
+    p->classify_tol = d.accept;
+    p->max_points = d.maxPoints;
+}
+
+axs_status axs_cdt_generate(axs_cdt** out, int n, const double* x, const double* y, int nSeg, const int* seg,
+                            const axs_cdt_params* params) {
+    if (!out || n <= 0 || !x || !y || nSeg < 0 || (nSeg > 0 && !seg)) return AXS_ERR_INVALID;
+    *out = nullptr;
+    return guarded(nullptr, [&] {
+        axs::cdt::Params prm;
+        if (params) {
+            prm.skinny = params->B;
+            prm.grading = params->beta;
+            prm.accept = params->classify_tol;
+            if (params->max_points > 0) prm.maxPoints = params->max_points;
+        }
+        std::vector<axs::cdt::Pt> pts(n);
+        for (int i = 0; i < n; i++) pts[i] = {x[i], y[i]};
+        std::vector<axs::cdt::Seg> segs(nSeg);
+        for (int k = 0; k < nSeg; k++) segs[k] = {seg[2 * k], seg[2 * k + 1]};
+        axs_cdt* h = new axs_cdt();
+        try {
+            h->r = axs::cdt::generate(pts, segs, prm);
+        } catch (const axs::cdt::MeshError& e) {
+            delete h;
+            throw std::invalid_argument(e.what());
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+axs_status axs_cdt_insert_points(axs_cdt** out, int n, const double* x, const double* y) {
+    if (!out || n <= 0 || !x || !y) return AXS_ERR_INVALID;
+    *out = nullptr;
+    return guarded(nullptr, [&] {
+        axs_cdt* h = new axs_cdt();
+        h->r.points.resize(n);
+        for (int i = 0; i < n; i++) h->r.points[i] = {x[i], y[i]};
+        try {
+            h->r.tris = axs::cdt::insertAll(h->r.points);
+        } catch (const axs::cdt::MeshError& e) {
+            delete h;
+            throw std::invalid_argument(e.what());
+        }
+        *out = h;
+    });
+}
+
+axs_status axs_cdt_counts(const axs_cdt* h, int* nPoints, int* nTris, int* nSeg) {
+    if (!h) return AXS_ERR_INVALID;
+    if (nPoints) *nPoints = (int)h->r.points.size();
+    if (nTris) *nTris = (int)h->r.tris.size();
+    if (nSeg) *nSeg = (int)h->r.segs.size();
+    return AXS_OK;
+}
+
+axs_status axs_cdt_get(const axs_cdt* h, double* x, double* y, int* triV, int* triAdj, int* seg) {
+    if (!h) return AXS_ERR_INVALID;
+    const auto& r = h->r;
+    for (size_t i = 0; i < r.points.size(); i++) {
+        if (x) x[i] = r.points[i].x;
+        if (y) y[i] = r.points[i].y;
+    }
+    for (size_t t = 0; t < r.tris.size(); t++)
+        for (int k = 0; k < 3; k++) {
+            if (triV) triV[3 * t + k] = r.tris[t].v[k];
+            if (triAdj) triAdj[3 * t + k] = r.tris[t].adj[k];
+        }
+    if (seg)
+        for (size_t k = 0; k < r.segs.size(); k++) {
+            seg[2 * k] = r.segs[k].a;
+            seg[2 * k + 1] = r.segs[k].b;
+        }
+    return AXS_OK;
+}
+
+void axs_cdt_destroy(axs_cdt* h) { delete h; }
 
 // Quality::buildQuality (solver/source/quality.cpp:160-166) on the GPU: the mesh
 // arrays and the cells' corner rings go up, one thread per cell computes the

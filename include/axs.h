@@ -25,6 +25,7 @@
  *   axs_build_coloring            buildMeshColoring                           manager/source/memory_manger.cpp:334-409
  *   axs_mesh_*                    FVMesh producers: toPackedMesh / createUnstructuredMesh
  *                                                                             structs/multiblock.h:59-444, solver/source/mesh.cpp:1835-2002
+ *   axs_cdt_*                     AxiMesh::generateMesh / insertPoints        aximesh/include/aximesh/aximesh.h:87-100, aximesh/source/aximesh.cpp
  */
 #ifndef AXS_H
 #define AXS_H
@@ -492,6 +493,49 @@ axs_status axs_mesh_get_triangles(const axs_mesh* m, int* n_points, double* poin
  * for a mesh without them.  NULL arrays query the count. */
 axs_status axs_mesh_get_quads(const axs_mesh* m, int* n_cells, double* z4, double* r4);
 void axs_mesh_destroy(axs_mesh* m);
+
+/* ---------------- Unstructured mesher (aximesh) ---------------- */
+/* AxiMesh::Params (aximesh/include/aximesh/aximesh.h:56-61).  max_points is
+ * not in the reference: refinement stops with AXS_ERR_INVALID when the mesh
+ * would pass that many points (0: 2^25) instead of exhausting memory. */
+typedef struct axs_cdt_params {
+    double B;             /* skinny bound of the reference's disabled Ruppert pass; unused */
+    double beta;          /* size-field grading: |h(v) - h(w)| <= beta |vw| */
+    double classify_tol;  /* a triangle is accepted when sqrt(3) R <= classify_tol * mean h */
+    long long max_points;
+} axs_cdt_params;
+/* The reference defaults: B 1.4, beta 0.2, classify_tol 1.4. */
+void axs_cdt_params_defaults(axs_cdt_params* p);
+
+typedef struct axs_cdt axs_cdt;
+/* AxiMesh::generateMesh(points, segments, size, params)
+ * (aximesh/source/aximesh.cpp:1105-1175; called by
+ * Mesh::runAxiMeshTriangulation, solver/source/mesh.cpp:1208-1251): the
+ * constrained Delaunay triangulation of n_points points with the n_segments
+ * segments segments[2k] - segments[2k+1] (caller indices), the outside removed
+ * by a parity flood fill across the segments, refined by Rebay's frontal
+ * method to the size field of the segment lengths.  The result is the
+ * reference's, index for index and bit for bit: points in bin order followed
+ * by the refinement points, triangles (counter-clockwise, adj[i] across edge
+ * v[i] -> v[i+1], -1 on the boundary), segments renumbered.  Failures the
+ * reference throws (a point search or size field that does not settle:
+ * "point search has stalled") and inputs on which the reference's behaviour
+ * is undefined (no area, a segment from a point to itself, non-finite
+ * coordinates) return AXS_ERR_INVALID, indices out of range AXS_ERR_RANGE;
+ * the text is in axs_last_error(NULL).  params NULL: the defaults. */
+axs_status axs_cdt_generate(axs_cdt** out, int n_points, const double* x, const double* y,
+                            int n_segments, const int* segments, const axs_cdt_params* params);
+/* AxiMesh::insertPoints(points, size) (aximesh.cpp:466-489): the super
+ * triangle (-100,-100) (100,-100) (0,100), appended as points n..n+2, then
+ * each point inserted in the given order with no scaling or sorting; 2n + 1
+ * triangles. */
+axs_status axs_cdt_insert_points(axs_cdt** out, int n_points, const double* x, const double* y);
+/* Sizes of a result (any pointer may be NULL). */
+axs_status axs_cdt_counts(const axs_cdt* m, int* n_points, int* n_triangles, int* n_segments);
+/* Copies of a result: x, y n_points each; tri_v, tri_adj 3 per triangle;
+ * segments 2 per segment.  NULL arrays are skipped. */
+axs_status axs_cdt_get(const axs_cdt* m, double* x, double* y, int* tri_v, int* tri_adj, int* segments);
+void axs_cdt_destroy(axs_cdt* m);
 
 /* ---------------- Output formats (manager/source/file_manager.cpp) ---------------- */
 /* The cell polygons a solution export writes beside the fields: corners 4 (quads
