@@ -26,6 +26,7 @@ TIME_BDF1, TIME_BDF2 = range(2)
 NORM_L1, NORM_L2, NORM_LINF = range(3)
 SCALE_NONE, SCALE_N, SCALE_SQRT_N, SCALE_DIAGONAL, SCALE_CONTINUITY = range(5)
 PREC_FP64, PREC_MIXED = range(2)
+REFINE_CYCLE, REFINE_SOLVE = range(2)
 MG_SMOOTH_JACOBI, MG_SMOOTH_RBGS = range(2)
 VAR_U, VAR_V, VAR_P, VAR_T, VAR_C = range(5)
 RES_U, RES_V, RES_CONTINUITY, RES_TEMPERATURE, RES_CONCENTRATION = range(5)
@@ -94,6 +95,7 @@ class Config(C.Structure):
         ("save_key_frame_iter", C.c_int), ("max_time_frames", C.c_int),
         ("solve_energy", C.c_int), ("solve_concentration", C.c_int),
         ("cp", C.c_double), ("k", C.c_double), ("diffusivity", C.c_double),
+        ("mixed_refine", C.c_int),
     ]
 
 
@@ -197,6 +199,7 @@ def load_library(path: os.PathLike | None = None) -> C.CDLL:
         "axs_mg_levels": (C.c_int, [S]),
         "axs_mg_level_info": (C.c_int, [S, C.c_int, _ip, _ip]),
         "axs_mg_tail_info": (C.c_int, [S, C.c_int, _ip, _ip]),
+        "axs_mg_set_mixed": (C.c_int, [S, C.c_int, C.c_int]),
         "axs_mg_get_maps": (C.c_int, [S, C.c_int, _ip, _ip, _ip, _ip]),
         "axs_mg_solve": (C.c_int, [S, _dp, _dp, _dp, _dp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
                                    C.c_int, C.c_int, C.c_int]),
@@ -863,6 +866,11 @@ class Multigrid:
         _check(lib().axs_mg_get_maps(self._h, level, _iptr(fs), _iptr(fn),
                                      _iptr(c2c) if c2c.size else None, _iptr(s2s) if s2s.size else None))
         return {"face_start": fs, "face_neighbor": fn, "cell_to_coarse": c2c, "slot_to_coarse_slot": s2s}
+
+    def set_mixed(self, refine: int = REFINE_CYCLE, fuse_level0: bool = True) -> None:
+        """Mixed-precision refinement (REFINE_CYCLE / REFINE_SOLVE) of the
+        solves below, and whether level 0 may run fused across cycles."""
+        _check(lib().axs_mg_set_mixed(self._h, refine, int(fuse_level0)))
 
     def solve(self, AC, AF, b, x, cycles=3, weight=0.6, pre=1, post=1, coarse=30, precision=PREC_FP64,
               coarse_direct=0, smoother=MG_SMOOTH_JACOBI) -> np.ndarray:

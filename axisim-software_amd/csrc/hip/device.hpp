@@ -124,6 +124,12 @@ struct DState {
     // FP32 copies written by kMomentum in mixed precision (null otherwise)
     float* AF32;
     float2* uvInv32;  // (1/AC_u, 1/AC_v) interleaved
+    // FP32 copies of the p' system for the mixed cycles refined once per solve
+    // (null otherwise): kPPCoeff writes AC and AF, kPPRhs b when it zeroes p'
+    float *ppAC32, *ppAF32, *ppb32;
+    // the momentum defect b - A u0 (u, v) in FP32, written by kMomentum for the
+    // mixed path's fused FP32 sweeps (null otherwise)
+    float2* momR32;
     double *partial;   // per-block residual partial sums (9 per block, then 8 of the scalars)
     double *hist;      // residual history: kHistStride values per check
     Ctl* ctl;
@@ -277,9 +283,14 @@ void launchMGProlong(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl*
 // `sweeps` weighted-Jacobi sweeps (after the prolongation when `prolong`), then
 // the residual into f.res; the final x in f.xNew.  Supported: 1 sweep without
 // the prolongation (a cycle's start), 2 sweeps after it (one cycle's end and
-// the next one's start).
+// the next one's start).  fp32: on the FP32 copies (x32, b32, res32, ...).
 void launchMGLevel0Block(hipStream_t s, const DLevel& f, const DLevel& c, const DSweepBlocks& sb, int sweeps,
-                         bool prolong, double w, const Ctl* ctl);
+                         bool prolong, double w, bool fp32, const Ctl* ctl);
+// the first FP32 launch of a solve refined once (1 sweep from e = 0, then the
+// residual): defect: its right-hand side is the FP64 defect b - A x formed by
+// the launch (and written to b32), else b32 holds it already and x = 0
+void launchMGLevel0BlockStart32(hipStream_t s, const DLevel& f, const DLevel& c, const DSweepBlocks& sb, double w,
+                                bool defect, const Ctl* ctl);
 // multi-colour Gauss-Seidel smoothing: one colour (order[begin .. begin+count)),
 // or every sweep and colour of a level of at most kSmoothBlockGSMax cells in one
 // workgroup (colorStart: device copy, nColors + 1 entries)
@@ -308,7 +319,7 @@ struct L1Fused {
     const int16_t* loc = nullptr;  // nAggBlocks * kBlock * kL1Slots: LDS index of each slot's neighbour, -1: none
 };
 void launchMGRestrictFused01(hipStream_t s, const DLevel& l0, const DLevel& l1, const DLevel& l2, L1Fused t,
-                             double w, int sweep1, int sweep2, const Ctl* ctl);
+                             double w, int sweep1, int sweep2, bool fp32, const Ctl* ctl);
 // residual per fine cell + ordered gather per coarse cell (sweep1: also the
 // coarse level's first smoothing sweep from zero); fp32: the FP32 copies
 void launchMGResidual(hipStream_t s, const DLevel& f, const Ctl* ctl);

@@ -454,7 +454,9 @@ __global__ void __launch_bounds__(kBlock) kGradField(DMesh m, DBC bc, const doub
 // HO: higher-order convection (the neighbour's u, v are read); the upwind
 // instance holds fewer registers.  PG: the pressure gradient is computed
 // here (else kGradUV has already written it, from the same p).
-template <int B, bool HO, bool PG>
+// DEF: also the FP64 defect b - A u0 into s.momR32 (the mixed path's fused
+// FP32 sweeps start from it)
+template <int B, bool HO, bool PG, bool DEF>
 __global__ void __launch_bounds__(kBlock) kMomentum(DMesh m, DBC bu, DBC bv, DBC bp, DState s, Physics ph) {
     if (finished(s.ctl)) return;
     const int n = xcdBlock() * blockDim.x + threadIdx.x;
@@ -464,7 +466,9 @@ __global__ void __launch_bounds__(kBlock) kMomentum(DMesh m, DBC bu, DBC bv, DBC
     int k0, ns;
     rowRange(m.faceStart, m.uniformSlots, n, k0, ns);
     const bool needW = ph.nonOrtho || ph.scheme == 1;
-    const bool needUV = HO;
+    // the neighbours' u, v: for higher-order convection, and for the FP64
+    // defect of the mixed path's first sweep
+    const bool needUV = HO || DEF;
     const double uN = s.u[n], vN = s.v[n];
     const double guzN = ph.nonOrtho ? s.gUZ[n] : 0.0, gurN = ph.nonOrtho ? s.gUR[n] : 0.0;
     const double gvzN = ph.nonOrtho ? s.gVZ[n] : 0.0, gvrN = ph.nonOrtho ? s.gVR[n] : 0.0;
@@ -513,10 +517,13 @@ __global__ void __launch_bounds__(kBlock) kMomentum(DMesh m, DBC bu, DBC bv, DBC
         boundaryTransport(m, bv, n, f, area, mu, F, acV, bV);
         return 0.0;
     };
+    // kept for the defect at the end (batched rows)
+    int o[B];
+    double uo[B], vo[B], af[B];
     if (ns > 0 && ns <= B) {
-        int o[B], fid[B];
+        int fid[B];
         uint8_t fl[B];
-        double area[B], inv[B], w[B], md[B], uo[B], vo[B], af[B];
+        double area[B], inv[B], w[B], md[B];
         double noU[B], noV[B];
 #pragma unroll
         for (int q = 0; q < B; q++) o[q] = m.slotNb[slotAt(k0, ns, q)];
@@ -639,6 +646,29 @@ __global__ void __launch_bounds__(kBlock) kMomentum(DMesh m, DBC bu, DBC bv, DBC
     s.DU[n] = V * iU;
     s.DV[n] = V * iV;
     if (s.uvInv32) s.uvInv32[n] = make_float2((float)iU, (float)iV);
+    if (DEF) {
+        // the FP64 defect b - A u0 of the mixed path's FP32 sweeps, as
+        // kMomDefect forms it (diagonal term first, then the slots in order)
+        double au = acU * uN, av = acV * vN;
+        if (ns > 0 && ns <= B) {
+#pragma unroll
+            for (int q = 0; q < B; q++)
+                if (q < ns && o[q] >= 0) {
+                    au += af[q] * uo[q];
+                    av += af[q] * vo[q];
+                }
+        } else {
+            for (int k = k0; k < k0 + ns; k++) {
+                const int nb = m.slotNb[k];
+                if (nb >= 0) {
+                    const double a = s.AFuv[k];
+                    au += a * s.u[nb];
+                    av += a * s.v[nb];
+                }
+            }
+        }
+        s.momR32[n] = make_float2((float)(bU - au), (float)(bV - av));
+    }
 }
 
 // ===========================================================================
@@ -930,15 +960,22 @@ __global__ void __launch_bounds__(kBlock) kPPCoeff(DMesh m, DBC bp, DState s, Ph
             if (q < ns) af[q] = step(k0 + q, o[q], area[q], nz[q], nr[q], w[q], inv[q], du[q], dv[q]);
 #pragma unroll
         for (int q = 0; q < B; q++)
-            if (q < ns) s.ppAF[k0 + q] = af[q];
+            if (q < ns) {
+                s.ppAF[k0 + q] = af[q];
+                if (s.ppAF32) s.ppAF32[k0 + q] = (float)af[q];
+            }
     } else {
         for (int k = k0; k < k0 + ns; k++) {
             const int o = m.slotNb[k];
             const int oc = o >= 0 ? o : n;
-            s.ppAF[k] = step(k, o, m.sArea[k], m.sNZ[k], m.sNR[k], m.sW[k], m.sInvDPN[k], s.DU[oc], s.DV[oc]);
+            const double af =
+                step(k, o, m.sArea[k], m.sNZ[k], m.sNR[k], m.sW[k], m.sInvDPN[k], s.DU[oc], s.DV[oc]);
+            s.ppAF[k] = af;
+            if (s.ppAF32) s.ppAF32[k] = (float)af;
         }
     }
     s.ppAC[n] = ac;
+    if (s.ppAC32) s.ppAC32[n] = (float)ac;
     s.ppInv[n] = (fabs(ac) < 1.0e-30) ? 0.0 : 1.0 / ac;
 }
 
@@ -1020,6 +1057,7 @@ __global__ void __launch_bounds__(kBlock) kPPRhs(DMesh m, DState s, Physics ph, 
         }
     }
     s.ppb[n] = -imb + cs;
+    if (zero && s.ppb32) s.ppb32[n] = (float)(-imb + cs);
 }
 
 // computeGradient(p', ppBC) fused with updateMomentumPressure (simple.cu:166-179)
@@ -2034,17 +2072,17 @@ __global__ void __launch_bounds__(kBlock) kMGProlongSmooth(DLevel f, DLevel c, T
 // The final x goes to f.xNew.  One launch replaces the post-sweep of cycle c,
 // the pre-sweep of cycle c + 1 and its residual, each row evaluated as in the
 // one-pass kernels, so the result is bit-identical.
-template <int R>
+template <class T, int R>
 struct MGRowReg {
-    double af[R];
+    T af[R];
     int nb[R];
     int loc[R];  // LDS slots as in SweepRow
     int k0, ns;
-    double ac, b;
+    T ac, b;
 };
 
-template <int R>
-__device__ __forceinline__ void loadMGRow(MGRowReg<R>& r, const DLevel& L, int c) {
+template <class T, int R>
+__device__ __forceinline__ void loadMGRow(MGRowReg<T, R>& r, const DLevel& L, const LV<T>& v, int c) {
     int k0, ns;
     rowRange(L.start, L.uniformRow, c, k0, ns);
     r.k0 = k0;
@@ -2054,35 +2092,77 @@ __device__ __forceinline__ void loadMGRow(MGRowReg<R>& r, const DLevel& L, int c
     for (int q = 0; q < R; q++) {
         const int k = ns > 0 ? k0 + min(q, ns - 1) : 0;
         const int nb = L.nb[k];
-        const double af = L.AF[k];
+        const T af = v.AF[k];
         r.nb[q] = (q < ns) ? nb : -1;
-        r.af[q] = (q < ns) ? af : 0.0;
+        r.af[q] = (q < ns) ? af : T(0);
     }
-    r.ac = L.AC[c];
-    r.b = L.b[c];
+    r.ac = v.AC[c];
+    r.b = v.b[c];
 }
 
-template <int R, bool LDS = false, typename XAt>
-__device__ __forceinline__ double mgJacobiReg(const MGRowReg<R>& r, XAt xat, double xn, double w) {
-    if (fabs(r.ac) < 1.0e-30) return xn;
-    double xv[R];
+template <class T, int R, bool LDS = false, typename XAt>
+__device__ __forceinline__ T mgJacobiReg(const MGRowReg<T, R>& r, XAt xat, T xn, T w) {
+    if (tabs(r.ac) < T(1.0e-30)) return xn;
+    T xv[R];
 #pragma unroll
     for (int q = 0; q < R; q++) xv[q] = xat(q);  // slots without a neighbour: read, not used (as in sweepRow)
-    double Ax = r.ac * xn;
+    T Ax = r.ac * xn;
 #pragma unroll
     for (int q = 0; q < R; q++)
         if (LDS ? r.loc[q] >= 0 : r.nb[q] != -1) Ax += r.af[q] * xv[q];
     return xn + w * (r.b - Ax) / r.ac;
 }
 
+// The first launch of a solve refined once (FP32 cycles on the correction e,
+// e = 0 at the start) takes its right-hand side in one of two ways:
+//   kL0Zero:   x = 0 and b32 already holds b (written with the right-hand side);
+//              no iterate is read, every x is the zero it would read;
+//   kL0Defect: the rows' right-hand side is the FP64 defect b - A x, formed from
+//              the FP64 row and x as kMGDefect forms it, rounded to FP32 (and
+//              written to b32 for the block's own cells); the FP32 row is the
+//              FP64 one rounded, as kMGCast writes it.
+enum { kL0X = 0, kL0Zero = 1, kL0Defect = 2 };
+
+template <int R>
+__device__ __forceinline__ void loadMGRowDefect(MGRowReg<float, R>& r, const DLevel& L, int c) {
+    int k0, ns;
+    rowRange(L.start, L.uniformRow, c, k0, ns);
+    r.k0 = k0;
+    r.ns = ns;
+    double a[R];
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+        const int k = ns > 0 ? k0 + min(q, ns - 1) : 0;
+        const int nb = L.nb[k];
+        const double af = L.AF[k];
+        r.nb[q] = (q < ns) ? nb : -1;
+        a[q] = (q < ns) ? af : 0.0;
+    }
+    const double ac = L.AC[c];
+    double xo[R];
+#pragma unroll
+    for (int q = 0; q < R; q++) xo[q] = L.x[r.nb[q] >= 0 ? r.nb[q] : c];
+    double Ax = ac * L.x[c];
+#pragma unroll
+    for (int q = 0; q < R; q++)
+        if (r.nb[q] >= 0) Ax += a[q] * xo[q];
+    r.b = (float)(L.b[c] - Ax);
+    r.ac = (float)ac;
+#pragma unroll
+    for (int q = 0; q < R; q++) r.af[q] = (float)a[q];
+}
+
 // held to 64 VGPRs (two 1024-thread blocks per CU), which these rows fit
-// without spilling
-template <int C, int S, bool P, bool RES, int R>
+// without spilling.  T: FP64, or the FP32 copies (the mixed path's cycles).
+template <class T, int C, int S, bool P, bool RES, int R, int X0 = kL0X>
 __global__ void __launch_bounds__(C) __attribute__((amdgpu_waves_per_eu(8)))
-kMGLevel0Block(DLevel f, DLevel c, DSweepBlocks sb, double w, const Ctl* ctl) {
+kMGLevel0Block(DLevel f, DLevel c, DSweepBlocks sb, T w, const Ctl* ctl) {
+    static_assert(X0 == kL0X || (!P && sizeof(T) == 4), "zero / defect starts are FP32 first launches");
     constexpr int H = sweepHalo<C>();
     constexpr int kSets = (S - 1 + (RES ? 1 : 0)) > 1 ? 2 : 1;  // LDS iterate buffers in use
-    __shared__ double buf[kSets][C + H];
+    __shared__ T buf[kSets][C + H];
+    const LV<T> v = view<T>(f);
+    const LV<T> cv = view<T>(c);
     if (finished(ctl)) return;
     const int t = threadIdx.x, blk = xcdBlock();
     const int a = blk * C, nOwn = min(C, f.n - a);
@@ -2095,27 +2175,38 @@ kMGLevel0Block(DLevel f, DLevel c, DSweepBlocks sb, double w, const Ctl* ctl) {
     const int n = sb.own ? sb.own[(size_t)blk * C + t] : a + t;
     const bool mine = sb.own ? n >= 0 : t < nOwn, haloRow = t < nHalo1;
     const int hc = haloRow ? sb.halo[h0 + t] : 0;
-    MGRowReg<R> own, hal;
-    if (mine) loadMGRow(own, f, n);
-    if (haloRow) loadMGRow(hal, f, hc);
+    MGRowReg<T, R> own, hal;
+    if constexpr (X0 == kL0Defect) {
+        if (mine) {
+            loadMGRowDefect(own, f, n);
+            v.b[n] = own.b;
+        }
+        if (haloRow) loadMGRowDefect(hal, f, hc);
+    } else {
+        if (mine) loadMGRow(own, f, v, n);
+        if (haloRow) loadMGRow(hal, f, v, hc);
+    }
     // the first sweep's input at any cell: x, plus the prolonged correction
-    auto x0 = [&](int cell) -> double { return P ? f.x[cell] + c.x[f.toCoarse[cell]] : f.x[cell]; };
+    auto x0 = [&](int cell) -> T {
+        if constexpr (X0 != kL0X) return T(0);
+        else return P ? v.x[cell] + cv.x[f.toCoarse[cell]] : v.x[cell];
+    };
     // sweep j (1-based) writes LDS buffer (j - 1) % kSets; the last one without
     // a residual writes f.xNew directly
-    double lastOwn = 0.0;
+    T lastOwn = T(0);
     {
-        const double v =
-            mine ? mgJacobiReg(own, [&](int q) { return x0(own.nb[q] != -1 ? own.nb[q] : n); }, x0(n), w) : 0.0;
-        const double vh =
+        const T vo = mine ? mgJacobiReg(own, [&](int q) { return x0(own.nb[q] != -1 ? own.nb[q] : n); }, x0(n), w)
+                          : T(0);
+        const T vh =
             haloRow ? mgJacobiReg(hal, [&](int q) { return x0(hal.nb[q] != -1 ? hal.nb[q] : hc); }, x0(hc), w)
-                    : 0.0;
+                    : T(0);
         if (S == 1 && !RES) {
-            if (mine) f.xNew[n] = v;
+            if (mine) v.xNew[n] = vo;
             return;
         }
-        if (mine) buf[0][t] = v;
+        if (mine) buf[0][t] = vo;
         if (haloRow) buf[0][C + t] = vh;
-        lastOwn = v;
+        lastOwn = vo;
     }
     // the LDS slots the later sweeps and the residual read, loaded while the
     // barrier waits
@@ -2124,37 +2215,37 @@ kMGLevel0Block(DLevel f, DLevel c, DSweepBlocks sb, double w, const Ctl* ctl) {
     __syncthreads();
 #pragma unroll
     for (int j = 2; j <= S; j++) {
-        const double* in = buf[(j - 2) % kSets];
-        double* out = buf[(j - 1) % kSets];
-        const double v =
-            mine ? mgJacobiReg<R, true>(own, [&](int q) { return in[own.loc[q] >= 0 ? own.loc[q] : t]; }, in[t], w)
-                 : 0.0;
+        const T* in = buf[(j - 2) % kSets];
+        T* out = buf[(j - 1) % kSets];
+        const T vo =
+            mine ? mgJacobiReg<T, R, true>(own, [&](int q) { return in[own.loc[q] >= 0 ? own.loc[q] : t]; }, in[t], w)
+                 : T(0);
         const int nHaloJ = (S - j + (RES ? 1 : 0)) > 0 ? sb.ringEnd[blk * sb.rings + (S - j + (RES ? 1 : 0)) - 1] : 0;
-        const double vh =
-            (t < nHaloJ)
-                ? mgJacobiReg<R, true>(hal, [&](int q) { return in[hal.loc[q] >= 0 ? hal.loc[q] : C + t]; }, in[C + t], w)
-                : 0.0;
+        const T vh = (t < nHaloJ) ? mgJacobiReg<T, R, true>(
+                                        hal, [&](int q) { return in[hal.loc[q] >= 0 ? hal.loc[q] : C + t]; },
+                                        in[C + t], w)
+                                  : T(0);
         if (j == S && !RES) {
-            if (mine) f.xNew[n] = v;
+            if (mine) v.xNew[n] = vo;
             return;
         }
-        if (mine) out[t] = v;
+        if (mine) out[t] = vo;
         if (t < nHaloJ) out[C + t] = vh;
-        lastOwn = v;
+        lastOwn = vo;
         __syncthreads();
     }
     // residual (kMGResidual) from the last sweep's iterate
     if (RES && mine) {
-        const double* xs = buf[(S - 1) % kSets];
-        f.xNew[n] = lastOwn;
-        double xv[R];
+        const T* xs = buf[(S - 1) % kSets];
+        v.xNew[n] = lastOwn;
+        T xv[R];
 #pragma unroll
         for (int q = 0; q < R; q++) xv[q] = xs[own.loc[q] >= 0 ? own.loc[q] : t];
-        double Ax = own.ac * lastOwn;
+        T Ax = own.ac * lastOwn;
 #pragma unroll
         for (int q = 0; q < R; q++)
             if (own.loc[q] >= 0) Ax += own.af[q] * xv[q];
-        f.res[n] = own.b - Ax;
+        v.res[n] = own.b - Ax;
     }
 }
 
@@ -2238,60 +2329,64 @@ __global__ void __launch_bounds__(kBlock) kMGResidualRestrict(DLevel f, DLevel c
 // residual of its row reads the x it needs from LDS through the host-built
 // slot table (t.loc), in slot order as in kMGResidual; after the next barrier
 // the level-2 sums run as in kMGResidualRestrict.  Same values, bit for bit.
-template <int B>
-__device__ __forceinline__ double memberSum(const DLevel& g, int i) {
+template <class T, int B>
+__device__ __forceinline__ T memberSum(const DLevel& g, const T* res, int i) {
     const int j0 = g.memberStart[i], nm = g.memberStart[i + 1] - j0;
     int mb[B];
-    double r[B];
+    T r[B];
 #pragma unroll
     for (int q = 0; q < B; q++) mb[q] = g.members[nm > 0 ? slotAt(j0, nm, q) : 0];
 #pragma unroll
-    for (int q = 0; q < B; q++) r[q] = g.res[mb[q]];
-    double acc = 0.0;
+    for (int q = 0; q < B; q++) r[q] = res[mb[q]];
+    T acc = T(0);
 #pragma unroll
     for (int q = 0; q < B; q++)
         if (q < nm) acc += r[q];
-    for (int j = j0 + B; j < j0 + nm; j++) acc += g.res[g.members[j]];
+    for (int j = j0 + B; j < j0 + nm; j++) acc += res[g.members[j]];
     return acc;
 }
 
-__global__ void __launch_bounds__(kBlock) kMGRestrictFused01(DLevel g, DLevel f, DLevel c, L1Fused tb, double w,
+template <class T>
+__global__ void __launch_bounds__(kBlock) kMGRestrictFused01(DLevel g, DLevel f, DLevel c, L1Fused tb, T w,
                                                              int sweep1, int sweep2, const Ctl* ctl) {
-    __shared__ double xs[2 * kBlock];
-    __shared__ double r[kBlock];
+    __shared__ T xs[2 * kBlock];
+    __shared__ T r[kBlock];
     if (finished(ctl)) return;
+    const LV<T> gv = view<T>(g);
+    const LV<T> fv = view<T>(f);
+    const LV<T> cv = view<T>(c);
     const int blk = xcdBlock(), t = threadIdx.x;
     const AggThread me = f.aggThreads[(size_t)blk * kBlock + t];
     const int hc = tb.halo[(size_t)blk * kBlock + t];
     const int j = f.aggBlocks[blk].j0 + t;
-    double bOwn = 0.0, xOwn = 0.0;
+    T bOwn = T(0), xOwn = T(0);
     if (me.m >= 0) {
-        bOwn = memberSum<6>(g, me.m);
-        xOwn = sweep1 ? sweepFromZero(bOwn, f.AC[me.m], w) : 0.0;
-        f.b[me.m] = bOwn;
-        f.x[me.m] = xOwn;
+        bOwn = memberSum<T, 6>(g, gv.res, me.m);
+        xOwn = sweep1 ? sweepFromZero(bOwn, fv.AC[me.m], w) : T(0);
+        fv.b[me.m] = bOwn;
+        fv.x[me.m] = xOwn;
         xs[t] = xOwn;
     }
     if (hc >= 0) {
-        const double bh = memberSum<6>(g, hc);
-        xs[kBlock + t] = sweep1 ? sweepFromZero(bh, f.AC[hc], w) : 0.0;
+        const T bh = memberSum<T, 6>(g, gv.res, hc);
+        xs[kBlock + t] = sweep1 ? sweepFromZero(bh, fv.AC[hc], w) : T(0);
     }
-    const double acj = (sweep2 && me.sum >= 0) ? c.AC[j] : 0.0;
+    const T acj = (sweep2 && me.sum >= 0) ? cv.AC[j] : T(0);
     __syncthreads();
     if (me.m >= 0) {
         constexpr int K = kL1Slots;
         const int k0 = me.k0, ns = me.ns;
         int lc[K];
-        double a[K], xv[K];
+        T a[K], xv[K];
         const int16_t* l = tb.loc + ((size_t)blk * kBlock + t) * K;
 #pragma unroll
         for (int q = 0; q < K; q++) {
             lc[q] = l[q];
-            a[q] = f.AF[ns > 0 ? slotAt(k0, ns, q) : 0];
+            a[q] = fv.AF[ns > 0 ? slotAt(k0, ns, q) : 0];
         }
 #pragma unroll
         for (int q = 0; q < K; q++) xv[q] = xs[lc[q] >= 0 ? lc[q] : t];
-        double Ax = f.AC[me.m] * xOwn;
+        T Ax = fv.AC[me.m] * xOwn;
 #pragma unroll
         for (int q = 0; q < K; q++)
             if (q < ns && lc[q] >= 0) Ax += a[q] * xv[q];
@@ -2300,10 +2395,10 @@ __global__ void __launch_bounds__(kBlock) kMGRestrictFused01(DLevel g, DLevel f,
     __syncthreads();
     if (me.sum >= 0) {
         const int a = me.sum & 0xffff, e = me.sum >> 16;
-        double acc = 0.0;
+        T acc = T(0);
         for (int q = a; q < e; q++) acc += r[q];
-        c.b[j] = acc;
-        c.x[j] = sweep2 ? sweepFromZero(acc, acj, w) : 0.0;
+        cv.b[j] = acc;
+        cv.x[j] = sweep2 ? sweepFromZero(acc, acj, w) : T(0);
     }
 }
 
@@ -2764,10 +2859,11 @@ void launchMGResidualRestrict(hipStream_t s, const DLevel& f, const DLevel& c, d
     else launchMGResidualRestrictT<double>(s, f, c, w, sweep1, ctl);
 }
 void launchMGRestrictFused01(hipStream_t s, const DLevel& l0, const DLevel& l1, const DLevel& l2, L1Fused t,
-                             double w, int sweep1, int sweep2, const Ctl* ctl) {
+                             double w, int sweep1, int sweep2, bool fp32, const Ctl* ctl) {
     if (l1.nAggBlocks <= 0 || !t.halo || !t.loc || l1.maxRow > kL1Slots)
         throw std::logic_error("level 1 has no fused restriction tables");
-    kMGRestrictFused01<<<l1.nAggBlocks, kBlock, 0, s>>>(l0, l1, l2, t, w, sweep1, sweep2, ctl);
+    if (fp32) kMGRestrictFused01<float><<<l1.nAggBlocks, kBlock, 0, s>>>(l0, l1, l2, t, (float)w, sweep1, sweep2, ctl);
+    else kMGRestrictFused01<double><<<l1.nAggBlocks, kBlock, 0, s>>>(l0, l1, l2, t, w, sweep1, sweep2, ctl);
 }
 void launchMGRestrictSum(hipStream_t s, const DLevel& f, const DLevel& c, double w, int sweep1, bool fp32,
                          const Ctl* ctl) {
@@ -2802,7 +2898,9 @@ void launchGradField(hipStream_t s, const DMesh& m, DBC bc, const double* phi, d
 void launchMomentum(hipStream_t s, const DMesh& m, DBC bu, DBC bv, DBC bp, DState st, Physics ph, bool gradPDone) {
     const dim3 g(blocksFor(m.N)), b(kBlock);
     const bool ho = ph.scheme != 0;
-#define AXS_MOMENTUM(BB, HO, PG) kMomentum<BB, HO, PG><<<g, b, 0, s>>>(m, bu, bv, bp, st, ph)
+#define AXS_MOMENTUM(BB, HO, PG)                                              \
+    (st.momR32 ? kMomentum<BB, HO, PG, true><<<g, b, 0, s>>>(m, bu, bv, bp, st, ph) \
+               : kMomentum<BB, HO, PG, false><<<g, b, 0, s>>>(m, bu, bv, bp, st, ph))
     if (m.maxSlots <= 3) {
         if (ho) gradPDone ? AXS_MOMENTUM(3, true, false) : AXS_MOMENTUM(3, true, true);
         else gradPDone ? AXS_MOMENTUM(3, false, false) : AXS_MOMENTUM(3, false, true);
@@ -2947,31 +3045,50 @@ void launchMGSmoothBlock(hipStream_t s, const DLevel& L, double w, int sweeps, c
 void launchMGProlongSmooth(hipStream_t s, const DLevel& f, const DLevel& c, double w, const Ctl* ctl) {
     launchMGProlongSmoothT<double>(s, f, c, w, ctl);
 }
-void launchMGLevel0Block(hipStream_t s, const DLevel& f, const DLevel& c, const DSweepBlocks& sb, int sweeps,
-                         bool prolong, double w, const Ctl* ctl) {
-    if (sb.cells != 512 && sb.cells != 1024) throw std::invalid_argument("fused level-0 sweeps take 512 or 1024 cells");
-    const bool tri = f.maxRow <= 3;
-    const int C = sb.cells;
+namespace {
+template <class T, int C, int S, bool P, int X0 = kL0X>
+void launchMGLevel0BlockT(hipStream_t s, const DLevel& f, const DLevel& c, const DSweepBlocks& sb, T w,
+                          const Ctl* ctl) {
+    if (f.maxRow <= 3) kMGLevel0Block<T, C, S, P, true, 3, X0><<<sb.nBlocks, C, 0, s>>>(f, c, sb, w, ctl);
+    else kMGLevel0Block<T, C, S, P, true, kSweepRegSlots, X0><<<sb.nBlocks, C, 0, s>>>(f, c, sb, w, ctl);
+}
+template <class T, int S, bool P>
+void launchMGLevel0BlockS(hipStream_t s, const DLevel& f, const DLevel& c, const DSweepBlocks& sb, T w,
+                          const Ctl* ctl) {
+    if (sb.cells == 512) launchMGLevel0BlockT<T, 512, S, P>(s, f, c, sb, w, ctl);
+    else launchMGLevel0BlockT<T, 1024, S, P>(s, f, c, sb, w, ctl);
+}
+template <class T>
+void launchMGLevel0BlockV(hipStream_t s, const DLevel& f, const DLevel& c, const DSweepBlocks& sb, int sweeps,
+                          bool prolong, T w, const Ctl* ctl) {
     if (sweeps == 1 && !prolong) {
         if (sb.rings < 1) throw std::invalid_argument("fused level-0 sweeps need one halo ring");
-        if (C == 512) {
-            if (tri) kMGLevel0Block<512, 1, false, true, 3><<<sb.nBlocks, C, 0, s>>>(f, c, sb, w, ctl);
-            else kMGLevel0Block<512, 1, false, true, kSweepRegSlots><<<sb.nBlocks, C, 0, s>>>(f, c, sb, w, ctl);
-        } else {
-            if (tri) kMGLevel0Block<1024, 1, false, true, 3><<<sb.nBlocks, C, 0, s>>>(f, c, sb, w, ctl);
-            else kMGLevel0Block<1024, 1, false, true, kSweepRegSlots><<<sb.nBlocks, C, 0, s>>>(f, c, sb, w, ctl);
-        }
+        launchMGLevel0BlockS<T, 1, false>(s, f, c, sb, w, ctl);
     } else if (sweeps == 2 && prolong) {
         if (sb.rings < 2) throw std::invalid_argument("fused level-0 sweeps need two halo rings");
-        if (C == 512) {
-            if (tri) kMGLevel0Block<512, 2, true, true, 3><<<sb.nBlocks, C, 0, s>>>(f, c, sb, w, ctl);
-            else kMGLevel0Block<512, 2, true, true, kSweepRegSlots><<<sb.nBlocks, C, 0, s>>>(f, c, sb, w, ctl);
-        } else {
-            if (tri) kMGLevel0Block<1024, 2, true, true, 3><<<sb.nBlocks, C, 0, s>>>(f, c, sb, w, ctl);
-            else kMGLevel0Block<1024, 2, true, true, kSweepRegSlots><<<sb.nBlocks, C, 0, s>>>(f, c, sb, w, ctl);
-        }
+        launchMGLevel0BlockS<T, 2, true>(s, f, c, sb, w, ctl);
     } else {
         throw std::invalid_argument("unsupported fused level-0 sweep combination");
+    }
+}
+}  // namespace
+void launchMGLevel0Block(hipStream_t s, const DLevel& f, const DLevel& c, const DSweepBlocks& sb, int sweeps,
+                         bool prolong, double w, bool fp32, const Ctl* ctl) {
+    if (sb.cells != 512 && sb.cells != 1024) throw std::invalid_argument("fused level-0 sweeps take 512 or 1024 cells");
+    if (fp32) launchMGLevel0BlockV<float>(s, f, c, sb, sweeps, prolong, (float)w, ctl);
+    else launchMGLevel0BlockV<double>(s, f, c, sb, sweeps, prolong, w, ctl);
+}
+void launchMGLevel0BlockStart32(hipStream_t s, const DLevel& f, const DLevel& c, const DSweepBlocks& sb, double w,
+                                bool defect, const Ctl* ctl) {
+    if (sb.cells != 512 && sb.cells != 1024) throw std::invalid_argument("fused level-0 sweeps take 512 or 1024 cells");
+    if (sb.rings < 1) throw std::invalid_argument("fused level-0 sweeps need one halo ring");
+    const float w32 = (float)w;
+    if (defect) {
+        if (sb.cells == 512) launchMGLevel0BlockT<float, 512, 1, false, kL0Defect>(s, f, c, sb, w32, ctl);
+        else launchMGLevel0BlockT<float, 1024, 1, false, kL0Defect>(s, f, c, sb, w32, ctl);
+    } else {
+        if (sb.cells == 512) launchMGLevel0BlockT<float, 512, 1, false, kL0Zero>(s, f, c, sb, w32, ctl);
+        else launchMGLevel0BlockT<float, 1024, 1, false, kL0Zero>(s, f, c, sb, w32, ctl);
     }
 }
 void launchMGProlong(hipStream_t s, const DLevel& f, const DLevel& c, const Ctl* ctl) {

@@ -178,13 +178,20 @@ __device__ __forceinline__ float2 sweepRow32(const SweepRow32<R>& w, DAt dat, fl
     return make_float2((w.inv.x != 0.0f) ? au * w.inv.x : self.x, (w.inv.y != 0.0f) ? av * w.inv.y : self.y);
 }
 
-template <int C, int S, int R>
-__global__ void __launch_bounds__(C) kMomSweep32Block(DMesh m, const Ctl* ctl, MixedState x, DSweepBlocks sb,
+// The last launch of a solve can end in u += delta, v += delta (APPLY:
+// kMomSweepApply's update after the last sweep).  The first launch starts from
+// the FP64 defect r0 = b - A u0 that kMomentum wrote into x.r (DState::momR32):
+// its first sweep is the pointwise delta_1 = r0 / AC (ZERO, kMomDefect's first
+// sweep).
+enum { kSweepPlain = 0, kSweepApply = 2, kSweepZero = 3 };
+
+template <int C, int S, int R, int MODE>
+__global__ void __launch_bounds__(C) kMomSweep32Block(DMesh m, DState st, MixedState x, DSweepBlocks sb,
                                                       const float2* __restrict__ dO, float2* __restrict__ dN) {
     constexpr int H = sweepHalo<C>();
     constexpr int kMask = S > 2 ? 1 : 0;  // sweep j < S writes LDS buffer (j - 1) & kMask
     __shared__ float2 buf[kMask + 1][C + H];
-    if (finished(ctl)) return;
+    if (finished(st.ctl)) return;
     const int t = threadIdx.x, blk = xcdBlock();
     const int a = blk * C, nOwn = min(C, m.N - a);
     const int h0 = sb.haloStart[blk];
@@ -195,11 +202,22 @@ __global__ void __launch_bounds__(C) kMomSweep32Block(DMesh m, const Ctl* ctl, M
     const bool mine = sb.own ? n >= 0 : t < nOwn, haloRow = t < nHalo1;
     const int hc = haloRow ? sb.halo[h0 + t] : 0;
     SweepRow32<R> own, hal;
-    if (mine) loadSweepRow32(own, m, x, n);
-    if (haloRow) loadSweepRow32(hal, m, x, hc);
     float2* out = buf[0];
-    if (mine) out[t] = sweepRow32(own, [&](int q) { return dO[own.nb[q] >= 0 ? own.nb[q] : n]; }, dO[n]);
-    if (haloRow) out[C + t] = sweepRow32(hal, [&](int q) { return dO[hal.nb[q] >= 0 ? hal.nb[q] : hc]; }, dO[hc]);
+    if (MODE == kSweepZero) {
+        if (mine) {
+            loadSweepRow32(own, m, x, n);
+            out[t] = make_float2(own.inv.x * own.r.x, own.inv.y * own.r.y);
+        }
+        if (haloRow) {
+            loadSweepRow32(hal, m, x, hc);
+            out[C + t] = make_float2(hal.inv.x * hal.r.x, hal.inv.y * hal.r.y);
+        }
+    } else {
+        if (mine) loadSweepRow32(own, m, x, n);
+        if (haloRow) loadSweepRow32(hal, m, x, hc);
+        if (mine) out[t] = sweepRow32(own, [&](int q) { return dO[own.nb[q] >= 0 ? own.nb[q] : n]; }, dO[n]);
+        if (haloRow) out[C + t] = sweepRow32(hal, [&](int q) { return dO[hal.nb[q] >= 0 ? hal.nb[q] : hc]; }, dO[hc]);
+    }
     if (mine) loadLoc(own, sb.locOwn + own.k0, R);
     if (t < nHaloLds) loadLoc(hal, sb.locHalo + (size_t)(h0 + t) * kSweepRegSlots, R);
     __syncthreads();
@@ -207,7 +225,14 @@ __global__ void __launch_bounds__(C) kMomSweep32Block(DMesh m, const Ctl* ctl, M
     for (int j = 2; j <= S; j++) {
         const float2* in = buf[(j - 2) & kMask];
         if (j == S) {
-            if (mine) dN[n] = sweepRow32<R, true>(own, [&](int q) { return in[own.loc[q] >= 0 ? own.loc[q] : t]; }, in[t]);
+            if (!mine) return;
+            const float2 d = sweepRow32<R, true>(own, [&](int q) { return in[own.loc[q] >= 0 ? own.loc[q] : t]; }, in[t]);
+            if (MODE == kSweepApply) {
+                st.u[n] += (double)d.x;
+                st.v[n] += (double)d.y;
+            } else {
+                dN[n] = d;
+            }
         } else {
             float2* o = buf[(j - 1) & kMask];
             const int nHaloJ = sb.ringEnd[blk * sb.rings + (S - j) - 1];
@@ -458,28 +483,41 @@ void launchCoarseApply(hipStream_t s, const DLevel& L, const double* inv, const 
 }
 
 namespace {
-template <int C, int S>
-void launchMomSweep32BlockCS(hipStream_t s, const DMesh& m, const Ctl* ctl, MixedState x, const DSweepBlocks& sb,
+template <int C, int S, int MODE>
+void launchMomSweep32BlockCS(hipStream_t s, const DMesh& m, const DState& st, MixedState x, const DSweepBlocks& sb,
                              const float2* dO, float2* dN) {
-    if (m.maxSlots <= 3) kMomSweep32Block<C, S, 3><<<sb.nBlocks, C, 0, s>>>(m, ctl, x, sb, dO, dN);
-    else kMomSweep32Block<C, S, kSweepRegSlots><<<sb.nBlocks, C, 0, s>>>(m, ctl, x, sb, dO, dN);
+    if (m.maxSlots <= 3) kMomSweep32Block<C, S, 3, MODE><<<sb.nBlocks, C, 0, s>>>(m, st, x, sb, dO, dN);
+    else kMomSweep32Block<C, S, kSweepRegSlots, MODE><<<sb.nBlocks, C, 0, s>>>(m, st, x, sb, dO, dN);
 }
-template <int C>
-void launchMomSweep32BlockC(hipStream_t s, const DMesh& m, const Ctl* ctl, MixedState x, const DSweepBlocks& sb,
+template <int C, int MODE>
+void launchMomSweep32BlockC(hipStream_t s, const DMesh& m, const DState& st, MixedState x, const DSweepBlocks& sb,
                             int sweeps, const float2* dO, float2* dN) {
-    if (sweeps == 2) launchMomSweep32BlockCS<C, 2>(s, m, ctl, x, sb, dO, dN);
-    else if (sweeps == 3) launchMomSweep32BlockCS<C, 3>(s, m, ctl, x, sb, dO, dN);
-    else if (sweeps == 4) launchMomSweep32BlockCS<C, 4>(s, m, ctl, x, sb, dO, dN);
-    else launchMomSweep32BlockCS<C, 5>(s, m, ctl, x, sb, dO, dN);
+    if (sweeps == 2) launchMomSweep32BlockCS<C, 2, MODE>(s, m, st, x, sb, dO, dN);
+    else if (sweeps == 3) launchMomSweep32BlockCS<C, 3, MODE>(s, m, st, x, sb, dO, dN);
+    else if (sweeps == 4) launchMomSweep32BlockCS<C, 4, MODE>(s, m, st, x, sb, dO, dN);
+    else launchMomSweep32BlockCS<C, 5, MODE>(s, m, st, x, sb, dO, dN);
 }
-}  // namespace
-void launchMomSweep32Block(hipStream_t s, const DMesh& m, const Ctl* ctl, MixedState x, const DSweepBlocks& sb,
-                           int sweeps, const float2* dO, float2* dN) {
+template <int MODE>
+void launchMomSweep32BlockM(hipStream_t s, const DMesh& m, const DState& st, MixedState x, const DSweepBlocks& sb,
+                            int sweeps, const float2* dO, float2* dN) {
     if (sweeps < 2 || sweeps > kMaxFusedSweeps || sweeps > sb.rings + 1)
         throw std::invalid_argument("bad fused sweep count");
-    if (sb.cells == 512) launchMomSweep32BlockC<512>(s, m, ctl, x, sb, sweeps, dO, dN);
-    else if (sb.cells == 1024) launchMomSweep32BlockC<1024>(s, m, ctl, x, sb, sweeps, dO, dN);
+    if (sb.cells == 512) launchMomSweep32BlockC<512, MODE>(s, m, st, x, sb, sweeps, dO, dN);
+    else if (sb.cells == 1024) launchMomSweep32BlockC<1024, MODE>(s, m, st, x, sb, sweeps, dO, dN);
     else throw std::invalid_argument("fused sweeps take blocks of 512 or 1024 cells");
+}
+}  // namespace
+void launchMomSweep32Block(hipStream_t s, const DMesh& m, const DState& st, MixedState x, const DSweepBlocks& sb,
+                           int sweeps, const float2* dO, float2* dN) {
+    launchMomSweep32BlockM<kSweepPlain>(s, m, st, x, sb, sweeps, dO, dN);
+}
+void launchMomSweep32BlockApply(hipStream_t s, const DMesh& m, const DState& st, MixedState x,
+                                const DSweepBlocks& sb, int sweeps, const float2* dO) {
+    launchMomSweep32BlockM<kSweepApply>(s, m, st, x, sb, sweeps, dO, nullptr);
+}
+void launchMomSweep32BlockFromDefect(hipStream_t s, const DMesh& m, const DState& st, MixedState x,
+                                     const DSweepBlocks& sb, int sweeps, float2* dN) {
+    launchMomSweep32BlockM<kSweepZero>(s, m, st, x, sb, sweeps, nullptr, dN);
 }
 
 }  // namespace axs

@@ -21,8 +21,16 @@ void launchMomDefect(hipStream_t s, const DMesh& m, DState st, MixedState x);
 void launchMomSweep32(hipStream_t s, const DMesh& m, const Ctl* ctl, MixedState x, const float2* dO, float2* dN);
 // `sweeps` (2 .. kMaxFusedSweeps, at most sb.rings + 1) FP32 sweeps in one launch
 // over the fused momentum blocks; bit-identical to as many launchMomSweep32
-void launchMomSweep32Block(hipStream_t s, const DMesh& m, const Ctl* ctl, MixedState x, const DSweepBlocks& sb,
+void launchMomSweep32Block(hipStream_t s, const DMesh& m, const DState& st, MixedState x, const DSweepBlocks& sb,
                            int sweeps, const float2* dO, float2* dN);
+// the same, its last sweep ending in u += delta, v += delta
+// (launchMomSweepApply) / its first sweep from the FP64 defect kMomentum wrote
+// into x.r (DState::momR32; launchMomDefect's first sweep); bit-identical to
+// those launches
+void launchMomSweep32BlockApply(hipStream_t s, const DMesh& m, const DState& st, MixedState x,
+                                const DSweepBlocks& sb, int sweeps, const float2* dO);
+void launchMomSweep32BlockFromDefect(hipStream_t s, const DMesh& m, const DState& st, MixedState x,
+                                     const DSweepBlocks& sb, int sweeps, float2* dN);
 void launchMomSweepApply(hipStream_t s, const DMesh& m, DState st, MixedState x, const float2* dO, int sweep);
 void launchMGDefect(hipStream_t s, const DLevel& L, double w, int firstSweep, const Ctl* ctl);
 void launchMGJacobi32(hipStream_t s, const DLevel& L, float w, const Ctl* ctl);

@@ -88,6 +88,7 @@ void axs_config_defaults(axs_config* c) {
     c->cp = 4180.0;                           // src/header/setting.cuh:66
     c->k = 0.6;                               // :67
     c->diffusivity = 3.0277e-9;               // :69
+    c->mixed_refine = AXS_REFINE_SOLVE;
 }
 
 axs_status axs_create(axs_solver** out, int device) {
@@ -334,11 +335,20 @@ static MGParams mgParams(int cycles, double weight, int pre, int post, int coars
     return p;
 }
 
+axs_status axs_mg_set_mixed(axs_mg* h, int refine, int fuse_level0) {
+    if (!h) return AXS_ERR_INVALID;
+    if (refine != AXS_REFINE_CYCLE && refine != AXS_REFINE_SOLVE) return AXS_ERR_INVALID;
+    h->refine = refine;
+    h->mg.setFuseLevel0(fuse_level0 != 0);
+    return AXS_OK;
+}
+
 axs_status axs_mg_solve(axs_mg* h, const double* AC, const double* AF, const double* b, double* x, int cycles,
                         double weight, int pre, int post, int coarse, int precision, int direct, int smoother) {
     if (!h || !AC || !AF || !b || !x) return AXS_ERR_INVALID;
     return guarded(nullptr, [&] {
-        const MGParams p = mgParams(cycles, weight, pre, post, coarse, precision, direct, smoother);
+        MGParams p = mgParams(cycles, weight, pre, post, coarse, precision, direct, smoother);
+        p.refine = h->refine;
         AXS_HIP(hipSetDevice(h->device));
         mgUpload(h, AC, AF, b, x);
         h->mg.enqueueSolve(h->stream, p, nullptr, true);
@@ -353,7 +363,8 @@ axs_status axs_mg_bench(axs_mg* h, const double* AC, const double* AF, const dou
                         double* ms, double* bytes, double* x_out) {
     if (!h || !AC || !AF || !b || !ms || !bytes || repeats <= 0) return AXS_ERR_INVALID;
     return guarded(nullptr, [&] {
-        const MGParams p = mgParams(cycles, weight, pre, post, coarse, precision, direct, smoother);
+        MGParams p = mgParams(cycles, weight, pre, post, coarse, precision, direct, smoother);
+        p.refine = h->refine;
         AXS_HIP(hipSetDevice(h->device));
         mgUpload(h, AC, AF, b, nullptr);
         h->mg.enqueueSolve(h->stream, p, nullptr, true);  // warm-up

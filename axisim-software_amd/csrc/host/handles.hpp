@@ -65,4 +65,5 @@ struct axs_mg {
     axs::Arena arena;
     double *AC = nullptr, *b = nullptr, *AF = nullptr, *x = nullptr, *xNew = nullptr;
     int n = 0, refs = 0;
+    int refine = AXS_REFINE_CYCLE;  // axs_mg_set_mixed
 };

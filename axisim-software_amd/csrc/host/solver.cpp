@@ -484,35 +484,57 @@ void MGDevice::vcycle(hipStream_t s, int l, int preDone, const MGParams& p, cons
 void MGDevice::swapX32(int l) { std::swap(dev[l].x32, dev[l].xNew32); }
 
 bool MGDevice::fusesLevel0(const MGParams& p) const {
-    return fuseL0_ && host.size() >= 2 && p.pre == 1 && p.post == 1 && p.smoother == AXS_MG_SMOOTH_JACOBI;
+    return fuseL0_ && !fuseL0Off_ && host.size() >= 2 && p.pre == 1 && p.post == 1 &&
+           p.smoother == AXS_MG_SMOOTH_JACOBI;
 }
 
 // The V-cycles of one solve with the level-0 work fused across cycles: each
 // cycle's pre-sweep and residual run in one launch together with the previous
 // cycle's prolongation and post-sweep; the last cycle's post-sweep runs alone.
-// The same sweeps and residuals as vcycle(0) repeated, bit for bit.
-void MGDevice::cyclesFused(hipStream_t s, int cycles, const MGParams& p, const Ctl* ctl) {
-    const int s1 = host[1].coarsest() ? (p.coarseSweeps >= 1) : (p.pre >= 1);
+// The same sweeps and residuals as vcycle(0) repeated, bit for bit.  fp32: the
+// same on the FP32 copies, as vcycle32(0) repeated, the last post-sweep also
+// adding the iterate into `apply` (FP64) when given.
+void MGDevice::cyclesFused(hipStream_t s, int cycles, const MGParams& p, const Ctl* ctl, bool fp32, double* apply) {
+    // a coarsest level solved directly (FP32 cycles only) takes no sweep from zero
+    const bool direct = fp32 && p.coarseDirect;
+    auto sweep1Of = [&](int l) { return host[l].coarsest() ? (!direct && p.coarseSweeps >= 1) : (p.pre >= 1); };
+    const int s1 = sweep1Of(1);
     // level 1 folded into the restriction when no pre-sweep runs there between
-    const bool l1 = fuseL1_ && s1 == 1 && p.pre == 1;
-    const int s2 = host.size() < 3 ? 0 : host[2].coarsest() ? (p.coarseSweeps >= 1) : (p.pre >= 1);
+    const bool l1 = fuseL1_ && s1 == 1 && p.pre == 1 && !(fp32 && tail32_.holds(1));
+    const int s2 = host.size() < 3 ? 0 : sweep1Of(2);
     for (int c = 0; c < cycles; c++) {
-        launchMGLevel0Block(s, dev[0], dev[1], dl0_, c == 0 ? 1 : 2, c > 0, p.weight, ctl);
-        swapX(0);
+        // FP32: the first launch starts from e = 0 on b32 (x = 0) or on the
+        // FP64 defect it forms itself
+        if (fp32 && c == 0) launchMGLevel0BlockStart32(s, dev[0], dev[1], dl0_, p.weight, !p.startZero, ctl);
+        else launchMGLevel0Block(s, dev[0], dev[1], dl0_, c == 0 ? 1 : 2, c > 0, p.weight, fp32, ctl);
+        fp32 ? swapX32(0) : swapX(0);
         if (l1) {
             // vcycle(1) with its restriction taken by the fused launch
-            launchMGRestrictFused01(s, dev[0], dev[1], dev[2], l1f_, p.weight, s1, s2, ctl);
-            vcycle(s, 2, s2, p, ctl);
-            launchMGProlongSmooth(s, dev[1], dev[2], p.weight, ctl);
-            swapX(1);
-            smooth(s, 1, p.post - 1, p, ctl);
+            launchMGRestrictFused01(s, dev[0], dev[1], dev[2], l1f_, p.weight, s1, s2, fp32, ctl);
+            if (fp32) {
+                vcycle32(s, 2, s2, p, ctl);
+                launchMGProlongSmooth32(s, dev[1], dev[2], (float)p.weight, ctl);
+                swapX32(1);
+                smooth32(s, 1, p.post - 1, p, ctl);
+            } else {
+                vcycle(s, 2, s2, p, ctl);
+                launchMGProlongSmooth(s, dev[1], dev[2], p.weight, ctl);
+                swapX(1);
+                smooth(s, 1, p.post - 1, p, ctl);
+            }
             continue;
         }
-        launchMGRestrictSum(s, dev[0], dev[1], p.weight, s1, false, ctl);
-        vcycle(s, 1, s1, p, ctl);
+        launchMGRestrictSum(s, dev[0], dev[1], p.weight, s1, fp32, ctl);
+        if (fp32) vcycle32(s, 1, s1, p, ctl);
+        else vcycle(s, 1, s1, p, ctl);
     }
-    launchMGProlongSmooth(s, dev[0], dev[1], p.weight, ctl);
-    swapX(0);
+    if (fp32) {
+        launchMGProlongSmooth32(s, dev[0], dev[1], (float)p.weight, ctl, apply);
+        swapX32(0);
+    } else {
+        launchMGProlongSmooth(s, dev[0], dev[1], p.weight, ctl);
+        swapX(0);
+    }
 }
 
 // Greedy colouring of every level's graph (buildMeshColoring order), built on
@@ -663,6 +685,19 @@ void MGDevice::enqueueSolveMixed(hipStream_t s, const MGParams& p, const Ctl* ct
     const int first = (single ? (!p.coarseDirect && p.coarseSweeps >= 1) : (p.pre >= 1)) ? 1 : 0;
     // x += e rides on level 0's post-sweep when that sweep is the cycle's last
     const bool fuseApply = !single && tail32_.lt != 0 && p.post == 1;
+    if (p.refine == AXS_REFINE_SOLVE) {
+        // one FP64 defect, then every cycle in FP32 on the correction e (from
+        // e = 0), x += e once at the end; fused, the defect is formed by the
+        // first level-0 launch (or is b itself when x = 0)
+        if (fusesLevel0(p) && (tail32_.lt < 0 || tail32_.lt >= 2)) {
+            cyclesFused(s, cycles, p, ctl, true, dev[0].x);
+            return;
+        }
+        launchMGDefect(s, dev[0], p.weight, 0, ctl);
+        for (int c = 0; c < cycles; c++) vcycle32(s, 0, 0, p, ctl, (fuseApply && c + 1 == cycles) ? dev[0].x : nullptr);
+        if (!fuseApply) launchMGApply(s, dev[0], ctl);
+        return;
+    }
     for (int c = 0; c < cycles; c++) {
         launchMGDefect(s, dev[0], p.weight, first, ctl);
         vcycle32(s, 0, first, p, ctl, fuseApply ? dev[0].x : nullptr);
@@ -673,7 +708,7 @@ void MGDevice::enqueueSolveMixed(hipStream_t s, const MGParams& p, const Ctl* ct
 void MGDevice::enqueueOperators(hipStream_t s, const MGParams& p, const Ctl* ctl) {
     if (p.precision == AXS_PREC_MIXED) {
         ensureMixed();
-        launchMGCast(s, dev[0], ctl);
+        if (!p.l0Copies) launchMGCast(s, dev[0], ctl);
         buildOperators(s, ctl);  // also writes the FP32 copies
         if (p.coarseDirect) launchCoarseInvert(s, dev[host.size() - 1], aug, inv, invStatus, ctl);
     } else {
@@ -806,22 +841,30 @@ double MGDevice::bytesPerSolve(const MGParams& p, bool buildOperators) const {
         return cyc;
     };
     const int cycles = p.cycles > 1 ? p.cycles : 1;
-    if (!mixed && fusesLevel0(p)) {
+    const bool once = mixed && p.refine == AXS_REFINE_SOLVE;
+    // the FP64 defect of a solve refined once (AC, b, x, start + AF/nb; b32 and
+    // x32 written) and its x += e (x read and written)
+    const double defect = 40.0 * host[0].n + 12.0 * host[0].refs(), applyX = 16.0 * host[0].n;
+    if (fusesLevel0(p) && (!mixed || (once && (tail32_.lt < 0 || tail32_.lt >= 2)))) {
         // level 0 as cyclesFused launches it: (sweep + residual), then per later
         // cycle (prolongation + 2 sweeps + residual) -- rows AC, b, x, start,
         // AF/nb (+ toCoarse, coarse x), x and the residual written -- the
         // restriction per cycle (residual, members; coarse b, x, AC), and the
         // last post-sweep
         const double n = host[0].n, S = host[0].refs(), nc = host[1].n;
-        const int s1 = host[1].coarsest() ? (p.coarseSweeps >= 1) : (p.pre >= 1);
-        double l0 = (5.0 * v + 4.0) * n + (v + 4.0) * S;
+        const int s1 = host[1].coarsest() ? (!(mixed && p.coarseDirect) && p.coarseSweeps >= 1) : (p.pre >= 1);
+        double l0 = (5.0 * v + 4.0) * n + (v + 4.0) * S + (once ? defect + applyX : 0.0);
         l0 += (cycles - 1) * ((5.0 * v + 8.0) * n + (v + 4.0) * S + v * nc);
         l0 += cycles * ((v + 4.0) * n + (2.0 * v + 4.0) * nc + (s1 ? v * nc : 0.0));
         l0 += (4.0 * v + 8.0) * n + (v + 4.0) * S + v * nc;
         // with level 1 folded into the restriction, its b and x come from LDS,
         // not from a second read
-        const double l1 = (fuseL1_ && s1 == 1 && p.pre == 1) ? 2.0 * v * nc : 0.0;
+        const double l1 = (fuseL1_ && s1 == 1 && p.pre == 1 && !(mixed && tail32_.holds(1))) ? 2.0 * v * nc : 0.0;
         return bytes + l0 + cycles * (cycleFrom(1, s1) - l1);
+    }
+    if (once) {
+        const bool fuseApply = L > 1 && tail32_.lt != 0 && p.post == 1;
+        return bytes + defect + cycles * cycleFrom(0, 0) + (fuseApply ? applyX : 20.0 * host[0].n);
     }
     double cyc = 0.0;
     int preDone = 0;
@@ -962,6 +1005,7 @@ static MGParams mgParamsOf(const axs_config& c) {
     mp.precision = c.precision;
     mp.coarseDirect = c.coarse_direct;
     mp.smoother = c.mg_smoother;
+    mp.refine = c.mixed_refine;
     return mp;
 }
 
@@ -970,6 +1014,8 @@ void Solver::configure(const axs_config& c) {
     if (c.precision != AXS_PREC_FP64 && c.precision != AXS_PREC_MIXED) throw std::invalid_argument("bad precision");
     if (c.mg_smoother != AXS_MG_SMOOTH_JACOBI && c.mg_smoother != AXS_MG_SMOOTH_RBGS)
         throw std::invalid_argument("bad multigrid smoother");
+    if (c.mixed_refine != AXS_REFINE_CYCLE && c.mixed_refine != AXS_REFINE_SOLVE)
+        throw std::invalid_argument("bad mixed-precision refinement");
     cfg_ = c;
     haveCfg_ = true;
 }
@@ -1162,6 +1208,12 @@ void Solver::enqueueIteration(hipStream_t s) {
     // are needed (one walk over the geometry instead of two)
     const bool gradUV = ph.nonOrtho || needGrad;
     if (gradUV) launchGradUV(s, dm_, bu, bv, bp, st, ph.gradScheme, true);
+    // mixed, fused FP32 sweeps: the momentum assembly also forms the FP64
+    // defect b - A u0 the sweeps start from
+    const std::vector<int> chunks = sweepChunks(c.linear_max_iter);
+    bool fusedMixed = c.precision == AXS_PREC_MIXED && chunks.size() >= 2;
+    for (int n : chunks) fusedMixed = fusedMixed && n >= 2;
+    st.momR32 = fusedMixed ? mx_.r : nullptr;
     launchMomentum(s, dm_, bu, bv, bp, st, ph, gradUV);
 
     // momentum solves (LinearSolver::run, linear_solver.cu:91-148, 209-271)
@@ -1169,19 +1221,30 @@ void Solver::enqueueIteration(hipStream_t s) {
         // FP32 Jacobi on the FP64 defect: sweep 1 inside kMomDefect, sweeps
         // 2..L-1 in FP32, sweep L fused with u += delta
         const int L = c.linear_max_iter;
-        launchMomDefect(s, dm_, st, mx_);
         float2 *d = mx_.d, *d2 = mx_.d2;
-        // sweeps 2 .. L-1 in chunks of up to fuse_ per launch (the FP64 path's
-        // blocks), avoiding a single sweep at the end where possible
-        for (int left = L - 2; left > 0;) {
-            int n = std::min(fuse_, left);
-            if (n > 2 && left - n == 1) n--;
-            if (n == 1) launchMomSweep32(s, dm_, st.ctl, mx_, d, d2);
-            else launchMomSweep32Block(s, dm_, st.ctl, mx_, dsb_, n, d, d2);
-            std::swap(d, d2);
-            left -= n;
+        if (fusedMixed) {
+            // the same sweeps in the fused FP64 path's chunks: the first launch
+            // starts from the defect kMomentum wrote, the last ends in u += delta
+            for (size_t i = 0; i < chunks.size(); i++) {
+                if (i == 0) launchMomSweep32BlockFromDefect(s, dm_, st, mx_, dsb_, chunks[i], d2);
+                else if (i + 1 == chunks.size()) launchMomSweep32BlockApply(s, dm_, st, mx_, dsb_, chunks[i], d);
+                else launchMomSweep32Block(s, dm_, st, mx_, dsb_, chunks[i], d, d2);
+                std::swap(d, d2);
+            }
+        } else {
+            launchMomDefect(s, dm_, st, mx_);
+            // sweeps 2 .. L-1 in chunks of up to fuse_ per launch (the FP64 path's
+            // blocks), avoiding a single sweep at the end where possible
+            for (int left = L - 2; left > 0;) {
+                int n = std::min(fuse_, left);
+                if (n > 2 && left - n == 1) n--;
+                if (n == 1) launchMomSweep32(s, dm_, st.ctl, mx_, d, d2);
+                else launchMomSweep32Block(s, dm_, st, mx_, dsb_, n, d, d2);
+                std::swap(d, d2);
+                left -= n;
+            }
+            launchMomSweepApply(s, dm_, st, mx_, d, L >= 2 ? 1 : 0);
         }
-        launchMomSweepApply(s, dm_, st, mx_, d, L >= 2 ? 1 : 0);
     } else if (c.linear_solver == AXS_LINEAR_GS_RB && nColors_ > 0) {
         for (int it = 0; it < c.linear_max_iter; it++)
             for (int col = 0; col < nColors_; col++) {
@@ -1193,9 +1256,7 @@ void Solver::enqueueIteration(hipStream_t s) {
         // iterates in between interleaved; the last one lands in u, v directly
         // Fused: chunks of up to fuse_ sweeps per launch, the same sequence of
         // iterates; no launch reads and writes the same array.
-        const int L = c.linear_max_iter;
         double2 *a = st.uvA, *b = st.uvB;
-        const std::vector<int> chunks = sweepChunks(L);
         for (size_t i = 0; i < chunks.size(); i++) {
             const bool first = i == 0, last = i + 1 == chunks.size();
             const double *uo = first ? st.u : nullptr, *vo = first ? st.v : nullptr;
@@ -1218,11 +1279,15 @@ void Solver::enqueueIteration(hipStream_t s) {
     // by the first right-hand-side launch
     const bool guess = c.use_multigrid && net_ != nullptr && netLevel_ > 0;
 
-    const MGParams mp = mgParamsOf(c);
+    MGParams mp = mgParamsOf(c);
+    mp.l0Copies = c.precision == AXS_PREC_MIXED;
     for (int corr = 0; corr <= ph.nonOrtho; corr++) {
         const int cross = corr > 0 ? 1 : 0;
         if (cross) launchGradField(s, dm_, ppbc_, st.pp, st.gPZ, st.gPR, ph.gradScheme, st.ctl);
-        launchPPRhs(s, dm_, st, ph, cross, (corr == 0 && !guess) ? st.pp : nullptr);
+        // p' = 0 here: the right-hand side launch zeroes it (and writes b in
+        // FP32 for the once-refined mixed solve)
+        mp.startZero = corr == 0 && !guess;
+        launchPPRhs(s, dm_, st, ph, cross, mp.startZero ? st.pp : nullptr);
         if (c.use_multigrid && corr == 0 && guess) {
             // operators, then the learned guess in place of p' = 0, then the cycles
             enqueueGuess(s, mp);
@@ -1529,9 +1594,15 @@ void Solver::run(axs_progress_cb cb, void* user, bool continueRun) {
         mg_.ensureMixed();
         if (c.coarse_direct && mg_.coarsestCells() > kDirectMax)
             throw std::domain_error("coarsest level too large for the dense FP64 solve");
+        // level 0's FP32 operator straight from the p' assembly, and with one
+        // defect per solve its right-hand side from the first one
+        ds_.ppAC32 = mg_.dev[0].AC32;
+        ds_.ppAF32 = mg_.dev[0].AF32;
+        ds_.ppb32 = c.mixed_refine == AXS_REFINE_SOLVE ? mg_.dev[0].b32 : nullptr;
     } else {
         ds_.AF32 = nullptr;
         ds_.uvInv32 = nullptr;
+        ds_.ppAC32 = ds_.ppAF32 = ds_.ppb32 = nullptr;
     }
     prepareSweepBlocks();
     if (c.linear_solver == AXS_LINEAR_GS_RB && nColors_ == 0) {
@@ -2064,6 +2135,8 @@ void Solver::setSweepFusion(int sweeps, int blockCells) {
 // halo is the smaller share) whose halos fit the kernel's LDS and whose cells'
 // rows fit its registers; fuse_ = 1 when none does.
 void Solver::prepareSweepBlocks() {
+    // one sweep per launch asked for: the multigrid's level 0 unfused too
+    mg_.setFuseLevel0(fuseReq_ >= 2);
     fuse_ = 1;
     for (int s = fuseReq_; s >= 2; s--)
         for (int cells : {1024, 512}) {

@@ -68,6 +68,11 @@ struct MGParams {
     int precision = AXS_PREC_FP64;
     int coarseDirect = 0;
     int smoother = AXS_MG_SMOOTH_JACOBI;
+    int refine = AXS_REFINE_CYCLE;  // mixed precision: FP64 defect per cycle or per solve
+    // level 0's FP32 operator copies are written by the p' assembly (no cast)
+    bool l0Copies = false;
+    // x = 0 and level 0's b32 holds b (written with the right-hand side)
+    bool startZero = false;
 };
 
 class MGDevice {
@@ -118,6 +123,9 @@ public:
     void unbindWork(const Bound& saved);
     // first level of the LDS-resident tail (-1: no tail) and its LDS bytes
     int tailLevel(bool fp32) const { return (fp32 ? tail32_ : tail64_).lt; }
+    // run level 0 as one-pass launches even where the fused cycles apply
+    // (the fused cycles' yardstick in tests)
+    void setFuseLevel0(bool on) { fuseL0Off_ = !on; }
     size_t tailBytes(bool fp32) const { return (fp32 ? tail32_ : tail64_).bytes; }
 
 private:
@@ -160,6 +168,7 @@ private:
     SweepBlocks l0Blocks_;
     DSweepBlocks dl0_{};
     bool fuseL0_ = false;
+    bool fuseL0Off_ = false;
     // the restriction into level 1 and level 1's residual + restriction in one
     // launch (fused level-0 cycles): per level-1 aggregate workgroup, the halo
     // of level-1 cells its rows read and each row slot's LDS index
@@ -168,7 +177,8 @@ private:
     std::vector<std::vector<AggThread>> aggThreadsHost_;
     static constexpr int kFuseL0MinCells = 16384;
     bool fusesLevel0(const MGParams& p) const;
-    void cyclesFused(hipStream_t s, int cycles, const MGParams& p, const Ctl* ctl);
+    void cyclesFused(hipStream_t s, int cycles, const MGParams& p, const Ctl* ctl, bool fp32 = false,
+                     double* apply = nullptr);
 };
 
 class Solver {

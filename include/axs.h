@@ -64,6 +64,12 @@ enum { AXS_SCALE_NONE = 0, AXS_SCALE_N = 1, AXS_SCALE_SQRT_N = 2, AXS_SCALE_DIAG
 enum { AXS_PREC_FP64 = 0,      /* reference arithmetic and schedule                  */
        AXS_PREC_MIXED = 1 };   /* FP32 smoothers + V-cycles, FP64 residual/flux/coarse */
 enum { AXS_MG_SMOOTH_JACOBI = 0, AXS_MG_SMOOTH_RBGS = 1 };
+/* Mixed precision, pressure correction: the FP64 defect b - A x before every
+ * FP32 V-cycle (x += e after each), or once per solve (all cycles FP32 on the
+ * correction e from 0, with the level-0 work fused across cycles as on the
+ * FP64 path, x += e at the end; the outer SIMPLE loop, whose continuity
+ * imbalance and fluxes are FP64, refines). */
+enum { AXS_REFINE_CYCLE = 0, AXS_REFINE_SOLVE = 1 };
 /* Variables with boundary conditions (BoundaryVariable, structs/boundary_struct.h:88-97
  * order U, V, P, StaticTemperature, Concentration). */
 enum { AXS_VAR_U = 0, AXS_VAR_V = 1, AXS_VAR_P = 2, AXS_VAR_T = 3, AXS_VAR_C = 4, AXS_VAR_COUNT = 5 };
@@ -178,6 +184,8 @@ typedef struct axs_config {
     double cp;
     double k;
     double diffusivity;
+    /* ---- build extension ---- */
+    int mixed_refine;     /* AXS_REFINE_* (precision = AXS_PREC_MIXED)          */
 } axs_config;
 
 /* Reference defaults (structs/solver_struct.h, src/header/setting.cuh:61-71,
@@ -341,7 +349,8 @@ axs_status axs_autotune(axs_solver* s, int repeats, double* ms_default, double* 
  * bit-identical to one sweep per launch.  The next run uses the largest count
  * <= sweeps that the mesh's halos allow (axs_stats::sweeps_per_launch,
  * sweep_block_cells).  Default 5 sweeps, block size 0; axs_autotune also picks
- * them. */
+ * them.  sweeps = 1 also runs the multigrid's level 0 one pass per launch
+ * where its cycles would otherwise be fused (the same results). */
 axs_status axs_set_sweep_fusion(axs_solver* s, int sweeps, int block_cells);
 
 /* ---------------- Agglomeration multigrid, stand-alone ---------------- */
@@ -384,6 +393,11 @@ axs_status axs_mg_solve(axs_mg* mg, const double* AC, const double* AF, const do
 axs_status axs_mg_solve_noalloc(axs_mg* mg, const double* AC, const double* AF, const double* b, double* x,
                                 int cycles, double weight, int pre_sweeps, int post_sweeps, int coarse_sweeps,
                                 void* stream);
+/* Mixed-precision options of the solves below (build extension): refine =
+ * AXS_REFINE_* (default AXS_REFINE_CYCLE); fuse_level0 = 0 runs level 0 as
+ * one-pass launches where the fused level-0 cycles would apply (default 1;
+ * the results are the same bit for bit). */
+axs_status axs_mg_set_mixed(axs_mg* mg, int refine, int fuse_level0);
 /* Benchmark form: the system is uploaded once (x = 0), one solve of `cycles`
  * V-cycles (coarse operators included) is captured in a HIP graph, as the
  * solver's iteration graph runs it, and `repeats` replays are timed with HIP

@@ -91,15 +91,33 @@ def test_fused_sweeps_patch_blocks_bit_exact(require_gpu, sweeps):
         np.testing.assert_array_equal(s.field(name), ref.field(name), err_msg=name)
 
 
+@pytest.mark.parametrize("linear_iters", [20, 7, 4])
 @pytest.mark.parametrize("sweeps,cells", [(2, 512), (3, 1024), (4, 1024), (5, 1024)])
-def test_fused_fp32_sweeps_bit_exact(require_gpu, nozzle_mesh, sweeps, cells):
+def test_fused_fp32_sweeps_bit_exact(require_gpu, nozzle_mesh, sweeps, cells, linear_iters):
     """The mixed path's FP32 defect sweeps fused over the same blocks: S sweeps
-    per launch against one per launch, bit for bit (18 FP32 sweeps per solve,
-    so the last chunk is shorter)."""
-    case = axs.nozzle_case(mesh=nozzle_mesh, re_throat=100, max_iter=8, tol=0.0, precision=axs.PREC_MIXED)
+    per launch against one per launch, bit for bit.  Fused, the first launch
+    forms the FP64 defect in its first sweep and the last ends in u += delta;
+    unfused, those are launches of their own (sweep counts that do not divide
+    the solve's sweeps, and solves of fewer sweeps than one launch holds)."""
+    case = axs.nozzle_case(mesh=nozzle_mesh, re_throat=100, max_iter=8, tol=0.0, precision=axs.PREC_MIXED,
+                           linear_max_iter=linear_iters)
     ref = run(case, 1)
     s = run(case, sweeps, cells)
     assert s.stats().sweeps_per_launch == sweeps
     assert s.stats().iterations == ref.stats().iterations == 8
     for name in FIELDS:
         np.testing.assert_array_equal(s.field(name), ref.field(name), err_msg=name)
+
+
+@pytest.mark.parametrize("sweeps", [3, 5])
+def test_fused_fp32_sweeps_quad_and_patch_blocks_bit_exact(require_gpu, sweeps):
+    """The FP32 defect sweeps with 4-slot rows: the quad pipe numbered in short
+    rows (index blocks) and in long rows (breadth-first patches), fused against
+    one sweep per launch."""
+    for nz, nr in ((32, 640), (640, 40)):
+        case = axs.pipe_case(nz=nz, nr=nr, max_iter=6, tol=0.0, precision=axs.PREC_MIXED)
+        ref = run(case, 1)
+        s = run(case, sweeps, 0)
+        assert s.stats().sweeps_per_launch == sweeps
+        for name in FIELDS:
+            np.testing.assert_array_equal(s.field(name), ref.field(name), err_msg=f"{nz}x{nr} {name}")

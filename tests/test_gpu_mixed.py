@@ -71,12 +71,58 @@ def test_mixed_refinement_tracks_fp64_cycles(require_gpu):
     assert r32 < 1.01 * r64
 
 
-def test_mixed_simple_converges_to_fp64_solution(require_gpu):
+@pytest.fixture(scope="module")
+def nozzle_system():
+    # 25k triangles: level 0 runs fused across cycles
+    return pp_system(axs.nozzle_case(mesh=axs.nozzle_mesh(spacing=1.1e-4)))
+
+
+def solve_refined_once(g, AC, AF, b, x0, cycles, fused):
+    g.set_mixed(axs.REFINE_SOLVE, fuse_level0=fused)
+    try:
+        return g.solve(AC, AF, b, x0, cycles=cycles, precision=axs.PREC_MIXED)
+    finally:
+        g.set_mixed(axs.REFINE_CYCLE, fuse_level0=True)
+
+
+@pytest.mark.parametrize("cycles", [1, 2, 3, 4])
+def test_refine_per_solve_fused_level0_bit_exact(require_gpu, nozzle_system, cycles):
+    """One FP64 defect per solve, then every V-cycle in FP32 on the correction:
+    level 0 fused across cycles (the FP32 copies through the FP64 path's
+    level-0 blocks and fused level-1 restriction) against one-pass launches,
+    bit for bit, from a non-zero start; and the answer is the FP64 cycles'
+    up to FP32 rounding."""
+    fs, fn, AC, AF, b = nozzle_system
+    assert fs.size - 1 >= 16384
+    g = axs.Multigrid(fs, fn)
+    x0 = np.linspace(-1.0, 1.0, b.size) * 1e-3
+    xf = solve_refined_once(g, AC, AF, b, x0, cycles, True)
+    xu = solve_refined_once(g, AC, AF, b, x0, cycles, False)
+    np.testing.assert_array_equal(xf, xu)
+    x64 = g.solve(AC, AF, b, x0, cycles=cycles)
+    assert rel(xf - x0, x64 - x0) < 1e-5
+
+
+@pytest.mark.parametrize("nz,nr", [(32, 640), (640, 40)])
+def test_refine_per_solve_fused_level0_quad_and_patch_blocks(require_gpu, nz, nr):
+    """The same on quad meshes (4-slot rows): numbered in short rows (index
+    blocks) and in long rows (breadth-first patches)."""
+    fs, fn, AC, AF, b = pp_system(axs.pipe_case(nz=nz, nr=nr), iters=2)
+    g = axs.Multigrid(fs, fn)
+    x0 = np.zeros_like(b)
+    xf = solve_refined_once(g, AC, AF, b, x0, 3, True)
+    xu = solve_refined_once(g, AC, AF, b, x0, 3, False)
+    np.testing.assert_array_equal(xf, xu)
+    assert rel(xf, g.solve(AC, AF, b, x0, cycles=3)) < 1e-5
+
+
+@pytest.mark.parametrize("refine", [axs.REFINE_CYCLE, axs.REFINE_SOLVE])
+def test_mixed_simple_converges_to_fp64_solution(require_gpu, refine):
     case = axs.pipe_case()
     o = Oracle(case.mesh, case.bcs, case.config)
     o.run()
     assert o.converged
-    mcase = axs.pipe_case(precision=axs.PREC_MIXED)
+    mcase = axs.pipe_case(precision=axs.PREC_MIXED, mixed_refine=refine)
     s = axs.make_solver(mcase)
     s.run()
     st = s.stats()
@@ -104,7 +150,8 @@ def test_mixed_direct_coarse_converges_to_fp64_solution(require_gpu):
         assert e < 1e-8, (name, e)
 
 
-def test_mixed_expansion_converges_to_fp64_solution(require_gpu):
+@pytest.mark.parametrize("refine", [axs.REFINE_CYCLE, axs.REFINE_SOLVE])
+def test_mixed_expansion_converges_to_fp64_solution(require_gpu, refine):
     """The sudden expansion (SOU, least-squares gradients, graded multiblock
     grid) at 32,256 cells and throat Re 100, where the steady iteration
     converges to 1e-6 (at Re 200 and above it does not: the residuals stall,
@@ -112,7 +159,7 @@ def test_mixed_expansion_converges_to_fp64_solution(require_gpu):
     kw = dict(nz=(48, 320), nr=(32, 64), re_throat=100.0, max_iter=20000)
     a = axs.make_solver(axs.expansion_case(**kw))
     a.run()
-    b = axs.make_solver(axs.expansion_case(precision=axs.PREC_MIXED, **kw))
+    b = axs.make_solver(axs.expansion_case(precision=axs.PREC_MIXED, mixed_refine=refine, **kw))
     b.run()
     assert a.stats().converged and b.stats().converged
     print("iterations fp64", a.stats().iterations, "mixed", b.stats().iterations)
