@@ -207,6 +207,8 @@ def load_library(path: os.PathLike | None = None) -> C.CDLL:
                                            C.c_int, C.c_int, C.c_int, C.c_void_p]),
         "axs_mg_bench": (C.c_int, [S, _dp, _dp, _dp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int,
                                    C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]),
+        "axs_mg_bench_cold": (C.c_int, [S, _dp, _dp, _dp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int,
+                                        C.c_int, C.c_int, C.c_int, _dp, _dp]),
         "axs_build_coloring": (C.c_int, [C.c_int, _ip, _ip, _ip, _ip, _ip]),
         "axs_net_create": (C.c_int, [C.POINTER(S), C.c_int, C.c_uint64, C.c_int]),
         "axs_net_destroy": (None, [S]),
@@ -887,9 +889,16 @@ class Multigrid:
         _check(lib().axs_mg_solve_noalloc(self._h, AC, AF, b, x, cycles, weight, pre, post, coarse, stream))
 
     def bench(self, AC, AF, b, cycles=3, weight=0.6, pre=1, post=1, coarse=30, precision=PREC_FP64,
-              coarse_direct=0, smoother=MG_SMOOTH_JACOBI, repeats=20):
+              coarse_direct=0, smoother=MG_SMOOTH_JACOBI, repeats=20, cold=False):
+        """(ms per solve, algorithmic bytes per solve, x); cold: each replay after
+        a 512 MiB cache flush (x is then not returned: None)."""
         AC, AF, b = (np.ascontiguousarray(a, dtype=np.float64) for a in (AC, AF, b))
         ms, by = C.c_double(), C.c_double()
+        if cold:
+            _check(lib().axs_mg_bench_cold(self._h, _dptr(AC), _dptr(AF), _dptr(b), cycles, weight, pre, post,
+                                           coarse, precision, coarse_direct, smoother, repeats, C.byref(ms),
+                                           C.byref(by)))
+            return ms.value, by.value, None
         x = np.zeros(self.n)
         _check(lib().axs_mg_bench(self._h, _dptr(AC), _dptr(AF), _dptr(b), cycles, weight, pre, post, coarse,
                                   precision, coarse_direct, smoother, repeats, C.byref(ms), C.byref(by),

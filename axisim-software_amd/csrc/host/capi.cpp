@@ -358,6 +358,73 @@ axs_status axs_mg_solve(axs_mg* h, const double* AC, const double* AF, const dou
     });
 }
 
+// One solve (operators + cycles) captured in a graph, as the solver's
+// iteration graph runs it, and `repeats` replays timed with HIP events on the
+// hierarchy's stream.  cold: a kernel streams 512 MiB (more than the L2 of
+// each XCD and the 256 MB last-level cache hold) before each replay, and the
+// solve's time is what `repeats` (flush, replay) pairs take beyond as many
+// flushes alone.
+static void mgBench(axs_mg* h, const double* AC, const double* AF, const double* b, const MGParams& p, int repeats,
+                    bool cold, double* ms, double* bytes, double* x_out) {
+    AXS_HIP(hipSetDevice(h->device));
+    mgUpload(h, AC, AF, b, nullptr);
+    h->mg.enqueueSolve(h->stream, p, nullptr, true);  // warm-up
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    AXS_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    try {
+        h->mg.enqueueSolve(h->stream, p, nullptr, true);
+    } catch (...) {
+        (void)hipStreamEndCapture(h->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        throw;
+    }
+    AXS_HIP(hipStreamEndCapture(h->stream, &g));
+    const hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    AXS_HIP(ie);
+    const size_t flushN = cold ? ((size_t)64 << 20) : 0;  // doubles
+    double *flush = nullptr, *sink = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float f = 0.0f;
+    hipError_t err = hipGraphLaunch(ge, h->stream);  // warm-up of the graph
+    if (err == hipSuccess) err = hipEventCreate(&e0);
+    if (err == hipSuccess) err = hipEventCreate(&e1);
+    if (cold && err == hipSuccess) err = hipMalloc((void**)&flush, flushN * sizeof(double));
+    if (cold && err == hipSuccess) err = hipMalloc((void**)&sink, 2048 * sizeof(double));
+    auto timed = [&](bool solve, bool flushFirst) {
+        float t = 0.0f;
+        if (flushFirst) launchStreamRead(h->stream, flush, flushN, sink, 2048, -1.0);
+        if (err == hipSuccess) err = hipEventRecord(e0, h->stream);
+        for (int r = 0; r < repeats && err == hipSuccess; r++) {
+            if (flushFirst) launchStreamRead(h->stream, flush, flushN, sink, 2048, -1.0);
+            if (solve) err = hipGraphLaunch(ge, h->stream);
+        }
+        if (err == hipSuccess) err = hipEventRecord(e1, h->stream);
+        if (err == hipSuccess) err = hipEventSynchronize(e1);
+        if (err == hipSuccess) err = hipEventElapsedTime(&t, e0, e1);
+        return t;
+    };
+    if (!cold) {
+        f = timed(true, false);
+    } else if (err == hipSuccess) {
+        launchFill(h->stream, flush, 0.0, (int)flushN, nullptr);
+        const float flushOnly = timed(false, true);
+        f = std::max(0.0f, timed(true, true) - flushOnly);
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (flush) (void)hipFree(flush);
+    if (sink) (void)hipFree(sink);
+    (void)hipGraphExecDestroy(ge);
+    AXS_HIP(err);
+    AXS_HIP(hipGetLastError());
+    *ms = f / repeats;
+    *bytes = h->mg.bytesPerSolve(p, true);
+    if (x_out) AXS_HIP(hipMemcpy(x_out, h->x, h->n * sizeof(double), hipMemcpyDeviceToHost));
+}
+
 axs_status axs_mg_bench(axs_mg* h, const double* AC, const double* AF, const double* b, int cycles, double weight,
                         int pre, int post, int coarse, int precision, int direct, int smoother, int repeats,
                         double* ms, double* bytes, double* x_out) {
@@ -365,44 +432,18 @@ axs_status axs_mg_bench(axs_mg* h, const double* AC, const double* AF, const dou
     return guarded(nullptr, [&] {
         MGParams p = mgParams(cycles, weight, pre, post, coarse, precision, direct, smoother);
         p.refine = h->refine;
-        AXS_HIP(hipSetDevice(h->device));
-        mgUpload(h, AC, AF, b, nullptr);
-        h->mg.enqueueSolve(h->stream, p, nullptr, true);  // warm-up
-        // timed as the solver runs it: one solve (operators + cycles) captured
-        // in a graph and replayed
-        hipGraph_t g = nullptr;
-        hipGraphExec_t ge = nullptr;
-        AXS_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-        try {
-            h->mg.enqueueSolve(h->stream, p, nullptr, true);
-        } catch (...) {
-            (void)hipStreamEndCapture(h->stream, &g);
-            if (g) (void)hipGraphDestroy(g);
-            (void)hipGetLastError();
-            throw;
-        }
-        AXS_HIP(hipStreamEndCapture(h->stream, &g));
-        const hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        AXS_HIP(ie);
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        float f = 0.0f;
-        hipError_t err = hipGraphLaunch(ge, h->stream);  // warm-up of the graph
-        if (err == hipSuccess) err = hipEventCreate(&e0);
-        if (err == hipSuccess) err = hipEventCreate(&e1);
-        if (err == hipSuccess) err = hipEventRecord(e0, h->stream);
-        for (int r = 0; r < repeats && err == hipSuccess; r++) err = hipGraphLaunch(ge, h->stream);
-        if (err == hipSuccess) err = hipEventRecord(e1, h->stream);
-        if (err == hipSuccess) err = hipEventSynchronize(e1);
-        if (err == hipSuccess) err = hipEventElapsedTime(&f, e0, e1);
-        if (e0) (void)hipEventDestroy(e0);
-        if (e1) (void)hipEventDestroy(e1);
-        (void)hipGraphExecDestroy(ge);
-        AXS_HIP(err);
-        AXS_HIP(hipGetLastError());
-        *ms = f / repeats;
-        *bytes = h->mg.bytesPerSolve(p, true);
-        if (x_out) AXS_HIP(hipMemcpy(x_out, h->x, h->n * sizeof(double), hipMemcpyDeviceToHost));
+        mgBench(h, AC, AF, b, p, repeats, false, ms, bytes, x_out);
+    });
+}
+
+axs_status axs_mg_bench_cold(axs_mg* h, const double* AC, const double* AF, const double* b, int cycles,
+                             double weight, int pre, int post, int coarse, int precision, int direct, int smoother,
+                             int repeats, double* ms, double* bytes) {
+    if (!h || !AC || !AF || !b || !ms || !bytes || repeats <= 0) return AXS_ERR_INVALID;
+    return guarded(nullptr, [&] {
+        MGParams p = mgParams(cycles, weight, pre, post, coarse, precision, direct, smoother);
+        p.refine = h->refine;
+        mgBench(h, AC, AF, b, p, repeats, true, ms, bytes, nullptr);
     });
 }
 

@@ -408,6 +408,13 @@ axs_status axs_mg_bench(axs_mg* mg, const double* AC, const double* AF, const do
                         int cycles, double weight, int pre_sweeps, int post_sweeps,
                         int coarse_sweeps, int precision, int coarse_direct, int smoother,
                         int repeats, double* ms, double* bytes, double* x_out);
+/* The same with cold caches: a kernel streams 512 MiB (beyond the L2s and the
+ * 256 MB last-level cache) before each replay; *ms = the replays' time beyond
+ * that of the flushes alone, per solve. */
+axs_status axs_mg_bench_cold(axs_mg* mg, const double* AC, const double* AF, const double* b,
+                             int cycles, double weight, int pre_sweeps, int post_sweeps,
+                             int coarse_sweeps, int precision, int coarse_direct, int smoother,
+                             int repeats, double* ms, double* bytes);
 
 /* ---------------- Learned initial guess (build extension) ----------------
  * No counterpart in the reference.  A message-passing network on a cell graph
