@@ -979,17 +979,40 @@ __global__ void __launch_bounds__(kBlock) kPPCoeff(DMesh m, DBC bp, DState s, Ph
     s.ppInv[n] = (fabs(ac) < 1.0e-30) ? 0.0 : 1.0 / ac;
 }
 
-// computeFaceMassFluxRhieChow, solver_util.cu:171-203
+// computeFaceMassFluxRhieChow, solver_util.cu:171-203.  An interior face's
+// data (the face's geometry, both cells' u, v, p, grad p, DU, DV) are loaded
+// in one round, unconditionally (a boundary face reads its owner twice), and
+// rhieChow's interior arithmetic runs on them in its order; boundary faces
+// take rhieChow itself.
 __global__ void __launch_bounds__(kBlock) kFaceFlux(DMesh m, DBC bu, DBC bv, DBC bp, DState s, Physics ph) {
     if (finished(s.ctl)) return;
     const int f = xcdBlock() * blockDim.x + threadIdx.x;
     if (f >= m.F) return;
-    const double area = m.area[f];
+    const int c = m.owner[f], nb = m.neighbor[f];
+    const int o = nb >= 0 ? nb : c;
+    const double area = m.area[f], nz = m.nZ[f], nr = m.nR[f], w = m.wP[f], inv = m.invDPN[f];
+    const double uc = s.u[c], vc = s.v[c], pc = s.p[c], gzc = s.gPZ[c], grc = s.gPR[c], duc = s.DU[c], dvc = s.DV[c];
+    const double uo = s.u[o], vo = s.v[o], po = s.p[o], gzo = s.gPZ[o], gro = s.gPR[o], duo = s.DU[o], dvo = s.DV[o];
     if (area <= 1.0e-30) {
         s.mDot[f] = 0.0;
         return;
     }
-    const double un = rhieChow(m, bu, bv, bp, s, m.owner[f], f);
+    if (nb < 0) {
+        s.mDot[f] = ph.rho * rhieChow(m, bu, bv, bp, s, c, f) * area;
+        return;
+    }
+    // interpToFace / normalCorrToFace seen from the owner: its weight is wP
+    const double uF = w * uc + (1.0 - w) * uo;
+    const double vF = w * vc + (1.0 - w) * vo;
+    const double unLin = uF * nz + vF * nr;
+    const double gPN = (po - pc) * inv;
+    const double gzF = w * gzc + (1.0 - w) * gzo;
+    const double grF = w * grc + (1.0 - w) * gro;
+    const double gFN = gzF * nz + grF * nr;
+    const double DP = duc * nz * nz + dvc * nr * nr;
+    const double DN = duo * nz * nz + dvo * nr * nr;
+    const double Df = w * DP + (1.0 - w) * DN;
+    const double un = unLin - Df * (gPN - gFN);
     s.mDot[f] = ph.rho * un * area;
 }
 
@@ -1076,24 +1099,47 @@ __global__ void __launch_bounds__(kBlock) kGradPPUpdate(DMesh m, DBC ppbc, DStat
 }
 
 // updateMassFlux, simple.cu:181-258
+// An interior face's data are loaded in one round, unconditionally (a
+// boundary face reads its owner twice), and the update runs on them in the
+// order of normalCorrToFace and nonOrthoFlux seen from the owner (weight wP,
+// the normal as stored, d = x_nb - x_owner); boundary faces take those helpers.
 __global__ void __launch_bounds__(kBlock) kMassFluxUpdate(DMesh m, DBC bp, DState s, Physics ph) {
     if (finished(s.ctl)) return;
     const int f = xcdBlock() * blockDim.x + threadIdx.x;
     if (f >= m.F) return;
     const int ow = m.owner[f], nb = m.neighbor[f];
-    const double area = m.area[f];
+    const int o = nb >= 0 ? nb : ow;
+    const double area = m.area[f], nz = m.nZ[f], nr = m.nR[f], w = m.wP[f], inv = m.invDPN[f];
+    const double mdf = s.mDot[f];
+    const double duc = s.DU[ow], dvc = s.DV[ow], ppc = s.pp[ow];
+    const double duo = s.DU[o], dvo = s.DV[o], ppo = s.pp[o];
+    const bool no = ph.nonOrtho != 0;
+    const double zc = no ? m.cZ[ow] : 0.0, rc = no ? m.cR[ow] : 0.0, zo = no ? m.cZ[o] : 0.0, ro = no ? m.cR[o] : 0.0;
+    const double gzc = no ? s.gPZ[ow] : 0.0, grc = no ? s.gPR[ow] : 0.0;
+    const double gzo = no ? s.gPZ[o] : 0.0, gro = no ? s.gPR[o] : 0.0;
     if (area <= 1.0e-30) {
         s.mDot[f] = 0.0;
         return;
     }
-    const double Df = normalCorrToFace(m, s, ow, f);
     if (nb >= 0) {
-        double md = s.mDot[f];
-        md -= ph.rho * area * Df * (s.pp[nb] - s.pp[ow]) * m.invDPN[f];
-        if (ph.nonOrtho) md -= nonOrthoFlux(m, ow, f, s.gPZ, s.gPR, ph.rho * Df);
+        const double DP = duc * nz * nz + dvc * nr * nr;
+        const double DN = duo * nz * nz + dvo * nr * nr;
+        const double Df = w * DP + (1.0 - w) * DN;
+        double md = mdf;
+        md -= ph.rho * area * Df * (ppo - ppc) * inv;
+        if (no) {
+            const double dz = zo - zc, dr = ro - rc;
+            const double aOverNd = area * inv;
+            const double Tz = area * nz - aOverNd * dz;
+            const double Tr = area * nr - aOverNd * dr;
+            const double gfz = w * gzc + (1.0 - w) * gzo;
+            const double gfr = w * grc + (1.0 - w) * gro;
+            md -= (ph.rho * Df) * (Tz * gfz + Tr * gfr);
+        }
         s.mDot[f] = md;
         return;
     }
+    const double Df = normalCorrToFace(m, s, ow, f);
     const int g = m.group[f];
     if (g < 0 || g >= bp.nGroups) return;
     if (bp.type[g] == 0) {

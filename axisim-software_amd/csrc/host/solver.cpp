@@ -2128,6 +2128,7 @@ void Solver::setSweepFusion(int sweeps, int blockCells) {
         throw std::invalid_argument("fused blocks hold 512 or 1024 cells (0: either)");
     fuseReq_ = sweeps;
     fuseCellsReq_ = blockCells;
+    mgFuseL0_ = sweeps >= 2;
 }
 
 // Build (or reuse) the blocks and halo rings for the largest sweep count
@@ -2135,8 +2136,9 @@ void Solver::setSweepFusion(int sweeps, int blockCells) {
 // halo is the smaller share) whose halos fit the kernel's LDS and whose cells'
 // rows fit its registers; fuse_ = 1 when none does.
 void Solver::prepareSweepBlocks() {
-    // one sweep per launch asked for: the multigrid's level 0 unfused too
-    mg_.setFuseLevel0(fuseReq_ >= 2);
+    // one sweep per launch asked for (axs_set_sweep_fusion): the multigrid's
+    // level 0 unfused too
+    mg_.setFuseLevel0(mgFuseL0_);
     fuse_ = 1;
     for (int s = fuseReq_; s >= 2; s--)
         for (int cells : {1024, 512}) {

@@ -296,6 +296,7 @@ private:
     // and halo rings (host and device copies, built for fuseCellsReq_ and
     // fuse_ - 1 rings)
     int fuseReq_ = kMaxFusedSweeps, fuseCellsReq_ = 0, fuse_ = 1;
+    bool mgFuseL0_ = true;  // the caller's choice only (the autotuner leaves it)
     int graphNodes_ = 0;  // nodes of the captured iteration graph
     SweepBlocks sweepBlocks_;
     DSweepBlocks dsb_{};
