@@ -487,6 +487,10 @@ def traffic_probe() -> int:
               axs.KERNEL_PP_COEFF):
         ms, by = s.bench_kernel(k, 10)
         log(f"kernel {k}: {ms * 1000:.1f} us, {by:.0f} algorithmic bytes")
+    # the mixed path's kernels: 5 outer iterations on the same mesh
+    sm = axs.make_solver(axs.nozzle_case(mesh=case.mesh, max_iter=5, mg_max_levels=6, precision=axs.PREC_MIXED))
+    sm.run()
+    log("mixed: 5 outer iterations")
     x = torch.empty(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
     x.fill_(1.0)
     y = torch.empty_like(x)
