@@ -432,6 +432,195 @@ __global__ void __launch_bounds__(1024) kCoarseInvert(DLevel L, double* aug, dou
     if (t == 0) *status = singular;
 }
 
+// Larger coarsest levels: blocked in-place Gauss-Jordan inversion of the
+// dense operator, padded to N (a multiple of kGJ) with the identity, in tiles
+// of kGJ x kGJ.  No pivoting: the coarsest operator is symmetric positive
+// definite (a Galerkin product of the p' matrix), and so is its padding.
+// Block step k with P = (M_kk)^-1:
+//   M_kk <- P,  M_kj <- P M_kj,  M_ik <- -M_ik P,
+//   M_ij <- M_ij - M_ik (P M_kj)   (i, j != k)
+// after the last step M holds the inverse.  One launch per step, one wave per
+// tile, reading M and writing its copy (no tile is read after it is
+// rewritten); the tile (k+1, k+1) also inverts its new value for the next step.
+constexpr int kGJ = 32;
+
+// the dense, padded operator: row r of M (ld N) per workgroup
+__global__ void __launch_bounds__(kBlock) kCoarseDense(DLevel L, double* M, int N, const Ctl* ctl) {
+    if (finished(ctl)) return;
+    const int r = blockIdx.x, n = L.n;
+    for (int c = threadIdx.x; c < N; c += blockDim.x)
+        M[(size_t)r * N + c] = (c == r) ? (r < n ? L.AC[r] : 1.0) : 0.0;
+    __syncthreads();
+    if (threadIdx.x == 0 && r < n)
+        for (int k = L.start[r]; k < L.start[r + 1]; k++)
+            if (L.nb[k] >= 0) M[(size_t)r * N + L.nb[k]] += L.AF[k];
+}
+
+// In-place Gauss-Jordan inversion of the kGJ x kGJ tile T (LDS, row stride
+// kGJ + 1) by one wave; lane l holds row l / 2, columns 16 (l % 2) .. + 15.
+// Each pivot reads the old values it needs before any lane writes (one wave:
+// its LDS accesses complete in program order).
+__device__ void invertTileWave(double* T) {
+    constexpr int S = kGJ + 1;
+    const int lane = threadIdx.x & 63, i = lane >> 1, j0 = (lane & 1) * 16;
+    double v[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) v[q] = T[i * S + j0 + q];
+    for (int p = 0; p < kGJ; p++) {
+        const double d = 1.0 / T[p * S + p];
+        const double fip = T[i * S + p];
+        double rp[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++) rp[q] = T[p * S + j0 + q];
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int j = j0 + q;
+            if (i == p) v[q] = (j == p) ? d : rp[q] * d;
+            else if (j == p) v[q] = -(fip * d);
+            else v[q] = v[q] - fip * (rp[q] * d);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll
+        for (int q = 0; q < 16; q++) T[i * S + j0 + q] = v[q];
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+}
+
+// P0 = (M_00)^-1
+__global__ void __launch_bounds__(64) kGJPivot0(const double* M, int N, double* P, const Ctl* ctl) {
+    __shared__ double T[kGJ * (kGJ + 1)];
+    if (finished(ctl)) return;
+    const int lane = threadIdx.x;
+    for (int e = lane; e < kGJ * kGJ; e += 64) T[(e / kGJ) * (kGJ + 1) + e % kGJ] = M[(size_t)(e / kGJ) * N + e % kGJ];
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    invertTileWave(T);
+    for (int e = lane; e < kGJ * kGJ; e += 64) P[e] = T[(e / kGJ) * (kGJ + 1) + e % kGJ];
+}
+
+// tile loads into LDS (row stride kGJ + 1): every load of the tile issued
+// before the first store, so they are in flight together; and the 4 x 4 block
+// of a product
+constexpr int kTileQ = kGJ * kGJ / 64;  // elements per lane
+__device__ __forceinline__ void loadTile(double* T, const double* src, int ld) {
+    const int lane = threadIdx.x & 63;
+    double v[kTileQ];
+#pragma unroll
+    for (int q = 0; q < kTileQ; q++) {
+        const int e = lane + 64 * q;
+        v[q] = src[(size_t)(e / kGJ) * ld + e % kGJ];
+    }
+#pragma unroll
+    for (int q = 0; q < kTileQ; q++) {
+        const int e = lane + 64 * q;
+        T[(e / kGJ) * (kGJ + 1) + e % kGJ] = v[q];
+    }
+}
+__device__ __forceinline__ void tileProduct(const double* A, const double* B, int r0, int c0, double acc[4][4]) {
+    constexpr int S = kGJ + 1;
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) acc[a][c] = 0.0;
+    for (int t = 0; t < kGJ; t++) {
+        double x[4], y[4];
+#pragma unroll
+        for (int a = 0; a < 4; a++) x[a] = A[(r0 + a) * S + t];
+#pragma unroll
+        for (int c = 0; c < 4; c++) y[c] = B[t * S + c0 + c];
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) acc[a][c] += x[a] * y[c];
+    }
+}
+
+__global__ void __launch_bounds__(64) kGJStep(const double* __restrict__ M, double* __restrict__ Mn, int N, int k,
+                                              const double* __restrict__ P, double* __restrict__ Pn,
+                                              const Ctl* ctl) {
+    constexpr int S = kGJ + 1;
+    __shared__ double A[kGJ * S], B[kGJ * S], T[kGJ * S];
+    if (finished(ctl)) return;
+    const int K = N / kGJ, ti = blockIdx.x / K, tj = blockIdx.x % K;
+    const int lane = threadIdx.x, r0 = 4 * (lane >> 3), c0 = 4 * (lane & 7);
+    const size_t base = (size_t)ti * kGJ * N + (size_t)tj * kGJ;
+    const double* Mkj = M + (size_t)k * kGJ * N + (size_t)tj * kGJ;
+    const double* Mik = M + (size_t)ti * kGJ * N + (size_t)k * kGJ;
+    double acc[4][4];
+    if (ti == k && tj == k) {
+#pragma unroll
+        for (int q = 0; q < kTileQ; q++) {
+            const int e = lane + 64 * q;
+            Mn[base + (size_t)(e / kGJ) * N + e % kGJ] = P[e];
+        }
+        return;
+    }
+    if (ti == k || tj == k) {
+        // P M_kj, or -M_ik P
+        if (ti == k) {
+            loadTile(A, P, kGJ);
+            loadTile(B, Mkj, N);
+        } else {
+            loadTile(A, Mik, N);
+            loadTile(B, P, kGJ);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        tileProduct(A, B, r0, c0, acc);
+        const double sg = ti == k ? 1.0 : -1.0;
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) Mn[base + (size_t)(r0 + a) * N + c0 + c] = sg * acc[a][c];
+        return;
+    }
+    // M_ij - M_ik (P M_kj): the three tiles and this lane's block of M_ij are
+    // loaded before any product
+    double mij[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) mij[a][c] = M[base + (size_t)(r0 + a) * N + c0 + c];
+    loadTile(A, P, kGJ);
+    loadTile(B, Mkj, N);
+    loadTile(T, Mik, N);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    tileProduct(A, B, r0, c0, acc);  // P M_kj
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) B[(r0 + a) * S + c0 + c] = acc[a][c];
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    tileProduct(T, B, r0, c0, acc);  // M_ik (P M_kj)
+    const bool nextPivot = ti == k + 1 && tj == k + 1;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const double v = mij[a][c] - acc[a][c];
+            Mn[base + (size_t)(r0 + a) * N + c0 + c] = v;
+            if (nextPivot) B[(r0 + a) * S + c0 + c] = v;
+        }
+    if (!nextPivot) return;
+    // the next step's pivot block, inverted here
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    invertTileWave(B);
+    for (int e = lane; e < kGJ * kGJ; e += 64) Pn[e] = B[(e / kGJ) * S + e % kGJ];
+}
+
+// x_c = A_c^-1 b_c with one wave per row (FP64 products summed in a fixed
+// order: lane l takes columns l, l + 64, ..., then a fixed shuffle tree);
+// inv has row stride ld
+__global__ void __launch_bounds__(kBlock) kCoarseApplyWide(DLevel L, const double* inv, int ld, const Ctl* ctl) {
+    if (finished(ctl)) return;
+    const int lane = threadIdx.x & 63, r = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    if (r >= L.n) return;  // the whole wave
+    double acc = 0.0;
+    for (int c = lane; c < L.n; c += 64) acc += inv[(size_t)r * ld + c] * (double)L.b32[c];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if (lane == 0) L.x32[r] = (float)acc;
+}
+
 // x_c = A_c^-1 b_c (FP64 product, FP32 in / out)
 __global__ void __launch_bounds__(kBlock) kCoarseApply(DLevel L, const double* inv, const Ctl* ctl) {
     if (finished(ctl)) return;
@@ -475,11 +664,28 @@ void launchMGApply(hipStream_t s, const DLevel& L, const Ctl* ctl) {
 void launchMGCast(hipStream_t s, const DLevel& L, const Ctl* ctl) {
     kMGCast<<<blocksFor(L.n, L.block), L.block, 0, s>>>(L, ctl);
 }
+int coarseInverseLd(int n) { return n <= kDirectOneBlock ? n : (n + kGJ - 1) / kGJ * kGJ; }
+size_t coarseInverseWork(int n) {
+    const size_t N = (size_t)coarseInverseLd(n);
+    return n <= kDirectOneBlock ? 2 * N * N : N * N + 2 * (size_t)kGJ * kGJ;
+}
 void launchCoarseInvert(hipStream_t s, const DLevel& L, double* aug, double* inv, int* status, const Ctl* ctl) {
-    kCoarseInvert<<<1, 1024, 0, s>>>(L, aug, inv, status, ctl);
+    if (L.n <= kDirectOneBlock) {
+        kCoarseInvert<<<1, 1024, 0, s>>>(L, aug, inv, status, ctl);
+        return;
+    }
+    // aug: the dense operator (N x N) and two pivot inverses; the steps
+    // alternate between it and inv, so that the last one writes inv
+    const int N = coarseInverseLd(L.n), K = N / kGJ;
+    double* P[2] = {aug + (size_t)N * N, aug + (size_t)N * N + kGJ * kGJ};
+    double* m[2] = {K % 2 == 0 ? inv : aug, K % 2 == 0 ? aug : inv};
+    kCoarseDense<<<N, kBlock, 0, s>>>(L, m[0], N, ctl);
+    kGJPivot0<<<1, 64, 0, s>>>(m[0], N, P[0], ctl);
+    for (int k = 0; k < K; k++)
+        kGJStep<<<K * K, 64, 0, s>>>(m[k & 1], m[(k + 1) & 1], N, k, P[k & 1], P[(k + 1) & 1], ctl);
 }
 void launchCoarseApply(hipStream_t s, const DLevel& L, const double* inv, const Ctl* ctl) {
-    kCoarseApply<<<blocksFor(L.n), kBlock, 0, s>>>(L, inv, ctl);
+    kCoarseApplyWide<<<(L.n + kBlock / 64 - 1) / (kBlock / 64), kBlock, 0, s>>>(L, inv, coarseInverseLd(L.n), ctl);
 }
 
 namespace {

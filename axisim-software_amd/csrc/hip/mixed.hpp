@@ -5,7 +5,12 @@
 
 namespace axs {
 
-constexpr int kDirectMax = 256;  // largest coarsest level solved by the dense FP64 inverse
+constexpr int kDirectMax = 1024;      // largest coarsest level solved by the dense FP64 inverse
+constexpr int kDirectOneBlock = 256;  // up to this size the inverse is formed by one workgroup
+// the coarsest inverse's row stride (n, or n padded for the blocked inverse)
+// and the work buffer (doubles) launchCoarseInvert needs beside the inverse
+int coarseInverseLd(int n);
+size_t coarseInverseWork(int n);
 
 // FP32 copies of the momentum operator and the defect iterates.
 // u and v interleaved as (u, v) pairs, so a neighbour's pair is one 8-byte load.

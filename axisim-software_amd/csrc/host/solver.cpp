@@ -618,8 +618,9 @@ void MGDevice::ensureMixed() {
     }
     const int nc = coarsestCells();
     if (nc <= kDirectMax) {
-        aug = arena32.zeros<double>((size_t)2 * nc * nc);
-        inv = arena32.zeros<double>((size_t)nc * nc);
+        const size_t ld = (size_t)coarseInverseLd(nc);
+        aug = arena32.zeros<double>(coarseInverseWork(nc));
+        inv = arena32.zeros<double>(ld * ld);
         invStatus = arena32.zeros<int>(1);
     }
     mixedReady = true;
@@ -639,7 +640,11 @@ void MGDevice::smooth32(hipStream_t s, int l, int sweeps, const MGParams& p, con
 }
 
 void MGDevice::vcycle32(hipStream_t s, int l, int preDone, const MGParams& p, const Ctl* ctl, double* apply) {
-    if (l == tail32_.lt) {
+    // a coarsest level solved directly that is beyond one workgroup's inverse
+    // is applied across the GPU (one wave per row), not inside the tail: the
+    // tail's levels then run as ordinary launches
+    const bool wideDirect = p.coarseDirect && coarsestCells() > kDirectOneBlock;
+    if (l == tail32_.lt && !wideDirect) {
         launchMGTail(s, tail32_.launch(), p.weight, preDone, p.pre, p.post, p.coarseSweeps, true,
                      p.coarseDirect ? inv : nullptr, ctl);
         return;

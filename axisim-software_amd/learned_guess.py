@@ -122,3 +122,12 @@ def cycles_to_tolerance(mg: axs.Multigrid, A, AC, AF, b, x0, tol=1e-6, max_cycle
         c += 1
         hist.append(relative_residual(A, b, x))
     return c, np.array(hist)
+
+
+def guess_start(net: axs.Net, hier: Hierarchy, fs, fn, AC, AF, b, level):
+    """The network's start for A x = b: its (energy-scaled) guess on the
+    Galerkin system of `level`, prolonged to the mesh."""
+    A = csr_matrix(fs, fn, AC, AF)
+    fsk, fnk, ACk, AFk = hier.level_system(A, level)
+    xk = net.predict(fsk, fnk, ACk, AFk, hier.restrict(b, level))
+    return hier.prolong(xk, level)

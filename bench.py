@@ -247,7 +247,9 @@ def main() -> int:
                  "rel_l2_vs_fp64": err}
         log(f"learned guess: {gst.iterations} iterations to 1e-6 in {wall:.2f} s (mixed {plain32['iterations']}, "
             f"fp64 {plain['iterations']})")
-        del sg, net
+        del sg
+        guess["per_solve"] = guess_per_solve(net, m, args.mg_levels)
+        del net
 
     # batched sweep: many cases of uneven size in one solver, every launch
     # covers all; each case stops on its own residuals in every time step
@@ -397,6 +399,45 @@ def main() -> int:
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def guess_per_solve(net, m, mg_levels: int, iterations=(20, 200, 2000), checkpoints=(3, 30, 300, 3000)) -> dict:
+    """The learned guess as the start of single p' solves: p' systems of the 1M
+    nozzle captured after the given outer iterations, each solved by FP64
+    V-cycles from zero and from the network's start (its guess on the guess
+    level, prolonged); the relative residual (to that of x = 0) at the start and
+    after 3 (the cycles SIMPLE runs), 30, 300 and 3000 cycles."""
+    import numpy as np
+
+    import axs
+    import learned_guess as lg
+
+    t0 = time.time()
+    case = axs.nozzle_case(mesh=m, mg_max_levels=mg_levels)
+    systems = lg.solver_systems(case, list(iterations))
+    fs, fn = m.arrays["cell_face_start"], _slot_neighbours(m)
+    mg = axs.Multigrid(fs, fn, max_levels=mg_levels)
+    hier, k = lg.Hierarchy(mg), lg.guess_level(mg)
+    k_cells = mg.level_info(k)[0]
+    rows = []
+    for it, (AC, AF, b) in zip(iterations, systems):
+        A = lg.csr_matrix(fs, fn, AC, AF)
+        starts = {"zero": np.zeros_like(b), "guess": lg.guess_start(net, hier, fs, fn, AC, AF, b, k)}
+        row = {"outer_iteration": it}
+        for name, x in starts.items():
+            hist, done = {"0": float(f"{lg.relative_residual(A, b, x):.4g}")}, 0
+            for c in checkpoints:
+                x = mg.solve(AC, AF, b, x, cycles=c - done)  # continues from x: the same cycles
+                done = c
+                hist[str(c)] = float(f"{lg.relative_residual(A, b, x):.4g}")
+            row[name] = hist
+        rows.append(row)
+        log(f"guess per solve, system after iteration {it}: residual after 3 / 3000 cycles: zero "
+            f"{row['zero']['3']:.3g} / {row['zero']['3000']:.3g}, guess {row['guess']['3']:.3g} / "
+            f"{row['guess']['3000']:.3g}")
+    mg.close()
+    return {"what": "relative residual of single p' solves from zero and from the network's start, by V-cycles",
+            "guess_level": k, "guess_level_cells": k_cells, "systems": rows, "seconds": round(time.time() - t0, 1)}
 
 
 def converge_pair(make, cap: int, keep: dict | None = None) -> dict:

@@ -38,18 +38,22 @@ def test_mixed_vcycle_is_fp64_vcycle_up_to_rounding(require_gpu):
     assert rel(x32, x64) < 1e-5
 
 
-@pytest.mark.parametrize("direct", [False, True])
-def test_mixed_cycle_matches_textbook(require_gpu, direct):
+@pytest.mark.parametrize("direct,levels", [(False, 0), (True, 0), (True, 4)])
+def test_mixed_cycle_matches_textbook(require_gpu, direct, levels):
     """FP32 V-cycles in FP64 refinement, with and without the FP64 coarsest
     solve, against the textbook FP64 cycle (scipy.sparse).  Note: with the
     exact coarsest solve this unsmoothed-aggregation cycle over-corrects and the
     L1 residual grows over the first cycles on this system -- the textbook cycle
-    does the same; the reference's 30 coarse sweeps damp that correction."""
+    does the same; the reference's 30 coarse sweeps damp that correction.
+    Four levels leave a coarsest level of a few hundred cells: its inverse is
+    the blocked Gauss-Jordan one, applied one wave per row."""
     from mg_textbook import TextbookMG, spmat
 
     case = axs.nozzle_case(mesh=axs.nozzle_mesh(spacing=2e-4))
     fs, fn, AC, AF, b = pp_system(case)
-    g = axs.Multigrid(fs, fn)
+    g = axs.Multigrid(fs, fn, max_levels=levels)
+    if levels:
+        assert 256 < g.level_info(g.levels - 1)[0] <= 1024
     x = g.solve(AC, AF, b, np.zeros_like(b), cycles=3, precision=axs.PREC_MIXED, coarse_direct=int(direct))
     tb = TextbookMG(spmat(fs, fn, AC, AF), [g.maps(l)["cell_to_coarse"] for l in range(g.levels - 1)])
     ref = tb.solve(b, np.zeros_like(b), 3, direct=direct)
@@ -134,13 +138,21 @@ def test_mixed_simple_converges_to_fp64_solution(require_gpu, refine):
     assert abs(st.iterations - o.iterations) <= 1
 
 
-def test_mixed_direct_coarse_converges_to_fp64_solution(require_gpu):
+@pytest.mark.parametrize("which", ["pipe", "nozzle"])
+def test_mixed_direct_coarse_converges_to_fp64_solution(require_gpu, which):
     """With the exact FP64 coarsest solve the outer iterations take another
-    path (fewer of them) to the same solution: both runs go to 1e-10, where the
-    fields agree to 1e-8."""
-    a = axs.make_solver(axs.pipe_case(tol=1e-10))
+    path (fewer of them) to the same solution: both runs go to 1e-10 (the
+    nozzle, whose small radial velocity needs it, to 1e-12), where the fields
+    agree to 1e-8.  The nozzle at three levels has a coarsest level of a few
+    hundred cells (the blocked inverse)."""
+    if which == "pipe":
+        make = lambda **kw: axs.pipe_case(tol=1e-10, **kw)
+    else:
+        mesh = axs.nozzle_mesh(axs.nozzle_spacing_for(6000))
+        make = lambda **kw: axs.nozzle_case(mesh=mesh, re_throat=100, tol=1e-12, mg_max_levels=3, **kw)
+    a = axs.make_solver(make())
     a.run()
-    b = axs.make_solver(axs.pipe_case(tol=1e-10, precision=axs.PREC_MIXED, coarse_direct=1))
+    b = axs.make_solver(make(precision=axs.PREC_MIXED, coarse_direct=1))
     b.run()
     assert a.stats().converged and b.stats().converged
     print("iterations fp64", a.stats().iterations, "mixed + direct coarse", b.stats().iterations)
