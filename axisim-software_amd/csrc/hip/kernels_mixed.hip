@@ -456,156 +456,256 @@ __global__ void __launch_bounds__(kBlock) kCoarseDense(DLevel L, double* M, int 
             if (L.nb[k] >= 0) M[(size_t)r * N + L.nb[k]] += L.AF[k];
 }
 
-// In-place Gauss-Jordan inversion of the kGJ x kGJ tile T (LDS, row stride
-// kGJ + 1) by one wave; lane l holds row l / 2, columns 16 (l % 2) .. + 15.
-// Each pivot reads the old values it needs before any lane writes (one wave:
-// its LDS accesses complete in program order).
-__device__ void invertTileWave(double* T) {
-    constexpr int S = kGJ + 1;
-    const int lane = threadIdx.x & 63, i = lane >> 1, j0 = (lane & 1) * 16;
-    double v[16];
+// One workgroup of kGJThreads per tile; thread t holds the 2 x 2 block of rows
+// 2 (t / 16) + {0, 1}, columns 2 (t % 16) + {0, 1}.  Tiles sit in LDS with a
+// row stride of kGJS doubles (16-byte aligned rows); a left operand is stored
+// transposed, so that both operands of a product step are read as pairs.
+constexpr int kGJThreads = 256;
+constexpr int kGJS = kGJ + 2;
+
+// global tiles (row stride ld) -> LDS, as they are or transposed: this
+// thread's elements of a tile are loaded into registers first (gjFetch), so
+// that the loads of every tile a launch needs are in flight together, then
+// stored (gjStore)
+constexpr int kGJQ = kGJ * kGJ / kGJThreads;  // elements of a tile per thread
+__device__ __forceinline__ void gjFetch(double v[kGJQ], const double* src, int ld) {
 #pragma unroll
-    for (int q = 0; q < 16; q++) v[q] = T[i * S + j0 + q];
-    for (int p = 0; p < kGJ; p++) {
-        const double d = 1.0 / T[p * S + p];
-        const double fip = T[i * S + p];
-        double rp[16];
+    for (int q = 0; q < kGJQ; q++) {
+        const int e = threadIdx.x + kGJThreads * q;
+        v[q] = src[(size_t)(e / kGJ) * ld + e % kGJ];
+    }
+}
+template <bool TRANSPOSE>
+__device__ __forceinline__ void gjStore(double* T, const double v[kGJQ]) {
 #pragma unroll
-        for (int q = 0; q < 16; q++) rp[q] = T[p * S + j0 + q];
+    for (int q = 0; q < kGJQ; q++) {
+        const int e = threadIdx.x + kGJThreads * q, r = e / kGJ, c = e % kGJ;
+        T[TRANSPOSE ? c * kGJS + r : r * kGJS + c] = v[q];
+    }
+}
+template <bool TRANSPOSE>
+__device__ __forceinline__ void gjLoad(double* T, const double* src, int ld) {
+    double v[kGJQ];
+    gjFetch(v, src, ld);
+    gjStore<TRANSPOSE>(T, v);
+}
+
+// acc = A B for this thread's 2 x 2 block (At: A transposed); the sums run
+// over t = 0 .. kGJ - 1 in order
+__device__ __forceinline__ void gjProduct(const double* At, const double* B, double acc[2][2]) {
+    const int r0 = 2 * (threadIdx.x >> 4), c0 = 2 * (threadIdx.x & 15);
+    acc[0][0] = acc[0][1] = acc[1][0] = acc[1][1] = 0.0;
+#pragma unroll 8
+    for (int t = 0; t < kGJ; t++) {
+        const double2 x = *(const double2*)&At[t * kGJS + r0];
+        const double2 y = *(const double2*)&B[t * kGJS + c0];
+        acc[0][0] += x.x * y.x;
+        acc[0][1] += x.x * y.y;
+        acc[1][0] += x.y * y.x;
+        acc[1][1] += x.y * y.y;
+    }
+}
+
+// 4 x 4 in-place Gauss-Jordan inverse in registers (no pivoting)
+__device__ __forceinline__ void inv4(double q[4][4]) {
 #pragma unroll
-        for (int q = 0; q < 16; q++) {
-            const int j = j0 + q;
-            if (i == p) v[q] = (j == p) ? d : rp[q] * d;
-            else if (j == p) v[q] = -(fip * d);
-            else v[q] = v[q] - fip * (rp[q] * d);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    for (int p = 0; p < 4; p++) {
+        const double d = 1.0 / q[p][p];
+        double r[4], c[4];
 #pragma unroll
-        for (int q = 0; q < 16; q++) T[i * S + j0 + q] = v[q];
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        for (int j = 0; j < 4; j++) r[j] = q[p][j];
+#pragma unroll
+        for (int i = 0; i < 4; i++) c[i] = q[i][p];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                if (i == p) q[i][j] = (j == p) ? d : r[j] * d;
+                else if (j == p) q[i][j] = -(c[i] * d);
+                else q[i][j] = q[i][j] - c[i] * (r[j] * d);
+            }
+    }
+}
+
+// In-place blocked Gauss-Jordan inverse of the LDS tile T (row stride kGJS)
+// by the whole workgroup, with 4 x 4 pivot blocks: each pivot block is
+// inverted by every thread in registers, then each thread updates its 2 x 2
+// block from the pivot rows and columns it read before the barrier.
+__device__ void gjInvertTile(double* T) {
+    const int r0 = 2 * (threadIdx.x >> 4), c0 = 2 * (threadIdx.x & 15);
+    double own[2][2];  // this thread's block, current between pivot blocks
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) own[i][j] = T[(r0 + i) * kGJS + c0 + j];
+    for (int p0 = 0; p0 < kGJ; p0 += 4) {
+        double q[4][4], tr[2][4], tc[4][2];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) q[i][j] = T[(p0 + i) * kGJS + p0 + j];
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int s2 = 0; s2 < 4; s2++) tr[i][s2] = T[(r0 + i) * kGJS + p0 + s2];
+#pragma unroll
+        for (int s2 = 0; s2 < 4; s2++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) tc[s2][j] = T[(p0 + s2) * kGJS + c0 + j];
+        inv4(q);
+        const bool rin = r0 >= p0 && r0 < p0 + 4, cin = c0 >= p0 && c0 < p0 + 4;
+        const bool rlo = r0 == p0, clo = c0 == p0;  // which half of the pivot block, where inside
+        // this thread's rows of the pivot block's inverse (where rin) and its
+        // columns (where cin), picked by value so that q stays in registers
+        double qrow[2][4], qcol[4][2];
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int u = 0; u < 4; u++) qrow[i][u] = rlo ? q[i][u] : q[2 + i][u];
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) qcol[u][j] = clo ? q[u][j] : q[u][2 + j];
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                double v;
+                if (rin && cin) {
+                    v = clo ? qrow[i][j] : qrow[i][2 + j];
+                } else if (rin) {
+                    v = 0.0;
+#pragma unroll
+                    for (int u = 0; u < 4; u++) v += qrow[i][u] * tc[u][j];
+                } else if (cin) {
+                    v = 0.0;
+#pragma unroll
+                    for (int u = 0; u < 4; u++) v += tr[i][u] * qcol[u][j];
+                    v = -v;
+                } else {
+                    double w = 0.0;
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        double ws = 0.0;
+#pragma unroll
+                        for (int x = 0; x < 4; x++) ws += q[u][x] * tc[x][j];
+                        w += tr[i][u] * ws;
+                    }
+                    v = own[i][j] - w;
+                }
+                own[i][j] = v;
+            }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) T[(r0 + i) * kGJS + c0 + j] = own[i][j];
+        __syncthreads();
     }
 }
 
 // P0 = (M_00)^-1
-__global__ void __launch_bounds__(64) kGJPivot0(const double* M, int N, double* P, const Ctl* ctl) {
-    __shared__ double T[kGJ * (kGJ + 1)];
+__global__ void __launch_bounds__(kGJThreads) kGJPivot0(const double* M, int N, double* P, const Ctl* ctl) {
+    __shared__ __align__(16) double T[kGJ * kGJS];
     if (finished(ctl)) return;
-    const int lane = threadIdx.x;
-    for (int e = lane; e < kGJ * kGJ; e += 64) T[(e / kGJ) * (kGJ + 1) + e % kGJ] = M[(size_t)(e / kGJ) * N + e % kGJ];
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    invertTileWave(T);
-    for (int e = lane; e < kGJ * kGJ; e += 64) P[e] = T[(e / kGJ) * (kGJ + 1) + e % kGJ];
+    gjLoad<false>(T, M, N);
+    __syncthreads();
+    gjInvertTile(T);
+    const int r0 = 2 * (threadIdx.x >> 4), c0 = 2 * (threadIdx.x & 15);
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) P[(r0 + i) * kGJ + c0 + j] = T[(r0 + i) * kGJS + c0 + j];
 }
 
-// tile loads into LDS (row stride kGJ + 1): every load of the tile issued
-// before the first store, so they are in flight together; and the 4 x 4 block
-// of a product
-constexpr int kTileQ = kGJ * kGJ / 64;  // elements per lane
-__device__ __forceinline__ void loadTile(double* T, const double* src, int ld) {
-    const int lane = threadIdx.x & 63;
-    double v[kTileQ];
-#pragma unroll
-    for (int q = 0; q < kTileQ; q++) {
-        const int e = lane + 64 * q;
-        v[q] = src[(size_t)(e / kGJ) * ld + e % kGJ];
-    }
-#pragma unroll
-    for (int q = 0; q < kTileQ; q++) {
-        const int e = lane + 64 * q;
-        T[(e / kGJ) * (kGJ + 1) + e % kGJ] = v[q];
-    }
-}
-__device__ __forceinline__ void tileProduct(const double* A, const double* B, int r0, int c0, double acc[4][4]) {
-    constexpr int S = kGJ + 1;
-#pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-        for (int c = 0; c < 4; c++) acc[a][c] = 0.0;
-    for (int t = 0; t < kGJ; t++) {
-        double x[4], y[4];
-#pragma unroll
-        for (int a = 0; a < 4; a++) x[a] = A[(r0 + a) * S + t];
-#pragma unroll
-        for (int c = 0; c < 4; c++) y[c] = B[t * S + c0 + c];
-#pragma unroll
-        for (int a = 0; a < 4; a++)
-#pragma unroll
-            for (int c = 0; c < 4; c++) acc[a][c] += x[a] * y[c];
-    }
-}
-
-__global__ void __launch_bounds__(64) kGJStep(const double* __restrict__ M, double* __restrict__ Mn, int N, int k,
-                                              const double* __restrict__ P, double* __restrict__ Pn,
-                                              const Ctl* ctl) {
-    constexpr int S = kGJ + 1;
-    __shared__ double A[kGJ * S], B[kGJ * S], T[kGJ * S];
+__global__ void __launch_bounds__(kGJThreads) kGJStep(const double* __restrict__ M, double* __restrict__ Mn, int N,
+                                                      int k, const double* __restrict__ P, double* __restrict__ Pn,
+                                                      const Ctl* ctl) {
+    __shared__ __align__(16) double A[kGJ * kGJS], B[kGJ * kGJS], C[kGJ * kGJS], U[kGJ * kGJS];
     if (finished(ctl)) return;
-    const int K = N / kGJ, ti = blockIdx.x / K, tj = blockIdx.x % K;
-    const int lane = threadIdx.x, r0 = 4 * (lane >> 3), c0 = 4 * (lane & 7);
-    const size_t base = (size_t)ti * kGJ * N + (size_t)tj * kGJ;
+    // workgroup 0 takes the tile (k + 1, k + 1), whose launch is the longest
+    // (it also inverts the next pivot block), so that it starts first
+    const int K = N / kGJ, kn = k + 1 < K ? k + 1 : 0;
+    const int id = blockIdx.x == 0 ? kn * K + kn : (blockIdx.x == kn * K + kn ? 0 : (int)blockIdx.x);
+    const int ti = id / K, tj = id % K;
+    const int r0 = 2 * (threadIdx.x >> 4), c0 = 2 * (threadIdx.x & 15);
+    double* out = Mn + (size_t)ti * kGJ * N + (size_t)tj * kGJ;
+    const double* Mij = M + (size_t)ti * kGJ * N + (size_t)tj * kGJ;
     const double* Mkj = M + (size_t)k * kGJ * N + (size_t)tj * kGJ;
     const double* Mik = M + (size_t)ti * kGJ * N + (size_t)k * kGJ;
-    double acc[4][4];
+    auto put = [&](double v[2][2], double sg) {
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) out[(size_t)(r0 + i) * N + c0 + j] = sg * v[i][j];
+    };
+    double acc[2][2];
     if (ti == k && tj == k) {
 #pragma unroll
-        for (int q = 0; q < kTileQ; q++) {
-            const int e = lane + 64 * q;
-            Mn[base + (size_t)(e / kGJ) * N + e % kGJ] = P[e];
-        }
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) acc[i][j] = P[(r0 + i) * kGJ + c0 + j];
+        put(acc, 1.0);
         return;
     }
-    if (ti == k || tj == k) {
-        // P M_kj, or -M_ik P
-        if (ti == k) {
-            loadTile(A, P, kGJ);
-            loadTile(B, Mkj, N);
-        } else {
-            loadTile(A, Mik, N);
-            loadTile(B, P, kGJ);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        tileProduct(A, B, r0, c0, acc);
-        const double sg = ti == k ? 1.0 : -1.0;
-#pragma unroll
-        for (int a = 0; a < 4; a++)
-#pragma unroll
-            for (int c = 0; c < 4; c++) Mn[base + (size_t)(r0 + a) * N + c0 + c] = sg * acc[a][c];
+    double va[kGJQ], vb[kGJQ], vc[kGJQ];
+    if (ti == k) {  // P M_kj
+        gjFetch(va, P, kGJ);
+        gjFetch(vb, Mkj, N);
+        gjStore<true>(A, va);
+        gjStore<false>(B, vb);
+        __syncthreads();
+        gjProduct(A, B, acc);
+        put(acc, 1.0);
         return;
     }
-    // M_ij - M_ik (P M_kj): the three tiles and this lane's block of M_ij are
-    // loaded before any product
-    double mij[4][4];
+    if (tj == k) {  // -M_ik P
+        gjFetch(va, Mik, N);
+        gjFetch(vb, P, kGJ);
+        gjStore<true>(A, va);
+        gjStore<false>(B, vb);
+        __syncthreads();
+        gjProduct(A, B, acc);
+        put(acc, -1.0);
+        return;
+    }
+    // M_ij - M_ik (P M_kj)
+    double mij[2][2];
 #pragma unroll
-    for (int a = 0; a < 4; a++)
+    for (int i = 0; i < 2; i++)
 #pragma unroll
-        for (int c = 0; c < 4; c++) mij[a][c] = M[base + (size_t)(r0 + a) * N + c0 + c];
-    loadTile(A, P, kGJ);
-    loadTile(B, Mkj, N);
-    loadTile(T, Mik, N);
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    tileProduct(A, B, r0, c0, acc);  // P M_kj
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        for (int j = 0; j < 2; j++) mij[i][j] = Mij[(size_t)(r0 + i) * N + c0 + j];
+    gjFetch(va, P, kGJ);
+    gjFetch(vb, Mkj, N);
+    gjFetch(vc, Mik, N);
+    gjStore<true>(A, va);
+    gjStore<false>(B, vb);
+    gjStore<true>(C, vc);
+    __syncthreads();
+    gjProduct(A, B, acc);
 #pragma unroll
-    for (int a = 0; a < 4; a++)
+    for (int i = 0; i < 2; i++)
 #pragma unroll
-        for (int c = 0; c < 4; c++) B[(r0 + a) * S + c0 + c] = acc[a][c];
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    tileProduct(T, B, r0, c0, acc);  // M_ik (P M_kj)
-    const bool nextPivot = ti == k + 1 && tj == k + 1;
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        for (int j = 0; j < 2; j++) U[(r0 + i) * kGJS + c0 + j] = acc[i][j];
+    __syncthreads();
+    gjProduct(C, U, acc);
 #pragma unroll
-    for (int a = 0; a < 4; a++)
+    for (int i = 0; i < 2; i++)
 #pragma unroll
-        for (int c = 0; c < 4; c++) {
-            const double v = mij[a][c] - acc[a][c];
-            Mn[base + (size_t)(r0 + a) * N + c0 + c] = v;
-            if (nextPivot) B[(r0 + a) * S + c0 + c] = v;
-        }
-    if (!nextPivot) return;
-    // the next step's pivot block, inverted here
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    invertTileWave(B);
-    for (int e = lane; e < kGJ * kGJ; e += 64) Pn[e] = B[(e / kGJ) * S + e % kGJ];
+        for (int j = 0; j < 2; j++) mij[i][j] -= acc[i][j];
+    put(mij, 1.0);
+    if (!(ti == k + 1 && tj == k + 1)) return;
+    // the next step's pivot block, inverted here (B is free again)
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) B[(r0 + i) * kGJS + c0 + j] = mij[i][j];
+    __syncthreads();
+    gjInvertTile(B);
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) Pn[(r0 + i) * kGJ + c0 + j] = B[(r0 + i) * kGJS + c0 + j];
 }
 
 // x_c = A_c^-1 b_c with one wave per row (FP64 products summed in a fixed
@@ -680,9 +780,9 @@ void launchCoarseInvert(hipStream_t s, const DLevel& L, double* aug, double* inv
     double* P[2] = {aug + (size_t)N * N, aug + (size_t)N * N + kGJ * kGJ};
     double* m[2] = {K % 2 == 0 ? inv : aug, K % 2 == 0 ? aug : inv};
     kCoarseDense<<<N, kBlock, 0, s>>>(L, m[0], N, ctl);
-    kGJPivot0<<<1, 64, 0, s>>>(m[0], N, P[0], ctl);
+    kGJPivot0<<<1, kGJThreads, 0, s>>>(m[0], N, P[0], ctl);
     for (int k = 0; k < K; k++)
-        kGJStep<<<K * K, 64, 0, s>>>(m[k & 1], m[(k + 1) & 1], N, k, P[k & 1], P[(k + 1) & 1], ctl);
+        kGJStep<<<K * K, kGJThreads, 0, s>>>(m[k & 1], m[(k + 1) & 1], N, k, P[k & 1], P[(k + 1) & 1], ctl);
 }
 void launchCoarseApply(hipStream_t s, const DLevel& L, const double* inv, const Ctl* ctl) {
     kCoarseApplyWide<<<(L.n + kBlock / 64 - 1) / (kBlock / 64), kBlock, 0, s>>>(L, inv, coarseInverseLd(L.n), ctl);

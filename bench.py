@@ -168,9 +168,9 @@ def main() -> int:
     # same number of iterations from the same cold start
     mixed = None
     if not args.no_mixed:
-        def mixed_leg(refine):
+        def mixed_leg(refine, coarse_direct=0):
             mcase = axs.nozzle_case(mesh=m, max_iter=pre_iters, precision=axs.PREC_MIXED,
-                                    mg_max_levels=args.mg_levels, mixed_refine=refine)
+                                    mg_max_levels=args.mg_levels, mixed_refine=refine, coarse_direct=coarse_direct)
             sm = axs.make_solver(mcase)
             sm.run()
             mms = sm.run_timed(args.steps)
@@ -188,6 +188,10 @@ def main() -> int:
         # the FP64 defect before every V-cycle
         cit_s, cms, cerr, cnodes = mixed_leg(axs.REFINE_CYCLE)
         log(f"mixed, refined every cycle: {cit_s:.1f} it/s ({cit_s / it_s:.2f}x)")
+        # with the FP64 direct solve of the coarsest level (its inverse formed
+        # once per outer iteration)
+        dit_s, dms, derr, dnodes = mixed_leg(axs.REFINE_SOLVE, coarse_direct=1)
+        log(f"mixed, direct coarsest solve: {dit_s:.1f} it/s ({dit_s / it_s:.2f}x)")
         mixed = {"value": round(mit_s, 3), "unit": "outer_iter/s", "ms_per_step": round(mms / args.steps, 5),
                  "speedup_vs_fp64": round(mit_s / it_s, 3),
                  "dtype": "f32 sweeps/V-cycles, f64 defects, assembly, fluxes and residuals",
@@ -198,7 +202,11 @@ def main() -> int:
                  "refined_every_cycle": {"value": round(cit_s, 3), "speedup_vs_fp64": round(cit_s / it_s, 3),
                                          "ms_per_step": round(cms / args.steps, 5), "graph_nodes_per_iteration": cnodes,
                                          "rel_l2_vs_fp64": cerr, "mg_solve_ms": round(mg32c_ms, 4),
-                                         "mg_solve_gbs": round(mg32c_gbs, 1)}}
+                                         "mg_solve_gbs": round(mg32c_gbs, 1)},
+                 "direct_coarsest": {"value": round(dit_s, 3), "speedup_vs_fp64": round(dit_s / it_s, 3),
+                                     "ms_per_step": round(dms / args.steps, 5), "graph_nodes_per_iteration": dnodes,
+                                     "note": "per iteration; its gain is in the outer iterations to the tolerance "
+                                             "(to_tolerance.nozzle_1M.mixed_direct)"}}
 
     # runs to the 1e-6 residual from a cold start: the nozzle (the metric's
     # workload), the pipe and the expansion, each FP64 and mixed, with the
@@ -215,38 +223,43 @@ def main() -> int:
         }
 
     # the learned initial guess (weights trained by train_guess.py on other
-    # nozzles and the expansion): the nozzle to 1e-6 on the mixed path with the
-    # guess before each outer iteration's first p' solve
+    # nozzles and the expansion): the nozzle to 1e-6 on the full mixed path
+    # (FP32 cycles, FP64 refinement and direct coarsest solve) with the guess
+    # before each outer iteration's first p' solve
     guess = None
     weights = ROOT / "axisim-software_amd" / "data" / "guess_weights.npy"
     if not args.no_guess and not args.no_converge and weights.exists():
         net = axs.Net(layers=4)
         net.set_params(np.load(weights))
         gcase = axs.nozzle_case(mesh=m, mg_max_levels=args.mg_levels, max_iter=args.converge_cap,
-                                precision=axs.PREC_MIXED)
+                                precision=axs.PREC_MIXED, coarse_direct=1)
         sg = axs.make_solver(gcase)
         sg.set_learned_guess(net, -1)
         t = time.time()
         sg.run()
         wall = time.time() - t
         gst = sg.stats()
-        plain, plain32 = to_tol["nozzle_1M"]["fp64"], to_tol["nozzle_1M"]["mixed"]
+        nz = to_tol["nozzle_1M"]
+        plain, plain32, direct = nz["fp64"], nz["mixed"], nz["mixed_direct"]
         err = {}
         for f in ("u", "v", "p"):
             ref = nozzle_fp64[f]
             err[f] = float(np.linalg.norm(sg.field(f) - ref) / max(np.linalg.norm(ref), 1e-300))
-        guess = {"workload": "the 1M nozzle from a cold start to 1e-6, mixed precision with the learned guess",
+        guess = {"workload": "the 1M nozzle from a cold start to 1e-6, mixed precision (FP32 cycles, FP64 "
+                             "refinement and direct coarsest solve) with the learned guess",
                  "weights": "axisim-software_amd/data/guess_weights.npy (train_guess.py)",
                  "iterations": gst.iterations, "converged": bool(gst.converged), "wall_s": round(wall, 3),
                  "solve_s": round(gst.solve_ms / 1e3, 3),
-                 "iterations_saved_vs_mixed": plain32["iterations"] - gst.iterations,
+                 "iterations_saved_vs_mixed_direct": direct["iterations"] - gst.iterations,
                  "wall_vs_fp64": round(gst.solve_ms / 1e3 / plain["solve_s"], 4),
-                 "wall_vs_mixed": round(gst.solve_ms / 1e3 / plain32["solve_s"], 4),
+                 "speedup_to_tolerance_vs_fp64": round(plain["solve_s"] / (gst.solve_ms / 1e3), 3),
+                 "wall_vs_mixed_direct": round(gst.solve_ms / 1e3 / direct["solve_s"], 4),
                  "plain_fp64": {"iterations": plain["iterations"], "solve_s": plain["solve_s"]},
                  "mixed": {"iterations": plain32["iterations"], "solve_s": plain32["solve_s"]},
+                 "mixed_direct": {"iterations": direct["iterations"], "solve_s": direct["solve_s"]},
                  "rel_l2_vs_fp64": err}
-        log(f"learned guess: {gst.iterations} iterations to 1e-6 in {wall:.2f} s (mixed {plain32['iterations']}, "
-            f"fp64 {plain['iterations']})")
+        log(f"learned guess: {gst.iterations} iterations to 1e-6 in {wall:.2f} s (mixed + direct "
+            f"{direct['iterations']}, mixed {plain32['iterations']}, fp64 {plain['iterations']})")
         del sg
         guess["per_solve"] = guess_per_solve(net, m, args.mg_levels)
         del net
@@ -442,16 +455,19 @@ def guess_per_solve(net, m, mg_levels: int, iterations=(20, 200, 2000), checkpoi
 
 def converge_pair(make, cap: int, keep: dict | None = None) -> dict:
     """One case from a cold start to its residual tolerance (at most `cap`
-    iterations), FP64 and mixed: iterations, wall time and rate of each, and
-    the mixed fields' relative L2 error against the FP64 fields (kept in
-    `keep` when given)."""
+    iterations), FP64, mixed, and mixed with the FP64 direct solve of the
+    coarsest level: iterations, wall time and rate of each, and the mixed
+    fields' relative L2 error against the FP64 fields (kept in `keep` when
+    given)."""
     import numpy as np
 
     import axs
 
     out, fields = {}, {}
-    for name, prec in (("fp64", axs.PREC_FP64), ("mixed", axs.PREC_MIXED)):
-        case = make(max_iter=cap, precision=prec)
+    runs = (("fp64", dict(precision=axs.PREC_FP64)), ("mixed", dict(precision=axs.PREC_MIXED)),
+            ("mixed_direct", dict(precision=axs.PREC_MIXED, coarse_direct=1)))
+    for name, kw in runs:
+        case = make(max_iter=cap, **kw)
         s = axs.make_solver(case)
         t = time.time()
         s.run()
@@ -467,10 +483,11 @@ def converge_pair(make, cap: int, keep: dict | None = None) -> dict:
     if keep is not None:
         keep.update(fields["fp64"])
     out["cells"] = case.mesh.n_cells
-    out["mixed_rel_l2_vs_fp64"] = {
-        f: float(np.linalg.norm(fields["mixed"][f] - fields["fp64"][f]) / max(np.linalg.norm(fields["fp64"][f]), 1e-300))
-        for f in ("u", "v", "p")}
-    out["mixed_speedup"] = round(out["fp64"]["solve_s"] / out["mixed"]["solve_s"], 3)
+    for name in ("mixed", "mixed_direct"):
+        out[f"{name}_rel_l2_vs_fp64"] = {
+            f: float(np.linalg.norm(fields[name][f] - fields["fp64"][f]) / max(np.linalg.norm(fields["fp64"][f]), 1e-300))
+            for f in ("u", "v", "p")}
+        out[f"{name}_speedup"] = round(out["fp64"]["solve_s"] / out[name]["solve_s"], 3)
     return out
 
 
