@@ -169,7 +169,7 @@ typedef struct axs_config {
     int mg_max_levels;    /* 0 = reference rule (coarsen to <= 64 cells)        */
     int mg_smoother;      /* AXS_MG_SMOOTH_*                                    */
     int precision;        /* AXS_PREC_*                                         */
-    int coarse_direct;    /* 1 = FP64 LU solve on the coarsest level            */
+    int coarse_direct;    /* 1 = FP64 inverse on the coarsest level (<= 1024)   */
     /* Transient frames: ConfigSolver::saveKeyFrameIter (structs/solver_struct.h:230,
      * clamped to >= 1 by runCheck, solver/source/solver.cpp:557-559) and
      * Solver::maxTimeFrames (solver/header/solver.h:108) */
