@@ -715,7 +715,7 @@ void MGDevice::enqueueOperators(hipStream_t s, const MGParams& p, const Ctl* ctl
         ensureMixed();
         if (!p.l0Copies) launchMGCast(s, dev[0], ctl);
         buildOperators(s, ctl);  // also writes the FP32 copies
-        if (p.coarseDirect) launchCoarseInvert(s, dev[host.size() - 1], aug, inv, invStatus, ctl);
+        if (p.coarseDirect && p.formInverse) launchCoarseInvert(s, dev[host.size() - 1], aug, inv, invStatus, ctl);
     } else {
         buildOperators(s, ctl);
     }
@@ -1021,6 +1021,7 @@ void Solver::configure(const axs_config& c) {
         throw std::invalid_argument("bad multigrid smoother");
     if (c.mixed_refine != AXS_REFINE_CYCLE && c.mixed_refine != AXS_REFINE_SOLVE)
         throw std::invalid_argument("bad mixed-precision refinement");
+    if (c.coarse_direct < 0) throw std::invalid_argument("coarse_direct must be 0 (off) or the refresh interval");
     cfg_ = c;
     haveCfg_ = true;
 }
@@ -1286,6 +1287,7 @@ void Solver::enqueueIteration(hipStream_t s) {
 
     MGParams mp = mgParamsOf(c);
     mp.l0Copies = c.precision == AXS_PREC_MIXED;
+    mp.formInverse = formInverse_;
     for (int corr = 0; corr <= ph.nonOrtho; corr++) {
         const int cross = corr > 0 ? 1 : 0;
         if (cross) launchGradField(s, dm_, ppbc_, st.pp, st.gPZ, st.gPR, ph.gradScheme, st.ctl);
@@ -1480,18 +1482,20 @@ void Solver::enqueueScalars(hipStream_t s) {
 }
 
 void Solver::dropGraph() {
-    if (exec_) {
-        AXS_HIP(hipGraphExecDestroy(exec_));
-        exec_ = nullptr;
-    }
-    if (graph_) {
-        AXS_HIP(hipGraphDestroy(graph_));
-        graph_ = nullptr;
-    }
+    for (hipGraphExec_t* e : {&exec_, &execLag_})
+        if (*e) {
+            AXS_HIP(hipGraphExecDestroy(*e));
+            *e = nullptr;
+        }
+    for (hipGraph_t* g : {&graph_, &graphLag_})
+        if (*g) {
+            AXS_HIP(hipGraphDestroy(*g));
+            *g = nullptr;
+        }
 }
 
-void Solver::buildGraph() {
-    dropGraph();
+// One outer iteration captured as a graph (formInverse_ as given).
+void Solver::captureIteration(hipGraph_t* graph, hipGraphExec_t* exec) {
     AXS_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     try {
         enqueueIteration(stream_);
@@ -1500,18 +1504,35 @@ void Solver::buildGraph() {
         (void)hipStreamEndCapture(stream_, &g);
         if (g) (void)hipGraphDestroy(g);
         (void)hipGetLastError();  // leave no capture error behind for the next call
+        formInverse_ = true;
         throw;
     }
-    AXS_HIP(hipStreamEndCapture(stream_, &graph_));
+    AXS_HIP(hipStreamEndCapture(stream_, graph));
+    AXS_HIP(hipGraphInstantiate(exec, *graph, nullptr, nullptr, 0));
+    AXS_HIP(hipGraphUpload(*exec, stream_));
+}
+
+void Solver::buildGraph() {
+    dropGraph();
+    formInverse_ = true;
+    captureIteration(&graph_, &exec_);
     size_t nodes = 0;
     AXS_HIP(hipGraphGetNodes(graph_, nullptr, &nodes));
     graphNodes_ = (int)nodes;
-    AXS_HIP(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
-    AXS_HIP(hipGraphUpload(exec_, stream_));
+    if (cfg_.precision == AXS_PREC_MIXED && cfg_.coarse_direct > 1 && cfg_.use_multigrid) {
+        formInverse_ = false;
+        captureIteration(&graphLag_, &execLag_);
+        formInverse_ = true;
+    }
+    launches_ = 0;
 }
 
+// n outer iterations; with coarse_direct = K > 1 the first of every K forms
+// the coarsest inverse and the others apply it.
 void Solver::launchBatch(int n) {
-    for (int i = 0; i < n; i++) AXS_HIP(hipGraphLaunch(exec_, stream_));
+    const int every = cfg_.coarse_direct > 1 ? cfg_.coarse_direct : 1;
+    for (int i = 0; i < n; i++, launches_++)
+        AXS_HIP(hipGraphLaunch(execLag_ && launches_ % every != 0 ? execLag_ : exec_, stream_));
 }
 
 // Read ctl and the new history entries, report them.

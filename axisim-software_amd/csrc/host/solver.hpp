@@ -73,6 +73,9 @@ struct MGParams {
     bool l0Copies = false;
     // x = 0 and level 0's b32 holds b (written with the right-hand side)
     bool startZero = false;
+    // coarse_direct: the coarsest inverse is formed with the operators (else
+    // the one formed last is applied)
+    bool formInverse = true;
 };
 
 class MGDevice {
@@ -303,6 +306,7 @@ private:
     Arena sweepArena_;
     void prepareSweepBlocks();
     void dropGraph();
+    void captureIteration(hipGraph_t* graph, hipGraphExec_t* exec);
     std::vector<int> sweepChunks(int sweeps) const;
     std::vector<int> colorStart_;
     int* dOrder_ = nullptr;
@@ -311,6 +315,12 @@ private:
 
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;
+    // coarse_direct = K > 1: the iteration without the coarsest inverse's
+    // formation, launched on all but every K-th outer iteration
+    hipGraph_t graphLag_ = nullptr;
+    hipGraphExec_t execLag_ = nullptr;
+    long long launches_ = 0;     // graph launches since the graphs were built
+    bool formInverse_ = true;    // read by enqueueIteration while capturing
 
     Ctl* hctl_ = nullptr;   // pinned mirror
     double* hhist_ = nullptr;

@@ -169,7 +169,9 @@ typedef struct axs_config {
     int mg_max_levels;    /* 0 = reference rule (coarsen to <= 64 cells)        */
     int mg_smoother;      /* AXS_MG_SMOOTH_*                                    */
     int precision;        /* AXS_PREC_*                                         */
-    int coarse_direct;    /* 1 = FP64 inverse on the coarsest level (<= 1024)   */
+    int coarse_direct;    /* K >= 1: FP64 inverse on the coarsest level (at most
+                           * 1024 cells), formed on every K-th outer iteration
+                           * and applied in between; 0 = the coarsest sweeps */
     /* Transient frames: ConfigSolver::saveKeyFrameIter (structs/solver_struct.h:230,
      * clamped to >= 1 by runCheck, solver/source/solver.cpp:557-559) and
      * Solver::maxTimeFrames (solver/header/solver.h:108) */

@@ -138,13 +138,14 @@ def test_mixed_simple_converges_to_fp64_solution(require_gpu, refine):
     assert abs(st.iterations - o.iterations) <= 1
 
 
-@pytest.mark.parametrize("which", ["pipe", "nozzle"])
-def test_mixed_direct_coarse_converges_to_fp64_solution(require_gpu, which):
+@pytest.mark.parametrize("which,every", [("pipe", 1), ("nozzle", 1), ("pipe", 4), ("nozzle", 7)])
+def test_mixed_direct_coarse_converges_to_fp64_solution(require_gpu, which, every):
     """With the exact FP64 coarsest solve the outer iterations take another
     path (fewer of them) to the same solution: both runs go to 1e-10 (the
     nozzle, whose small radial velocity needs it, to 1e-12), where the fields
     agree to 1e-8.  The nozzle at three levels has a coarsest level of a few
-    hundred cells (the blocked inverse)."""
+    hundred cells (the blocked inverse).  every > 1: the inverse formed on every
+    `every`-th outer iteration and applied, one operator behind, in between."""
     if which == "pipe":
         make = lambda **kw: axs.pipe_case(tol=1e-10, **kw)
     else:
@@ -152,7 +153,7 @@ def test_mixed_direct_coarse_converges_to_fp64_solution(require_gpu, which):
         make = lambda **kw: axs.nozzle_case(mesh=mesh, re_throat=100, tol=1e-12, mg_max_levels=3, **kw)
     a = axs.make_solver(make())
     a.run()
-    b = axs.make_solver(make(precision=axs.PREC_MIXED, coarse_direct=1))
+    b = axs.make_solver(make(precision=axs.PREC_MIXED, coarse_direct=every))
     b.run()
     assert a.stats().converged and b.stats().converged
     print("iterations fp64", a.stats().iterations, "mixed + direct coarse", b.stats().iterations)
@@ -180,3 +181,35 @@ def test_mixed_expansion_converges_to_fp64_solution(require_gpu, refine):
         e = rel(b.field(name), a.field(name))
         print(name, "rel L2 mixed vs FP64", e)
         assert e < 1e-8, (name, e)
+
+
+@pytest.mark.parametrize("levels", [0, 3])
+def test_direct_refresh_interval_first_iterations(require_gpu, levels):
+    """coarse_direct = K forms the inverse on the first outer iteration of a
+    run and on every K-th after it: one iteration is bit-identical to K = 1's,
+    and the run to 1e-6 with the inverse one to four operators behind takes
+    about as many outer iterations as K = 1 (measured on the 1M nozzle: 6903
+    at K = 1, 6904-6915 at K = 2-64)."""
+    mesh = axs.nozzle_mesh(axs.nozzle_spacing_for(6000))
+    make = lambda **kw: axs.nozzle_case(mesh=mesh, re_throat=100, mg_max_levels=levels,
+                                        precision=axs.PREC_MIXED, **kw)
+    one = axs.make_solver(make(coarse_direct=1, max_iter=1))
+    one.run()
+    lag = axs.make_solver(make(coarse_direct=5, max_iter=1))
+    lag.run()
+    for name in ("u", "v", "p"):
+        assert np.array_equal(one.field(name), lag.field(name)), name
+    full = axs.make_solver(make(coarse_direct=1))
+    full.run()
+    every = axs.make_solver(make(coarse_direct=5))
+    every.run()
+    n1, n5 = full.stats().iterations, every.stats().iterations
+    print("iterations to 1e-6: every iteration", n1, "every fifth", n5)
+    assert full.stats().converged and every.stats().converged
+    assert n5 <= 1.1 * n1 + 5
+
+
+def test_direct_refresh_interval_rejects_negative(require_gpu):
+    case = axs.pipe_case(nz=32, nr=8, precision=axs.PREC_MIXED, coarse_direct=-1)
+    with pytest.raises(Exception):
+        axs.make_solver(case).run()
