@@ -710,13 +710,26 @@ __global__ void __launch_bounds__(kGJThreads) kGJStep(const double* __restrict__
 
 // x_c = A_c^-1 b_c with one wave per row (FP64 products summed in a fixed
 // order: lane l takes columns l, l + 64, ..., then a fixed shuffle tree);
-// inv has row stride ld
+// inv has row stride ld.  NC >= ceil(n / 64) column rounds per lane, all
+// loaded at once (indices clamped into the row, the extra ones not summed).
+template <int NC>
 __global__ void __launch_bounds__(kBlock) kCoarseApplyWide(DLevel L, const double* inv, int ld, const Ctl* ctl) {
     if (finished(ctl)) return;
     const int lane = threadIdx.x & 63, r = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
     if (r >= L.n) return;  // the whole wave
+    const double* row = inv + (size_t)r * ld;
+    double a[NC];
+    float x[NC];
+#pragma unroll
+    for (int i = 0; i < NC; i++) {
+        const int c = min(lane + 64 * i, L.n - 1);
+        a[i] = row[c];
+        x[i] = L.b32[c];
+    }
     double acc = 0.0;
-    for (int c = lane; c < L.n; c += 64) acc += inv[(size_t)r * ld + c] * (double)L.b32[c];
+#pragma unroll
+    for (int i = 0; i < NC; i++)
+        if (lane + 64 * i < L.n) acc += a[i] * (double)x[i];
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
     if (lane == 0) L.x32[r] = (float)acc;
 }
@@ -784,8 +797,14 @@ void launchCoarseInvert(hipStream_t s, const DLevel& L, double* aug, double* inv
     for (int k = 0; k < K; k++)
         kGJStep<<<K * K, kGJThreads, 0, s>>>(m[k & 1], m[(k + 1) & 1], N, k, P[k & 1], P[(k + 1) & 1], ctl);
 }
+static_assert(kDirectMax <= 16 * 64, "kCoarseApplyWide<16> covers at most 1024 columns");
 void launchCoarseApply(hipStream_t s, const DLevel& L, const double* inv, const Ctl* ctl) {
-    kCoarseApplyWide<<<(L.n + kBlock / 64 - 1) / (kBlock / 64), kBlock, 0, s>>>(L, inv, coarseInverseLd(L.n), ctl);
+    if (L.n > kDirectMax) throw std::invalid_argument("coarsest level too large for the dense FP64 solve");
+    const dim3 g((L.n + kBlock / 64 - 1) / (kBlock / 64)), b(kBlock);
+    const int ld = coarseInverseLd(L.n);
+    if (L.n <= 256) kCoarseApplyWide<4><<<g, b, 0, s>>>(L, inv, ld, ctl);
+    else if (L.n <= 512) kCoarseApplyWide<8><<<g, b, 0, s>>>(L, inv, ld, ctl);
+    else kCoarseApplyWide<16><<<g, b, 0, s>>>(L, inv, ld, ctl);
 }
 
 namespace {
