@@ -66,7 +66,7 @@ struct ConfigMultigrid {
     // build extensions (no reference counterpart)
     int maxLevels = 0;
     int precision = AXS_PREC_FP64;
-    int coarseDirect = 0;
+    int coarseDirect = 0;  // K >= 1: FP64 coarsest inverse formed every K-th outer iteration
 };
 
 struct ConfigSolver {
