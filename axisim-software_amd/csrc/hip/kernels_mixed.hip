@@ -712,26 +712,28 @@ __global__ void __launch_bounds__(kGJThreads) kGJStep(const double* __restrict__
 // order: lane l takes columns l, l + 64, ..., then a fixed shuffle tree);
 // inv has row stride ld.  NC >= ceil(n / 64) column rounds per lane, all
 // loaded at once (indices clamped into the row, the extra ones not summed).
-template <int NC>
-__global__ void __launch_bounds__(kBlock) kCoarseApplyWide(DLevel L, const double* inv, int ld, const Ctl* ctl) {
+// T: the level's value type (float for the FP32 cycles, double for FP64 ones).
+template <int NC, typename T>
+__global__ void __launch_bounds__(kBlock) kCoarseApplyWide(int n, const double* inv, int ld, const T* __restrict__ b,
+                                                           T* __restrict__ xOut, const Ctl* ctl) {
     if (finished(ctl)) return;
     const int lane = threadIdx.x & 63, r = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-    if (r >= L.n) return;  // the whole wave
+    if (r >= n) return;  // the whole wave
     const double* row = inv + (size_t)r * ld;
     double a[NC];
-    float x[NC];
+    T x[NC];
 #pragma unroll
     for (int i = 0; i < NC; i++) {
-        const int c = min(lane + 64 * i, L.n - 1);
+        const int c = min(lane + 64 * i, n - 1);
         a[i] = row[c];
-        x[i] = L.b32[c];
+        x[i] = b[c];
     }
     double acc = 0.0;
 #pragma unroll
     for (int i = 0; i < NC; i++)
-        if (lane + 64 * i < L.n) acc += a[i] * (double)x[i];
+        if (lane + 64 * i < n) acc += a[i] * (double)x[i];
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-    if (lane == 0) L.x32[r] = (float)acc;
+    if (lane == 0) xOut[r] = (T)acc;
 }
 
 // x_c = A_c^-1 b_c (FP64 product, FP32 in / out)
@@ -798,13 +800,22 @@ void launchCoarseInvert(hipStream_t s, const DLevel& L, double* aug, double* inv
         kGJStep<<<K * K, kGJThreads, 0, s>>>(m[k & 1], m[(k + 1) & 1], N, k, P[k & 1], P[(k + 1) & 1], ctl);
 }
 static_assert(kDirectMax <= 16 * 64, "kCoarseApplyWide<16> covers at most 1024 columns");
+namespace {
+template <typename T>
+void coarseApply(hipStream_t s, int n, const double* inv, const T* b, T* x, const Ctl* ctl) {
+    if (n > kDirectMax) throw std::invalid_argument("coarsest level too large for the dense FP64 solve");
+    const dim3 g((n + kBlock / 64 - 1) / (kBlock / 64)), blk(kBlock);
+    const int ld = coarseInverseLd(n);
+    if (n <= 256) kCoarseApplyWide<4, T><<<g, blk, 0, s>>>(n, inv, ld, b, x, ctl);
+    else if (n <= 512) kCoarseApplyWide<8, T><<<g, blk, 0, s>>>(n, inv, ld, b, x, ctl);
+    else kCoarseApplyWide<16, T><<<g, blk, 0, s>>>(n, inv, ld, b, x, ctl);
+}
+}  // namespace
 void launchCoarseApply(hipStream_t s, const DLevel& L, const double* inv, const Ctl* ctl) {
-    if (L.n > kDirectMax) throw std::invalid_argument("coarsest level too large for the dense FP64 solve");
-    const dim3 g((L.n + kBlock / 64 - 1) / (kBlock / 64)), b(kBlock);
-    const int ld = coarseInverseLd(L.n);
-    if (L.n <= 256) kCoarseApplyWide<4><<<g, b, 0, s>>>(L, inv, ld, ctl);
-    else if (L.n <= 512) kCoarseApplyWide<8><<<g, b, 0, s>>>(L, inv, ld, ctl);
-    else kCoarseApplyWide<16><<<g, b, 0, s>>>(L, inv, ld, ctl);
+    coarseApply<float>(s, L.n, inv, L.b32, L.x32, ctl);
+}
+void launchCoarseApply64(hipStream_t s, const DLevel& L, const double* inv, const Ctl* ctl) {
+    coarseApply<double>(s, L.n, inv, L.b, L.x, ctl);
 }
 
 namespace {

@@ -50,5 +50,7 @@ void launchMGApply(hipStream_t s, const DLevel& L, const Ctl* ctl);
 void launchMGCast(hipStream_t s, const DLevel& L, const Ctl* ctl);
 void launchCoarseInvert(hipStream_t s, const DLevel& L, double* aug, double* inv, int* status, const Ctl* ctl);
 void launchCoarseApply(hipStream_t s, const DLevel& L, const double* inv, const Ctl* ctl);
+// the same on the level's FP64 b and x (the FP64 cycles)
+void launchCoarseApply64(hipStream_t s, const DLevel& L, const double* inv, const Ctl* ctl);
 
 }  // namespace axs

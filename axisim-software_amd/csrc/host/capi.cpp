@@ -320,8 +320,7 @@ static MGParams mgParams(int cycles, double weight, int pre, int post, int coars
     if (precision != AXS_PREC_FP64 && precision != AXS_PREC_MIXED) throw std::invalid_argument("bad precision");
     if (smoother != AXS_MG_SMOOTH_JACOBI && smoother != AXS_MG_SMOOTH_RBGS)
         throw std::invalid_argument("bad multigrid smoother");
-    if (direct && precision != AXS_PREC_MIXED)
-        throw std::domain_error("the FP64 coarsest direct solve is built for mixed precision");
+    if (direct < 0) throw std::invalid_argument("coarse_direct must be 0 or 1");
     if (pre < 0 || post < 0 || coarse < 0) throw std::invalid_argument("negative sweep count");
     MGParams p;
     p.cycles = cycles;

@@ -460,16 +460,21 @@ void MGDevice::restrictTo(hipStream_t s, int l, int sweep1, bool fp32, const MGP
 // vCycle, multigrid.cu:607-662.  preDone: pre-smoothing sweeps of level l
 // already applied (by the restriction above it).
 void MGDevice::vcycle(hipStream_t s, int l, int preDone, const MGParams& p, const Ctl* ctl) {
-    if (l == tail64_.lt) {
-        launchMGTail(s, tail64_.launch(), p.weight, preDone, p.pre, p.post, p.coarseSweeps, false, nullptr, ctl);
+    // coarse_direct: as in vcycle32, the inverse applied inside the tail when
+    // one workgroup formed it, else across the GPU after ordinary launches
+    const bool wideDirect = p.coarseDirect && coarsestCells() > kDirectOneBlock;
+    if (l == tail64_.lt && !wideDirect) {
+        launchMGTail(s, tail64_.launch(), p.weight, preDone, p.pre, p.post, p.coarseSweeps, false,
+                     p.coarseDirect ? inv : nullptr, ctl);
         return;
     }
     if (host[l].coarsest()) {
-        smooth(s, l, p.coarseSweeps - preDone, p, ctl);
+        if (p.coarseDirect) launchCoarseApply64(s, dev[l], inv, ctl);
+        else smooth(s, l, p.coarseSweeps - preDone, p, ctl);
         return;
     }
     smooth(s, l, p.pre - preDone, p, ctl);
-    const int s1 = host[l + 1].coarsest() ? (p.coarseSweeps >= 1) : (p.pre >= 1);
+    const int s1 = host[l + 1].coarsest() ? (!p.coarseDirect && p.coarseSweeps >= 1) : (p.pre >= 1);
     restrictTo(s, l, s1, false, p, ctl);
     vcycle(s, l + 1, s1, p, ctl);
     if (p.post >= 1) {
@@ -495,8 +500,8 @@ bool MGDevice::fusesLevel0(const MGParams& p) const {
 // same on the FP32 copies, as vcycle32(0) repeated, the last post-sweep also
 // adding the iterate into `apply` (FP64) when given.
 void MGDevice::cyclesFused(hipStream_t s, int cycles, const MGParams& p, const Ctl* ctl, bool fp32, double* apply) {
-    // a coarsest level solved directly (FP32 cycles only) takes no sweep from zero
-    const bool direct = fp32 && p.coarseDirect;
+    // a coarsest level solved directly takes no sweep from zero
+    const bool direct = p.coarseDirect != 0;
     auto sweep1Of = [&](int l) { return host[l].coarsest() ? (!direct && p.coarseSweeps >= 1) : (p.pre >= 1); };
     const int s1 = sweep1Of(1);
     // level 1 folded into the restriction when no pre-sweep runs there between
@@ -571,7 +576,8 @@ void MGDevice::smoothGS(hipStream_t s, int l, int sweeps, bool fp32, const Ctl* 
 // on every level, the prolongation is a plain x += P x_c.
 void MGDevice::vcycleGS(hipStream_t s, int l, bool fp32, const MGParams& p, const Ctl* ctl) {
     if (host[l].coarsest()) {
-        if (fp32 && p.coarseDirect) launchCoarseApply(s, dev[l], inv, ctl);
+        if (p.coarseDirect && fp32) launchCoarseApply(s, dev[l], inv, ctl);
+        else if (p.coarseDirect) launchCoarseApply64(s, dev[l], inv, ctl);
         else smoothGS(s, l, p.coarseSweeps, fp32, ctl);
         return;
     }
@@ -624,6 +630,16 @@ void MGDevice::ensureMixed() {
         invStatus = arena32.zeros<int>(1);
     }
     mixedReady = true;
+}
+
+void MGDevice::ensureInverse() {
+    if (inv) return;
+    const int nc = coarsestCells();
+    if (nc > kDirectMax) return;  // the solve rejects it
+    const size_t ld = (size_t)coarseInverseLd(nc);
+    aug = arena32.zeros<double>(coarseInverseWork(nc));
+    inv = arena32.zeros<double>(ld * ld);
+    invStatus = arena32.zeros<int>(1);
 }
 
 void MGDevice::smooth32(hipStream_t s, int l, int sweeps, const MGParams& p, const Ctl* ctl) {
@@ -718,6 +734,10 @@ void MGDevice::enqueueOperators(hipStream_t s, const MGParams& p, const Ctl* ctl
         if (p.coarseDirect && p.formInverse) launchCoarseInvert(s, dev[host.size() - 1], aug, inv, invStatus, ctl);
     } else {
         buildOperators(s, ctl);
+        if (p.coarseDirect && p.formInverse) {
+            ensureInverse();
+            launchCoarseInvert(s, dev[host.size() - 1], aug, inv, invStatus, ctl);
+        }
     }
 }
 
@@ -739,7 +759,10 @@ void MGDevice::enqueueSolve(hipStream_t s, const MGParams& p, const Ctl* ctl, bo
         enqueueSolveMixed(s, p, ctl, buildOperators);
         return;
     }
-    if (p.coarseDirect) throw std::domain_error("the FP64 coarsest direct solve is built for mixed precision only");
+    if (p.coarseDirect) {
+        if (coarsestCells() > kDirectMax) throw std::domain_error("coarsest level too large for the dense FP64 solve");
+        ensureInverse();
+    }
     if (buildOperators) enqueueOperators(s, p, ctl);
     const int cycles = p.cycles > 1 ? p.cycles : 1;  // cycleCount, multigrid.cuh:226
     if (p.smoother == AXS_MG_SMOOTH_RBGS) {
@@ -789,7 +812,7 @@ double MGDevice::bytesPerSolve(const MGParams& p, bool buildOperators) const {
             bytes += (dev[l - 1].nAggBlocks > 0 ? 24.0 : 32.0) * f.n + 16.0 * f.refs() + 12.0 * X +
                      (12.0 + w32) * c.n + (12.0 + w32) * c.refs();
         }
-        if (mixed && p.coarseDirect) bytes += 8.0 * 3.0 * host.back().n * host.back().n;
+        if (p.coarseDirect) bytes += 8.0 * 3.0 * host.back().n * host.back().n;
     }
     if (p.smoother == AXS_MG_SMOOTH_RBGS) {
         // a colour sweep pass reads AC, b, start, the colour order and the row (AF, nb)
@@ -803,7 +826,7 @@ double MGDevice::bytesPerSolve(const MGParams& p, bool buildOperators) const {
         for (int l = 0; l < L; l++) {
             const double n = host[l].n;
             if (host[l].coarsest()) {
-                cyc += (mixed && p.coarseDirect) ? 8.0 * n * n + 2.0 * v * n : gs(l, p.coarseSweeps);
+                cyc += p.coarseDirect ? 8.0 * n * n + 2.0 * v * n : gs(l, p.coarseSweeps);
                 break;
             }
             const double nc = host[l + 1].n;
@@ -829,14 +852,14 @@ double MGDevice::bytesPerSolve(const MGParams& p, bool buildOperators) const {
                 break;
             }
             if (host[l].coarsest()) {
-                cyc += (mixed && p.coarseDirect) ? 8.0 * host[l].n * host[l].n + 2.0 * v * host[l].n
+                cyc += p.coarseDirect ? 8.0 * host[l].n * host[l].n + 2.0 * v * host[l].n
                                                  : sm(l, p.coarseSweeps - preDone);
                 break;
             }
             const double n = host[l].n, S = host[l].refs(), nc = host[l + 1].n;
             cyc += sm(l, p.pre - preDone);
             const int s1 =
-                host[l + 1].coarsest() ? (!(mixed && p.coarseDirect) && p.coarseSweeps >= 1) : (p.pre >= 1);
+                host[l + 1].coarsest() ? (!p.coarseDirect && p.coarseSweeps >= 1) : (p.pre >= 1);
             // restriction (+ coarse AC for the folded sweep)
             cyc += restrictBytes(l, v) + (s1 ? v * nc : 0.0);
             if (p.post >= 1) cyc += (4.0 * v + 8.0) * n + (v + 4.0) * S + v * nc + sm(l, p.post - 1);
@@ -857,7 +880,7 @@ double MGDevice::bytesPerSolve(const MGParams& p, bool buildOperators) const {
         // restriction per cycle (residual, members; coarse b, x, AC), and the
         // last post-sweep
         const double n = host[0].n, S = host[0].refs(), nc = host[1].n;
-        const int s1 = host[1].coarsest() ? (!(mixed && p.coarseDirect) && p.coarseSweeps >= 1) : (p.pre >= 1);
+        const int s1 = host[1].coarsest() ? (!p.coarseDirect && p.coarseSweeps >= 1) : (p.pre >= 1);
         double l0 = (5.0 * v + 4.0) * n + (v + 4.0) * S + (once ? defect + applyX : 0.0);
         l0 += (cycles - 1) * ((5.0 * v + 8.0) * n + (v + 4.0) * S + v * nc);
         l0 += cycles * ((v + 4.0) * n + (2.0 * v + 4.0) * nc + (s1 ? v * nc : 0.0));
@@ -1519,7 +1542,7 @@ void Solver::buildGraph() {
     size_t nodes = 0;
     AXS_HIP(hipGraphGetNodes(graph_, nullptr, &nodes));
     graphNodes_ = (int)nodes;
-    if (cfg_.precision == AXS_PREC_MIXED && cfg_.coarse_direct > 1 && cfg_.use_multigrid) {
+    if (cfg_.coarse_direct > 1 && cfg_.use_multigrid) {
         formInverse_ = false;
         captureIteration(&graphLag_, &execLag_);
         formInverse_ = true;
@@ -1591,7 +1614,6 @@ void Solver::run(axs_progress_cb cb, void* user, bool continueRun) {
     if (mixed && c.linear_solver != AXS_LINEAR_JACOBI)
         throw std::domain_error("mixed precision momentum sweeps are Jacobi only");
     if (mixed && !c.use_multigrid) throw std::domain_error("mixed precision needs the multigrid pressure solve");
-    if (c.coarse_direct && !mixed) throw std::domain_error("the FP64 coarsest direct solve is built for mixed precision");
 
     if (!meshUploaded_) uploadMesh();
     std::string why;
@@ -1629,6 +1651,11 @@ void Solver::run(axs_progress_cb cb, void* user, bool continueRun) {
         ds_.AF32 = nullptr;
         ds_.uvInv32 = nullptr;
         ds_.ppAC32 = ds_.ppAF32 = ds_.ppb32 = nullptr;
+        if (c.coarse_direct && c.use_multigrid) {
+            if (mg_.coarsestCells() > kDirectMax)
+                throw std::domain_error("coarsest level too large for the dense FP64 solve");
+            mg_.ensureInverse();
+        }
     }
     prepareSweepBlocks();
     if (c.linear_solver == AXS_LINEAR_GS_RB && nColors_ == 0) {

@@ -95,6 +95,9 @@ public:
     void bindLevel0(double* AC, double* b, double* AF, double* x, double* xNew);
     // Allocate the FP32 buffers of every level (idempotent).
     void ensureMixed();
+    // the coarsest inverse's arrays (coarse_direct on the FP64 cycles);
+    // allocates, so it runs before any capture
+    void ensureInverse();
     // Colour every level for the Gauss-Seidel smoother (idempotent; allocates,
     // so it runs before any graph capture).
     void ensureColoring();

@@ -170,8 +170,8 @@ def main() -> int:
     # same number of iterations from the same cold start
     mixed = None
     if not args.no_mixed:
-        def mixed_leg(refine, coarse_direct=0):
-            mcase = axs.nozzle_case(mesh=m, max_iter=pre_iters, precision=axs.PREC_MIXED,
+        def mixed_leg(refine, coarse_direct=0, precision=axs.PREC_MIXED):
+            mcase = axs.nozzle_case(mesh=m, max_iter=pre_iters, precision=precision,
                                     mg_max_levels=args.mg_levels, mixed_refine=refine, coarse_direct=coarse_direct)
             sm = axs.make_solver(mcase)
             sm.run()
@@ -197,6 +197,10 @@ def main() -> int:
         kit_s, kms, kerr, knodes = mixed_leg(axs.REFINE_SOLVE, coarse_direct=args.direct_every)
         log(f"mixed, direct coarsest solve formed every {args.direct_every} iterations: {kit_s:.1f} it/s "
             f"({kit_s / it_s:.2f}x)")
+        # the FP64 path with the same direct coarsest solve
+        fit_s, fms, _, fnodes = mixed_leg(axs.REFINE_SOLVE, coarse_direct=args.direct_every, precision=axs.PREC_FP64)
+        log(f"fp64, direct coarsest solve formed every {args.direct_every} iterations: {fit_s:.1f} it/s "
+            f"({fit_s / it_s:.2f}x)")
         mixed = {"value": round(mit_s, 3), "unit": "outer_iter/s", "ms_per_step": round(mms / args.steps, 5),
                  "speedup_vs_fp64": round(mit_s / it_s, 3),
                  "dtype": "f32 sweeps/V-cycles, f64 defects, assembly, fluxes and residuals",
@@ -215,7 +219,11 @@ def main() -> int:
                  "direct_coarsest_interval": {"formed_every": args.direct_every, "value": round(kit_s, 3),
                                               "speedup_vs_fp64": round(kit_s / it_s, 3),
                                               "ms_per_step": round(kms / args.steps, 5),
-                                              "graph_nodes_per_iteration": knodes}}
+                                              "graph_nodes_per_iteration": knodes},
+                 "fp64_direct_coarsest_interval": {"formed_every": args.direct_every, "value": round(fit_s, 3),
+                                                   "speedup_vs_fp64": round(fit_s / it_s, 3),
+                                                   "ms_per_step": round(fms / args.steps, 5),
+                                                   "graph_nodes_per_iteration": fnodes}}
 
     # runs to the 1e-6 residual from a cold start: the nozzle (the metric's
     # workload), the pipe and the expansion, each FP64 and mixed, with the
@@ -470,18 +478,19 @@ def guess_per_solve(net, m, mg_levels: int, iterations=(20, 200, 2000), checkpoi
 
 def converge_pair(make, cap: int, every: int, keep: dict | None = None, tight_tol: float = 1e-9) -> dict:
     """One case from a cold start to its residual tolerance (at most `cap`
-    iterations), FP64, mixed, and mixed with the FP64 direct solve of the
+    iterations), FP64, mixed, and both with the FP64 direct solve of the
     coarsest level (its inverse formed on every `every`-th outer iteration),
-    the last also to `tight_tol` when it converged: iterations,
-    wall time and rate of each, and the mixed fields' relative L2 error
-    against the FP64 fields (kept in `keep` when given)."""
+    those two also to `tight_tol` when they converged: iterations, wall time
+    and rate of each, and each run's fields' relative L2 error against the
+    plain FP64 fields (kept in `keep` when given)."""
     import numpy as np
 
     import axs
 
     out, fields = {}, {}
     runs = (("fp64", dict(precision=axs.PREC_FP64)), ("mixed", dict(precision=axs.PREC_MIXED)),
-            ("mixed_direct", dict(precision=axs.PREC_MIXED, coarse_direct=every)))
+            ("mixed_direct", dict(precision=axs.PREC_MIXED, coarse_direct=every)),
+            ("fp64_direct", dict(precision=axs.PREC_FP64, coarse_direct=every)))
     for name, kw in runs:
         case = make(max_iter=cap, **kw)
         s = axs.make_solver(case)
@@ -507,23 +516,25 @@ def converge_pair(make, cap: int, every: int, keep: dict | None = None, tight_to
     # nozzle, measured against a run to 1e-10: FP64 at 1e-6 u 1.1e-5,
     # v 6.8e-5, p 4.4e-5; mixed with the direct solve at 1e-9 u 3.8e-6,
     # v 1.1e-4, p 3.6e-6; at 1e-6 u 2.0e-3, v 4.4e-2, p 1.2e-3)
-    if out["mixed_direct"]["converged"]:
-        case = make(max_iter=cap, tol=tight_tol, precision=axs.PREC_MIXED, coarse_direct=every)
+    for base, prec in (("mixed_direct", axs.PREC_MIXED), ("fp64_direct", axs.PREC_FP64)):
+        if not out[base]["converged"]:
+            continue
+        case = make(max_iter=cap, tol=tight_tol, precision=prec, coarse_direct=every)
         s = axs.make_solver(case)
         t = time.time()
         s.run()
         wall = time.time() - t
         st = s.stats()
-        out["mixed_direct_tight"] = {
+        out[f"{base}_tight"] = {
             "tolerance": tight_tol, "iterations": st.iterations, "converged": bool(st.converged),
             "cap_hit": st.iterations >= cap, "wall_s": round(wall, 3), "solve_s": round(st.solve_ms / 1e3, 3),
             "it_per_s": round(st.iterations / (st.solve_ms / 1e3), 2),
             "last_residual": [float(f"{x:.4g}") for x in st.last_residual[:3]]}
-        fields["mixed_direct_tight"] = {f: s.field(f) for f in ("u", "v", "p")}
-        log(f"{case.name} mixed_direct to {tight_tol:g}: {st.iterations} iterations, "
+        fields[f"{base}_tight"] = {f: s.field(f) for f in ("u", "v", "p")}
+        log(f"{case.name} {base} to {tight_tol:g}: {st.iterations} iterations, "
             f"converged {bool(st.converged)}, {wall:.2f} s")
         del s
-    for name in ("mixed", "mixed_direct", "mixed_direct_tight"):
+    for name in ("mixed", "mixed_direct", "mixed_direct_tight", "fp64_direct", "fp64_direct_tight"):
         if name not in fields:
             continue
         out[f"{name}_rel_l2_vs_fp64"] = {

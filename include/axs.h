@@ -375,7 +375,10 @@ axs_status axs_mg_get_maps(const axs_mg* mg, int level, int* face_start, int* fa
                            int* cell_to_coarse, int* fine_slot_to_coarse_slot);
 /* Solve A x = b with `cycles` V-cycles from the given x (host arrays, level-0
  * CSR order).  AC, AF, b, x as in Coefficients (solver_struct.h:103-120).
- * smoother: AXS_MG_SMOOTH_* (the reference smooths with weighted Jacobi). */
+ * smoother: AXS_MG_SMOOTH_* (the reference smooths with weighted Jacobi).
+ * coarse_direct = 1: the coarsest level solved with its FP64 inverse (formed
+ * once per call; at most 1024 cells, else AXS_ERR_UNSUPPORTED) instead of
+ * coarse_sweeps, in either precision. */
 axs_status axs_mg_solve(axs_mg* mg, const double* AC, const double* AF, const double* b,
                         double* x, int cycles, double weight, int pre_sweeps, int post_sweeps,
                         int coarse_sweeps, int precision, int coarse_direct, int smoother);

@@ -213,3 +213,46 @@ def test_direct_refresh_interval_rejects_negative(require_gpu):
     case = axs.pipe_case(nz=32, nr=8, precision=axs.PREC_MIXED, coarse_direct=-1)
     with pytest.raises(Exception):
         axs.make_solver(case).run()
+
+
+@pytest.mark.parametrize("levels", [0, 4])
+def test_fp64_cycle_with_direct_coarse_matches_textbook(require_gpu, levels):
+    """The FP64 V-cycles with the exact coarsest solve (the one-workgroup
+    inverse inside the tail at full depth, the blocked inverse applied across
+    the GPU at four levels) against the textbook FP64 cycle with a direct
+    coarse solve (scipy.sparse): the same cycles up to summation order."""
+    from mg_textbook import TextbookMG, spmat
+
+    case = axs.nozzle_case(mesh=axs.nozzle_mesh(spacing=2e-4))
+    fs, fn, AC, AF, b = pp_system(case)
+    g = axs.Multigrid(fs, fn, max_levels=levels)
+    if levels:
+        assert 256 < g.level_info(g.levels - 1)[0] <= 1024
+    x = g.solve(AC, AF, b, np.zeros_like(b), cycles=3, coarse_direct=1)
+    tb = TextbookMG(spmat(fs, fn, AC, AF), [g.maps(l)["cell_to_coarse"] for l in range(g.levels - 1)])
+    ref = tb.solve(b, np.zeros_like(b), 3, direct=True)
+    e = rel(x, ref)
+    print("rel L2 vs the textbook cycle", e)
+    assert e < 1e-8
+
+
+@pytest.mark.parametrize("which,every", [("pipe", 1), ("nozzle", 4)])
+def test_fp64_direct_coarse_converges_to_fp64_solution(require_gpu, which, every):
+    """The FP64 path with the exact coarsest solve (formed every `every`-th
+    outer iteration) reaches the plain FP64 path's converged fields (both to
+    1e-10, the nozzle to 1e-12) within 1e-8."""
+    if which == "pipe":
+        make = lambda **kw: axs.pipe_case(tol=1e-10, **kw)
+    else:
+        mesh = axs.nozzle_mesh(axs.nozzle_spacing_for(6000))
+        make = lambda **kw: axs.nozzle_case(mesh=mesh, re_throat=100, tol=1e-12, mg_max_levels=3, **kw)
+    a = axs.make_solver(make())
+    a.run()
+    b = axs.make_solver(make(coarse_direct=every))
+    b.run()
+    assert a.stats().converged and b.stats().converged
+    print("iterations fp64", a.stats().iterations, "fp64 + direct coarse", b.stats().iterations)
+    for name in ("u", "v", "p"):
+        e = rel(b.field(name), a.field(name))
+        print(name, "rel L2 vs FP64", e)
+        assert e < 1e-8, (name, e)
