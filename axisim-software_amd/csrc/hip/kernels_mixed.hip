@@ -357,7 +357,7 @@ __global__ void __launch_bounds__(kBlock) kMGCast(DLevel L, const Ctl* ctl) {
 
 // ---------------------------------------------------------------------------
 // coarsest level: FP64 dense inverse (Gauss-Jordan, partial pivoting), one
-// workgroup; n <= kDirectMax.  inv is n x n row-major scratch, aug 2 n x n.
+// workgroup; n <= kDirectOneBlock.  inv is n x n row-major scratch, aug 2 n x n.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(1024) kCoarseInvert(DLevel L, double* aug, double* inv, int* status,
                                                      const Ctl* ctl) {
@@ -799,7 +799,7 @@ void launchCoarseInvert(hipStream_t s, const DLevel& L, double* aug, double* inv
     for (int k = 0; k < K; k++)
         kGJStep<<<K * K, kGJThreads, 0, s>>>(m[k & 1], m[(k + 1) & 1], N, k, P[k & 1], P[(k + 1) & 1], ctl);
 }
-static_assert(kDirectMax <= 16 * 64, "kCoarseApplyWide<16> covers at most 1024 columns");
+static_assert(kDirectMax <= 32 * 64, "kCoarseApplyWide<32> covers at most 2048 columns");
 namespace {
 template <typename T>
 void coarseApply(hipStream_t s, int n, const double* inv, const T* b, T* x, const Ctl* ctl) {
@@ -808,7 +808,8 @@ void coarseApply(hipStream_t s, int n, const double* inv, const T* b, T* x, cons
     const int ld = coarseInverseLd(n);
     if (n <= 256) kCoarseApplyWide<4, T><<<g, blk, 0, s>>>(n, inv, ld, b, x, ctl);
     else if (n <= 512) kCoarseApplyWide<8, T><<<g, blk, 0, s>>>(n, inv, ld, b, x, ctl);
-    else kCoarseApplyWide<16, T><<<g, blk, 0, s>>>(n, inv, ld, b, x, ctl);
+    else if (n <= 1024) kCoarseApplyWide<16, T><<<g, blk, 0, s>>>(n, inv, ld, b, x, ctl);
+    else kCoarseApplyWide<32, T><<<g, blk, 0, s>>>(n, inv, ld, b, x, ctl);
 }
 }  // namespace
 void launchCoarseApply(hipStream_t s, const DLevel& L, const double* inv, const Ctl* ctl) {

@@ -5,7 +5,7 @@
 
 namespace axs {
 
-constexpr int kDirectMax = 1024;      // largest coarsest level solved by the dense FP64 inverse
+constexpr int kDirectMax = 2048;      // largest coarsest level solved by the dense FP64 inverse
 constexpr int kDirectOneBlock = 256;  // up to this size the inverse is formed by one workgroup
 // the coarsest inverse's row stride (n, or n padded for the blocked inverse)
 // and the work buffer (doubles) launchCoarseInvert needs beside the inverse

@@ -170,7 +170,7 @@ typedef struct axs_config {
     int mg_smoother;      /* AXS_MG_SMOOTH_*                                    */
     int precision;        /* AXS_PREC_*                                         */
     int coarse_direct;    /* K >= 1: FP64 inverse on the coarsest level (at most
-                           * 1024 cells), formed on every K-th outer iteration
+                           * 2048 cells), formed on every K-th outer iteration
                            * and applied in between; 0 = the coarsest sweeps */
     /* Transient frames: ConfigSolver::saveKeyFrameIter (structs/solver_struct.h:230,
      * clamped to >= 1 by runCheck, solver/source/solver.cpp:557-559) and
@@ -377,7 +377,7 @@ axs_status axs_mg_get_maps(const axs_mg* mg, int level, int* face_start, int* fa
  * CSR order).  AC, AF, b, x as in Coefficients (solver_struct.h:103-120).
  * smoother: AXS_MG_SMOOTH_* (the reference smooths with weighted Jacobi).
  * coarse_direct = 1: the coarsest level solved with its FP64 inverse (formed
- * once per call; at most 1024 cells, else AXS_ERR_UNSUPPORTED) instead of
+ * once per call; at most 2048 cells, else AXS_ERR_UNSUPPORTED) instead of
  * coarse_sweeps, in either precision. */
 axs_status axs_mg_solve(axs_mg* mg, const double* AC, const double* AF, const double* b,
                         double* x, int cycles, double weight, int pre_sweeps, int post_sweeps,
