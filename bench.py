@@ -235,7 +235,7 @@ def main() -> int:
             "tolerance": 1e-6,
             "direct_formed_every": args.direct_every,
             "nozzle_1M": converge_pair(lambda **kw: axs.nozzle_case(mesh=m, mg_max_levels=args.mg_levels, **kw),
-                                       args.converge_cap, args.direct_every, keep=nozzle_fp64),
+                                       args.converge_cap, args.direct_every, keep=nozzle_fp64, one_cycle=True),
             "pipe_128x32": converge_pair(lambda **kw: axs.pipe_case(**kw), args.converge_cap, args.direct_every),
             "expansion_238k": converge_pair(lambda **kw: axs.expansion_case(**kw), args.expansion_cap,
                                             args.direct_every),
@@ -476,13 +476,16 @@ def guess_per_solve(net, m, mg_levels: int, iterations=(20, 200, 2000), checkpoi
             "guess_level": k, "guess_level_cells": k_cells, "systems": rows, "seconds": round(time.time() - t0, 1)}
 
 
-def converge_pair(make, cap: int, every: int, keep: dict | None = None, tight_tol: float = 1e-9) -> dict:
+def converge_pair(make, cap: int, every: int, keep: dict | None = None, tight_tol: float = 1e-9,
+                  one_cycle: bool = False) -> dict:
     """One case from a cold start to its residual tolerance (at most `cap`
     iterations), FP64, mixed, and both with the FP64 direct solve of the
     coarsest level (its inverse formed on every `every`-th outer iteration),
     those two also to `tight_tol` when they converged: iterations, wall time
     and rate of each, and each run's fields' relative L2 error against the
-    plain FP64 fields (kept in `keep` when given)."""
+    plain FP64 fields (kept in `keep` when given).  one_cycle: also the mixed
+    path with the direct solve and one V-cycle per p' solve (mg_max_iter = 1)
+    to both tolerances."""
     import numpy as np
 
     import axs
@@ -516,10 +519,26 @@ def converge_pair(make, cap: int, every: int, keep: dict | None = None, tight_to
     # nozzle, measured against a run to 1e-10: FP64 at 1e-6 u 1.1e-5,
     # v 6.8e-5, p 4.4e-5; mixed with the direct solve at 1e-9 u 3.8e-6,
     # v 1.1e-4, p 3.6e-6; at 1e-6 u 2.0e-3, v 4.4e-2, p 1.2e-3)
-    for base, prec in (("mixed_direct", axs.PREC_MIXED), ("fp64_direct", axs.PREC_FP64)):
+    legs = [("mixed_direct", dict(precision=axs.PREC_MIXED, coarse_direct=every)),
+            ("fp64_direct", dict(precision=axs.PREC_FP64, coarse_direct=every))]
+    if one_cycle:
+        kw = dict(precision=axs.PREC_MIXED, coarse_direct=every, mg_max_iter=1)
+        case = make(max_iter=cap, **kw)
+        s = axs.make_solver(case)
+        s.run()
+        st = s.stats()
+        out["mixed_direct_1cycle"] = {"iterations": st.iterations, "converged": bool(st.converged),
+                                      "solve_s": round(st.solve_ms / 1e3, 3),
+                                      "it_per_s": round(st.iterations / (st.solve_ms / 1e3), 2)}
+        fields["mixed_direct_1cycle"] = {f: s.field(f) for f in ("u", "v", "p")}
+        log(f"{case.name} mixed_direct, one V-cycle per p' solve: {st.iterations} iterations, "
+            f"converged {bool(st.converged)}, {st.solve_ms / 1e3:.2f} s")
+        del s
+        legs.append(("mixed_direct_1cycle", kw))
+    for base, kw in legs:
         if not out[base]["converged"]:
             continue
-        case = make(max_iter=cap, tol=tight_tol, precision=prec, coarse_direct=every)
+        case = make(max_iter=cap, tol=tight_tol, **kw)
         s = axs.make_solver(case)
         t = time.time()
         s.run()
@@ -534,7 +553,8 @@ def converge_pair(make, cap: int, every: int, keep: dict | None = None, tight_to
         log(f"{case.name} {base} to {tight_tol:g}: {st.iterations} iterations, "
             f"converged {bool(st.converged)}, {wall:.2f} s")
         del s
-    for name in ("mixed", "mixed_direct", "mixed_direct_tight", "fp64_direct", "fp64_direct_tight"):
+    for name in ("mixed", "mixed_direct", "mixed_direct_tight", "fp64_direct", "fp64_direct_tight",
+                 "mixed_direct_1cycle", "mixed_direct_1cycle_tight"):
         if name not in fields:
             continue
         out[f"{name}_rel_l2_vs_fp64"] = {
