@@ -124,3 +124,16 @@ def test_triangles_builder_face_numbering():
     # the shared face's normal points from cell 0 towards cell 1
     assert a["face_normal_z"][1] > 0 and a["face_normal_r"][1] > 0
     np.testing.assert_allclose(a["cell_volume"], 2 * np.pi * a["cell_center_r"] * 0.5)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; asked for several it says so and exits with
+    status 2 before touching a device."""
+    import sys
+
+    root = Path(__file__).resolve().parents[1]
+    r = subprocess.run([sys.executable, str(root / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120, cwd=root)
+    assert r.returncode == 2
+    assert "one GPU only" in r.stderr
+    assert r.stdout.strip() == ""
